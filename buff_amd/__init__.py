@@ -1,0 +1,26 @@
+"""MI355X-native BUFF (decomposed bounded floats) engine.
+
+The hot path (encode, equality/range filters, sum, max, materialize) runs as
+hand-written HIP kernels for gfx950 in libbuffgpu.so behind the C ABI of
+include/buffgpu.h. This package is the Python binding of that ABI.
+"""
+from ._native import (  # noqa: F401
+    BUFF_FLIP,
+    BUFF_OP_EQ,
+    BUFF_OP_GT,
+    BUFF_SEMANTIC,
+    BuffError,
+    Chunk,
+    Context,
+    DeviceBuffer,
+    OutOfMemory,
+    RefPanic,
+    Unsupported,
+    lib,
+)
+from .split_double import SplitBDDoubleCompress  # noqa: F401
+
+__all__ = [
+    "BUFF_FLIP", "BUFF_OP_EQ", "BUFF_OP_GT", "BUFF_SEMANTIC", "BuffError", "Chunk", "Context", "DeviceBuffer",
+    "OutOfMemory", "RefPanic", "Unsupported", "SplitBDDoubleCompress", "lib",
+]

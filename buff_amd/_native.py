@@ -1,0 +1,296 @@
+"""ctypes binding of libbuffgpu.so (the C ABI in include/buffgpu.h).
+
+This is the Python side of the drop-in boundary: every call goes through the
+exported `extern "C"` entry points of the HIP library. There is no CPU
+fallback; loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbuffgpu.so")
+
+BUFF_OK = 0
+BUFF_E_INVALID = 1
+BUFF_E_OOM = 2
+BUFF_E_DEVICE = 3
+BUFF_E_REF_PANIC = 4
+BUFF_E_UNSUPPORTED = 5
+BUFF_E_SMALL_BUFFER = 6
+
+BUFF_FLIP = 0x1
+BUFF_SEMANTIC = 0x2
+BUFF_OP_GT = 0
+BUFF_OP_EQ = 1
+
+
+class BuffError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("buffgpu status %d: %s" % (code, msg))
+        self.code = code
+
+
+class RefPanic(BuffError):
+    """The reference panics on this input (BUFF_E_REF_PANIC)."""
+
+
+class Unsupported(BuffError):
+    """BUFF_E_UNSUPPORTED."""
+
+
+class OutOfMemory(BuffError):
+    """BUFF_E_OOM: the HBM budget or the device is exhausted."""
+
+
+class ChunkInfo(ctypes.Structure):
+    _fields_ = [
+        ("base", ctypes.c_int64),
+        ("n", ctypes.c_uint32),
+        ("ilen", ctypes.c_uint32),
+        ("dlen", ctypes.c_uint32),
+        ("width", ctypes.c_uint32),
+        ("planes", ctypes.c_uint32),
+        ("tail_bits", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+        ("scale", ctypes.c_uint64),
+        ("stream_bytes", ctypes.c_uint64),
+        ("device_bytes", ctypes.c_uint64),
+    ]
+
+
+class SplitDouble(ctypes.Structure):
+    _fields_ = [("chunksize", ctypes.c_uint64), ("batchsize", ctypes.c_uint64), ("scale", ctypes.c_uint64)]
+
+
+P = ctypes.POINTER
+vp = ctypes.c_void_p
+u8p, u32p, u64p, f64p = P(ctypes.c_uint8), P(ctypes.c_uint32), P(ctypes.c_uint64), P(ctypes.c_double)
+u64, u32, i32, f64, cint = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32, ctypes.c_double, ctypes.c_int
+
+# name -> argtypes (every function returns int status except buff_last_error)
+SIGNATURES = {
+    "buff_ctx_create": [cint, u64, P(vp)],
+    "buff_ctx_destroy": [vp],
+    "buff_ctx_synchronize": [vp],
+    "buff_ctx_hbm_used": [vp, u64p],
+    "buff_dev_alloc": [vp, u64, P(vp)],
+    "buff_dev_free": [vp, vp],
+    "buff_memcpy_h2d": [vp, vp, vp, u64],
+    "buff_memcpy_d2h": [vp, vp, vp, u64],
+    "buff_prof_enable": [vp, cint],
+    "buff_prof_reset": [vp],
+    "buff_prof_query": [vp, ctypes.c_char_p, u64p, f64p],
+    "buff_encode_device": [vp, vp, u64, u64, u32, P(vp)],
+    "buff_encode": [vp, f64p, u64, u64, u32, P(vp)],
+    "buff_chunk_info_get": [vp, P(ChunkInfo)],
+    "buff_chunk_serialize": [vp, vp, u8p, u64p],
+    "buff_chunk_load": [vp, u8p, u64, u64, u32, P(vp)],
+    "buff_chunk_free": [vp],
+    "buff_filter": [vp, vp, cint, f64, u32, u32p, u64, u64p],
+    "buff_filter_device": [vp, vp, cint, f64, u32, vp, vp],
+    "buff_sum": [vp, vp, f64p, f64p, u64p],
+    "buff_max": [vp, vp, f64p, u64p],
+    "buff_materialize": [vp, vp, i32, u32, f64p],
+    "buff_materialize_device": [vp, vp, i32, u32, vp],
+    "buff_sd_byte_fixed_encode": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
+    "buff_sd_buff_simd256_encode": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
+    "buff_sd_byte_fixed_decode": [P(SplitDouble), u8p, u64, f64p, u64],
+    "buff_sd_buff_simd256_decode": [P(SplitDouble), u8p, u64, f64p, u64],
+    "buff_sd_buff_range_filter": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
+    "buff_sd_buff_equal_filter": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
+    "buff_sd_buff_simd_range_filter": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
+    "buff_sd_buff_simd_equal_filter": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
+    "buff_sd_byte_fixed_sum": [P(SplitDouble), u8p, u64, f64p],
+    "buff_sd_byte_fixed_max": [P(SplitDouble), u8p, u64, f64p],
+    "buff_sd_byte_residue_decode_with_precision": [P(SplitDouble), u8p, u64, u32, f64p, u64],
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libbuffgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libbuffgpu.so is not built (run `python -m buff_amd.build`): " + LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = args
+        L.buff_last_error.restype = ctypes.c_char_p
+        L.buff_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc == BUFF_OK:
+        return
+    msg = lib().buff_last_error().decode(errors="replace")
+    cls = {BUFF_E_REF_PANIC: RefPanic, BUFF_E_UNSUPPORTED: Unsupported, BUFF_E_OOM: OutOfMemory}.get(rc, BuffError)
+    raise cls(rc, msg)
+
+
+def ptr(a, ct):
+    return a.ctypes.data_as(P(ct))
+
+
+class Chunk:
+    """A resident column (buff_chunk*)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    def info(self):
+        inf = ChunkInfo()
+        check(lib().buff_chunk_info_get(self.handle, ctypes.byref(inf)))
+        return inf
+
+    @property
+    def n(self):
+        return self.info().n
+
+    def serialize(self):
+        L = lib()
+        n = ctypes.c_uint64(0)
+        check(L.buff_chunk_serialize(self.ctx.handle, self.handle, None, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=np.uint8)
+        check(L.buff_chunk_serialize(self.ctx.handle, self.handle, ptr(out, ctypes.c_uint8), ctypes.byref(n)))
+        return out
+
+    def filter(self, op, pred, flags=0):
+        """Returns (u32 words in BitVec layout, count)."""
+        n = self.n
+        words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
+        cnt = ctypes.c_uint64()
+        opc = BUFF_OP_GT if op in ("range", "gt", BUFF_OP_GT) else BUFF_OP_EQ
+        check(lib().buff_filter(self.ctx.handle, self.handle, opc, float(pred), flags, ptr(words, ctypes.c_uint32),
+                                len(words), ctypes.byref(cnt)))
+        return words[: (n + 31) // 32], cnt.value
+
+    def filter_device(self, op, pred, words_dev, flags=0, stats_dev=None):
+        opc = BUFF_OP_GT if op in ("range", "gt", BUFF_OP_GT) else BUFF_OP_EQ
+        check(lib().buff_filter_device(self.ctx.handle, self.handle, opc, float(pred), flags, words_dev, stats_dev))
+
+    def sum(self):
+        """Returns (reference f64 sum, exact sum, per-sub-column u64 sums[9])."""
+        r, e = ctypes.c_double(), ctypes.c_double()
+        ps = np.zeros(9, dtype=np.uint64)
+        check(lib().buff_sum(self.ctx.handle, self.handle, ctypes.byref(r), ctypes.byref(e), ptr(ps, ctypes.c_uint64)))
+        return r.value, e.value, ps
+
+    def max(self):
+        """Returns (max value as the reference prints it, max stored offset)."""
+        v, o = ctypes.c_double(), ctypes.c_uint64()
+        check(lib().buff_max(self.ctx.handle, self.handle, ctypes.byref(v), ctypes.byref(o)))
+        return v.value, o.value
+
+    def materialize(self, precision=-1, flags=0):
+        out = np.zeros(max(self.n, 1), dtype=np.float64)
+        check(lib().buff_materialize(self.ctx.handle, self.handle, precision, flags, ptr(out, ctypes.c_double)))
+        return out[: self.n]
+
+    def materialize_device(self, out_dev, precision=-1, flags=0):
+        check(lib().buff_materialize_device(self.ctx.handle, self.handle, precision, flags, out_dev))
+
+    def free(self):
+        if self.handle:
+            check(lib().buff_chunk_free(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """Device memory owned by a context (counted against its HBM budget)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        p = ctypes.c_void_p()
+        check(lib().buff_dev_alloc(ctx.handle, nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        check(lib().buff_memcpy_h2d(self.ctx.handle, self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self, dtype, count):
+        out = np.zeros(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().buff_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(lib().buff_dev_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """buff_ctx*: one HIP stream on one device with an HBM budget (0 = none)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        h = ctypes.c_void_p()
+        check(lib().buff_ctx_create(device, hbm_budget, ctypes.byref(h)))
+        self.handle = h.value
+
+    def encode(self, x, scale, flags=0):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        h = ctypes.c_void_p()
+        check(lib().buff_encode(self.handle, ptr(x, ctypes.c_double), len(x), scale, flags, ctypes.byref(h)))
+        return Chunk(self, h.value)
+
+    def encode_device(self, x_dev, n, scale, flags=0):
+        h = ctypes.c_void_p()
+        check(lib().buff_encode_device(self.handle, x_dev, n, scale, flags, ctypes.byref(h)))
+        return Chunk(self, h.value)
+
+    def load(self, stream, scale, flags=0):
+        s = np.ascontiguousarray(stream, dtype=np.uint8)
+        h = ctypes.c_void_p()
+        check(lib().buff_chunk_load(self.handle, ptr(s, ctypes.c_uint8), len(s), scale, flags, ctypes.byref(h)))
+        return Chunk(self, h.value)
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def synchronize(self):
+        check(lib().buff_ctx_synchronize(self.handle))
+
+    def hbm_used(self):
+        v = ctypes.c_uint64()
+        check(lib().buff_ctx_hbm_used(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def prof_enable(self, on=True):
+        check(lib().buff_prof_enable(self.handle, int(on)))
+
+    def prof_reset(self):
+        check(lib().buff_prof_reset(self.handle))
+
+    def prof_query(self, kernel):
+        n, ms = ctypes.c_uint64(), ctypes.c_double()
+        check(lib().buff_prof_query(self.handle, kernel.encode(), ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def destroy(self):
+        if self.handle:
+            check(lib().buff_ctx_destroy(self.handle))
+            self.handle = None

@@ -1,0 +1,1010 @@
+// Host runtime and C ABI of the BUFF engine (include/buffgpu.h).
+//
+// A buff_ctx owns one HIP stream on one device, a caching device allocator
+// that enforces the context's HBM budget, small pinned result buffers, per-op
+// launch shapes (set by the autotuner) and optional per-kernel event timing.
+// A buff_chunk is one column in the resident layout of buff_common.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/buffgpu.h"
+#include "buff_common.h"
+#include "buff_kernels.h"
+#include "buff_runtime.h"
+
+namespace buffgpu {
+
+// ---------------------------------------------------------------------------
+// errors
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return fail(BUFF_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_));      \
+  } while (0)
+
+#define BUFF_TRY(expr)          \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != BUFF_OK) return rc_; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// precision helpers (the reference's per-call setup, split_double.rs:304-316)
+
+int prec_of_scale(uint64_t scale) {
+  const float s = (float)scale;          // usize as f32
+  const float l = log10f(s);
+  if (std::isnan(l)) return 0;           // f32 as i32 saturates, NaN -> 0
+  if (l <= -2147483648.0f) return INT32_MIN;
+  if (l >= 2147483648.0f) return INT32_MAX;
+  return (int)l;
+}
+
+int precision_map(int p) {  // database/src/compress/mod.rs:29-46
+  static const int tab[16] = {-1, 5, 8, 11, 15, 18, 21, 25, 28, 31, 35, 38, 50, 10, 10, 10};
+  return (p >= 1 && p <= 15) ? tab[p] : -1;
+}
+
+int32_t precision_exp_of_prec(int prec) {  // prec_double.rs:28-33 of get_precision_bound(p), 273-281
+  std::string s = "0.";
+  for (int i = 0; i < prec; ++i) s.push_back('0');
+  s += "49";
+  const double b = strtod(s.c_str(), nullptr);
+  uint64_t u;
+  memcpy(&u, &b, 8);
+  return (int32_t)((u & 0x7FF0000000000000ull) >> 52) - 1023;
+}
+
+double powi2(int32_t e) {  // 2.0f64.powi(e) as lowered by compiler-rt __powidf2
+  double r = 1, a = 2.0;
+  int64_t b = e;
+  const bool recip = b < 0;
+  while (true) {
+    if (b & 1) r *= a;
+    b /= 2;
+    if (b == 0) break;
+    a *= a;
+  }
+  return recip ? 1 / r : r;
+}
+
+// `(delta as f64).log2().ceil() as usize` (split_double.rs:338-344)
+uint32_t width_of_delta(int64_t delta) {
+  if (delta == 0) return 0;
+  const double c = std::ceil(std::log2((double)delta));
+  if (std::isnan(c) || c <= 0) return 0;
+  if (c >= 64) return 64;
+  return (uint32_t)c;
+}
+
+// ---------------------------------------------------------------------------
+// context
+
+Ctx::~Ctx() {
+  if (stream) {
+    (void)hipStreamSynchronize(stream);
+  }
+  for (auto& kv : free_blocks) (void)hipFree(kv.second);
+  free_blocks.clear();
+  for (auto& p : prof_pool) (void)hipEventDestroy(p);
+  if (pinned) (void)hipHostFree(pinned);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Ctx::alloc(uint64_t bytes, void** out) {
+  const uint64_t need = round_up(std::max<uint64_t>(bytes, 256), 256);
+  std::lock_guard<std::mutex> lk(mu);
+  // reuse a cached block of close size
+  auto it = free_blocks.lower_bound(need);
+  if (it != free_blocks.end() && it->first <= need + need / 4 + (1u << 20)) {
+    *out = it->second;
+    live[it->second] = it->first;
+    free_blocks.erase(it);
+    return BUFF_OK;
+  }
+  if (budget && held + need > budget) {
+    // give cached blocks back to the device first
+    for (auto& kv : free_blocks) {
+      (void)hipFree(kv.second);
+      held -= kv.first;
+    }
+    free_blocks.clear();
+    if (held + need > budget)
+      return fail(BUFF_E_OOM, "HBM budget exceeded: %llu held + %llu requested > %llu",
+                  (unsigned long long)held, (unsigned long long)need, (unsigned long long)budget);
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, need);
+  if (e != hipSuccess) {
+    for (auto& kv : free_blocks) {
+      (void)hipFree(kv.second);
+      held -= kv.first;
+    }
+    free_blocks.clear();
+    e = hipMalloc(&p, need);
+    if (e != hipSuccess) return fail(BUFF_E_OOM, "hipMalloc(%llu): %s", (unsigned long long)need, hipGetErrorString(e));
+  }
+  held += need;
+  live[p] = need;
+  *out = p;
+  return BUFF_OK;
+}
+
+int Ctx::release(void* p) {
+  if (!p) return BUFF_OK;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = live.find(p);
+  if (it == live.end()) return fail(BUFF_E_INVALID, "pointer %p was not allocated by this context", p);
+  free_blocks.emplace(it->second, p);
+  live.erase(it);
+  return BUFF_OK;
+}
+
+int Ctx::scratch(Scratch& s, uint64_t bytes) {
+  if (s.bytes >= bytes) return BUFF_OK;
+  if (s.ptr) {
+    // the previous buffer may still be read by queued kernels
+    HIP_TRY(hipStreamSynchronize(stream));
+    BUFF_TRY(release(s.ptr));
+    s.ptr = nullptr;
+    s.bytes = 0;
+  }
+  BUFF_TRY(alloc(bytes, &s.ptr));
+  s.bytes = bytes;
+  return BUFF_OK;
+}
+
+void Ctx::prof_begin(const char* name, hipEvent_t* ev) {
+  ev[0] = ev[1] = nullptr;
+  if (!prof_on) return;
+  for (int i = 0; i < 2; ++i) {
+    if (prof_pool.empty()) {
+      (void)hipEventCreate(&ev[i]);
+    } else {
+      ev[i] = prof_pool.back();
+      prof_pool.pop_back();
+    }
+  }
+  (void)hipEventRecord(ev[0], stream);
+  (void)name;
+}
+
+void Ctx::prof_end(const char* name, hipEvent_t* ev) {
+  if (!ev[0]) return;
+  (void)hipEventRecord(ev[1], stream);
+  prof_pending.push_back(ProfRec{name, ev[0], ev[1]});
+}
+
+int Ctx::prof_collect() {
+  if (prof_pending.empty()) return BUFF_OK;
+  HIP_TRY(hipStreamSynchronize(stream));
+  for (auto& r : prof_pending) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    auto& s = prof_stats[r.name];
+    s.first += 1;
+    s.second += ms;
+    prof_pool.push_back(r.a);
+    prof_pool.push_back(r.b);
+  }
+  prof_pending.clear();
+  return BUFF_OK;
+}
+
+LaunchShape Ctx::shape(Op op, uint64_t ntiles) const {
+  auto it = tuned.find(tune_key(op, ntiles));
+  if (it != tuned.end()) return it->second;
+  return LaunchShape{4, 2048};
+}
+
+uint32_t tune_bucket(uint64_t ntiles) {
+  uint32_t b = 0;
+  while ((1ull << (b + 1)) <= ntiles) ++b;
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// chunk helpers
+
+uint64_t stream_bytes_of(uint64_t n, uint32_t P, uint32_t r) {
+  return kStreamHeader + (uint64_t)P * n + (n * r + 7) / 8;
+}
+
+int chunk_alloc(Ctx* ctx, Chunk* ch) {
+  const uint64_t ntiles = (ch->n + kTileRows - 1) / kTileRows;
+  ch->view.n = ch->n;
+  ch->view.ntiles = ntiles;
+  ch->view.pitch = ntiles * kTileRows;
+  const uint64_t plane_bytes = (uint64_t)ch->view.P * ch->view.pitch;
+  const uint64_t tail_bytes = ntiles * 16ull * ch->view.r * 8ull;
+  ch->device_bytes = std::max<uint64_t>(plane_bytes + tail_bytes, 256);
+  void* p = nullptr;
+  BUFF_TRY(ctx->alloc(ch->device_bytes, &p));
+  ch->mem = p;
+  ch->view.planes = (uint8_t*)p;
+  ch->view.tail = (uint64_t*)((uint8_t*)p + plane_bytes);
+  return BUFF_OK;
+}
+
+void chunk_set_width(Chunk* ch, uint32_t F) {
+  ch->view.F = F;
+  ch->view.P = F >= 8 ? F / 8 : 0;
+  ch->view.r = F >= 8 ? F % 8 : F;
+}
+
+// Two pinned u64 slots per context are plenty for the per-call results.
+int d2h_small(Ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (bytes > Ctx::kPinnedBytes) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return BUFF_OK;
+  }
+  HIP_TRY(hipMemcpyAsync(ctx->pinned, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  memcpy(dst, ctx->pinned, bytes);
+  return BUFF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// operations
+
+int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_t flags, Chunk** out) {
+  if (!ctx || !out) return fail(BUFF_E_INVALID, "null argument");
+  if (n > 0xFFFFFFFFull) return fail(BUFF_E_INVALID, "a chunk holds at most 2^32-1 rows (n is stored as u32)");
+  if (n && !x) return fail(BUFF_E_INVALID, "null input");
+  const int prec = scale == 0 ? 0 : prec_of_scale(scale);
+  const int dl = precision_map(prec);
+  if (dl < 0) return fail(BUFF_E_REF_PANIC, "precision %d (scale %llu) is not in PRECISION_MAP", prec,
+                          (unsigned long long)scale);
+  const uint32_t dlen = (uint32_t)dl;
+  const int32_t pexp = precision_exp_of_prec(prec);
+
+  int64_t mn = INT64_MAX, mx = INT64_MIN;
+  if (n > 0) {
+    const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
+    const uint32_t nwg = (uint32_t)std::min<uint64_t>(ntiles, 1024);
+    BUFF_TRY(ctx->scratch(ctx->s_partial, 3ull * nwg * sizeof(int64_t)));
+    BUFF_TRY(ctx->scratch(ctx->s_minmax, sizeof(MinMaxOut)));
+    MinMaxOut* dmm = (MinMaxOut*)ctx->s_minmax.ptr;
+    int64_t* part = (int64_t*)ctx->s_partial.ptr;
+    if (!ctx->minmax_ready) {
+      HIP_TRY(hipMemsetAsync(dmm, 0, sizeof(MinMaxOut), ctx->stream));
+      ctx->minmax_ready = true;
+    }
+    {
+      ProfScope ps(ctx, "k_minmax");
+      HIP_TRY(launch_minmax(x, n, pexp, dlen, part, part + 2 * nwg, dmm, nwg, false, ctx->stream));
+    }
+    MinMaxOut mm;
+    BUFF_TRY(d2h_small(ctx, &mm, dmm, sizeof(mm)));
+    mn = mm.min_all;
+    if (n == 1) {
+      mx = mm.first == INT64_MAX ? INT64_MAX : INT64_MIN;
+    } else if (mm.max_rest >= mm.first) {
+      mx = mm.max_rest;
+    } else {
+      // row 0 holds the unique maximum: replay the `else if` exactly
+      {
+        ProfScope ps(ctx, "k_minmax_prefix");
+        HIP_TRY(launch_minmax(x, n, pexp, dlen, part, part + 2 * nwg, dmm, nwg, true, ctx->stream));
+      }
+      BUFF_TRY(d2h_small(ctx, &mm, dmm, sizeof(mm)));
+      mx = mm.qmax;
+      if (mm.first == INT64_MAX) mx = INT64_MAX;
+    }
+  }
+  const int64_t delta = (int64_t)((uint64_t)mx - (uint64_t)mn);
+  const uint32_t F = width_of_delta(delta);
+  if (F == 0 && n >= 1 && !(flags & BUFF_SEMANTIC))
+    return fail(BUFF_E_REF_PANIC, "all %llu values map to one fixed-point value (F = 0): the reference panics",
+                (unsigned long long)n);
+  if (F > 63) return fail(BUFF_E_UNSUPPORTED, "bit width %u exceeds 63", F);
+
+  Chunk* ch = new (std::nothrow) Chunk();
+  if (!ch) return fail(BUFF_E_OOM, "host allocation");
+  ch->ctx = ctx;
+  ch->n = n;
+  ch->scale = scale;
+  ch->flags = flags & BUFF_FLIP;
+  ch->dlen = dlen;
+  ch->ilen = (uint32_t)((uint64_t)F - dlen);
+  ch->view.base = mn;
+  ch->view.dlen = dlen;
+  chunk_set_width(ch, F);
+  int rc = chunk_alloc(ctx, ch);
+  if (rc != BUFF_OK) {
+    delete ch;
+    return rc;
+  }
+  if (n > 0 && (ch->view.P || ch->view.r)) {
+    ProfScope ps(ctx, "k_encode");
+    hipError_t e = launch_encode(x, ch->view, pexp, ctx->stream, ctx->shape(Op::Encode, ch->view.ntiles));
+    if (e != hipSuccess) {
+      (void)ctx->release(ch->mem);
+      delete ch;
+      return fail(BUFF_E_DEVICE, "k_encode launch: %s", hipGetErrorString(e));
+    }
+  }
+  *out = ch;
+  return BUFF_OK;
+}
+
+int chunk_serialize(Ctx* ctx, const Chunk* ch, uint8_t* dst, uint64_t* len) {
+  if (!ctx || !ch || !len) return fail(BUFF_E_INVALID, "null argument");
+  const uint64_t total = stream_bytes_of(ch->n, ch->view.P, ch->view.r);
+  const uint64_t need = *len;
+  *len = total;
+  if (!dst) return BUFF_OK;
+  if (need < total) return fail(BUFF_E_SMALL_BUFFER, "stream needs %llu bytes", (unsigned long long)total);
+  const uint64_t ubase = (uint64_t)ch->view.base;
+  const uint32_t hdr[5] = {(uint32_t)ubase, (uint32_t)(ubase >> 32), (uint32_t)ch->n, ch->ilen, ch->dlen};
+  memcpy(dst, hdr, 20);  // little-endian u32s, as BitPack::write(v, 32) lays them out
+  const uint64_t body = total - kStreamHeader;
+  if (body == 0) return BUFF_OK;
+  BUFF_TRY(ctx->scratch(ctx->s_stream, body));
+  const uint64_t tail_bytes = (ch->n * ch->view.r + 7) / 8;
+  {
+    ProfScope ps(ctx, "k_serialize");
+    HIP_TRY(launch_serialize(ch->view, (ch->flags & BUFF_FLIP) ? 0x80 : 0, (uint8_t*)ctx->s_stream.ptr, tail_bytes,
+                             ctx->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(dst + kStreamHeader, ctx->s_stream.ptr, body, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return BUFF_OK;
+}
+
+int chunk_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_t flags, Chunk** out) {
+  if (!ctx || !s || !out) return fail(BUFF_E_INVALID, "null argument");
+  if (len < kStreamHeader) return fail(BUFF_E_INVALID, "stream of %llu bytes has no 20-byte header",
+                                       (unsigned long long)len);
+  uint32_t hdr[5];
+  memcpy(hdr, s, 20);
+  const int64_t base = (int64_t)((uint64_t)hdr[0] | ((uint64_t)hdr[1] << 32));
+  const uint64_t n = hdr[2];
+  const uint32_t ilen = hdr[3], dlen = hdr[4];
+  const uint32_t F = dlen + ilen;  // u32 wrap, as the readers compute `dlen + ilen`
+  if (F > 63) return fail(BUFF_E_INVALID, "stored width dlen+ilen = %u is not a BUFF stream width", F);
+  Chunk* ch = new (std::nothrow) Chunk();
+  if (!ch) return fail(BUFF_E_OOM, "host allocation");
+  ch->ctx = ctx;
+  ch->n = n;
+  ch->scale = scale;
+  ch->flags = flags & BUFF_FLIP;
+  ch->dlen = dlen;
+  ch->ilen = ilen;
+  ch->view.base = base;
+  ch->view.dlen = dlen;
+  chunk_set_width(ch, F);
+  const uint64_t expect = stream_bytes_of(n, ch->view.P, ch->view.r);
+  if (len < expect) {
+    delete ch;
+    return fail(BUFF_E_INVALID, "stream is %llu bytes, its header needs %llu", (unsigned long long)len,
+                (unsigned long long)expect);
+  }
+  int rc = chunk_alloc(ctx, ch);
+  if (rc != BUFF_OK) {
+    delete ch;
+    return rc;
+  }
+  const uint64_t body = expect - kStreamHeader;
+  if (n > 0 && (ch->view.P || ch->view.r)) {
+    rc = ctx->scratch(ctx->s_stream, body);
+    if (rc == BUFF_OK) {
+      hipError_t e = hipMemcpyAsync(ctx->s_stream.ptr, s + kStreamHeader, body, hipMemcpyHostToDevice, ctx->stream);
+      if (e == hipSuccess) {
+        ProfScope ps(ctx, "k_load");
+        e = launch_load(ch->view, (ch->flags & BUFF_FLIP) ? 0x80 : 0, (const uint8_t*)ctx->s_stream.ptr,
+                        (n * ch->view.r + 7) / 8, ctx->stream);
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // the host stream may go away
+      if (e != hipSuccess) rc = fail(BUFF_E_DEVICE, "stream upload: %s", hipGetErrorString(e));
+    }
+    if (rc != BUFF_OK) {
+      (void)ctx->release(ch->mem);
+      delete ch;
+      return rc;
+    }
+  }
+  *out = ch;
+  return BUFF_OK;
+}
+
+// Predicate -> per-plane target, or a constant answer. Returns true when the
+// kernel must run; otherwise *fill is 0 (no rows) or 1 (all rows).
+bool filter_plan(const Chunk* ch, int op, double pred, uint32_t flags, FilterTarget* tg, int* fill) {
+  const ChunkView& v = ch->view;
+  const int prec = prec_of_scale(ch->scale);
+  const int32_t pexp = precision_exp_of_prec(prec);
+  const int64_t fixed = fetch_fixed_aligned_bits([&] { uint64_t u; memcpy(&u, &pred, 8); return u; }(), pexp,
+                                                 ch->dlen);
+  memset(tg, 0, sizeof(*tg));
+  if (fixed < v.base) {  // buff_simd.rs:4531-4534 / 4949-4952
+    *fill = op == BUFF_OP_GT ? 1 : 0;
+    return false;
+  }
+  const uint64_t t = (uint64_t)fixed - (uint64_t)v.base;
+  const bool semantic = (flags & BUFF_SEMANTIC) != 0;
+  if (v.F >= 8) {
+    if (semantic && v.F < 64 && t >= (1ull << v.F)) {  // predicate above every stored offset
+      *fill = 0;
+      return false;
+    }
+    if (op == BUFF_OP_EQ && v.r == 0 && !semantic) {  // matches stay in rb1; res is empty
+      *fill = 0;
+      return false;
+    }
+    for (uint32_t p = 0; p < v.P; ++p) tg->digit[p] = (uint32_t)((t >> (v.F - 8 * (p + 1))) & 0xFF);
+    tg->tail = v.r ? (t & ((1ull << v.r) - 1)) : 0;
+    return true;
+  }
+  // F < 8: the whole value is the tail and compares against the full target
+  if (v.F == 0) {
+    *fill = (op == BUFF_OP_EQ && t == 0) ? 1 : 0;
+    return false;
+  }
+  if (t >= (1ull << v.F)) {
+    *fill = 0;
+    return false;
+  }
+  tg->tail = t;
+  return true;
+}
+
+int filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, uint64_t* words,
+                  unsigned long long* stats) {
+  if (!ctx || !ch || !words) return fail(BUFF_E_INVALID, "null argument");
+  if (op != BUFF_OP_GT && op != BUFF_OP_EQ) return fail(BUFF_E_INVALID, "unknown filter op %d", op);
+  const uint64_t nwords = (ch->n + 63) / 64;
+  if (nwords == 0) return BUFF_OK;
+  FilterTarget tg;
+  int fill = 0;
+  if (!filter_plan(ch, op, pred, flags, &tg, &fill)) {
+    ProfScope ps(ctx, "k_fill_bitmap");
+    HIP_TRY(launch_fill_bitmap(words, nwords, ch->n, fill, ctx->stream));
+    return BUFF_OK;
+  }
+  ProfScope ps(ctx, op == BUFF_OP_GT ? "k_filter_gt" : "k_filter_eq");
+  HIP_TRY(launch_filter(ch->view, op == BUFF_OP_GT ? 0 : 1, tg, words, nwords, stats, ctx->stream,
+                        ctx->shape(op == BUFF_OP_GT ? Op::FilterGt : Op::FilterEq, ch->view.ntiles)));
+  return BUFF_OK;
+}
+
+int filter_host(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, uint32_t* words, uint64_t nwords32,
+                uint64_t* count) {
+  if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  const uint64_t need32 = (ch->n + 31) / 32;
+  if (words && nwords32 < need32)
+    return fail(BUFF_E_SMALL_BUFFER, "bitmap needs %llu u32 words", (unsigned long long)need32);
+  const uint64_t nwords64 = (ch->n + 63) / 64;
+  std::vector<uint64_t> host(nwords64);
+  if (nwords64) {
+    BUFF_TRY(ctx->scratch(ctx->s_bitmap, nwords64 * 8));
+    BUFF_TRY(filter_device(ctx, ch, op, pred, flags, (uint64_t*)ctx->s_bitmap.ptr, nullptr));
+    HIP_TRY(hipMemcpyAsync(host.data(), ctx->s_bitmap.ptr, nwords64 * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  uint64_t cnt = 0;
+  for (uint64_t w : host) cnt += (uint64_t)__builtin_popcountll(w);
+  if (count) *count = cnt;
+  if (words && need32) memcpy(words, host.data(), need32 * 4);
+  return BUFF_OK;
+}
+
+// split_double.rs:2127-2208 fold of the per-sub-column sums.
+double reference_sum_fold(const Chunk* ch, const uint64_t* ps) {
+  const uint32_t len = (uint32_t)ch->n, ilen = ch->ilen, dlen = ch->dlen;
+  uint32_t remain = ilen + dlen;
+  int32_t processed = 0;
+  double sum_base = (double)len * (double)ch->view.base;
+  double sum = 0.0;
+  sum_base = sum_base / powi2((int32_t)dlen);
+  uint32_t k = 0;
+  while (remain >= 8) {
+    remain -= 8;
+    processed += 8;
+    const double dec_scl = powi2((int32_t)((uint32_t)processed - ilen));
+    sum = sum + (double)ps[k++] / dec_scl;
+    if (remain == 0) return sum;   // returns before adding the base term
+  }
+  if (remain > 0) {
+    processed += (int32_t)remain;
+    const double dec_scl = powi2((int32_t)((uint32_t)processed - ilen));
+    sum = sum + (double)ps[k] / dec_scl;
+  }
+  sum += sum_base;
+  return sum;
+}
+
+int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64_t* plane_sums) {
+  if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  uint64_t ps[kMaxPlanes + 1] = {0};
+  const ChunkView& v = ch->view;
+  if (ch->n > 0 && (v.P || v.r)) {
+    BUFF_TRY(ctx->scratch(ctx->s_sums, sizeof(ps)));
+    HIP_TRY(hipMemsetAsync(ctx->s_sums.ptr, 0, sizeof(ps), ctx->stream));
+    {
+      ProfScope sc(ctx, "k_sum");
+      HIP_TRY(launch_sum(v, (unsigned long long*)ctx->s_sums.ptr, ctx->stream, ctx->shape(Op::Sum, v.ntiles)));
+    }
+    BUFF_TRY(d2h_small(ctx, ps, ctx->s_sums.ptr, sizeof(ps)));
+  }
+  if (plane_sums) memcpy(plane_sums, ps, sizeof(ps));
+  if (ref_sum) *ref_sum = reference_sum_fold(ch, ps);
+  if (exact_sum) {
+    __int128 tot = (__int128)ch->n * (__int128)v.base;
+    for (uint32_t p = 0; p < v.P; ++p) tot += (__int128)ps[p] << (v.F - 8 * (p + 1));
+    tot += (__int128)ps[v.P];
+    *exact_sum = (double)tot / powi2((int32_t)ch->dlen);
+  }
+  return BUFF_OK;
+}
+
+int max_op(Ctx* ctx, const Chunk* ch, double* value, uint64_t* max_off) {
+  if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  uint64_t best = 0;
+  const ChunkView& v = ch->view;
+  if (ch->n > 0 && (v.P || v.r)) {
+    BUFF_TRY(ctx->scratch(ctx->s_best, 8));
+    HIP_TRY(hipMemsetAsync(ctx->s_best.ptr, 0, 8, ctx->stream));
+    {
+      ProfScope sc(ctx, "k_max");
+      HIP_TRY(launch_max(v, (unsigned long long*)ctx->s_best.ptr, ctx->stream, ctx->shape(Op::Max, v.ntiles)));
+    }
+    BUFF_TRY(d2h_small(ctx, &best, ctx->s_best.ptr, 8));
+  }
+  if (max_off) *max_off = best;
+  // split_double.rs:4102: (max as i64 + base_int) as f64 / 2^dlen
+  if (value) *value = (double)(int64_t)(best + (uint64_t)v.base) / powi2((int32_t)ch->dlen);
+  return BUFF_OK;
+}
+
+// Reduced-precision read rule of split_double.rs:1642-1664 (u32 arithmetic).
+int precision_bits(const Chunk* ch, int32_t precision, uint32_t* R) {
+  const uint32_t ilen = ch->ilen, dlen = ch->dlen;
+  if (precision == 0) {
+    uint32_t remain = ilen;
+    uint32_t byte_needed = ilen / 8;
+    if (ilen % 8 > 0) {
+      byte_needed += 1;
+      remain = byte_needed * 8;
+    }
+    *R = remain;
+    return BUFF_OK;
+  }
+  const int bn = precision_map(precision);
+  if (bn < 0) return fail(BUFF_E_REF_PANIC, "precision %d is not in PRECISION_MAP", precision);
+  const uint32_t bits_needed = (uint32_t)bn;
+  if (!(dlen >= bits_needed)) return fail(BUFF_E_REF_PANIC, "requested %d decimals (%u bits) but dlen is %u",
+                                          precision, bits_needed, dlen);
+  uint32_t remain = ilen + bits_needed;
+  const uint32_t fixed_byte = (dlen + ilen) / 8;
+  uint32_t byte_needed = (ilen + bits_needed) / 8;
+  const uint32_t extra_bits = (ilen + bits_needed) % 8;
+  if (byte_needed < fixed_byte && extra_bits > 0) {
+    byte_needed += 1;
+    remain = byte_needed * 8;
+  }
+  *R = remain;
+  return BUFF_OK;
+}
+
+int materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* out) {
+  if (!ctx || !ch || (!out && ch->n)) return fail(BUFF_E_INVALID, "null argument");
+  const ChunkView& v = ch->view;
+  MatParams mp;
+  memset(&mp, 0, sizeof(mp));
+  mp.inv_scale = std::ldexp(1.0, -(int)ch->dlen);
+  mp.mode = kMatFull;
+  mp.planes = v.P;
+  mp.tail_bits = v.r;
+  const bool semantic = (flags & BUFF_SEMANTIC) != 0;
+  if (precision >= 0) {
+    uint32_t R = 0;
+    int rc = precision_bits(ch, precision, &R);
+    if (rc == BUFF_E_REF_PANIC && semantic) {
+      R = v.F;  // more decimals than stored: everything
+    } else if (rc != BUFF_OK) {
+      return rc;
+    }
+    if (!semantic) {
+      if (ch->ilen > 15)
+        return fail(BUFF_E_UNSUPPORTED, "ilen %u > 15: the reference parses an outlier section here", ch->ilen);
+      if (R / 8 > v.P)
+        return fail(BUFF_E_UNSUPPORTED, "the reference would read %u whole bytes of a %u-plane stream", R / 8, v.P);
+      mp.mode = kMatRefPrecision;
+      mp.planes = R / 8;
+      mp.tail_bits = R % 8;
+    } else if (R < v.F) {
+      mp.mode = kMatSemantic;
+      if (R <= 8 * v.P) {
+        mp.planes = (R + 7) / 8;
+        mp.tail_bits = 0;
+        mp.drop = 8 * mp.planes - R;
+      } else {
+        mp.planes = v.P;
+        mp.tail_bits = R - 8 * v.P;
+        mp.drop = 0;
+      }
+      mp.shift_back = v.F - R;
+    }
+  }
+  if (ch->n == 0) return BUFF_OK;
+  ProfScope sc(ctx, "k_materialize");
+  HIP_TRY(launch_materialize(v, mp, out, ctx->stream, ctx->shape(Op::Materialize, v.ntiles)));
+  return BUFF_OK;
+}
+
+int materialize_host(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* out) {
+  if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->n == 0) return BUFF_OK;
+  if (!out) return fail(BUFF_E_INVALID, "null output");
+  BUFF_TRY(ctx->scratch(ctx->s_values, ch->n * 8));
+  BUFF_TRY(materialize_device(ctx, ch, precision, flags, (double*)ctx->s_values.ptr));
+  HIP_TRY(hipMemcpyAsync(out, ctx->s_values.ptr, ch->n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return BUFF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// process-wide default context for the reference-shaped entry points
+
+static std::mutex g_default_mu;
+static Ctx* g_default = nullptr;
+
+int default_ctx(Ctx** out) {
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if (!g_default) {
+    buff_ctx* c = nullptr;
+    int rc = buff_ctx_create(0, 0, &c);
+    if (rc != BUFF_OK) return rc;
+    g_default = reinterpret_cast<Ctx*>(c);
+  }
+  *out = g_default;
+  return BUFF_OK;
+}
+
+}  // namespace buffgpu
+
+using namespace buffgpu;
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+const char* buff_last_error(void) { return g_last_error.c_str(); }
+
+int buff_ctx_create(int device, uint64_t hbm_budget_bytes, buff_ctx** out) {
+  if (!out) return fail(BUFF_E_INVALID, "null argument");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) return fail(BUFF_E_DEVICE, "no HIP device: %s", hipGetErrorString(e));
+  if (device < 0 || device >= ndev) return fail(BUFF_E_INVALID, "device %d of %d", device, ndev);
+  HIP_TRY(hipSetDevice(device));
+  Ctx* c = new (std::nothrow) Ctx();
+  if (!c) return fail(BUFF_E_OOM, "host allocation");
+  c->device = device;
+  c->budget = hbm_budget_bytes;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc(&c->pinned, Ctx::kPinnedBytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(BUFF_E_DEVICE, "context setup: %s", hipGetErrorString(e));
+  }
+  *out = reinterpret_cast<buff_ctx*>(c);
+  return BUFF_OK;
+}
+
+int buff_ctx_destroy(buff_ctx* ctx) {
+  if (!ctx) return BUFF_OK;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  {
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    if (c == g_default) g_default = nullptr;
+  }
+  (void)hipSetDevice(c->device);
+  delete c;
+  return BUFF_OK;
+}
+
+int buff_ctx_synchronize(buff_ctx* ctx) {
+  if (!ctx) return fail(BUFF_E_INVALID, "null argument");
+  HIP_TRY(hipStreamSynchronize(reinterpret_cast<Ctx*>(ctx)->stream));
+  return BUFF_OK;
+}
+
+int buff_ctx_hbm_used(buff_ctx* ctx, uint64_t* bytes) {
+  if (!ctx || !bytes) return fail(BUFF_E_INVALID, "null argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  std::lock_guard<std::mutex> lk(c->mu);
+  *bytes = c->held;
+  return BUFF_OK;
+}
+
+int buff_dev_alloc(buff_ctx* ctx, uint64_t bytes, void** dptr) {
+  if (!ctx || !dptr) return fail(BUFF_E_INVALID, "null argument");
+  return reinterpret_cast<Ctx*>(ctx)->alloc(bytes, dptr);
+}
+
+int buff_dev_free(buff_ctx* ctx, void* dptr) {
+  if (!ctx) return fail(BUFF_E_INVALID, "null argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return c->release(dptr);
+}
+
+int buff_memcpy_h2d(buff_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(BUFF_E_INVALID, "null argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return BUFF_OK;
+}
+
+int buff_memcpy_d2h(buff_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(BUFF_E_INVALID, "null argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return BUFF_OK;
+}
+
+int buff_prof_enable(buff_ctx* ctx, int on) {
+  if (!ctx) return fail(BUFF_E_INVALID, "null argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  BUFF_TRY(c->prof_collect());
+  c->prof_on = on != 0;
+  return BUFF_OK;
+}
+
+int buff_prof_reset(buff_ctx* ctx) {
+  if (!ctx) return fail(BUFF_E_INVALID, "null argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  BUFF_TRY(c->prof_collect());
+  c->prof_stats.clear();
+  return BUFF_OK;
+}
+
+int buff_prof_query(buff_ctx* ctx, const char* kernel, uint64_t* launches, double* total_ms) {
+  if (!ctx || !kernel) return fail(BUFF_E_INVALID, "null argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  BUFF_TRY(c->prof_collect());
+  auto it = c->prof_stats.find(kernel);
+  if (launches) *launches = it == c->prof_stats.end() ? 0 : it->second.first;
+  if (total_ms) *total_ms = it == c->prof_stats.end() ? 0.0 : it->second.second;
+  return BUFF_OK;
+}
+
+int buff_encode_device(buff_ctx* ctx, const double* x_dev, uint64_t n, uint64_t scale, uint32_t flags,
+                       buff_chunk** out) {
+  Chunk* ch = nullptr;
+  int rc = encode_device(reinterpret_cast<Ctx*>(ctx), x_dev, n, scale, flags, &ch);
+  if (rc == BUFF_OK) *out = reinterpret_cast<buff_chunk*>(ch);
+  return rc;
+}
+
+int buff_encode(buff_ctx* ctx, const double* x_host, uint64_t n, uint64_t scale, uint32_t flags, buff_chunk** out) {
+  if (!ctx || !out || (n && !x_host)) return fail(BUFF_E_INVALID, "null argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  void* dx = nullptr;
+  if (n) {
+    BUFF_TRY(c->alloc(n * 8, &dx));
+    hipError_t e = hipMemcpyAsync(dx, x_host, n * 8, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) {
+      (void)c->release(dx);
+      return fail(BUFF_E_DEVICE, "upload: %s", hipGetErrorString(e));
+    }
+  }
+  int rc = buff_encode_device(ctx, (const double*)dx, n, scale, flags, out);
+  if (dx) {
+    (void)hipStreamSynchronize(c->stream);
+    c->release(dx);
+  }
+  return rc;
+}
+
+int buff_chunk_info_get(const buff_chunk* chunk, buff_chunk_info* info) {
+  if (!chunk || !info) return fail(BUFF_E_INVALID, "null argument");
+  const Chunk* ch = reinterpret_cast<const Chunk*>(chunk);
+  info->base = ch->view.base;
+  info->n = (uint32_t)ch->n;
+  info->ilen = ch->ilen;
+  info->dlen = ch->dlen;
+  info->width = ch->view.F;
+  info->planes = ch->view.P;
+  info->tail_bits = ch->view.r;
+  info->flags = ch->flags;
+  info->scale = ch->scale;
+  info->stream_bytes = stream_bytes_of(ch->n, ch->view.P, ch->view.r);
+  info->device_bytes = ch->device_bytes;
+  return BUFF_OK;
+}
+
+int buff_chunk_serialize(buff_ctx* ctx, const buff_chunk* chunk, uint8_t* dst, uint64_t* len) {
+  return chunk_serialize(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), dst, len);
+}
+
+int buff_chunk_load(buff_ctx* ctx, const uint8_t* stream, uint64_t len, uint64_t scale, uint32_t flags,
+                    buff_chunk** out) {
+  Chunk* ch = nullptr;
+  int rc = chunk_load(reinterpret_cast<Ctx*>(ctx), stream, len, scale, flags, &ch);
+  if (rc == BUFF_OK) *out = reinterpret_cast<buff_chunk*>(ch);
+  return rc;
+}
+
+int buff_chunk_free(buff_chunk* chunk) {
+  if (!chunk) return BUFF_OK;
+  Chunk* ch = reinterpret_cast<Chunk*>(chunk);
+  Ctx* c = ch->ctx;
+  (void)hipStreamSynchronize(c->stream);
+  int rc = c->release(ch->mem);
+  delete ch;
+  return rc;
+}
+
+int buff_filter(buff_ctx* ctx, const buff_chunk* chunk, int op, double pred, uint32_t flags, uint32_t* words,
+                uint64_t nwords, uint64_t* count) {
+  return filter_host(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), op, pred, flags, words,
+                     nwords, count);
+}
+
+int buff_filter_device(buff_ctx* ctx, const buff_chunk* chunk, int op, double pred, uint32_t flags,
+                       uint64_t* words_dev, uint64_t* stats_dev) {
+  return filter_device(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), op, pred, flags,
+                       words_dev, (unsigned long long*)stats_dev);
+}
+
+int buff_sum(buff_ctx* ctx, const buff_chunk* chunk, double* ref_sum, double* exact_sum, uint64_t* plane_sums) {
+  return sum_op(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), ref_sum, exact_sum, plane_sums);
+}
+
+int buff_max(buff_ctx* ctx, const buff_chunk* chunk, double* max_value, uint64_t* max_off) {
+  return max_op(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), max_value, max_off);
+}
+
+int buff_materialize(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags, double* out) {
+  return materialize_host(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), precision, flags,
+                          out);
+}
+
+int buff_materialize_device(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags,
+                            double* out_dev) {
+  return materialize_device(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), precision, flags,
+                            out_dev);
+}
+
+// ---- reference-shaped entry points ---------------------------------------------
+
+static int sd_encode(const buff_split_double* sd, const double* data, uint64_t n, uint32_t flags, uint8_t* out,
+                     uint64_t cap, uint64_t* out_len) {
+  if (!sd || !out_len) return fail(BUFF_E_INVALID, "null argument");
+  Ctx* c = nullptr;
+  BUFF_TRY(default_ctx(&c));
+  buff_chunk* ch = nullptr;
+  BUFF_TRY(buff_encode(reinterpret_cast<buff_ctx*>(c), data, n, sd->scale, flags, &ch));
+  uint64_t len = cap;
+  int rc = buff_chunk_serialize(reinterpret_cast<buff_ctx*>(c), ch, out, &len);
+  *out_len = len;
+  (void)buff_chunk_free(ch);
+  return rc;
+}
+
+static int sd_load(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, uint32_t flags, Ctx** c,
+                   buff_chunk** ch) {
+  if (!sd || !bytes) return fail(BUFF_E_INVALID, "null argument");
+  BUFF_TRY(default_ctx(c));
+  return buff_chunk_load(reinterpret_cast<buff_ctx*>(*c), bytes, len, sd->scale, flags, ch);
+}
+
+int buff_sd_byte_fixed_encode(const buff_split_double* sd, const double* data, uint64_t n, uint8_t* out, uint64_t cap,
+                              uint64_t* out_len) {
+  return sd_encode(sd, data, n, 0, out, cap, out_len);
+}
+
+int buff_sd_buff_simd256_encode(const buff_split_double* sd, const double* data, uint64_t n, uint8_t* out,
+                                uint64_t cap, uint64_t* out_len) {
+  return sd_encode(sd, data, n, BUFF_FLIP, out, cap, out_len);
+}
+
+static int sd_decode(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, uint32_t flags, int32_t prec,
+                     double* out, uint64_t cap) {
+  Ctx* c = nullptr;
+  buff_chunk* ch = nullptr;
+  BUFF_TRY(sd_load(sd, bytes, len, flags, &c, &ch));
+  const uint64_t n = reinterpret_cast<Chunk*>(ch)->n;
+  int rc = BUFF_OK;
+  if (cap < n) rc = fail(BUFF_E_SMALL_BUFFER, "output needs %llu doubles", (unsigned long long)n);
+  if (rc == BUFF_OK) rc = buff_materialize(reinterpret_cast<buff_ctx*>(c), ch, prec, 0, out);
+  (void)buff_chunk_free(ch);
+  return rc;
+}
+
+int buff_sd_byte_fixed_decode(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* out,
+                              uint64_t cap) {
+  return sd_decode(sd, bytes, len, 0, -1, out, cap);
+}
+
+int buff_sd_buff_simd256_decode(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* out,
+                                uint64_t cap) {
+  return sd_decode(sd, bytes, len, BUFF_FLIP, -1, out, cap);
+}
+
+int buff_sd_byte_residue_decode_with_precision(const buff_split_double* sd, const uint8_t* bytes, uint64_t len,
+                                               uint32_t precision, double* out, uint64_t cap) {
+  if (precision > 0x7FFFFFFFu) return fail(BUFF_E_REF_PANIC, "precision %u is not in PRECISION_MAP", precision);
+  return sd_decode(sd, bytes, len, 0, (int32_t)precision, out, cap);
+}
+
+static int sd_filter(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, uint32_t flags, int op,
+                     double pred, uint32_t* words, uint64_t nwords, uint64_t* count) {
+  Ctx* c = nullptr;
+  buff_chunk* ch = nullptr;
+  BUFF_TRY(sd_load(sd, bytes, len, flags, &c, &ch));
+  int rc = buff_filter(reinterpret_cast<buff_ctx*>(c), ch, op, pred, 0, words, nwords, count);
+  (void)buff_chunk_free(ch);
+  return rc;
+}
+
+int buff_sd_buff_range_filter(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double pred,
+                              uint32_t* words, uint64_t nwords, uint64_t* count) {
+  return sd_filter(sd, bytes, len, 0, BUFF_OP_GT, pred, words, nwords, count);
+}
+
+int buff_sd_buff_equal_filter(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double pred,
+                              uint32_t* words, uint64_t nwords, uint64_t* count) {
+  return sd_filter(sd, bytes, len, 0, BUFF_OP_EQ, pred, words, nwords, count);
+}
+
+int buff_sd_buff_simd_range_filter(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double pred,
+                                   uint32_t* words, uint64_t nwords, uint64_t* count) {
+  return sd_filter(sd, bytes, len, BUFF_FLIP, BUFF_OP_GT, pred, words, nwords, count);
+}
+
+int buff_sd_buff_simd_equal_filter(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double pred,
+                                   uint32_t* words, uint64_t nwords, uint64_t* count) {
+  return sd_filter(sd, bytes, len, BUFF_FLIP, BUFF_OP_EQ, pred, words, nwords, count);
+}
+
+int buff_sd_byte_fixed_sum(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* s) {
+  Ctx* c = nullptr;
+  buff_chunk* ch = nullptr;
+  BUFF_TRY(sd_load(sd, bytes, len, 0, &c, &ch));
+  int rc = buff_sum(reinterpret_cast<buff_ctx*>(c), ch, s, nullptr, nullptr);
+  (void)buff_chunk_free(ch);
+  return rc;
+}
+
+int buff_sd_byte_fixed_max(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* max_value) {
+  Ctx* c = nullptr;
+  buff_chunk* ch = nullptr;
+  BUFF_TRY(sd_load(sd, bytes, len, 0, &c, &ch));
+  int rc = buff_max(reinterpret_cast<buff_ctx*>(c), ch, max_value, nullptr);
+  (void)buff_chunk_free(ch);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+// Shared host/device definitions of the resident BUFF layout.
+//
+// Resident layout of one chunk (n rows, F = P*8 + r bits per row), designed for
+// wave64 lanes and ballots:
+//   * Rows are grouped in tiles of 1024. Inside a tile, row 64*g + t (g = 0..15,
+//     t = 0..63) belongs to lane t, byte g of that lane's 16 bytes.
+//   * Byte sub-column ("plane") p is `pitch` bytes long; tile c of plane p is the
+//     1024 bytes at planes + p*pitch + 1024*c, lane t's 16 bytes at offset 16*t.
+//     A dwordx4 load per lane reads a whole tile of a plane coalesced, and
+//     `__ballot(byte g of each lane ...)` yields the 64-row bitmap word of rows
+//     64*g .. 64*g+63 of the tile directly in BitVec order.
+//   * The r tail bits are kept as bit-planes: tile c owns 16*r u64 words at
+//     tail + 16*r*c; word g*r + j holds bit j of rows 64*g+t (bit t).
+//   * Rows >= n are zero in every plane and bit-plane.
+// The reference byte stream (split_double.rs:349-414) is produced from and parsed
+// into this form by the serialize/load kernels.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define BUFF_HD __host__ __device__ __forceinline__
+#else
+#define BUFF_HD inline
+#endif
+
+namespace buffgpu {
+
+constexpr uint32_t kTileRows = 1024;
+constexpr uint32_t kGroups = 16;     // 64-row groups per tile
+constexpr uint32_t kMaxPlanes = 8;   // F <= 63 in practice; 8 slots
+constexpr uint32_t kStreamHeader = 20;
+
+// Device view of a resident chunk.
+struct ChunkView {
+  uint8_t* planes;    // plane p at planes + p * pitch
+  uint64_t* tail;     // bit-plane words, 16*r per tile
+  uint64_t pitch;     // bytes per plane = ntiles * 1024
+  uint64_t n;         // rows
+  uint64_t ntiles;    // ceil(n / 1024)
+  int64_t base;       // min fixed value
+  uint32_t F;         // stored width
+  uint32_t P;         // full byte planes (F >= 8 ? F/8 : 0)
+  uint32_t r;         // tail bits (F >= 8 ? F%8 : F)
+  uint32_t dlen;
+};
+
+// Number of valid rows in 64-row group g of tile c, as a 64-bit mask.
+BUFF_HD uint64_t group_valid_mask(uint64_t n, uint64_t tile, uint32_t g) {
+  uint64_t r0 = tile * kTileRows + (uint64_t)g * 64;
+  if (r0 + 64 <= n) return ~0ull;
+  if (r0 >= n) return 0ull;
+  return (1ull << (n - r0)) - 1ull;
+}
+
+// database/src/methods/prec_double.rs:190-205 (fetch_fixed_aligned), with Rust
+// release semantics: the shift amount is masked to 6 bits, negation wraps.
+BUFF_HD int64_t fetch_fixed_aligned_bits(uint64_t bdu, int32_t precision_exp, uint32_t dlen) {
+  int32_t exp = (int32_t)((bdu & 0x7FF0000000000000ull) >> 52) - 1023;
+  uint64_t fixed = 0;
+  if (exp >= precision_exp) {
+    int32_t sh = 63 - exp - (int32_t)dlen;
+    fixed = ((bdu << 11) | 0x8000000000000000ull) >> ((uint32_t)sh & 63u);
+    if (bdu & 0x8000000000000000ull) fixed = ~(fixed - 1);
+  }
+  return (int64_t)fixed;
+}
+
+// Per-filter constants computed on the host from the predicate.
+struct FilterTarget {
+  uint32_t digit[kMaxPlanes];  // target byte per plane: (t >> (F - 8(k+1))) as u8
+  uint64_t tail;               // target tail bits (t & (2^r - 1); the full t when P == 0)
+};
+
+// Encode-pass reduction results (minmax kernel -> host).
+struct MinMaxOut {
+  int64_t min_all;    // min over all rows
+  int64_t max_rest;   // max over rows i >= 1
+  int64_t first;      // fixed value of row 0
+  int64_t qmax;       // quirk max from the prefix-min pass
+  uint32_t done;      // workgroups finished (last-block reduction)
+  uint32_t pad;
+};
+
+}  // namespace buffgpu

@@ -1,0 +1,46 @@
+// Launchers of the HIP kernels in buff_kernels.hip (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "buff_common.h"
+
+namespace buffgpu {
+
+// Launch geometry of the tile kernels (chosen per op by the autotuner).
+struct LaunchShape {
+  uint32_t waves_per_wg;  // 1, 2, 4 (a workgroup is 64*waves_per_wg threads)
+  uint32_t max_wg;        // grid cap; waves grid-stride over tiles beyond it
+};
+
+enum : uint32_t { kMatFull = 0, kMatSemantic = 1, kMatRefPrecision = 2 };
+
+// Materialization recipe: read the top `planes` byte planes and `tail_bits`
+// bits below them, then value = base + ((v >> drop) << shift_back)
+// (kMatRefPrecision: tail_bits are read as fields of that width at that stride
+// from the packed reference tail, drop = shift_back = 0).
+struct MatParams {
+  uint32_t mode;
+  uint32_t planes;
+  uint32_t tail_bits;
+  uint32_t drop;
+  uint32_t shift_back;
+  uint32_t pad;
+  double inv_scale;  // exactly 2^-dlen
+};
+
+hipError_t launch_minmax(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t* partial,
+                         int64_t* partial2, MinMaxOut* out, uint32_t nwg, bool prefix, hipStream_t s);
+hipError_t launch_encode(const double* x, const ChunkView& cv, int32_t pexp, hipStream_t s, const LaunchShape& ls);
+hipError_t launch_filter(const ChunkView& cv, int op, const FilterTarget& tg, uint64_t* out, uint64_t nwords,
+                         unsigned long long* stats, hipStream_t s, const LaunchShape& ls);
+hipError_t launch_fill_bitmap(uint64_t* out, uint64_t nwords, uint64_t n, int ones, hipStream_t s);
+hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls);
+hipError_t launch_max(const ChunkView& cv, unsigned long long* best, hipStream_t s, const LaunchShape& ls);
+hipError_t launch_materialize(const ChunkView& cv, const MatParams& mp, double* out, hipStream_t s,
+                              const LaunchShape& ls);
+hipError_t launch_serialize(const ChunkView& cv, uint8_t flipmask, uint8_t* body, uint64_t tail_bytes,
+                            hipStream_t s);
+hipError_t launch_load(const ChunkView& cv, uint8_t flipmask, const uint8_t* body, uint64_t tail_bytes,
+                       hipStream_t s);
+
+}  // namespace buffgpu

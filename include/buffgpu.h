@@ -1,0 +1,187 @@
+/*
+ * buffgpu.h — C ABI of the MI355X-native BUFF (decomposed bounded floats) engine.
+ *
+ * The reference exposes its codec and queries as inherent methods of
+ * `SplitBDDoubleCompress` (database/src/compress/split_double.rs:29-39) and of the
+ * same struct in database/src/compress/buff_simd.rs. It has no FFI of its own; the
+ * entry points below are what a Rust crate keeping that API binds with
+ * `extern "C"` (ptr + len for every `Vec<u8>` / `&[f64]`, plain status codes
+ * instead of panics). Every `buff_sd_*` function names the reference method it
+ * replaces; the `buff_*` chunk functions are the resident (HBM) form those
+ * methods are built on.
+ *
+ * Conventions
+ *  - Every function returns a status code (BUFF_OK on success) and never unwinds.
+ *  - Host buffers are caller-owned. Device chunks are owned by the library and
+ *    released with buff_chunk_free.
+ *  - Functions taking a buff_ctx serialize on that context's HIP stream.
+ *    Use one context per host thread for concurrency.
+ *  - Bitmaps use the reference's BitVec<u32> layout (bit-vec/src/lib.rs:399-436,
+ *    619-627): row i is bit i%32 of u32 word i/32, unused high bits are 0.
+ *    On little-endian hosts a bitmap of ceil(n/64) u64 words has the same bytes.
+ */
+#ifndef BUFFGPU_H
+#define BUFFGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define BUFF_OK 0
+#define BUFF_E_INVALID 1      /* malformed stream, bad argument */
+#define BUFF_E_OOM 2          /* HBM budget or device allocation exhausted */
+#define BUFF_E_DEVICE 3       /* HIP runtime error */
+#define BUFF_E_REF_PANIC 4    /* an input on which the reference panics */
+#define BUFF_E_UNSUPPORTED 5  /* outside what the cited reference branch covers */
+#define BUFF_E_SMALL_BUFFER 6 /* caller buffer too small; the needed size is returned */
+
+/* ---- flags --------------------------------------------------------------- */
+/* Stream layout of buff_simd256_encode (buff_simd.rs:3116-3250): sub-column bytes
+ * XOR 0x80, tail bits unflipped. Only the serialized stream differs; the resident
+ * form is the same. */
+#define BUFF_FLIP 0x1u
+/* Return the mathematically intended answer where the reference has a quirk
+ * (equality bitmap left in rb1 when F%8==0, target-byte truncation for
+ * predicates above the column maximum, sum without the base term when F%8==0,
+ * unscaled reduced-precision decode). Default (0) reproduces the reference. */
+#define BUFF_SEMANTIC 0x2u
+
+/* filter operators */
+#define BUFF_OP_GT 0 /* x > pred (buff_range_filter, buff_simd.rs:4507-4679) */
+#define BUFF_OP_EQ 1 /* x == pred (buff_equal_filter, buff_simd.rs:4925-5093) */
+
+typedef struct buff_ctx buff_ctx;
+typedef struct buff_chunk buff_chunk;
+
+/* Header fields of a chunk (split_double.rs:349-354) plus derived widths. */
+typedef struct {
+  int64_t base;          /* min fixed value */
+  uint32_t n;            /* rows */
+  uint32_t ilen;         /* (F - dlen) mod 2^32, as stored */
+  uint32_t dlen;         /* PRECISION_MAP[p] */
+  uint32_t width;        /* F = ceil(log2(max - min)) */
+  uint32_t planes;       /* floor(F/8) full byte sub-columns (0 if F < 8) */
+  uint32_t tail_bits;    /* F mod 8 (F itself if F < 8) */
+  uint32_t flags;        /* BUFF_FLIP if the chunk serializes to the flipped layout */
+  uint64_t scale;        /* the SplitBDDoubleCompress scale it was encoded with */
+  uint64_t stream_bytes; /* size of the serialized reference stream */
+  uint64_t device_bytes; /* HBM held by the resident form */
+} buff_chunk_info;
+
+/* ---- context --------------------------------------------------------------- */
+int buff_ctx_create(int device, uint64_t hbm_budget_bytes /* 0 = no limit */, buff_ctx** out);
+int buff_ctx_destroy(buff_ctx* ctx);
+int buff_ctx_synchronize(buff_ctx* ctx);
+/* HBM bytes currently held by chunks and scratch of this context. */
+int buff_ctx_hbm_used(buff_ctx* ctx, uint64_t* bytes);
+/* Message of the last failing call on this thread ("" if none). */
+const char* buff_last_error(void);
+
+/* Plain device memory for callers that keep inputs resident (counted against
+ * the HBM budget). */
+int buff_dev_alloc(buff_ctx* ctx, uint64_t bytes, void** dptr);
+int buff_dev_free(buff_ctx* ctx, void* dptr);
+int buff_memcpy_h2d(buff_ctx* ctx, void* dst_dev, const void* src_host, uint64_t bytes);
+int buff_memcpy_d2h(buff_ctx* ctx, void* dst_host, const void* src_dev, uint64_t bytes);
+
+/* Per-kernel timing with HIP events recorded on the context's stream.
+ * buff_prof_query reports launches and summed durations for one kernel name. */
+int buff_prof_enable(buff_ctx* ctx, int on);
+int buff_prof_reset(buff_ctx* ctx);
+int buff_prof_query(buff_ctx* ctx, const char* kernel, uint64_t* launches, double* total_ms);
+
+/* ---- encode / resident chunks ---------------------------------------------- */
+/* byte_fixed_encode (split_double.rs:298-432) into the resident form.
+ * x_dev: n doubles in device memory. BUFF_FLIP selects buff_simd256_encode's
+ * serialized layout. Returns BUFF_E_REF_PANIC where the reference panics
+ * (F == 0 with n >= 1; scale whose precision is not in PRECISION_MAP). */
+int buff_encode_device(buff_ctx* ctx, const double* x_dev, uint64_t n, uint64_t scale, uint32_t flags,
+                       buff_chunk** out);
+/* Same, from host memory (uploads first). */
+int buff_encode(buff_ctx* ctx, const double* x_host, uint64_t n, uint64_t scale, uint32_t flags,
+                buff_chunk** out);
+int buff_chunk_info_get(const buff_chunk* chunk, buff_chunk_info* info);
+/* Writes the reference byte stream. dst == NULL: only *len is set. */
+int buff_chunk_serialize(buff_ctx* ctx, const buff_chunk* chunk, uint8_t* dst, uint64_t* len);
+/* Parses a reference byte stream (BUFF_FLIP: a buff_simd256 stream) into the
+ * resident form. `scale` is the caller's SplitBDDoubleCompress scale. */
+int buff_chunk_load(buff_ctx* ctx, const uint8_t* stream, uint64_t len, uint64_t scale, uint32_t flags,
+                    buff_chunk** out);
+int buff_chunk_free(buff_chunk* chunk);
+
+/* ---- queries on a resident chunk ------------------------------------------ */
+/* Filter into host words: ceil(n/32) u32 words; *count = set bits (the count the
+ * reference prints). */
+int buff_filter(buff_ctx* ctx, const buff_chunk* chunk, int op, double pred, uint32_t flags, uint32_t* words,
+                uint64_t nwords, uint64_t* count);
+/* Filter into device memory (ceil(n/64) u64 words, 8-byte aligned); no host sync.
+ * stats (nullable, device, 9 u64, zeroed by the caller) accumulates rows
+ * entering sub-column k >= 1 (index k) and the tail (index planes). */
+int buff_filter_device(buff_ctx* ctx, const buff_chunk* chunk, int op, double pred, uint32_t flags,
+                       uint64_t* words_dev, uint64_t* stats_dev);
+/* byte_fixed_sum (split_double.rs:2127-2208). ref_sum reproduces the reference
+ * f64 fold bit for bit; exact_sum = (n*base + sum(off)) / 2^dlen rounded once;
+ * plane_sums (nullable, 9 u64) are the exact per-sub-column byte sums (tail last). */
+int buff_sum(buff_ctx* ctx, const buff_chunk* chunk, double* ref_sum, double* exact_sum, uint64_t* plane_sums);
+/* byte_fixed_max (split_double.rs:3919-4105): the printed max value and the max
+ * stored offset. */
+int buff_max(buff_ctx* ctx, const buff_chunk* chunk, double* max_value, uint64_t* max_off);
+/* byte_fixed_decode (split_double.rs:1243-1350) for precision < 0, else
+ * byte_residue_decode_with_precision (split_double.rs:1612-1893) at `precision`
+ * decimals. out: n doubles (host). */
+int buff_materialize(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags, double* out);
+int buff_materialize_device(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags,
+                            double* out_dev);
+
+/* ---- reference-shaped entry points (byte streams in host memory) ----------
+ * Each takes the SplitBDDoubleCompress fields {chunksize, batchsize, scale}
+ * (split_double.rs:29-39) and the stream as ptr+len, mirrors one method, and
+ * runs it on the GPU of a process-wide default context (device 0). */
+typedef struct {
+  uint64_t chunksize;
+  uint64_t batchsize;
+  uint64_t scale;
+} buff_split_double;
+
+/* split_double.rs:298 byte_fixed_encode / buff_simd.rs:3116 buff_simd256_encode.
+ * out == NULL: size query. */
+int buff_sd_byte_fixed_encode(const buff_split_double* sd, const double* data, uint64_t n, uint8_t* out,
+                              uint64_t cap, uint64_t* out_len);
+int buff_sd_buff_simd256_encode(const buff_split_double* sd, const double* data, uint64_t n, uint8_t* out,
+                                uint64_t cap, uint64_t* out_len);
+/* split_double.rs:1243 byte_fixed_decode / buff_simd.rs:3252 buff_simd256_decode.
+ * out: n doubles (n is in the stream header). */
+int buff_sd_byte_fixed_decode(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* out,
+                              uint64_t cap);
+int buff_sd_buff_simd256_decode(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* out,
+                                uint64_t cap);
+/* buff_simd.rs:4507 buff_range_filter / 4925 buff_equal_filter, and the
+ * buff-simd twins buff_simd.rs:3361 buff_simd_range_filter / 3549
+ * buff_simd_equal_filter over a flipped stream. The reference prints the count;
+ * here it is returned with the bitmap (words: ceil(n/32) u32, nullable). */
+int buff_sd_buff_range_filter(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double pred,
+                              uint32_t* words, uint64_t nwords, uint64_t* count);
+int buff_sd_buff_equal_filter(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double pred,
+                              uint32_t* words, uint64_t nwords, uint64_t* count);
+int buff_sd_buff_simd_range_filter(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double pred,
+                                   uint32_t* words, uint64_t nwords, uint64_t* count);
+int buff_sd_buff_simd_equal_filter(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double pred,
+                                   uint32_t* words, uint64_t nwords, uint64_t* count);
+/* split_double.rs:2127 byte_fixed_sum */
+int buff_sd_byte_fixed_sum(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* sum);
+/* split_double.rs:3919 byte_fixed_max (prints the max; returned here) */
+int buff_sd_byte_fixed_max(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* max_value);
+/* split_double.rs:1612 byte_residue_decode_with_precision on a byte_fixed
+ * stream (the ilen <= 15 branch; larger ilen is BUFF_E_UNSUPPORTED). */
+int buff_sd_byte_residue_decode_with_precision(const buff_split_double* sd, const uint8_t* bytes, uint64_t len,
+                                               uint32_t precision, double* out, uint64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BUFFGPU_H */

@@ -1,0 +1,227 @@
+"""ctypes loader for the oracle (oracle/librefcpu.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker and by bench.py's cpu_baseline leg. The product (buff_amd/) never
+imports it.
+
+Each wrapper returns numpy arrays / Python scalars and raises RefPanic where
+the reference would panic (status 4) and RefUnsupported where the stream shape
+is outside the cited branch (status 5).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "librefcpu.so")
+
+REF_OK, REF_PANIC, REF_UNSUPPORTED, REF_SMALL_BUFFER = 0, 4, 5, 6
+
+
+class RefPanic(RuntimeError):
+    """The reference would panic on this input."""
+
+
+class RefUnsupported(RuntimeError):
+    """The cited reference branch does not cover this stream."""
+
+
+def build(force=False):
+    """Compile librefcpu.so from refcpu.cpp with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.POINTER
+        u8p, u32p, u64p, i64p, f64p = (P(ctypes.c_uint8), P(ctypes.c_uint32), P(ctypes.c_uint64),
+                                       P(ctypes.c_int64), P(ctypes.c_double))
+        sig = {
+            "ref_prec_of_scale": (ctypes.c_int, [ctypes.c_uint64]),
+            "ref_get_precision_bound": (ctypes.c_double, [ctypes.c_int32]),
+            "ref_precision_exp": (ctypes.c_int32, [ctypes.c_double]),
+            "ref_precision_map": (ctypes.c_int, [ctypes.c_int32, P(ctypes.c_int32)]),
+            "ref_fetch_fixed_aligned": (ctypes.c_int64, [ctypes.c_double, ctypes.c_int32, ctypes.c_uint64]),
+            "ref_powi": (ctypes.c_double, [ctypes.c_double, ctypes.c_int32]),
+            "ref_bitpack_write": (ctypes.c_int, [u32p, u32p, u8p, ctypes.c_uint64, u8p, ctypes.c_uint64, u64p]),
+            "ref_bitpack_read": (ctypes.c_int, [u8p, ctypes.c_uint64, u32p, ctypes.c_uint64, u32p]),
+            "ref_encode": (ctypes.c_int, [f64p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, u8p,
+                                          ctypes.c_uint64, u64p, i64p]),
+            "ref_decode": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, f64p,
+                                          ctypes.c_uint64, u64p]),
+            "ref_filter": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_double,
+                                          u32p, ctypes.c_uint64, u64p, u64p]),
+            "ref_simd_filter": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                               ctypes.c_double, u32p, ctypes.c_uint64, u64p]),
+            "ref_sum": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, f64p, u64p]),
+            "ref_max": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, f64p, u64p, u64p]),
+            "ref_decode_with_precision": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                                         f64p, ctypes.c_uint64, u64p]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def _check(rc):
+    if rc == REF_PANIC:
+        raise RefPanic("reference panics on this input")
+    if rc == REF_UNSUPPORTED:
+        raise RefUnsupported("outside the cited reference branch")
+    if rc != REF_OK:
+        raise RuntimeError("oracle status %d" % rc)
+
+
+def prec_of_scale(scale):
+    return lib().ref_prec_of_scale(scale)
+
+
+def get_precision_bound(p):
+    return lib().ref_get_precision_bound(p)
+
+
+def precision_exp(precision):
+    return lib().ref_precision_exp(precision)
+
+
+def precision_map(p):
+    d = ctypes.c_int32()
+    _check(lib().ref_precision_map(p, ctypes.byref(d)))
+    return d.value
+
+
+def fetch_fixed_aligned(x, precision_exp_, dlen):
+    return lib().ref_fetch_fixed_aligned(float(x), precision_exp_, dlen)
+
+
+def powi(a, b):
+    return lib().ref_powi(a, b)
+
+
+def bitpack_write(ops):
+    """ops: list of (kind, value, bits); kind in {'write','write_bits','write_byte','finish_write_byte'}."""
+    kinds = {"write": 0, "write_bits": 1, "write_byte": 2, "finish_write_byte": 3}
+    k = len(ops)
+    vals = np.array([o[1] for o in ops], dtype=np.uint32)
+    bits = np.array([o[2] for o in ops], dtype=np.uint32)
+    kind = np.array([kinds[o[0]] for o in ops], dtype=np.uint8)
+    cap = 64 + sum(int(b) for b in bits) // 8 + k
+    out = np.zeros(cap, dtype=np.uint8)
+    n = ctypes.c_uint64()
+    _check(lib().ref_bitpack_write(_ptr(vals, ctypes.c_uint32), _ptr(bits, ctypes.c_uint32),
+                                   _ptr(kind, ctypes.c_uint8), k, _ptr(out, ctypes.c_uint8), cap, ctypes.byref(n)))
+    return out[: n.value].copy()
+
+
+def bitpack_read(buf, bits):
+    buf = np.ascontiguousarray(buf, dtype=np.uint8)
+    bits = np.ascontiguousarray(bits, dtype=np.uint32)
+    vals = np.zeros(len(bits), dtype=np.uint32)
+    _check(lib().ref_bitpack_read(_ptr(buf, ctypes.c_uint8), len(buf), _ptr(bits, ctypes.c_uint32), len(bits),
+                                  _ptr(vals, ctypes.c_uint32)))
+    return vals
+
+
+def encode(x, scale, flip=False, with_minmax=False):
+    """byte_fixed_encode (flip=False) / buff_simd256_encode (flip=True) -> uint8 stream."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = ctypes.c_uint64()
+    mm = np.zeros(2, dtype=np.int64)
+    L = lib()
+    _check(L.ref_encode(_ptr(x, ctypes.c_double), len(x), scale, int(flip), None, 0, ctypes.byref(n),
+                        _ptr(mm, ctypes.c_int64)))
+    out = np.zeros(n.value, dtype=np.uint8)
+    _check(L.ref_encode(_ptr(x, ctypes.c_double), len(x), scale, int(flip), _ptr(out, ctypes.c_uint8), n.value,
+                        ctypes.byref(n), _ptr(mm, ctypes.c_int64)))
+    return (out, int(mm[0]), int(mm[1])) if with_minmax else out
+
+
+def header(stream):
+    s = np.asarray(stream, dtype=np.uint8)
+    h = s[:20].tobytes()
+    base = int.from_bytes(h[0:8], "little", signed=True)
+    n = int.from_bytes(h[8:12], "little")
+    ilen = int.from_bytes(h[12:16], "little")
+    dlen = int.from_bytes(h[16:20], "little")
+    return base, n, ilen, dlen
+
+
+def decode(stream, scale, flip=False):
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    n = header(s)[1]
+    out = np.zeros(max(n, 1), dtype=np.float64)
+    got = ctypes.c_uint64()
+    _check(lib().ref_decode(_ptr(s, ctypes.c_uint8), len(s), scale, int(flip), _ptr(out, ctypes.c_double),
+                            len(out), ctypes.byref(got)))
+    return out[: got.value]
+
+
+def buff_filter(stream, scale, op, pred):
+    """op 'range' (x > pred) or 'equal'. Returns (u32 words, count, candidates[9])."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    n = header(s)[1]
+    words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
+    cnt = ctypes.c_uint64()
+    cand = np.zeros(9, dtype=np.uint64)
+    _check(lib().ref_filter(_ptr(s, ctypes.c_uint8), len(s), scale, 0 if op == "range" else 1, float(pred),
+                            _ptr(words, ctypes.c_uint32), len(words), ctypes.byref(cnt), _ptr(cand, ctypes.c_uint64)))
+    return words[: (n + 31) // 32], cnt.value, cand
+
+
+def simd_filter(stream, scale, op, pred):
+    """AVX2 buff-simd filter over a flipped stream. Returns (u32 words, count)."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    n = header(s)[1]
+    words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
+    cnt = ctypes.c_uint64()
+    _check(lib().ref_simd_filter(_ptr(s, ctypes.c_uint8), len(s), scale, 0 if op == "range" else 1, float(pred),
+                                 _ptr(words, ctypes.c_uint32), len(words), ctypes.byref(cnt)))
+    return words[: (n + 31) // 32], cnt.value
+
+
+def byte_fixed_sum(stream, scale):
+    """byte_fixed_sum -> (f64 sum, per-sub-column u64 sums[9])."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    out = ctypes.c_double()
+    ps = np.zeros(9, dtype=np.uint64)
+    _check(lib().ref_sum(_ptr(s, ctypes.c_uint8), len(s), scale, ctypes.byref(out), _ptr(ps, ctypes.c_uint64)))
+    return out.value, ps
+
+
+def byte_fixed_max(stream, scale):
+    """byte_fixed_max -> (printed max value, max stored offset, argmax count)."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    out = ctypes.c_double()
+    off = ctypes.c_uint64()
+    cnt = ctypes.c_uint64()
+    _check(lib().ref_max(_ptr(s, ctypes.c_uint8), len(s), scale, ctypes.byref(out), ctypes.byref(off),
+                         ctypes.byref(cnt)))
+    return out.value, off.value, cnt.value
+
+
+def decode_with_precision(stream, scale, precision):
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    n = header(s)[1]
+    out = np.zeros(max(n, 1), dtype=np.float64)
+    got = ctypes.c_uint64()
+    _check(lib().ref_decode_with_precision(_ptr(s, ctypes.c_uint8), len(s), scale, precision,
+                                           _ptr(out, ctypes.c_double), len(out), ctypes.byref(got)))
+    return out[: got.value]
+

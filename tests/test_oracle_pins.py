@@ -1,0 +1,247 @@
+"""Pins the oracle (oracle/refcpu.cpp) before it is trusted as the GPU checker.
+
+Known answers come from the reference's own tests (BitPack goldens), from the
+reference's tables, and otherwise from closed forms and brute force
+(tests/closed_form.py) that a plausible slip in the line-by-line restatement
+(a dropped term, a wrong shift, an off-by-one in a cursor) would fail.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests import closed_form as cf
+
+
+# ---- BitPack: database/src/methods/bit_packing.rs ---------------------------------
+
+def test_bitpack_doc_golden(oracle):
+    # bit_packing.rs:57-71, 141-153, 297-312: write(10,4), write(1021,10), write(3,2) -> [218, 255]
+    out = oracle.bitpack_write([("write", 10, 4), ("write", 1021, 10), ("write", 3, 2)])
+    assert list(out) == [218, 255]
+    assert list(oracle.bitpack_read(out, [4, 10, 2])) == [10, 1021, 3]
+
+
+def test_bitpack_smallbit(oracle):  # bit_packing.rs:771-783
+    out = oracle.bitpack_write([("write", 1, 1), ("write", 0, 1), ("write", 0, 1), ("write", 1, 1)])
+    assert list(oracle.bitpack_read(out, [1, 1, 1, 1])) == [1, 0, 0, 1]
+
+
+def test_bitpack_bigbit(oracle):  # bit_packing.rs:786-800
+    vals = [(255, 8), (65535, 16), (65535, 16), (255, 8), (65535, 16)]
+    out = oracle.bitpack_write([("write", v, b) for v, b in vals])
+    assert list(oracle.bitpack_read(out, [b for _, b in vals])) == [v for v, _ in vals]
+
+
+def test_bitpack_moresmallbit(oracle):  # bit_packing.rs:803-819
+    inp = [1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 1, 1, 1, 1, 0, 0, 1, 1, 0, 1, 1, 1, 0, 0, 0, 1, 1, 0, 1, 0, 0, 0, 1,
+           0, 0, 1, 0, 0, 1, 0, 0, 0, 1, 0, 1, 1, 0, 0, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 1]
+    out = oracle.bitpack_write([("write", b, 1) for b in inp])
+    assert list(oracle.bitpack_read(out, [1] * len(inp))) == inp
+
+
+def test_bitpack_lsb_first_random(oracle):
+    # write(v, b) appends the low b bits of v LSB-first: compare with numpy packbits
+    rng = np.random.default_rng(7)
+    bits = rng.integers(1, 33, 300)
+    vals = [int(rng.integers(0, 1 << int(b))) for b in bits]
+    out = oracle.bitpack_write([("write", v, int(b)) for v, b in zip(vals, bits)])
+    stream = np.concatenate([[(v >> k) & 1 for k in range(int(b))] for v, b in zip(vals, bits)]).astype(np.uint8)
+    assert np.array_equal(out, np.packbits(stream, bitorder="little"))
+    assert list(oracle.bitpack_read(out, bits)) == vals
+
+
+def test_bitpack_finish_write_byte_layout(oracle):
+    # header, two whole bytes, then finish_write_byte and 3-bit fields: the
+    # tail starts on the byte after the pushed bytes (split_double.rs:363-414)
+    ops = [("write", 0xDEADBEEF, 32), ("write_byte", 7, 0), ("write_byte", 9, 0), ("finish_write_byte", 0, 0),
+           ("write_bits", 5, 3), ("write_bits", 2, 3), ("write_bits", 7, 3)]
+    out = oracle.bitpack_write(ops)
+    assert list(out[:4]) == [0xEF, 0xBE, 0xAD, 0xDE]
+    assert list(out[4:6]) == [7, 9]
+    assert list(out[6:]) == list(np.packbits(np.array([1, 0, 1, 0, 1, 0, 1, 1, 1], np.uint8), bitorder="little"))
+
+
+# ---- precision setup: compress/mod.rs:29-46, prec_double.rs:28-33, 273-281 ------
+
+def test_precision_map_table(oracle):
+    assert [oracle.precision_map(p) for p in range(1, 16)] == [cf.PRECISION_MAP[p] for p in range(1, 16)]
+    for bad in (0, 16, -1):
+        with pytest.raises(oracle.RefPanic):
+            oracle.precision_map(bad)
+
+
+def test_prec_of_scale(oracle):
+    # (scale as f32).log10() as i32 == p for 10^p, p = 1..7
+    assert [oracle.prec_of_scale(10 ** p) for p in range(1, 8)] == list(range(1, 8))
+    assert oracle.prec_of_scale(99) == 1 and oracle.prec_of_scale(100) == 2
+
+
+def test_precision_exp_equals_minus_dlen(oracle):
+    # the bound 0.0..049 has binary exponent -PRECISION_MAP[p] for p = 1..11 and not for 12
+    for p in range(1, 12):
+        assert oracle.precision_exp(oracle.get_precision_bound(p)) == -cf.PRECISION_MAP[p]
+        assert oracle.get_precision_bound(p) == float("0." + "0" * p + "49")
+    assert oracle.precision_exp(oracle.get_precision_bound(12)) != -cf.PRECISION_MAP[12]
+
+
+def _fetch_fixed_literal(bits, pexp, dlen):
+    """prec_double.rs:190-205 transcribed with Python ints and masked shifts."""
+    exp = ((bits >> 52) & 0x7FF) - 1023
+    if exp < pexp:
+        return 0
+    sh = (63 - exp - dlen) & 63
+    fixed = ((((bits << 11) & ((1 << 64) - 1)) | (1 << 63)) >> sh)
+    if bits >> 63:
+        fixed = (~(fixed - 1)) & ((1 << 64) - 1)
+    return fixed - (1 << 64) if fixed >= 1 << 63 else fixed
+
+
+def test_fetch_fixed_closed_form(oracle):
+    rng = np.random.default_rng(11)
+    for p in range(1, 7):
+        dlen = cf.PRECISION_MAP[p]
+        pexp = -dlen
+        mags = 10.0 ** rng.uniform(-8, 12, 4000)
+        x = np.round(mags * np.where(rng.random(4000) < 0.5, -1, 1), p)
+        x = np.concatenate([x, [0.0, -0.0, 2.0 ** -dlen, -(2.0 ** -dlen), 2.0 ** -(dlen + 1), 1.0, -1.0, 0.5,
+                                2.0 ** 40, 5e-324, 1e-310]])
+        x = x[np.abs(x) < 2.0 ** (60 - dlen)]
+        got = np.array([oracle.fetch_fixed_aligned(v, pexp, dlen) for v in x], dtype=np.int64)
+        assert np.array_equal(got, cf.fixed(x, dlen)), p
+        # symmetry: fixed(-x) == -fixed(x)
+        got_neg = np.array([oracle.fetch_fixed_aligned(-v, pexp, dlen) for v in x], dtype=np.int64)
+        assert np.array_equal(got_neg, -got)
+
+
+def test_fetch_fixed_random_bit_patterns(oracle):
+    # out-of-contract inputs (huge, NaN, Inf) follow Rust's masked shifts
+    rng = np.random.default_rng(12)
+    pats = rng.integers(0, 2 ** 63, 3000, dtype=np.uint64) * 2 + rng.integers(0, 2, 3000, dtype=np.uint64)
+    for dlen, pexp in ((15, -15), (8, -8), (21, -21)):
+        for b in pats[:1000]:
+            v = np.array([b], dtype=np.uint64).view(np.float64)[0]
+            assert oracle.fetch_fixed_aligned(v, pexp, dlen) == _fetch_fixed_literal(int(b), pexp, dlen)
+
+
+# ---- codec and queries against the closed forms ---------------------------------
+
+def _cases():
+    from buff_amd import datagen as dg
+
+    rng = np.random.default_rng(3)
+    yield "walk4", dg.random_walk(5000, 4, 1), 10 ** 4
+    yield "walk4_odd", dg.random_walk(4099, 4, 2), 10 ** 4
+    yield "uniform2", dg.uniform(3333, -1000, 1000, 2, 3), 10 ** 2
+    yield "outlier3", dg.outlier_heavy(4000, 3, 4), 10 ** 3
+    yield "tiny_F", np.round(rng.integers(0, 5, 1000) * 0.1, 1), 10       # F < 8
+    yield "F16", np.round(np.linspace(0, 2047.9, 2000), 1), 10        # F = 16: F % 8 == 0
+    yield "F16_base", np.round(np.linspace(100, 2147.9, 2000), 1), 10  # F = 16 with a non-zero base
+    yield "desc_first_max", np.concatenate([[50.0], np.linspace(0, 10, 3000)]), 10 ** 2   # Q1: row 0 is the unique max
+    yield "neg_walk", dg.random_walk(2500, 3, 9) - 500, 10 ** 3
+    yield "p6", dg.random_walk(3000, 6, 10) * 1e-3, 10 ** 6   # F < dlen: ilen wraps
+    yield "single", np.array([3.25, 3.5]), 10 ** 2
+
+
+@pytest.mark.parametrize("name,x,scale", list(_cases()))
+def test_encode_matches_closed_form(oracle, name, x, scale):
+    for flip in (False, True):
+        got = oracle.encode(x, scale, flip=flip)
+        exp = cf.encode(x, scale, flip=flip)
+        assert np.array_equal(got, exp), name
+
+
+@pytest.mark.parametrize("name,x,scale", list(_cases()))
+def test_decode_and_queries_match_closed_form(oracle, name, x, scale):
+    s = oracle.encode(x, scale)
+    h = cf.parse(s)
+    assert np.array_equal(oracle.decode(s, scale).view(np.uint64), cf.decode(s).view(np.uint64))
+    sf = oracle.encode(x, scale, flip=True)
+    assert np.array_equal(oracle.decode(sf, scale, flip=True).view(np.uint64), cf.decode(s).view(np.uint64))
+    rng = np.random.default_rng(5)
+    preds = list(rng.choice(x, 6)) + [float(x.min()) - 1, float(x.max()) + 1, float(np.median(x)),
+                                      float(x.max()) * 4 + 1e4]
+    for pred in preds:
+        for op in ("range", "equal"):
+            words, cnt, _ = oracle.buff_filter(s, scale, op, pred)
+            m = cf.filter_bits(s, scale, op, pred)
+            assert np.array_equal(words, cf.to_words(m)), (name, op, pred)
+            assert cnt == int(m.sum())
+    # sum: exact per-plane sums; the f64 result within 1e-12 of the exact
+    # value (without the base term when F % 8 == 0, as the reference returns early)
+    ssum, ps = oracle.byte_fixed_sum(s, scale)
+    assert list(ps[: len(cf.plane_sums(s))]) == cf.plane_sums(s)
+    tot, base_term = cf.exact_sum(s)
+    want = tot - base_term if (h["F"] >= 8 and h["F"] % 8 == 0) else tot
+    assert abs(ssum - float(want)) <= 1e-12 * max(1.0, abs(float(want))) + 2 ** -h["dlen"]
+    mval, moff, _ = oracle.byte_fixed_max(s, scale)
+    assert moff == int(h["off"].max())
+    assert mval == cf.decode(s).max()
+
+
+def test_quirk_max_changes_width(oracle):
+    # Q1: row 0 is the unique maximum; the `else if` loop never records it
+    x = np.array([100.0, 1.0, 2.0, 3.0, 1.5])
+    s, mn, mx = oracle.encode(x, 10, with_minmax=True)
+    f = cf.fixed(x, 5)
+    assert mn == f.min() and mx == f[1:].max()
+    assert cf.parse(s)["F"] == math.ceil(math.log2(int(f[1:].max() - f.min())))
+
+
+def test_width_zero_panics(oracle):
+    with pytest.raises(oracle.RefPanic):
+        oracle.encode(np.full(10, 1.25), 100)
+    with pytest.raises(oracle.RefPanic):
+        oracle.encode(np.array([1.0, 2.0]), 1)   # scale 1 -> precision 0 not in PRECISION_MAP
+
+
+def test_empty_column(oracle):
+    s = oracle.encode(np.zeros(0), 10 ** 4)
+    assert len(s) == 20
+    assert oracle.header(s) == (2 ** 63 - 1, 0, (0 - 15) & 0xFFFFFFFF, 15)
+
+
+@pytest.mark.parametrize("name,x,scale", list(_cases()))
+def test_precision_decode(oracle, name, x, scale):
+    s = oracle.encode(x, scale)
+    h = cf.parse(s)
+    for p in range(0, 7):
+        try:
+            R = cf.precision_R(h, p)
+        except (ValueError, KeyError):
+            with pytest.raises(oracle.RefPanic):
+                oracle.decode_with_precision(s, scale, p)
+            continue
+        if h["ilen"] > 15:
+            with pytest.raises(oracle.RefUnsupported):
+                oracle.decode_with_precision(s, scale, p)
+            continue
+        if R % 8 == 0 and R // 8 <= h["P"] and R <= h["F"]:
+            # whole planes only: value = base + (off >> (F - R)), unshifted (Q12)
+            got = oracle.decode_with_precision(s, scale, p)
+            top = h["off"] >> np.uint64(h["F"] - R)
+            v = (np.uint64(h["base"] & ((1 << 64) - 1)) + top).view(np.int64).astype(np.float64) / 2.0 ** h["dlen"]
+            assert np.array_equal(got.view(np.uint64), v.view(np.uint64)), (name, p)
+        if R == h["F"]:
+            got = oracle.decode_with_precision(s, scale, p)
+            assert np.array_equal(got.view(np.uint64), cf.decode(s).view(np.uint64))
+
+
+@pytest.mark.parametrize("name,x,scale", list(_cases()))
+def test_simd_filters_match_scalar_on_full_words(oracle, name, x, scale):
+    # Q11: the AVX2 stages see floor(n/32)*32 rows; on those rows the answer is the scalar one
+    s = oracle.encode(x, scale)
+    sf = oracle.encode(x, scale, flip=True)
+    n = len(x)
+    full = (n // 32) * 32
+    rng = np.random.default_rng(8)
+    for pred in list(rng.choice(x, 4)) + [float(np.median(x))]:
+        for op in ("range", "equal"):
+            m = cf.filter_bits(s, scale, op, pred)
+            w, _ = oracle.simd_filter(sf, scale, op, pred)
+            got = np.unpackbits(w.view(np.uint8), bitorder="little")[:n].astype(bool)
+            h = cf.parse(s)
+            if h["F"] < 8 or cf.fixed_pred(pred, scale, h["dlen"]) < h["base"]:
+                assert np.array_equal(got, m)
+            else:
+                assert np.array_equal(got[:full], m[:full]), (name, op, pred)
