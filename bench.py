@@ -1,0 +1,317 @@
+"""BUFF hot path on one MI355X: encode, range filters, sum, max, materialize.
+
+Workload (one step): the 268,435,456-value job of two float64 columns of
+134,217,728 rows each, resident in HBM before timing starts:
+  walk    — random walk x_{t+1} = x_t + N(0,1), 4 decimals, scale 10^4
+  uniform — U[-1000, 1000], 2 decimals, scale 10^2
+A step encodes both columns, runs range filters (x > pred) at 0.1%, 10% and
+50% selectivity on each, sums and maxes both, and materializes the walk at 4
+and at 2 decimals and the uniform column at 2 decimals (its 4-decimal request
+is a reference panic). `value` is GB/s of float64 input: 8 bytes per row per
+operation over the step time.
+
+Per-kernel durations come from HIP events recorded on the library's stream
+during the timed steps. `roofline` is reported for the kernel with the largest
+share of the step, from its algorithmic bytes (HBM bytes an ideal kernel must
+move) over its average duration against 8 TB/s. A cold-cache pass (memory-side
+cache evicted before each launch) is reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "range-filter/sum/encode GB/s vs HBM roofline; batched-query p50/p99 latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--rows", type=int, default=1 << 27, help="rows per column (default 134,217,728)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-rows", type=int, default=1 << 25)
+    return ap.parse_args()
+
+
+class Column:
+    def __init__(self, name, x, scale, mat_precisions):
+        self.name = name
+        self.n = len(x)
+        self.scale = scale
+        self.mat_precisions = mat_precisions
+        self.preds = {sel: float(np.quantile(x, 1.0 - sel, method="lower")) for sel in (0.001, 0.10, 0.50)}
+
+
+def algorithmic_bytes(info, op, stats=None, precision=-1):
+    """HBM bytes an ideal implementation of one launch must move."""
+    n, F, P, r = info.n, info.width, info.planes, info.tail_bits
+    if op == "k_minmax":
+        return 8.0 * n
+    if op == "k_encode":
+        return 8.0 * n + n * F / 8.0
+    if op == "k_filter_gt":
+        b = n * (1 + 1 / 8.0)
+        if stats is not None:
+            b += float(sum(int(stats[k]) for k in range(1, P))) + float(stats[P]) * r / 8.0
+        return b
+    if op == "k_sum":
+        return n * (P + r / 8.0)
+    if op == "k_max":
+        return float(n)
+    if op == "k_materialize":
+        bits = F if precision < 0 else precision
+        return n * (bits / 8.0 + 8.0)
+    raise KeyError(op)
+
+
+PRECISION_MAP = {1: 5, 2: 8, 3: 11, 4: 15, 5: 18, 6: 21}
+
+
+def precision_R(info, p):
+    """Stored bits a reduced-precision read keeps (split_double.rs:1642-1664)."""
+    if p < 0:
+        return info.width
+    ilen, dlen = info.ilen, info.dlen
+    bits = PRECISION_MAP[p]
+    if bits > dlen:
+        return info.width
+    R = (ilen + bits) & 0xFFFFFFFF
+    if R // 8 < ((dlen + ilen) & 0xFFFFFFFF) // 8 and R % 8:
+        R = (R // 8 + 1) * 8
+    return min(R, info.width)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every GPU box here has one MI355X, and this build has no multi-GPU path "
+              "(chunks shard independently; the design is in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import buff_amd
+    from buff_amd import datagen
+
+    n = args.rows
+    t0 = time.time()
+    log("bench: generating 2 x %d rows" % n)
+    x_walk = datagen.random_walk(n, 4, 2)
+    x_uni = datagen.uniform(n, -1000.0, 1000.0, 2, 3)
+    cols = [Column("walk", x_walk, 10 ** 4, [-1, 2]), Column("uniform", x_uni, 10 ** 2, [-1])]
+    log("bench: data ready in %.1fs" % (time.time() - t0))
+
+    ctx = buff_amd.Context(0)
+    dx = []
+    for c, x in zip(cols, (x_walk, x_uni)):
+        b = ctx.alloc(x.nbytes)
+        b.upload(x)
+        dx.append(b)
+    words = ctx.alloc(((n + 63) // 64) * 8)
+    out = ctx.alloc(n * 8)
+    flush = ctx.alloc(1 << 30)
+
+    def step(record=None):
+        chunks = []
+        for c, b in zip(cols, dx):
+            chunks.append(ctx.encode_device(b.ptr, c.n, c.scale))
+        ops = 0
+        for sel in (0.001, 0.10, 0.50):
+            for c, ch in zip(cols, chunks):
+                ch.filter_device("range", c.preds[sel], words.ptr)
+                ops += 1
+        for ch in chunks:
+            ch.sum()
+            ch.max()
+            ops += 2
+        for c, ch in zip(cols, chunks):
+            for p in c.mat_precisions:
+                ch.materialize_device(out.ptr, p, flags=buff_amd.BUFF_SEMANTIC)
+                ops += 1
+        ctx.synchronize()
+        if record is not None:
+            record.extend(chunks)
+        else:
+            for ch in chunks:
+                ch.free()
+        return ops + len(chunks)
+
+    log("bench: warmup %d" % args.warmup)
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    log("bench: timing %d steps" % args.steps)
+    ctx.synchronize()
+    t = time.perf_counter()
+    ops = 0
+    for i in range(args.steps):
+        ops = step()
+        log("bench: step %d done" % (i + 1))
+    ctx.synchronize()
+    dt = time.perf_counter() - t
+    ctx.prof_enable(False)
+    ms_per_step = dt * 1e3 / args.steps
+    logical_bytes = 8.0 * n * ops
+    value = logical_bytes / (dt / args.steps) / 1e9
+
+    # ---- per-kernel accounting over the timed region ----------------------------
+    keep = []
+    step(record=keep)   # chunks for byte accounting (outside the timed region)
+    infos = [ch.info() for ch in keep]
+    stats_dev = ctx.alloc(9 * 8)
+    filt = {}
+    for c, ch, info in zip(cols, keep, infos):
+        for sel in (0.001, 0.10, 0.50):
+            stats_dev.fill(0)
+            ch.filter_device("range", c.preds[sel], words.ptr, stats_dev=stats_dev.ptr)
+            filt[(c.name, sel)] = algorithmic_bytes(info, "k_filter_gt", stats_dev.download(np.uint64, 9))
+    mat = 0.0
+    for c, info in zip(cols, infos):
+        for p in c.mat_precisions:
+            mat += algorithmic_bytes(info, "k_materialize", precision=precision_R(info, p))
+    per_step = {
+        "k_minmax": sum(algorithmic_bytes(i, "k_minmax") for i in infos),
+        "k_encode": sum(algorithmic_bytes(i, "k_encode") for i in infos),
+        "k_filter_gt": sum(filt.values()),
+        "k_sum": sum(algorithmic_bytes(i, "k_sum") for i in infos),
+        "k_max": sum(algorithmic_bytes(i, "k_max") for i in infos),
+        "k_materialize": mat,
+    }
+    kernels = {}
+    for k, bytes_per_step in per_step.items():
+        launches, total_ms = ctx.prof_query(k)
+        if not launches:
+            continue
+        avg_ms = total_ms / launches
+        bytes_per_launch = bytes_per_step / (launches / args.steps)
+        gbps = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        kernels[k] = {"launches": launches, "avg_us": round(avg_ms * 1e3, 2),
+                      "share_of_step": round(total_ms / args.steps / ms_per_step, 4),
+                      "algorithmic_bytes_per_launch": round(bytes_per_launch),
+                      "achieved_GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+    dominant = max(kernels, key=lambda k: kernels[k]["share_of_step"])
+
+    # ---- cold-cache pass: evict the memory-side cache before every launch ------
+    cold_bytes = {"k_filter_gt": [filt[(c.name, 0.10)] for c in cols],
+                  "k_sum": [algorithmic_bytes(i, "k_sum") for i in infos],
+                  "k_max": [algorithmic_bytes(i, "k_max") for i in infos],
+                  "k_materialize": [algorithmic_bytes(i, "k_materialize") for i in infos]}
+    cold = {}
+    ctx.prof_reset()
+    trials = 3
+    for trial in range(trials):
+        for c, ch in zip(cols, keep):
+            for fn in (lambda: ch.filter_device("range", c.preds[0.10], words.ptr), lambda: ch.sum(),
+                       lambda: ch.max(), lambda: ch.materialize_device(out.ptr, -1)):
+                flush.fill(trial & 0xFF)
+                ctx.synchronize()
+                ctx.prof_enable(True)
+                fn()
+                ctx.synchronize()
+                ctx.prof_enable(False)
+    for k, per_col in cold_bytes.items():
+        launches, total_ms = ctx.prof_query(k)
+        if launches:
+            b = sum(per_col) / len(per_col)
+            gbps = b / ((total_ms / launches) * 1e-3) / 1e9
+            cold[k] = {"launches": launches, "avg_us": round(total_ms / launches * 1e3, 2),
+                       "achieved_GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4),
+                       "note": "10% filter" if k == "k_filter_gt" else ("full precision" if k == "k_materialize"
+                                                                       else "")}
+    for ch in keep:
+        ch.free()
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64->int64 (byte planes, u64 sums)",
+        "data": "synthetic (seeded random walk 4 decimals + uniform[-1000,1000] 2 decimals)",
+        "config": {"workload": "plain BUFF, 2 columns x %d float64 rows (walk p=4, uniform p=2): encode, "
+                               "range filters at 0.1/10/50%%, sum, max, materialize" % n,
+                   "rows_per_column": n, "columns": 2, "ops_per_step": ops, "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": kernels[dominant]["achieved_GBps"],
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": kernels[dominant]["frac"], "traffic": None},
+        "kernels": kernels,
+        "cold_cache": cold,
+    }
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(args.cpu_rows)
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(rows):
+    """The oracle's restatement of the reference CPU path on a bounded sample.
+
+    buff-simd (AVX2) encode/decode/range filters and the scalar buff sum/max,
+    single-threaded as the reference, on `rows` rows of each column.
+    """
+    from buff_amd import datagen
+    from oracle import refcpu
+
+    refcpu.build()
+    t_all = 0.0
+    logical = 0.0
+    samples = [(datagen.random_walk(rows, 4, 2), 10 ** 4), (datagen.uniform(rows, -1000.0, 1000.0, 2, 3), 10 ** 2)]
+    for x, scale in samples:
+        t = time.perf_counter()
+        s = refcpu.encode(x, scale, flip=True)
+        t_all += time.perf_counter() - t
+        logical += 8.0 * rows
+        for sel in (0.001, 0.10, 0.50):
+            pred = float(np.quantile(x, 1.0 - sel, method="lower"))
+            t = time.perf_counter()
+            refcpu.simd_filter(s, scale, "range", pred)
+            t_all += time.perf_counter() - t
+            logical += 8.0 * rows
+        plain = refcpu.encode(x, scale)
+        for fn in (refcpu.byte_fixed_sum, refcpu.byte_fixed_max):
+            t = time.perf_counter()
+            fn(plain, scale)
+            t_all += time.perf_counter() - t
+            logical += 8.0 * rows
+        t = time.perf_counter()
+        refcpu.decode(s, scale, flip=True)
+        t_all += time.perf_counter() - t
+        logical += 8.0 * rows
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(logical / t_all / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": "%d rows of each column (walk p=4, uniform p=2): buff-simd encode, 3 range filters (AVX2), "
+                      "decode; scalar buff sum and max; single thread on %s (nproc %d)" % (rows, cpu,
+                                                                                         os.cpu_count() or 0)}
+
+
+if __name__ == "__main__":
+    main()

@@ -79,6 +79,7 @@ SIGNATURES = {
     "buff_dev_free": [vp, vp],
     "buff_memcpy_h2d": [vp, vp, vp, u64],
     "buff_memcpy_d2h": [vp, vp, vp, u64],
+    "buff_dev_fill": [vp, vp, ctypes.c_uint8, u64],
     "buff_prof_enable": [vp, cint],
     "buff_prof_reset": [vp],
     "buff_prof_query": [vp, ctypes.c_char_p, u64p, f64p],
@@ -230,6 +231,9 @@ class DeviceBuffer:
         assert out.nbytes <= self.nbytes
         check(lib().buff_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes))
         return out
+
+    def fill(self, value=0):
+        check(lib().buff_dev_fill(self.ctx.handle, self.ptr, value, self.nbytes))
 
     def free(self):
         if self.ptr:

@@ -240,13 +240,13 @@ int chunk_alloc(Ctx* ctx, Chunk* ch) {
   ch->view.ntiles = ntiles;
   ch->view.pitch = ntiles * kTileRows;
   const uint64_t plane_bytes = (uint64_t)ch->view.P * ch->view.pitch;
-  const uint64_t tail_bytes = ntiles * 16ull * ch->view.r * 8ull;
+  const uint64_t tail_bytes = ntiles * 64ull * ch->view.r * 2ull;
   ch->device_bytes = std::max<uint64_t>(plane_bytes + tail_bytes, 256);
   void* p = nullptr;
   BUFF_TRY(ctx->alloc(ch->device_bytes, &p));
   ch->mem = p;
   ch->view.planes = (uint8_t*)p;
-  ch->view.tail = (uint64_t*)((uint8_t*)p + plane_bytes);
+  ch->view.tail = (uint16_t*)((uint8_t*)p + plane_bytes);
   return BUFF_OK;
 }
 
@@ -568,13 +568,17 @@ int max_op(Ctx* ctx, const Chunk* ch, double* value, uint64_t* max_off) {
   uint64_t best = 0;
   const ChunkView& v = ch->view;
   if (ch->n > 0 && (v.P || v.r)) {
-    BUFF_TRY(ctx->scratch(ctx->s_best, 8));
-    HIP_TRY(hipMemsetAsync(ctx->s_best.ptr, 0, 8, ctx->stream));
+    // [best u64 | pad to 256 | per-tile maxima | global top byte]
+    const uint64_t ms = max_scratch_bytes(v.ntiles);
+    BUFF_TRY(ctx->scratch(ctx->s_best, 256 + ms));
+    uint8_t* sb = (uint8_t*)ctx->s_best.ptr;
+    HIP_TRY(hipMemsetAsync(sb, 0, 8, ctx->stream));
+    HIP_TRY(hipMemsetAsync(sb + 256 + ms - 256, 0, 256, ctx->stream));
     {
       ProfScope sc(ctx, "k_max");
-      HIP_TRY(launch_max(v, (unsigned long long*)ctx->s_best.ptr, ctx->stream, ctx->shape(Op::Max, v.ntiles)));
+      HIP_TRY(launch_max(v, (unsigned long long*)sb, sb + 256, ctx->stream, ctx->shape(Op::Max, v.ntiles)));
     }
-    BUFF_TRY(d2h_small(ctx, &best, ctx->s_best.ptr, 8));
+    BUFF_TRY(d2h_small(ctx, &best, sb, 8));
   }
   if (max_off) *max_off = best;
   // split_double.rs:4102: (max as i64 + base_int) as f64 / 2^dlen
@@ -633,8 +637,13 @@ int materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t fl
     if (!semantic) {
       if (ch->ilen > 15)
         return fail(BUFF_E_UNSUPPORTED, "ilen %u > 15: the reference parses an outlier section here", ch->ilen);
-      if (R / 8 > v.P)
+      if (R / 8 > v.P) {
+        // The reference reads R/8 whole n-byte sub-columns: past the end of the
+        // stream (a panic) unless the column is tiny.
+        if (kStreamHeader + (uint64_t)(R / 8) * ch->n > stream_bytes_of(ch->n, v.P, v.r))
+          return fail(BUFF_E_REF_PANIC, "the reference reads %u sub-columns of a %u-plane stream", R / 8, v.P);
         return fail(BUFF_E_UNSUPPORTED, "the reference would read %u whole bytes of a %u-plane stream", R / 8, v.P);
+      }
       mp.mode = kMatRefPrecision;
       mp.planes = R / 8;
       mp.tail_bits = R % 8;
@@ -770,6 +779,14 @@ int buff_memcpy_d2h(buff_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return BUFF_OK;
+}
+
+int buff_dev_fill(buff_ctx* ctx, void* dptr, uint8_t value, uint64_t bytes) {
+  if (!ctx || (bytes && !dptr)) return fail(BUFF_E_INVALID, "null argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  ProfScope ps(c, "k_fill_bytes");
+  HIP_TRY(launch_fill_bytes(dptr, bytes, value, c->stream));
   return BUFF_OK;
 }
 

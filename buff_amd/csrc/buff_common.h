@@ -9,8 +9,10 @@
 //     A dwordx4 load per lane reads a whole tile of a plane coalesced, and
 //     `__ballot(byte g of each lane ...)` yields the 64-row bitmap word of rows
 //     64*g .. 64*g+63 of the tile directly in BitVec order.
-//   * The r tail bits are kept as bit-planes: tile c owns 16*r u64 words at
-//     tail + 16*r*c; word g*r + j holds bit j of rows 64*g+t (bit t).
+//   * The r tail bits are kept as per-lane bit-planes: tile c owns r*64 u16
+//     words at tail + 64*r*c; word j*64 + t holds bit j of lane t's 16 rows
+//     (bit g <-> row 64*g + t). A wave reads or writes one bit-plane of a tile
+//     with one coalesced 128-byte access.
 //   * Rows >= n are zero in every plane and bit-plane.
 // The reference byte stream (split_double.rs:349-414) is produced from and parsed
 // into this form by the serialize/load kernels.
@@ -33,7 +35,7 @@ constexpr uint32_t kStreamHeader = 20;
 // Device view of a resident chunk.
 struct ChunkView {
   uint8_t* planes;    // plane p at planes + p * pitch
-  uint64_t* tail;     // bit-plane words, 16*r per tile
+  uint16_t* tail;     // per-lane bit-planes, 64*r u16 per tile
   uint64_t pitch;     // bytes per plane = ntiles * 1024
   uint64_t n;         // rows
   uint64_t ntiles;    // ceil(n / 1024)
@@ -53,16 +55,14 @@ BUFF_HD uint64_t group_valid_mask(uint64_t n, uint64_t tile, uint32_t g) {
 }
 
 // database/src/methods/prec_double.rs:190-205 (fetch_fixed_aligned), with Rust
-// release semantics: the shift amount is masked to 6 bits, negation wraps.
+// release semantics: the shift amount is masked to 6 bits, negation wraps
+// (`!(fixed - 1)` == `0 - fixed`). Written without branches.
 BUFF_HD int64_t fetch_fixed_aligned_bits(uint64_t bdu, int32_t precision_exp, uint32_t dlen) {
-  int32_t exp = (int32_t)((bdu & 0x7FF0000000000000ull) >> 52) - 1023;
-  uint64_t fixed = 0;
-  if (exp >= precision_exp) {
-    int32_t sh = 63 - exp - (int32_t)dlen;
-    fixed = ((bdu << 11) | 0x8000000000000000ull) >> ((uint32_t)sh & 63u);
-    if (bdu & 0x8000000000000000ull) fixed = ~(fixed - 1);
-  }
-  return (int64_t)fixed;
+  const int32_t exp = (int32_t)((bdu >> 52) & 0x7FFu) - 1023;
+  const uint32_t sh = (uint32_t)(63 - exp - (int32_t)dlen) & 63u;
+  const uint64_t mag = ((bdu << 11) | 0x8000000000000000ull) >> sh;
+  const uint64_t fixed = (bdu >> 63) ? 0ull - mag : mag;
+  return exp >= precision_exp ? (int64_t)fixed : 0;
 }
 
 // Per-filter constants computed on the host from the predicate.

@@ -34,8 +34,13 @@ hipError_t launch_encode(const double* x, const ChunkView& cv, int32_t pexp, hip
 hipError_t launch_filter(const ChunkView& cv, int op, const FilterTarget& tg, uint64_t* out, uint64_t nwords,
                          unsigned long long* stats, hipStream_t s, const LaunchShape& ls);
 hipError_t launch_fill_bitmap(uint64_t* out, uint64_t nwords, uint64_t n, int ones, hipStream_t s);
+hipError_t launch_fill_bytes(void* out, uint64_t bytes, uint8_t value, hipStream_t s);
 hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls);
-hipError_t launch_max(const ChunkView& cv, unsigned long long* best, hipStream_t s, const LaunchShape& ls);
+// Two passes; `scratch` holds max_scratch_bytes(ntiles) bytes whose last 256
+// bytes (the global top byte) the caller zeroes, as it zeroes `best`.
+uint64_t max_scratch_bytes(uint64_t ntiles);
+hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* scratch, hipStream_t s,
+                      const LaunchShape& ls);
 hipError_t launch_materialize(const ChunkView& cv, const MatParams& mp, double* out, hipStream_t s,
                               const LaunchShape& ls);
 hipError_t launch_serialize(const ChunkView& cv, uint8_t flipmask, uint8_t* body, uint64_t tail_bytes,

@@ -2,10 +2,11 @@
 //
 // Every kernel is HBM-bound integer/byte work; nothing here is a contraction, so
 // no MFMA. The designs share three ideas:
-//   * one wave owns one 1024-row tile per step; a lane reads its 16 bytes of a
-//     plane with one dwordx4 load (the whole wave: 1 KiB contiguous),
+//   * a wave owns 1024-row tiles; a lane reads its 16 bytes of a plane with one
+//     dwordx4 load (the wave: 1 KiB contiguous), and a wave issues the loads of
+//     several tiles before it consumes any, so enough bytes are in flight;
 //   * per-row predicates become bitmap words with __ballot (v_cmp_*_sdwa picks
-//     the byte straight out of the loaded dword), and
+//     the byte straight out of the loaded dword);
 //   * uniform 64-bit masks leave the scalar registers through v_writelane into
 //     vector registers and are written with vector stores only.
 #include <hip/hip_runtime.h>
@@ -19,8 +20,8 @@ namespace buffgpu {
 // wave helpers
 
 __device__ __forceinline__ uint32_t writelane(uint32_t old, uint32_t val, uint32_t lane) {
-  // GFX9 reads at most one SGPR per VALU op: the lane select goes through M0.
   // `val` is wave-uniform; readfirstlane pins it to an SGPR for the asm operand.
+  // GFX9 reads at most one SGPR per VALU op: the lane select goes through M0.
   const uint32_t sv = __builtin_amdgcn_readfirstlane(val);
   asm volatile("v_writelane_b32 %0, %1, m0" : "+v"(old) : "s"(sv), "{m0}"(lane));
   return old;
@@ -65,42 +66,70 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// Stores 16 uniform u64 words (128 bytes) at dst[0..15] through lanes 0..31,
-// keeping only words with index < limit.
-__device__ __forceinline__ void store_words16(uint64_t* dst, const uint64_t (&w)[16], uint32_t limit) {
-  const uint32_t lane = __lane_id();
-  uint32_t v = 0;
-#pragma unroll
-  for (uint32_t g = 0; g < 16; ++g) {
-    v = writelane(v, (uint32_t)w[g], 2 * g);
-    v = writelane(v, (uint32_t)(w[g] >> 32), 2 * g + 1);
-  }
-  if (lane < 32 && (lane >> 1) < limit) reinterpret_cast<uint32_t*>(dst)[lane] = v;
-}
+// ----------------------------------------------------------------------------
+// Tile access. A full tile needs no row guards; only the last tile of a chunk
+// can be partial.
 
-// Turns per-lane offsets of one tile into the tile's 16*r tail bit-plane words
-// (word g*r + j = bit j of rows 64g + lane) and stores them at dst.
-__device__ __forceinline__ void store_tail_bits(uint64_t* dst, const uint64_t (&off)[16], uint32_t r) {
-  const uint32_t lane = __lane_id();
-  uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;   // lane L holds dwords 4L..4L+3
+__device__ __forceinline__ bool tile_full(uint64_t n, uint64_t c) { return (c + 1) * kTileRows <= n; }
+
+__device__ __forceinline__ void load_tile_f64(const double* __restrict__ x, uint64_t n, uint64_t c, uint32_t lane,
+                                              bool full, uint64_t (&v)[16]) {
+  const double* xt = x + c * kTileRows + lane;
+  if (full) {
 #pragma unroll
-  for (uint32_t g = 0; g < 16; ++g) {
-    for (uint32_t j = 0; j < r; ++j) {
-      uint64_t m = __ballot((off[g] >> j) & 1ull);
-      uint32_t d = 2 * (g * r + j);          // dword index of the low half
-      uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
-      // d is even: the pair lands in components (0,1) or (2,3) of lane d/4
-      if ((d & 3) == 0) { v0 = writelane(v0, lo, d >> 2); v1 = writelane(v1, hi, d >> 2); }
-      else { v2 = writelane(v2, lo, d >> 2); v3 = writelane(v3, hi, d >> 2); }
+    for (uint32_t g = 0; g < 16; ++g) v[g] = __double_as_longlong(xt[64 * g]);
+  } else {
+#pragma unroll
+    for (uint32_t g = 0; g < 16; ++g) {
+      const uint64_t row = c * kTileRows + 64 * g + lane;
+      v[g] = row < n ? __double_as_longlong(xt[64 * g]) : 0;
     }
   }
-  if (lane < 8 * r) reinterpret_cast<uint4*>(dst)[lane] = make_uint4(v0, v1, v2, v3);
+}
+
+__device__ __forceinline__ uint4 load_plane(const ChunkView& cv, uint32_t p, uint64_t c, uint32_t lane) {
+  return reinterpret_cast<const uint4*>(cv.planes + p * cv.pitch + c * kTileRows)[lane];
+}
+
+// bit-plane j of tile c, this lane (bit g <-> row 64g + lane)
+__device__ __forceinline__ uint32_t load_tail(const ChunkView& cv, uint32_t j, uint64_t c, uint32_t lane) {
+  return cv.tail[(c * cv.r + j) * 64 + lane];
+}
+
+// Lane's 16 tail values (r bits each) -> per-bit 16-bit planes.
+__device__ __forceinline__ void store_tail_planes(uint16_t* __restrict__ dst, const uint64_t (&off)[16], uint32_t r,
+                                                  uint32_t lane) {
+  for (uint32_t j = 0; j < r; ++j) {
+    uint32_t h = 0;
+#pragma unroll
+    for (uint32_t g = 0; g < 16; ++g) h |= (uint32_t)((off[g] >> j) & 1ull) << g;
+    dst[j * 64 + lane] = (uint16_t)h;
+  }
+}
+
+// Tail value (r bits) of group g from the lane's bit-planes h[0..r).
+__device__ __forceinline__ uint32_t tail_of(const uint32_t (&h)[8], uint32_t r, uint32_t g) {
+  uint32_t v = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j)
+    if (j < r) v |= ((h[j] >> g) & 1u) << j;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t lane_valid16(uint64_t n, uint64_t c, uint32_t lane) {
+  const uint64_t r0 = c * kTileRows + lane;
+  if (r0 + 64 * 15 < n) return 0xFFFFu;
+  uint32_t m = 0;
+#pragma unroll
+  for (uint32_t g = 0; g < 16; ++g) m |= (r0 + 64 * g < n ? 1u : 0u) << g;
+  return m;
 }
 
 // ----------------------------------------------------------------------------
 // K1: fixed-point conversion + min / max reduction (encode pass 1).
 // split_double.rs:321-330. Workgroup w owns the contiguous tile range
 // [w*tpw, (w+1)*tpw), so its partial min is a segment min the prefix pass reuses.
+// max is taken over rows i >= 1 (row 0 only ever updates the min).
 
 __global__ __launch_bounds__(256) void k_minmax(const double* __restrict__ x, uint64_t n, uint64_t ntiles,
                                                 uint64_t tpw, int32_t pexp, uint32_t dlen,
@@ -111,21 +140,25 @@ __global__ __launch_bounds__(256) void k_minmax(const double* __restrict__ x, ui
   const uint64_t t1 = min(t0 + tpw, ntiles);
   int64_t mn = INT64_MAX, mx = INT64_MIN;
   for (uint64_t c = t0 + wave; c < t1; c += 4) {
-    const double* xt = x + c * kTileRows;
-    const bool full = (c + 1) * kTileRows <= n;
     uint64_t v[16];
+    if (tile_full(n, c) && c != 0) {
+      load_tile_f64(x, n, c, lane, true, v);
 #pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) {
-      const uint64_t row = c * kTileRows + 64 * g + lane;
-      v[g] = (full || row < n) ? __double_as_longlong(xt[64 * g + lane]) : 0;
-    }
-#pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) {
-      const uint64_t row = c * kTileRows + 64 * g + lane;
-      const int64_t f = fetch_fixed_aligned_bits(v[g], pexp, dlen);
-      if (full || row < n) {
+      for (uint32_t g = 0; g < 16; ++g) {
+        const int64_t f = fetch_fixed_aligned_bits(v[g], pexp, dlen);
         mn = min(mn, f);
-        if (row != 0) mx = max(mx, f);
+        mx = max(mx, f);
+      }
+    } else {
+      load_tile_f64(x, n, c, lane, false, v);
+#pragma unroll
+      for (uint32_t g = 0; g < 16; ++g) {
+        const uint64_t row = c * kTileRows + 64 * g + lane;
+        const int64_t f = fetch_fixed_aligned_bits(v[g], pexp, dlen);
+        if (row < n) {
+          mn = min(mn, f);
+          if (row != 0) mx = max(mx, f);
+        }
       }
     }
   }
@@ -223,24 +256,19 @@ __global__ __launch_bounds__(64) void k_minmax_prefix(const double* __restrict__
 __global__ __launch_bounds__(256) void k_encode(const double* __restrict__ x, uint64_t n, uint64_t ntiles,
                                                 int32_t pexp, uint32_t dlen, int64_t base, uint32_t F, uint32_t P,
                                                 uint32_t r, uint8_t* __restrict__ planes, uint64_t pitch,
-                                                uint64_t* __restrict__ tail) {
+                                                uint16_t* __restrict__ tail) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t c = wave; c < ntiles; c += nwaves) {
-    const double* xt = x + c * kTileRows;
-    const bool full = (c + 1) * kTileRows <= n;
     uint64_t off[16];
+    const bool full = tile_full(n, c);
+    load_tile_f64(x, n, c, lane, full, off);
 #pragma unroll
     for (uint32_t g = 0; g < 16; ++g) {
-      const uint64_t row = c * kTileRows + 64 * g + lane;
-      off[g] = (full || row < n) ? __double_as_longlong(xt[64 * g + lane]) : 0;
-    }
-#pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) {
-      const uint64_t row = c * kTileRows + 64 * g + lane;
       const int64_t f = fetch_fixed_aligned_bits(off[g], pexp, dlen);
-      off[g] = (full || row < n) ? (uint64_t)f - (uint64_t)base : 0;
+      const uint64_t o = (uint64_t)f - (uint64_t)base;
+      off[g] = (full || c * kTileRows + 64 * g + lane < n) ? o : 0;
     }
     for (uint32_t p = 0; p < P; ++p) {
       const uint32_t sh = F - 8 * (p + 1);
@@ -252,88 +280,173 @@ __global__ __launch_bounds__(256) void k_encode(const double* __restrict__ x, ui
       }
       reinterpret_cast<uint4*>(planes + p * pitch + c * kTileRows)[lane] = make_uint4(d[0], d[1], d[2], d[3]);
     }
-    if (r) store_tail_bits(tail + c * 16 * r, off, r);
+    if (r) store_tail_planes(tail + c * 64 * r, off, r, lane);
   }
 }
 
 // ----------------------------------------------------------------------------
 // K3/K4: range (x > pred) and equality filters with early termination.
-// buff_simd.rs:4507-4679 / 4925-5093. Plane 0 is read for every tile; plane k
-// (and then the tail) only for tiles that still hold candidate rows, i.e. rows
-// whose higher bytes all equal the target's. Bitmap words come from ballots.
+// buff_simd.rs:4507-4679 / 4925-5093.
+//
+// A wave takes U consecutive tiles per step and issues their plane-0 loads
+// together. Per tile, the common case costs 32 ballots: the `>` words go
+// straight into the output register (range), and the `==` words are only
+// OR-ed to see whether any row still ties the target. When some tiles hold
+// ties, they are refined together, one sub-column at a time (the loads of all
+// of them in flight at once), with per-lane 16-bit candidate masks; the tail
+// is read only for tiles whose candidates survive every byte. The final words
+// of a refined tile come from 16 ballots.
 
 template <int MODE>
+__device__ __forceinline__ void refine_init(const uint4& w0, uint32_t d0, uint32_t valid, bool has_planes,
+                                            uint32_t& gtm, uint32_t& eqm) {
+  gtm = 0;
+  eqm = 0;
+  if (has_planes) {
+#pragma unroll
+    for (uint32_t g = 0; g < 16; ++g) {
+      const uint32_t b = byte_of(w0, g);
+      if (MODE == 0) gtm |= (b > d0 ? 1u : 0u) << g;
+      eqm |= (b == d0 ? 1u : 0u) << g;
+    }
+  } else {
+    eqm = 0xFFFFu;
+  }
+  eqm &= valid;
+}
+
+template <int MODE>
+__device__ __forceinline__ void refine_step(const uint4& w, uint32_t dp, uint32_t& gtm, uint32_t& eqm) {
+#pragma unroll
+  for (uint32_t g = 0; g < 16; ++g) {
+    const uint32_t b = byte_of(w, g);
+    const uint32_t bit = 1u << g;
+    if (MODE == 0 && b > dp) gtm |= eqm & bit;
+    if (b != dp) eqm &= ~bit;
+  }
+}
+
+__device__ __forceinline__ uint32_t words_from_mask(uint32_t res) {
+  uint32_t o = 0;
+#pragma unroll
+  for (uint32_t g = 0; g < 16; ++g) {
+    const uint64_t m = __ballot((res >> g) & 1u);
+    o = writelane(o, (uint32_t)m, 2 * g);
+    o = writelane(o, (uint32_t)(m >> 32), 2 * g + 1);
+  }
+  return o;
+}
+
+template <int MODE, int U>
 __global__ __launch_bounds__(256) void k_filter(ChunkView cv, FilterTarget tg, uint64_t* __restrict__ out,
                                                 uint64_t nwords, unsigned long long* __restrict__ stats) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t c = wave; c < cv.ntiles; c += nwaves) {
-    uint64_t gt[16], eq[16];
-    uint64_t any = 0;
-    if (cv.P > 0) {
-      const uint4 w = reinterpret_cast<const uint4*>(cv.planes + c * kTileRows)[lane];
-      const uint32_t d0 = tg.digit[0];
+  const uint64_t ngroups = (cv.ntiles + U - 1) / U;
+  const uint32_t d0 = tg.digit[0];
+  const bool has_planes = cv.P > 0;
+  for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
+    const uint64_t c0 = grp * U;
+    uint4 w[U];
+    uint32_t o[U];
+    uint32_t need = 0;   // tiles of the group with ties (uniform bit mask)
 #pragma unroll
-      for (uint32_t g = 0; g < 16; ++g) {
-        const uint32_t b = byte_of(w, g);
-        if (MODE == 0) gt[g] = __ballot(b > d0);
-        eq[g] = __ballot(b == d0) & group_valid_mask(cv.n, c, g);
-        any |= eq[g];
-      }
-    } else {
+    for (uint32_t u = 0; u < U; ++u)
+      w[u] = (has_planes && c0 + u < cv.ntiles) ? load_plane(cv, 0, c0 + u, lane) : make_uint4(0, 0, 0, 0);
 #pragma unroll
-      for (uint32_t g = 0; g < 16; ++g) {
-        gt[g] = 0;
-        eq[g] = group_valid_mask(cv.n, c, g);
-        any |= eq[g];
+    for (uint32_t u = 0; u < U; ++u) {
+      o[u] = 0;
+      if (c0 + u >= cv.ntiles) continue;
+      uint64_t any = 0;
+      if (has_planes) {
+#pragma unroll
+        for (uint32_t g = 0; g < 16; ++g) {
+          const uint32_t b = byte_of(w[u], g);
+          if (MODE == 0) {
+            const uint64_t m = __ballot(b > d0);
+            o[u] = writelane(o[u], (uint32_t)m, 2 * g);
+            o[u] = writelane(o[u], (uint32_t)(m >> 32), 2 * g + 1);
+          }
+          any |= __ballot(b == d0);
+        }
+      } else {
+        any = 1;
       }
+      if (any) need |= 1u << u;
     }
-    for (uint32_t p = 1; p < cv.P && any; ++p) {
-      if (stats) {
-        uint64_t k = 0;
+    if (need) {
+      uint32_t gtm[U], eqm[U], valid[U];
 #pragma unroll
-        for (uint32_t g = 0; g < 16; ++g) k += __popcll(eq[g]);
-        if (lane == 0) atomicAdd(&stats[p], (unsigned long long)k);
+      for (uint32_t u = 0; u < U; ++u) {
+        valid[u] = lane_valid16(cv.n, c0 + u, lane);
+        if (need & (1u << u)) refine_init<MODE>(w[u], d0, valid[u], has_planes, gtm[u], eqm[u]);
+        else { gtm[u] = 0; eqm[u] = 0; }
       }
-      const uint4 w = reinterpret_cast<const uint4*>(cv.planes + p * cv.pitch + c * kTileRows)[lane];
-      const uint32_t dp = tg.digit[p];
-      any = 0;
+      for (uint32_t p = 1; p < cv.P; ++p) {
+        uint32_t live = 0;
 #pragma unroll
-      for (uint32_t g = 0; g < 16; ++g) {
-        const uint32_t b = byte_of(w, g);
-        if (MODE == 0) gt[g] |= eq[g] & __ballot(b > dp);
-        eq[g] &= __ballot(b == dp);
-        any |= eq[g];
+        for (uint32_t u = 0; u < U; ++u)
+          if (__ballot(eqm[u] != 0)) live |= 1u << u;
+        if (!live) break;
+        if (stats) {
+          uint32_t k = 0;
+#pragma unroll
+          for (uint32_t u = 0; u < U; ++u) k += __popc(eqm[u]);
+          const uint64_t tot = wave_sum_u64(k);
+          if (lane == 0) atomicAdd(&stats[p], (unsigned long long)tot);
+        }
+        uint4 wp[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+          wp[u] = (live & (1u << u)) ? load_plane(cv, p, c0 + u, lane) : make_uint4(0, 0, 0, 0);
+        const uint32_t dp = tg.digit[p];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+          if (live & (1u << u)) refine_step<MODE>(wp[u], dp, gtm[u], eqm[u]);
       }
-    }
-    if (cv.r && any) {
-      const uint64_t* tw = cv.tail + c * 16 * cv.r;
-      if (stats) {
-        uint64_t k = 0;
+      if (cv.r) {
+        uint32_t live = 0;
 #pragma unroll
-        for (uint32_t g = 0; g < 16; ++g) k += __popcll(eq[g]);
-        if (lane == 0) atomicAdd(&stats[cv.P], (unsigned long long)k);
-      }
+        for (uint32_t u = 0; u < U; ++u)
+          if (__ballot(eqm[u] != 0)) live |= 1u << u;
+        if (live) {
+          if (stats) {
+            uint32_t k = 0;
 #pragma unroll
-      for (uint32_t g = 0; g < 16; ++g) {
-        if (!eq[g]) continue;
-        for (int j = (int)cv.r - 1; j >= 0; --j) {
-          const uint64_t bits = tw[g * cv.r + j];
-          if ((tg.tail >> j) & 1) {
-            eq[g] &= bits;
-          } else {
-            if (MODE == 0) gt[g] |= eq[g] & bits;
-            eq[g] &= ~bits;
+            for (uint32_t u = 0; u < U; ++u) k += __popc(eqm[u]);
+            const uint64_t tot = wave_sum_u64(k);
+            if (lane == 0) atomicAdd(&stats[cv.P], (unsigned long long)tot);
+          }
+          const uint64_t dt = tg.tail;
+#pragma unroll
+          for (uint32_t u = 0; u < U; ++u) {
+            if (!(live & (1u << u))) continue;
+            uint32_t h[8];
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j) h[j] = j < cv.r ? load_tail(cv, j, c0 + u, lane) : 0u;
+#pragma unroll
+            for (uint32_t g = 0; g < 16; ++g) {
+              const uint64_t t = tail_of(h, cv.r, g);
+              const uint32_t bit = 1u << g;
+              if (MODE == 0 && t > dt) gtm[u] |= eqm[u] & bit;
+              if (t != dt) eqm[u] &= ~bit;
+            }
           }
         }
       }
-    }
-    uint64_t res[16];
 #pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) res[g] = MODE == 0 ? (gt[g] & group_valid_mask(cv.n, c, g)) : eq[g];
-    const uint64_t w0 = c * 16;
-    store_words16(out + w0, res, nwords > w0 ? (uint32_t)min<uint64_t>(16, nwords - w0) : 0u);
+      for (uint32_t u = 0; u < U; ++u)
+        if (need & (1u << u)) o[u] = words_from_mask((MODE == 0 ? gtm[u] : eqm[u]) & valid[u]);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint64_t c = c0 + u;
+      if (c >= cv.ntiles) continue;
+      const uint64_t w0 = c * 16;
+      const uint32_t limit = nwords > w0 ? (uint32_t)min<uint64_t>(16, nwords - w0) : 0u;
+      if (lane < 32 && (lane >> 1) < limit) reinterpret_cast<uint32_t*>(out + w0)[lane] = o[u];
+    }
   }
 }
 
@@ -349,99 +462,154 @@ __global__ void k_fill_bitmap(uint64_t* __restrict__ out, uint64_t nwords, uint6
   }
 }
 
+__global__ void k_fill_bytes(uint4* __restrict__ out, uint64_t n16, uint32_t v) {
+  const uint4 w = make_uint4(v, v, v, v);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = w;
+}
+
 // ----------------------------------------------------------------------------
 // K5: per-sub-column byte sums (split_double.rs:2158-2204), exact in u64.
-// Bytes are summed with v_sad_u8; the tail from bit-plane popcounts.
+// Bytes are summed with v_sad_u8, tail bit-planes with popcounts weighted by
+// their bit position. P is a template parameter so that all plane and tail
+// loads of a step are issued at once.
 
+template <int P, int U>
 __global__ __launch_bounds__(256) void k_sum(ChunkView cv, unsigned long long* __restrict__ sums) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  uint32_t acc[kMaxPlanes] = {0, 0, 0, 0, 0, 0, 0, 0};
-  uint64_t tacc = 0;
-  const uint32_t twords = 16 * cv.r;
-  uint64_t wide[kMaxPlanes] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint64_t ngroups = (cv.ntiles + U - 1) / U;
+  const uint32_t r = cv.r;
+  constexpr int PP = P > 0 ? P : 1;
+  uint32_t acc[PP];
+  uint64_t wide[PP];
+#pragma unroll
+  for (int p = 0; p < PP; ++p) { acc[p] = 0; wide[p] = 0; }
+  uint32_t tacc[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // per-bit popcounts
   uint32_t iter = 0;
-  for (uint64_t c = wave; c < cv.ntiles; c += nwaves) {
+  for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
+    const uint64_t c0 = grp * U;
+    uint4 w[U][PP];
+    uint32_t h[U][8];
 #pragma unroll
-    for (uint32_t p = 0; p < kMaxPlanes; ++p) {
-      if (p < cv.P) {
-        const uint4 w = reinterpret_cast<const uint4*>(cv.planes + p * cv.pitch + c * kTileRows)[lane];
-        uint32_t a = __builtin_amdgcn_sad_u8(w.x, 0u, acc[p]);
-        a = __builtin_amdgcn_sad_u8(w.y, 0u, a);
-        a = __builtin_amdgcn_sad_u8(w.z, 0u, a);
-        acc[p] = __builtin_amdgcn_sad_u8(w.w, 0u, a);
-      }
+    for (int u = 0; u < U; ++u) {
+      const bool ok = c0 + u < cv.ntiles;
+#pragma unroll
+      for (int p = 0; p < P; ++p) w[u][p] = ok ? load_plane(cv, p, c0 + u, lane) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+      for (uint32_t j = 0; j < 8; ++j) h[u][j] = (ok && j < r) ? load_tail(cv, j, c0 + u, lane) : 0u;
     }
-    for (uint32_t i = lane; i < twords; i += 64) tacc += (uint64_t)__popcll(cv.tail[c * twords + i]) << (i % cv.r);
-    // a lane adds at most 16*255 per tile: fold into u64 well before u32 overflow
-    if (++iter == (1u << 16)) {
 #pragma unroll
-      for (uint32_t p = 0; p < kMaxPlanes; ++p) { wide[p] += acc[p]; acc[p] = 0; }
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        uint32_t a = __builtin_amdgcn_sad_u8(w[u][p].x, 0u, acc[p]);
+        a = __builtin_amdgcn_sad_u8(w[u][p].y, 0u, a);
+        a = __builtin_amdgcn_sad_u8(w[u][p].z, 0u, a);
+        acc[p] = __builtin_amdgcn_sad_u8(w[u][p].w, 0u, a);
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < 8; ++j) tacc[j] += __popc(h[u][j]);
+    }
+    // a lane adds at most 16*255 per tile and plane: fold into u64 long before u32 overflow
+    if (++iter == (1u << 14)) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) { wide[p] += acc[p]; acc[p] = 0; }
       iter = 0;
     }
   }
 #pragma unroll
-  for (uint32_t p = 0; p < kMaxPlanes; ++p) {
-    if (p < cv.P) {
-      const uint64_t s = wave_sum_u64(wide[p] + acc[p]);
-      if (lane == 0 && s) atomicAdd(&sums[p], (unsigned long long)s);
-    }
+  for (int p = 0; p < P; ++p) {
+    const uint64_t s = wave_sum_u64(wide[p] + acc[p]);
+    if (lane == 0 && s) atomicAdd(&sums[p], (unsigned long long)s);
   }
-  if (cv.r) {
-    const uint64_t s = wave_sum_u64(tacc);
-    if (lane == 0 && s) atomicAdd(&sums[cv.P], (unsigned long long)s);
+  if (r) {
+    uint64_t t = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) t += (uint64_t)tacc[j] << j;
+    const uint64_t s = wave_sum_u64(t);
+    if (lane == 0 && s) atomicAdd(&sums[P], (unsigned long long)s);
   }
 }
 
 // ----------------------------------------------------------------------------
-// K6: max stored offset (split_double.rs:3919-4105). The reference narrows
-// candidates byte by byte; its result is the largest stored offset. Each wave
-// keeps its running best and only reconstructs whole offsets of a tile whose
-// top byte can reach it, so most tiles cost one plane read.
+// K6: max stored offset (split_double.rs:3919-4105). The reference narrows the
+// candidates byte by byte; its result is the largest stored offset, whose top
+// byte is the largest top-plane byte. Pass 1 streams the top plane once and
+// records each tile's largest byte plus the global one; pass 2 rebuilds whole
+// offsets only in tiles whose largest byte equals the global one.
 
-__device__ __forceinline__ uint64_t tail_value(const uint64_t* tw, uint32_t r, uint32_t g, uint32_t lane) {
-  uint64_t v = 0;
-  for (uint32_t j = 0; j < r; ++j) v |= ((tw[g * r + j] >> lane) & 1ull) << j;
-  return v;
-}
-
-__global__ __launch_bounds__(256) void k_max(ChunkView cv, unsigned long long* __restrict__ best_out) {
+template <int U>
+__global__ __launch_bounds__(256) void k_max_top(ChunkView cv, uint8_t* __restrict__ tile_max,
+                                                 unsigned int* __restrict__ top) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  uint64_t best = 0;
-  bool have = false;
-  const uint32_t top_shift = cv.F >= 8 ? cv.F - 8 : 0;
-  for (uint64_t c = wave; c < cv.ntiles; c += nwaves) {
-    uint4 w0 = make_uint4(0, 0, 0, 0);
-    if (cv.P > 0) {
-      w0 = reinterpret_cast<const uint4*>(cv.planes + c * kTileRows)[lane];
+  const uint64_t ngroups = (cv.ntiles + U - 1) / U;
+  uint32_t best = 0;
+  for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
+    const uint64_t c0 = grp * U;
+    uint4 w[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u)
+      w[u] = c0 + u < cv.ntiles ? load_plane(cv, 0, c0 + u, lane) : make_uint4(0, 0, 0, 0);
+    uint32_t mine = 0;   // lane u holds tile c0+u's max after the loop
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
       uint32_t mb = 0;
 #pragma unroll
-      for (uint32_t g = 0; g < 16; ++g) mb = max(mb, byte_of(w0, g));
+      for (uint32_t g = 0; g < 16; ++g) mb = max(mb, byte_of(w[u], g));
       mb = wave_max_u32(mb);
-      if (have && (uint64_t)mb < (best >> top_shift)) continue;
+      best = max(best, mb);
+      if (lane == u) mine = mb;
     }
-    uint64_t off[16];
+    if (lane < U && c0 + lane < cv.ntiles) tile_max[c0 + lane] = (uint8_t)mine;
+  }
+  if (lane == 0) atomicMax(top, best);
+}
+
+__global__ __launch_bounds__(256) void k_max_refine(ChunkView cv, const uint8_t* __restrict__ tile_max,
+                                                    const unsigned int* __restrict__ top,
+                                                    unsigned long long* __restrict__ best_out) {
+  const uint32_t lane = __lane_id();
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const bool has_planes = cv.P > 0;
+  const uint32_t t = has_planes ? *top : 0u;
+  uint64_t best = 0;
+  bool have = false;
+  // 64 tiles per probe: one byte per lane
+  for (uint64_t base = wave * 64; base < cv.ntiles; base += nwaves * 64) {
+    const uint64_t c = base + lane;
+    const bool hit = c < cv.ntiles && (!has_planes || tile_max[c] == t);
+    uint64_t hits = __ballot(hit);
+    while (hits) {
+      const uint32_t k = (uint32_t)__builtin_ctzll(hits);
+      hits &= hits - 1;
+      const uint64_t ct = base + k;
+      uint64_t off[16];
 #pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) off[g] = cv.P > 0 ? (uint64_t)byte_of(w0, g) : 0;
-    for (uint32_t p = 1; p < cv.P; ++p) {
-      const uint4 w = reinterpret_cast<const uint4*>(cv.planes + p * cv.pitch + c * kTileRows)[lane];
+      for (uint32_t g = 0; g < 16; ++g) off[g] = 0;
+      for (uint32_t p = 0; p < cv.P; ++p) {
+        const uint4 w = load_plane(cv, p, ct, lane);
 #pragma unroll
-      for (uint32_t g = 0; g < 16; ++g) off[g] = (off[g] << 8) | byte_of(w, g);
+        for (uint32_t g = 0; g < 16; ++g) off[g] = (off[g] << 8) | byte_of(w, g);
+      }
+      if (cv.r) {
+        uint32_t h[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) h[j] = j < cv.r ? load_tail(cv, j, ct, lane) : 0u;
+#pragma unroll
+        for (uint32_t g = 0; g < 16; ++g) off[g] = (off[g] << cv.r) | tail_of(h, cv.r, g);
+      }
+      uint64_t m = 0;
+#pragma unroll
+      for (uint32_t g = 0; g < 16; ++g) m = off[g] > m ? off[g] : m;   // rows >= n are zero
+      m = uniform_u64(wave_max_u64(m));
+      best = m > best ? m : best;
+      have = true;
     }
-    uint64_t m = 0;
-    const uint64_t* tw = cv.tail + c * 16 * cv.r;
-#pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) {
-      uint64_t v = off[g];
-      if (cv.r) v = (v << cv.r) | tail_value(tw, cv.r, g, lane);
-      m = v > m ? v : m;   // rows >= n are zero and never exceed a real row
-    }
-    m = uniform_u64(wave_max_u64(m));
-    best = m > best ? m : best;
-    have = true;
   }
   if (have && lane == 0) atomicMax(best_out, (unsigned long long)best);
 }
@@ -459,58 +627,54 @@ __device__ __forceinline__ uint32_t packed_tail_bit(const ChunkView& cv, uint64_
   if (row >= cv.n) return 0;
   const uint64_t c = row / kTileRows;
   const uint32_t g = (uint32_t)((row % kTileRows) / 64), t = (uint32_t)(row % 64);
-  return (uint32_t)((cv.tail[c * 16 * cv.r + g * cv.r + j] >> t) & 1ull);
+  return (cv.tail[(c * cv.r + j) * 64 + t] >> g) & 1u;
 }
 
+template <int NP, int U>
 __global__ __launch_bounds__(256) void k_materialize(ChunkView cv, MatParams mp, double* __restrict__ out) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t c = wave; c < cv.ntiles; c += nwaves) {
-    uint64_t off[16];
+  const uint64_t ngroups = (cv.ntiles + U - 1) / U;
+  const bool tail_top = mp.tail_bits && mp.mode != kMatRefPrecision;
+  const uint32_t lo = cv.r - mp.tail_bits;   // lowest tail bit kept
+  for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
+    const uint64_t c0 = grp * U;
+    constexpr int NPP = NP > 0 ? NP : 1;
+    uint4 w[U][NPP];
+    uint32_t h[U][8];
 #pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) off[g] = 0;
-    // whole planes read (top `mp.planes` bytes)
-    for (uint32_t p = 0; p < mp.planes; ++p) {
-      const uint4 w = reinterpret_cast<const uint4*>(cv.planes + p * cv.pitch + c * kTileRows)[lane];
+    for (int u = 0; u < U; ++u) {
+      const bool ok = c0 + u < cv.ntiles;
 #pragma unroll
-      for (uint32_t g = 0; g < 16; ++g) off[g] = (off[g] << 8) | byte_of(w, g);
+      for (int p = 0; p < NP; ++p) w[u][p] = ok ? load_plane(cv, p, c0 + u, lane) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+      for (uint32_t j = 0; j < 8; ++j)
+        h[u][j] = (ok && tail_top && j >= lo && j < cv.r) ? load_tail(cv, j, c0 + u, lane) : 0u;
     }
-    if (mp.mode == kMatFull || mp.mode == kMatSemantic) {
-      if (mp.tail_bits) {
-        const uint64_t* tw = cv.tail + c * 16 * cv.r;
 #pragma unroll
-        for (uint32_t g = 0; g < 16; ++g) {
-          // top mp.tail_bits of the r tail bits
-          uint64_t v = 0;
-          for (uint32_t k = 0; k < mp.tail_bits; ++k)
-            v = (v << 1) | ((tw[g * cv.r + (cv.r - 1 - k)] >> lane) & 1ull);
-          off[g] = (off[g] << mp.tail_bits) | v;
+    for (int u = 0; u < U; ++u) {
+      const uint64_t c = c0 + u;
+      if (c >= cv.ntiles) break;
+      const bool full = tile_full(cv.n, c);
+      double* ot = out + c * kTileRows + lane;
+#pragma unroll 4
+      for (uint32_t g = 0; g < 16; ++g) {
+        uint64_t v = 0;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) v = (v << 8) | byte_of(w[u][p], g);
+        const uint64_t row = c * kTileRows + 64 * g + lane;
+        if (tail_top) {
+          v = (v << mp.tail_bits) | (tail_of(h[u], cv.r, g) >> lo);
+        } else if (mp.tail_bits) {
+          uint64_t t = 0;
+          if (row < cv.n)
+            for (uint32_t q = 0; q < mp.tail_bits; ++q) t |= (uint64_t)packed_tail_bit(cv, row * mp.tail_bits + q) << q;
+          v = (v << mp.tail_bits) | t;
         }
+        const uint64_t o = (v >> mp.drop) << mp.shift_back;
+        if (full || row < cv.n) ot[64 * g] = (double)(int64_t)((uint64_t)cv.base + o) * mp.inv_scale;
       }
-#pragma unroll
-      for (uint32_t g = 0; g < 16; ++g) off[g] = (off[g] >> mp.drop) << mp.shift_back;
-    } else {  // kMatRefPrecision: field of s bits at stride s in the packed tail
-      if (mp.tail_bits) {
-#pragma unroll
-        for (uint32_t g = 0; g < 16; ++g) {
-          const uint64_t row = c * kTileRows + 64 * g + lane;
-          uint64_t v = 0;
-          if (row < cv.n) {
-            for (uint32_t q = 0; q < mp.tail_bits; ++q)
-              v |= (uint64_t)packed_tail_bit(cv, row * mp.tail_bits + q) << q;
-          }
-          off[g] = (off[g] << mp.tail_bits) | v;
-        }
-      }
-    }
-    double* ot = out + c * kTileRows;
-    const bool full = (c + 1) * kTileRows <= cv.n;
-#pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) {
-      const uint64_t row = c * kTileRows + 64 * g + lane;
-      const double v = (double)(int64_t)((uint64_t)cv.base + off[g]) * mp.inv_scale;
-      if (full || row < cv.n) ot[64 * g + lane] = v;
     }
   }
 }
@@ -525,10 +689,10 @@ __global__ __launch_bounds__(256) void k_serialize(ChunkView cv, uint8_t flipmas
   const uint32_t wslot = threadIdx.x >> 6;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  __shared__ uint64_t tw_s[4][16 * 8];
+  __shared__ uint16_t tw_s[4][8 * 64];
   for (uint64_t c = wave; c < cv.ntiles; c += nwaves) {
     for (uint32_t p = 0; p < cv.P; ++p) {
-      const uint4 w = reinterpret_cast<const uint4*>(cv.planes + p * cv.pitch + c * kTileRows)[lane];
+      const uint4 w = load_plane(cv, p, c, lane);
       uint8_t* dst = body + p * cv.n;
 #pragma unroll
       for (uint32_t g = 0; g < 16; ++g) {
@@ -537,10 +701,10 @@ __global__ __launch_bounds__(256) void k_serialize(ChunkView cv, uint8_t flipmas
       }
     }
     if (cv.r) {
-      const uint32_t nw = 16 * cv.r;
-      for (uint32_t i = lane; i < nw; i += 64) tw_s[wslot][i] = cv.tail[c * nw + i];
+      for (uint32_t j = 0; j < cv.r; ++j) tw_s[wslot][j * 64 + lane] = (uint16_t)load_tail(cv, j, c, lane);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       uint8_t* tdst = body + cv.P * cv.n;
       const uint32_t tile_bytes = 128 * cv.r;   // 1024 rows * r bits
       for (uint32_t m = lane; m < tile_bytes; m += 64) {
@@ -551,12 +715,13 @@ __global__ __launch_bounds__(256) void k_serialize(ChunkView cv, uint8_t flipmas
           const uint32_t bit = 8 * m + q;
           const uint32_t row = bit / cv.r, j = bit % cv.r;
           const uint32_t g = row / 64, t = row % 64;
-          byte |= (uint32_t)((tw_s[wslot][g * cv.r + j] >> t) & 1ull) << q;
+          byte |= (uint32_t)((tw_s[wslot][j * 64 + t] >> g) & 1u) << q;
         }
         tdst[gb] = (uint8_t)byte;
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
 }
@@ -594,7 +759,7 @@ __global__ __launch_bounds__(256) void k_load(ChunkView cv, uint8_t flipmask, co
         }
         off[g] = v;
       }
-      store_tail_bits(cv.tail + c * 16 * cv.r, off, cv.r);
+      store_tail_planes(cv.tail + c * 64 * cv.r, off, cv.r, lane);
     }
   }
 }
@@ -602,8 +767,8 @@ __global__ __launch_bounds__(256) void k_load(ChunkView cv, uint8_t flipmask, co
 // ----------------------------------------------------------------------------
 // Host-side launchers (declared in buff_kernels.h).
 
-static inline uint32_t grid_for(uint64_t ntiles, uint32_t waves_per_wg, uint32_t max_wg) {
-  uint64_t wgs = (ntiles + waves_per_wg - 1) / waves_per_wg;
+static inline uint32_t grid_for(uint64_t units, uint32_t waves_per_wg, uint32_t max_wg) {
+  uint64_t wgs = (units + waves_per_wg - 1) / waves_per_wg;
   if (wgs < 1) wgs = 1;
   if (wgs > max_wg) wgs = max_wg;
   return (uint32_t)wgs;
@@ -631,11 +796,12 @@ hipError_t launch_encode(const double* x, const ChunkView& cv, int32_t pexp, hip
 
 hipError_t launch_filter(const ChunkView& cv, int op, const FilterTarget& tg, uint64_t* out, uint64_t nwords,
                          unsigned long long* stats, hipStream_t s, const LaunchShape& ls) {
-  const uint32_t wg = grid_for(cv.ntiles, ls.waves_per_wg, ls.max_wg);
+  constexpr int U = 4;
+  const uint32_t wg = grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, ls.max_wg);
   if (op == 0)
-    hipLaunchKernelGGL(k_filter<0>, dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, tg, out, nwords, stats);
+    hipLaunchKernelGGL((k_filter<0, U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, tg, out, nwords, stats);
   else
-    hipLaunchKernelGGL(k_filter<1>, dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, tg, out, nwords, stats);
+    hipLaunchKernelGGL((k_filter<1, U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, tg, out, nwords, stats);
   return hipGetLastError();
 }
 
@@ -646,22 +812,77 @@ hipError_t launch_fill_bitmap(uint64_t* out, uint64_t nwords, uint64_t n, int on
   return hipGetLastError();
 }
 
-hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
-  const uint32_t wg = grid_for(cv.ntiles, ls.waves_per_wg, ls.max_wg);
-  hipLaunchKernelGGL(k_sum, dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, sums);
+hipError_t launch_fill_bytes(void* out, uint64_t bytes, uint8_t value, hipStream_t s) {
+  const uint32_t v = 0x01010101u * value;
+  const uint64_t n16 = bytes / 16;
+  if (n16) hipLaunchKernelGGL(k_fill_bytes, dim3(2048), dim3(256), 0, s, (uint4*)out, n16, v);
+  if (bytes % 16) {
+    hipError_t e = hipMemsetAsync((uint8_t*)out + n16 * 16, value, bytes % 16, s);
+    if (e != hipSuccess) return e;
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_max(const ChunkView& cv, unsigned long long* best, hipStream_t s, const LaunchShape& ls) {
-  const uint32_t wg = grid_for(cv.ntiles, ls.waves_per_wg, ls.max_wg);
-  hipLaunchKernelGGL(k_max, dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, best);
+template <int P>
+static void launch_sum_p(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
+  constexpr int U = P <= 2 ? 4 : 2;
+  const uint32_t wg = grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, ls.max_wg);
+  hipLaunchKernelGGL((k_sum<P, U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, sums);
+}
+
+hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
+  switch (cv.P) {
+    case 0: launch_sum_p<0>(cv, sums, s, ls); break;
+    case 1: launch_sum_p<1>(cv, sums, s, ls); break;
+    case 2: launch_sum_p<2>(cv, sums, s, ls); break;
+    case 3: launch_sum_p<3>(cv, sums, s, ls); break;
+    case 4: launch_sum_p<4>(cv, sums, s, ls); break;
+    case 5: launch_sum_p<5>(cv, sums, s, ls); break;
+    case 6: launch_sum_p<6>(cv, sums, s, ls); break;
+    case 7: launch_sum_p<7>(cv, sums, s, ls); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
+}
+
+uint64_t max_scratch_bytes(uint64_t ntiles) { return ((ntiles + 255) / 256) * 256 + 256; }
+
+// scratch: per-tile maxima (ntiles bytes), then the 4-byte global top byte;
+// the caller zeroes the top word and best.
+hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* scratch, hipStream_t s,
+                      const LaunchShape& ls) {
+  unsigned int* top = reinterpret_cast<unsigned int*>(scratch + ((cv.ntiles + 255) / 256) * 256);
+  if (cv.P > 0) {
+    constexpr int U = 8;
+    const uint32_t wg = grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, ls.max_wg);
+    hipLaunchKernelGGL((k_max_top<U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, scratch, top);
+  }
+  const uint32_t wg2 = grid_for((cv.ntiles + 63) / 64, 4, 1024);
+  hipLaunchKernelGGL(k_max_refine, dim3(wg2), dim3(256), 0, s, cv, (const uint8_t*)scratch, (const unsigned int*)top,
+                     best);
+  return hipGetLastError();
+}
+
+template <int NP>
+static void launch_mat_p(const ChunkView& cv, const MatParams& mp, double* out, hipStream_t s, const LaunchShape& ls) {
+  constexpr int U = NP <= 2 ? 4 : 2;
+  const uint32_t wg = grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, ls.max_wg);
+  hipLaunchKernelGGL((k_materialize<NP, U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, mp, out);
 }
 
 hipError_t launch_materialize(const ChunkView& cv, const MatParams& mp, double* out, hipStream_t s,
                               const LaunchShape& ls) {
-  const uint32_t wg = grid_for(cv.ntiles, ls.waves_per_wg, ls.max_wg);
-  hipLaunchKernelGGL(k_materialize, dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, mp, out);
+  switch (mp.planes) {
+    case 0: launch_mat_p<0>(cv, mp, out, s, ls); break;
+    case 1: launch_mat_p<1>(cv, mp, out, s, ls); break;
+    case 2: launch_mat_p<2>(cv, mp, out, s, ls); break;
+    case 3: launch_mat_p<3>(cv, mp, out, s, ls); break;
+    case 4: launch_mat_p<4>(cv, mp, out, s, ls); break;
+    case 5: launch_mat_p<5>(cv, mp, out, s, ls); break;
+    case 6: launch_mat_p<6>(cv, mp, out, s, ls); break;
+    case 7: launch_mat_p<7>(cv, mp, out, s, ls); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
