@@ -291,10 +291,6 @@ int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_
     BUFF_TRY(ctx->scratch(ctx->s_minmax, sizeof(MinMaxOut)));
     MinMaxOut* dmm = (MinMaxOut*)ctx->s_minmax.ptr;
     int64_t* part = (int64_t*)ctx->s_partial.ptr;
-    if (!ctx->minmax_ready) {
-      HIP_TRY(hipMemsetAsync(dmm, 0, sizeof(MinMaxOut), ctx->stream));
-      ctx->minmax_ready = true;
-    }
     {
       ProfScope ps(ctx, "k_minmax");
       HIP_TRY(launch_minmax(x, n, pexp, dlen, part, part + 2 * nwg, dmm, nwg, false, ctx->stream));
@@ -544,11 +540,11 @@ int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64
   uint64_t ps[kMaxPlanes + 1] = {0};
   const ChunkView& v = ch->view;
   if (ch->n > 0 && (v.P || v.r)) {
-    BUFF_TRY(ctx->scratch(ctx->s_sums, sizeof(ps)));
-    HIP_TRY(hipMemsetAsync(ctx->s_sums.ptr, 0, sizeof(ps), ctx->stream));
+    const LaunchShape ls = ctx->shape(Op::Sum, v.ntiles);
+    BUFF_TRY(ctx->scratch(ctx->s_sums, sizeof(ps) + sum_scratch_bytes(ls)));
     {
       ProfScope sc(ctx, "k_sum");
-      HIP_TRY(launch_sum(v, (unsigned long long*)ctx->s_sums.ptr, ctx->stream, ctx->shape(Op::Sum, v.ntiles)));
+      HIP_TRY(launch_sum(v, (unsigned long long*)ctx->s_sums.ptr, ctx->stream, ls));
     }
     BUFF_TRY(d2h_small(ctx, ps, ctx->s_sums.ptr, sizeof(ps)));
   }
@@ -568,12 +564,11 @@ int max_op(Ctx* ctx, const Chunk* ch, double* value, uint64_t* max_off) {
   uint64_t best = 0;
   const ChunkView& v = ch->view;
   if (ch->n > 0 && (v.P || v.r)) {
-    // [best u64 | pad to 256 | per-tile maxima | global top byte]
+    // [best u64 | pad to 256 | per-tile maxima | per-workgroup top bytes]
     const uint64_t ms = max_scratch_bytes(v.ntiles);
     BUFF_TRY(ctx->scratch(ctx->s_best, 256 + ms));
     uint8_t* sb = (uint8_t*)ctx->s_best.ptr;
     HIP_TRY(hipMemsetAsync(sb, 0, 8, ctx->stream));
-    HIP_TRY(hipMemsetAsync(sb + 256 + ms - 256, 0, 256, ctx->stream));
     {
       ProfScope sc(ctx, "k_max");
       HIP_TRY(launch_max(v, (unsigned long long*)sb, sb + 256, ctx->stream, ctx->shape(Op::Max, v.ntiles)));

@@ -77,7 +77,7 @@ struct MinMaxOut {
   int64_t max_rest;   // max over rows i >= 1
   int64_t first;      // fixed value of row 0
   int64_t qmax;       // quirk max from the prefix-min pass
-  uint32_t done;      // workgroups finished (last-block reduction)
+  uint32_t unused0;
   uint32_t pad;
 };
 

@@ -35,9 +35,11 @@ hipError_t launch_filter(const ChunkView& cv, int op, const FilterTarget& tg, ui
                          unsigned long long* stats, hipStream_t s, const LaunchShape& ls);
 hipError_t launch_fill_bitmap(uint64_t* out, uint64_t nwords, uint64_t n, int ones, hipStream_t s);
 hipError_t launch_fill_bytes(void* out, uint64_t bytes, uint8_t value, hipStream_t s);
+// sums: 9 u64 results followed by sum_scratch_bytes(ls) of per-workgroup rows.
+uint64_t sum_scratch_bytes(const LaunchShape& ls);
 hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls);
-// Two passes; `scratch` holds max_scratch_bytes(ntiles) bytes whose last 256
-// bytes (the global top byte) the caller zeroes, as it zeroes `best`.
+// Two passes; `scratch` holds max_scratch_bytes(ntiles) bytes; the caller
+// zeroes `best`.
 uint64_t max_scratch_bytes(uint64_t ntiles);
 hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* scratch, hipStream_t s,
                       const LaunchShape& ls);

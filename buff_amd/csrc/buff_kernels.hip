@@ -133,7 +133,7 @@ __device__ __forceinline__ uint32_t lane_valid16(uint64_t n, uint64_t c, uint32_
 
 __global__ __launch_bounds__(256) void k_minmax(const double* __restrict__ x, uint64_t n, uint64_t ntiles,
                                                 uint64_t tpw, int32_t pexp, uint32_t dlen,
-                                                int64_t* __restrict__ partial, MinMaxOut* __restrict__ out) {
+                                                int64_t* __restrict__ partial) {
   const uint32_t lane = __lane_id();
   const uint32_t wave = threadIdx.x >> 6;
   const uint64_t t0 = (uint64_t)blockIdx.x * tpw;
@@ -167,23 +167,22 @@ __global__ __launch_bounds__(256) void k_minmax(const double* __restrict__ x, ui
   __shared__ int64_t smn[4], smx[4];
   if (lane == 0) { smn[wave] = mn; smx[wave] = mx; }
   __syncthreads();
-  __shared__ bool last;
   if (threadIdx.x == 0) {
-    mn = min(min(smn[0], smn[1]), min(smn[2], smn[3]));
-    mx = max(max(smx[0], smx[1]), max(smx[2], smx[3]));
-    partial[blockIdx.x] = mn;
-    partial[gridDim.x + blockIdx.x] = mx;
-    __threadfence();
-    last = atomicAdd(&out->done, 1u) == gridDim.x - 1;
+    partial[blockIdx.x] = min(min(smn[0], smn[1]), min(smn[2], smn[3]));
+    partial[gridDim.x + blockIdx.x] = max(max(smx[0], smx[1]), max(smx[2], smx[3]));
   }
-  __syncthreads();
-  if (!last) return;
-  // Last workgroup folds the partials (threadfence reduction).
-  __threadfence();
+}
+
+// Folds the k_minmax partials (one workgroup).
+__global__ __launch_bounds__(256) void k_minmax_final(const double* __restrict__ x, uint64_t n, int32_t pexp,
+                                                      uint32_t dlen, const int64_t* __restrict__ partial,
+                                                      uint32_t nwg, MinMaxOut* __restrict__ out) {
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  __shared__ int64_t smn[4], smx[4];
   int64_t a = INT64_MAX, b = INT64_MIN;
-  for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
-    a = min(a, __atomic_load_n(&partial[i], __ATOMIC_RELAXED));
-    b = max(b, __atomic_load_n(&partial[gridDim.x + i], __ATOMIC_RELAXED));
+  for (uint32_t i = threadIdx.x; i < nwg; i += blockDim.x) {
+    a = min(a, partial[i]);
+    b = max(b, partial[nwg + i]);
   }
   a = wave_min_i64(a);
   b = wave_max_i64(b);
@@ -193,7 +192,6 @@ __global__ __launch_bounds__(256) void k_minmax(const double* __restrict__ x, ui
     out->min_all = min(min(smn[0], smn[1]), min(smn[2], smn[3]));
     out->max_rest = max(max(smx[0], smx[1]), max(smx[2], smx[3]));
     out->first = n ? fetch_fixed_aligned_bits(__double_as_longlong(x[0]), pexp, dlen) : 0;
-    out->done = 0;
   }
 }
 
@@ -205,7 +203,7 @@ __global__ __launch_bounds__(256) void k_minmax(const double* __restrict__ x, ui
 __global__ __launch_bounds__(64) void k_minmax_prefix(const double* __restrict__ x, uint64_t n, uint64_t ntiles,
                                                       uint64_t tpw, int32_t pexp, uint32_t dlen,
                                                       const int64_t* __restrict__ partial,
-                                                      int64_t* __restrict__ partial2, MinMaxOut* __restrict__ out) {
+                                                      int64_t* __restrict__ partial2) {
   const uint32_t lane = __lane_id();
   int64_t carry = INT64_MAX;
   for (uint32_t i = lane; i < blockIdx.x; i += 64) carry = min(carry, partial[i]);
@@ -231,22 +229,15 @@ __global__ __launch_bounds__(64) void k_minmax_prefix(const double* __restrict__
     }
   }
   qmax = wave_max_i64(qmax);
-  __shared__ bool last;
-  if (lane == 0) {
-    partial2[blockIdx.x] = qmax;
-    __threadfence();
-    last = atomicAdd(&out->done, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+  if (lane == 0) partial2[blockIdx.x] = qmax;
+}
+
+__global__ __launch_bounds__(64) void k_minmax_prefix_final(const int64_t* __restrict__ partial2, uint32_t nwg,
+                                                            MinMaxOut* __restrict__ out) {
   int64_t b = INT64_MIN;
-  for (uint32_t i = lane; i < gridDim.x; i += 64) b = max(b, __atomic_load_n(&partial2[i], __ATOMIC_RELAXED));
+  for (uint32_t i = __lane_id(); i < nwg; i += 64) b = max(b, partial2[i]);
   b = wave_max_i64(b);
-  if (lane == 0) {
-    out->qmax = b;
-    out->done = 0;
-  }
+  if (__lane_id() == 0) out->qmax = b;
 }
 
 // ----------------------------------------------------------------------------
@@ -351,9 +342,13 @@ __global__ __launch_bounds__(256) void k_filter(ChunkView cv, FilterTarget tg, u
     uint4 w[U];
     uint32_t o[U];
     uint32_t need = 0;   // tiles of the group with ties (uniform bit mask)
+    if (has_planes) {
 #pragma unroll
-    for (uint32_t u = 0; u < U; ++u)
-      w[u] = (has_planes && c0 + u < cv.ntiles) ? load_plane(cv, 0, c0 + u, lane) : make_uint4(0, 0, 0, 0);
+      for (uint32_t u = 0; u < U; ++u) w[u] = load_plane(cv, 0, min(c0 + u, cv.ntiles - 1), lane);
+    } else {
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) w[u] = make_uint4(0, 0, 0, 0);
+    }
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {
       o[u] = 0;
@@ -398,8 +393,7 @@ __global__ __launch_bounds__(256) void k_filter(ChunkView cv, FilterTarget tg, u
         }
         uint4 wp[U];
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-          wp[u] = (live & (1u << u)) ? load_plane(cv, p, c0 + u, lane) : make_uint4(0, 0, 0, 0);
+        for (uint32_t u = 0; u < U; ++u) wp[u] = load_plane(cv, p, min(c0 + u, cv.ntiles - 1), lane);
         const uint32_t dp = tg.digit[p];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
@@ -424,7 +418,7 @@ __global__ __launch_bounds__(256) void k_filter(ChunkView cv, FilterTarget tg, u
             if (!(live & (1u << u))) continue;
             uint32_t h[8];
 #pragma unroll
-            for (uint32_t j = 0; j < 8; ++j) h[j] = j < cv.r ? load_tail(cv, j, c0 + u, lane) : 0u;
+            for (uint32_t j = 0; j < 8; ++j) h[j] = load_tail(cv, min(j, cv.r - 1), c0 + u, lane);
 #pragma unroll
             for (uint32_t g = 0; g < 16; ++g) {
               const uint64_t t = tail_of(h, cv.r, g);
@@ -471,37 +465,41 @@ __global__ void k_fill_bytes(uint4* __restrict__ out, uint64_t n16, uint32_t v) 
 // ----------------------------------------------------------------------------
 // K5: per-sub-column byte sums (split_double.rs:2158-2204), exact in u64.
 // Bytes are summed with v_sad_u8, tail bit-planes with popcounts weighted by
-// their bit position. P is a template parameter so that all plane and tail
-// loads of a step are issued at once.
+// their bit position. P (planes) and R (tail bits) are template parameters and
+// every load of a step is unconditional (the last group clamps its tile index
+// and masks the duplicate), so all of them are in flight together.
 
-template <int P, int U>
+template <int P, int R, int U>
 __global__ __launch_bounds__(256) void k_sum(ChunkView cv, unsigned long long* __restrict__ sums) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const uint64_t ngroups = (cv.ntiles + U - 1) / U;
-  const uint32_t r = cv.r;
   constexpr int PP = P > 0 ? P : 1;
+  constexpr int RR = R > 0 ? R : 1;
   uint32_t acc[PP];
   uint64_t wide[PP];
 #pragma unroll
   for (int p = 0; p < PP; ++p) { acc[p] = 0; wide[p] = 0; }
-  uint32_t tacc[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // per-bit popcounts
+  uint32_t tacc[RR];   // per-bit popcounts
+#pragma unroll
+  for (int j = 0; j < RR; ++j) tacc[j] = 0;
   uint32_t iter = 0;
   for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
     const uint64_t c0 = grp * U;
     uint4 w[U][PP];
-    uint32_t h[U][8];
+    uint32_t h[U][RR];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const bool ok = c0 + u < cv.ntiles;
+      const uint64_t c = min(c0 + u, cv.ntiles - 1);
 #pragma unroll
-      for (int p = 0; p < P; ++p) w[u][p] = ok ? load_plane(cv, p, c0 + u, lane) : make_uint4(0, 0, 0, 0);
+      for (int p = 0; p < P; ++p) w[u][p] = load_plane(cv, p, c, lane);
 #pragma unroll
-      for (uint32_t j = 0; j < 8; ++j) h[u][j] = (ok && j < r) ? load_tail(cv, j, c0 + u, lane) : 0u;
+      for (int j = 0; j < R; ++j) h[u][j] = cv.tail[(c * R + j) * 64 + lane];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
+      if (c0 + u >= cv.ntiles) break;   // clamped duplicate of the last tile
 #pragma unroll
       for (int p = 0; p < P; ++p) {
         uint32_t a = __builtin_amdgcn_sad_u8(w[u][p].x, 0u, acc[p]);
@@ -510,7 +508,7 @@ __global__ __launch_bounds__(256) void k_sum(ChunkView cv, unsigned long long* _
         acc[p] = __builtin_amdgcn_sad_u8(w[u][p].w, 0u, a);
       }
 #pragma unroll
-      for (uint32_t j = 0; j < 8; ++j) tacc[j] += __popc(h[u][j]);
+      for (int j = 0; j < R; ++j) tacc[j] += __popc(h[u][j]);
     }
     // a lane adds at most 16*255 per tile and plane: fold into u64 long before u32 overflow
     if (++iter == (1u << 14)) {
@@ -519,18 +517,43 @@ __global__ __launch_bounds__(256) void k_sum(ChunkView cv, unsigned long long* _
       iter = 0;
     }
   }
+  // wave sums -> workgroup row in LDS -> one partial row per workgroup
+  __shared__ uint64_t rows[16][kMaxPlanes + 1];
+  const uint32_t wslot = threadIdx.x >> 6;
 #pragma unroll
   for (int p = 0; p < P; ++p) {
     const uint64_t s = wave_sum_u64(wide[p] + acc[p]);
-    if (lane == 0 && s) atomicAdd(&sums[p], (unsigned long long)s);
+    if (lane == 0) rows[wslot][p] = s;
   }
-  if (r) {
-    uint64_t t = 0;
+  uint64_t t = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < 8; ++j) t += (uint64_t)tacc[j] << j;
-    const uint64_t s = wave_sum_u64(t);
-    if (lane == 0 && s) atomicAdd(&sums[P], (unsigned long long)s);
+  for (int j = 0; j < R; ++j) t += (uint64_t)tacc[j] << j;
+  t = wave_sum_u64(t);
+  if (lane == 0) rows[wslot][P] = t;
+  __syncthreads();
+  if (threadIdx.x <= (uint32_t)P) {
+    uint64_t v = 0;
+    for (uint32_t k = 0; k < (blockDim.x >> 6); ++k) v += rows[k][threadIdx.x];
+    sums[(uint64_t)blockIdx.x * (kMaxPlanes + 1) + threadIdx.x] = v;
   }
+}
+
+// Sums the per-workgroup rows of k_sum (one workgroup).
+__global__ __launch_bounds__(256) void k_sum_final(const unsigned long long* __restrict__ rows, uint32_t nrows,
+                                                   uint32_t width, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long part[256 / 64][kMaxPlanes + 1];
+  const uint32_t lane = __lane_id(), wslot = threadIdx.x >> 6;
+  for (uint32_t k = 0; k < width; ++k) {
+    uint64_t v = 0;
+    for (uint32_t i = threadIdx.x; i < nrows; i += blockDim.x) v += rows[(uint64_t)i * (kMaxPlanes + 1) + k];
+    v = wave_sum_u64(v);
+    if (lane == 0) part[wslot][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxPlanes + 1)
+    out[threadIdx.x] = threadIdx.x < width ? part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                                                  part[3][threadIdx.x]
+                                            : 0ull;
 }
 
 // ----------------------------------------------------------------------------
@@ -552,8 +575,7 @@ __global__ __launch_bounds__(256) void k_max_top(ChunkView cv, uint8_t* __restri
     const uint64_t c0 = grp * U;
     uint4 w[U];
 #pragma unroll
-    for (uint32_t u = 0; u < U; ++u)
-      w[u] = c0 + u < cv.ntiles ? load_plane(cv, 0, c0 + u, lane) : make_uint4(0, 0, 0, 0);
+    for (uint32_t u = 0; u < U; ++u) w[u] = load_plane(cv, 0, min(c0 + u, cv.ntiles - 1), lane);
     uint32_t mine = 0;   // lane u holds tile c0+u's max after the loop
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {
@@ -566,17 +588,33 @@ __global__ __launch_bounds__(256) void k_max_top(ChunkView cv, uint8_t* __restri
     }
     if (lane < U && c0 + lane < cv.ntiles) tile_max[c0 + lane] = (uint8_t)mine;
   }
-  if (lane == 0) atomicMax(top, best);
+  __shared__ uint32_t wbest[16];
+  if (lane == 0) wbest[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t b = 0;
+    for (uint32_t k = 0; k < (blockDim.x >> 6); ++k) b = max(b, wbest[k]);
+    top[blockIdx.x] = b;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_max_refine(ChunkView cv, const uint8_t* __restrict__ tile_max,
-                                                    const unsigned int* __restrict__ top,
+                                                    const unsigned int* __restrict__ top, uint32_t ntop,
                                                     unsigned long long* __restrict__ best_out) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const bool has_planes = cv.P > 0;
-  const uint32_t t = has_planes ? *top : 0u;
+  // fold the per-workgroup top bytes of pass 1
+  __shared__ uint32_t wtop[16];
+  uint32_t tl = 0;
+  if (has_planes)
+    for (uint32_t i = threadIdx.x; i < ntop; i += blockDim.x) tl = max(tl, top[i]);
+  tl = wave_max_u32(tl);
+  if (lane == 0) wtop[threadIdx.x >> 6] = tl;
+  __syncthreads();
+  uint32_t t = 0;
+  for (uint32_t k = 0; k < (blockDim.x >> 6); ++k) t = max(t, wtop[k]);
   uint64_t best = 0;
   bool have = false;
   // 64 tiles per probe: one byte per lane
@@ -630,27 +668,26 @@ __device__ __forceinline__ uint32_t packed_tail_bit(const ChunkView& cv, uint64_
   return (cv.tail[(c * cv.r + j) * 64 + t] >> g) & 1u;
 }
 
-template <int NP, int U>
+template <int NP, int NT, int U>
 __global__ __launch_bounds__(256) void k_materialize(ChunkView cv, MatParams mp, double* __restrict__ out) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const uint64_t ngroups = (cv.ntiles + U - 1) / U;
-  const bool tail_top = mp.tail_bits && mp.mode != kMatRefPrecision;
-  const uint32_t lo = cv.r - mp.tail_bits;   // lowest tail bit kept
+  const uint32_t lo = cv.r - NT;   // lowest tail bit kept
+  constexpr int NPP = NP > 0 ? NP : 1;
+  constexpr int NTT = NT > 0 ? NT : 1;
   for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
     const uint64_t c0 = grp * U;
-    constexpr int NPP = NP > 0 ? NP : 1;
     uint4 w[U][NPP];
-    uint32_t h[U][8];
+    uint32_t h[U][NTT];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const bool ok = c0 + u < cv.ntiles;
+      const uint64_t c = min(c0 + u, cv.ntiles - 1);
 #pragma unroll
-      for (int p = 0; p < NP; ++p) w[u][p] = ok ? load_plane(cv, p, c0 + u, lane) : make_uint4(0, 0, 0, 0);
+      for (int p = 0; p < NP; ++p) w[u][p] = load_plane(cv, p, c, lane);
 #pragma unroll
-      for (uint32_t j = 0; j < 8; ++j)
-        h[u][j] = (ok && tail_top && j >= lo && j < cv.r) ? load_tail(cv, j, c0 + u, lane) : 0u;
+      for (int k = 0; k < NT; ++k) h[u][k] = cv.tail[(c * cv.r + lo + k) * 64 + lane];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -663,18 +700,33 @@ __global__ __launch_bounds__(256) void k_materialize(ChunkView cv, MatParams mp,
         uint64_t v = 0;
 #pragma unroll
         for (int p = 0; p < NP; ++p) v = (v << 8) | byte_of(w[u][p], g);
-        const uint64_t row = c * kTileRows + 64 * g + lane;
-        if (tail_top) {
-          v = (v << mp.tail_bits) | (tail_of(h[u], cv.r, g) >> lo);
-        } else if (mp.tail_bits) {
-          uint64_t t = 0;
-          if (row < cv.n)
-            for (uint32_t q = 0; q < mp.tail_bits; ++q) t |= (uint64_t)packed_tail_bit(cv, row * mp.tail_bits + q) << q;
-          v = (v << mp.tail_bits) | t;
-        }
+#pragma unroll
+        for (int k = NT - 1; k >= 0; --k) v = (v << 1) | ((h[u][k] >> g) & 1u);
         const uint64_t o = (v >> mp.drop) << mp.shift_back;
-        if (full || row < cv.n) ot[64 * g] = (double)(int64_t)((uint64_t)cv.base + o) * mp.inv_scale;
+        if (full || c * kTileRows + 64 * g + lane < cv.n)
+          ot[64 * g] = (double)(int64_t)((uint64_t)cv.base + o) * mp.inv_scale;
       }
+    }
+  }
+}
+
+// The reference's reduced-precision layout reading (rare, compatibility only).
+__global__ __launch_bounds__(256) void k_materialize_ref(ChunkView cv, MatParams mp, double* __restrict__ out) {
+  const uint32_t lane = __lane_id();
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t c = wave; c < cv.ntiles; c += nwaves) {
+    for (uint32_t g = 0; g < 16; ++g) {
+      const uint64_t row = c * kTileRows + 64 * g + lane;
+      if (row >= cv.n) continue;
+      uint64_t v = 0;
+      for (uint32_t p = 0; p < mp.planes; ++p) v = (v << 8) | cv.planes[p * cv.pitch + c * kTileRows + 16 * lane + g];
+      if (mp.tail_bits) {
+        uint64_t t = 0;
+        for (uint32_t q = 0; q < mp.tail_bits; ++q) t |= (uint64_t)packed_tail_bit(cv, row * mp.tail_bits + q) << q;
+        v = (v << mp.tail_bits) | t;
+      }
+      out[row] = (double)(int64_t)((uint64_t)cv.base + v) * mp.inv_scale;
     }
   }
 }
@@ -779,10 +831,13 @@ hipError_t launch_minmax(const double* x, uint64_t n, int32_t pexp, uint32_t dle
   const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
   const uint64_t tpw = (ntiles + nwg - 1) / nwg;
   if (!prefix) {
-    hipLaunchKernelGGL(k_minmax, dim3(nwg), dim3(256), 0, s, x, n, ntiles, tpw, pexp, dlen, partial, out);
+    hipLaunchKernelGGL(k_minmax, dim3(nwg), dim3(256), 0, s, x, n, ntiles, tpw, pexp, dlen, partial);
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, s, x, n, pexp, dlen, (const int64_t*)partial, nwg,
+                       out);
   } else {
     hipLaunchKernelGGL(k_minmax_prefix, dim3(nwg), dim3(64), 0, s, x, n, ntiles, tpw, pexp, dlen,
-                       (const int64_t*)partial, partial2, out);
+                       (const int64_t*)partial, partial2);
+    hipLaunchKernelGGL(k_minmax_prefix_final, dim3(1), dim3(64), 0, s, (const int64_t*)partial2, nwg, out);
   }
   return hipGetLastError();
 }
@@ -823,67 +878,110 @@ hipError_t launch_fill_bytes(void* out, uint64_t bytes, uint8_t value, hipStream
   return hipGetLastError();
 }
 
-template <int P>
-static void launch_sum_p(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
-  constexpr int U = P <= 2 ? 4 : 2;
+template <int P, int R>
+static void launch_sum_pr(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
+  constexpr int U = (P + R) <= 3 ? 4 : 2;
   const uint32_t wg = grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, ls.max_wg);
-  hipLaunchKernelGGL((k_sum<P, U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, sums);
+  unsigned long long* rows = sums + (kMaxPlanes + 1);
+  hipLaunchKernelGGL((k_sum<P, R, U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, rows);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, (const unsigned long long*)rows, wg, (uint32_t)(P + 1),
+                     sums);
+}
+
+uint64_t sum_scratch_bytes(const LaunchShape& ls) { return (uint64_t)(ls.max_wg + 1) * (kMaxPlanes + 1) * 8; }
+
+template <int P>
+static hipError_t launch_sum_p(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
+  switch (cv.r) {
+    case 0: launch_sum_pr<P, 0>(cv, sums, s, ls); break;
+    case 1: launch_sum_pr<P, 1>(cv, sums, s, ls); break;
+    case 2: launch_sum_pr<P, 2>(cv, sums, s, ls); break;
+    case 3: launch_sum_pr<P, 3>(cv, sums, s, ls); break;
+    case 4: launch_sum_pr<P, 4>(cv, sums, s, ls); break;
+    case 5: launch_sum_pr<P, 5>(cv, sums, s, ls); break;
+    case 6: launch_sum_pr<P, 6>(cv, sums, s, ls); break;
+    case 7: launch_sum_pr<P, 7>(cv, sums, s, ls); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
   switch (cv.P) {
-    case 0: launch_sum_p<0>(cv, sums, s, ls); break;
-    case 1: launch_sum_p<1>(cv, sums, s, ls); break;
-    case 2: launch_sum_p<2>(cv, sums, s, ls); break;
-    case 3: launch_sum_p<3>(cv, sums, s, ls); break;
-    case 4: launch_sum_p<4>(cv, sums, s, ls); break;
-    case 5: launch_sum_p<5>(cv, sums, s, ls); break;
-    case 6: launch_sum_p<6>(cv, sums, s, ls); break;
-    case 7: launch_sum_p<7>(cv, sums, s, ls); break;
+    case 0: return launch_sum_p<0>(cv, sums, s, ls);
+    case 1: return launch_sum_p<1>(cv, sums, s, ls);
+    case 2: return launch_sum_p<2>(cv, sums, s, ls);
+    case 3: return launch_sum_p<3>(cv, sums, s, ls);
+    case 4: return launch_sum_p<4>(cv, sums, s, ls);
+    case 5: return launch_sum_p<5>(cv, sums, s, ls);
+    case 6: return launch_sum_p<6>(cv, sums, s, ls);
+    case 7: return launch_sum_p<7>(cv, sums, s, ls);
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
-uint64_t max_scratch_bytes(uint64_t ntiles) { return ((ntiles + 255) / 256) * 256 + 256; }
+uint64_t max_scratch_bytes(uint64_t ntiles) { return ((ntiles + 255) / 256) * 256 + 4 * 8192; }
 
-// scratch: per-tile maxima (ntiles bytes), then the 4-byte global top byte;
-// the caller zeroes the top word and best.
+// scratch: per-tile maxima (ntiles bytes, padded), then one top byte per
+// pass-1 workgroup; the caller zeroes `best`.
 hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* scratch, hipStream_t s,
                       const LaunchShape& ls) {
   unsigned int* top = reinterpret_cast<unsigned int*>(scratch + ((cv.ntiles + 255) / 256) * 256);
+  uint32_t wg1 = 0;
   if (cv.P > 0) {
     constexpr int U = 8;
-    const uint32_t wg = grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, ls.max_wg);
-    hipLaunchKernelGGL((k_max_top<U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, scratch, top);
+    wg1 = grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, std::min<uint32_t>(ls.max_wg, 8192));
+    hipLaunchKernelGGL((k_max_top<U>), dim3(wg1), dim3(64 * ls.waves_per_wg), 0, s, cv, scratch, top);
   }
   const uint32_t wg2 = grid_for((cv.ntiles + 63) / 64, 4, 1024);
   hipLaunchKernelGGL(k_max_refine, dim3(wg2), dim3(256), 0, s, cv, (const uint8_t*)scratch, (const unsigned int*)top,
-                     best);
+                     wg1, best);
   return hipGetLastError();
 }
 
-template <int NP>
-static void launch_mat_p(const ChunkView& cv, const MatParams& mp, double* out, hipStream_t s, const LaunchShape& ls) {
-  constexpr int U = NP <= 2 ? 4 : 2;
+template <int NP, int NT>
+static void launch_mat_pt(const ChunkView& cv, const MatParams& mp, double* out, hipStream_t s,
+                          const LaunchShape& ls) {
+  constexpr int U = (NP + NT) <= 3 ? 4 : 2;
   const uint32_t wg = grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, ls.max_wg);
-  hipLaunchKernelGGL((k_materialize<NP, U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, mp, out);
+  hipLaunchKernelGGL((k_materialize<NP, NT, U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, mp, out);
+}
+
+template <int NP>
+static hipError_t launch_mat_p(const ChunkView& cv, const MatParams& mp, double* out, hipStream_t s,
+                               const LaunchShape& ls) {
+  switch (mp.tail_bits) {
+    case 0: launch_mat_pt<NP, 0>(cv, mp, out, s, ls); break;
+    case 1: launch_mat_pt<NP, 1>(cv, mp, out, s, ls); break;
+    case 2: launch_mat_pt<NP, 2>(cv, mp, out, s, ls); break;
+    case 3: launch_mat_pt<NP, 3>(cv, mp, out, s, ls); break;
+    case 4: launch_mat_pt<NP, 4>(cv, mp, out, s, ls); break;
+    case 5: launch_mat_pt<NP, 5>(cv, mp, out, s, ls); break;
+    case 6: launch_mat_pt<NP, 6>(cv, mp, out, s, ls); break;
+    case 7: launch_mat_pt<NP, 7>(cv, mp, out, s, ls); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_materialize(const ChunkView& cv, const MatParams& mp, double* out, hipStream_t s,
                               const LaunchShape& ls) {
+  if (mp.mode == kMatRefPrecision) {
+    const uint32_t wg = grid_for(cv.ntiles, 4, 2048);
+    hipLaunchKernelGGL(k_materialize_ref, dim3(wg), dim3(256), 0, s, cv, mp, out);
+    return hipGetLastError();
+  }
   switch (mp.planes) {
-    case 0: launch_mat_p<0>(cv, mp, out, s, ls); break;
-    case 1: launch_mat_p<1>(cv, mp, out, s, ls); break;
-    case 2: launch_mat_p<2>(cv, mp, out, s, ls); break;
-    case 3: launch_mat_p<3>(cv, mp, out, s, ls); break;
-    case 4: launch_mat_p<4>(cv, mp, out, s, ls); break;
-    case 5: launch_mat_p<5>(cv, mp, out, s, ls); break;
-    case 6: launch_mat_p<6>(cv, mp, out, s, ls); break;
-    case 7: launch_mat_p<7>(cv, mp, out, s, ls); break;
+    case 0: return launch_mat_p<0>(cv, mp, out, s, ls);
+    case 1: return launch_mat_p<1>(cv, mp, out, s, ls);
+    case 2: return launch_mat_p<2>(cv, mp, out, s, ls);
+    case 3: return launch_mat_p<3>(cv, mp, out, s, ls);
+    case 4: return launch_mat_p<4>(cv, mp, out, s, ls);
+    case 5: return launch_mat_p<5>(cv, mp, out, s, ls);
+    case 6: return launch_mat_p<6>(cv, mp, out, s, ls);
+    case 7: return launch_mat_p<7>(cv, mp, out, s, ls);
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 hipError_t launch_serialize(const ChunkView& cv, uint8_t flipmask, uint8_t* body, uint64_t tail_bytes,
