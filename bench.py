@@ -13,8 +13,9 @@ operation over the step time.
 Per-kernel durations come from HIP events recorded on the library's stream
 during the timed steps. `roofline` is reported for the kernel with the largest
 share of the step, from its algorithmic bytes (HBM bytes an ideal kernel must
-move) over its average duration against 8 TB/s. A cold-cache pass (memory-side
-cache evicted before each launch) is reported beside it.
+move) over its average duration against 8 TB/s. A cold-cache pass (the
+memory-side cache refilled with 1 GiB of other, clean data before each launch)
+is reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N] [--no-cpu]
 """
@@ -213,13 +214,32 @@ def main():
                   "k_max": [algorithmic_bytes(i, "k_max") for i in infos],
                   "k_materialize": [algorithmic_bytes(i, "k_materialize") for i in infos]}
     cold = {}
-    ctx.prof_reset()
     trials = 3
+    # each range filter alone, cold: per column and selectivity
+    filters = {}
+    for c, ch in zip(cols, keep):
+        for sel in (0.001, 0.10, 0.50):
+            ctx.prof_reset()
+            for trial in range(trials):
+                flush.touch()
+                ctx.synchronize()
+                ctx.prof_enable(True)
+                ch.filter_device("range", c.preds[sel], words.ptr)
+                ctx.synchronize()
+                ctx.prof_enable(False)
+            launches, total_ms = ctx.prof_query("k_filter_gt")
+            if launches:
+                us = total_ms / launches * 1e3
+                gbps = filt[(c.name, sel)] / (us * 1e-6) / 1e9
+                filters["%s@%g%%" % (c.name, sel * 100)] = {"avg_us": round(us, 2), "achieved_GBps": round(gbps, 1),
+                                                            "frac": round(gbps / HBM_PEAK_GBPS, 4),
+                                                            "algorithmic_bytes": round(filt[(c.name, sel)])}
+    ctx.prof_reset()
     for trial in range(trials):
         for c, ch in zip(cols, keep):
             for fn in (lambda: ch.filter_device("range", c.preds[0.10], words.ptr), lambda: ch.sum(),
                        lambda: ch.max(), lambda: ch.materialize_device(out.ptr, -1)):
-                flush.fill(trial & 0xFF)
+                flush.touch()
                 ctx.synchronize()
                 ctx.prof_enable(True)
                 fn()
@@ -257,6 +277,7 @@ def main():
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": kernels[dominant]["frac"], "traffic": None},
         "kernels": kernels,
         "cold_cache": cold,
+        "cold_filters": filters,
     }
 
     if not args.no_cpu:

@@ -80,6 +80,7 @@ SIGNATURES = {
     "buff_memcpy_h2d": [vp, vp, vp, u64],
     "buff_memcpy_d2h": [vp, vp, vp, u64],
     "buff_dev_fill": [vp, vp, ctypes.c_uint8, u64],
+    "buff_dev_touch": [vp, vp, u64],
     "buff_prof_enable": [vp, cint],
     "buff_prof_reset": [vp],
     "buff_prof_query": [vp, ctypes.c_char_p, u64p, f64p],
@@ -234,6 +235,10 @@ class DeviceBuffer:
 
     def fill(self, value=0):
         check(lib().buff_dev_fill(self.ctx.handle, self.ptr, value, self.nbytes))
+
+    def touch(self):
+        """Reads the buffer on the device (clean-line cache eviction)."""
+        check(lib().buff_dev_touch(self.ctx.handle, self.ptr, self.nbytes))
 
     def free(self):
         if self.ptr:

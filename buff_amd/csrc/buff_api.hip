@@ -785,6 +785,15 @@ int buff_dev_fill(buff_ctx* ctx, void* dptr, uint8_t value, uint64_t bytes) {
   return BUFF_OK;
 }
 
+int buff_dev_touch(buff_ctx* ctx, const void* dptr, uint64_t bytes) {
+  if (!ctx || (bytes && !dptr)) return fail(BUFF_E_INVALID, "null argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  BUFF_TRY(c->scratch(c->s_sink, 256));
+  ProfScope ps(c, "k_touch_bytes");
+  HIP_TRY(launch_touch_bytes(dptr, bytes, (uint32_t*)c->s_sink.ptr, c->stream));
+  return BUFF_OK;
+}
+
 int buff_prof_enable(buff_ctx* ctx, int on) {
   if (!ctx) return fail(BUFF_E_INVALID, "null argument");
   Ctx* c = reinterpret_cast<Ctx*>(ctx);

@@ -1,18 +1,20 @@
 // Shared host/device definitions of the resident BUFF layout.
 //
 // Resident layout of one chunk (n rows, F = P*8 + r bits per row), designed for
-// wave64 lanes and ballots:
-//   * Rows are grouped in tiles of 1024. Inside a tile, row 64*g + t (g = 0..15,
-//     t = 0..63) belongs to lane t, byte g of that lane's 16 bytes.
+// wave64 lanes:
+//   * Rows are grouped in tiles of 1024; lane t of a wave owns the 16
+//     consecutive rows 16t .. 16t+15 of a tile.
 //   * Byte sub-column ("plane") p is `pitch` bytes long; tile c of plane p is the
 //     1024 bytes at planes + p*pitch + 1024*c, lane t's 16 bytes at offset 16*t.
-//     A dwordx4 load per lane reads a whole tile of a plane coalesced, and
-//     `__ballot(byte g of each lane ...)` yields the 64-row bitmap word of rows
-//     64*g .. 64*g+63 of the tile directly in BitVec order.
-//   * The r tail bits are kept as per-lane bit-planes: tile c owns r*64 u16
-//     words at tail + 64*r*c; word j*64 + t holds bit j of lane t's 16 rows
-//     (bit g <-> row 64*g + t). A wave reads or writes one bit-plane of a tile
-//     with one coalesced 128-byte access.
+//     Inside them, dword i byte k holds row 16t + 4k + i, so the byte-wise
+//     compare bits of a dword land on rows i, i+4, i+8, i+12 and four dwords
+//     gather into one 16-bit row mask with shifts (see k_filter).
+//     A dwordx4 load per lane reads a whole tile of a plane coalesced.
+//   * The r tail bits are kept as bit-planes: tile c owns r*64 u16 words at
+//     tail + 64*r*c; word j*64 + t holds bit j of rows 16t .. 16t+15 (bit b <->
+//     row 16t + b). Bit-plane j of a tile is therefore a 1024-row bitmap in
+//     BitVec order, and a filter's 16-bit row mask of lane t is word t of the
+//     tile's result bitmap.
 //   * Rows >= n are zero in every plane and bit-plane.
 // The reference byte stream (split_double.rs:349-414) is produced from and parsed
 // into this form by the serialize/load kernels.
@@ -35,7 +37,7 @@ constexpr uint32_t kStreamHeader = 20;
 // Device view of a resident chunk.
 struct ChunkView {
   uint8_t* planes;    // plane p at planes + p * pitch
-  uint16_t* tail;     // per-lane bit-planes, 64*r u16 per tile
+  uint16_t* tail;     // bit-planes, 64*r u16 per tile
   uint64_t pitch;     // bytes per plane = ntiles * 1024
   uint64_t n;         // rows
   uint64_t ntiles;    // ceil(n / 1024)
@@ -45,14 +47,6 @@ struct ChunkView {
   uint32_t r;         // tail bits (F >= 8 ? F%8 : F)
   uint32_t dlen;
 };
-
-// Number of valid rows in 64-row group g of tile c, as a 64-bit mask.
-BUFF_HD uint64_t group_valid_mask(uint64_t n, uint64_t tile, uint32_t g) {
-  uint64_t r0 = tile * kTileRows + (uint64_t)g * 64;
-  if (r0 + 64 <= n) return ~0ull;
-  if (r0 >= n) return 0ull;
-  return (1ull << (n - r0)) - 1ull;
-}
 
 // database/src/methods/prec_double.rs:190-205 (fetch_fixed_aligned), with Rust
 // release semantics: the shift amount is masked to 6 bits, negation wraps

@@ -35,6 +35,7 @@ hipError_t launch_filter(const ChunkView& cv, int op, const FilterTarget& tg, ui
                          unsigned long long* stats, hipStream_t s, const LaunchShape& ls);
 hipError_t launch_fill_bitmap(uint64_t* out, uint64_t nwords, uint64_t n, int ones, hipStream_t s);
 hipError_t launch_fill_bytes(void* out, uint64_t bytes, uint8_t value, hipStream_t s);
+hipError_t launch_touch_bytes(const void* in, uint64_t bytes, uint32_t* sink, hipStream_t s);
 // sums: 9 u64 results followed by sum_scratch_bytes(ls) of per-workgroup rows.
 uint64_t sum_scratch_bytes(const LaunchShape& ls);
 hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls);

@@ -87,9 +87,11 @@ int buff_dev_alloc(buff_ctx* ctx, uint64_t bytes, void** dptr);
 int buff_dev_free(buff_ctx* ctx, void* dptr);
 int buff_memcpy_h2d(buff_ctx* ctx, void* dst_dev, const void* src_host, uint64_t bytes);
 int buff_memcpy_d2h(buff_ctx* ctx, void* dst_host, const void* src_dev, uint64_t bytes);
-/* Fills device bytes with `value` on the context's stream (no host sync). Used
- * to evict the memory-side cache before cold-cache measurements. */
+/* Fills device bytes with `value` on the context's stream (no host sync). */
 int buff_dev_fill(buff_ctx* ctx, void* dptr, uint8_t value, uint64_t bytes);
+/* Reads device bytes and discards them (no host sync): refills the memory-side
+ * cache with clean lines before a cold-cache measurement. */
+int buff_dev_touch(buff_ctx* ctx, const void* dptr, uint64_t bytes);
 
 /* Per-kernel timing with HIP events recorded on the context's stream.
  * buff_prof_query reports launches and summed durations for one kernel name. */
