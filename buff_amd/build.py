@@ -9,6 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbuffgpu.so")
+CLI = os.path.join(HERE, "comp_profiler")
 SOURCES = ["buff_kernels.hip", "buff_api.hip"]
 ARCH = os.environ.get("BUFF_OFFLOAD_ARCH", "gfx950")
 
@@ -43,6 +44,13 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    # the reference-shaped CLI on top of the C ABI (host C++ only)
+    cli = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "host", "comp_profiler.cpp"), "-o", CLI, "-L", HERE, "-lbuffgpu",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cli), flush=True)
+    subprocess.run(cli, check=True)
     return LIB
 
 

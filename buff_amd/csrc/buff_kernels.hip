@@ -550,10 +550,16 @@ __global__ __launch_bounds__(256) void k_sum_final(const unsigned long long* __r
                                                    uint32_t width, unsigned long long* __restrict__ out) {
   __shared__ unsigned long long part[256 / 64][kMaxPlanes + 1];
   const uint32_t lane = __lane_id(), wslot = threadIdx.x >> 6;
-  for (uint32_t k = 0; k < width; ++k) {
-    uint64_t v = 0;
-    for (uint32_t i = threadIdx.x; i < nrows; i += blockDim.x) v += rows[(uint64_t)i * (kMaxPlanes + 1) + k];
-    v = wave_sum_u64(v);
+  uint64_t acc[kMaxPlanes + 1];
+#pragma unroll
+  for (uint32_t k = 0; k <= kMaxPlanes; ++k) acc[k] = 0;
+  for (uint32_t i = threadIdx.x; i < nrows; i += blockDim.x) {
+#pragma unroll
+    for (uint32_t k = 0; k <= kMaxPlanes; ++k) acc[k] += rows[(uint64_t)i * (kMaxPlanes + 1) + k];
+  }
+#pragma unroll
+  for (uint32_t k = 0; k <= kMaxPlanes; ++k) {
+    const uint64_t v = wave_sum_u64(acc[k]);
     if (lane == 0) part[wslot][k] = v;
   }
   __syncthreads();
@@ -607,6 +613,17 @@ __global__ __launch_bounds__(256) void k_max_top(ChunkView cv, uint8_t* __restri
   }
 }
 
+// Folds the per-workgroup top bytes of pass 1 into top[ntop] (one workgroup).
+__global__ __launch_bounds__(256) void k_max_top_final(unsigned int* __restrict__ top, uint32_t ntop) {
+  __shared__ uint32_t wtop[4];
+  uint32_t tl = 0;
+  for (uint32_t i = threadIdx.x; i < ntop; i += blockDim.x) tl = max(tl, top[i]);
+  tl = wave_max_u32(tl);
+  if (__lane_id() == 0) wtop[threadIdx.x >> 6] = tl;
+  __syncthreads();
+  if (threadIdx.x == 0) top[ntop] = max(max(wtop[0], wtop[1]), max(wtop[2], wtop[3]));
+}
+
 __global__ __launch_bounds__(256) void k_max_refine(ChunkView cv, const uint8_t* __restrict__ tile_max,
                                                     const unsigned int* __restrict__ top, uint32_t ntop,
                                                     unsigned long long* __restrict__ best_out) {
@@ -614,16 +631,7 @@ __global__ __launch_bounds__(256) void k_max_refine(ChunkView cv, const uint8_t*
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const bool has_planes = cv.P > 0;
-  // fold the per-workgroup top bytes of pass 1
-  __shared__ uint32_t wtop[16];
-  uint32_t tl = 0;
-  if (has_planes)
-    for (uint32_t i = threadIdx.x; i < ntop; i += blockDim.x) tl = max(tl, top[i]);
-  tl = wave_max_u32(tl);
-  if (lane == 0) wtop[threadIdx.x >> 6] = tl;
-  __syncthreads();
-  uint32_t t = 0;
-  for (uint32_t k = 0; k < (blockDim.x >> 6); ++k) t = max(t, wtop[k]);
+  const uint32_t t = has_planes ? top[ntop] : 0u;
   uint64_t best = 0;
   bool have = false;
   // 64 tiles per probe: one byte per lane
@@ -933,7 +941,7 @@ hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t
   }
 }
 
-uint64_t max_scratch_bytes(uint64_t ntiles) { return ((ntiles + 255) / 256) * 256 + 4 * 8192; }
+uint64_t max_scratch_bytes(uint64_t ntiles) { return ((ntiles + 255) / 256) * 256 + 4 * 8192 + 256; }
 
 // scratch: per-tile maxima (ntiles bytes, padded), then one top byte per
 // pass-1 workgroup; the caller zeroes `best`.
@@ -945,6 +953,7 @@ hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* sc
     constexpr int U = 8;
     wg1 = grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, std::min<uint32_t>(ls.max_wg, 8192));
     hipLaunchKernelGGL((k_max_top<U>), dim3(wg1), dim3(64 * ls.waves_per_wg), 0, s, cv, scratch, top);
+    hipLaunchKernelGGL(k_max_top_final, dim3(1), dim3(256), 0, s, top, wg1);
   }
   const uint32_t wg2 = grid_for((cv.ntiles + 63) / 64, 4, 1024);
   hipLaunchKernelGGL(k_max_refine, dim3(wg2), dim3(256), 0, s, cv, (const uint8_t*)scratch, (const unsigned int*)top,
