@@ -35,6 +35,28 @@ HBM_PEAK_GBPS = 8000.0
 METRIC = "range-filter/sum/encode GB/s vs HBM roofline; batched-query p50/p99 latency"
 
 
+TRAFFIC_FILE = "profiles/pmc_traffic_bench_268M.json"
+TRAFFIC_ROWS = 1 << 27
+# kernel family of the bench's profiler names -> prefix of the rocprofv3 names
+TRAFFIC_FAMILY = {"k_materialize": "void k_materialize<", "k_sum": "void k_sum<", "k_filter_gt": "void k_filter<0",
+                  "k_encode": "k_encode", "k_minmax": "k_minmax", "k_max": "void k_max_top<"}
+
+
+def pmc_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, calibrated), measured by rocprofv3 --pmc on
+    this same workload and committed beside the kernel-trace summary; None for other row counts."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    if n != TRAFFIC_ROWS or kernel not in TRAFFIC_FAMILY or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)["kernels"]
+    pre = TRAFFIC_FAMILY[kernel]
+    shapes = [v for k, v in d.items() if k == pre or (pre.endswith("<") or pre.endswith("<0")) and k.startswith(pre)]
+    if not shapes:
+        return None
+    return round(1024 * sum(v["read_kib"] + v["write_kib"] for v in shapes) / len(shapes))
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -274,7 +296,8 @@ def main():
                                "range filters at 0.1/10/50%%, sum, max, materialize" % n,
                    "rows_per_column": n, "columns": 2, "ops_per_step": ops, "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": kernels[dominant]["achieved_GBps"],
-                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": kernels[dominant]["frac"], "traffic": None},
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": kernels[dominant]["frac"],
+                     "traffic": pmc_traffic(dominant, n), "traffic_source": TRAFFIC_FILE},
         "kernels": kernels,
         "cold_cache": cold,
         "cold_filters": filters,

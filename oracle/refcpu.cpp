@@ -26,6 +26,8 @@
 #include <string>
 #include <vector>
 
+#include "snappy_ref.h"
+
 namespace {
 
 constexpr int REF_OK = 0;
@@ -224,6 +226,10 @@ struct BitPackReader {  // BitPack<&[u8]>
     return out;
   }
   void skip_n_byte(size_t n) { cursor += n; }                            // 262-266
+  uint32_t peek8() const {   // what read(8) would return, reader unchanged
+    BitPackReader c = *this;
+    return c.read(8);
+  }
   void skip(size_t nb) {                                                 // 268-283 (Err ignored by callers)
     if (len * BYTE_BITS < sum_bits() + nb) return;
     size_t bytes = nb / BYTE_BITS, left = nb % BYTE_BITS;
@@ -252,8 +258,41 @@ struct BitVec {
     size_t extra = nbits % 32;
     if (extra && !storage.empty()) storage.back() &= (1u << extra) - 1;
   }
-  void and_with(const BitVec& o) { for (size_t i = 0; i < storage.size(); ++i) storage[i] &= o.storage[i]; }
-  void or_with(const BitVec& o) { for (size_t i = 0; i < storage.size(); ++i) storage[i] |= o.storage[i]; }
+  // lib.rs:466-480 `process`: assert_eq! on the bit lengths, then a zip over the blocks.
+  size_t process(const BitVec& o) const {
+    if (o.nbits != nbits) throw RefPanic("BitVec length mismatch");
+    return storage.size() < o.storage.size() ? storage.size() : o.storage.size();
+  }
+  void and_with(const BitVec& o) { size_t m = process(o); for (size_t i = 0; i < m; ++i) storage[i] &= o.storage[i]; }
+  void or_with(const BitVec& o) { size_t m = process(o); for (size_t i = 0; i < m; ++i) storage[i] |= o.storage[i]; }
+  void negate() {                                                        // lib.rs:696-700
+    for (uint32_t& w : storage) w = ~w;
+    fix_last_block();
+  }
+  void clear() { for (uint32_t& w : storage) w = 0; }                    // lib.rs:1491-1494
+  std::vector<uint8_t> to_binary() const {                               // lib.rs:399-407
+    std::vector<uint8_t> b(8 + 4 * storage.size());
+    uint64_t nb = nbits;
+    std::memcpy(b.data(), &nb, 8);
+    if (!storage.empty()) std::memcpy(b.data() + 8, storage.data(), 4 * storage.size());
+    return b;
+  }
+  static BitVec from_binary(const std::vector<uint8_t>& data) {          // lib.rs:411-428
+    if (data.size() < 8) throw RefPanic("from_binary: short input");
+    uint64_t nb;
+    std::memcpy(&nb, data.data(), 8);
+    BitVec bv;
+    bv.nbits = (size_t)nb;
+    size_t cnt = 8, len = data.size() - 4;
+    while (cnt <= len) {
+      uint32_t w;
+      std::memcpy(&w, data.data() + cnt, 4);
+      bv.storage.push_back(w);
+      cnt += 4;
+    }
+    bv.fix_last_block();
+    return bv;
+  }
 };
 
 // database/src/query/bit_vec_iter.rs:35-47 (BVIter: set bits in ascending order)
@@ -1112,6 +1151,699 @@ int copy_filter(const FilterOut& fo, uint32_t* words, uint64_t nwords_cap, uint6
   return REF_OK;
 }
 
+
+// ===========================================================================
+// buff-major: majority / outlier sparse columns (buff_simd.rs:317-2861).
+
+constexpr size_t SAMPLE = 2000;          // split_double.rs:25
+constexpr float MAJOR_R = 0.6f;          // split_double.rs:27
+
+std::vector<uint8_t> snappy_compress(const std::vector<uint8_t>& in) {   // bit_vec_iter.rs:61-64
+  return snappy_ref::compress(in.data(), in.size());
+}
+std::vector<uint8_t> snappy_decompress(const uint8_t* p, size_t n) {     // bit_vec_iter.rs:69-71 (unwrap)
+  try {
+    return snappy_ref::decompress(p, n);
+  } catch (const snappy_ref::Error& e) {
+    throw RefPanic(e.what());
+  }
+}
+
+// The Boyer-Moore vote of buff_simd.rs:402-412 / 424-436 over `(sample >> rsf) as u8`.
+void majority_vote(const std::vector<uint64_t>& head, size_t rsf, uint8_t* major, int32_t* frequency) {
+  int32_t f = 0;
+  uint8_t m = 0;
+  for (uint64_t smp : head) {
+    uint8_t u = (uint8_t)(smp >> (rsf & 63));
+    if (f == 0) { m = u; f = 1; }
+    else if (m == u) f += 1;
+    else f -= 1;
+  }
+  *major = m;
+  *frequency = f;
+}
+
+// buff_simd.rs:317-630 (buff_encode_majority).
+std::vector<uint8_t> buff_encode_majority(const double* data, size_t n, uint64_t scale, int64_t* out_min,
+                                          int64_t* out_max) {
+  std::vector<int64_t> fixed_vec;
+  fixed_vec.reserve(n);
+  uint32_t t = (uint32_t)n;
+  int32_t prec = scale == 0 ? 0 : prec_of_scale(scale);
+  double prec_delta = get_precision_bound(prec);
+  int32_t pexp = precision_exp_of(prec_delta);
+  uint64_t dec_len = (uint64_t)precision_map(prec);
+  int64_t mn = INT64_MAX, mx = INT64_MIN;
+  for (size_t i = 0; i < n; ++i) {          // 340-349: `if` / `if`
+    int64_t f = fetch_fixed_aligned(data[i], pexp, dec_len);
+    if (f < mn) mn = f;
+    if (f > mx) mx = f;
+    fixed_vec.push_back(f);
+  }
+  if (out_min) *out_min = mn;
+  if (out_max) *out_max = mx;
+  int64_t delta = (int64_t)((uint64_t)mx - (uint64_t)mn);
+  int64_t base_fixed = mn;
+  uint64_t ubase = (uint64_t)base_fixed;
+  double cal_int_length = 0.0;
+  if (delta != 0) cal_int_length = std::ceil(std::log2((double)delta));
+  size_t fixed_len = (size_t)f64_as_usize(cal_int_length);
+  size_t ilen = fixed_len - (size_t)dec_len;
+  size_t dlen = (size_t)dec_len;
+
+  BitPackWriter bp;
+  bp.write((uint32_t)ubase, 32);
+  bp.write((uint32_t)(ubase >> 32), 32);
+  bp.write(t, 32);
+  bp.write((uint32_t)ilen, 32);
+  bp.write((uint32_t)dlen, 32);
+
+  size_t remain = fixed_len;
+  std::vector<uint8_t> freq_vec;
+  size_t rsf = fixed_len >= 8 ? fixed_len - 8 : 0;
+  int32_t m_limit = (int32_t)((float)SAMPLE * MAJOR_R);
+  if (n < SAMPLE) throw RefPanic("fixed_vec[0..SAMPLE] out of range");     // 397
+  std::vector<uint64_t> head(SAMPLE);
+  for (size_t i = 0; i < SAMPLE; ++i) head[i] = (uint64_t)fixed_vec[i] - (uint64_t)base_fixed;
+  uint8_t major;
+  int32_t frequency;
+  majority_vote(head, rsf, &major, &frequency);
+  if (frequency > m_limit) {
+    freq_vec.push_back(major);
+    while (rsf > 7) {
+      rsf -= 8;
+      majority_vote(head, rsf, &major, &frequency);
+      if (frequency > m_limit) freq_vec.push_back(major);
+      else break;
+    }
+  }
+  uint32_t cols_outlier = (uint32_t)freq_vec.size();
+  if (cols_outlier > 0) {
+    // With fixed_len < 8 `remain -= 8` wraps: a debug build panics, a release
+    // build's `while remain >= 8` then writes ~2^61 planes.
+    if (remain < 8) throw RefPanic("remain -= 8 underflows (F < 8 with a majority byte)");
+    bp.write(cols_outlier, 8);
+    std::vector<uint64_t> fixed_u64(n);
+    for (size_t i = 0; i < n; ++i) fixed_u64[i] = (uint64_t)fixed_vec[i] - (uint64_t)base_fixed;
+    for (uint32_t col = 0; col < cols_outlier; ++col) {
+      uint8_t f_item = freq_vec[col];
+      remain -= 8;
+      BitVec bm_outlier(n);
+      std::vector<uint8_t> outlier;
+      for (size_t i = 0; i < n; ++i) {
+        uint8_t cur = (uint8_t)(fixed_u64[i] >> (remain & 63));
+        if (cur != f_item) {
+          outlier.push_back(cur);
+          bm_outlier.set(i);
+        }
+      }
+      uint32_t n_outlier = bm_outlier.cardinality();
+      std::vector<uint8_t> bm_vec = snappy_compress(bm_outlier.to_binary());
+      bp.write(f_item, 8);
+      bp.write((uint32_t)bm_vec.size(), 32);
+      bp.buff.insert(bp.buff.end(), bm_vec.begin(), bm_vec.end());   // write_bytes: append, cursor kept
+      bp.finish_write_byte();
+      bp.write(n_outlier, 32);
+      bp.buff.insert(bp.buff.end(), outlier.begin(), outlier.end());
+      bp.finish_write_byte();
+    }
+    while (remain >= 8) {
+      remain -= 8;
+      for (size_t i = 0; i < n; ++i)
+        bp.write_byte(remain > 0 ? (uint8_t)(fixed_u64[i] >> (remain & 63)) : (uint8_t)fixed_u64[i]);
+    }
+    if (remain > 0) {
+      bp.finish_write_byte();
+      for (size_t i = 0; i < n; ++i) bp.write_bits((uint32_t)fixed_u64[i], remain);
+    }
+  } else {
+    bp.write_byte(0);
+    if (remain < 8) {
+      // write_bits starts at the cursor, which still points at the byte just pushed.
+      for (size_t i = 0; i < n; ++i) bp.write_bits((uint32_t)((uint64_t)fixed_vec[i] - (uint64_t)base_fixed), remain);
+    } else {
+      remain -= 8;
+      std::vector<uint64_t> fixed_u64(n);
+      for (size_t i = 0; i < n; ++i) {
+        uint64_t cur = (uint64_t)fixed_vec[i] - (uint64_t)base_fixed;
+        bp.write_byte(remain > 0 ? (uint8_t)(cur >> (remain & 63)) : (uint8_t)cur);
+        fixed_u64[i] = cur;
+      }
+      while (remain >= 8) {
+        remain -= 8;
+        for (size_t i = 0; i < n; ++i)
+          bp.write_byte(remain > 0 ? (uint8_t)(fixed_u64[i] >> (remain & 63)) : (uint8_t)fixed_u64[i]);
+      }
+      if (remain > 0) {
+        bp.finish_write_byte();
+        for (size_t i = 0; i < n; ++i) bp.write_bits((uint32_t)fixed_u64[i], remain);
+      }
+    }
+  }
+  return bp.buff;
+}
+
+// One sparse column as the readers walk it (e.g. buff_simd.rs:672-681).
+struct SparseCol {
+  uint8_t major;
+  BitVec bitmap;
+  uint32_t n_outlier;
+  const uint8_t* outliers;
+};
+SparseCol read_sparse_col(BitPackReader& bp, bool skip_outliers) {
+  SparseCol c;
+  c.major = (uint8_t)bp.read(8);
+  uint32_t bm_len = bp.read(32);
+  const uint8_t* bm_mat = bp.read_n_byte(bm_len);
+  c.bitmap = BitVec::from_binary(snappy_decompress(bm_mat, bm_len));
+  bp.finish_read_byte();
+  c.n_outlier = bp.read(32);
+  if (skip_outliers) {
+    bp.skip_n_byte(c.n_outlier);
+    c.outliers = nullptr;
+  } else {
+    c.outliers = bp.read_n_byte(c.n_outlier);
+  }
+  bp.finish_read_byte();
+  return c;
+}
+
+// A majority stream whose bit width is below 8: only the no-majority encoder
+// branch writes one, and it ORs the tail bits into the outlier-column count
+// byte (buff_simd.rs:550-555), so the readers misparse it.
+void reject_narrow(uint32_t remain) {
+  if (remain < 8) throw RefPanic("buff-major stream with F < 8 cannot be read back");
+}
+
+// buff_simd.rs:633-861 (buff_decode_majority).
+std::vector<double> buff_decode_majority(const uint8_t* bytes, size_t blen, uint64_t scale) {
+  (void)scale;
+  BitPackReader bp(bytes, blen);
+  Header h = read_header(bp);
+  std::vector<double> out;
+  std::vector<uint64_t> fixed_vec;
+  double dec_scl = powi(2.0, (int32_t)h.dlen);
+  uint32_t remain = h.dlen + h.ilen;
+  reject_narrow(remain);
+  auto value = [&](uint64_t f) { return (double)(int64_t)((uint64_t)h.base + f) / dec_scl; };
+  uint32_t outlier_cols = bp.read(8);
+  if (outlier_cols > 0) {
+    for (uint32_t col = 0; col < outlier_cols; ++col) {
+      SparseCol c = read_sparse_col(bp, false);
+      remain -= 8;
+      if (col == 0) fixed_vec.assign(h.len, (uint64_t)c.major << (remain & 63));
+      uint64_t major_u64 = (uint64_t)c.major << (remain & 63);
+      if (c.n_outlier == 0) throw RefPanic("outlier_iter.next().unwrap() on no outliers");
+      size_t oi = 0;
+      uint8_t outlier_value = c.outliers[oi++];
+      BVIter bm(c.bitmap);
+      size_t ind;
+      if (!bm.next(&ind)) throw RefPanic("bm_iterator.next().unwrap()");
+      size_t i = 0, counter = 0;
+      for (uint64_t& x : fixed_vec) {
+        if (i == ind) {
+          if (col == 0) x = (uint64_t)outlier_value << (remain & 63);
+          else x |= (uint64_t)outlier_value << (remain & 63);
+          counter += 1;
+          if (counter == c.n_outlier) break;
+          if (!bm.next(&ind)) throw RefPanic("bm_iterator.next().unwrap()");
+          outlier_value = c.outliers[oi++];
+        } else if (col > 0) {
+          x |= major_u64 << (remain & 63);   // 733: the major byte is shifted twice
+        }
+        i += 1;
+      }
+    }
+  } else if (remain >= 8) {
+    remain -= 8;
+    const uint8_t* chunk = bp.read_n_byte(h.len);
+    if (remain == 0) {
+      for (uint32_t i = 0; i < h.len; ++i) out.push_back(value(chunk[i]));
+    } else {
+      for (uint32_t i = 0; i < h.len; ++i) fixed_vec.push_back((uint64_t)chunk[i] << (remain & 63));
+    }
+  }
+  while (remain >= 8) {
+    remain -= 8;
+    const uint8_t* chunk = bp.read_n_byte(h.len);
+    if (remain == 0) {
+      size_t m = std::min(fixed_vec.size(), (size_t)h.len);
+      for (size_t i = 0; i < m; ++i) out.push_back(value(fixed_vec[i] | (uint64_t)chunk[i]));
+    } else {
+      for (size_t i = 0; i < fixed_vec.size(); ++i) {
+        if (i >= h.len) throw RefPanic("chunk iterator exhausted");
+        fixed_vec[i] |= (uint64_t)chunk[i] << (remain & 63);
+      }
+    }
+  }
+  if (remain > 0) {
+    bp.finish_read_byte();
+    for (size_t i = 0; i < fixed_vec.size(); ++i) out.push_back(value(fixed_vec[i] | (uint64_t)bp.read_bits(remain)));
+  }
+  return out;
+}
+
+// The candidate walk over one dense plane shared by the majority filters
+// (buff_simd.rs:990-1052): reads the plane bytes of the rb1 rows through the
+// reader, returns false (the caller breaks) when rb1 is empty.
+template <typename Visit>
+bool walk_plane(BitPackReader& bp, const BitVec& rb1, size_t len, Visit visit) {
+  if (rb1.cardinality() == 0) {
+    bp.skip_n_byte(len);
+    return false;
+  }
+  BVIter it(rb1);
+  size_t idx, dec_pre = 0;
+  bool first = true;
+  while (it.next(&idx)) {
+    if (first) {
+      if (idx != 0) bp.skip_n_byte(idx);
+      first = false;
+    } else {
+      size_t delta = idx - dec_pre;
+      if (delta != 1) bp.skip_n_byte(delta - 1);
+    }
+    visit(idx, bp.read_byte());
+    dec_pre = idx;
+  }
+  if (len - dec_pre > 1) bp.skip_n_byte(len - dec_pre - 1);
+  return true;
+}
+// The tail walk (buff_simd.rs:1092-1130): `remain`-bit values of the rb1 rows.
+template <typename Visit>
+bool walk_tail(BitPackReader& bp, const BitVec& rb1, uint32_t remain, Visit visit) {
+  bp.finish_read_byte();
+  if (rb1.cardinality() == 0) return false;
+  BVIter it(rb1);
+  size_t idx, dec_pre = 0;
+  bool first = true;
+  while (it.next(&idx)) {
+    if (first) {
+      if (idx != 0) bp.skip(idx * remain);
+      first = false;
+    } else {
+      size_t delta = idx - dec_pre;
+      if (delta != 1) bp.skip((delta - 1) * remain);
+    }
+    visit(idx, bp.read_bits(remain));
+    dec_pre = idx;
+  }
+  return true;
+}
+
+// buff_simd.rs:863-1277 (buff_range_filter_majority): x > pred.
+FilterOut buff_range_filter_majority(const uint8_t* bytes, size_t blen, uint64_t scale, double pred) {
+  BitPackReader bp(bytes, blen);
+  Header h = read_header(bp);
+  size_t len = h.len;
+  uint32_t remain = h.dlen + h.ilen;
+  Bound bound = bound_for(scale, h.dlen);
+  BitVec rb1(len), res(len);
+  FilterOut fo;
+  int64_t fixed_part = bound.fixed(pred);
+  if (fixed_part < h.base) {              // 886-889: prints len, no bitmap
+    fo.res = BitVec(len);
+    for (size_t i = 0; i < len; ++i) fo.res.set(i);
+    fo.count = len;
+    return fo;
+  }
+  reject_narrow(remain);
+  uint64_t fixed_target = (uint64_t)fixed_part - (uint64_t)h.base;
+  uint32_t outlier_cols = bp.read(8);
+  if (outlier_cols > 0) {
+    for (uint32_t col = 0; col < outlier_cols; ++col) {
+      remain -= 8;
+      uint8_t cur_tar = (uint8_t)(fixed_target >> (remain & 63));
+      uint8_t major = (uint8_t)bp.peek8();
+      if (cur_tar == 0 && major == 0) {
+        SparseCol c = read_sparse_col(bp, true);
+        if (col == 0) {                    // 903-915
+          res.or_with(c.bitmap);
+          c.bitmap.negate();
+          rb1.or_with(c.bitmap);
+        } else {                           // 971-984: rb1 ends empty
+          rb1.and_with(c.bitmap);
+          res.or_with(rb1);
+          c.bitmap.negate();
+          rb1.and_with(c.bitmap);
+        }
+      } else {
+        SparseCol c = read_sparse_col(bp, false);
+        BVIter bm(c.bitmap);
+        BitVec cur_rb(len);
+        BitVec& eq_target = col == 0 ? rb1 : cur_rb;
+        for (uint32_t k = 0; k < c.n_outlier; ++k) {
+          size_t ind;
+          if (!bm.next(&ind)) throw RefPanic("bm_iterator.next().unwrap()");
+          uint8_t val = c.outliers[k];
+          if (val == cur_tar) eq_target.set(ind);
+          else if (val > cur_tar) res.set(ind);   // rows after column 0: not masked by rb1
+        }
+        c.bitmap.negate();
+        if (col == 0) {                    // 941-946
+          if (cur_tar < major) res.or_with(c.bitmap);
+          else if (cur_tar == major) rb1.or_with(c.bitmap);
+        } else {                           // 1011-1019
+          if (cur_tar < major) {
+            c.bitmap.and_with(rb1);
+            res.or_with(c.bitmap);
+          } else if (cur_tar == major) {
+            cur_rb.or_with(c.bitmap);
+          }
+          rb1.and_with(cur_rb);
+        }
+      }
+    }
+  } else if (remain < 8) {
+    for (size_t i = 0; i < len; ++i) {
+      uint8_t cur = bp.read_bits(remain);
+      if ((uint64_t)cur > fixed_target) res.set(i);
+    }
+    remain = 0;
+  } else {
+    remain -= 8;
+    const uint8_t* chunk = bp.read_n_byte(len);
+    uint8_t cur_tar = (uint8_t)(fixed_target >> (remain & 63));
+    for (size_t i = 0; i < len; ++i) {
+      if (chunk[i] > cur_tar) res.set(i);
+      else if (chunk[i] == cur_tar) rb1.set(i);
+    }
+  }
+  while (remain > 0) {
+    if (remain >= 8) {
+      remain -= 8;
+      uint8_t cur_tar = (uint8_t)(fixed_target >> (remain & 63));
+      BitVec cur_rb(len);
+      if (!walk_plane(bp, rb1, len, [&](size_t i, uint8_t dec) {
+            if (dec > cur_tar) res.set(i);
+            else if (dec == cur_tar) cur_rb.set(i);
+          }))
+        break;
+      rb1 = std::move(cur_rb);
+    } else {
+      uint8_t sh = (uint8_t)(8 - remain);
+      uint8_t cur_tar = (uint8_t)((uint8_t)((uint8_t)fixed_target << sh) >> sh);
+      if (!walk_tail(bp, rb1, remain, [&](size_t i, uint8_t dec) {
+            if (dec > cur_tar) res.set(i);
+          }))
+        break;
+      remain = 0;
+    }
+  }
+  fo.count = res.cardinality();
+  fo.res = std::move(res);
+  return fo;
+}
+
+// buff_simd.rs:1916-2292 (buff_equal_filter_majority).
+FilterOut buff_equal_filter_majority(const uint8_t* bytes, size_t blen, uint64_t scale, double pred) {
+  BitPackReader bp(bytes, blen);
+  Header h = read_header(bp);
+  size_t len = h.len;
+  uint32_t remain = h.dlen + h.ilen;
+  Bound bound = bound_for(scale, h.dlen);
+  BitVec rb1(len), res(len);
+  FilterOut fo;
+  int64_t fixed_part = bound.fixed(pred);
+  if (fixed_part < h.base) {              // 1939-1942
+    fo.res = BitVec(len);
+    fo.count = 0;
+    return fo;
+  }
+  reject_narrow(remain);
+  uint64_t fixed_target = (uint64_t)fixed_part - (uint64_t)h.base;
+  uint32_t outlier_cols = bp.read(8);
+  if (outlier_cols > 0) {
+    for (uint32_t col = 0; col < outlier_cols; ++col) {
+      remain -= 8;
+      uint8_t dec_byte = (uint8_t)(fixed_target >> (remain & 63));
+      uint8_t major = (uint8_t)bp.peek8();
+      if (dec_byte == major) {
+        SparseCol c = read_sparse_col(bp, true);
+        c.bitmap.negate();
+        if (col == 0) rb1.or_with(c.bitmap);
+        else rb1.and_with(c.bitmap);
+      } else {
+        SparseCol c = read_sparse_col(bp, false);
+        BVIter bm(c.bitmap);
+        BitVec cur_rb(len);
+        BitVec& hit = col == 0 ? rb1 : cur_rb;
+        for (uint32_t k = 0; k < c.n_outlier; ++k) {
+          size_t ind;
+          if (!bm.next(&ind)) throw RefPanic("bm_iterator.next().unwrap()");
+          if (c.outliers[k] == dec_byte) hit.set(ind);
+        }
+        if (col > 0) rb1.and_with(cur_rb);
+      }
+    }
+  } else if (remain < 8) {
+    for (size_t i = 0; i < len; ++i) {
+      uint8_t cur = bp.read_bits(remain);
+      if ((uint64_t)cur == fixed_target) res.set(i);
+    }
+    remain = 0;
+  } else {
+    remain -= 8;
+    const uint8_t* chunk = bp.read_n_byte(len);
+    uint8_t dec_byte = (uint8_t)(fixed_target >> (remain & 63));
+    for (size_t i = 0; i < len; ++i)
+      if (chunk[i] == dec_byte) rb1.set(i);
+  }
+  while (remain > 0) {
+    if (remain >= 8) {
+      remain -= 8;
+      uint8_t dec_byte = (uint8_t)(fixed_target >> (remain & 63));
+      BitVec cur_rb(len);
+      if (!walk_plane(bp, rb1, len, [&](size_t i, uint8_t dec) {
+            if (dec == dec_byte) cur_rb.set(i);
+          }))
+        break;
+      rb1 = std::move(cur_rb);
+    } else {
+      uint8_t sh = (uint8_t)(8 - remain);
+      uint8_t dec_byte = (uint8_t)((uint8_t)((uint8_t)fixed_target << sh) >> sh);
+      if (!walk_tail(bp, rb1, remain, [&](size_t i, uint8_t dec) {
+            if (dec == dec_byte) res.set(i);
+          }))
+        break;
+      remain = 0;
+    }
+  }
+  fo.count = res.cardinality();
+  fo.res = std::move(res);
+  return fo;
+}
+
+// buff_simd.rs:2294-2442 (buff_sum_majority). `col_sums` receives the exact
+// per-byte-column sums (sparse and dense columns in stream order, then the tail).
+double buff_sum_majority(const uint8_t* bytes, size_t blen, uint64_t scale, uint64_t* col_sums) {
+  (void)scale;
+  BitPackReader bp(bytes, blen);
+  Header h = read_header(bp);
+  uint32_t remain = h.ilen + h.dlen;
+  reject_narrow(remain);
+  int32_t processed = 0;
+  double sum_base = (double)h.len * (double)h.base;
+  double sum = 0.0;
+  double dec_scl = powi(2.0, (int32_t)h.dlen);
+  sum_base = sum_base / dec_scl;
+  int k = 0;
+  uint32_t outlier_cols = bp.read(8);
+  for (uint32_t col = 0; col < outlier_cols; ++col) {
+    uint8_t major = (uint8_t)bp.read(8);
+    remain -= 8;
+    processed += 8;
+    uint32_t bm_len = bp.read(32);
+    bp.read_n_byte(bm_len);
+    bp.finish_read_byte();
+    uint32_t n_outlier = bp.read(32);
+    const uint8_t* outliers = bp.read_n_byte(n_outlier);
+    dec_scl = powi(2.0, processed - (int32_t)h.ilen);
+    uint64_t sum_fixed = 0;
+    for (uint32_t j = 0; j < n_outlier; ++j) sum_fixed += outliers[j];
+    sum_fixed += (uint64_t)major * ((uint64_t)h.len - (uint64_t)n_outlier);
+    if (col_sums) col_sums[k++] = sum_fixed;
+    sum = sum + (double)sum_fixed / dec_scl;
+    bp.finish_read_byte();
+  }
+  while (remain >= 8) {
+    remain -= 8;
+    processed += 8;
+    dec_scl = powi(2.0, processed - (int32_t)h.ilen);
+    const uint8_t* chunk = bp.read_n_byte(h.len);
+    uint64_t sum_fixed = 0;
+    for (uint32_t i = 0; i < h.len; ++i) sum_fixed += chunk[i];
+    if (col_sums) col_sums[k++] = sum_fixed;
+    sum = sum + (double)sum_fixed / dec_scl;
+    if (remain == 0) return sum;           // 2370-2372 / 2412-2414: no base term
+  }
+  if (remain > 0) {
+    if (processed >= 8) bp.finish_read_byte();
+    processed += (int32_t)remain;
+    dec_scl = powi(2.0, processed - (int32_t)h.ilen);
+    uint64_t sum_fixed = 0;
+    for (uint32_t i = 0; i < h.len; ++i) sum_fixed += bp.read_bits(remain);
+    if (col_sums) col_sums[k++] = sum_fixed;
+    sum = sum + (double)sum_fixed / dec_scl;
+  }
+  sum += sum_base;
+  return sum;
+}
+
+// buff_simd.rs:2444-2861 (buff_max_majority). Returns the printed max value.
+double buff_max_majority(const uint8_t* bytes, size_t blen, uint64_t scale, uint64_t* max_off,
+                         uint64_t* argmax_count) {
+  (void)scale;
+  BitPackReader bp(bytes, blen);
+  Header h = read_header(bp);
+  size_t len = h.len;
+  uint32_t remain = h.dlen + h.ilen;
+  reject_narrow(remain);
+  BitVec rb1(len), res(len);
+  uint64_t max = 0;
+  uint32_t outlier_cols = bp.read(8);
+  if (outlier_cols > 0) {
+    for (uint32_t col = 0; col < outlier_cols; ++col) {
+      SparseCol c = read_sparse_col(bp, false);
+      remain -= 8;
+      uint8_t byte_max = c.major;
+      BVIter bm(c.bitmap);
+      if (col == 0) {                      // 2481-2511
+        rb1 = c.bitmap;
+        rb1.negate();
+        for (uint32_t k = 0; k < c.n_outlier; ++k) {
+          size_t ind;
+          if (!bm.next(&ind)) throw RefPanic("bm_iterator.next().unwrap()");
+          uint8_t val = c.outliers[k];
+          if (val > byte_max) {
+            byte_max = val;
+            rb1.clear();
+            rb1.set(ind);
+          } else if (val == byte_max) {
+            rb1.set(ind);
+          }
+        }
+        max = max | ((uint64_t)byte_max << (remain & 63));
+        continue;
+      }
+      BitVec cur_rb = c.bitmap;              // 2522-2532
+      cur_rb.negate();
+      cur_rb.and_with(rb1);
+      rb1.and_with(c.bitmap);
+      if (rb1.cardinality() == 0) {          // 2534-2538: no candidate is an outlier
+        rb1 = std::move(cur_rb);
+        continue;
+      }
+      BVIter mi(rb1);
+      size_t max_cur = 0;
+      mi.next(&max_cur);
+      for (uint32_t k = 0; k < c.n_outlier; ++k) {
+        size_t ind;
+        if (!bm.next(&ind)) throw RefPanic("bm_iterator.next().unwrap()");
+        if (ind == max_cur) {
+          uint8_t val = c.outliers[k];
+          if (val > byte_max) {
+            byte_max = val;
+            cur_rb.clear();
+            cur_rb.set(ind);
+          } else if (val == byte_max) {
+            cur_rb.set(ind);
+          }
+          if (!mi.next(&max_cur)) break;
+        }
+      }
+      rb1 = std::move(cur_rb);
+      max = max | ((uint64_t)byte_max << (remain & 63));
+    }
+  } else if (remain < 8) {
+    for (uint32_t i = 0; i < len; ++i) {
+      uint64_t cur = bp.read_bits(remain);
+      if (cur > max) {
+        max = cur;
+        res.clear();
+        res.set(i);
+      } else if (cur == max) {
+        res.set(i);
+      }
+    }
+    remain = 0;
+  } else {
+    remain -= 8;
+    const uint8_t* chunk = bp.read_n_byte(len);
+    uint8_t byte_max = 0;
+    for (size_t i = 0; i < len; ++i) {
+      if (chunk[i] > byte_max) {
+        byte_max = chunk[i];
+        rb1.clear();
+        rb1.set(i);
+      } else if (chunk[i] == byte_max) {
+        rb1.set(i);
+      }
+    }
+    max = (uint64_t)byte_max << (remain & 63);
+  }
+  while (remain > 0) {
+    uint8_t byte_max = 0;
+    if (remain >= 8) {
+      remain -= 8;
+      BitVec cur_rb(len);
+      // 2597-2650: an empty rb1 still skips len - 1 bytes
+      BVIter it(rb1);
+      size_t idx, dec_pre = 0;
+      bool first = true;
+      while (it.next(&idx)) {
+        if (first) {
+          if (idx != 0) bp.skip_n_byte(idx);
+        } else {
+          size_t delta = idx - dec_pre;
+          if (delta != 1) bp.skip_n_byte(delta - 1);
+        }
+        uint8_t dec = bp.read_byte();
+        if (dec > byte_max) {
+          byte_max = dec;
+          if (!first) cur_rb.clear();
+          cur_rb.set(idx);
+        } else if (dec == byte_max) {
+          cur_rb.set(idx);
+        }
+        first = false;
+        dec_pre = idx;
+      }
+      if (len - dec_pre > 1) bp.skip_n_byte(len - dec_pre - 1);
+      max = max | ((uint64_t)byte_max << (remain & 63));
+      rb1 = std::move(cur_rb);
+    } else {
+      bp.finish_read_byte();
+      BVIter it(rb1);
+      size_t idx, dec_pre = 0;
+      bool first = true;
+      while (it.next(&idx)) {
+        if (first) {
+          if (idx != 0) bp.skip(idx * remain);
+        } else {
+          size_t delta = idx - dec_pre;
+          if (delta != 1) bp.skip((delta - 1) * remain);
+        }
+        uint8_t dec = bp.read_bits(remain);
+        if (dec > byte_max) {
+          byte_max = dec;
+          if (!first) res.clear();
+          res.set(idx);
+        } else if (dec == byte_max) {
+          res.set(idx);
+        }
+        first = false;
+        dec_pre = idx;
+      }
+      remain = 0;
+      max = max | (uint64_t)byte_max;
+    }
+  }
+  if (max_off) *max_off = max;
+  if (argmax_count) *argmax_count = res.cardinality();
+  return (double)(int64_t)(max + (uint64_t)h.base) / powi(2.0, (int32_t)h.dlen);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1227,6 +1959,75 @@ int ref_decode_with_precision(const uint8_t* s, uint64_t len, uint64_t scale, ui
     std::memcpy(out, v.data(), v.size() * 8);
     return REF_OK;
   } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+// buff-major (buff_simd.rs:317-2861). Call ref_major_encode with out==NULL for the size.
+int ref_major_encode(const double* x, uint64_t n, uint64_t scale, uint8_t* out, uint64_t cap, uint64_t* out_len,
+                     int64_t* minmax) {
+  try {
+    int64_t mn, mx;
+    std::vector<uint8_t> s = buff_encode_majority(x, n, scale, &mn, &mx);
+    if (minmax) { minmax[0] = mn; minmax[1] = mx; }
+    *out_len = s.size();
+    if (!out) return REF_OK;
+    if (s.size() > cap) return REF_SMALL_BUFFER;
+    std::memcpy(out, s.data(), s.size());
+    return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+int ref_major_decode(const uint8_t* s, uint64_t len, uint64_t scale, double* out, uint64_t cap, uint64_t* n_out) {
+  try {
+    std::vector<double> v = buff_decode_majority(s, len, scale);
+    *n_out = v.size();
+    if (v.size() > cap) return REF_SMALL_BUFFER;
+    if (!v.empty()) std::memcpy(out, v.data(), v.size() * 8);
+    return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+int ref_major_filter(const uint8_t* s, uint64_t len, uint64_t scale, int op, double pred, uint32_t* words,
+                     uint64_t nwords_cap, uint64_t* count) {
+  try {
+    FilterOut fo = op == 0 ? buff_range_filter_majority(s, len, scale, pred)
+                           : buff_equal_filter_majority(s, len, scale, pred);
+    return copy_filter(fo, words, nwords_cap, count, nullptr);
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+// col_sums: up to 9 u64 (byte columns in stream order, then the tail).
+int ref_major_sum(const uint8_t* s, uint64_t len, uint64_t scale, double* sum, uint64_t* col_sums) {
+  try {
+    if (col_sums) std::memset(col_sums, 0, 9 * 8);
+    *sum = buff_sum_majority(s, len, scale, col_sums);
+    return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+int ref_major_max(const uint8_t* s, uint64_t len, uint64_t scale, double* mx, uint64_t* max_off,
+                  uint64_t* argmax_count) {
+  try {
+    *mx = buff_max_majority(s, len, scale, max_off, argmax_count);
+    return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+// Raw snappy of the restatement above (cap bytes of output; out_len always set).
+int ref_snappy_compress(const uint8_t* in, uint64_t n, uint8_t* out, uint64_t cap, uint64_t* out_len) {
+  std::vector<uint8_t> v = snappy_ref::compress(in, n);
+  *out_len = v.size();
+  if (v.size() > cap) return REF_SMALL_BUFFER;
+  std::memcpy(out, v.data(), v.size());
+  return REF_OK;
+}
+int ref_snappy_decompress(const uint8_t* in, uint64_t n, uint8_t* out, uint64_t cap, uint64_t* out_len) {
+  try {
+    std::vector<uint8_t> v = snappy_ref::decompress(in, n);
+    *out_len = v.size();
+    if (v.size() > cap) return REF_SMALL_BUFFER;
+    if (!v.empty()) std::memcpy(out, v.data(), v.size());
+    return REF_OK;
+  } catch (const snappy_ref::Error&) { return REF_PANIC; }
 }
 
 }  // extern "C"

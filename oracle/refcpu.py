@@ -67,6 +67,15 @@ def lib():
             "ref_max": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, f64p, u64p, u64p]),
             "ref_decode_with_precision": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
                                                          f64p, ctypes.c_uint64, u64p]),
+            "ref_major_encode": (ctypes.c_int, [f64p, ctypes.c_uint64, ctypes.c_uint64, u8p, ctypes.c_uint64, u64p,
+                                                i64p]),
+            "ref_major_decode": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, f64p, ctypes.c_uint64, u64p]),
+            "ref_major_filter": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_double,
+                                                u32p, ctypes.c_uint64, u64p]),
+            "ref_major_sum": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, f64p, u64p]),
+            "ref_major_max": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, f64p, u64p, u64p]),
+            "ref_snappy_compress": (ctypes.c_int, [u8p, ctypes.c_uint64, u8p, ctypes.c_uint64, u64p]),
+            "ref_snappy_decompress": (ctypes.c_int, [u8p, ctypes.c_uint64, u8p, ctypes.c_uint64, u64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -225,3 +234,79 @@ def decode_with_precision(stream, scale, precision):
                                            _ptr(out, ctypes.c_double), len(out), ctypes.byref(got)))
     return out[: got.value]
 
+
+
+# --- buff-major (buff_simd.rs:317-2861) -------------------------------------
+
+def major_encode(x, scale, with_minmax=False):
+    """buff_encode_majority -> uint8 stream."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = ctypes.c_uint64()
+    mm = np.zeros(2, dtype=np.int64)
+    L = lib()
+    _check(L.ref_major_encode(_ptr(x, ctypes.c_double), len(x), scale, None, 0, ctypes.byref(n),
+                              _ptr(mm, ctypes.c_int64)))
+    out = np.zeros(n.value, dtype=np.uint8)
+    _check(L.ref_major_encode(_ptr(x, ctypes.c_double), len(x), scale, _ptr(out, ctypes.c_uint8), n.value,
+                              ctypes.byref(n), _ptr(mm, ctypes.c_int64)))
+    return (out, int(mm[0]), int(mm[1])) if with_minmax else out
+
+
+def major_decode(stream, scale):
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    n = header(s)[1]
+    out = np.zeros(max(n, 1), dtype=np.float64)
+    got = ctypes.c_uint64()
+    _check(lib().ref_major_decode(_ptr(s, ctypes.c_uint8), len(s), scale, _ptr(out, ctypes.c_double), len(out),
+                                  ctypes.byref(got)))
+    return out[: got.value]
+
+
+def major_filter(stream, scale, op, pred):
+    """buff_range_filter_majority / buff_equal_filter_majority -> (u32 words, count)."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    n = header(s)[1]
+    words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
+    cnt = ctypes.c_uint64()
+    _check(lib().ref_major_filter(_ptr(s, ctypes.c_uint8), len(s), scale, 0 if op == "range" else 1, float(pred),
+                                  _ptr(words, ctypes.c_uint32), len(words), ctypes.byref(cnt)))
+    return words[: (n + 31) // 32], cnt.value
+
+
+def major_sum(stream, scale):
+    """buff_sum_majority -> (f64 sum, exact byte-column sums[9])."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    out = ctypes.c_double()
+    cs = np.zeros(9, dtype=np.uint64)
+    _check(lib().ref_major_sum(_ptr(s, ctypes.c_uint8), len(s), scale, ctypes.byref(out), _ptr(cs, ctypes.c_uint64)))
+    return out.value, cs
+
+
+def major_max(stream, scale):
+    """buff_max_majority -> (printed max value, max offset, argmax count)."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    out = ctypes.c_double()
+    off = ctypes.c_uint64()
+    cnt = ctypes.c_uint64()
+    _check(lib().ref_major_max(_ptr(s, ctypes.c_uint8), len(s), scale, ctypes.byref(out), ctypes.byref(off),
+                               ctypes.byref(cnt)))
+    return out.value, off.value, cnt.value
+
+
+def snappy_compress(data):
+    d = np.frombuffer(bytes(data), dtype=np.uint8)
+    cap = 32 + len(d) + len(d) // 6
+    out = np.zeros(cap, dtype=np.uint8)
+    n = ctypes.c_uint64()
+    _check(lib().ref_snappy_compress(_ptr(d, ctypes.c_uint8), len(d), _ptr(out, ctypes.c_uint8), cap,
+                                     ctypes.byref(n)))
+    return out[: n.value].tobytes()
+
+
+def snappy_decompress(data, size):
+    d = np.frombuffer(bytes(data), dtype=np.uint8)
+    out = np.zeros(max(size, 1), dtype=np.uint8)
+    n = ctypes.c_uint64()
+    _check(lib().ref_snappy_decompress(_ptr(d, ctypes.c_uint8), len(d), _ptr(out, ctypes.c_uint8), len(out),
+                                       ctypes.byref(n)))
+    return out[: n.value].tobytes()

@@ -209,3 +209,278 @@ def decode_semantic_precision(stream, precision):
     off = (h["off"] >> sh) << sh
     v = (np.uint64(h["base"] & ((1 << 64) - 1)) + off).view(np.int64)
     return v.astype(np.float64) / 2.0 ** h["dlen"]
+
+
+# --- buff-major (buff_simd.rs:317-2861) -------------------------------------
+# Stream: header | u8 ncols | per sparse column [u8 major][u32 bm_len]
+# [snappy(u64 nbits | u32 words)][u32 n_out][n_out outlier bytes] | one zero byte
+# (only when ncols > 0) | dense MSB-first planes | LSB-first tail.
+
+SAMPLE, M_LIMIT = 2000, 1200
+
+
+def snappy_decompress(buf):
+    """Raw snappy decoder written from the format description."""
+    buf = bytes(buf)
+    pos, n, shift = 0, 0, 0
+    while True:
+        b = buf[pos]
+        pos += 1
+        n |= (b & 0x7F) << shift
+        if not b & 0x80:
+            break
+        shift += 7
+    out = bytearray()
+    while pos < len(buf):
+        tag = buf[pos]
+        pos += 1
+        kind = tag & 3
+        if kind == 0:
+            ln = (tag >> 2) + 1
+            if ln > 60:
+                nb = ln - 60
+                ln = int.from_bytes(buf[pos:pos + nb], "little") + 1
+                pos += nb
+            out += buf[pos:pos + ln]
+            pos += ln
+            continue
+        if kind == 1:
+            ln, off = ((tag >> 2) & 7) + 4, ((tag >> 5) << 8) | buf[pos]
+            pos += 1
+        elif kind == 2:
+            ln, off = (tag >> 2) + 1, int.from_bytes(buf[pos:pos + 2], "little")
+            pos += 2
+        else:
+            ln, off = (tag >> 2) + 1, int.from_bytes(buf[pos:pos + 4], "little")
+            pos += 4
+        assert 0 < off <= len(out)
+        if off >= ln:
+            out += out[len(out) - off:len(out) - off + ln]
+        else:
+            for _ in range(ln):
+                out.append(out[-off])
+    assert len(out) == n
+    return bytes(out)
+
+
+def boyer_moore(values):
+    major, freq = 0, 0
+    for v in values:
+        if freq == 0:
+            major, freq = v, 1
+        elif v == major:
+            freq += 1
+        else:
+            freq -= 1
+    return major, freq
+
+
+def major_layout(x, scale):
+    """What buff_encode_majority stores, from its definition: base/F from the
+    plain min and max, the majority bytes from the Boyer-Moore vote over the
+    first 2000 offsets (accepted while the counter exceeds 1200), then every
+    byte column and the tail."""
+    _, dlen = dlen_of_scale(scale)
+    f = fixed(x, dlen)
+    n = len(f)
+    if n < SAMPLE:
+        raise ValueError("reference panics (n < 2000)")
+    mn, mx = int(f.min()), int(f.max())
+    F = width(mn, mx)
+    off = f.astype(np.uint64) - np.uint64(mn & ((1 << 64) - 1))
+    head = [int(v) for v in off[:SAMPLE]]
+    majors = []
+    rsf = F - 8 if F >= 8 else 0
+    m, fr = boyer_moore([(v >> rsf) & 0xFF for v in head])
+    if fr > M_LIMIT:
+        majors.append(m)
+        while rsf > 7:
+            rsf -= 8
+            m, fr = boyer_moore([(v >> rsf) & 0xFF for v in head])
+            if fr > M_LIMIT:
+                majors.append(m)
+            else:
+                break
+    S = len(majors)
+    if S and F < 8:
+        raise ValueError("reference does not terminate (F < 8 with a majority byte)")
+    cols = [((off >> np.uint64(F - 8 * (j + 1))) & np.uint64(0xFF)).astype(np.uint8) for j in range((F // 8) if F >= 8 else 0)]
+    rr = F - 8 * len(cols)
+    tail = (off & np.uint64((1 << rr) - 1)) if rr else np.zeros(n, np.uint64)
+    return dict(base=mn, n=n, F=F, dlen=dlen, ilen=(F - dlen) & 0xFFFFFFFF, S=S, majors=majors, cols=cols,
+                r=rr, tail=tail)
+
+
+def major_parse(stream):
+    """Parse a buff-major stream into its byte columns (outliers scattered back)."""
+    s = bytes(np.asarray(stream, dtype=np.uint8).tobytes())
+    base = int.from_bytes(s[0:8], "little", signed=True)
+    n = int.from_bytes(s[8:12], "little")
+    ilen = int.from_bytes(s[12:16], "little")
+    dlen = int.from_bytes(s[16:20], "little")
+    F = (ilen + dlen) & 0xFFFFFFFF
+    S = s[20]
+    pos = 21
+    majors, masks, cols, bm_lens, outliers = [], [], [], [], []
+    for _ in range(S):
+        major = s[pos]
+        bm_len = int.from_bytes(s[pos + 1:pos + 5], "little")
+        binary = snappy_decompress(s[pos + 5:pos + 5 + bm_len])
+        pos += 5 + bm_len
+        nout = int.from_bytes(s[pos:pos + 4], "little")
+        ob = np.frombuffer(s[pos + 4:pos + 4 + nout], dtype=np.uint8)
+        pos += 4 + nout
+        nbits = int.from_bytes(binary[:8], "little")
+        assert nbits == n and len(binary) == 8 + 4 * ((n + 31) // 32)
+        mask = np.unpackbits(np.frombuffer(binary[8:], dtype=np.uint8), bitorder="little")[:n].astype(bool)
+        assert int(mask.sum()) == nout
+        col = np.full(n, major, dtype=np.uint8)
+        col[mask] = ob
+        majors.append(major)
+        masks.append(mask)
+        cols.append(col)
+        bm_lens.append(bm_len)
+        outliers.append(ob)
+    if S:
+        assert s[pos] == 0
+        pos += 1
+    remain = F - 8 * S
+    P, r = remain // 8, remain % 8
+    for _ in range(P):
+        cols.append(np.frombuffer(s[pos:pos + n], dtype=np.uint8))
+        pos += n
+    tail = np.zeros(n, dtype=np.uint64)
+    if r:
+        nb = (n * r + 7) // 8
+        bits = np.unpackbits(np.frombuffer(s[pos:pos + nb], dtype=np.uint8), bitorder="little")[: n * r]
+        tail = (bits.reshape(n, r).astype(np.uint64) << np.arange(r, dtype=np.uint64)[None, :]).sum(axis=1)
+        tail = tail.astype(np.uint64)
+        pos += nb
+    assert pos == len(s), (pos, len(s))
+    return dict(base=base, n=n, ilen=ilen, dlen=dlen, F=F, S=S, majors=majors, masks=masks, cols=cols, r=r,
+                tail=tail, bm_lens=bm_lens, outliers=outliers)
+
+
+def _shift(h, j):
+    return h["F"] - 8 * (j + 1)
+
+
+def _as_f64(h, fixed_u64):
+    v = (np.uint64(h["base"] & ((1 << 64) - 1)) + fixed_u64).view(np.int64)
+    return v.astype(np.float64) / 2.0 ** h["dlen"]
+
+
+def major_decode(h):
+    """buff_decode_majority's output: sparse columns after the first OR the
+    majority byte shifted twice, and only for rows before that column's last
+    outlier; no output when the sparse columns end the stream; a sparse column
+    without outliers is an unwrap panic."""
+    n, S = h["n"], h["S"]
+    M64 = (1 << 64) - 1
+    fx = np.zeros(n, dtype=np.uint64)
+    for j in range(S):
+        sh = _shift(h, j)
+        mask = h["masks"][j]
+        if not mask.any():
+            raise ValueError("reference panics (sparse column without outliers)")
+        b = h["cols"][j].astype(np.uint64) << np.uint64(sh)
+        if j == 0:
+            fx = b
+            continue
+        last = int(np.flatnonzero(mask)[-1])
+        twice = np.uint64(((h["majors"][j] << sh) << sh) & M64)
+        part = np.where(mask, b, twice)
+        part[last + 1:] = 0
+        fx |= part
+    remain = h["F"] - 8 * S
+    if S and remain == 0:
+        return np.zeros(0, dtype=np.float64)
+    for k in range(S, len(h["cols"])):
+        fx |= h["cols"][k].astype(np.uint64) << np.uint64(_shift(h, k))
+    if h["r"]:
+        fx |= h["tail"]
+    return _as_f64(h, fx)
+
+
+def major_filter(h, scale, op, pred):
+    """Row mask of buff_range_filter_majority / buff_equal_filter_majority."""
+    n, S = h["n"], h["S"]
+    fp = fixed_pred(pred, scale, h["dlen"])
+    if fp < h["base"]:
+        return np.ones(n, bool) if op == "range" else np.zeros(n, bool)
+    t = (fp - h["base"]) & ((1 << 64) - 1)
+    res = np.zeros(n, bool)
+    cand = np.ones(n, bool)
+    for j, col in enumerate(h["cols"]):
+        tb = (t >> _shift(h, j)) & 0xFF
+        if op == "equal":
+            cand &= col == tb
+            continue
+        if j >= S or j == 0:
+            res |= cand & (col > tb)
+            cand &= col == tb
+            continue
+        mask, major = h["masks"][j], h["majors"][j]
+        if tb == 0 and major == 0:            # candidates are dropped
+            res |= cand & mask
+            cand[:] = False
+        else:
+            res |= mask & (col > tb)          # outliers: not masked by the candidates
+            if major > tb:
+                res |= cand & ~mask
+            cand &= col == tb
+    if h["r"]:
+        tt = t & ((1 << h["r"]) - 1)
+        hit = h["tail"] > np.uint64(tt) if op == "range" else h["tail"] == np.uint64(tt)
+        res |= cand & hit
+    return res
+
+
+def major_sum(h):
+    """buff_sum_majority: the per-column fold; the base term is dropped when a
+    dense plane ends the stream."""
+    ilen_i32 = h["ilen"] - (1 << 32) if h["ilen"] >= 1 << 31 else h["ilen"]
+    total, processed = 0.0, 0
+    sums = [int(c.astype(np.uint64).sum()) for c in h["cols"]]
+    for k, sk in enumerate(sums):
+        processed += 8
+        total = total + float(sk) / 2.0 ** (processed - ilen_i32)
+        if k >= h["S"] and k == len(sums) - 1 and h["r"] == 0:
+            return total, sums
+    if h["r"]:
+        processed += h["r"]
+        st = int(h["tail"].sum())
+        sums.append(st)
+        total = total + float(st) / 2.0 ** (processed - ilen_i32)
+    base_term = float(h["n"]) * float(h["base"]) / 2.0 ** h["dlen"]
+    return total + base_term, sums
+
+
+def major_max(h):
+    """buff_max_majority: the byte-wise progressive max with its sparse-column rules."""
+    n, S = h["n"], h["S"]
+    cand = np.ones(n, bool)
+    mx = 0
+    for j, col in enumerate(h["cols"]):
+        sh = _shift(h, j)
+        if j < S:
+            mask, major = h["masks"][j], h["majors"][j]
+            cout = cand & mask
+            if j > 0 and not cout.any():       # the major byte is not ORed in
+                cand &= ~mask
+                continue
+            m = int(col[cout].max()) if cout.any() else -1
+            chosen = m if m > major else major
+        else:
+            chosen = int(col[cand].max()) if cand.any() else 0
+        mx |= chosen << sh
+        cand &= col == chosen
+    argmax = 0
+    if h["r"]:
+        tc = h["tail"][cand]
+        chosen = int(tc.max()) if len(tc) else 0
+        mx |= chosen
+        argmax = int((tc == chosen).sum())
+    v = (mx + h["base"]) & ((1 << 64) - 1)
+    v = v - (1 << 64) if v >= 1 << 63 else v
+    return float(np.float64(np.int64(v))) / 2.0 ** h["dlen"], mx, argmax
