@@ -22,6 +22,7 @@ BUFF_E_SMALL_BUFFER = 6
 
 BUFF_FLIP = 0x1
 BUFF_SEMANTIC = 0x2
+BUFF_MAJORITY = 0x4
 BUFF_OP_GT = 0
 BUFF_OP_EQ = 1
 
@@ -57,6 +58,9 @@ class ChunkInfo(ctypes.Structure):
         ("scale", ctypes.c_uint64),
         ("stream_bytes", ctypes.c_uint64),
         ("device_bytes", ctypes.c_uint64),
+        ("sparse_cols", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("decoded_rows", ctypes.c_uint64),
     ]
 
 
@@ -90,10 +94,12 @@ SIGNATURES = {
     "buff_chunk_serialize": [vp, vp, u8p, u64p],
     "buff_chunk_load": [vp, u8p, u64, u64, u32, P(vp)],
     "buff_chunk_free": [vp],
+    "buff_chunk_sparse_col": [vp, u32, u32p, u64p, u64p],
     "buff_filter": [vp, vp, cint, f64, u32, u32p, u64, u64p],
     "buff_filter_device": [vp, vp, cint, f64, u32, vp, vp],
     "buff_sum": [vp, vp, f64p, f64p, u64p],
     "buff_max": [vp, vp, f64p, u64p],
+    "buff_max_ex": [vp, vp, u32, f64p, u64p, u64p],
     "buff_materialize": [vp, vp, i32, u32, f64p],
     "buff_materialize_device": [vp, vp, i32, u32, vp],
     "buff_sd_byte_fixed_encode": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
@@ -107,6 +113,12 @@ SIGNATURES = {
     "buff_sd_byte_fixed_sum": [P(SplitDouble), u8p, u64, f64p],
     "buff_sd_byte_fixed_max": [P(SplitDouble), u8p, u64, f64p],
     "buff_sd_byte_residue_decode_with_precision": [P(SplitDouble), u8p, u64, u32, f64p, u64],
+    "buff_sd_buff_encode_majority": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
+    "buff_sd_buff_decode_majority": [P(SplitDouble), u8p, u64, f64p, u64, u64p],
+    "buff_sd_buff_range_filter_majority": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
+    "buff_sd_buff_equal_filter_majority": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
+    "buff_sd_buff_sum_majority": [P(SplitDouble), u8p, u64, f64p],
+    "buff_sd_buff_max_majority": [P(SplitDouble), u8p, u64, f64p],
 }
 
 _lib = None
@@ -192,10 +204,24 @@ class Chunk:
         check(lib().buff_max(self.ctx.handle, self.handle, ctypes.byref(v), ctypes.byref(o)))
         return v.value, o.value
 
+    def max_ex(self, flags=0):
+        """Returns (max value, max stored offset, rows tying the max in the tail stage)."""
+        v, o, a = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().buff_max_ex(self.ctx.handle, self.handle, flags, ctypes.byref(v), ctypes.byref(o), ctypes.byref(a)))
+        return v.value, o.value, a.value
+
+    def sparse_col(self, j):
+        """buff-major sparse column j: (majority byte, outlier rows, last outlier row)."""
+        m, k, last = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().buff_chunk_sparse_col(self.handle, j, ctypes.byref(m), ctypes.byref(k), ctypes.byref(last)))
+        return m.value, k.value, last.value
+
     def materialize(self, precision=-1, flags=0):
-        out = np.zeros(max(self.n, 1), dtype=np.float64)
+        inf = self.info()
+        rows = inf.n if (precision >= 0 or flags & BUFF_SEMANTIC) else inf.decoded_rows
+        out = np.zeros(max(inf.n, 1), dtype=np.float64)
         check(lib().buff_materialize(self.ctx.handle, self.handle, precision, flags, ptr(out, ctypes.c_double)))
-        return out[: self.n]
+        return out[:rows]
 
     def materialize_device(self, out_dev, precision=-1, flags=0):
         check(lib().buff_materialize_device(self.ctx.handle, self.handle, precision, flags, out_dev))

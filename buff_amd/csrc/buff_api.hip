@@ -40,18 +40,6 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                                              \
-  do {                                                                                             \
-    hipError_t e_ = (expr);                                                                        \
-    if (e_ != hipSuccess) return fail(BUFF_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_));      \
-  } while (0)
-
-#define BUFF_TRY(expr)          \
-  do {                          \
-    int rc_ = (expr);           \
-    if (rc_ != BUFF_OK) return rc_; \
-  } while (0)
-
 // ---------------------------------------------------------------------------
 // precision helpers (the reference's per-call setup, split_double.rs:304-316)
 
@@ -274,6 +262,7 @@ int d2h_small(Ctx* ctx, void* dst, const void* src, uint64_t bytes) {
 
 int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_t flags, Chunk** out) {
   if (!ctx || !out) return fail(BUFF_E_INVALID, "null argument");
+  if (flags & BUFF_MAJORITY) return major_encode_device(ctx, x, n, scale, flags, out);
   if (n > 0xFFFFFFFFull) return fail(BUFF_E_INVALID, "a chunk holds at most 2^32-1 rows (n is stored as u32)");
   if (n && !x) return fail(BUFF_E_INVALID, "null input");
   const int prec = scale == 0 ? 0 : prec_of_scale(scale);
@@ -328,6 +317,7 @@ int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_
   ch->flags = flags & BUFF_FLIP;
   ch->dlen = dlen;
   ch->ilen = (uint32_t)((uint64_t)F - dlen);
+  ch->F = F;
   ch->view.base = mn;
   ch->view.dlen = dlen;
   chunk_set_width(ch, F);
@@ -351,6 +341,7 @@ int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_
 
 int chunk_serialize(Ctx* ctx, const Chunk* ch, uint8_t* dst, uint64_t* len) {
   if (!ctx || !ch || !len) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->flags & BUFF_MAJORITY) return major_serialize(ctx, ch, dst, len);
   const uint64_t total = stream_bytes_of(ch->n, ch->view.P, ch->view.r);
   const uint64_t need = *len;
   *len = total;
@@ -375,6 +366,7 @@ int chunk_serialize(Ctx* ctx, const Chunk* ch, uint8_t* dst, uint64_t* len) {
 
 int chunk_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_t flags, Chunk** out) {
   if (!ctx || !s || !out) return fail(BUFF_E_INVALID, "null argument");
+  if (flags & BUFF_MAJORITY) return major_load(ctx, s, len, scale, flags, out);
   if (len < kStreamHeader) return fail(BUFF_E_INVALID, "stream of %llu bytes has no 20-byte header",
                                        (unsigned long long)len);
   uint32_t hdr[5];
@@ -392,6 +384,7 @@ int chunk_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_
   ch->flags = flags & BUFF_FLIP;
   ch->dlen = dlen;
   ch->ilen = ilen;
+  ch->F = F;
   ch->view.base = base;
   ch->view.dlen = dlen;
   chunk_set_width(ch, F);
@@ -476,6 +469,7 @@ int filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags
   if (op != BUFF_OP_GT && op != BUFF_OP_EQ) return fail(BUFF_E_INVALID, "unknown filter op %d", op);
   const uint64_t nwords = (ch->n + 63) / 64;
   if (nwords == 0) return BUFF_OK;
+  if (ch->S > 0) return major_filter_device(ctx, ch, op, pred, flags, words);
   FilterTarget tg;
   int fill = 0;
   if (!filter_plan(ch, op, pred, flags, &tg, &fill)) {
@@ -537,6 +531,7 @@ double reference_sum_fold(const Chunk* ch, const uint64_t* ps) {
 
 int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64_t* plane_sums) {
   if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->S > 0) return major_sum(ctx, ch, 0, ref_sum, exact_sum, plane_sums);
   uint64_t ps[kMaxPlanes + 1] = {0};
   const ChunkView& v = ch->view;
   if (ch->n > 0 && (v.P || v.r)) {
@@ -559,8 +554,10 @@ int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64
   return BUFF_OK;
 }
 
-int max_op(Ctx* ctx, const Chunk* ch, double* value, uint64_t* max_off) {
+int max_op(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t* max_off, uint64_t* argmax) {
   if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  if (argmax) *argmax = 0;
+  if (ch->S > 0) return major_max(ctx, ch, flags, value, max_off, argmax);
   uint64_t best = 0;
   const ChunkView& v = ch->view;
   if (ch->n > 0 && (v.P || v.r)) {
@@ -613,6 +610,11 @@ int precision_bits(const Chunk* ch, int32_t precision, uint32_t* R) {
 
 int materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* out) {
   if (!ctx || !ch || (!out && ch->n)) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->flags & BUFF_MAJORITY) {
+    if (precision >= 0)
+      return fail(BUFF_E_UNSUPPORTED, "the reference has no reduced-precision decode of a buff-major stream");
+    if (ch->S > 0) return major_materialize_device(ctx, ch, precision, flags, out, nullptr);
+  }
   const ChunkView& v = ch->view;
   MatParams mp;
   memset(&mp, 0, sizeof(mp));
@@ -668,7 +670,8 @@ int materialize_host(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flag
   if (!out) return fail(BUFF_E_INVALID, "null output");
   BUFF_TRY(ctx->scratch(ctx->s_values, ch->n * 8));
   BUFF_TRY(materialize_device(ctx, ch, precision, flags, (double*)ctx->s_values.ptr));
-  HIP_TRY(hipMemcpyAsync(out, ctx->s_values.ptr, ch->n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  const uint64_t rows = (ch->flags & BUFF_MAJORITY) ? major_decoded_rows(ch, flags) : ch->n;
+  if (rows) HIP_TRY(hipMemcpyAsync(out, ctx->s_values.ptr, rows * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   return BUFF_OK;
 }
@@ -855,13 +858,31 @@ int buff_chunk_info_get(const buff_chunk* chunk, buff_chunk_info* info) {
   info->n = (uint32_t)ch->n;
   info->ilen = ch->ilen;
   info->dlen = ch->dlen;
-  info->width = ch->view.F;
+  info->width = ch->F;
   info->planes = ch->view.P;
   info->tail_bits = ch->view.r;
   info->flags = ch->flags;
   info->scale = ch->scale;
   info->stream_bytes = stream_bytes_of(ch->n, ch->view.P, ch->view.r);
+  if (ch->flags & BUFF_MAJORITY) {
+    // the snappy-compressed bitmaps are only sized by buff_chunk_serialize(dst = NULL)
+    info->stream_bytes = ch->S ? 0 : (ch->F >= 8 ? info->stream_bytes + 1 : info->stream_bytes);
+  }
   info->device_bytes = ch->device_bytes;
+  info->sparse_cols = ch->S;
+  info->reserved = 0;
+  info->decoded_rows = (ch->flags & BUFF_MAJORITY) ? major_decoded_rows(ch, 0) : ch->n;
+  return BUFF_OK;
+}
+
+int buff_chunk_sparse_col(const buff_chunk* chunk, uint32_t j, uint32_t* major, uint64_t* outliers,
+                          uint64_t* last_row) {
+  if (!chunk) return fail(BUFF_E_INVALID, "null argument");
+  const Chunk* ch = reinterpret_cast<const Chunk*>(chunk);
+  if (j >= ch->S) return fail(BUFF_E_INVALID, "chunk has %u sparse columns", ch->S);
+  if (major) *major = ch->sp[j].major;
+  if (outliers) *outliers = ch->sp[j].n_out;
+  if (last_row) *last_row = ch->sp[j].last;
   return BUFF_OK;
 }
 
@@ -882,6 +903,7 @@ int buff_chunk_free(buff_chunk* chunk) {
   Chunk* ch = reinterpret_cast<Chunk*>(chunk);
   Ctx* c = ch->ctx;
   (void)hipStreamSynchronize(c->stream);
+  major_release(ch);
   int rc = c->release(ch->mem);
   delete ch;
   return rc;
@@ -904,7 +926,13 @@ int buff_sum(buff_ctx* ctx, const buff_chunk* chunk, double* ref_sum, double* ex
 }
 
 int buff_max(buff_ctx* ctx, const buff_chunk* chunk, double* max_value, uint64_t* max_off) {
-  return max_op(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), max_value, max_off);
+  return max_op(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), 0, max_value, max_off, nullptr);
+}
+
+int buff_max_ex(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, double* max_value, uint64_t* max_off,
+                uint64_t* argmax_count) {
+  return max_op(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), flags, max_value, max_off,
+                argmax_count);
 }
 
 int buff_materialize(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags, double* out) {
@@ -1023,6 +1051,56 @@ int buff_sd_byte_fixed_max(const buff_split_double* sd, const uint8_t* bytes, ui
   Ctx* c = nullptr;
   buff_chunk* ch = nullptr;
   BUFF_TRY(sd_load(sd, bytes, len, 0, &c, &ch));
+  int rc = buff_max(reinterpret_cast<buff_ctx*>(c), ch, max_value, nullptr);
+  (void)buff_chunk_free(ch);
+  return rc;
+}
+
+// ---- buff-major (buff_simd.rs:317-2861) ----------------------------------------
+
+int buff_sd_buff_encode_majority(const buff_split_double* sd, const double* data, uint64_t n, uint8_t* out,
+                                 uint64_t cap, uint64_t* out_len) {
+  return sd_encode(sd, data, n, BUFF_MAJORITY, out, cap, out_len);
+}
+
+int buff_sd_buff_decode_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* out,
+                                 uint64_t cap, uint64_t* n_out) {
+  if (!n_out) return fail(BUFF_E_INVALID, "null argument");
+  Ctx* c = nullptr;
+  buff_chunk* ch = nullptr;
+  BUFF_TRY(sd_load(sd, bytes, len, BUFF_MAJORITY, &c, &ch));
+  const uint64_t rows = major_decoded_rows(reinterpret_cast<Chunk*>(ch), 0);
+  int rc = BUFF_OK;
+  if (cap < rows) rc = fail(BUFF_E_SMALL_BUFFER, "output needs %llu doubles", (unsigned long long)rows);
+  if (rc == BUFF_OK) rc = buff_materialize(reinterpret_cast<buff_ctx*>(c), ch, -1, 0, out);
+  *n_out = rc == BUFF_OK ? rows : 0;
+  (void)buff_chunk_free(ch);
+  return rc;
+}
+
+int buff_sd_buff_range_filter_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double pred,
+                                       uint32_t* words, uint64_t nwords, uint64_t* count) {
+  return sd_filter(sd, bytes, len, BUFF_MAJORITY, BUFF_OP_GT, pred, words, nwords, count);
+}
+
+int buff_sd_buff_equal_filter_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double pred,
+                                       uint32_t* words, uint64_t nwords, uint64_t* count) {
+  return sd_filter(sd, bytes, len, BUFF_MAJORITY, BUFF_OP_EQ, pred, words, nwords, count);
+}
+
+int buff_sd_buff_sum_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* s) {
+  Ctx* c = nullptr;
+  buff_chunk* ch = nullptr;
+  BUFF_TRY(sd_load(sd, bytes, len, BUFF_MAJORITY, &c, &ch));
+  int rc = buff_sum(reinterpret_cast<buff_ctx*>(c), ch, s, nullptr, nullptr);
+  (void)buff_chunk_free(ch);
+  return rc;
+}
+
+int buff_sd_buff_max_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* max_value) {
+  Ctx* c = nullptr;
+  buff_chunk* ch = nullptr;
+  BUFF_TRY(sd_load(sd, bytes, len, BUFF_MAJORITY, &c, &ch));
   int rc = buff_max(reinterpret_cast<buff_ctx*>(c), ch, max_value, nullptr);
   (void)buff_chunk_free(ch);
   return rc;

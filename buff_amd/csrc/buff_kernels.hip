@@ -13,133 +13,10 @@
 #include <hip/hip_runtime.h>
 
 #include "buff_common.h"
+#include "buff_device.h"
 #include "buff_kernels.h"
 
 namespace buffgpu {
-
-// ----------------------------------------------------------------------------
-// wave helpers
-
-__device__ __forceinline__ int64_t wave_min_i64(int64_t v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = min(v, (int64_t)__shfl_xor((long long)v, m, 64));
-  return v;
-}
-__device__ __forceinline__ int64_t wave_max_i64(int64_t v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = max(v, (int64_t)__shfl_xor((long long)v, m, 64));
-  return v;
-}
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    uint64_t o = (uint64_t)__shfl_xor((unsigned long long)v, m, 64);
-    v = o > v ? o : v;
-  }
-  return v;
-}
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += (uint64_t)__shfl_xor((unsigned long long)v, m, 64);
-  return v;
-}
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, m, 64));
-  return v;
-}
-__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
-  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return ((uint64_t)hi << 32) | lo;
-}
-// Orders this wave's LDS writes before its later LDS reads (and vice versa).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// ----------------------------------------------------------------------------
-// Tile access
-
-__device__ __forceinline__ bool tile_full(uint64_t n, uint64_t c) { return (c + 1) * kTileRows <= n; }
-
-__device__ __forceinline__ uint4 load_plane(const ChunkView& cv, uint32_t p, uint64_t c, uint32_t lane) {
-  return reinterpret_cast<const uint4*>(cv.planes + p * cv.pitch + c * kTileRows)[lane];
-}
-
-// bit-plane j of tile c, lane's 16 rows (bit b <-> row 16*lane + b)
-__device__ __forceinline__ uint32_t load_tail(const ChunkView& cv, uint32_t j, uint64_t c, uint32_t lane) {
-  return cv.tail[(c * cv.r + j) * 64 + lane];
-}
-
-// byte of row 16*lane + b inside the lane's 16 bytes (dword b%4, byte b/4)
-__device__ __forceinline__ uint32_t byte_at(const uint4& w, uint32_t b) {
-  const uint32_t i = b & 3, k = b >> 2;
-  const uint32_t d = i == 0 ? w.x : i == 1 ? w.y : i == 2 ? w.z : w.w;
-  return (d >> (8 * k)) & 0xFFu;
-}
-
-// 16 per-row bytes (row offsets b = 0..15) -> the lane's uint4
-__device__ __forceinline__ uint4 pack_lane(const uint32_t (&by)[16]) {
-  uint32_t d[4];
-#pragma unroll
-  for (uint32_t i = 0; i < 4; ++i) d[i] = by[i] | (by[4 + i] << 8) | (by[8 + i] << 16) | (by[12 + i] << 24);
-  return make_uint4(d[0], d[1], d[2], d[3]);
-}
-
-// valid-row mask of the lane's 16 rows
-__device__ __forceinline__ uint32_t lane_valid16(uint64_t n, uint64_t c, uint32_t lane) {
-  const uint64_t r0 = c * kTileRows + 16ull * lane;
-  if (r0 + 16 <= n) return 0xFFFFu;
-  if (r0 >= n) return 0u;
-  return (1u << (uint32_t)(n - r0)) - 1u;
-}
-
-// ---- SWAR byte compares (4 rows per dword) --------------------------------------
-// bit 7 of every byte of the result holds the per-byte predicate.
-
-// unsigned x > d for each byte, given D1 = (d+1) replicated and D1lo = D1 & 0x7f7f7f7f
-// (x >= d+1: the low 7 bits compare without borrows across bytes, the high bits
-// decide where they differ).
-__device__ __forceinline__ uint32_t swar_gt(uint32_t x, uint32_t D1, uint32_t D1lo) {
-  const uint32_t diff = (x | 0x80808080u) - D1lo;
-  const uint32_t e = x ^ D1;
-  return ((e & x) | (~e & diff)) & 0x80808080u;
-}
-// x == d for each byte, D = d replicated
-__device__ __forceinline__ uint32_t swar_eq(uint32_t x, uint32_t D) {
-  const uint32_t t = x ^ D;
-  return ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & 0x80808080u;
-}
-// bit 7 of byte k of gi -> bit 4k + i: the lane's 16-row mask
-__device__ __forceinline__ uint32_t gather16(uint32_t g0, uint32_t g1, uint32_t g2, uint32_t g3) {
-  const uint32_t G = (g0 >> 7) | (g1 >> 6) | (g2 >> 5) | (g3 >> 4);
-  const uint32_t G2 = G | (G >> 4);
-  return (G2 & 0xFFu) | ((G2 >> 8) & 0xFF00u);
-}
-
-struct Digit {  // one target byte, prepared for the SWAR compares
-  uint32_t D, D1, D1lo;
-  bool gt_possible;  // d < 255
-};
-__device__ __forceinline__ Digit make_digit(uint32_t d) {
-  Digit r;
-  r.D = 0x01010101u * d;
-  r.gt_possible = d < 255;
-  r.D1 = 0x01010101u * ((d + 1) & 0xFF);
-  r.D1lo = r.D1 & 0x7F7F7F7Fu;
-  return r;
-}
-__device__ __forceinline__ uint32_t mask_gt(const uint4& w, const Digit& dg) {
-  if (!dg.gt_possible) return 0u;
-  return gather16(swar_gt(w.x, dg.D1, dg.D1lo), swar_gt(w.y, dg.D1, dg.D1lo), swar_gt(w.z, dg.D1, dg.D1lo),
-                  swar_gt(w.w, dg.D1, dg.D1lo));
-}
-__device__ __forceinline__ uint32_t mask_eq(const uint4& w, const Digit& dg) {
-  return gather16(swar_eq(w.x, dg.D), swar_eq(w.y, dg.D), swar_eq(w.z, dg.D), swar_eq(w.w, dg.D));
-}
 
 // ----------------------------------------------------------------------------
 // K1: fixed-point conversion + min / max reduction (encode pass 1).
@@ -147,21 +24,6 @@ __device__ __forceinline__ uint32_t mask_eq(const uint4& w, const Digit& dg) {
 // [w*tpw, (w+1)*tpw), so its partial min is a segment min the prefix pass reuses.
 // max is taken over rows i >= 1 (row 0 only ever updates the min). Row order
 // does not matter here, so lanes read rows 64g + t (coalesced).
-
-__device__ __forceinline__ void load_tile_f64(const double* __restrict__ x, uint64_t n, uint64_t c, uint32_t lane,
-                                              bool full, uint64_t (&v)[16]) {
-  const double* xt = x + c * kTileRows + lane;
-  if (full) {
-#pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) v[g] = __double_as_longlong(xt[64 * g]);
-  } else {
-#pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) {
-      const uint64_t row = c * kTileRows + 64 * g + lane;
-      v[g] = row < n ? __double_as_longlong(xt[64 * g]) : 0;
-    }
-  }
-}
 
 __global__ __launch_bounds__(256) void k_minmax(const double* __restrict__ x, uint64_t n, uint64_t ntiles,
                                                 uint64_t tpw, int32_t pexp, uint32_t dlen,
@@ -279,10 +141,6 @@ __global__ __launch_bounds__(64) void k_minmax_prefix_final(const int64_t* __res
 // (one pad word per 16 rows against bank conflicts) and read back their own 16
 // consecutive rows.
 
-constexpr uint32_t kLdsRow = 17;  // u64 slots per 16 rows (+1 pad)
-
-__device__ __forceinline__ uint32_t lds_slot(uint32_t row_in_tile) { return row_in_tile + (row_in_tile >> 4); }
-
 __global__ __launch_bounds__(256) void k_encode(const double* __restrict__ x, uint64_t n, uint64_t ntiles,
                                                 int32_t pexp, uint32_t dlen, int64_t base, uint32_t F, uint32_t P,
                                                 uint32_t r, uint8_t* __restrict__ planes, uint64_t pitch,
@@ -336,25 +194,6 @@ __global__ __launch_bounds__(256) void k_encode(const double* __restrict__ x, ui
 // refined tile costs the 64-byte sectors that hold ties. The tail compares
 // bit-plane by bit-plane on the 16-row masks. The result mask is the lane's
 // 16 bits of the tile bitmap.
-
-template <int MODE>
-__device__ __forceinline__ void refine_tail(const ChunkView& cv, uint64_t c, uint64_t dt, uint32_t lane, uint32_t& gtm,
-                                            uint32_t& eqm) {
-  uint32_t h[8];
-  const uint32_t src = eqm ? lane : 0u;
-#pragma unroll
-  for (uint32_t j = 0; j < 8; ++j) h[j] = load_tail(cv, min(j, cv.r - 1), c, src);
-#pragma unroll
-  for (int j = 7; j >= 0; --j) {
-    if ((uint32_t)j >= cv.r) continue;
-    if ((dt >> j) & 1) {
-      eqm &= h[j];
-    } else {
-      if (MODE == 0) gtm |= eqm & h[j];
-      eqm &= ~h[j];
-    }
-  }
-}
 
 template <int MODE, int U>
 __global__ __launch_bounds__(256) void k_filter(ChunkView cv, FilterTarget tg, uint64_t* __restrict__ out,
@@ -576,9 +415,6 @@ __global__ __launch_bounds__(256) void k_sum_final(const unsigned long long* __r
 // records each tile's largest byte plus one per workgroup; pass 2 rebuilds
 // whole offsets only in tiles whose largest byte equals the global one.
 
-__device__ __forceinline__ uint32_t max_byte4(uint32_t d) {
-  return max(max(d & 0xFFu, (d >> 8) & 0xFFu), max((d >> 16) & 0xFFu, d >> 24));
-}
 
 template <int U>
 __global__ __launch_bounds__(256) void k_max_top(ChunkView cv, uint8_t* __restrict__ tile_max,

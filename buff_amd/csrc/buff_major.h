@@ -1,0 +1,87 @@
+// Resident form and kernel launchers of the buff-major (majority / outlier)
+// layout (database/src/compress/buff_simd.rs:317-2861).
+//
+// A buff-major chunk of width F keeps its S top byte columns sparse. Sparse
+// column j (byte (off >> (F - 8(j+1))) as u8) stores:
+//   * the majority byte chosen by the encoder's vote,
+//   * the outlier bitmap (rows whose byte differs) in row order, LSB first:
+//     exactly the BitVec<u32> storage the reference serializes (u16 word
+//     64c + t holds rows 1024c + 16t .. +15, the 16 rows lane t owns in tile c),
+//   * the outlier bytes in row order, compacted,
+//   * per tile c the number of outliers in tiles before c (prefix[c]), so a wave
+//     finds the position of its lanes' outlier bytes with one exclusive scan.
+// The low F - 8S bits are an ordinary resident chunk (buff_common.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "buff_common.h"
+#include "buff_kernels.h"
+
+namespace buffgpu {
+
+constexpr uint32_t kMaxSparse = 8;
+
+struct SparseDev {
+  const uint16_t* mask16;  // ntiles * 64 words
+  const uint32_t* prefix;  // ntiles + 1 (prefix[ntiles] = outliers)
+  const uint8_t* outl;     // compacted outlier bytes
+  uint64_t last;           // row of the column's last outlier
+  uint32_t major;          // majority byte
+  uint32_t shift;          // F - 8(j+1)
+};
+
+struct MajorView {
+  ChunkView dense;               // the low F - 8S bits (base, dlen, n, ntiles set)
+  SparseDev col[kMaxSparse];
+  uint32_t S;                    // sparse columns
+  uint32_t F;                    // full width
+};
+
+// Encode passes. Pass 1 writes the dense planes / tail, the outlier bitmaps and
+// per-tile outlier counts; pass 2 (after the scan) writes the outlier bytes.
+struct MajorEnc {
+  uint32_t S;
+  uint32_t major[kMaxSparse];
+  uint32_t shift[kMaxSparse];
+  uint16_t* mask16[kMaxSparse];
+  uint32_t* counts[kMaxSparse];
+  const uint32_t* prefix[kMaxSparse];
+  uint8_t* outl[kMaxSparse];
+};
+
+// Per-query constants: target byte of every byte column (sparse then dense,
+// most significant first) and the tail target.
+struct MajorTarget {
+  uint32_t digit[kMaxPlanes];
+  uint64_t tail;
+};
+
+// Device state of the staged max: chosen byte per stage and the accumulated
+// max offset; argmax = rows that tie the max in the tail stage.
+struct MajorMaxState {
+  uint32_t chosen[16];
+  unsigned long long acc;
+  unsigned long long argmax;
+};
+
+hipError_t launch_major_encode(int pass, const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t base,
+                               const ChunkView& dense, const MajorEnc& me, hipStream_t s);
+// counts (S arrays of ntiles) -> prefix (S arrays of ntiles+1); totals[2j] = outliers,
+// totals[2j+1] = last outlier row (UINT64_MAX if none).
+hipError_t launch_major_scan(const MajorEnc& me, uint64_t ntiles, unsigned long long* totals, hipStream_t s);
+hipError_t launch_major_filter(const MajorView& mv, int op, bool semantic, const MajorTarget& tg, uint64_t* out,
+                               uint64_t nwords, hipStream_t s, const LaunchShape& ls);
+hipError_t launch_major_decode(const MajorView& mv, bool semantic, double inv_scale, double* out, hipStream_t s,
+                               const LaunchShape& ls);
+// sums[j] = sum of the outlier bytes of sparse column j; scratch: major_sum_scratch_bytes().
+uint64_t major_sum_scratch_bytes(uint32_t S);
+hipError_t launch_major_outlier_sums(const MajorView& mv, const uint64_t* n_out, unsigned long long* sums,
+                                     unsigned long long* scratch, hipStream_t s);
+// Staged max over the byte columns and the tail; scratch: major_max_scratch_bytes(ntiles).
+uint64_t major_max_scratch_bytes(uint64_t ntiles);
+hipError_t launch_major_max(const MajorView& mv, MajorMaxState* st, uint8_t* scratch, hipStream_t s,
+                            const LaunchShape& ls, bool semantic);
+// Outlier bitmap words of rows >= n are zero; counts per tile from the masks (load path).
+hipError_t launch_major_counts(const MajorEnc& me, uint64_t ntiles, hipStream_t s);
+
+}  // namespace buffgpu

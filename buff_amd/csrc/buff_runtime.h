@@ -16,6 +16,26 @@ namespace buffgpu {
 
 int fail(int code, const char* fmt, ...);
 
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return fail(BUFF_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_));      \
+  } while (0)
+
+#define BUFF_TRY(expr)          \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != BUFF_OK) return rc_; \
+  } while (0)
+
+// precision helpers and small transfers (buff_api.hip)
+int prec_of_scale(uint64_t scale);
+int precision_map(int p);
+int32_t precision_exp_of_prec(int prec);
+double powi2(int32_t e);
+uint32_t width_of_delta(int64_t delta);
+uint64_t stream_bytes_of(uint64_t n, uint32_t P, uint32_t r);
+
 inline uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
 enum class Op : uint32_t { Encode = 0, FilterGt, FilterEq, Sum, Max, Materialize, kCount };
@@ -45,7 +65,7 @@ struct Ctx {
   std::multimap<uint64_t, void*> free_blocks;
   void* pinned = nullptr;
 
-  Scratch s_partial, s_minmax, s_stream, s_bitmap, s_sums, s_best, s_values, s_sink;
+  Scratch s_partial, s_minmax, s_stream, s_bitmap, s_sums, s_best, s_values, s_sink, s_major;
 
   bool prof_on = false;
   std::vector<hipEvent_t> prof_pool;
@@ -72,16 +92,49 @@ struct ProfScope {
   ~ProfScope() { ctx->prof_end(name, ev); }
 };
 
+// One sparse (majority / outlier) byte column of a buff-major chunk.
+struct SparseCol {
+  uint32_t major = 0;
+  uint32_t shift = 0;       // F - 8(j+1)
+  uint64_t n_out = 0;       // outlier rows
+  uint64_t last = ~0ull;    // row of the last outlier (~0 when there is none)
+  uint16_t* mask16 = nullptr;
+  uint32_t* prefix = nullptr;
+  uint8_t* outl = nullptr;
+};
+
 struct Chunk {
   Ctx* ctx = nullptr;
   uint64_t n = 0;
   uint64_t scale = 0;
-  uint32_t flags = 0;  // BUFF_FLIP
+  uint32_t flags = 0;  // BUFF_FLIP | BUFF_MAJORITY
   uint32_t dlen = 0;
   uint32_t ilen = 0;
+  uint32_t F = 0;      // full stored width; `view` holds the low F - 8*S bits
+  uint32_t S = 0;      // sparse columns (buff-major chunks only)
   ChunkView view{};
   void* mem = nullptr;
+  void* mem_sparse = nullptr;  // bitmaps and prefixes of the sparse columns
+  void* mem_outl = nullptr;    // compacted outlier bytes
+  SparseCol sp[8];
   uint64_t device_bytes = 0;
 };
+
+int d2h_small(Ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int chunk_alloc(Ctx* ctx, Chunk* ch);
+void chunk_set_width(Chunk* ch, uint32_t F);
+int default_ctx(Ctx** out);
+
+// buff-major (buff_major_api.hip)
+int major_encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_t flags, Chunk** out);
+int major_serialize(Ctx* ctx, const Chunk* ch, uint8_t* dst, uint64_t* len);
+int major_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_t flags, Chunk** out);
+int major_filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, uint64_t* words);
+int major_sum(Ctx* ctx, const Chunk* ch, uint32_t flags, double* ref_sum, double* exact_sum, uint64_t* col_sums);
+int major_max(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t* max_off, uint64_t* argmax);
+int major_materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* out,
+                             uint64_t* rows);
+uint64_t major_decoded_rows(const Chunk* ch, uint32_t flags);
+void major_release(Chunk* ch);
 
 }  // namespace buffgpu

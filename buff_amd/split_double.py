@@ -97,3 +97,37 @@ class SplitBDDoubleCompress:
         N.check(N.lib().buff_sd_byte_fixed_max(ctypes.byref(self._sd), N.ptr(b, ctypes.c_uint8), len(b),
                                                ctypes.byref(m)))
         return m.value
+
+    # ---- buff-major (buff_simd.rs:317-2861) -------------------------------------
+
+    def buff_encode_majority(self, data):  # buff_simd.rs:317
+        return self._encode("buff_sd_buff_encode_majority", data)
+
+    def buff_decode_majority(self, b):  # buff_simd.rs:633 (empty when the stream ends in sparse columns)
+        b = _u8(b)
+        n = _stream_n(b)
+        out = np.zeros(max(n, 1), dtype=np.float64)
+        got = ctypes.c_uint64()
+        N.check(N.lib().buff_sd_buff_decode_majority(ctypes.byref(self._sd), N.ptr(b, ctypes.c_uint8), len(b),
+                                                     N.ptr(out, ctypes.c_double), len(out), ctypes.byref(got)))
+        return out[: got.value]
+
+    def buff_range_filter_majority(self, b, pred):  # buff_simd.rs:863
+        return self._filter("buff_sd_buff_range_filter_majority", b, pred)
+
+    def buff_equal_filter_majority(self, b, pred):  # buff_simd.rs:1916
+        return self._filter("buff_sd_buff_equal_filter_majority", b, pred)
+
+    def buff_sum_majority(self, b):  # buff_simd.rs:2294
+        b = _u8(b)
+        s = ctypes.c_double()
+        N.check(N.lib().buff_sd_buff_sum_majority(ctypes.byref(self._sd), N.ptr(b, ctypes.c_uint8), len(b),
+                                                  ctypes.byref(s)))
+        return s.value
+
+    def buff_max_majority(self, b):  # buff_simd.rs:2444
+        b = _u8(b)
+        m = ctypes.c_double()
+        N.check(N.lib().buff_sd_buff_max_majority(ctypes.byref(self._sd), N.ptr(b, ctypes.c_uint8), len(b),
+                                                  ctypes.byref(m)))
+        return m.value

@@ -49,6 +49,13 @@ extern "C" {
  * predicates above the column maximum, sum without the base term when F%8==0,
  * unscaled reduced-precision decode). Default (0) reproduces the reference. */
 #define BUFF_SEMANTIC 0x2u
+/* buff_encode_majority layout (buff_simd.rs:317-630): while the top byte of the
+ * first 2000 offsets has a Boyer-Moore majority (vote counter > 1200), that byte
+ * column is stored sparse as {majority byte, snappy-compressed outlier bitmap,
+ * outlier bytes}; the remaining bits are byte planes and a tail as in
+ * byte_fixed_encode. Accepted by buff_encode, buff_encode_device and
+ * buff_chunk_load; every query dispatches on it. */
+#define BUFF_MAJORITY 0x4u
 
 /* filter operators */
 #define BUFF_OP_GT 0 /* x > pred (buff_range_filter, buff_simd.rs:4507-4679) */
@@ -70,6 +77,10 @@ typedef struct {
   uint64_t scale;        /* the SplitBDDoubleCompress scale it was encoded with */
   uint64_t stream_bytes; /* size of the serialized reference stream */
   uint64_t device_bytes; /* HBM held by the resident form */
+  uint32_t sparse_cols;  /* buff-major: sparse byte columns (planes/tail_bits then describe the rest) */
+  uint32_t reserved;
+  uint64_t decoded_rows; /* values buff_materialize writes at full precision (buff-major: 0 when the
+                            stream ends in sparse columns, buff_simd.rs:788-845) */
 } buff_chunk_info;
 
 /* ---- context --------------------------------------------------------------- */
@@ -117,6 +128,10 @@ int buff_chunk_serialize(buff_ctx* ctx, const buff_chunk* chunk, uint8_t* dst, u
 int buff_chunk_load(buff_ctx* ctx, const uint8_t* stream, uint64_t len, uint64_t scale, uint32_t flags,
                     buff_chunk** out);
 int buff_chunk_free(buff_chunk* chunk);
+/* Sparse column j of a buff-major chunk (buff_simd.rs:451-530): its majority
+ * byte, outlier rows and the row of its last outlier (UINT64_MAX if none). */
+int buff_chunk_sparse_col(const buff_chunk* chunk, uint32_t j, uint32_t* major, uint64_t* outliers,
+                          uint64_t* last_row);
 
 /* ---- queries on a resident chunk ------------------------------------------ */
 /* Filter into host words: ceil(n/32) u32 words; *count = set bits (the count the
@@ -135,9 +150,16 @@ int buff_sum(buff_ctx* ctx, const buff_chunk* chunk, double* ref_sum, double* ex
 /* byte_fixed_max (split_double.rs:3919-4105): the printed max value and the max
  * stored offset. */
 int buff_max(buff_ctx* ctx, const buff_chunk* chunk, double* max_value, uint64_t* max_off);
+/* buff_max with flags (BUFF_SEMANTIC: the true maximum of a buff-major chunk)
+ * and, for buff-major chunks, the rows tying the max in the tail stage that
+ * buff_max_majority prints (buff_simd.rs:2856-2858; 0 otherwise). */
+int buff_max_ex(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, double* max_value, uint64_t* max_off,
+                uint64_t* argmax_count);
 /* byte_fixed_decode (split_double.rs:1243-1350) for precision < 0, else
  * byte_residue_decode_with_precision (split_double.rs:1612-1893) at `precision`
- * decimals. out: n doubles (host). */
+ * decimals. out: n doubles (host). A buff-major chunk decodes as
+ * buff_decode_majority (buff_simd.rs:633-861) and writes decoded_rows values
+ * (buff_chunk_info); it has no reduced-precision form (BUFF_E_UNSUPPORTED). */
 int buff_materialize(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags, double* out);
 int buff_materialize_device(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags,
                             double* out_dev);
@@ -184,6 +206,20 @@ int buff_sd_byte_fixed_max(const buff_split_double* sd, const uint8_t* bytes, ui
  * stream (the ilen <= 15 branch; larger ilen is BUFF_E_UNSUPPORTED). */
 int buff_sd_byte_residue_decode_with_precision(const buff_split_double* sd, const uint8_t* bytes, uint64_t len,
                                                uint32_t precision, double* out, uint64_t cap);
+/* buff-major: buff_simd.rs:317 buff_encode_majority (out == NULL: size query),
+ * :633 buff_decode_majority (*n_out = values written; 0 when the stream ends in
+ * sparse columns), :863 buff_range_filter_majority, :1916
+ * buff_equal_filter_majority, :2294 buff_sum_majority, :2444 buff_max_majority. */
+int buff_sd_buff_encode_majority(const buff_split_double* sd, const double* data, uint64_t n, uint8_t* out,
+                                 uint64_t cap, uint64_t* out_len);
+int buff_sd_buff_decode_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* out,
+                                 uint64_t cap, uint64_t* n_out);
+int buff_sd_buff_range_filter_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double pred,
+                                       uint32_t* words, uint64_t nwords, uint64_t* count);
+int buff_sd_buff_equal_filter_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double pred,
+                                       uint32_t* words, uint64_t nwords, uint64_t* count);
+int buff_sd_buff_sum_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* sum);
+int buff_sd_buff_max_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* max_value);
 
 #ifdef __cplusplus
 }
