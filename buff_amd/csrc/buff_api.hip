@@ -557,7 +557,10 @@ int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64
 int max_op(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t* max_off, uint64_t* argmax) {
   if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
   if (argmax) *argmax = 0;
-  if (ch->S > 0) return major_max(ctx, ch, flags, value, max_off, argmax);
+  // buff-major chunks report the tail-stage ties (buff_simd.rs:2856); a chunk
+  // without sparse columns runs the same stages over its planes
+  if (ch->S > 0 || ((ch->flags & BUFF_MAJORITY) && argmax && ch->n > 0))
+    return major_max(ctx, ch, flags, value, max_off, argmax);
   uint64_t best = 0;
   const ChunkView& v = ch->view;
   if (ch->n > 0 && (v.P || v.r)) {
