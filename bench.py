@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--rows", type=int, default=1 << 27, help="rows per column (default 134,217,728)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=1 << 25)
+    ap.add_argument("--no-config3", action="store_true", help="skip the outlier-heavy (buff vs buff-major) section")
+    ap.add_argument("--config3-rows", type=int, default=1 << 26)
     return ap.parse_args()
 
 
@@ -118,6 +120,123 @@ def precision_R(info, p):
     if R // 8 < ((dlen + ilen) & 0xFFFFFFFF) // 8 and R % 8:
         R = (R // 8 + 1) * 8
     return min(R, info.width)
+
+
+def _fixed(x, dlen):
+    return np.trunc(np.ldexp(x, dlen)).astype(np.int64)
+
+
+def major_filter_rows(x, info, majors, pred, dlen):
+    """Rows a buff-major range filter must touch: its candidates entering each
+    dense plane and the tail (the compat rules of buff_simd.rs:863-1277)."""
+    base, F, S = info.base, info.width, info.sparse_cols
+    off = (_fixed(x, dlen) - base).astype(np.uint64)
+    t = int(_fixed(np.array([pred]), dlen)[0]) - base
+    if t < 0:
+        return 0, 0
+    cand = np.ones(len(x), bool)
+    for j in range(S):
+        sh = np.uint64(F - 8 * (j + 1))
+        b = ((off >> sh) & np.uint64(0xFF)).astype(np.int64)
+        tb = (t >> int(sh)) & 0xFF
+        if j > 0 and tb == 0 and majors[j] == 0:
+            cand[:] = False
+        else:
+            cand &= b == tb
+    rows = 0
+    for p in range(info.planes):
+        rows += int(cand.sum())
+        sh = np.uint64(F - 8 * (S + p + 1))
+        cand &= ((off >> sh) & np.uint64(0xFF)).astype(np.int64) == ((t >> int(sh)) & 0xFF)
+    return rows, int(cand.sum()) if info.tail_bits else 0
+
+
+def bench_config3(ctx, args, buff_amd, datagen):
+    """Outlier-heavy column (95% N(100, 0.5), 5% +-10^U[6,9], 3 decimals, seed 4) in
+    the plain and the buff-major layout: encode, range filter at 10%, equality on a
+    majority value and on an outlier value, sum, max, full decode. Per-op device
+    time from the library's HIP events; GB/s of float64 input = 8 n / time."""
+    n = args.config3_rows
+    scale, dlen = 10 ** 3, 11
+    x = datagen.outlier_heavy(n, 3, 4)
+    xb = ctx.alloc(x.nbytes)
+    xb.upload(x)
+    words = ctx.alloc(((n + 63) // 64) * 8)
+    out = ctx.alloc(n * 8)
+    pred10 = datagen.percentile_pred(x, 0.10)
+    in_major = float(x[np.argsort(np.abs(x - 100.0))[n // 3]])
+    outlier = float(x[int(np.flatnonzero(np.abs(x) > 1e5)[7])])
+    reps = 5
+    res = {"workload": "outlier-heavy column, %d float64 rows (95%% N(100,0.5), 5%% +-10^U[6,9]), 3 decimals" % n,
+           "rows": n, "predicates": {"range_10pct": pred10, "equal_majority": in_major, "equal_outlier": outlier}}
+    for fmt, flags in (("buff", 0), ("buff-major", buff_amd.BUFF_MAJORITY)):
+        ops = {
+            "encode": lambda: ctx.encode_device(xb.ptr, n, scale, flags=flags),
+        }
+        ch = ops["encode"]()
+        info = ch.info()
+        majors = [ch.sparse_col(j)[0] for j in range(info.sparse_cols)]
+        n_out = [ch.sparse_col(j)[1] for j in range(info.sparse_cols)]
+        queries = {
+            "range_10pct": lambda: ch.filter_device("range", pred10, words.ptr),
+            "equal_majority": lambda: ch.filter_device("equal", in_major, words.ptr),
+            "equal_outlier": lambda: ch.filter_device("equal", outlier, words.ptr),
+            "sum": lambda: ch.sum(),
+            "max": lambda: ch.max(),
+            "decode": lambda: ch.materialize_device(out.ptr, -1),
+        }
+        fres = {"F": info.width, "sparse_cols": info.sparse_cols, "majors": majors, "outliers": n_out,
+                "planes": info.planes, "tail_bits": info.tail_bits, "device_bytes": info.device_bytes, "ops": {}}
+        for name, fn in [("encode", None)] + list(queries.items()):
+            ctx.synchronize()
+            ctx.prof_reset()
+            ctx.prof_enable(True)
+            t = time.perf_counter()
+            for _ in range(reps):
+                if name == "encode":
+                    ops["encode"]().free()
+                else:
+                    fn()
+            ctx.synchronize()
+            wall = (time.perf_counter() - t) / reps
+            ctx.prof_enable(False)
+            kern = {}
+            for k in ("k_minmax", "k_encode", "k_major_encode", "k_major_scan", "k_major_compact", "k_filter_gt",
+                      "k_filter_eq", "k_major_filter_gt", "k_major_filter_eq", "k_fill_bitmap", "k_sum",
+                      "k_major_sum", "k_max", "k_major_max", "k_materialize", "k_major_decode"):
+                l, ms = ctx.prof_query(k)
+                if l:
+                    kern[k] = round(ms / reps * 1e3, 2)
+            dev_us = sum(kern.values())
+            fres["ops"][name] = {"device_us": round(dev_us, 2), "wall_us": round(wall * 1e6, 1),
+                                 "GBps_f64_input": round(8.0 * n / (dev_us * 1e-6) / 1e9, 1) if dev_us else None,
+                                 "kernels_us": kern}
+        # algorithmic bytes of the two ops the headline metric names
+        S, P, r = info.sparse_cols, info.planes, info.tail_bits
+        sparse_b = S * n / 8.0 + sum(n_out) + S * ((n + 1023) // 1024) * 4.0
+        if fmt == "buff-major":
+            rows_p, rows_t = major_filter_rows(x, info, majors, pred10, dlen)
+            fb = sparse_b + rows_p + rows_t * r / 8.0 + n / 8.0
+            sb = sum(n_out) + n * (P + r / 8.0)
+        else:
+            stats = ctx.alloc(9 * 8)
+            stats.fill(0)
+            ch.filter_device("range", pred10, words.ptr, stats_dev=stats.ptr)
+            st = stats.download(np.uint64, 9)
+            fb = n * (1 + 1 / 8.0) + float(sum(int(st[k]) for k in range(1, P))) + float(st[P]) * r / 8.0
+            sb = n * (P + r / 8.0)
+            stats.free()
+        for name, b in (("range_10pct", fb), ("sum", sb)):
+            us = fres["ops"][name]["device_us"]
+            if us:
+                g = b / (us * 1e-6) / 1e9
+                fres["ops"][name]["roofline"] = {"algorithmic_bytes": round(b), "achieved_GBps": round(g, 1),
+                                                 "frac": round(g / HBM_PEAK_GBPS, 4)}
+        ch.free()
+        res[fmt] = fres
+    for b in (xb, words, out):
+        b.free()
+    return res
 
 
 def main():
@@ -303,6 +422,13 @@ def main():
         "cold_filters": filters,
     }
 
+    if not args.no_config3:
+        for b in (flush, words, out):
+            b.free()
+        for b in dx:
+            b.free()
+        log("bench: outlier-heavy section (%d rows)" % args.config3_rows)
+        result["config3"] = bench_config3(ctx, args, buff_amd, datagen)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_rows)
     print(json.dumps(result), flush=True)

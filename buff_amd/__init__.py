@@ -6,6 +6,7 @@ include/buffgpu.h. This package is the Python binding of that ABI.
 """
 from ._native import (  # noqa: F401
     BUFF_FLIP,
+    BUFF_MAJORITY,
     BUFF_OP_EQ,
     BUFF_OP_GT,
     BUFF_SEMANTIC,
@@ -21,6 +22,6 @@ from ._native import (  # noqa: F401
 from .split_double import SplitBDDoubleCompress  # noqa: F401
 
 __all__ = [
-    "BUFF_FLIP", "BUFF_OP_EQ", "BUFF_OP_GT", "BUFF_SEMANTIC", "BuffError", "Chunk", "Context", "DeviceBuffer",
+    "BUFF_FLIP", "BUFF_MAJORITY", "BUFF_OP_EQ", "BUFF_OP_GT", "BUFF_SEMANTIC", "BuffError", "Chunk", "Context", "DeviceBuffer",
     "OutOfMemory", "RefPanic", "Unsupported", "SplitBDDoubleCompress", "lib",
 ]
