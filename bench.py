@@ -201,7 +201,7 @@ def bench_config3(ctx, args, buff_amd, datagen):
             wall = (time.perf_counter() - t) / reps
             ctx.prof_enable(False)
             kern = {}
-            for k in ("k_minmax", "k_encode", "k_major_encode", "k_major_scan", "k_major_compact", "k_filter_gt",
+            for k in ("k_minmax", "k_encode", "k_major_encode", "k_major_scan", "k_major_place", "k_filter_gt",
                       "k_filter_eq", "k_major_filter_gt", "k_major_filter_eq", "k_fill_bitmap", "k_sum",
                       "k_major_sum", "k_max", "k_major_max", "k_materialize", "k_major_decode"):
                 l, ms = ctx.prof_query(k)

@@ -37,15 +37,23 @@ struct MajorView {
   uint32_t F;                    // full width
 };
 
-// Encode passes. Pass 1 writes the dense planes / tail, the outlier bitmaps and
-// per-tile outlier counts; pass 2 (after the scan) writes the outlier bytes.
+// Encode passes. Workgroup w owns the contiguous tiles [w*tpw, (w+1)*tpw).
+// Pass 1 writes the dense planes / tail, the outlier bitmaps, each lane's
+// outlier bytes packed into its 16-byte staging slot, and per-workgroup outlier
+// totals. A one-workgroup scan turns the totals into bases; pass 2 walks each
+// workgroup's tiles in order, writes the tile prefixes and moves the staged
+// bytes to their final place.
 struct MajorEnc {
   uint32_t S;
+  uint32_t nwg;
+  uint64_t tpw;
   uint32_t major[kMaxSparse];
   uint32_t shift[kMaxSparse];
   uint16_t* mask16[kMaxSparse];
-  uint32_t* counts[kMaxSparse];
-  const uint32_t* prefix[kMaxSparse];
+  uint32_t* counts[kMaxSparse];          // per tile
+  uint8_t* stage[kMaxSparse];            // ntiles * 1024 bytes: 16-byte slot per lane
+  uint32_t* wgsum[kMaxSparse];           // per workgroup: total, then exclusive base
+  uint32_t* prefix[kMaxSparse];
   uint8_t* outl[kMaxSparse];
 };
 
@@ -64,11 +72,16 @@ struct MajorMaxState {
   unsigned long long argmax;
 };
 
-hipError_t launch_major_encode(int pass, const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t base,
+// Pass 1 (x read once); me.nwg / me.tpw set by major_enc_geometry().
+void major_enc_geometry(uint64_t ntiles, MajorEnc* me);
+hipError_t launch_major_encode(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t base,
                                const ChunkView& dense, const MajorEnc& me, hipStream_t s);
-// counts (S arrays of ntiles) -> prefix (S arrays of ntiles+1); totals[2j] = outliers,
-// totals[2j+1] = last outlier row (UINT64_MAX if none).
+// Workgroup totals -> bases; totals[2j] = outliers, totals[2j+1] = last outlier
+// row (UINT64_MAX if none); prefix[j][ntiles] = outliers.
 hipError_t launch_major_scan(const MajorEnc& me, uint64_t ntiles, unsigned long long* totals, hipStream_t s);
+// Pass 2: tile prefixes (per workgroup range), then the outlier bytes moved to
+// their final place (grid-stride over tiles).
+hipError_t launch_major_place(const MajorEnc& me, uint64_t ntiles, hipStream_t s);
 hipError_t launch_major_filter(const MajorView& mv, int op, bool semantic, const MajorTarget& tg, uint64_t* out,
                                uint64_t nwords, hipStream_t s, const LaunchShape& ls);
 hipError_t launch_major_decode(const MajorView& mv, bool semantic, double inv_scale, double* out, hipStream_t s,
@@ -81,7 +94,5 @@ hipError_t launch_major_outlier_sums(const MajorView& mv, const uint64_t* n_out,
 uint64_t major_max_scratch_bytes(uint64_t ntiles);
 hipError_t launch_major_max(const MajorView& mv, MajorMaxState* st, uint8_t* scratch, hipStream_t s,
                             const LaunchShape& ls, bool semantic);
-// Outlier bitmap words of rows >= n are zero; counts per tile from the masks (load path).
-hipError_t launch_major_counts(const MajorEnc& me, uint64_t ntiles, hipStream_t s);
 
 }  // namespace buffgpu

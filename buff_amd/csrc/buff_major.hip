@@ -17,14 +17,25 @@
 
 namespace buffgpu {
 
+// Exclusive prefix sum over the 64 lanes with DPP: Hillis-Steele inside each
+// 16-lane row (row_shr 1, 2, 4, 8), then the row totals carried by row_bcast15
+// (into rows 1 and 3) and row_bcast31 (into rows 2 and 3). Lanes whose DPP
+// source lies outside the row read 0.
 __device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t lane) {
+  (void)lane;
   uint32_t s = v;
-#pragma unroll
-  for (uint32_t o = 1; o < 64; o <<= 1) {
-    const uint32_t u = (uint32_t)__shfl_up((int)s, o, 64);
-    if (lane >= o) s += u;
-  }
+  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x111, 0xf, 0xf, false);   // row_shr:1
+  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x112, 0xf, 0xf, false);   // row_shr:2
+  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x114, 0xf, 0xf, false);   // row_shr:4
+  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x118, 0xf, 0xf, false);   // row_shr:8
+  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x142, 0xa, 0xf, false);   // row_bcast:15
+  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x143, 0xc, 0xf, false);   // row_bcast:31
   return s - v;
+}
+
+__device__ __forceinline__ uint32_t wave_total_u32(uint32_t v) {
+  const uint32_t incl = wave_excl_scan_u32(v, 0) + v;
+  return (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
 }
 
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
@@ -47,20 +58,82 @@ __device__ __forceinline__ void set_byte(uint32_t& d0, uint32_t& d1, uint32_t& d
 
 // The lane's 16 bytes of a sparse column in tile c: the majority byte, with the
 // outlier bytes of the rows in `want` (a subset of m16) patched in. Every lane of
-// the wave calls it together (the exclusive scan).
+// the wave calls it together (the exclusive scan). A lane's outlier bytes are
+// contiguous in the compacted array, so it fetches them as one 20-byte window of
+// aligned dwords (the array is padded past its end) and patches from registers.
 __device__ __forceinline__ uint4 sparse_bytes(const SparseDev& sd, uint64_t c, uint32_t lane, uint32_t m16,
                                               uint32_t want) {
   const uint32_t excl = wave_excl_scan_u32(__popc(m16), lane);
   const uint32_t M = 0x01010101u * sd.major;
   uint32_t d0 = M, d1 = M, d2 = M, d3 = M;
   if (want) {
-    const uint8_t* src = sd.outl + (uint64_t)sd.prefix[c] + excl;
+    const uint64_t pos = (uint64_t)sd.prefix[c] + excl;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(sd.outl + (pos & ~3ull));
+    const uint32_t sh = 8u * (uint32_t)(pos & 3);
+    const uint64_t a = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+    const uint64_t b = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
+    const uint64_t e = (uint64_t)w[4];
+    const uint64_t lo = sh ? (a >> sh) | (b << (64 - sh)) : a;    // outlier bytes 0..7 of this lane
+    const uint64_t hi = sh ? (b >> sh) | (e << (64 - sh)) : b;    // outlier bytes 8..15
     uint32_t bits = want;
     while (bits) {
-      const uint32_t b = (uint32_t)__builtin_ctz(bits);
+      const uint32_t bit = (uint32_t)__builtin_ctz(bits);
       bits &= bits - 1;
-      set_byte(d0, d1, d2, d3, b, src[__popc(m16 & ((1u << b) - 1u))]);
+      const uint32_t k = __popc(m16 & ((1u << bit) - 1u));
+      const uint32_t v = (uint32_t)(((k < 8 ? lo : hi) >> (8 * (k & 7))) & 0xFF);
+      set_byte(d0, d1, d2, d3, bit, v);
     }
+  }
+  return make_uint4(d0, d1, d2, d3);
+}
+
+// Masks and outlier windows of every sparse column of tile c, with all their
+// loads issued before any is consumed (the columns are independent).
+struct SparseLanes {
+  uint32_t m[kMaxSparse];
+  uint64_t lo[kMaxSparse];
+  uint64_t hi[kMaxSparse];
+};
+
+__device__ __forceinline__ void load_sparse(const MajorView& mv, uint64_t c, uint32_t lane, SparseLanes& sl) {
+  uint32_t pre[kMaxSparse];
+#pragma unroll
+  for (uint32_t j = 0; j < kMaxSparse; ++j) {
+    sl.m[j] = j < mv.S ? (uint32_t)mv.col[j].mask16[c * 64 + lane] : 0u;
+    pre[j] = j < mv.S ? mv.col[j].prefix[c] : 0u;
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kMaxSparse; ++j) {
+    sl.lo[j] = 0;
+    sl.hi[j] = 0;
+    if (j < mv.S) {
+      const uint32_t excl = wave_excl_scan_u32(__popc(sl.m[j]), lane);
+      if (sl.m[j]) {
+        const uint64_t pos = (uint64_t)pre[j] + excl;
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(mv.col[j].outl + (pos & ~3ull));
+        const uint32_t sh = 8u * (uint32_t)(pos & 3);
+        const uint64_t a = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+        const uint64_t b = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
+        const uint64_t e = (uint64_t)w[4];
+        sl.lo[j] = sh ? (a >> sh) | (b << (64 - sh)) : a;
+        sl.hi[j] = sh ? (b >> sh) | (e << (64 - sh)) : b;
+      }
+    }
+  }
+}
+
+// Column j's 16 bytes for the lane from SparseLanes: majority byte, outlier
+// bytes patched in for the rows in `want`.
+__device__ __forceinline__ uint4 sparse_word(const SparseLanes& sl, uint32_t j, uint32_t major, uint32_t want) {
+  const uint32_t M = 0x01010101u * major;
+  uint32_t d0 = M, d1 = M, d2 = M, d3 = M;
+  const uint32_t m16 = sl.m[j];
+  uint32_t bits = want & m16;
+  while (bits) {
+    const uint32_t bit = (uint32_t)__builtin_ctz(bits);
+    bits &= bits - 1;
+    const uint32_t k = __popc(m16 & ((1u << bit) - 1u));
+    set_byte(d0, d1, d2, d3, bit, (uint32_t)(((k < 8 ? sl.lo[j] : sl.hi[j]) >> (8 * (k & 7))) & 0xFF));
   }
   return make_uint4(d0, d1, d2, d3);
 }
@@ -74,20 +147,43 @@ __device__ __forceinline__ uint32_t masked_max_plus1(const uint4& w, uint32_t ma
 }
 
 // ----------------------------------------------------------------------------
-// Encode (buff_simd.rs:451-555). Pass 1: dense planes / tail of the low F - 8S
-// bits, outlier bitmaps, per-tile outlier counts. Pass 2: outlier bytes at
-// prefix[c] + lane offset. Both passes convert the float64 input the same way.
+// Encode (buff_simd.rs:451-555). Pass 1 reads the float64 column once and, per
+// sparse column, stores each lane's outlier bytes packed into its own 16-byte
+// slot (one aligned store) next to the outlier bitmap; per-workgroup outlier
+// totals come from per-lane popcounts. The one-workgroup scan makes the totals
+// bases (and finds the last outlier row); pass 2 walks each workgroup's tiles in
+// order and moves the packed bytes to their place in the compacted array.
 
-template <int PASS>
+__device__ __forceinline__ void pack_outliers(const uint64_t (&off)[16], uint32_t sh, uint32_t m, uint64_t& lo,
+                                              uint64_t& hi) {
+  lo = 0;
+  hi = 0;
+  uint32_t k = 0;
+#pragma unroll
+  for (uint32_t b = 0; b < 16; ++b) {
+    const uint64_t v = (off[b] >> sh) & 0xFF;
+    if ((m >> b) & 1u) {
+      if (k < 8) lo |= v << (8 * k);
+      else hi |= v << (8 * (k - 8));
+      ++k;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_major_encode(const double* __restrict__ x, uint64_t n, uint64_t ntiles,
                                                       int32_t pexp, uint32_t dlen, int64_t base, ChunkView dense,
                                                       MajorEnc me) {
   __shared__ uint64_t s_off[4][64 * kLdsRow];
+  __shared__ uint32_t s_cnt[kMaxSparse][4];
   const uint32_t lane = __lane_id();
-  uint64_t* so = s_off[threadIdx.x >> 6];
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t c = wave; c < ntiles; c += nwaves) {
+  const uint32_t wslot = threadIdx.x >> 6;
+  uint64_t* so = s_off[wslot];
+  const uint64_t t0 = (uint64_t)blockIdx.x * me.tpw;
+  const uint64_t t1 = min(t0 + me.tpw, ntiles);
+  uint32_t lcnt[kMaxSparse];
+#pragma unroll
+  for (uint32_t j = 0; j < kMaxSparse; ++j) lcnt[j] = 0;
+  for (uint64_t c = t0 + wslot; c < t1; c += 4) {
     uint64_t v[16];
     const bool full = tile_full(n, c);
     load_tile_f64(x, n, c, lane, full, v);
@@ -102,116 +198,162 @@ __global__ __launch_bounds__(256) void k_major_encode(const double* __restrict__
     for (uint32_t b = 0; b < 16; ++b) off[b] = so[kLdsRow * lane + b];
     wave_lds_sync();
     const uint32_t valid = lane_valid16(n, c, lane);
-    if (PASS == 1) {
-      for (uint32_t p = 0; p < dense.P; ++p) {
-        const uint32_t sh = dense.F - 8 * (p + 1);
-        uint32_t by[16];
+    for (uint32_t p = 0; p < dense.P; ++p) {
+      const uint32_t sh = dense.F - 8 * (p + 1);
+      uint32_t by[16];
 #pragma unroll
-        for (uint32_t b = 0; b < 16; ++b) by[b] = (uint32_t)((off[b] >> sh) & 0xFF);
-        reinterpret_cast<uint4*>(dense.planes + p * dense.pitch + c * kTileRows)[lane] = pack_lane(by);
-      }
-      for (uint32_t j = 0; j < dense.r; ++j) {
-        uint32_t h = 0;
-#pragma unroll
-        for (uint32_t b = 0; b < 16; ++b) h |= (uint32_t)((off[b] >> j) & 1ull) << b;
-        dense.tail[(c * dense.r + j) * 64 + lane] = (uint16_t)h;
-      }
+      for (uint32_t b = 0; b < 16; ++b) by[b] = (uint32_t)((off[b] >> sh) & 0xFF);
+      reinterpret_cast<uint4*>(dense.planes + p * dense.pitch + c * kTileRows)[lane] = pack_lane(by);
     }
+    for (uint32_t j = 0; j < dense.r; ++j) {
+      uint32_t h = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < kMaxSparse; ++j) {
-      if (j >= me.S) break;
+      for (uint32_t b = 0; b < 16; ++b) h |= (uint32_t)((off[b] >> j) & 1ull) << b;
+      dense.tail[(c * dense.r + j) * 64 + lane] = (uint16_t)h;
+    }
+#pragma unroll 1
+    for (uint32_t j = 0; j < me.S; ++j) {
       const uint32_t sh = me.shift[j], mj = me.major[j];
       uint32_t m = 0;
 #pragma unroll
       for (uint32_t b = 0; b < 16; ++b) m |= (uint32_t)(((off[b] >> sh) & 0xFF) != mj) << b;
       m &= valid;
-      if (PASS == 1) {
-        me.mask16[j][c * 64 + lane] = (uint16_t)m;
-        const uint32_t cnt = wave_sum_u32(__popc(m));
-        if (lane == 0) me.counts[j][c] = cnt;
-      } else {
-        const uint32_t excl = wave_excl_scan_u32(__popc(m), lane);
-        uint8_t* dst = me.outl[j] + (uint64_t)me.prefix[j][c] + excl;
-        uint32_t k = 0;
-#pragma unroll
-        for (uint32_t b = 0; b < 16; ++b) {
-          if ((m >> b) & 1u) {
-            dst[k] = (uint8_t)(off[b] >> sh);
-            ++k;
-          }
-        }
+      me.mask16[j][c * 64 + lane] = (uint16_t)m;
+      const uint32_t tile_cnt = wave_total_u32(__popc(m));
+      if (lane == 0) me.counts[j][c] = tile_cnt;
+      if (m) {
+        uint64_t lo, hi;
+        pack_outliers(off, sh, m, lo, hi);
+        reinterpret_cast<uint4*>(me.stage[j] + c * kTileRows)[lane] =
+            make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
       }
+      const uint32_t pc = __popc(m);
+#pragma unroll
+      for (uint32_t jj = 0; jj < kMaxSparse; ++jj) lcnt[jj] += jj == j ? pc : 0u;   // registers, not scratch
+    }
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kMaxSparse; ++j) {
+    if (j >= me.S) break;
+    const uint32_t w = wave_sum_u32(lcnt[j]);
+    if (lane == 0) s_cnt[j][wslot] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < me.S) {
+    const uint32_t j = threadIdx.x;
+    me.wgsum[j][blockIdx.x] = s_cnt[j][0] + s_cnt[j][1] + s_cnt[j][2] + s_cnt[j][3];
+  }
+}
+
+// One workgroup: per column, the workgroup totals become exclusive bases, and
+// the last outlier row is found in the last workgroup range holding outliers.
+__global__ __launch_bounds__(256) void k_major_scan(MajorEnc me, uint64_t ntiles,
+                                                    unsigned long long* __restrict__ totals) {
+  __shared__ uint32_t s_sum[256];
+  __shared__ int32_t s_lastwg[256];
+  __shared__ unsigned long long s_row[256];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t per = (me.nwg + 255) / 256;
+  for (uint32_t j = 0; j < me.S; ++j) {
+    uint32_t* ws = me.wgsum[j];
+    const uint32_t a0 = min(tid * per, me.nwg), a1 = min(a0 + per, me.nwg);
+    uint32_t local = 0;
+    int32_t lastwg = -1;
+    for (uint32_t i = a0; i < a1; ++i) {
+      const uint32_t k = ws[i];
+      local += k;
+      if (k) lastwg = (int32_t)i;
+    }
+    s_sum[tid] = local;
+    s_lastwg[tid] = lastwg;
+    __syncthreads();
+    for (uint32_t o = 1; o < 256; o <<= 1) {
+      const uint32_t a = tid >= o ? s_sum[tid - o] : 0u;
+      __syncthreads();
+      s_sum[tid] += a;
+      __syncthreads();
+    }
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+      if (tid < o) s_lastwg[tid] = max(s_lastwg[tid], s_lastwg[tid + o]);
+      __syncthreads();
+    }
+    uint32_t run = s_sum[tid] - local;
+    for (uint32_t i = a0; i < a1; ++i) {
+      const uint32_t k = ws[i];
+      ws[i] = run;
+      run += k;
+    }
+    // last outlier row: highest set bit in the masks of that workgroup's tiles
+    const int32_t lw = s_lastwg[0];
+    unsigned long long best = 0;   // row + 1
+    if (lw >= 0) {
+      const uint64_t w0 = (uint64_t)lw * me.tpw * 64;
+      const uint64_t w1 = min(((uint64_t)lw + 1) * me.tpw, ntiles) * 64;
+      for (uint64_t w = w0 + tid; w < w1; w += 256) {
+        const uint32_t m = me.mask16[j][w];
+        if (m) best = max(best, (unsigned long long)(16 * w + (32u - (uint32_t)__clz(m))));
+      }
+    }
+    s_row[tid] = best;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+      if (tid < o) s_row[tid] = max(s_row[tid], s_row[tid + o]);
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const uint32_t total = s_sum[255];
+      me.prefix[j][ntiles] = total;
+      totals[2 * j] = total;
+      totals[2 * j + 1] = s_row[0] ? s_row[0] - 1 : ~0ull;
+    }
+    __syncthreads();
+  }
+}
+
+// Pass 2a: workgroup w scans the counts of its tiles (256 at a time) from its base.
+__global__ __launch_bounds__(256) void k_major_tileprefix(MajorEnc me, uint64_t ntiles) {
+  __shared__ uint32_t s_sum[256];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t t0 = (uint64_t)blockIdx.x * me.tpw;
+  const uint64_t t1 = min(t0 + me.tpw, ntiles);
+  for (uint32_t j = 0; j < me.S; ++j) {
+    uint32_t carry = me.wgsum[j][blockIdx.x];
+    for (uint64_t g = t0; g < t1; g += 256) {
+      const uint64_t c = g + tid;
+      const uint32_t cnt = c < t1 ? me.counts[j][c] : 0u;
+      s_sum[tid] = cnt;
+      __syncthreads();
+      for (uint32_t o = 1; o < 256; o <<= 1) {
+        const uint32_t a = tid >= o ? s_sum[tid - o] : 0u;
+        __syncthreads();
+        s_sum[tid] += a;
+        __syncthreads();
+      }
+      if (c < t1) me.prefix[j][c] = carry + s_sum[tid] - cnt;
+      carry += s_sum[255];
+      __syncthreads();
     }
   }
 }
 
-// Per-tile outlier counts from the bitmaps (stream load path).
-__global__ __launch_bounds__(256) void k_major_counts(MajorEnc me, uint64_t ntiles) {
+// Pass 2b: every tile's staged outlier bytes to prefix[c] + lane offset.
+__global__ __launch_bounds__(256) void k_major_copy(MajorEnc me, uint64_t ntiles) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t c = wave; c < ntiles; c += nwaves) {
-#pragma unroll
-    for (uint32_t j = 0; j < kMaxSparse; ++j) {
-      if (j >= me.S) break;
-      const uint32_t cnt = wave_sum_u32(__popc((uint32_t)me.mask16[j][c * 64 + lane]));
-      if (lane == 0) me.counts[j][c] = cnt;
-    }
-  }
-}
-
-// Exclusive scan of one column's tile counts (one workgroup per column), its
-// total and the row of its last outlier.
-__global__ __launch_bounds__(256) void k_major_scan(MajorEnc me, uint64_t ntiles,
-                                                    unsigned long long* __restrict__ totals) {
-  const uint32_t j = blockIdx.x;
-  const uint32_t tid = threadIdx.x;
-  const uint32_t* cnt = me.counts[j];
-  uint32_t* pre = const_cast<uint32_t*>(me.prefix[j]);
-  const uint64_t per = (ntiles + 255) / 256;
-  const uint64_t t0 = min((uint64_t)tid * per, ntiles), t1 = min(t0 + per, ntiles);
-  uint64_t local = 0;
-  int64_t lastc = -1;
-  for (uint64_t c = t0; c < t1; ++c) {
-    const uint32_t k = cnt[c];
-    local += k;
-    if (k) lastc = (int64_t)c;
-  }
-  __shared__ uint64_t s_sum[256];
-  __shared__ int64_t s_last[256];
-  s_sum[tid] = local;
-  s_last[tid] = lastc;
-  __syncthreads();
-  for (uint32_t o = 1; o < 256; o <<= 1) {
-    const uint64_t a = tid >= o ? s_sum[tid - o] : 0;
-    const int64_t l = tid >= o ? s_last[tid - o] : -1;
-    __syncthreads();
-    s_sum[tid] += a;
-    s_last[tid] = max(s_last[tid], l);
-    __syncthreads();
-  }
-  uint64_t run = s_sum[tid] - local;
-  for (uint64_t c = t0; c < t1; ++c) {
-    pre[c] = (uint32_t)run;
-    run += cnt[c];
-  }
-  const uint64_t total = s_sum[255];
-  const int64_t lc = s_last[255];
-  if (tid == 0) pre[ntiles] = (uint32_t)total;
-  if (tid < 64) {
-    uint64_t row = 0;
-    bool have = false;
-    if (lc >= 0) {
-      const uint32_t m = me.mask16[j][(uint64_t)lc * 64 + tid];
+#pragma unroll 1
+    for (uint32_t j = 0; j < me.S; ++j) {
+      const uint32_t m = me.mask16[j][c * 64 + lane];
+      const uint32_t pre = me.prefix[j][c];
+      const uint32_t pc = __popc(m);
+      const uint32_t excl = wave_excl_scan_u32(pc, lane);
       if (m) {
-        have = true;
-        row = (uint64_t)lc * kTileRows + 16ull * tid + (31u - (uint32_t)__clz(m));
+        const uint4 w = reinterpret_cast<const uint4*>(me.stage[j] + c * kTileRows)[lane];
+        const uint64_t lo = (uint64_t)w.x | ((uint64_t)w.y << 32), hi = (uint64_t)w.z | ((uint64_t)w.w << 32);
+        uint8_t* dst = me.outl[j] + pre + excl;
+        for (uint32_t k = 0; k < pc; ++k) dst[k] = (uint8_t)(((k < 8 ? lo : hi) >> (8 * (k & 7))) & 0xFF);
       }
-    }
-    const uint64_t best = wave_max_u64(have ? row + 1 : 0);
-    if (tid == 0) {
-      totals[2 * j] = total;
-      totals[2 * j + 1] = best ? best - 1 : ~0ull;
     }
   }
 }
@@ -228,55 +370,150 @@ __global__ __launch_bounds__(256) void k_major_scan(MajorEnc me, uint64_t ntiles
 //   dense planes and tail as in the plain filter.
 // SEM drops the two column>0 rules (plain digit-wise compare).
 
-template <int MODE, int SEM>
-__global__ __launch_bounds__(256) void k_major_filter(MajorView mv, MajorTarget tg, uint64_t* __restrict__ out,
-                                                      uint64_t nwords) {
+// U tiles per wave iteration, every load of a phase for all of them in flight:
+// masks / tile prefixes / first dense plane, then the outlier windows, then the
+// candidates' later planes and tail. S (sparse columns) is a template argument so
+// the per-column state stays in registers.
+// The lane's packed outlier bytes (k-th outlier of the lane = byte k) as four
+// dwords, from a 20-byte window of aligned loads at its compacted position.
+struct Packed16 {
+  uint32_t d0, d1, d2, d3;
+};
+__device__ __forceinline__ Packed16 load_packed(const uint8_t* outl, uint64_t pos) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(outl + (pos & ~3ull));
+  const uint32_t sh = (uint32_t)(pos & 3);
+  const uint32_t a = w[0], b = w[1], c = w[2], d = w[3], e = w[4];
+  Packed16 p;
+  p.d0 = __builtin_amdgcn_alignbyte(b, a, sh);
+  p.d1 = __builtin_amdgcn_alignbyte(c, b, sh);
+  p.d2 = __builtin_amdgcn_alignbyte(d, c, sh);
+  p.d3 = __builtin_amdgcn_alignbyte(e, d, sh);
+  return p;
+}
+__device__ __forceinline__ uint32_t packed_byte(const Packed16& p, uint32_t k) {
+  const uint32_t i = k >> 2;
+  const uint32_t d = i == 0 ? p.d0 : i == 1 ? p.d1 : i == 2 ? p.d2 : p.d3;
+  return __builtin_amdgcn_ubfe(d, 8 * (k & 3), 8);
+}
+
+// Row masks (bit b <-> row 16*lane + b) of one sparse column against target byte
+// t: outlier rows compared byte by byte from the packed bytes (`want` selects
+// which), majority rows all at once.
+__device__ __forceinline__ void sparse_cmp(const Packed16& pk, uint32_t m16, uint32_t want, uint32_t major, uint32_t t,
+                                           uint32_t& gt, uint32_t& eq) {
+  const uint32_t nm = ~m16 & 0xFFFFu;
+  gt = major > t ? nm : 0u;
+  eq = major == t ? nm : 0u;
+  uint32_t rem = want;
+  while (rem) {
+    const uint32_t pos = (uint32_t)__builtin_ctz(rem);
+    rem &= rem - 1;
+    const uint32_t v = packed_byte(pk, __popc(m16 & ((1u << pos) - 1u)));
+    gt |= (uint32_t)(v > t) << pos;
+    eq |= (uint32_t)(v == t) << pos;
+  }
+}
+
+// U tiles per wave iteration, every load of a phase for all of them in flight:
+// masks / tile prefixes / first dense plane, then the outlier windows, then the
+// candidates' later planes and tail. S (sparse columns) is a template argument so
+// the per-column state stays in registers.
+template <int MODE, int S, int U>
+__global__ __launch_bounds__(256) void k_major_filter(MajorView mv, MajorTarget tg, uint32_t sem,
+                                                      uint64_t* __restrict__ out, uint64_t nwords) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const ChunkView& dv = mv.dense;
   uint16_t* out16 = reinterpret_cast<uint16_t*>(out);
   const uint64_t nout16 = 4 * nwords;
-  for (uint64_t c = wave; c < dv.ntiles; c += nwaves) {
-    const uint32_t valid = lane_valid16(dv.n, c, lane);
-    uint32_t res = 0, cand = valid;
+  const uint64_t ngroups = (dv.ntiles + U - 1) / U;
+  const bool all_outliers = MODE == 0 && !sem;   // the range rules read every outlier byte
+  for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
+    const uint64_t c0 = grp * U;
+    uint32_t m[U][S], pre[U][S];
+    uint4 w0[U];
 #pragma unroll
-    for (uint32_t j = 0; j < kMaxSparse; ++j) {
-      if (j >= mv.S) break;
-      const SparseDev& sd = mv.col[j];
-      const uint32_t m16 = sd.mask16[c * 64 + lane];
-      const uint32_t want = (MODE == 0 && !SEM) ? m16 : (m16 & cand);
-      const uint4 w = sparse_bytes(sd, c, lane, m16, want);
-      const uint32_t t = tg.digit[j];
-      const Digit dg = make_digit(t);
-      if (MODE == 1) {
-        cand &= mask_eq(w, dg);
-        continue;
+    for (int u = 0; u < U; ++u) {
+      const uint64_t c = min(c0 + u, dv.ntiles - 1);
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        m[u][j] = mv.col[j].mask16[c * 64 + lane];
+        pre[u][j] = mv.col[j].prefix[c];
       }
-      const uint32_t gt = mask_gt(w, dg);
-      if (SEM || j == 0) {
-        res |= cand & gt;
-        cand &= mask_eq(w, dg);
-      } else if (t == 0 && sd.major == 0) {
-        res |= cand & m16;
-        cand = 0;
-      } else {
-        res |= m16 & gt;
-        if (sd.major > t) res |= cand & ~m16;
-        cand &= mask_eq(w, dg);
+      w0[u] = dv.P ? load_plane(dv, 0, c, lane) : make_uint4(0, 0, 0, 0);
+    }
+    Packed16 pk[U][S];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const uint32_t excl = wave_excl_scan_u32(__popc(m[u][j]), lane);
+        pk[u][j] = m[u][j] ? load_packed(mv.col[j].outl, (uint64_t)pre[u][j] + excl) : Packed16{0, 0, 0, 0};
+      }
+    }
+    uint32_t res[U], cand[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      res[u] = 0;
+      cand[u] = lane_valid16(dv.n, c0 + u, lane);
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const uint32_t m16 = m[u][j];
+        const uint32_t major = mv.col[j].major;
+        const uint32_t t = tg.digit[j];
+        uint32_t gt, eq;
+        sparse_cmp(pk[u][j], m16, all_outliers ? m16 : (m16 & cand[u]), major, t, gt, eq);
+        if (MODE == 1) {
+          cand[u] &= eq;
+          continue;
+        }
+        if (sem || j == 0) {
+          res[u] |= cand[u] & gt;
+          cand[u] &= eq;
+        } else if (t == 0 && major == 0) {
+          res[u] |= cand[u] & m16;
+          cand[u] = 0;
+        } else {
+          res[u] |= m16 & gt;
+          if (major > t) res[u] |= cand[u] & ~m16;
+          cand[u] &= eq;
+        }
       }
     }
     for (uint32_t p = 0; p < dv.P; ++p) {
-      if (!__ballot(cand != 0)) break;
-      const uint4 w = load_plane(dv, p, c, cand ? lane : 0u);
-      const Digit dg = make_digit(tg.digit[mv.S + p]);
-      if (MODE == 0) res |= cand & mask_gt(w, dg);
-      cand &= mask_eq(w, dg);
+      bool live = false;
+#pragma unroll
+      for (int u = 0; u < U; ++u) live |= cand[u] != 0;
+      if (!__ballot(live)) break;
+      uint4 wp[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        wp[u] = p == 0 ? w0[u] : load_plane(dv, p, min(c0 + u, dv.ntiles - 1), cand[u] ? lane : 0u);
+      const Digit dg = make_digit(tg.digit[S + p]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (MODE == 0) res[u] |= cand[u] & mask_gt(wp[u], dg);
+        cand[u] &= mask_eq(wp[u], dg);
+      }
     }
-    if (dv.r && __ballot(cand != 0)) refine_tail<MODE>(dv, c, tg.tail, lane, res, cand);
-    const uint32_t r = MODE == 0 ? (res & valid) : cand;
-    const uint64_t idx = c * 64 + lane;
-    if (idx < nout16) out16[idx] = (uint16_t)r;
+    if (dv.r) {
+      bool live = false;
+#pragma unroll
+      for (int u = 0; u < U; ++u) live |= cand[u] != 0;
+      if (__ballot(live)) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (c0 + u < dv.ntiles) refine_tail<MODE>(dv, c0 + u, tg.tail, lane, res[u], cand[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t c = c0 + u;
+      const uint32_t r = MODE == 0 ? (res[u] & lane_valid16(dv.n, c, lane)) : cand[u];
+      const uint64_t idx = c * 64 + lane;
+      if (c < dv.ntiles && idx < nout16) out16[idx] = (uint16_t)r;
+    }
   }
 }
 
@@ -295,6 +532,14 @@ __global__ __launch_bounds__(256) void k_major_decode(MajorView mv, double inv_s
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const ChunkView& dv = mv.dense;
   for (uint64_t c = wave; c < dv.ntiles; c += nwaves) {
+    SparseLanes sl;
+    load_sparse(mv, c, lane, sl);
+    uint4 wp[kMaxPlanes];
+    uint32_t h[8];
+#pragma unroll
+    for (uint32_t p = 0; p < kMaxPlanes; ++p) wp[p] = p < dv.P ? load_plane(dv, p, c, lane) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) h[j] = j < dv.r ? load_tail(dv, j, c, lane) : 0u;
     uint64_t fx[16];
 #pragma unroll
     for (uint32_t b = 0; b < 16; ++b) fx[b] = 0;
@@ -303,8 +548,8 @@ __global__ __launch_bounds__(256) void k_major_decode(MajorView mv, double inv_s
     for (uint32_t j = 0; j < kMaxSparse; ++j) {
       if (j >= mv.S) break;
       const SparseDev& sd = mv.col[j];
-      const uint32_t m16 = sd.mask16[c * 64 + lane];
-      const uint4 w = sparse_bytes(sd, c, lane, m16, m16);
+      const uint32_t m16 = sl.m[j];
+      const uint4 w = sparse_word(sl, j, sd.major, m16);
       const uint32_t sh = sd.shift;
       if (SEM || j == 0) {
 #pragma unroll
@@ -318,16 +563,17 @@ __global__ __launch_bounds__(256) void k_major_decode(MajorView mv, double inv_s
         }
       }
     }
-    for (uint32_t p = 0; p < dv.P; ++p) {
-      const uint4 w = load_plane(dv, p, c, lane);
+#pragma unroll
+    for (uint32_t p = 0; p < kMaxPlanes; ++p) {
+      if (p >= dv.P) break;
       const uint32_t sh = dv.F - 8 * (p + 1);
 #pragma unroll
-      for (uint32_t b = 0; b < 16; ++b) fx[b] |= (uint64_t)byte_at(w, b) << sh;
+      for (uint32_t b = 0; b < 16; ++b) fx[b] |= (uint64_t)byte_at(wp[p], b) << sh;
     }
-    for (uint32_t j = 0; j < dv.r; ++j) {
-      const uint32_t h = load_tail(dv, j, c, lane);
 #pragma unroll
-      for (uint32_t b = 0; b < 16; ++b) fx[b] |= (uint64_t)((h >> b) & 1u) << j;
+    for (uint32_t j = 0; j < 8; ++j) {
+#pragma unroll
+      for (uint32_t b = 0; b < 16; ++b) fx[b] |= (uint64_t)((h[j] >> b) & 1u) << j;
     }
 #pragma unroll
     for (uint32_t b = 0; b < 16; ++b) sv[kLdsRow * lane + b] = (double)(int64_t)((uint64_t)dv.base + fx[b]) * inv_scale;
@@ -518,42 +764,65 @@ static inline uint32_t tiles_grid(uint64_t ntiles, uint32_t max_wg) {
   return (uint32_t)wgs;
 }
 
-hipError_t launch_major_encode(int pass, const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t base,
-                               const ChunkView& dense, const MajorEnc& me, hipStream_t s) {
-  const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
-  const uint32_t grid = tiles_grid(ntiles, 4096);
-  if (pass == 1)
-    hipLaunchKernelGGL(k_major_encode<1>, dim3(grid), dim3(256), 0, s, x, n, ntiles, pexp, dlen, base, dense, me);
-  else
-    hipLaunchKernelGGL(k_major_encode<2>, dim3(grid), dim3(256), 0, s, x, n, ntiles, pexp, dlen, base, dense, me);
-  return hipGetLastError();
+void major_enc_geometry(uint64_t ntiles, MajorEnc* me) {
+  uint64_t nwg = (ntiles + 3) / 4;
+  if (nwg > 4096) nwg = 4096;
+  if (nwg < 1) nwg = 1;
+  me->tpw = (ntiles + nwg - 1) / nwg;
+  me->nwg = (uint32_t)((ntiles + me->tpw - 1) / me->tpw);
+  if (me->nwg < 1) me->nwg = 1;
 }
 
-hipError_t launch_major_counts(const MajorEnc& me, uint64_t ntiles, hipStream_t s) {
-  hipLaunchKernelGGL(k_major_counts, dim3(tiles_grid(ntiles, 4096)), dim3(256), 0, s, me, ntiles);
+hipError_t launch_major_encode(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t base,
+                               const ChunkView& dense, const MajorEnc& me, hipStream_t s) {
+  const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
+  hipLaunchKernelGGL(k_major_encode, dim3(me.nwg), dim3(256), 0, s, x, n, ntiles, pexp, dlen, base, dense, me);
   return hipGetLastError();
 }
 
 hipError_t launch_major_scan(const MajorEnc& me, uint64_t ntiles, unsigned long long* totals, hipStream_t s) {
-  hipLaunchKernelGGL(k_major_scan, dim3(me.S), dim3(256), 0, s, me, ntiles, totals);
+  hipLaunchKernelGGL(k_major_scan, dim3(1), dim3(256), 0, s, me, ntiles, totals);
+  return hipGetLastError();
+}
+
+hipError_t launch_major_place(const MajorEnc& me, uint64_t ntiles, hipStream_t s) {
+  hipLaunchKernelGGL(k_major_tileprefix, dim3(me.nwg), dim3(256), 0, s, me, ntiles);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_major_copy, dim3(tiles_grid(ntiles, 4096)), dim3(256), 0, s, me, ntiles);
+  return hipGetLastError();
+}
+
+template <int MODE, int S>
+static void launch_filter_s(const MajorView& mv, bool semantic, const MajorTarget& tg, uint64_t* out,
+                            uint64_t nwords, hipStream_t s, uint32_t max_wg) {
+  constexpr int U = 2;
+  const uint64_t groups = (mv.dense.ntiles + U - 1) / U;
+  const uint32_t grid = tiles_grid(groups, max_wg);
+  hipLaunchKernelGGL((k_major_filter<MODE, S, U>), dim3(grid), dim3(256), 0, s, mv, tg, semantic ? 1u : 0u, out,
+                     nwords);
+}
+
+template <int MODE>
+static hipError_t launch_filter_mode(const MajorView& mv, bool semantic, const MajorTarget& tg, uint64_t* out,
+                                     uint64_t nwords, hipStream_t s, uint32_t max_wg) {
+  switch (mv.S) {
+    case 1: launch_filter_s<MODE, 1>(mv, semantic, tg, out, nwords, s, max_wg); break;
+    case 2: launch_filter_s<MODE, 2>(mv, semantic, tg, out, nwords, s, max_wg); break;
+    case 3: launch_filter_s<MODE, 3>(mv, semantic, tg, out, nwords, s, max_wg); break;
+    case 4: launch_filter_s<MODE, 4>(mv, semantic, tg, out, nwords, s, max_wg); break;
+    case 5: launch_filter_s<MODE, 5>(mv, semantic, tg, out, nwords, s, max_wg); break;
+    case 6: launch_filter_s<MODE, 6>(mv, semantic, tg, out, nwords, s, max_wg); break;
+    case 7: launch_filter_s<MODE, 7>(mv, semantic, tg, out, nwords, s, max_wg); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_major_filter(const MajorView& mv, int op, bool semantic, const MajorTarget& tg, uint64_t* out,
                                uint64_t nwords, hipStream_t s, const LaunchShape& ls) {
-  const uint32_t grid = tiles_grid(mv.dense.ntiles, ls.max_wg);
-  if (op == 0) {
-    if (semantic)
-      hipLaunchKernelGGL((k_major_filter<0, 1>), dim3(grid), dim3(256), 0, s, mv, tg, out, nwords);
-    else
-      hipLaunchKernelGGL((k_major_filter<0, 0>), dim3(grid), dim3(256), 0, s, mv, tg, out, nwords);
-  } else {
-    if (semantic)
-      hipLaunchKernelGGL((k_major_filter<1, 1>), dim3(grid), dim3(256), 0, s, mv, tg, out, nwords);
-    else
-      hipLaunchKernelGGL((k_major_filter<1, 0>), dim3(grid), dim3(256), 0, s, mv, tg, out, nwords);
-  }
-  return hipGetLastError();
+  return op == 0 ? launch_filter_mode<0>(mv, semantic, tg, out, nwords, s, ls.max_wg)
+                 : launch_filter_mode<1>(mv, semantic, tg, out, nwords, s, ls.max_wg);
 }
 
 hipError_t launch_major_decode(const MajorView& mv, bool semantic, double inv_scale, double* out, hipStream_t s,

@@ -23,6 +23,8 @@ namespace {
 constexpr uint64_t kSample = 2000;   // split_double.rs:25 SAMPLE
 // split_double.rs:27 MAJOR_R = 0.6f32; (2000 as f32 * 0.6f32) as i32 == 1200 (buff_simd.rs:393)
 constexpr int32_t kMajorLimit = 1200;
+// readers fetch a lane's outlier bytes as one 20-byte window past its position
+constexpr uint64_t kOutlierPad = 64;
 
 // Boyer-Moore vote over `(offset >> rsf) as u8` (buff_simd.rs:402-412, 424-436).
 void vote(const std::vector<uint64_t>& head, uint32_t rsf, uint8_t* major, int32_t* freq) {
@@ -83,7 +85,7 @@ int sparse_alloc(Ctx* ctx, Chunk* ch) {
 // Outlier bytes of every column, each at a 256-byte aligned offset.
 int outlier_alloc(Ctx* ctx, Chunk* ch) {
   uint64_t tot = 0;
-  for (uint32_t j = 0; j < ch->S; ++j) tot += round_up(ch->sp[j].n_out + 1, 256);
+  for (uint32_t j = 0; j < ch->S; ++j) tot += round_up(ch->sp[j].n_out + kOutlierPad, 256);
   void* p = nullptr;
   BUFF_TRY(ctx->alloc(std::max<uint64_t>(tot, 256), &p));
   ch->mem_outl = p;
@@ -91,48 +93,46 @@ int outlier_alloc(Ctx* ctx, Chunk* ch) {
   uint64_t off = 0;
   for (uint32_t j = 0; j < ch->S; ++j) {
     ch->sp[j].outl = (uint8_t*)p + off;
-    off += round_up(ch->sp[j].n_out + 1, 256);
+    off += round_up(ch->sp[j].n_out + kOutlierPad, 256);
   }
   return BUFF_OK;
 }
 
-MajorEnc enc_of(const Chunk* ch, uint32_t* counts_base) {
+// Encode scratch in ctx->s_major: staging slots and tile counts of every column,
+// workgroup sums, then the per-column totals.
+struct EncScratch {
+  uint64_t counts, stage, wgsum, totals, bytes;
+};
+
+EncScratch enc_layout(uint32_t S, uint64_t ntiles, uint32_t nwg) {
+  EncScratch l;
+  l.stage = 0;
+  l.counts = (uint64_t)S * ntiles * kTileRows;
+  l.wgsum = l.counts + round_up((uint64_t)S * ntiles * 4, 256);
+  l.totals = l.wgsum + round_up((uint64_t)S * nwg * 4, 256);
+  l.bytes = l.totals + 16ull * kMaxSparse;
+  return l;
+}
+
+MajorEnc enc_of(const Chunk* ch, uint8_t* scratch) {
   MajorEnc me;
   memset(&me, 0, sizeof(me));
   me.S = ch->S;
+  major_enc_geometry(ch->view.ntiles, &me);
+  const EncScratch l = enc_layout(ch->S, ch->view.ntiles, me.nwg);
   for (uint32_t j = 0; j < ch->S; ++j) {
     me.major[j] = ch->sp[j].major;
     me.shift[j] = ch->sp[j].shift;
     me.mask16[j] = ch->sp[j].mask16;
-    me.counts[j] = counts_base ? counts_base + (uint64_t)j * ch->view.ntiles : nullptr;
     me.prefix[j] = ch->sp[j].prefix;
     me.outl[j] = ch->sp[j].outl;
+    if (scratch) {
+      me.stage[j] = scratch + l.stage + (uint64_t)j * ch->view.ntiles * kTileRows;
+      me.counts[j] = (uint32_t*)(scratch + l.counts) + (uint64_t)j * ch->view.ntiles;
+      me.wgsum[j] = (uint32_t*)(scratch + l.wgsum) + (uint64_t)j * me.nwg;
+    }
   }
   return me;
-}
-
-// counts -> prefixes, outlier totals and last rows (device), then to the chunk.
-int scan_columns(Ctx* ctx, Chunk* ch, const MajorEnc& me) {
-  unsigned long long* totals =
-      (unsigned long long*)((uint8_t*)ctx->s_major.ptr + round_up((uint64_t)ch->S * ch->view.ntiles * 4, 256));
-  {
-    ProfScope ps(ctx, "k_major_scan");
-    HIP_TRY(launch_major_scan(me, ch->view.ntiles, totals, ctx->stream));
-  }
-  uint64_t t[2 * kMaxSparse];
-  BUFF_TRY(d2h_small(ctx, t, totals, 16ull * ch->S));
-  for (uint32_t j = 0; j < ch->S; ++j) {
-    ch->sp[j].n_out = t[2 * j];
-    ch->sp[j].last = t[2 * j + 1];
-  }
-  return BUFF_OK;
-}
-
-int counts_scratch(Ctx* ctx, const Chunk* ch, uint32_t** counts) {
-  const uint64_t cb = round_up((uint64_t)ch->S * ch->view.ntiles * 4, 256);
-  BUFF_TRY(ctx->scratch(ctx->s_major, cb + 16 * kMaxSparse));
-  *counts = (uint32_t*)ctx->s_major.ptr;
-  return BUFF_OK;
 }
 
 Chunk* new_major_chunk(Ctx* ctx, uint64_t n, uint64_t scale, uint32_t flags, uint32_t dlen, uint32_t ilen, int64_t base,
@@ -254,37 +254,46 @@ int major_encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, u
     ch->sp[j].major = majors[j];
     ch->sp[j].shift = F - 8 * (j + 1);
   }
-  uint32_t* counts = nullptr;
   rc = sparse_alloc(ctx, ch);
-  if (rc == BUFF_OK) rc = counts_scratch(ctx, ch, &counts);
+  MajorEnc me;
+  major_enc_geometry(ch->view.ntiles, &me);
+  const EncScratch lay = enc_layout(S, ch->view.ntiles, me.nwg);
+  if (rc == BUFF_OK) rc = ctx->scratch(ctx->s_major, lay.bytes);
   if (rc != BUFF_OK) {
     free_partial(ctx, ch);
     return rc;
   }
-  MajorEnc me = enc_of(ch, counts);
+  uint8_t* scr = (uint8_t*)ctx->s_major.ptr;
+  me = enc_of(ch, scr);
   hipError_t e;
   {
     ProfScope ps(ctx, "k_major_encode");
-    e = launch_major_encode(1, x, n, pexp, dlen, mn, ch->view, me, ctx->stream);
+    e = launch_major_encode(x, n, pexp, dlen, mn, ch->view, me, ctx->stream);
+    if (e == hipSuccess) e = launch_major_scan(me, ch->view.ntiles, (unsigned long long*)(scr + lay.totals), ctx->stream);
   }
-  if (e != hipSuccess) {
+  uint64_t t[2 * kMaxSparse];
+  if (e == hipSuccess) rc = d2h_small(ctx, t, scr + lay.totals, 16ull * S);
+  if (e != hipSuccess || rc != BUFF_OK) {
     free_partial(ctx, ch);
-    return fail(BUFF_E_DEVICE, "k_major_encode: %s", hipGetErrorString(e));
+    return e != hipSuccess ? fail(BUFF_E_DEVICE, "k_major_encode: %s", hipGetErrorString(e)) : rc;
   }
-  rc = scan_columns(ctx, ch, me);
-  if (rc == BUFF_OK) rc = outlier_alloc(ctx, ch);
+  for (uint32_t j = 0; j < S; ++j) {
+    ch->sp[j].n_out = t[2 * j];
+    ch->sp[j].last = t[2 * j + 1];
+  }
+  rc = outlier_alloc(ctx, ch);
   if (rc != BUFF_OK) {
     free_partial(ctx, ch);
     return rc;
   }
-  me = enc_of(ch, counts);
+  me = enc_of(ch, scr);
   {
-    ProfScope ps(ctx, "k_major_compact");
-    e = launch_major_encode(2, x, n, pexp, dlen, mn, ch->view, me, ctx->stream);
+    ProfScope ps(ctx, "k_major_place");
+    e = launch_major_place(me, ch->view.ntiles, ctx->stream);
   }
   if (e != hipSuccess) {
     free_partial(ctx, ch);
-    return fail(BUFF_E_DEVICE, "k_major_encode pass 2: %s", hipGetErrorString(e));
+    return fail(BUFF_E_DEVICE, "k_major_place: %s", hipGetErrorString(e));
   }
   *out = ch;
   return BUFF_OK;
@@ -466,17 +475,35 @@ int major_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_
   }
   if (rc == BUFF_OK && S) rc = sparse_alloc(ctx, ch);
   if (rc == BUFF_OK && S) rc = outlier_alloc(ctx, ch);
-  uint32_t* counts = nullptr;
-  if (rc == BUFF_OK && S) rc = counts_scratch(ctx, ch, &counts);
   if (rc != BUFF_OK) {
     free_partial(ctx, ch);
     return rc;
+  }
+  // tile prefixes and last outlier rows from the bitmap words
+  std::vector<std::vector<uint32_t>> pre(S);
+  for (uint32_t j = 0; j < S; ++j) {
+    pre[j].assign(ntiles + 1, 0);
+    const uint8_t* wb = cols[j].binary.data() + 8;
+    uint64_t run = 0, last = ~0ull;
+    for (uint64_t c = 0; c < ntiles; ++c) {
+      pre[j][c] = (uint32_t)run;
+      for (uint64_t w = 32 * c; w < std::min<uint64_t>(32 * c + 32, nw); ++w) {
+        uint32_t v;
+        memcpy(&v, wb + 4 * w, 4);
+        run += (uint64_t)__builtin_popcount(v);
+        if (v) last = 32 * w + 31 - (uint64_t)__builtin_clz(v);
+      }
+    }
+    pre[j][ntiles] = (uint32_t)run;
+    ch->sp[j].last = last;
   }
   hipError_t e = hipSuccess;
   for (uint32_t j = 0; j < S && e == hipSuccess; ++j) {
     e = hipMemsetAsync(ch->sp[j].mask16, 0, mask_bytes(ntiles), ctx->stream);
     if (e == hipSuccess && nw)
       e = hipMemcpyAsync(ch->sp[j].mask16, cols[j].binary.data() + 8, 4 * nw, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(ch->sp[j].prefix, pre[j].data(), 4 * (ntiles + 1), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess && cols[j].n_out)
       e = hipMemcpyAsync(ch->sp[j].outl, cols[j].outl, cols[j].n_out, hipMemcpyHostToDevice, ctx->stream);
   }
@@ -490,11 +517,6 @@ int major_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_
                         (n * ch->view.r + 7) / 8, ctx->stream);
       }
     }
-  }
-  if (rc == BUFF_OK && e == hipSuccess && S) {
-    const MajorEnc me = enc_of(ch, counts);
-    e = launch_major_counts(me, ntiles, ctx->stream);
-    if (e == hipSuccess) rc = scan_columns(ctx, ch, me);
   }
   if (e == hipSuccess && rc == BUFF_OK) e = hipStreamSynchronize(ctx->stream);   // host buffers may go away
   if (e != hipSuccess) rc = fail(BUFF_E_DEVICE, "buff-major upload: %s", hipGetErrorString(e));
