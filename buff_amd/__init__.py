@@ -11,6 +11,7 @@ from ._native import (  # noqa: F401
     BUFF_OP_GT,
     BUFF_SEMANTIC,
     BuffError,
+    Batch,
     Chunk,
     Context,
     DeviceBuffer,
@@ -22,6 +23,6 @@ from ._native import (  # noqa: F401
 from .split_double import SplitBDDoubleCompress  # noqa: F401
 
 __all__ = [
-    "BUFF_FLIP", "BUFF_MAJORITY", "BUFF_OP_EQ", "BUFF_OP_GT", "BUFF_SEMANTIC", "BuffError", "Chunk", "Context", "DeviceBuffer",
+    "BUFF_FLIP", "BUFF_MAJORITY", "BUFF_OP_EQ", "BUFF_OP_GT", "BUFF_SEMANTIC", "Batch", "BuffError", "Chunk", "Context", "DeviceBuffer",
     "OutOfMemory", "RefPanic", "Unsupported", "SplitBDDoubleCompress", "lib",
 ]

@@ -64,6 +64,20 @@ class ChunkInfo(ctypes.Structure):
     ]
 
 
+class Query(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_uint32), ("chunk", ctypes.c_uint32), ("pred", ctypes.c_double),
+                ("precision", ctypes.c_int32), ("flags", ctypes.c_uint32)]
+
+
+class QueryResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("reserved", ctypes.c_uint32), ("count", ctypes.c_uint64),
+                ("value", ctypes.c_double), ("exact", ctypes.c_double), ("max_off", ctypes.c_uint64),
+                ("out_offset", ctypes.c_uint64)]
+
+
+BUFF_Q_GT, BUFF_Q_EQ, BUFF_Q_SUM, BUFF_Q_MAX, BUFF_Q_MAT = 0, 1, 2, 3, 4
+
+
 class SplitDouble(ctypes.Structure):
     _fields_ = [("chunksize", ctypes.c_uint64), ("batchsize", ctypes.c_uint64), ("scale", ctypes.c_uint64)]
 
@@ -102,6 +116,13 @@ SIGNATURES = {
     "buff_max_ex": [vp, vp, u32, f64p, u64p, u64p],
     "buff_materialize": [vp, vp, i32, u32, f64p],
     "buff_materialize_device": [vp, vp, i32, u32, vp],
+    "buff_batch_encode_device": [vp, vp, u64p, u64p, u32, u32, P(vp)],
+    "buff_batch_size": [vp, u32p],
+    "buff_batch_chunk": [vp, u32, P(vp)],
+    "buff_batch_free": [vp],
+    "buff_batch_query": [vp, vp, P(Query), u32, P(QueryResult)],
+    "buff_batch_arena": [vp, P(vp), P(vp)],
+    "buff_batch_tune": [vp, vp, P(Query), u32, u32p, u32p, u32p],
     "buff_sd_byte_fixed_encode": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
     "buff_sd_buff_simd256_encode": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
     "buff_sd_byte_fixed_decode": [P(SplitDouble), u8p, u64, f64p, u64],
@@ -156,9 +177,10 @@ def ptr(a, ct):
 class Chunk:
     """A resident column (buff_chunk*)."""
 
-    def __init__(self, ctx, handle):
+    def __init__(self, ctx, handle, borrowed=False):
         self.ctx = ctx
         self.handle = handle
+        self.borrowed = borrowed   # owned by a Batch
 
     def info(self):
         inf = ChunkInfo()
@@ -227,8 +249,80 @@ class Chunk:
         check(lib().buff_materialize_device(self.ctx.handle, self.handle, precision, flags, out_dev))
 
     def free(self):
-        if self.handle:
+        if self.handle and not self.borrowed:
             check(lib().buff_chunk_free(self.handle))
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def make_queries(qs):
+    """qs: iterable of (op, chunk, pred, precision, flags) -> ctypes array of Query."""
+    qs = list(qs)
+    arr = (Query * max(len(qs), 1))()
+    for i, (op, chunk, pred, precision, flags) in enumerate(qs):
+        arr[i] = Query(op, chunk, float(pred), int(precision), int(flags))
+    return arr, len(qs)
+
+
+class Batch:
+    """buff_batch*: chunks encoded together; queries over them run as one graph launch."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+        n = ctypes.c_uint32()
+        check(lib().buff_batch_size(handle, ctypes.byref(n)))
+        self.chunks = []
+        for i in range(n.value):
+            h = ctypes.c_void_p()
+            check(lib().buff_batch_chunk(handle, i, ctypes.byref(h)))
+            self.chunks.append(Chunk(ctx, h.value, borrowed=True))
+
+    def query(self, qs):
+        """Returns a list of QueryResult (see include/buffgpu.h)."""
+        arr, nq = make_queries(qs)
+        res = (QueryResult * max(nq, 1))()
+        check(lib().buff_batch_query(self.ctx.handle, self.handle, arr, nq, res))
+        return [res[i] for i in range(nq)]
+
+    def tune(self, qs):
+        arr, nq = make_queries(qs)
+        g, t, f = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().buff_batch_tune(self.ctx.handle, self.handle, arr, nq, ctypes.byref(g), ctypes.byref(t),
+                                    ctypes.byref(f)))
+        return g.value, t.value, f.value
+
+    def arena(self):
+        b, v = ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().buff_batch_arena(self.handle, ctypes.byref(b), ctypes.byref(v)))
+        return b.value, v.value
+
+    def bitmap(self, result, n):
+        """The filter bitmap of one query as BitVec<u32> words (host copy)."""
+        b, _ = self.arena()
+        nw = (n + 63) // 64
+        out = np.zeros(max(nw, 1), dtype=np.uint64)
+        if nw:
+            check(lib().buff_memcpy_d2h(self.ctx.handle, out.ctypes.data, b + 8 * result.out_offset, 8 * nw))
+        return out.view(np.uint32)[: (n + 31) // 32]
+
+    def values(self, result, n):
+        _, v = self.arena()
+        out = np.zeros(max(n, 1), dtype=np.float64)
+        if n:
+            check(lib().buff_memcpy_d2h(self.ctx.handle, out.ctypes.data, v + 8 * result.out_offset, 8 * n))
+        return out[:n]
+
+    def free(self):
+        if self.handle:
+            for c in self.chunks:
+                c.handle = None
+            check(lib().buff_batch_free(self.handle))
             self.handle = None
 
     def __del__(self):
@@ -305,6 +399,14 @@ class Context:
 
     def alloc(self, nbytes):
         return DeviceBuffer(self, nbytes)
+
+    def batch_encode_device(self, x_dev, row_off, scales, flags=0):
+        ro = np.ascontiguousarray(row_off, dtype=np.uint64)
+        sc = np.ascontiguousarray(scales, dtype=np.uint64)
+        h = ctypes.c_void_p()
+        check(lib().buff_batch_encode_device(self.handle, x_dev, ptr(ro, ctypes.c_uint64), ptr(sc, ctypes.c_uint64),
+                                             len(sc), flags, ctypes.byref(h)))
+        return Batch(self, h.value)
 
     def synchronize(self):
         check(lib().buff_ctx_synchronize(self.handle))

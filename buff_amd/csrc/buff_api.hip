@@ -260,18 +260,10 @@ int d2h_small(Ctx* ctx, void* dst, const void* src, uint64_t bytes) {
 // ---------------------------------------------------------------------------
 // operations
 
-int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_t flags, Chunk** out) {
-  if (!ctx || !out) return fail(BUFF_E_INVALID, "null argument");
-  if (flags & BUFF_MAJORITY) return major_encode_device(ctx, x, n, scale, flags, out);
-  if (n > 0xFFFFFFFFull) return fail(BUFF_E_INVALID, "a chunk holds at most 2^32-1 rows (n is stored as u32)");
-  if (n && !x) return fail(BUFF_E_INVALID, "null input");
-  const int prec = scale == 0 ? 0 : prec_of_scale(scale);
-  const int dl = precision_map(prec);
-  if (dl < 0) return fail(BUFF_E_REF_PANIC, "precision %d (scale %llu) is not in PRECISION_MAP", prec,
-                          (unsigned long long)scale);
-  const uint32_t dlen = (uint32_t)dl;
-  const int32_t pexp = precision_exp_of_prec(prec);
-
+// The running min and `else if` max of split_double.rs:321-330 over n rows of
+// device memory (n == 0: INT64_MAX / INT64_MIN, as the reference's initial values).
+int quirk_minmax(Ctx* ctx, const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t* mn_out,
+                 int64_t* mx_out) {
   int64_t mn = INT64_MAX, mx = INT64_MIN;
   if (n > 0) {
     const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
@@ -302,6 +294,25 @@ int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_
       if (mm.first == INT64_MAX) mx = INT64_MAX;
     }
   }
+  *mn_out = mn;
+  *mx_out = mx;
+  return BUFF_OK;
+}
+
+int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_t flags, Chunk** out) {
+  if (!ctx || !out) return fail(BUFF_E_INVALID, "null argument");
+  if (flags & BUFF_MAJORITY) return major_encode_device(ctx, x, n, scale, flags, out);
+  if (n > 0xFFFFFFFFull) return fail(BUFF_E_INVALID, "a chunk holds at most 2^32-1 rows (n is stored as u32)");
+  if (n && !x) return fail(BUFF_E_INVALID, "null input");
+  const int prec = scale == 0 ? 0 : prec_of_scale(scale);
+  const int dl = precision_map(prec);
+  if (dl < 0) return fail(BUFF_E_REF_PANIC, "precision %d (scale %llu) is not in PRECISION_MAP", prec,
+                          (unsigned long long)scale);
+  const uint32_t dlen = (uint32_t)dl;
+  const int32_t pexp = precision_exp_of_prec(prec);
+
+  int64_t mn = INT64_MAX, mx = INT64_MIN;
+  BUFF_TRY(quirk_minmax(ctx, x, n, pexp, dlen, &mn, &mx));
   const int64_t delta = (int64_t)((uint64_t)mx - (uint64_t)mn);
   const uint32_t F = width_of_delta(delta);
   if (F == 0 && n >= 1 && !(flags & BUFF_SEMANTIC))
@@ -611,13 +622,10 @@ int precision_bits(const Chunk* ch, int32_t precision, uint32_t* R) {
   return BUFF_OK;
 }
 
-int materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* out) {
-  if (!ctx || !ch || (!out && ch->n)) return fail(BUFF_E_INVALID, "null argument");
-  if (ch->flags & BUFF_MAJORITY) {
-    if (precision >= 0)
-      return fail(BUFF_E_UNSUPPORTED, "the reference has no reduced-precision decode of a buff-major stream");
-    if (ch->S > 0) return major_materialize_device(ctx, ch, precision, flags, out, nullptr);
-  }
+// The materialization recipe of a plain chunk: full precision
+// (split_double.rs:1243-1350), the reference's reduced-precision read
+// (1612-1893) or, with BUFF_SEMANTIC, the top R stored bits re-scaled.
+int mat_plan(const Chunk* ch, int32_t precision, uint32_t flags, MatParams* out) {
   const ChunkView& v = ch->view;
   MatParams mp;
   memset(&mp, 0, sizeof(mp));
@@ -661,9 +669,22 @@ int materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t fl
       mp.shift_back = v.F - R;
     }
   }
+  *out = mp;
+  return BUFF_OK;
+}
+
+int materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* out) {
+  if (!ctx || !ch || (!out && ch->n)) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->flags & BUFF_MAJORITY) {
+    if (precision >= 0)
+      return fail(BUFF_E_UNSUPPORTED, "the reference has no reduced-precision decode of a buff-major stream");
+    if (ch->S > 0) return major_materialize_device(ctx, ch, precision, flags, out, nullptr);
+  }
+  MatParams mp;
+  BUFF_TRY(mat_plan(ch, precision, flags, &mp));
   if (ch->n == 0) return BUFF_OK;
   ProfScope sc(ctx, "k_materialize");
-  HIP_TRY(launch_materialize(v, mp, out, ctx->stream, ctx->shape(Op::Materialize, v.ntiles)));
+  HIP_TRY(launch_materialize(ch->view, mp, out, ctx->stream, ctx->shape(Op::Materialize, ch->view.ntiles)));
   return BUFF_OK;
 }
 
@@ -904,6 +925,7 @@ int buff_chunk_load(buff_ctx* ctx, const uint8_t* stream, uint64_t len, uint64_t
 int buff_chunk_free(buff_chunk* chunk) {
   if (!chunk) return BUFF_OK;
   Chunk* ch = reinterpret_cast<Chunk*>(chunk);
+  if (ch->borrowed) return fail(BUFF_E_INVALID, "this chunk belongs to a batch: free the batch");
   Ctx* c = ch->ctx;
   (void)hipStreamSynchronize(c->stream);
   major_release(ch);

@@ -1,0 +1,102 @@
+// Batched execution over many resident chunks of uneven length and mixed
+// precision: one persistent launch encodes all chunks, and a batch of queries
+// (range / equality filters, sums, maxima, materializations) runs as a fixed
+// sequence of persistent launches that read their work lists from device memory,
+// so the sequence is captured once in a hipGraph and replayed for every batch.
+//
+// Work is cut into items: (chunk or query, tile range). A workgroup takes one
+// item at a time (its waves stride over the item's tiles); item sizes and grid
+// sizes are what the autotuner chooses.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "buff_common.h"
+#include "buff_kernels.h"
+
+namespace buffgpu {
+
+// ---- batch encode -------------------------------------------------------------
+
+struct BEncChunk {
+  const double* x;     // first row of the chunk
+  ChunkView view;      // planes / tail in the batch pool (n, ntiles, base, F, P, r, dlen)
+  int32_t pexp;
+  uint32_t item0;      // first item of the chunk
+  uint32_t nitems;
+  uint32_t pad;
+};
+
+struct BItem {
+  uint32_t owner;      // chunk (encode) or query (queries)
+  uint32_t pad;
+  uint64_t c0, c1;     // tile range [c0, c1)
+};
+
+// Per-chunk min/max of the `else if` pass (split_double.rs:321-330): min over all
+// rows, max over rows >= 1, and the fixed value of row 0.
+struct BMinMax {
+  int64_t min_all;
+  int64_t max_rest;
+  int64_t first;
+  int64_t pad;
+};
+
+hipError_t launch_batch_minmax(const BEncChunk* chunks, uint32_t nchunks, const BItem* items, uint32_t nitems,
+                               int64_t* item_part, BMinMax* out, uint32_t grid, hipStream_t s);
+hipError_t launch_batch_encode(const BEncChunk* chunks, const BItem* items, uint32_t nitems, uint32_t grid,
+                               hipStream_t s);
+
+// ---- batch queries ------------------------------------------------------------
+
+enum : uint32_t { kBqGt = 0, kBqEq = 1, kBqSum = 2, kBqMax = 3, kBqMat = 4 };
+
+struct BQuery {
+  uint32_t op;
+  uint32_t chunk;        // index into the batch's chunk views
+  uint32_t fill;         // filters: 0 = run, 1 = no rows, 2 = all rows (decided on the host)
+  uint32_t item0, nitems;
+  uint32_t pad;
+  FilterTarget tg;       // filters
+  MatParams mp;          // materialize
+  uint64_t out;          // filters: u64 word offset in the bitmap arena; materialize: double offset;
+                         // max: byte offset of its per-tile top bytes in the max arena
+};
+
+// Per-query results the host reads back.
+struct BResult {
+  unsigned long long v[kMaxPlanes + 1];   // sum: per-sub-column sums; filter: v[0] = count; max: v[0] = offset
+};
+
+// Device-side counters of the current batch (written with the descriptors).
+struct BHeader {
+  uint32_t nq, nitems;
+  uint32_t pad[2];
+};
+
+struct BatchBuffers {
+  const BHeader* hdr;
+  const ChunkView* views;
+  const BQuery* q;
+  const BItem* items;
+  uint64_t* bitmaps;                 // filter output arena (u64 words)
+  double* values;                    // materialize output arena
+  uint8_t* tile_top;                 // max: per-tile top byte
+  unsigned long long* item_acc;      // per item: kMaxPlanes + 1 u64
+  BResult* res;
+};
+
+// The first filter pass reads `first_planes` (1 or 2) byte planes of every tile
+// before refining; `grid` is the persistent grid size.
+struct BatchTuning {
+  uint32_t grid;
+  uint32_t first_planes;
+  uint32_t tiles_per_item;
+  uint32_t pad;
+};
+
+// The fixed launch sequence of one batch (captured in a graph by the host). The
+// kernels read every buffer address and count from `bb` (device memory), so one
+// captured graph serves every batch of the same tuning.
+hipError_t launch_batch_queries(const BatchBuffers* bb, const BatchTuning& tu, hipStream_t s);
+
+}  // namespace buffgpu

@@ -73,6 +73,9 @@ struct Ctx {
   std::map<std::string, std::pair<uint64_t, double>> prof_stats;
 
   std::map<uint64_t, LaunchShape> tuned;
+  // batched queries: persistent grid, tiles per work item, planes of the first filter pass
+  uint32_t bt_grid = 2048, bt_tiles = 16, bt_first = 1;
+  bool bt_tuned = false;
 
   ~Ctx();
   int alloc(uint64_t bytes, void** out);
@@ -118,12 +121,22 @@ struct Chunk {
   void* mem_outl = nullptr;    // compacted outlier bytes
   SparseCol sp[8];
   uint64_t device_bytes = 0;
+  bool borrowed = false;       // a view into a batch's pool (freed with the batch)
 };
 
 int d2h_small(Ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int chunk_alloc(Ctx* ctx, Chunk* ch);
 void chunk_set_width(Chunk* ch, uint32_t F);
 int default_ctx(Ctx** out);
+int quirk_minmax(Ctx* ctx, const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t* mn, int64_t* mx);
+bool filter_plan(const Chunk* ch, int op, double pred, uint32_t flags, FilterTarget* tg, int* fill);
+double reference_sum_fold(const Chunk* ch, const uint64_t* ps);
+int mat_plan(const Chunk* ch, int32_t precision, uint32_t flags, MatParams* out);
+int filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, uint64_t* words,
+                  unsigned long long* stats);
+int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64_t* plane_sums);
+int max_op(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t* max_off, uint64_t* argmax);
+int materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* out);
 
 // buff-major (buff_major_api.hip)
 int major_encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_t flags, Chunk** out);

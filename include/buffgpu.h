@@ -164,6 +164,62 @@ int buff_materialize(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, 
 int buff_materialize_device(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags,
                             double* out_dev);
 
+/* ---- batched execution ------------------------------------------------------
+ * Many chunks of uneven length and mixed precision encoded in one persistent
+ * launch (plus one min/max launch), and queries over them run as one captured
+ * hipGraph per batch. There is no reference counterpart: the reference runs one
+ * method call per chunk (the drivers of buff_simd.rs:167-311). Every result equals
+ * the single-chunk call's on the same chunk. */
+typedef struct buff_batch buff_batch;
+
+/* Encodes nchunks columns laid end to end in device memory: chunk i is rows
+ * [row_off[i], row_off[i+1]) of x_dev with scale scales[i] (host arrays). The
+ * chunks share one HBM pool owned by the batch. flags: BUFF_FLIP. A chunk on
+ * which byte_fixed_encode would panic fails the call (buff_last_error names it). */
+int buff_batch_encode_device(buff_ctx* ctx, const double* x_dev, const uint64_t* row_off, const uint64_t* scales,
+                             uint32_t nchunks, uint32_t flags, buff_batch** out);
+int buff_batch_size(const buff_batch* batch, uint32_t* nchunks);
+/* Chunk i as a buff_chunk usable with every single-chunk call; it belongs to the
+ * batch (buff_chunk_free refuses it). */
+int buff_batch_chunk(buff_batch* batch, uint32_t i, buff_chunk** chunk);
+int buff_batch_free(buff_batch* batch);
+
+#define BUFF_Q_GT 0  /* x > pred */
+#define BUFF_Q_EQ 1  /* x == pred */
+#define BUFF_Q_SUM 2
+#define BUFF_Q_MAX 3
+#define BUFF_Q_MAT 4 /* materialize at `precision` decimals (-1: full) */
+
+typedef struct {
+  uint32_t op;       /* BUFF_Q_* */
+  uint32_t chunk;    /* index in the batch */
+  double pred;       /* filters */
+  int32_t precision; /* BUFF_Q_MAT */
+  uint32_t flags;    /* BUFF_SEMANTIC */
+} buff_query;
+
+typedef struct {
+  int32_t status;      /* what the single-chunk call returns for this query */
+  uint32_t reserved;
+  uint64_t count;      /* filters: rows set */
+  double value;        /* sum: the reference f64 sum; max: the max value */
+  double exact;        /* sum: exact sum */
+  uint64_t max_off;    /* max: max stored offset */
+  uint64_t out_offset; /* filters: u64 word offset of the bitmap in the bitmap arena;
+                          materialize: double offset of the values in the value arena */
+} buff_query_result;
+
+/* Runs nq queries on one batch: host planning, one graph launch, one result
+ * copy. Filter bitmaps (ceil(n/64) u64 words each) and materialized values stay
+ * in device arenas that buff_batch_arena exposes until the next call. */
+int buff_batch_query(buff_ctx* ctx, buff_batch* batch, const buff_query* qs, uint32_t nq, buff_query_result* results);
+int buff_batch_arena(buff_batch* batch, const uint64_t** bitmaps_dev, const double** values_dev);
+/* Times candidate shapes (persistent grid size, tiles per work item, byte planes
+ * read by the first filter pass) on this query mix and keeps the fastest for the
+ * context's later batches. */
+int buff_batch_tune(buff_ctx* ctx, buff_batch* batch, const buff_query* qs, uint32_t nq, uint32_t* grid,
+                    uint32_t* tiles_per_item, uint32_t* first_planes);
+
 /* ---- reference-shaped entry points (byte streams in host memory) ----------
  * Each takes the SplitBDDoubleCompress fields {chunksize, batchsize, scale}
  * (split_double.rs:29-39) and the stream as ptr+len, mirrors one method, and
