@@ -71,6 +71,9 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=1 << 25)
     ap.add_argument("--no-config3", action="store_true", help="skip the outlier-heavy (buff vs buff-major) section")
     ap.add_argument("--config3-rows", type=int, default=1 << 26)
+    ap.add_argument("--no-config4", action="store_true", help="skip the batched-form section")
+    ap.add_argument("--config4-chunks", type=int, default=4096)
+    ap.add_argument("--config4-queries", type=int, default=16384)
     return ap.parse_args()
 
 
@@ -236,6 +239,100 @@ def bench_config3(ctx, args, buff_amd, datagen):
         res[fmt] = fres
     for b in (xb, words, out):
         b.free()
+    return res
+
+
+def bench_config4(ctx, args, buff_amd):
+    """Batched form: chunks with lengths log-uniform in [1K, 4M], each a seeded
+    random walk (x0 = 0, N(0,1) steps, chunk seed 5 + i) rounded to a per-chunk
+    precision uniform in {1..6}, generated on the host and uploaded before any
+    timed region. One batch encode (a min/max
+    launch and an encode launch over every tile of every chunk), then a batch of
+    mixed queries (1/4 each equality, range, sum, max; predicates are values of a
+    random row of the queried chunk) as one captured graph launch, with the shapes
+    the batch autotuner picks, against the same queries issued one call per query."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from buff_amd import datagen
+
+    nchunks, nq = args.config4_chunks, args.config4_queries
+    rng = np.random.default_rng(5)
+    lens = np.floor(np.exp(rng.uniform(np.log(1024), np.log(4 * 1024 * 1024), nchunks))).astype(np.int64)
+    precs = rng.integers(1, 7, nchunks)
+    row_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    total = int(row_off[-1])
+    xh = np.empty(total, dtype=np.float64)
+    t0 = time.time()
+
+    def gen(i):
+        a, b = int(row_off[i]), int(row_off[i + 1])
+        xh[a:b] = datagen.random_walk(b - a, int(precs[i]), 5 + i)
+
+    with ThreadPoolExecutor(16) as ex:
+        list(ex.map(gen, range(nchunks)))
+    xd = ctx.alloc(total * 8)
+    xd.upload(xh)
+    log("bench: batched data %d chunks, %d rows (%.1f GB) in %.1fs" % (nchunks, total, total * 8 / 1e9,
+                                                                      time.time() - t0))
+    scales = (10 ** precs).astype(np.uint64)
+    res = {"workload": "%d chunks, lengths log-uniform in [1024, 4194304], seeded random walks at 1..6 decimals" %
+                       nchunks, "chunks": nchunks, "rows": total}
+    # ---- one-launch batch encode
+    times = []
+    batch = None
+    for rep in range(3):
+        if batch is not None:
+            batch.free()
+        ctx.synchronize()
+        t = time.perf_counter()
+        batch = ctx.batch_encode_device(xd.ptr, row_off, scales)
+        ctx.synchronize()
+        times.append(time.perf_counter() - t)
+    enc = min(times)
+    res["encode"] = {"seconds": round(enc, 5), "GBps_f64_input": round(total * 8 / enc / 1e9, 1)}
+    # ---- the query batch
+    qrng = np.random.default_rng(6)
+    qchunk = qrng.integers(0, nchunks, nq)
+    qrow = (row_off[qchunk] + (qrng.random(nq) * lens[qchunk]).astype(np.uint64)).astype(np.int64)
+    preds = xh[qrow]
+    ops = np.array([buff_amd._native.BUFF_Q_EQ, buff_amd._native.BUFF_Q_GT, buff_amd._native.BUFF_Q_SUM,
+                    buff_amd._native.BUFF_Q_MAX])[qrng.integers(0, 4, nq)]
+    qs = [(int(o), int(c), float(p), -1, 0) for o, c, p in zip(ops, qchunk, preds)]
+    scanned = float(sum(int(lens[c]) for c in qchunk)) * 8.0
+    tuned = batch.tune(qs[: min(nq, 2048)])
+    res["autotuner"] = {"grid": tuned[0], "tiles_per_item": tuned[1], "first_filter_planes": tuned[2]}
+    bt = []
+    for rep in range(5):
+        ctx.synchronize()
+        t = time.perf_counter()
+        out = batch.query(qs)
+        bt.append(time.perf_counter() - t)
+    bt.sort()
+    assert all(r.status == 0 for r in out)
+    res["batched"] = {"queries": nq, "seconds": round(bt[len(bt) // 2], 5), "best_seconds": round(bt[0], 5),
+                      "queries_per_s": round(nq / bt[len(bt) // 2], 1),
+                      "GBps_f64_scanned": round(scanned / bt[len(bt) // 2] / 1e9, 1)}
+    # ---- the same queries, one call each
+    words = ctx.alloc(((int(lens.max()) + 63) // 64) * 8)
+    ctx.synchronize()
+    t = time.perf_counter()
+    for (op, c, p, _, _) in qs:
+        ch = batch.chunks[c]
+        if op == buff_amd._native.BUFF_Q_GT:
+            ch.filter_device("range", p, words.ptr)
+        elif op == buff_amd._native.BUFF_Q_EQ:
+            ch.filter_device("equal", p, words.ptr)
+        elif op == buff_amd._native.BUFF_Q_SUM:
+            ch.sum()
+        else:
+            ch.max()
+    ctx.synchronize()
+    single = time.perf_counter() - t
+    res["one_call_per_query"] = {"seconds": round(single, 5), "queries_per_s": round(nq / single, 1)}
+    res["batched_speedup"] = round(single / bt[len(bt) // 2], 2)
+    words.free()
+    batch.free()
+    xd.free()
     return res
 
 
@@ -429,6 +526,9 @@ def main():
             b.free()
         log("bench: outlier-heavy section (%d rows)" % args.config3_rows)
         result["config3"] = bench_config3(ctx, args, buff_amd, datagen)
+    if not args.no_config4:
+        log("bench: batched section (%d chunks, %d queries)" % (args.config4_chunks, args.config4_queries))
+        result["config4"] = bench_config4(ctx, args, buff_amd)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_rows)
     print(json.dumps(result), flush=True)
