@@ -17,6 +17,7 @@ from ._native import (  # noqa: F401
     DeviceBuffer,
     OutOfMemory,
     RefPanic,
+    Server,
     Unsupported,
     lib,
 )
@@ -24,5 +25,5 @@ from .split_double import SplitBDDoubleCompress  # noqa: F401
 
 __all__ = [
     "BUFF_FLIP", "BUFF_MAJORITY", "BUFF_OP_EQ", "BUFF_OP_GT", "BUFF_SEMANTIC", "Batch", "BuffError", "Chunk", "Context", "DeviceBuffer",
-    "OutOfMemory", "RefPanic", "Unsupported", "SplitBDDoubleCompress", "lib",
+    "OutOfMemory", "RefPanic", "Server", "Unsupported", "SplitBDDoubleCompress", "lib",
 ]

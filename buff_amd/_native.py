@@ -78,6 +78,10 @@ class QueryResult(ctypes.Structure):
 BUFF_Q_GT, BUFF_Q_EQ, BUFF_Q_SUM, BUFF_Q_MAX, BUFF_Q_MAT = 0, 1, 2, 3, 4
 
 
+class ServerCfg(ctypes.Structure):
+    _fields_ = [("max_batch", ctypes.c_uint32), ("slots", ctypes.c_uint32), ("arena_bytes", ctypes.c_uint64)]
+
+
 class SplitDouble(ctypes.Structure):
     _fields_ = [("chunksize", ctypes.c_uint64), ("batchsize", ctypes.c_uint64), ("scale", ctypes.c_uint64)]
 
@@ -123,6 +127,13 @@ SIGNATURES = {
     "buff_batch_query": [vp, vp, P(Query), u32, P(QueryResult)],
     "buff_batch_arena": [vp, P(vp), P(vp)],
     "buff_batch_tune": [vp, vp, P(Query), u32, u32p, u32p, u32p],
+    "buff_server_create": [vp, P(ServerCfg), P(vp)],
+    "buff_server_add_chunk": [vp, vp, u32p],
+    "buff_server_submit": [vp, P(Query), vp, u64p],
+    "buff_server_wait": [vp, u64, P(QueryResult), f64p],
+    "buff_server_run_trace": [vp, P(Query), f64p, P(vp), u32, P(QueryResult), f64p],
+    "buff_server_stats": [vp, u64p, u64p, u64p],
+    "buff_server_destroy": [vp],
     "buff_sd_byte_fixed_encode": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
     "buff_sd_buff_simd256_encode": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
     "buff_sd_byte_fixed_decode": [P(SplitDouble), u8p, u64, f64p, u64],
@@ -308,14 +319,14 @@ class Batch:
         nw = (n + 63) // 64
         out = np.zeros(max(nw, 1), dtype=np.uint64)
         if nw:
-            check(lib().buff_memcpy_d2h(self.ctx.handle, out.ctypes.data, b + 8 * result.out_offset, 8 * nw))
+            check(lib().buff_memcpy_d2h(self.ctx.handle, out.ctypes.data, b + result.out_offset, 8 * nw))
         return out.view(np.uint32)[: (n + 31) // 32]
 
     def values(self, result, n):
         _, v = self.arena()
         out = np.zeros(max(n, 1), dtype=np.float64)
         if n:
-            check(lib().buff_memcpy_d2h(self.ctx.handle, out.ctypes.data, v + 8 * result.out_offset, 8 * n))
+            check(lib().buff_memcpy_d2h(self.ctx.handle, out.ctypes.data, v + result.out_offset, 8 * n))
         return out[:n]
 
     def free(self):
@@ -328,6 +339,63 @@ class Batch:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """buff_server*: continuous batching over resident chunks (see include/buffgpu.h)."""
+
+    def __init__(self, ctx, max_batch=0, slots=0, arena_bytes=0):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        cfg = ServerCfg(max_batch, slots, arena_bytes)
+        check(lib().buff_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h.value
+
+    def add_chunk(self, chunk):
+        i = ctypes.c_uint32()
+        check(lib().buff_server_add_chunk(self.handle, chunk.handle, ctypes.byref(i)))
+        return i.value
+
+    def submit(self, op, chunk, pred=0.0, precision=-1, flags=0, out_dev=None):
+        q = Query(op, chunk, float(pred), int(precision), int(flags))
+        t = ctypes.c_uint64()
+        check(lib().buff_server_submit(self.handle, ctypes.byref(q), out_dev, ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        r = QueryResult()
+        lat = ctypes.c_double()
+        check(lib().buff_server_wait(self.handle, ticket, ctypes.byref(r), ctypes.byref(lat)))
+        return r, lat.value
+
+    def run_trace(self, qs, arrival_s, outs=None):
+        """Open-loop replay: request i submitted at arrival_s[i] seconds. Returns (results, latencies_us)."""
+        arr, n = make_queries(qs)
+        t = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        res = (QueryResult * max(n, 1))()
+        lat = np.zeros(max(n, 1), dtype=np.float64)
+        o = None
+        if outs is not None:
+            o = (ctypes.c_void_p * max(n, 1))(*[ctypes.c_void_p(v) if v else None for v in outs])
+        check(lib().buff_server_run_trace(self.handle, arr, ptr(t, ctypes.c_double), o, n, res,
+                                          ptr(lat, ctypes.c_double)))
+        return [res[i] for i in range(n)], lat[:n]
+
+    def stats(self):
+        b, q, m = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().buff_server_stats(self.handle, ctypes.byref(b), ctypes.byref(q), ctypes.byref(m)))
+        return b.value, q.value, m.value
+
+    def destroy(self):
+        if self.handle:
+            check(lib().buff_server_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
         except Exception:
             pass
 

@@ -58,8 +58,9 @@ struct BQuery {
   uint32_t pad;
   FilterTarget tg;       // filters
   MatParams mp;          // materialize
-  uint64_t out;          // filters: u64 word offset in the bitmap arena; materialize: double offset;
-                         // max: byte offset of its per-tile top bytes in the max arena
+  uint64_t out;          // filters: device address of the ceil(n/64) bitmap words;
+                         // materialize: device address of the n doubles
+  uint64_t top;          // max: byte offset of its per-tile top bytes in the max arena
 };
 
 // Per-query results the host reads back.
@@ -78,8 +79,6 @@ struct BatchBuffers {
   const ChunkView* views;
   const BQuery* q;
   const BItem* items;
-  uint64_t* bitmaps;                 // filter output arena (u64 words)
-  double* values;                    // materialize output arena
   uint8_t* tile_top;                 // max: per-tile top byte
   unsigned long long* item_acc;      // per item: kMaxPlanes + 1 u64
   BResult* res;

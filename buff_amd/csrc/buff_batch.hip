@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_bq_tiles(const BatchBuffers* __restrict
 #pragma unroll
     for (uint32_t k = 0; k <= kMaxPlanes; ++k) acc[k] = 0;
     if (op == kBqGt || op == kBqEq) {
-      uint16_t* out16 = reinterpret_cast<uint16_t*>(bb.bitmaps + q.out);
+      uint16_t* out16 = reinterpret_cast<uint16_t*>(q.out);
       for (uint64_t c = it.c0 + wave; c < it.c1; c += 4) {
         uint32_t r;
         if (q.fill == 1) r = 0;
@@ -242,14 +242,14 @@ __global__ __launch_bounds__(256) void k_bq_tiles(const BatchBuffers* __restrict
       uint32_t best = 0;
       for (uint64_t c = it.c0 + wave; c < it.c1; c += 4) {
         const uint32_t tb = tile_top_byte(cv, c, lane);
-        if (lane == 0) bb.tile_top[q.out + c] = (uint8_t)tb;
+        if (lane == 0) bb.tile_top[q.top + c] = (uint8_t)tb;
         best = max(best, tb);
       }
       acc[0] = best;
     } else {   // materialize
       double* sv = s_val[wave];
       const MatParams& mp = q.mp;
-      double* outv = bb.values + q.out;
+      double* outv = reinterpret_cast<double*>(q.out);
       for (uint64_t c = it.c0 + wave; c < it.c1; c += 4) {
         if (mp.mode == kMatRefPrecision) {
           // the reference's reduced-precision read: fields at stride tail_bits of the packed tail
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void k_bq_max_refine(const BatchBuffers* __res
     const uint32_t top = (uint32_t)bb.res[it.owner].v[1];
     unsigned long long best = 0;    // max offset + 1
     for (uint64_t c = it.c0 + wave; c < it.c1; c += 4) {
-      if (bb.tile_top[q.out + c] != top) continue;
+      if (bb.tile_top[q.top + c] != top) continue;
       const uint64_t m = tile_offset_max(cv, c, lane);
       best = max(best, (unsigned long long)m + 1);
     }

@@ -5,9 +5,14 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <mutex>
 #include <new>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/buffgpu.h"
@@ -47,18 +52,31 @@ void drop(Ctx* ctx, DevBuf& b) {
 
 }  // namespace
 
+// One in-flight query batch: descriptor / result buffers, the captured graph
+// per tuning, its stream and completion event. A Batch uses one on the
+// context's stream; a Server runs several on streams of its own.
+struct QueryExec {
+  Ctx* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  hipEvent_t done = nullptr;
+  DevBuf hdr, q, items, acc, res, top, arena, bbdev;
+  BatchBuffers bb{};
+  bool bb_valid = false;
+  std::map<uint32_t, hipGraphExec_t> graphs;   // by grid * 4 + first_planes
+  std::vector<BQuery> hq;
+  std::vector<BItem> hitems;
+  std::vector<BResult> hres;
+  std::vector<buff_query_result> res_host;
+  BHeader h{};
+  uint32_t nq = 0;
+};
+
 struct Batch {
   Ctx* ctx = nullptr;
   std::vector<Chunk*> chunks;
   DevBuf pool, views;
-  // query machinery
-  DevBuf hdr, q, items, acc, res, bitmaps, values, top, bbdev;
-  BatchBuffers bb{};
-  bool bb_valid = false;
-  std::map<uint32_t, hipGraphExec_t> graphs;   // by persistent grid size
-  std::vector<BQuery> hq;
-  std::vector<BItem> hitems;
-  std::vector<BResult> hres;
+  QueryExec ex;
 };
 
 namespace {
@@ -214,53 +232,92 @@ int batch_encode(Ctx* ctx, const double* x, const uint64_t* row_off, const uint6
 // ---------------------------------------------------------------------------
 // batched queries
 
-int graph_for(Batch* b, uint32_t grid, const BatchTuning& tu, hipGraphExec_t* out) {
-  auto it = b->graphs.find(grid * 4 + tu.first_planes);
-  if (it != b->graphs.end()) {
+int exec_init(QueryExec& ex, Ctx* ctx, bool own_stream) {
+  ex.ctx = ctx;
+  if (own_stream) {
+    HIP_TRY(hipStreamCreateWithFlags(&ex.stream, hipStreamNonBlocking));
+    ex.own_stream = true;
+  } else {
+    ex.stream = ctx->stream;
+  }
+  HIP_TRY(hipEventCreateWithFlags(&ex.done, hipEventDisableTiming));
+  return BUFF_OK;
+}
+
+void exec_free(QueryExec& ex) {
+  if (!ex.ctx) return;
+  (void)hipStreamSynchronize(ex.stream);
+  for (auto& kv : ex.graphs) (void)hipGraphExecDestroy(kv.second);
+  ex.graphs.clear();
+  for (DevBuf* d : {&ex.hdr, &ex.q, &ex.items, &ex.acc, &ex.res, &ex.top, &ex.arena, &ex.bbdev}) drop(ex.ctx, *d);
+  if (ex.done) (void)hipEventDestroy(ex.done);
+  if (ex.own_stream) (void)hipStreamDestroy(ex.stream);
+  ex.done = nullptr;
+  ex.stream = nullptr;
+}
+
+int graph_for(QueryExec& ex, const BatchTuning& tu, hipGraphExec_t* out) {
+  const uint32_t key = tu.grid * 4 + tu.first_planes;
+  auto it = ex.graphs.find(key);
+  if (it != ex.graphs.end()) {
     *out = it->second;
     return BUFF_OK;
   }
-  Ctx* ctx = b->ctx;
   hipGraph_t g = nullptr;
-  HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = launch_batch_queries((const BatchBuffers*)b->bbdev.p, tu, ctx->stream);
-  hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+  HIP_TRY(hipStreamBeginCapture(ex.stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = launch_batch_queries((const BatchBuffers*)ex.bbdev.p, tu, ex.stream);
+  hipError_t e2 = hipStreamEndCapture(ex.stream, &g);
   if (e != hipSuccess || e2 != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
     return fail(BUFF_E_DEVICE, "batch graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
   }
-  hipGraphExec_t ex = nullptr;
-  e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  hipGraphExec_t gx = nullptr;
+  e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return fail(BUFF_E_DEVICE, "batch graph instantiate: %s", hipGetErrorString(e));
-  b->graphs[grid * 4 + tu.first_planes] = ex;
-  *out = ex;
+  ex.graphs[key] = gx;
+  *out = gx;
   return BUFF_OK;
 }
 
-int batch_query(Ctx* ctx, Batch* b, const buff_query* qs, uint32_t nq, buff_query_result* results,
-                const BatchTuning& tu, float* graph_ms) {
-  if (!ctx || !b || (nq && (!qs || !results))) return fail(BUFF_E_INVALID, "null argument");
-  if (b->ctx != ctx) return fail(BUFF_E_INVALID, "the batch belongs to another context");
+// Bytes a query needs in the arena when the caller gives no output address.
+uint64_t arena_need(const Chunk* ch, const buff_query& in) {
+  if (in.op == BUFF_Q_GT || in.op == BUFF_Q_EQ) return round_up(((ch->n + 63) / 64) * 8, 256);
+  if (in.op == BUFF_Q_MAT) return round_up(ch->n * 8, 256);
+  return 0;
+}
+
+// Plans and launches nq queries (asynchronously on ex.stream); outs[k] is the
+// query's output address or 0 for the arena. Results come from exec_finish.
+int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* views_dev, const buff_query* qs,
+                uint32_t nq, const uint64_t* outs, const BatchTuning& tu) {
+  Ctx* ctx = ex.ctx;
   const uint64_t T = std::max<uint32_t>(tu.tiles_per_item, 1);
-  b->hq.assign(nq, BQuery{});
-  b->hitems.clear();
-  uint64_t bit_off = 0, val_off = 0, top_off = 0;
+  ex.hq.assign(nq, BQuery{});
+  ex.hitems.clear();
+  ex.res_host.assign(nq, buff_query_result{});
+  ex.nq = nq;
+  uint64_t arena_off = 0, top_off = 0;
+  std::vector<uint64_t> arena_at(nq, ~0ull);
   for (uint32_t k = 0; k < nq; ++k) {
     const buff_query& in = qs[k];
-    buff_query_result& r = results[k];
-    memset(&r, 0, sizeof(r));
-    BQuery& q = b->hq[k];
+    buff_query_result& r = ex.res_host[k];
+    BQuery& q = ex.hq[k];
     memset(&q, 0, sizeof(q));
     q.op = kBqSum;   // a query that fails planning gets no items
-    if (in.chunk >= b->chunks.size()) {
+    if (in.chunk >= chunks.size()) {
       r.status = fail(BUFF_E_INVALID, "query %u: chunk %u outside the batch", k, in.chunk);
       continue;
     }
-    const Chunk* ch = b->chunks[in.chunk];
+    const Chunk* ch = chunks[in.chunk];
     q.chunk = in.chunk;
     const ChunkView& v = ch->view;
     bool tiles = ch->n > 0;
+    const uint64_t need = (outs && outs[k]) ? 0 : arena_need(ch, in);
+    if (need) {
+      arena_at[k] = arena_off;
+      arena_off += need;
+    }
     switch (in.op) {
       case BUFF_Q_GT:
       case BUFF_Q_EQ: {
@@ -268,9 +325,6 @@ int batch_query(Ctx* ctx, Batch* b, const buff_query* qs, uint32_t nq, buff_quer
         int fill = 0;
         const bool run = filter_plan(ch, in.op == BUFF_Q_GT ? BUFF_OP_GT : BUFF_OP_EQ, in.pred, in.flags, &q.tg, &fill);
         q.fill = run ? 0 : (fill ? 2 : 1);
-        q.out = bit_off;
-        r.out_offset = bit_off;
-        bit_off += (ch->n + 63) / 64;
         break;
       }
       case BUFF_Q_SUM:
@@ -280,7 +334,7 @@ int batch_query(Ctx* ctx, Batch* b, const buff_query* qs, uint32_t nq, buff_quer
       case BUFF_Q_MAX:
         q.op = kBqMax;
         tiles = tiles && (v.P || v.r);
-        q.out = top_off;
+        q.top = top_off;
         top_off += v.ntiles;
         break;
       case BUFF_Q_MAT: {
@@ -292,78 +346,69 @@ int batch_query(Ctx* ctx, Batch* b, const buff_query* qs, uint32_t nq, buff_quer
           break;
         }
         q.op = kBqMat;
-        q.out = val_off;
-        r.out_offset = val_off;
-        val_off += ch->n;
         break;
       }
       default:
         r.status = fail(BUFF_E_INVALID, "query %u: unknown op %u", k, in.op);
         tiles = false;
     }
-    q.item0 = (uint32_t)b->hitems.size();
+    q.item0 = (uint32_t)ex.hitems.size();
     if (tiles)
-      for (uint64_t c = 0; c < v.ntiles; c += T) b->hitems.push_back(BItem{k, 0, c, std::min<uint64_t>(c + T, v.ntiles)});
-    q.nitems = (uint32_t)b->hitems.size() - q.item0;
+      for (uint64_t c = 0; c < v.ntiles; c += T) ex.hitems.push_back(BItem{k, 0, c, std::min<uint64_t>(c + T, v.ntiles)});
+    q.nitems = (uint32_t)ex.hitems.size() - q.item0;
   }
-  const uint32_t nitems = (uint32_t)b->hitems.size();
-  BUFF_TRY(ensure(ctx, b->hdr, sizeof(BHeader)));
-  BUFF_TRY(ensure(ctx, b->q, sizeof(BQuery) * std::max<uint32_t>(nq, 1)));
-  BUFF_TRY(ensure(ctx, b->items, sizeof(BItem) * std::max<uint32_t>(nitems, 1)));
-  BUFF_TRY(ensure(ctx, b->acc, 8ull * (kMaxPlanes + 1) * std::max<uint32_t>(nitems, 1)));
-  BUFF_TRY(ensure(ctx, b->res, sizeof(BResult) * std::max<uint32_t>(nq, 1)));
-  BUFF_TRY(ensure(ctx, b->bitmaps, 8 * bit_off));
-  BUFF_TRY(ensure(ctx, b->values, 8 * val_off));
-  BUFF_TRY(ensure(ctx, b->top, top_off));
-  BUFF_TRY(ensure(ctx, b->bbdev, sizeof(BatchBuffers)));
+  const uint32_t nitems = (uint32_t)ex.hitems.size();
+  BUFF_TRY(ensure(ctx, ex.hdr, sizeof(BHeader)));
+  BUFF_TRY(ensure(ctx, ex.q, sizeof(BQuery) * std::max<uint32_t>(nq, 1)));
+  BUFF_TRY(ensure(ctx, ex.items, sizeof(BItem) * std::max<uint32_t>(nitems, 1)));
+  BUFF_TRY(ensure(ctx, ex.acc, 8ull * (kMaxPlanes + 1) * std::max<uint32_t>(nitems, 1)));
+  BUFF_TRY(ensure(ctx, ex.res, sizeof(BResult) * std::max<uint32_t>(nq, 1)));
+  BUFF_TRY(ensure(ctx, ex.top, top_off));
+  BUFF_TRY(ensure(ctx, ex.arena, arena_off));
+  BUFF_TRY(ensure(ctx, ex.bbdev, sizeof(BatchBuffers)));
+  for (uint32_t k = 0; k < nq; ++k) {
+    if (ex.res_host[k].status != BUFF_OK) continue;
+    const uint64_t addr = (outs && outs[k]) ? outs[k] : (uint64_t)ex.arena.p + (arena_at[k] == ~0ull ? 0 : arena_at[k]);
+    ex.hq[k].out = addr;
+    ex.res_host[k].out_offset = addr;
+  }
   BatchBuffers bb;
-  bb.hdr = (const BHeader*)b->hdr.p;
-  bb.views = (const ChunkView*)b->views.p;
-  bb.q = (const BQuery*)b->q.p;
-  bb.items = (const BItem*)b->items.p;
-  bb.bitmaps = (uint64_t*)b->bitmaps.p;
-  bb.values = (double*)b->values.p;
-  bb.tile_top = (uint8_t*)b->top.p;
-  bb.item_acc = (unsigned long long*)b->acc.p;
-  bb.res = (BResult*)b->res.p;
-  if (!b->bb_valid || memcmp(&bb, &b->bb, sizeof(bb)) != 0) {
-    HIP_TRY(hipMemcpyAsync(b->bbdev.p, &bb, sizeof(bb), hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    b->bb = bb;
-    b->bb_valid = true;
+  bb.hdr = (const BHeader*)ex.hdr.p;
+  bb.views = (const ChunkView*)views_dev;
+  bb.q = (const BQuery*)ex.q.p;
+  bb.items = (const BItem*)ex.items.p;
+  bb.tile_top = (uint8_t*)ex.top.p;
+  bb.item_acc = (unsigned long long*)ex.acc.p;
+  bb.res = (BResult*)ex.res.p;
+  if (!ex.bb_valid || memcmp(&bb, &ex.bb, sizeof(bb)) != 0) {
+    HIP_TRY(hipMemcpyAsync(ex.bbdev.p, &bb, sizeof(bb), hipMemcpyHostToDevice, ex.stream));
+    HIP_TRY(hipStreamSynchronize(ex.stream));   // bb is a stack copy
+    ex.bb = bb;
+    ex.bb_valid = true;
   }
   if (nq == 0) return BUFF_OK;
-  const BHeader h{nq, nitems, {0, 0}};
-  HIP_TRY(hipMemcpyAsync(b->hdr.p, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(b->q.p, b->hq.data(), sizeof(BQuery) * nq, hipMemcpyHostToDevice, ctx->stream));
+  ex.h = BHeader{nq, nitems, {0, 0}};
+  HIP_TRY(hipMemcpyAsync(ex.hdr.p, &ex.h, sizeof(ex.h), hipMemcpyHostToDevice, ex.stream));
+  HIP_TRY(hipMemcpyAsync(ex.q.p, ex.hq.data(), sizeof(BQuery) * nq, hipMemcpyHostToDevice, ex.stream));
   if (nitems)
-    HIP_TRY(hipMemcpyAsync(b->items.p, b->hitems.data(), sizeof(BItem) * nitems, hipMemcpyHostToDevice, ctx->stream));
-  hipGraphExec_t ex = nullptr;
-  BUFF_TRY(graph_for(b, tu.grid, tu, &ex));
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  if (graph_ms) {
-    HIP_TRY(hipEventCreate(&ev[0]));
-    HIP_TRY(hipEventCreate(&ev[1]));
-    HIP_TRY(hipEventRecord(ev[0], ctx->stream));
-  }
-  {
-    ProfScope ps(ctx, "k_batch_graph");
-    HIP_TRY(hipGraphLaunch(ex, ctx->stream));
-  }
-  if (graph_ms) HIP_TRY(hipEventRecord(ev[1], ctx->stream));
-  b->hres.resize(nq);
-  HIP_TRY(hipMemcpyAsync(b->hres.data(), b->res.p, sizeof(BResult) * nq, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));   // hq / hitems are reused by the next call
-  if (graph_ms) {
-    (void)hipEventElapsedTime(graph_ms, ev[0], ev[1]);
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
-  }
-  for (uint32_t k = 0; k < nq; ++k) {
-    buff_query_result& r = results[k];
-    if (r.status != BUFF_OK || qs[k].chunk >= b->chunks.size()) continue;
-    const Chunk* ch = b->chunks[qs[k].chunk];
-    const BResult& br = b->hres[k];
+    HIP_TRY(hipMemcpyAsync(ex.items.p, ex.hitems.data(), sizeof(BItem) * nitems, hipMemcpyHostToDevice, ex.stream));
+  hipGraphExec_t gx = nullptr;
+  BUFF_TRY(graph_for(ex, tu, &gx));
+  HIP_TRY(hipGraphLaunch(gx, ex.stream));
+  ex.hres.resize(nq);
+  HIP_TRY(hipMemcpyAsync(ex.hres.data(), ex.res.p, sizeof(BResult) * nq, hipMemcpyDeviceToHost, ex.stream));
+  HIP_TRY(hipEventRecord(ex.done, ex.stream));
+  return BUFF_OK;
+}
+
+// Waits for the batch and turns the device results into buff_query_result.
+int exec_finish(QueryExec& ex, const std::vector<Chunk*>& chunks, const buff_query* qs) {
+  HIP_TRY(hipEventSynchronize(ex.done));
+  for (uint32_t k = 0; k < ex.nq; ++k) {
+    buff_query_result& r = ex.res_host[k];
+    if (r.status != BUFF_OK || qs[k].chunk >= chunks.size()) continue;
+    const Chunk* ch = chunks[qs[k].chunk];
+    const BResult& br = ex.hres[k];
     switch (qs[k].op) {
       case BUFF_Q_GT:
       case BUFF_Q_EQ:
@@ -391,17 +436,6 @@ int batch_query(Ctx* ctx, Batch* b, const buff_query* qs, uint32_t nq, buff_quer
   return BUFF_OK;
 }
 
-void batch_free(Batch* b) {
-  Ctx* ctx = b->ctx;
-  (void)hipStreamSynchronize(ctx->stream);
-  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
-  for (Chunk* ch : b->chunks) delete ch;
-  for (DevBuf* d : {&b->pool, &b->views, &b->hdr, &b->q, &b->items, &b->acc, &b->res, &b->bitmaps, &b->values,
-                    &b->top, &b->bbdev})
-    drop(ctx, *d);
-  delete b;
-}
-
 BatchTuning ctx_tuning(const Ctx* ctx) {
   BatchTuning t;
   t.grid = ctx->bt_grid;
@@ -411,8 +445,59 @@ BatchTuning ctx_tuning(const Ctx* ctx) {
   return t;
 }
 
-// Times every candidate on the caller's query mix (best of three graph runs
-// each) and keeps the fastest.
+// Batch queries: results for filters / materializations point into the batch's
+// arena (out_offset = byte offset from the arena base).
+int batch_query(Ctx* ctx, Batch* b, const buff_query* qs, uint32_t nq, buff_query_result* results,
+                const BatchTuning& tu, float* graph_ms) {
+  if (!ctx || !b || (nq && (!qs || !results))) return fail(BUFF_E_INVALID, "null argument");
+  if (b->ctx != ctx) return fail(BUFF_E_INVALID, "the batch belongs to another context");
+  if (!b->ex.ctx) BUFF_TRY(exec_init(b->ex, ctx, false));
+  hipEvent_t ev0 = nullptr;
+  if (graph_ms) {
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventRecord(ev0, b->ex.stream));
+  }
+  int rc;
+  {
+    ProfScope ps(ctx, "k_batch_graph");
+    rc = exec_launch(b->ex, b->chunks, b->views.p, qs, nq, nullptr, tu);
+  }
+  if (rc == BUFF_OK) rc = exec_finish(b->ex, b->chunks, qs);
+  if (graph_ms) {
+    if (rc == BUFF_OK) {
+      hipEvent_t ev1;
+      (void)hipEventCreate(&ev1);
+      (void)hipEventRecord(ev1, b->ex.stream);
+      (void)hipEventSynchronize(ev1);
+      (void)hipEventElapsedTime(graph_ms, ev0, ev1);
+      (void)hipEventDestroy(ev1);
+    }
+    (void)hipEventDestroy(ev0);
+  }
+  if (rc != BUFF_OK) return rc;
+  for (uint32_t k = 0; k < nq; ++k) {
+    results[k] = b->ex.res_host[k];
+    if (results[k].status == BUFF_OK &&
+        (qs[k].op == BUFF_Q_GT || qs[k].op == BUFF_Q_EQ || qs[k].op == BUFF_Q_MAT))
+      results[k].out_offset -= (uint64_t)b->ex.arena.p;
+    else
+      results[k].out_offset = 0;
+  }
+  return BUFF_OK;
+}
+
+void batch_free(Batch* b) {
+  Ctx* ctx = b->ctx;
+  (void)hipStreamSynchronize(ctx->stream);
+  exec_free(b->ex);
+  for (Chunk* ch : b->chunks) delete ch;
+  drop(ctx, b->pool);
+  drop(ctx, b->views);
+  delete b;
+}
+
+// Times every candidate on the caller's query mix (best of three runs each)
+// and keeps the fastest.
 int batch_tune(Ctx* ctx, Batch* b, const buff_query* qs, uint32_t nq, BatchTuning* best_out) {
   std::vector<buff_query_result> tmp(nq);
   BatchTuning best = ctx_tuning(ctx);
@@ -440,6 +525,157 @@ int batch_tune(Ctx* ctx, Batch* b, const buff_query* qs, uint32_t nq, BatchTunin
   ctx->bt_tuned = true;
   *best_out = best;
   return BUFF_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Server: resident chunks, a request queue and a worker thread doing continuous
+// batching. While one batch runs on the GPU the worker plans the next from the
+// requests that arrived meanwhile and launches it on another stream, so up to
+// `slots` batches overlap on the device. A request joins a batch when a slot is
+// free and the batch still has room: at most max_batch queries, and arena bytes
+// for its output when the caller gave no output address.
+
+struct Req {
+  buff_query q;
+  uint64_t out = 0;
+  double t_submit = 0, t_done = 0;
+  buff_query_result res{};
+  bool done = false;
+};
+
+struct Server {
+  Ctx* ctx = nullptr;
+  buff_server_cfg cfg{};
+  std::vector<Chunk*> chunks;
+  DevBuf views;
+  bool views_dirty = false;
+  std::vector<QueryExec> slots;
+  std::vector<std::vector<uint64_t>> slot_tickets;
+  std::vector<std::vector<buff_query>> slot_qs;
+  std::vector<std::vector<Chunk*>> slot_chunks;   // the chunk list a slot's batch was planned on
+  std::vector<bool> slot_busy;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<uint64_t> pending;
+  std::unordered_map<uint64_t, Req> reqs;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  int error = BUFF_OK;
+  std::thread worker;
+  uint64_t batches = 0, queries = 0, max_seen = 0;
+};
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void server_loop(Server* sv) {
+  const BatchTuning tu = ctx_tuning(sv->ctx);
+  const uint32_t nslots = (uint32_t)sv->slots.size();
+  std::unique_lock<std::mutex> lk(sv->mu);
+  uint32_t rr = 0;
+  while (true) {
+    bool progressed = false;
+    // completions
+    for (uint32_t s = 0; s < nslots; ++s) {
+      if (!sv->slot_busy[s]) continue;
+      if (hipEventQuery(sv->slots[s].done) != hipSuccess) continue;
+      lk.unlock();
+      int rc = exec_finish(sv->slots[s], sv->slot_chunks[s], sv->slot_qs[s].data());
+      const double t = now_s();
+      lk.lock();
+      if (rc != BUFF_OK) sv->error = rc;
+      for (uint32_t k = 0; k < sv->slot_tickets[s].size(); ++k) {
+        Req& r = sv->reqs[sv->slot_tickets[s][k]];
+        r.res = sv->slots[s].res_host[k];
+        if (rc != BUFF_OK) r.res.status = rc;
+        r.t_done = t;
+        r.done = true;
+      }
+      sv->slot_busy[s] = false;
+      sv->cv_done.notify_all();
+      progressed = true;
+    }
+    if (sv->stop && sv->pending.empty()) {
+      bool any = false;
+      for (uint32_t s = 0; s < nslots; ++s) any |= sv->slot_busy[s];
+      if (!any) break;
+    }
+    // launches: a free slot takes what fits
+    for (uint32_t i = 0; i < nslots && !sv->pending.empty(); ++i) {
+      const uint32_t s = (rr + i) % nslots;
+      if (sv->slot_busy[s]) continue;
+      std::vector<uint64_t>& tk = sv->slot_tickets[s];
+      std::vector<buff_query>& qv = sv->slot_qs[s];
+      std::vector<uint64_t> outs;
+      tk.clear();
+      qv.clear();
+      uint64_t arena = 0;
+      while (!sv->pending.empty() && tk.size() < sv->cfg.max_batch) {
+        const uint64_t id = sv->pending.front();
+        Req& r = sv->reqs[id];
+        const uint64_t need =
+            (r.out || r.q.chunk >= sv->chunks.size()) ? 0 : arena_need(sv->chunks[r.q.chunk], r.q);
+        if (!tk.empty() && arena + need > sv->cfg.arena_bytes) break;   // waits for memory to free up
+        arena += need;
+        sv->pending.pop_front();
+        tk.push_back(id);
+        qv.push_back(r.q);
+        outs.push_back(r.out);
+      }
+      const bool vdirty = sv->views_dirty;
+      sv->views_dirty = false;
+      sv->slot_chunks[s] = sv->chunks;
+      const std::vector<Chunk*>& chunks = sv->slot_chunks[s];
+      lk.unlock();
+      int rc = BUFF_OK;
+      if (vdirty) {
+        std::vector<ChunkView> v(chunks.size());
+        for (size_t k = 0; k < v.size(); ++k) v[k] = chunks[k]->view;
+        for (uint32_t o = 0; o < nslots; ++o) (void)hipStreamSynchronize(sv->slots[o].stream);
+        rc = ensure(sv->ctx, sv->views, sizeof(ChunkView) * std::max<size_t>(v.size(), 1));
+        if (rc == BUFF_OK && !v.empty() &&
+            hipMemcpy(sv->views.p, v.data(), sizeof(ChunkView) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
+          rc = fail(BUFF_E_DEVICE, "server views upload");
+      }
+      if (rc == BUFF_OK)
+        rc = exec_launch(sv->slots[s], chunks, sv->views.p, qv.data(), (uint32_t)qv.size(), outs.data(), tu);
+      lk.lock();
+      if (rc != BUFF_OK) {
+        sv->error = rc;
+        const double t = now_s();
+        for (uint64_t id : tk) {
+          Req& r = sv->reqs[id];
+          r.res.status = rc;
+          r.t_done = t;
+          r.done = true;
+        }
+        sv->cv_done.notify_all();
+      } else {
+        sv->slot_busy[s] = true;
+        sv->batches += 1;
+        sv->queries += tk.size();
+        sv->max_seen = std::max<uint64_t>(sv->max_seen, tk.size());
+      }
+      progressed = true;
+      rr = s + 1;
+    }
+    if (!progressed) {
+      bool busy = false;
+      for (uint32_t s = 0; s < nslots; ++s) busy |= sv->slot_busy[s];
+      if (busy) {
+        lk.unlock();
+        std::this_thread::yield();   // poll the in-flight batches
+        lk.lock();
+      } else {
+        sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty(); });
+      }
+    }
+  }
 }
 
 }  // namespace
@@ -487,8 +723,8 @@ int buff_batch_query(buff_ctx* ctx, buff_batch* batch, const buff_query* qs, uin
 int buff_batch_arena(buff_batch* batch, const uint64_t** bitmaps_dev, const double** values_dev) {
   if (!batch) return fail(BUFF_E_INVALID, "null argument");
   Batch* b = reinterpret_cast<Batch*>(batch);
-  if (bitmaps_dev) *bitmaps_dev = (const uint64_t*)b->bitmaps.p;
-  if (values_dev) *values_dev = (const double*)b->values.p;
+  if (bitmaps_dev) *bitmaps_dev = (const uint64_t*)b->ex.arena.p;
+  if (values_dev) *values_dev = (const double*)b->ex.arena.p;
   return BUFF_OK;
 }
 
@@ -501,6 +737,117 @@ int buff_batch_tune(buff_ctx* ctx, buff_batch* batch, const buff_query* qs, uint
   if (grid) *grid = t.grid;
   if (tiles_per_item) *tiles_per_item = t.tiles_per_item;
   if (first_planes) *first_planes = t.first_planes;
+  return BUFF_OK;
+}
+
+int buff_server_create(buff_ctx* ctx, const buff_server_cfg* cfg, buff_server** out) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !out) return fail(BUFF_E_INVALID, "null argument");
+  Server* sv = new (std::nothrow) Server();
+  if (!sv) return fail(BUFF_E_OOM, "host allocation");
+  sv->ctx = c;
+  sv->cfg.max_batch = cfg && cfg->max_batch ? cfg->max_batch : 4096;
+  sv->cfg.slots = cfg && cfg->slots ? std::min<uint32_t>(cfg->slots, 8) : 2;
+  sv->cfg.arena_bytes = cfg && cfg->arena_bytes ? cfg->arena_bytes : (1ull << 30);
+  sv->slots.resize(sv->cfg.slots);
+  sv->slot_tickets.resize(sv->cfg.slots);
+  sv->slot_qs.resize(sv->cfg.slots);
+  sv->slot_chunks.resize(sv->cfg.slots);
+  sv->slot_busy.assign(sv->cfg.slots, false);
+  for (auto& ex : sv->slots) {
+    int rc = exec_init(ex, c, true);
+    if (rc != BUFF_OK) {
+      for (auto& e2 : sv->slots) exec_free(e2);
+      delete sv;
+      return rc;
+    }
+  }
+  sv->worker = std::thread(server_loop, sv);
+  *out = reinterpret_cast<buff_server*>(sv);
+  return BUFF_OK;
+}
+
+int buff_server_add_chunk(buff_server* server, buff_chunk* chunk, uint32_t* id) {
+  Server* sv = reinterpret_cast<Server*>(server);
+  if (!sv || !chunk) return fail(BUFF_E_INVALID, "null argument");
+  Chunk* ch = reinterpret_cast<Chunk*>(chunk);
+  if (ch->ctx != sv->ctx) return fail(BUFF_E_INVALID, "the chunk belongs to another context");
+  if (ch->flags & BUFF_MAJORITY) return fail(BUFF_E_UNSUPPORTED, "served chunks use the byte_fixed layout");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (id) *id = (uint32_t)sv->chunks.size();
+  sv->chunks.push_back(ch);
+  sv->views_dirty = true;
+  return BUFF_OK;
+}
+
+int buff_server_submit(buff_server* server, const buff_query* q, void* out_dev, uint64_t* ticket) {
+  Server* sv = reinterpret_cast<Server*>(server);
+  if (!sv || !q || !ticket) return fail(BUFF_E_INVALID, "null argument");
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->stop) return fail(BUFF_E_INVALID, "server is shutting down");
+    const uint64_t id = sv->next_ticket++;
+    Req& r = sv->reqs[id];
+    r.q = *q;
+    r.out = (uint64_t)out_dev;
+    r.t_submit = now_s();
+    sv->pending.push_back(id);
+    *ticket = id;
+  }
+  sv->cv_work.notify_one();
+  return BUFF_OK;
+}
+
+int buff_server_wait(buff_server* server, uint64_t ticket, buff_query_result* res, double* latency_us) {
+  Server* sv = reinterpret_cast<Server*>(server);
+  if (!sv) return fail(BUFF_E_INVALID, "null argument");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return fail(BUFF_E_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
+  sv->cv_done.wait(lk, [&] { return it->second.done; });
+  if (res) *res = it->second.res;
+  if (latency_us) *latency_us = (it->second.t_done - it->second.t_submit) * 1e6;
+  sv->reqs.erase(it);
+  return BUFF_OK;
+}
+
+int buff_server_run_trace(buff_server* server, const buff_query* qs, const double* arrival_s, void* const* outs,
+                          uint32_t n, buff_query_result* res, double* latency_us) {
+  Server* sv = reinterpret_cast<Server*>(server);
+  if (!sv || (n && (!qs || !arrival_s))) return fail(BUFF_E_INVALID, "null argument");
+  std::vector<uint64_t> tickets(n);
+  const double t0 = now_s();
+  for (uint32_t i = 0; i < n; ++i) {
+    while (now_s() - t0 < arrival_s[i]) std::this_thread::yield();
+    BUFF_TRY(buff_server_submit(server, &qs[i], outs ? outs[i] : nullptr, &tickets[i]));
+  }
+  for (uint32_t i = 0; i < n; ++i)
+    BUFF_TRY(buff_server_wait(server, tickets[i], res ? &res[i] : nullptr, latency_us ? &latency_us[i] : nullptr));
+  return BUFF_OK;
+}
+
+int buff_server_stats(buff_server* server, uint64_t* batches, uint64_t* queries, uint64_t* largest_batch) {
+  Server* sv = reinterpret_cast<Server*>(server);
+  if (!sv) return fail(BUFF_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (batches) *batches = sv->batches;
+  if (queries) *queries = sv->queries;
+  if (largest_batch) *largest_batch = sv->max_seen;
+  return sv->error;
+}
+
+int buff_server_destroy(buff_server* server) {
+  Server* sv = reinterpret_cast<Server*>(server);
+  if (!sv) return BUFF_OK;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+  }
+  sv->cv_work.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  for (auto& ex : sv->slots) exec_free(ex);
+  drop(sv->ctx, sv->views);
+  delete sv;
   return BUFF_OK;
 }
 

@@ -205,13 +205,14 @@ typedef struct {
   double value;        /* sum: the reference f64 sum; max: the max value */
   double exact;        /* sum: exact sum */
   uint64_t max_off;    /* max: max stored offset */
-  uint64_t out_offset; /* filters: u64 word offset of the bitmap in the bitmap arena;
-                          materialize: double offset of the values in the value arena */
+  uint64_t out_offset; /* buff_batch_query: byte offset of a filter's bitmap / a materialization's
+                          values in the batch arena; server requests: their device address */
 } buff_query_result;
 
 /* Runs nq queries on one batch: host planning, one graph launch, one result
  * copy. Filter bitmaps (ceil(n/64) u64 words each) and materialized values stay
- * in device arenas that buff_batch_arena exposes until the next call. */
+ * in the batch's device arena (buff_batch_arena gives its base; both pointers are
+ * that base) until the next call. */
 int buff_batch_query(buff_ctx* ctx, buff_batch* batch, const buff_query* qs, uint32_t nq, buff_query_result* results);
 int buff_batch_arena(buff_batch* batch, const uint64_t** bitmaps_dev, const double** values_dev);
 /* Times candidate shapes (persistent grid size, tiles per work item, byte planes
@@ -219,6 +220,39 @@ int buff_batch_arena(buff_batch* batch, const uint64_t** bitmaps_dev, const doub
  * context's later batches. */
 int buff_batch_tune(buff_ctx* ctx, buff_batch* batch, const buff_query* qs, uint32_t nq, uint32_t* grid,
                     uint32_t* tiles_per_item, uint32_t* first_planes);
+
+/* ---- serving ------------------------------------------------------------------
+ * A request loop over resident chunks with continuous batching: requests that
+ * arrive while batches run on the GPU are planned into the next batch as soon as
+ * one of `slots` in-flight batch slots is free (each slot has its own stream, so
+ * batches overlap on the device) and while the batch has room: max_batch
+ * requests and arena_bytes of output memory for requests without an output
+ * address (the rest wait for the next batch). Results equal the single-chunk
+ * calls'. No reference counterpart (the reference has no query server). */
+typedef struct buff_server buff_server;
+typedef struct {
+  uint32_t max_batch;   /* requests per batch (0: 4096) */
+  uint32_t slots;       /* batches in flight (0: 2) */
+  uint64_t arena_bytes; /* output memory per batch for requests without out_dev (0: 1 GiB) */
+} buff_server_cfg;
+
+int buff_server_create(buff_ctx* ctx, const buff_server_cfg* cfg, buff_server** out);
+/* Makes a resident byte_fixed chunk of the same context servable; *id is the
+ * chunk index requests name. The caller keeps ownership (free it after the server). */
+int buff_server_add_chunk(buff_server* server, buff_chunk* chunk, uint32_t* id);
+/* Queues one request (buff_query.chunk = server chunk id). out_dev: where a
+ * filter writes its ceil(n/64) bitmap words or a materialization its n doubles
+ * (NULL: server memory, overwritten by later batches). */
+int buff_server_submit(buff_server* server, const buff_query* q, void* out_dev, uint64_t* ticket);
+/* Blocks until the request completes; latency_us = completion - submission. */
+int buff_server_wait(buff_server* server, uint64_t ticket, buff_query_result* result, double* latency_us);
+/* Submits request i at arrival_s[i] seconds after the call (a native open-loop
+ * client), then waits for all of them. outs may be NULL. */
+int buff_server_run_trace(buff_server* server, const buff_query* qs, const double* arrival_s, void* const* outs,
+                          uint32_t n, buff_query_result* results, double* latency_us);
+/* Batches launched, requests served, largest batch; returns a sticky device error if any. */
+int buff_server_stats(buff_server* server, uint64_t* batches, uint64_t* queries, uint64_t* largest_batch);
+int buff_server_destroy(buff_server* server);
 
 /* ---- reference-shaped entry points (byte streams in host memory) ----------
  * Each takes the SplitBDDoubleCompress fields {chunksize, batchsize, scale}

@@ -1,0 +1,148 @@
+"""The serving loop on the GPU: continuous batching must not change any answer.
+
+Resident chunks of outlier-heavy data (the served workload's distribution, at
+test sizes) take a request mix of 60% range filters, 25% sums and 15%
+materializations at 1-3 decimals. Every request's result is computed alone
+first (single-chunk calls), then all requests are served concurrently from
+several client threads and as an open-loop trace; each served result must be
+bit-identical to the alone result (bitmaps, counts, reference and exact sums,
+materialized doubles). Stats show requests were coalesced into batches, and a
+small output arena forces requests to wait for memory without losing any.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+import buff_amd
+from buff_amd import _native as N
+from buff_amd import datagen as dg
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(lens, seed, n):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        k = int(rng.integers(0, len(lens)))
+        u = rng.random()
+        if u < 0.6:
+            out.append((N.BUFF_Q_GT, k, float(rng.choice([99.2, 99.8, 100.0, 100.4, 101.0, -5e6, 3e8])), -1, 0))
+        elif u < 0.85:
+            out.append((N.BUFF_Q_SUM, k, 0.0, -1, 0))
+        else:
+            out.append((N.BUFF_Q_MAT, k, 0.0, int(rng.integers(1, 4)), N.BUFF_SEMANTIC))
+    return out
+
+
+@pytest.fixture(scope="module")
+def served(gpu_ctx):
+    rng = np.random.default_rng(3)
+    lens = [int(v) for v in np.exp(rng.uniform(np.log(2000), np.log(300_000), 20))]
+    chunks = [gpu_ctx.encode(dg.outlier_heavy(n, 3, 40 + i), 10 ** 3) for i, n in enumerate(lens)]
+    yield chunks, lens
+    for c in chunks:
+        c.free()
+
+
+def _alone(chunks, q):
+    op, k, pred, prec, flags = q
+    ch = chunks[k]
+    if op == N.BUFF_Q_GT:
+        return ch.filter("range", pred)
+    if op == N.BUFF_Q_SUM:
+        return ch.sum()[:2]
+    return ch.materialize(prec, flags)
+
+
+def _outs(ctx, lens, reqs):
+    bufs = []
+    for op, k, _, _, _ in reqs:
+        if op == N.BUFF_Q_GT:
+            bufs.append(ctx.alloc(((lens[k] + 63) // 64) * 8))
+        elif op == N.BUFF_Q_MAT:
+            bufs.append(ctx.alloc(lens[k] * 8))
+        else:
+            bufs.append(None)
+    return bufs
+
+
+def _same(ctx, lens, q, want, r, buf):
+    op, k = q[0], q[1]
+    assert r.status == 0, (q, r.status)
+    if op == N.BUFF_Q_GT:
+        words = buf.download(np.uint32, ((lens[k] + 63) // 64) * 2)[: (lens[k] + 31) // 32]
+        assert r.count == want[1] and np.array_equal(words, want[0]), q
+    elif op == N.BUFF_Q_SUM:
+        assert (r.value, r.exact) == want, q
+    else:
+        vals = buf.download(np.float64, lens[k])
+        assert np.array_equal(vals.view(np.int64), want.view(np.int64)), q
+
+
+def test_served_equals_alone_threads(gpu_ctx, served):
+    chunks, lens = served
+    reqs = _requests(lens, 1, 400)
+    alone = [_alone(chunks, q) for q in reqs]
+    bufs = _outs(gpu_ctx, lens, reqs)
+    srv = buff_amd.Server(gpu_ctx, max_batch=64, slots=2)
+    try:
+        ids = [srv.add_chunk(c) for c in chunks]
+        assert ids == list(range(len(chunks)))
+        results = [None] * len(reqs)
+
+        def client(part):
+            tickets = []
+            for i in part:
+                op, k, pred, prec, flags = reqs[i]
+                tickets.append((i, srv.submit(op, k, pred, prec, flags, bufs[i].ptr if bufs[i] else None)))
+            for i, t in tickets:
+                results[i] = srv.wait(t)[0]
+
+        th = [threading.Thread(target=client, args=(range(j, len(reqs), 4),)) for j in range(4)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        batches, served_n, largest = srv.stats()
+        assert served_n == len(reqs) and batches < len(reqs) and largest > 1
+        for q, want, r, b in zip(reqs, alone, results, bufs):
+            _same(gpu_ctx, lens, q, want, r, b)
+    finally:
+        srv.destroy()
+        for b in bufs:
+            if b:
+                b.free()
+
+
+def test_served_trace_and_memory_admission(gpu_ctx, served):
+    chunks, lens = served
+    reqs = _requests(lens, 2, 300)
+    alone = [_alone(chunks, q) for q in reqs]
+    bufs = _outs(gpu_ctx, lens, reqs)
+    # arena of 4 MiB: requests without an output address wait for memory to free up
+    srv = buff_amd.Server(gpu_ctx, max_batch=256, slots=2, arena_bytes=4 << 20)
+    try:
+        for c in chunks:
+            srv.add_chunk(c)
+        arrival = np.cumsum(np.random.default_rng(9).exponential(1 / 20000.0, len(reqs)))
+        res, lat = srv.run_trace(reqs, arrival, [b.ptr if b else 0 for b in bufs])
+        assert np.all(lat > 0)
+        for q, want, r, b in zip(reqs, alone, res, bufs):
+            _same(gpu_ctx, lens, q, want, r, b)
+        # the same requests without output buffers: only counts and sums come back
+        res2, _ = srv.run_trace(reqs, arrival)
+        for q, want, r in zip(reqs, alone, res2):
+            assert r.status == 0
+            if q[0] == N.BUFF_Q_GT:
+                assert r.count == want[1]
+            elif q[0] == N.BUFF_Q_SUM:
+                assert (r.value, r.exact) == want
+        batches, served_n, largest = srv.stats()
+        assert served_n == 2 * len(reqs)
+    finally:
+        srv.destroy()
+        for b in bufs:
+            if b:
+                b.free()
