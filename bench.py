@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--no-config4", action="store_true", help="skip the batched-form section")
     ap.add_argument("--config4-chunks", type=int, default=4096)
     ap.add_argument("--config4-queries", type=int, default=16384)
+    ap.add_argument("--no-config5", action="store_true", help="skip the served-form section")
+    ap.add_argument("--config5-gb", type=float, default=48.0, help="resident compressed bytes (GB, 1e9)")
+    ap.add_argument("--config5-requests", type=int, default=3000, help="requests per load point")
     return ap.parse_args()
 
 
@@ -336,6 +339,102 @@ def bench_config4(ctx, args, buff_amd):
     return res
 
 
+def bench_config5(ctx, args, buff_amd):
+    """Served form: outlier-heavy chunks (95% N(100,0.5), 5% +-10^U[6,9], 3
+    decimals) with lengths log-uniform in [2^20, 2^26], encoded resident until
+    the compressed bytes reach the target (the chunk data repeats a pool of 64
+    distinct generated columns). A Poisson request stream (60% range filters at
+    a random predicate, 25% sums, 15% materializations at 1-3 decimals) is
+    replayed open-loop by the server's native client at fixed rates; p50/p99
+    latency and throughput are reported for continuous batching and for one
+    launch per request (max_batch = 1, one slot) on the same GPU."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from buff_amd import datagen
+    from buff_amd import _native as N
+
+    target = args.config5_gb * 1e9
+    rng = np.random.default_rng(17)
+    K = 64
+    plens = np.floor(np.exp(rng.uniform(np.log(1 << 20), np.log(1 << 26), K))).astype(np.int64)
+    poff = np.concatenate([[0], np.cumsum(plens)]).astype(np.int64)
+    xh = np.empty(int(poff[-1]), dtype=np.float64)
+
+    def gen(i):
+        xh[poff[i]:poff[i + 1]] = datagen.outlier_heavy(int(plens[i]), 3, 1000 + i)
+
+    t0 = time.time()
+    with ThreadPoolExecutor(16) as ex:
+        list(ex.map(gen, range(K)))
+    xd = ctx.alloc(xh.nbytes)
+    xd.upload(xh)
+    chunks, lens = [], []
+    resident = 0
+    while resident < target:
+        i = len(chunks) % K
+        ch = ctx.encode_device(xd.ptr + 8 * int(poff[i]), int(plens[i]), 10 ** 3)
+        chunks.append(ch)
+        lens.append(int(plens[i]))
+        resident += ch.info().device_bytes
+    ctx.synchronize()
+    log("bench: served store %d chunks, %.1f GB resident, %d rows, built in %.1fs" % (
+        len(chunks), resident / 1e9, sum(lens), time.time() - t0))
+    res = {"workload": "outlier-heavy chunks, lengths log-uniform in [2^20, 2^26], 3 decimals, byte_fixed layout",
+           "chunks": len(chunks), "resident_bytes": int(resident), "rows": int(sum(lens)),
+           "mix": "60% range filter, 25% sum, 15% materialize at 1-3 decimals (semantic)"}
+    nreq = args.config5_requests
+
+    def trace(seed):
+        r = np.random.default_rng(seed)
+        qs, rows = [], 0
+        for _ in range(nreq):
+            k = int(r.integers(0, len(chunks)))
+            u = r.random()
+            if u < 0.6:
+                qs.append((N.BUFF_Q_GT, k, float(r.choice([99.5, 100.0, 100.5, 1e6])), -1, 0))
+            elif u < 0.85:
+                qs.append((N.BUFF_Q_SUM, k, 0.0, -1, 0))
+            else:
+                qs.append((N.BUFF_Q_MAT, k, 0.0, int(r.integers(1, 4)), N.BUFF_SEMANTIC))
+            rows += lens[k]
+        return qs, rows
+
+    def serve(max_batch, slots, qps, seed):
+        qs, rows = trace(seed)
+        arrival = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / qps, nreq)) if qps else np.zeros(nreq)
+        srv = buff_amd.Server(ctx, max_batch=max_batch, slots=slots, arena_bytes=8 << 30)
+        try:
+            for ch in chunks:
+                srv.add_chunk(ch)
+            srv.run_trace(qs[:50], np.zeros(50))   # views upload and graph capture
+            t = time.perf_counter()
+            out, lat = srv.run_trace(qs, arrival)
+            wall = time.perf_counter() - t
+            batches, _, largest = srv.stats()
+        finally:
+            srv.destroy()
+        assert all(o.status == 0 for o in out)
+        return {"offered_qps": qps or None, "achieved_qps": round(nreq / wall, 1),
+                "p50_us": round(float(np.percentile(lat, 50)), 1), "p99_us": round(float(np.percentile(lat, 99)), 1),
+                "GBps_f64_scanned": round(rows * 8 / wall / 1e9, 1), "batches": int(batches - 1),
+                "largest_batch": int(largest)}
+
+    res["saturated"] = {"continuous_batching": serve(4096, 2, 0, 50), "one_launch_per_request": serve(1, 1, 0, 50)}
+    res["speedup_saturated"] = round(res["saturated"]["continuous_batching"]["achieved_qps"] /
+                                     res["saturated"]["one_launch_per_request"]["achieved_qps"], 2)
+    sweep = []
+    cap = res["saturated"]["continuous_batching"]["achieved_qps"]
+    for frac in (0.25, 0.5, 0.8):
+        qps = max(100.0, cap * frac)
+        sweep.append({"load": frac, "continuous_batching": serve(4096, 2, qps, 60 + int(frac * 100)),
+                      "one_launch_per_request": serve(1, 1, qps, 60 + int(frac * 100))})
+    res["poisson_sweep"] = sweep
+    for ch in chunks:
+        ch.free()
+    xd.free()
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -529,6 +628,9 @@ def main():
     if not args.no_config4:
         log("bench: batched section (%d chunks, %d queries)" % (args.config4_chunks, args.config4_queries))
         result["config4"] = bench_config4(ctx, args, buff_amd)
+    if not args.no_config5:
+        log("bench: served section (%.0f GB resident)" % args.config5_gb)
+        result["config5"] = bench_config5(ctx, args, buff_amd)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_rows)
     print(json.dumps(result), flush=True)
