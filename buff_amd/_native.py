@@ -127,6 +127,7 @@ SIGNATURES = {
     "buff_batch_query": [vp, vp, P(Query), u32, P(QueryResult)],
     "buff_batch_arena": [vp, P(vp), P(vp)],
     "buff_batch_tune": [vp, vp, P(Query), u32, u32p, u32p, u32p],
+    "buff_batch_set_tuning": [vp, u32, u32, u32],
     "buff_server_create": [vp, P(ServerCfg), P(vp)],
     "buff_server_add_chunk": [vp, vp, u32p],
     "buff_server_submit": [vp, P(Query), vp, u64p],
@@ -467,6 +468,11 @@ class Context:
 
     def alloc(self, nbytes):
         return DeviceBuffer(self, nbytes)
+
+    def set_batch_tuning(self, grid, tiles_per_item, first_planes):
+        """Pins the batch shape (grid, tiles per item, 1 or 8 first filter planes)
+        for later batches and for servers created afterwards."""
+        check(lib().buff_batch_set_tuning(self.handle, grid, tiles_per_item, first_planes))
 
     def batch_encode_device(self, x_dev, row_off, scales, flags=0):
         ro = np.ascontiguousarray(row_off, dtype=np.uint64)

@@ -9,6 +9,7 @@
 // sizes are what the autotuner chooses.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 
 #include "buff_common.h"
 #include "buff_kernels.h"
@@ -28,9 +29,10 @@ struct BEncChunk {
 
 struct BItem {
   uint32_t owner;      // chunk (encode) or query (queries)
-  uint32_t pad;
+  uint32_t chunk;      // queries: the query's chunk (its view is read without going through the query)
   uint64_t c0, c1;     // tile range [c0, c1)
 };
+static_assert(sizeof(BItem) == 24 && offsetof(BItem, chunk) == 4 && offsetof(BItem, c0) == 8, "BItem layout");
 
 // Per-chunk min/max of the `else if` pass (split_double.rs:321-330): min over all
 // rows, max over rows >= 1, and the fixed value of row 0.
@@ -69,23 +71,31 @@ struct BResult {
 };
 
 // Device-side counters of the current batch (written with the descriptors).
+// Items [0, nscan) are filters, sums and maxima; [nscan, nitems) materializations.
+// The host assigns every query its item range (item0, nitems) and the first
+// kernel of the batch writes the items themselves (tiles_per_item tiles each).
 struct BHeader {
   uint32_t nq, nitems;
-  uint32_t pad[2];
+  uint32_t nscan;
+  uint32_t tiles_per_item;
 };
 
 struct BatchBuffers {
   const BHeader* hdr;
   const ChunkView* views;
   const BQuery* q;
-  const BItem* items;
+  BItem* items;                      // written by the batch's first kernel
   uint8_t* tile_top;                 // max: per-tile top byte
   unsigned long long* item_acc;      // per item: kMaxPlanes + 1 u64
   BResult* res;
 };
 
-// The first filter pass reads `first_planes` (1 or 2) byte planes of every tile
-// before refining; `grid` is the persistent grid size.
+// `grid` is the persistent grid size and `tiles_per_item` the work item length
+// (at most kMaxTilesPerItem, so per-item sums fit 32 bits). Filters read the top
+// byte plane of every tile first and the remaining planes only for tiles with
+// tied rows (first_planes = 1), or every plane in one pass (first_planes >=
+// kMaxPlanes), which saves a dependent round trip on columns where most rows tie.
+constexpr uint32_t kMaxTilesPerItem = 4096;
 struct BatchTuning {
   uint32_t grid;
   uint32_t first_planes;
@@ -93,9 +103,13 @@ struct BatchTuning {
   uint32_t pad;
 };
 
-// The fixed launch sequence of one batch (captured in a graph by the host). The
+// Kernels a batch needs: the scan (filters, sums, maxima and their fold), the
+// max refinement, the materialization.
+enum : uint32_t { kBqPartScan = 1, kBqPartMax = 2, kBqPartMat = 4 };
+
+// The launch sequence of one batch (captured in a graph by the host). The
 // kernels read every buffer address and count from `bb` (device memory), so one
-// captured graph serves every batch of the same tuning.
-hipError_t launch_batch_queries(const BatchBuffers* bb, const BatchTuning& tu, hipStream_t s);
+// captured graph serves every batch of the same tuning and parts.
+hipError_t launch_batch_queries(const BatchBuffers* bb, const BatchTuning& tu, uint32_t parts, hipStream_t s);
 
 }  // namespace buffgpu

@@ -63,9 +63,8 @@ struct QueryExec {
   DevBuf hdr, q, items, acc, res, top, arena, bbdev;
   BatchBuffers bb{};
   bool bb_valid = false;
-  std::map<uint32_t, hipGraphExec_t> graphs;   // by grid * 4 + first_planes
+  std::map<uint64_t, hipGraphExec_t> graphs;   // by (grid, parts, first_planes)
   std::vector<BQuery> hq;
-  std::vector<BItem> hitems;
   std::vector<BResult> hres;
   std::vector<buff_query_result> res_host;
   BHeader h{};
@@ -256,8 +255,8 @@ void exec_free(QueryExec& ex) {
   ex.stream = nullptr;
 }
 
-int graph_for(QueryExec& ex, const BatchTuning& tu, hipGraphExec_t* out) {
-  const uint32_t key = tu.grid * 4 + tu.first_planes;
+int graph_for(QueryExec& ex, const BatchTuning& tu, uint32_t parts, hipGraphExec_t* out) {
+  const uint64_t key = ((uint64_t)tu.grid << 32) | ((uint64_t)parts << 16) | tu.first_planes;
   auto it = ex.graphs.find(key);
   if (it != ex.graphs.end()) {
     *out = it->second;
@@ -265,7 +264,7 @@ int graph_for(QueryExec& ex, const BatchTuning& tu, hipGraphExec_t* out) {
   }
   hipGraph_t g = nullptr;
   HIP_TRY(hipStreamBeginCapture(ex.stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = launch_batch_queries((const BatchBuffers*)ex.bbdev.p, tu, ex.stream);
+  hipError_t e = launch_batch_queries((const BatchBuffers*)ex.bbdev.p, tu, parts, ex.stream);
   hipError_t e2 = hipStreamEndCapture(ex.stream, &g);
   if (e != hipSuccess || e2 != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
@@ -292,9 +291,10 @@ uint64_t arena_need(const Chunk* ch, const buff_query& in) {
 int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* views_dev, const buff_query* qs,
                 uint32_t nq, const uint64_t* outs, const BatchTuning& tu) {
   Ctx* ctx = ex.ctx;
-  const uint64_t T = std::max<uint32_t>(tu.tiles_per_item, 1);
+  const uint64_t Tmax = std::min<uint32_t>(std::max<uint32_t>(tu.tiles_per_item, 1), kMaxTilesPerItem);
   ex.hq.assign(nq, BQuery{});
-  ex.hitems.clear();
+  std::vector<uint64_t> qtiles(nq, 0);
+  uint64_t total_tiles = 0;
   ex.res_host.assign(nq, buff_query_result{});
   ex.nq = nq;
   uint64_t arena_off = 0, top_off = 0;
@@ -352,12 +352,28 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
         r.status = fail(BUFF_E_INVALID, "query %u: unknown op %u", k, in.op);
         tiles = false;
     }
-    q.item0 = (uint32_t)ex.hitems.size();
-    if (tiles)
-      for (uint64_t c = 0; c < v.ntiles; c += T) ex.hitems.push_back(BItem{k, 0, c, std::min<uint64_t>(c + T, v.ntiles)});
-    q.nitems = (uint32_t)ex.hitems.size() - q.item0;
+    qtiles[k] = tiles ? v.ntiles : 0;
+    total_tiles += qtiles[k];
   }
-  const uint32_t nitems = (uint32_t)ex.hitems.size();
+  // Item length: the tuned one, shortened when the batch has too few tiles to
+  // give every wave of the grid (4 per workgroup) an item.
+  const uint64_t waves = 4ull * std::max<uint32_t>(tu.grid, 1);
+  const uint64_t T = std::max<uint64_t>(1, std::min<uint64_t>(Tmax, (total_tiles + waves - 1) / waves));
+  // item ranges: scan items first, materializations after them (their item0 is
+  // rebased below); k_bq_items writes the items on the device
+  uint64_t n_scan = 0, n_mat = 0;
+  for (uint32_t k = 0; k < nq; ++k) {
+    BQuery& q = ex.hq[k];
+    uint64_t& next = q.op == kBqMat ? n_mat : n_scan;
+    q.item0 = (uint32_t)next;
+    q.nitems = (uint32_t)((qtiles[k] + T - 1) / T);
+    next += q.nitems;
+  }
+  if (n_scan + n_mat > 0xFFFFFFFFull) return fail(BUFF_E_INVALID, "batch work items exceed 2^32");
+  const uint32_t nscan = (uint32_t)n_scan;
+  for (BQuery& q : ex.hq)
+    if (q.op == kBqMat) q.item0 += nscan;
+  const uint32_t nitems = (uint32_t)(n_scan + n_mat);
   BUFF_TRY(ensure(ctx, ex.hdr, sizeof(BHeader)));
   BUFF_TRY(ensure(ctx, ex.q, sizeof(BQuery) * std::max<uint32_t>(nq, 1)));
   BUFF_TRY(ensure(ctx, ex.items, sizeof(BItem) * std::max<uint32_t>(nitems, 1)));
@@ -376,7 +392,7 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
   bb.hdr = (const BHeader*)ex.hdr.p;
   bb.views = (const ChunkView*)views_dev;
   bb.q = (const BQuery*)ex.q.p;
-  bb.items = (const BItem*)ex.items.p;
+  bb.items = (BItem*)ex.items.p;
   bb.tile_top = (uint8_t*)ex.top.p;
   bb.item_acc = (unsigned long long*)ex.acc.p;
   bb.res = (BResult*)ex.res.p;
@@ -387,13 +403,18 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
     ex.bb_valid = true;
   }
   if (nq == 0) return BUFF_OK;
-  ex.h = BHeader{nq, nitems, {0, 0}};
+  ex.h = BHeader{nq, nitems, nscan, (uint32_t)T};
   HIP_TRY(hipMemcpyAsync(ex.hdr.p, &ex.h, sizeof(ex.h), hipMemcpyHostToDevice, ex.stream));
   HIP_TRY(hipMemcpyAsync(ex.q.p, ex.hq.data(), sizeof(BQuery) * nq, hipMemcpyHostToDevice, ex.stream));
-  if (nitems)
-    HIP_TRY(hipMemcpyAsync(ex.items.p, ex.hitems.data(), sizeof(BItem) * nitems, hipMemcpyHostToDevice, ex.stream));
+  uint32_t parts = 0;
+  for (const BQuery& q : ex.hq) {
+    if (!q.nitems) continue;
+    parts |= q.op == kBqMat ? kBqPartMat : kBqPartScan;
+    if (q.op == kBqMax) parts |= kBqPartMax;
+  }
+  parts |= kBqPartScan;   // the fold also writes the results of queries without tiles
   hipGraphExec_t gx = nullptr;
-  BUFF_TRY(graph_for(ex, tu, &gx));
+  BUFF_TRY(graph_for(ex, tu, parts, &gx));
   HIP_TRY(hipGraphLaunch(gx, ex.stream));
   ex.hres.resize(nq);
   HIP_TRY(hipMemcpyAsync(ex.hres.data(), ex.res.p, sizeof(BResult) * nq, hipMemcpyDeviceToHost, ex.stream));
@@ -503,8 +524,8 @@ int batch_tune(Ctx* ctx, Batch* b, const buff_query* qs, uint32_t nq, BatchTunin
   BatchTuning best = ctx_tuning(ctx);
   float best_ms = 1e30f;
   for (uint32_t grid : {1024u, 2048u, 4096u}) {
-    for (uint32_t tiles : {4u, 16u, 64u}) {
-      for (uint32_t first : {1u, 2u}) {
+    for (uint32_t tiles : {16u, 64u, 256u}) {
+      for (uint32_t first : {1u, kMaxPlanes}) {
         BatchTuning t{grid, first, tiles, 0};
         float m = 1e30f;
         for (int rep = 0; rep < 3; ++rep) {
@@ -737,6 +758,21 @@ int buff_batch_tune(buff_ctx* ctx, buff_batch* batch, const buff_query* qs, uint
   if (grid) *grid = t.grid;
   if (tiles_per_item) *tiles_per_item = t.tiles_per_item;
   if (first_planes) *first_planes = t.first_planes;
+  return BUFF_OK;
+}
+
+int buff_batch_set_tuning(buff_ctx* ctx, uint32_t grid, uint32_t tiles_per_item, uint32_t first_planes) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return fail(BUFF_E_INVALID, "null argument");
+  if (grid < 1 || grid > 65536) return fail(BUFF_E_INVALID, "grid %u outside [1, 65536]", grid);
+  if (tiles_per_item < 1 || tiles_per_item > kMaxTilesPerItem)
+    return fail(BUFF_E_INVALID, "tiles_per_item %u outside [1, %u]", tiles_per_item, kMaxTilesPerItem);
+  if (first_planes != 1 && first_planes != kMaxPlanes)
+    return fail(BUFF_E_INVALID, "first_planes %u is neither 1 nor %u", first_planes, kMaxPlanes);
+  c->bt_grid = grid;
+  c->bt_tiles = tiles_per_item;
+  c->bt_first = first_planes;
+  c->bt_tuned = true;
   return BUFF_OK;
 }
 

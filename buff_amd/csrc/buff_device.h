@@ -38,6 +38,33 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
   for (int m = 32; m >= 1; m >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, m, 64));
   return v;
 }
+// Exclusive prefix sum over the 64 lanes with DPP: Hillis-Steele inside each
+// 16-lane row (row_shr 1, 2, 4, 8), then the row totals carried by row_bcast15
+// (into rows 1 and 3) and row_bcast31 (into rows 2 and 3). Lanes whose DPP
+// source lies outside the row read 0.
+__device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t lane) {
+  (void)lane;
+  uint32_t s = v;
+  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x111, 0xf, 0xf, false);   // row_shr:1
+  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x112, 0xf, 0xf, false);   // row_shr:2
+  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x114, 0xf, 0xf, false);   // row_shr:4
+  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x118, 0xf, 0xf, false);   // row_shr:8
+  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x142, 0xa, 0xf, false);   // row_bcast:15
+  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x143, 0xc, 0xf, false);   // row_bcast:31
+  return s - v;
+}
+
+__device__ __forceinline__ uint32_t wave_total_u32(uint32_t v) {
+  const uint32_t incl = wave_excl_scan_u32(v, 0) + v;
+  return (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += (uint32_t)__shfl_xor((int)v, m, 64);
+  return v;
+}
+
 __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
   uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
   uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
@@ -55,13 +82,47 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 __device__ __forceinline__ bool tile_full(uint64_t n, uint64_t c) { return (c + 1) * kTileRows <= n; }
 
+// Address-space views of generic pointers. Chunk data lives in global memory;
+// descriptors that no kernel writes while it reads them (work lists, chunk
+// views, query descriptors) are read as constant memory, so uniform reads of
+// them become scalar loads and their fields stay in scalar registers.
+#define BUFF_GLOBAL __attribute__((address_space(1)))
+#define BUFF_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const BUFF_GLOBAL T* gmem(const T* p) {
+  return (const BUFF_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ BUFF_GLOBAL T* gmem_w(T* p) {
+  return (BUFF_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ const BUFF_CONST T* cmem(const T* p) {
+  return (const BUFF_CONST T*)p;
+}
+// Element i of a descriptor array read as constant memory (word by word: class
+// types cannot be copied through an address-space pointer).
+template <typename T>
+__device__ __forceinline__ T cload(const T* base, uint64_t i = 0) {
+  static_assert(sizeof(T) % 4 == 0, "descriptor of whole words");
+  T out;
+  const BUFF_CONST uint32_t* src = (const BUFF_CONST uint32_t*)(base + i);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (uint32_t k = 0; k < sizeof(T) / 4; ++k) dst[k] = src[k];
+  return out;
+}
+
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ uint4 load_plane(const ChunkView& cv, uint32_t p, uint64_t c, uint32_t lane) {
-  return reinterpret_cast<const uint4*>(cv.planes + p * cv.pitch + c * kTileRows)[lane];
+  const u32x4_t v = gmem(reinterpret_cast<const u32x4_t*>(cv.planes + p * cv.pitch + c * kTileRows))[lane];
+  return make_uint4(v.x, v.y, v.z, v.w);
 }
 
 // bit-plane j of tile c, lane's 16 rows (bit b <-> row 16*lane + b)
 __device__ __forceinline__ uint32_t load_tail(const ChunkView& cv, uint32_t j, uint64_t c, uint32_t lane) {
-  return cv.tail[(c * cv.r + j) * 64 + lane];
+  return gmem(cv.tail)[(c * cv.r + j) * 64 + lane];
 }
 
 // byte of row 16*lane + b inside the lane's 16 bytes (dword b%4, byte b/4)
@@ -174,6 +235,14 @@ __device__ __forceinline__ void refine_tail(const ChunkView& cv, uint64_t c, uin
 
 __device__ __forceinline__ uint32_t max_byte4(uint32_t d) {
   return max(max(d & 0xFFu, (d >> 8) & 0xFFu), max((d >> 16) & 0xFFu, d >> 24));
+}
+
+// max over the rows in `mask` of (byte + 1); 0 when the mask is empty
+__device__ __forceinline__ uint32_t masked_max_plus1(const uint4& w, uint32_t mask) {
+  uint32_t v = 0;
+#pragma unroll
+  for (uint32_t b = 0; b < 16; ++b) v = ((mask >> b) & 1u) ? max(v, byte_at(w, b) + 1u) : v;
+  return v;
 }
 
 }  // namespace buffgpu

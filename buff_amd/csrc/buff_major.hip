@@ -17,33 +17,6 @@
 
 namespace buffgpu {
 
-// Exclusive prefix sum over the 64 lanes with DPP: Hillis-Steele inside each
-// 16-lane row (row_shr 1, 2, 4, 8), then the row totals carried by row_bcast15
-// (into rows 1 and 3) and row_bcast31 (into rows 2 and 3). Lanes whose DPP
-// source lies outside the row read 0.
-__device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t lane) {
-  (void)lane;
-  uint32_t s = v;
-  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x111, 0xf, 0xf, false);   // row_shr:1
-  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x112, 0xf, 0xf, false);   // row_shr:2
-  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x114, 0xf, 0xf, false);   // row_shr:4
-  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x118, 0xf, 0xf, false);   // row_shr:8
-  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x142, 0xa, 0xf, false);   // row_bcast:15
-  s += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x143, 0xc, 0xf, false);   // row_bcast:31
-  return s - v;
-}
-
-__device__ __forceinline__ uint32_t wave_total_u32(uint32_t v) {
-  const uint32_t incl = wave_excl_scan_u32(v, 0) + v;
-  return (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-}
-
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += (uint32_t)__shfl_xor((int)v, m, 64);
-  return v;
-}
-
 // Replaces byte b (row 16*lane + b: dword b%4, byte b/4) of a lane's 16 bytes.
 __device__ __forceinline__ void set_byte(uint32_t& d0, uint32_t& d1, uint32_t& d2, uint32_t& d3, uint32_t b,
                                          uint32_t v) {
@@ -136,14 +109,6 @@ __device__ __forceinline__ uint4 sparse_word(const SparseLanes& sl, uint32_t j, 
     set_byte(d0, d1, d2, d3, bit, (uint32_t)(((k < 8 ? sl.lo[j] : sl.hi[j]) >> (8 * (k & 7))) & 0xFF));
   }
   return make_uint4(d0, d1, d2, d3);
-}
-
-// max over the rows in `mask` of (byte + 1); 0 when the mask is empty
-__device__ __forceinline__ uint32_t masked_max_plus1(const uint4& w, uint32_t mask) {
-  uint32_t v = 0;
-#pragma unroll
-  for (uint32_t b = 0; b < 16; ++b) v = ((mask >> b) & 1u) ? max(v, byte_at(w, b) + 1u) : v;
-  return v;
 }
 
 // ----------------------------------------------------------------------------

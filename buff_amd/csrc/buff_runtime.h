@@ -74,7 +74,7 @@ struct Ctx {
 
   std::map<uint64_t, LaunchShape> tuned;
   // batched queries: persistent grid, tiles per work item, planes of the first filter pass
-  uint32_t bt_grid = 2048, bt_tiles = 16, bt_first = 1;
+  uint32_t bt_grid = 2048, bt_tiles = 64, bt_first = 1;
   bool bt_tuned = false;
 
   ~Ctx();

@@ -215,11 +215,17 @@ typedef struct {
  * that base) until the next call. */
 int buff_batch_query(buff_ctx* ctx, buff_batch* batch, const buff_query* qs, uint32_t nq, buff_query_result* results);
 int buff_batch_arena(buff_batch* batch, const uint64_t** bitmaps_dev, const double** values_dev);
-/* Times candidate shapes (persistent grid size, tiles per work item, byte planes
- * read by the first filter pass) on this query mix and keeps the fastest for the
- * context's later batches. */
+/* Times candidate shapes on this query mix and keeps the fastest for the
+ * context's later batches: the persistent grid size, the tiles per work item,
+ * and the planes a filter reads before it checks for tied rows (1: the top byte
+ * plane, then the rest only for tiles with ties; 8: every plane in one pass). */
 int buff_batch_tune(buff_ctx* ctx, buff_batch* batch, const buff_query* qs, uint32_t nq, uint32_t* grid,
                     uint32_t* tiles_per_item, uint32_t* first_planes);
+/* Sets the shape later batches of this context and servers created afterwards
+ * use (as if buff_batch_tune had chosen it): grid 1..65536, tiles_per_item
+ * 1..4096, first_planes 1 or 8. Call it while no batch of the context is being
+ * planned. */
+int buff_batch_set_tuning(buff_ctx* ctx, uint32_t grid, uint32_t tiles_per_item, uint32_t first_planes);
 
 /* ---- serving ------------------------------------------------------------------
  * A request loop over resident chunks with continuous batching: requests that
