@@ -17,55 +17,105 @@
 
 namespace buffgpu {
 
-// Replaces byte b (row 16*lane + b: dword b%4, byte b/4) of a lane's 16 bytes.
-__device__ __forceinline__ void set_byte(uint32_t& d0, uint32_t& d1, uint32_t& d2, uint32_t& d3, uint32_t b,
-                                         uint32_t v) {
-  const uint32_t sh = 8 * (b >> 2);
-  const uint32_t keep = ~(0xFFu << sh), val = v << sh;
-  const uint32_t i = b & 3;
-  d0 = i == 0 ? (d0 & keep) | val : d0;
-  d1 = i == 1 ? (d1 & keep) | val : d1;
-  d2 = i == 2 ? (d2 & keep) | val : d2;
-  d3 = i == 3 ? (d3 & keep) | val : d3;
+// The lane's packed outlier bytes (k-th outlier of the lane = byte k) as four
+// dwords, from a 20-byte window of aligned loads at its compacted position.
+struct Packed16 {
+  uint32_t d0, d1, d2, d3;
+};
+__device__ __forceinline__ Packed16 load_packed(const uint8_t* outl, uint64_t pos) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(outl + (pos & ~3ull));
+  const uint32_t sh = (uint32_t)(pos & 3);
+  const uint32_t a = w[0], b = w[1], c = w[2], d = w[3], e = w[4];
+  Packed16 p;
+  p.d0 = __builtin_amdgcn_alignbyte(b, a, sh);
+  p.d1 = __builtin_amdgcn_alignbyte(c, b, sh);
+  p.d2 = __builtin_amdgcn_alignbyte(d, c, sh);
+  p.d3 = __builtin_amdgcn_alignbyte(e, d, sh);
+  return p;
+}
+// Row masks (bit b <-> row 16*lane + b) of one sparse column against target byte
+// t: majority rows all at once, outlier rows from the packed bytes. The lane's
+// k-th outlier (k-th set bit of m16, ascending rows) is byte k of its packed
+// window; k runs with a wave-uniform bound in an unrolled loop, so every step
+// reads a fixed register and byte (no per-lane select of the byte). Only the
+// rows in `want` keep their results.
+__device__ __forceinline__ void sparse_cmp(const Packed16& pk, uint32_t m16, uint32_t want, uint32_t major, uint32_t t,
+                                           uint32_t& gt, uint32_t& eq) {
+  const uint32_t nm = ~m16 & 0xFFFFu;
+  const uint32_t d[4] = {pk.d0, pk.d1, pk.d2, pk.d3};
+  uint32_t go = 0, eo = 0, rem = m16;
+#pragma unroll
+  for (uint32_t k = 0; k < 16; ++k) {
+    if (!__ballot((rem & want) != 0)) break;
+    const uint32_t pos = (uint32_t)__builtin_ctz(rem | 0x10000u);   // 16 once the lane has no outlier left
+    rem &= rem - 1u;
+    const uint32_t v = (d[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+    go |= (uint32_t)(v > t) << pos;
+    eo |= (uint32_t)(v == t) << pos;
+  }
+  gt = (major > t ? nm : 0u) | (go & want);
+  eq = (major == t ? nm : 0u) | (eo & want);
 }
 
-// The lane's 16 bytes of a sparse column in tile c: the majority byte, with the
-// outlier bytes of the rows in `want` (a subset of m16) patched in. Every lane of
-// the wave calls it together (the exclusive scan). A lane's outlier bytes are
-// contiguous in the compacted array, so it fetches them as one 20-byte window of
-// aligned dwords (the array is padded past its end) and patches from registers.
-__device__ __forceinline__ uint4 sparse_bytes(const SparseDev& sd, uint64_t c, uint32_t lane, uint32_t m16,
-                                              uint32_t want) {
-  const uint32_t excl = wave_excl_scan_u32(__popc(m16), lane);
-  const uint32_t M = 0x01010101u * sd.major;
-  uint32_t d0 = M, d1 = M, d2 = M, d3 = M;
-  if (want) {
-    const uint64_t pos = (uint64_t)sd.prefix[c] + excl;
-    const uint32_t* w = reinterpret_cast<const uint32_t*>(sd.outl + (pos & ~3ull));
-    const uint32_t sh = 8u * (uint32_t)(pos & 3);
-    const uint64_t a = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
-    const uint64_t b = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
-    const uint64_t e = (uint64_t)w[4];
-    const uint64_t lo = sh ? (a >> sh) | (b << (64 - sh)) : a;    // outlier bytes 0..7 of this lane
-    const uint64_t hi = sh ? (b >> sh) | (e << (64 - sh)) : b;    // outlier bytes 8..15
-    uint32_t bits = want;
-    while (bits) {
-      const uint32_t bit = (uint32_t)__builtin_ctz(bits);
-      bits &= bits - 1;
-      const uint32_t k = __popc(m16 & ((1u << bit) - 1u));
-      const uint32_t v = (uint32_t)(((k < 8 ? lo : hi) >> (8 * (k & 7))) & 0xFF);
-      set_byte(d0, d1, d2, d3, bit, v);
-    }
+// Replaces byte b (row 16*lane + b: dword b%4, byte b/4) of a lane's 16 bytes
+// when `on` (written with selects: every lane runs the same instructions).
+__device__ __forceinline__ void set_byte(uint32_t& d0, uint32_t& d1, uint32_t& d2, uint32_t& d3, uint32_t b,
+                                         uint32_t v, bool on) {
+  const uint32_t sh = 8 * ((b >> 2) & 3);
+  const uint32_t keep = ~(0xFFu << sh), val = v << sh;
+  const uint32_t i = b & 3;
+  d0 = on && i == 0 ? (d0 & keep) | val : d0;
+  d1 = on && i == 1 ? (d1 & keep) | val : d1;
+  d2 = on && i == 2 ? (d2 & keep) | val : d2;
+  d3 = on && i == 3 ? (d3 & keep) | val : d3;
+}
+
+// The lane's 16 bytes of a sparse column: the majority byte, with the outlier
+// bytes of the rows in `want` (a subset of m16) patched in. The outliers are
+// walked by rank as in sparse_cmp.
+__device__ __forceinline__ uint4 patch_outliers(const Packed16& pk, uint32_t m16, uint32_t want, uint32_t major) {
+  const uint32_t M = 0x01010101u * major;
+  uint32_t o0 = M, o1 = M, o2 = M, o3 = M;
+  const uint32_t d[4] = {pk.d0, pk.d1, pk.d2, pk.d3};
+  uint32_t rem = m16;
+#pragma unroll
+  for (uint32_t k = 0; k < 16; ++k) {
+    if (!__ballot((rem & want) != 0)) break;
+    const uint32_t pos = (uint32_t)__builtin_ctz(rem | 0x10000u);
+    rem &= rem - 1u;
+    const uint32_t v = (d[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+    set_byte(o0, o1, o2, o3, pos, v, (want >> pos) & 1u);
   }
-  return make_uint4(d0, d1, d2, d3);
+  return make_uint4(o0, o1, o2, o3);
+}
+
+// max over the outlier rows in `want` of (byte + 1); 0 when there is none.
+__device__ __forceinline__ uint32_t sparse_max_plus1(const Packed16& pk, uint32_t m16, uint32_t want) {
+  const uint32_t d[4] = {pk.d0, pk.d1, pk.d2, pk.d3};
+  uint32_t rem = m16, best = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 16; ++k) {
+    if (!__ballot((rem & want) != 0)) break;
+    const uint32_t pos = (uint32_t)__builtin_ctz(rem | 0x10000u);
+    rem &= rem - 1u;
+    const uint32_t v = (d[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+    best = (want >> pos) & 1u ? max(best, v + 1u) : best;
+  }
+  return best;
+}
+
+// The outlier window of column sd for the lane in tile c (every lane of the
+// wave calls it together: the exclusive scan).
+__device__ __forceinline__ Packed16 lane_outliers(const SparseDev& sd, uint64_t c, uint32_t lane, uint32_t m16) {
+  const uint32_t excl = wave_excl_scan_u32(__popc(m16), lane);
+  return m16 ? load_packed(sd.outl, (uint64_t)sd.prefix[c] + excl) : Packed16{0, 0, 0, 0};
 }
 
 // Masks and outlier windows of every sparse column of tile c, with all their
 // loads issued before any is consumed (the columns are independent).
 struct SparseLanes {
   uint32_t m[kMaxSparse];
-  uint64_t lo[kMaxSparse];
-  uint64_t hi[kMaxSparse];
+  Packed16 pk[kMaxSparse];
 };
 
 __device__ __forceinline__ void load_sparse(const MajorView& mv, uint64_t c, uint32_t lane, SparseLanes& sl) {
@@ -77,38 +127,12 @@ __device__ __forceinline__ void load_sparse(const MajorView& mv, uint64_t c, uin
   }
 #pragma unroll
   for (uint32_t j = 0; j < kMaxSparse; ++j) {
-    sl.lo[j] = 0;
-    sl.hi[j] = 0;
+    sl.pk[j] = Packed16{0, 0, 0, 0};
     if (j < mv.S) {
       const uint32_t excl = wave_excl_scan_u32(__popc(sl.m[j]), lane);
-      if (sl.m[j]) {
-        const uint64_t pos = (uint64_t)pre[j] + excl;
-        const uint32_t* w = reinterpret_cast<const uint32_t*>(mv.col[j].outl + (pos & ~3ull));
-        const uint32_t sh = 8u * (uint32_t)(pos & 3);
-        const uint64_t a = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
-        const uint64_t b = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
-        const uint64_t e = (uint64_t)w[4];
-        sl.lo[j] = sh ? (a >> sh) | (b << (64 - sh)) : a;
-        sl.hi[j] = sh ? (b >> sh) | (e << (64 - sh)) : b;
-      }
+      if (sl.m[j]) sl.pk[j] = load_packed(mv.col[j].outl, (uint64_t)pre[j] + excl);
     }
   }
-}
-
-// Column j's 16 bytes for the lane from SparseLanes: majority byte, outlier
-// bytes patched in for the rows in `want`.
-__device__ __forceinline__ uint4 sparse_word(const SparseLanes& sl, uint32_t j, uint32_t major, uint32_t want) {
-  const uint32_t M = 0x01010101u * major;
-  uint32_t d0 = M, d1 = M, d2 = M, d3 = M;
-  const uint32_t m16 = sl.m[j];
-  uint32_t bits = want & m16;
-  while (bits) {
-    const uint32_t bit = (uint32_t)__builtin_ctz(bits);
-    bits &= bits - 1;
-    const uint32_t k = __popc(m16 & ((1u << bit) - 1u));
-    set_byte(d0, d1, d2, d3, bit, (uint32_t)(((k < 8 ? sl.lo[j] : sl.hi[j]) >> (8 * (k & 7))) & 0xFF));
-  }
-  return make_uint4(d0, d1, d2, d3);
 }
 
 // ----------------------------------------------------------------------------
@@ -339,50 +363,6 @@ __global__ __launch_bounds__(256) void k_major_copy(MajorEnc me, uint64_t ntiles
 // masks / tile prefixes / first dense plane, then the outlier windows, then the
 // candidates' later planes and tail. S (sparse columns) is a template argument so
 // the per-column state stays in registers.
-// The lane's packed outlier bytes (k-th outlier of the lane = byte k) as four
-// dwords, from a 20-byte window of aligned loads at its compacted position.
-struct Packed16 {
-  uint32_t d0, d1, d2, d3;
-};
-__device__ __forceinline__ Packed16 load_packed(const uint8_t* outl, uint64_t pos) {
-  const uint32_t* w = reinterpret_cast<const uint32_t*>(outl + (pos & ~3ull));
-  const uint32_t sh = (uint32_t)(pos & 3);
-  const uint32_t a = w[0], b = w[1], c = w[2], d = w[3], e = w[4];
-  Packed16 p;
-  p.d0 = __builtin_amdgcn_alignbyte(b, a, sh);
-  p.d1 = __builtin_amdgcn_alignbyte(c, b, sh);
-  p.d2 = __builtin_amdgcn_alignbyte(d, c, sh);
-  p.d3 = __builtin_amdgcn_alignbyte(e, d, sh);
-  return p;
-}
-__device__ __forceinline__ uint32_t packed_byte(const Packed16& p, uint32_t k) {
-  const uint32_t i = k >> 2;
-  const uint32_t d = i == 0 ? p.d0 : i == 1 ? p.d1 : i == 2 ? p.d2 : p.d3;
-  return __builtin_amdgcn_ubfe(d, 8 * (k & 3), 8);
-}
-
-// Row masks (bit b <-> row 16*lane + b) of one sparse column against target byte
-// t: outlier rows compared byte by byte from the packed bytes (`want` selects
-// which), majority rows all at once.
-__device__ __forceinline__ void sparse_cmp(const Packed16& pk, uint32_t m16, uint32_t want, uint32_t major, uint32_t t,
-                                           uint32_t& gt, uint32_t& eq) {
-  const uint32_t nm = ~m16 & 0xFFFFu;
-  gt = major > t ? nm : 0u;
-  eq = major == t ? nm : 0u;
-  uint32_t rem = want;
-  while (rem) {
-    const uint32_t pos = (uint32_t)__builtin_ctz(rem);
-    rem &= rem - 1;
-    const uint32_t v = packed_byte(pk, __popc(m16 & ((1u << pos) - 1u)));
-    gt |= (uint32_t)(v > t) << pos;
-    eq |= (uint32_t)(v == t) << pos;
-  }
-}
-
-// U tiles per wave iteration, every load of a phase for all of them in flight:
-// masks / tile prefixes / first dense plane, then the outlier windows, then the
-// candidates' later planes and tail. S (sparse columns) is a template argument so
-// the per-column state stays in registers.
 template <int MODE, int S, int U>
 __global__ __launch_bounds__(256) void k_major_filter(MajorView mv, MajorTarget tg, uint32_t sem,
                                                       uint64_t* __restrict__ out, uint64_t nwords) {
@@ -488,8 +468,10 @@ __global__ __launch_bounds__(256) void k_major_filter(MajorView mv, MajorTarget 
 // twice its shift, and only for rows up to its last outlier (SEM: every row gets
 // its byte once). Values are staged in LDS and written coalesced.
 
-template <int SEM>
+template <int SEM, int S>
 __global__ __launch_bounds__(256) void k_major_decode(MajorView mv, double inv_scale, double* __restrict__ out) {
+  constexpr int SS = S > 0 ? S : 1;
+  constexpr int NPM = (int)kMaxPlanes - S > 0 ? (int)kMaxPlanes - S : 1;   // dense planes at most (F <= 64)
   __shared__ double s_val[4][64 * kLdsRow];
   const uint32_t lane = __lane_id();
   double* sv = s_val[threadIdx.x >> 6];
@@ -497,51 +479,50 @@ __global__ __launch_bounds__(256) void k_major_decode(MajorView mv, double inv_s
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const ChunkView& dv = mv.dense;
   for (uint64_t c = wave; c < dv.ntiles; c += nwaves) {
-    SparseLanes sl;
-    load_sparse(mv, c, lane, sl);
-    uint4 wp[kMaxPlanes];
-    uint32_t h[8];
+    // every load of the tile first: masks and prefixes, dense planes and tail,
+    // then the outlier windows (they need the masks' scan)
+    uint32_t m[SS], pre[SS];
 #pragma unroll
-    for (uint32_t p = 0; p < kMaxPlanes; ++p) wp[p] = p < dv.P ? load_plane(dv, p, c, lane) : make_uint4(0, 0, 0, 0);
+    for (int j = 0; j < S; ++j) {
+      m[j] = mv.col[j].mask16[c * 64 + lane];
+      pre[j] = mv.col[j].prefix[c];
+    }
+    uint4 wp[NPM];
 #pragma unroll
-    for (uint32_t j = 0; j < 8; ++j) h[j] = j < dv.r ? load_tail(dv, j, c, lane) : 0u;
-    uint64_t fx[16];
+    for (int p = 0; p < NPM; ++p) wp[p] = (uint32_t)p < dv.P ? load_plane(dv, p, c, lane) : make_uint4(0, 0, 0, 0);
+    uint32_t h[7];
 #pragma unroll
-    for (uint32_t b = 0; b < 16; ++b) fx[b] = 0;
+    for (uint32_t j = 0; j < 7; ++j) h[j] = j < dv.r ? load_tail(dv, j, c, lane) : 0u;
+    uint4 ws[SS];
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const uint32_t excl = wave_excl_scan_u32(__popc(m[j]), lane);
+      const Packed16 pk = m[j] ? load_packed(mv.col[j].outl, (uint64_t)pre[j] + excl) : Packed16{0, 0, 0, 0};
+      ws[j] = patch_outliers(pk, m[j], m[j], mv.col[j].major);
+    }
     const uint64_t row0 = c * kTileRows + 16ull * lane;
 #pragma unroll
-    for (uint32_t j = 0; j < kMaxSparse; ++j) {
-      if (j >= mv.S) break;
-      const SparseDev& sd = mv.col[j];
-      const uint32_t m16 = sl.m[j];
-      const uint4 w = sparse_word(sl, j, sd.major, m16);
-      const uint32_t sh = sd.shift;
-      if (SEM || j == 0) {
+    for (uint32_t b = 0; b < 16; ++b) {
+      uint64_t v = 0;
 #pragma unroll
-        for (uint32_t b = 0; b < 16; ++b) fx[b] |= (uint64_t)byte_at(w, b) << sh;
-      } else {
-        const uint64_t twice = ((uint64_t)sd.major << sh) << sh;
-#pragma unroll
-        for (uint32_t b = 0; b < 16; ++b) {
-          const uint64_t v = ((m16 >> b) & 1u) ? ((uint64_t)byte_at(w, b) << sh) : twice;
-          fx[b] |= row0 + b <= sd.last ? v : 0ull;
+      for (int j = 0; j < S; ++j) {
+        const uint32_t sh = mv.col[j].shift;
+        const uint64_t byte = (uint64_t)byte_at(ws[j], b) << sh;
+        if (SEM || j == 0) {
+          v |= byte;
+        } else {
+          const uint64_t twice = ((uint64_t)mv.col[j].major << sh) << sh;
+          const uint64_t t = ((m[j] >> b) & 1u) ? byte : twice;
+          v |= row0 + b <= mv.col[j].last ? t : 0ull;
         }
       }
+#pragma unroll
+      for (int p = 0; p < NPM; ++p)
+        if ((uint32_t)p < dv.P) v |= (uint64_t)byte_at(wp[p], b) << (dv.F - 8 * (p + 1));
+#pragma unroll
+      for (uint32_t j = 0; j < 7; ++j) v |= (uint64_t)((h[j] >> b) & 1u) << j;
+      sv[kLdsRow * lane + b] = (double)(int64_t)((uint64_t)dv.base + v) * inv_scale;
     }
-#pragma unroll
-    for (uint32_t p = 0; p < kMaxPlanes; ++p) {
-      if (p >= dv.P) break;
-      const uint32_t sh = dv.F - 8 * (p + 1);
-#pragma unroll
-      for (uint32_t b = 0; b < 16; ++b) fx[b] |= (uint64_t)byte_at(wp[p], b) << sh;
-    }
-#pragma unroll
-    for (uint32_t j = 0; j < 8; ++j) {
-#pragma unroll
-      for (uint32_t b = 0; b < 16; ++b) fx[b] |= (uint64_t)((h[j] >> b) & 1u) << j;
-    }
-#pragma unroll
-    for (uint32_t b = 0; b < 16; ++b) sv[kLdsRow * lane + b] = (double)(int64_t)((uint64_t)dv.base + fx[b]) * inv_scale;
     wave_lds_sync();
     double* ot = out + c * kTileRows + lane;
     if (tile_full(dv.n, c)) {
@@ -642,8 +623,9 @@ __global__ __launch_bounds__(256) void k_major_max_stage(MajorView mv, uint32_t 
       if (prev < mv.S) {
         const SparseDev& sd = mv.col[prev];
         const uint32_t m16 = sd.mask16[c * 64 + lane];
-        const uint4 w = sparse_bytes(sd, c, lane, m16, m16 & cand);
-        cand &= mask_eq(w, make_digit(chosen));
+        uint32_t gt_unused, eq;
+        sparse_cmp(lane_outliers(sd, c, lane, m16), m16, m16 & cand, sd.major, chosen, gt_unused, eq);
+        cand &= eq;
       } else if (prev < ncol) {
         if (__ballot(cand != 0)) {
           const uint4 w = load_plane(dv, prev - mv.S, c, cand ? lane : 0u);
@@ -661,8 +643,7 @@ __global__ __launch_bounds__(256) void k_major_max_stage(MajorView mv, uint32_t 
       const SparseDev& sd = mv.col[k];
       const uint32_t m16 = sd.mask16[c * 64 + lane];
       const uint32_t co = cand & m16;
-      const uint4 w = sparse_bytes(sd, c, lane, m16, co);
-      acc = max(acc, masked_max_plus1(w, co));
+      acc = max(acc, sparse_max_plus1(lane_outliers(sd, c, lane, m16), m16, co));
       if (SEM && (cand & ~m16)) acc = max(acc, sd.major + 1u);
     } else if (KIND == kStDense) {
       if (__ballot(cand != 0)) {
@@ -790,14 +771,28 @@ hipError_t launch_major_filter(const MajorView& mv, int op, bool semantic, const
                  : launch_filter_mode<1>(mv, semantic, tg, out, nwords, s, ls.max_wg);
 }
 
+template <int SEM>
+static hipError_t launch_decode_sem(const MajorView& mv, double inv_scale, double* out, uint32_t grid,
+                                    hipStream_t s) {
+  switch (mv.S) {
+    case 0: hipLaunchKernelGGL((k_major_decode<SEM, 0>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 1: hipLaunchKernelGGL((k_major_decode<SEM, 1>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 2: hipLaunchKernelGGL((k_major_decode<SEM, 2>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 3: hipLaunchKernelGGL((k_major_decode<SEM, 3>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 4: hipLaunchKernelGGL((k_major_decode<SEM, 4>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 5: hipLaunchKernelGGL((k_major_decode<SEM, 5>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 6: hipLaunchKernelGGL((k_major_decode<SEM, 6>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 7: hipLaunchKernelGGL((k_major_decode<SEM, 7>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 8: hipLaunchKernelGGL((k_major_decode<SEM, 8>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_major_decode(const MajorView& mv, bool semantic, double inv_scale, double* out, hipStream_t s,
                                const LaunchShape& ls) {
   const uint32_t grid = tiles_grid(mv.dense.ntiles, ls.max_wg);
-  if (semantic)
-    hipLaunchKernelGGL(k_major_decode<1>, dim3(grid), dim3(256), 0, s, mv, inv_scale, out);
-  else
-    hipLaunchKernelGGL(k_major_decode<0>, dim3(grid), dim3(256), 0, s, mv, inv_scale, out);
-  return hipGetLastError();
+  return semantic ? launch_decode_sem<1>(mv, inv_scale, out, grid, s) : launch_decode_sem<0>(mv, inv_scale, out, grid, s);
 }
 
 static constexpr uint32_t kOutsumBlocks = 64;
