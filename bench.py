@@ -17,7 +17,18 @@ move) over its average duration against 8 TB/s. A cold-cache pass (the
 memory-side cache refilled with 1 GiB of other, clean data before each launch)
 is reported beside it.
 
+Further sections of the same JSON line (each skippable):
+  config3 — an outlier-heavy column of 67,108,864 rows in buff and buff-major:
+            per-operation device time and roofline fractions;
+  config4 — 4096 chunks of uneven length and precision: one-launch batch
+            encode, 16,384 mixed queries as one graph launch against one call
+            per query;
+  config5 — chunks encoded until 48 GB are resident, a request mix served by
+            the native server with continuous batching against one launch per
+            request, saturated and under Poisson arrivals (p50/p99).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N] [--no-cpu]
+       [--no-config3] [--no-config4] [--no-config5]
 """
 import argparse
 import json
