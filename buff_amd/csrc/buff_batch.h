@@ -81,6 +81,8 @@ struct BHeader {
 };
 
 struct BatchBuffers {
+  const uint8_t* host_in;            // pinned host: BHeader at 0, nq BQuery at kHostQueries
+  BResult* host_res;                 // pinned host: nq results, written by the last kernel
   const BHeader* hdr;
   const ChunkView* views;
   const BQuery* q;
@@ -102,6 +104,8 @@ struct BatchTuning {
   uint32_t tiles_per_item;
   uint32_t pad;
 };
+
+constexpr uint64_t kHostQueries = 256;   // byte offset of the queries in host_in
 
 // Kernels a batch needs: the scan (filters, sums, maxima and their fold), the
 // max refinement, the materialization.

@@ -139,6 +139,7 @@ hipError_t launch_batch_encode(const BEncChunk* chunks, const BItem* items, uint
 // ----------------------------------------------------------------------------
 // Batch queries. One wave works on one item at a time (items of different
 // queries interleave freely; the item loops hold no workgroup barrier):
+//   k_bq_fetch  copies the header and query descriptors from pinned host memory;
 //   k_bq_items  writes the work items from the per-query item ranges;
 //   k_bq_scan   items [0, nscan): filters write tile bitmaps and count them,
 //               sums add the sub-column bytes and tail bits, maxima record every
@@ -148,7 +149,8 @@ hipError_t launch_batch_encode(const BEncChunk* chunks, const BItem* items, uint
 //               give its max offset;
 //   k_bq_mat    items [nscan, nitems): materializations, staged through LDS for
 //               coalesced f64 stores (kept apart so the scan kernel holds no LDS
-//               and runs at full occupancy).
+//               and runs at full occupancy);
+//   k_bq_publish copies the results to pinned host memory.
 // Per-tile loads are issued together: a filter reads the top byte plane, then
 // every remaining plane and tail bit-plane of the tied rows at once; sums and
 // materializations read all their planes before combining them.
@@ -436,6 +438,35 @@ __global__ __launch_bounds__(256) void k_bq_scan(const BatchBuffers* __restrict_
   }
 }
 
+// Copies the batch's header and query descriptors from pinned host memory to
+// device memory (the host only writes them and launches the graph: no runtime
+// copy, which could queue behind another slot's transfers).
+__global__ __launch_bounds__(256) void k_bq_fetch(const BatchBuffers* __restrict__ bbp) {
+  const BatchBuffers bb = cload(bbp);
+  const BUFF_GLOBAL uint32_t* hin = gmem(reinterpret_cast<const uint32_t*>(bb.host_in));
+  BUFF_GLOBAL uint32_t* hdr = gmem_w(reinterpret_cast<uint32_t*>(const_cast<BHeader*>(bb.hdr)));
+  const uint32_t nq = hin[0];
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nthreads = (uint64_t)gridDim.x * blockDim.x;
+  if (tid < sizeof(BHeader) / 4) hdr[tid] = hin[tid];
+  const uint64_t nwords = (uint64_t)nq * (sizeof(BQuery) / 4);
+  const BUFF_GLOBAL uint32_t* src = hin + kHostQueries / 4;
+  BUFF_GLOBAL uint32_t* dst = gmem_w(reinterpret_cast<uint32_t*>(const_cast<BQuery*>(bb.q)));
+  for (uint64_t i = tid; i < nwords; i += nthreads) dst[i] = src[i];
+}
+
+// Writes the batch's results to pinned host memory, visible to the host once
+// the batch's completion event has fired.
+__global__ __launch_bounds__(256) void k_bq_publish(const BatchBuffers* __restrict__ bbp) {
+  const BatchBuffers bb = cload(bbp);
+  const uint32_t nq = cmem(bb.hdr)->nq;
+  const uint64_t nwords = (uint64_t)nq * (sizeof(BResult) / 8);
+  const BUFF_GLOBAL uint64_t* src = gmem(reinterpret_cast<const uint64_t*>(bb.res));
+  BUFF_GLOBAL uint64_t* dst = gmem_w(reinterpret_cast<uint64_t*>(bb.host_res));
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nthreads = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = tid; i < nwords; i += nthreads) dst[i] = src[i];
+  __threadfence_system();
+}
+
 // One workgroup per query: writes the query's work items (tile ranges of
 // tiles_per_item tiles over its chunk) at the range the host assigned it.
 __global__ __launch_bounds__(256) void k_bq_items(const BatchBuffers* __restrict__ bbp) {
@@ -663,6 +694,7 @@ __global__ __launch_bounds__(256) void k_bq_mat(const BatchBuffers* __restrict__
 }
 
 hipError_t launch_batch_queries(const BatchBuffers* bb, const BatchTuning& tu, uint32_t parts, hipStream_t s) {
+  hipLaunchKernelGGL(k_bq_fetch, dim3(64), dim3(256), 0, s, bb);
   hipLaunchKernelGGL(k_bq_items, dim3(1024), dim3(256), 0, s, bb);
   if (parts & kBqPartScan) {
     hipLaunchKernelGGL(k_bq_scan, dim3(tu.grid), dim3(256), 0, s, bb, tu);
@@ -674,6 +706,7 @@ hipError_t launch_batch_queries(const BatchBuffers* bb, const BatchTuning& tu, u
   }
   // four resident workgroups per CU fit the LDS staging of k_bq_mat
   if (parts & kBqPartMat) hipLaunchKernelGGL(k_bq_mat, dim3(tu.grid < 1024 ? tu.grid : 1024), dim3(256), 0, s, bb);
+  hipLaunchKernelGGL(k_bq_publish, dim3(16), dim3(256), 0, s, bb);
   return hipGetLastError();
 }
 

@@ -44,6 +44,21 @@ int ensure(Ctx* ctx, DevBuf& b, uint64_t bytes) {
   return BUFF_OK;
 }
 
+// Exactly `bytes` (no growth slack), for buffers sized up front.
+int reserve(Ctx* ctx, DevBuf& b, uint64_t bytes) {
+  bytes = round_up(std::max<uint64_t>(bytes, 256), 256);
+  if (b.bytes >= bytes) return BUFF_OK;
+  if (b.p) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    (void)ctx->release(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  BUFF_TRY(ctx->alloc(bytes, &b.p));
+  b.bytes = bytes;
+  return BUFF_OK;
+}
+
 void drop(Ctx* ctx, DevBuf& b) {
   if (b.p) (void)ctx->release(b.p);
   b.p = nullptr;
@@ -65,7 +80,12 @@ struct QueryExec {
   bool bb_valid = false;
   std::map<uint64_t, hipGraphExec_t> graphs;   // by (grid, parts, first_planes)
   std::vector<BQuery> hq;
-  std::vector<BResult> hres;
+  // pinned host staging [header | queries | results] so the copies are
+  // asynchronous (a copy to or from pageable memory can hold the host thread
+  // until the stream has drained, which would serialize a server's slots)
+  uint8_t* pin = nullptr;       // host address
+  uint8_t* pin_dev = nullptr;   // the same memory as the kernels address it
+  uint32_t pin_cap = 0;
   std::vector<buff_query_result> res_host;
   BHeader h{};
   uint32_t nq = 0;
@@ -243,12 +263,74 @@ int exec_init(QueryExec& ex, Ctx* ctx, bool own_stream) {
   return BUFF_OK;
 }
 
+int graph_for(QueryExec& ex, const BatchTuning& tu, uint32_t parts, hipGraphExec_t* out);
+
+uint64_t pin_queries_off() { return kHostQueries; }
+uint64_t pin_results_off(uint32_t cap) { return 256 + round_up(sizeof(BQuery) * (uint64_t)cap, 256); }
+
+int pin_reserve(QueryExec& ex, uint32_t nq) {
+  if (ex.pin && ex.pin_cap >= nq) return BUFF_OK;
+  const uint32_t cap = std::max<uint32_t>(nq, 64);
+  if (ex.pin) {
+    HIP_TRY(hipStreamSynchronize(ex.stream));
+    (void)hipHostFree(ex.pin);
+    ex.pin = nullptr;
+    ex.pin_dev = nullptr;
+    ex.pin_cap = 0;
+  }
+  const uint64_t bytes = pin_results_off(cap) + sizeof(BResult) * (uint64_t)cap;
+  // coherent (fine-grained): the kernels read and write it over the bus, uncached
+  void* p = nullptr;
+  hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocMapped | hipHostMallocCoherent);
+  if (e != hipSuccess) return fail(BUFF_E_OOM, "pinned host staging (%llu bytes): %s", (unsigned long long)bytes,
+                                   hipGetErrorString(e));
+  void* d = nullptr;
+  e = hipHostGetDevicePointer(&d, p, 0);
+  if (e != hipSuccess) {
+    (void)hipHostFree(p);
+    return fail(BUFF_E_DEVICE, "pinned host staging device address: %s", hipGetErrorString(e));
+  }
+  ex.pin = static_cast<uint8_t*>(p);
+  ex.pin_dev = static_cast<uint8_t*>(d);
+  ex.pin_cap = cap;
+  return BUFF_OK;
+}
+
+// Sizes a server slot's buffers and captures its graphs before it serves:
+// device allocations and graph instantiation can wait for work already running
+// on the device, which would serialize the slots. Larger batches still grow
+// the buffers when they need to.
+int exec_reserve(QueryExec& ex, uint32_t max_q, uint64_t arena_bytes, const BatchTuning& tu) {
+  Ctx* ctx = ex.ctx;
+  constexpr uint64_t kItems = 1u << 20;
+  BUFF_TRY(reserve(ctx, ex.hdr, sizeof(BHeader)));
+  BUFF_TRY(reserve(ctx, ex.q, sizeof(BQuery) * (uint64_t)max_q));
+  BUFF_TRY(reserve(ctx, ex.res, sizeof(BResult) * (uint64_t)max_q));
+  BUFF_TRY(reserve(ctx, ex.items, sizeof(BItem) * kItems));
+  BUFF_TRY(reserve(ctx, ex.acc, 8ull * (kMaxPlanes + 1) * kItems));
+  BUFF_TRY(reserve(ctx, ex.top, 64ull << 20));
+  BUFF_TRY(reserve(ctx, ex.arena, arena_bytes));
+  BUFF_TRY(reserve(ctx, ex.bbdev, sizeof(BatchBuffers)));
+  BUFF_TRY(pin_reserve(ex, max_q));
+  const uint32_t all_parts[4] = {kBqPartScan, kBqPartScan | kBqPartMax, kBqPartScan | kBqPartMat,
+                                 kBqPartScan | kBqPartMax | kBqPartMat};
+  for (uint32_t parts : all_parts) {
+    hipGraphExec_t gx = nullptr;
+    BUFF_TRY(graph_for(ex, tu, parts, &gx));
+  }
+  return BUFF_OK;
+}
+
 void exec_free(QueryExec& ex) {
   if (!ex.ctx) return;
   (void)hipStreamSynchronize(ex.stream);
   for (auto& kv : ex.graphs) (void)hipGraphExecDestroy(kv.second);
   ex.graphs.clear();
   for (DevBuf* d : {&ex.hdr, &ex.q, &ex.items, &ex.acc, &ex.res, &ex.top, &ex.arena, &ex.bbdev}) drop(ex.ctx, *d);
+  if (ex.pin) (void)hipHostFree(ex.pin);
+  ex.pin = nullptr;
+  ex.pin_dev = nullptr;
+  ex.pin_cap = 0;
   if (ex.done) (void)hipEventDestroy(ex.done);
   if (ex.own_stream) (void)hipStreamDestroy(ex.stream);
   ex.done = nullptr;
@@ -388,6 +470,7 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
     ex.hq[k].out = addr;
     ex.res_host[k].out_offset = addr;
   }
+  BUFF_TRY(pin_reserve(ex, nq));
   BatchBuffers bb;
   bb.hdr = (const BHeader*)ex.hdr.p;
   bb.views = (const ChunkView*)views_dev;
@@ -396,6 +479,8 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
   bb.tile_top = (uint8_t*)ex.top.p;
   bb.item_acc = (unsigned long long*)ex.acc.p;
   bb.res = (BResult*)ex.res.p;
+  bb.host_in = ex.pin_dev;
+  bb.host_res = (BResult*)(ex.pin_dev + pin_results_off(ex.pin_cap));
   if (!ex.bb_valid || memcmp(&bb, &ex.bb, sizeof(bb)) != 0) {
     HIP_TRY(hipMemcpyAsync(ex.bbdev.p, &bb, sizeof(bb), hipMemcpyHostToDevice, ex.stream));
     HIP_TRY(hipStreamSynchronize(ex.stream));   // bb is a stack copy
@@ -404,8 +489,8 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
   }
   if (nq == 0) return BUFF_OK;
   ex.h = BHeader{nq, nitems, nscan, (uint32_t)T};
-  HIP_TRY(hipMemcpyAsync(ex.hdr.p, &ex.h, sizeof(ex.h), hipMemcpyHostToDevice, ex.stream));
-  HIP_TRY(hipMemcpyAsync(ex.q.p, ex.hq.data(), sizeof(BQuery) * nq, hipMemcpyHostToDevice, ex.stream));
+  memcpy(ex.pin, &ex.h, sizeof(ex.h));
+  memcpy(ex.pin + pin_queries_off(), ex.hq.data(), sizeof(BQuery) * nq);
   uint32_t parts = 0;
   for (const BQuery& q : ex.hq) {
     if (!q.nitems) continue;
@@ -416,8 +501,6 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
   hipGraphExec_t gx = nullptr;
   BUFF_TRY(graph_for(ex, tu, parts, &gx));
   HIP_TRY(hipGraphLaunch(gx, ex.stream));
-  ex.hres.resize(nq);
-  HIP_TRY(hipMemcpyAsync(ex.hres.data(), ex.res.p, sizeof(BResult) * nq, hipMemcpyDeviceToHost, ex.stream));
   HIP_TRY(hipEventRecord(ex.done, ex.stream));
   return BUFF_OK;
 }
@@ -429,7 +512,7 @@ int exec_finish(QueryExec& ex, const std::vector<Chunk*>& chunks, const buff_que
     buff_query_result& r = ex.res_host[k];
     if (r.status != BUFF_OK || qs[k].chunk >= chunks.size()) continue;
     const Chunk* ch = chunks[qs[k].chunk];
-    const BResult& br = ex.hres[k];
+    const BResult& br = reinterpret_cast<const BResult*>(ex.pin + pin_results_off(ex.pin_cap))[k];
     switch (qs[k].op) {
       case BUFF_Q_GT:
       case BUFF_Q_EQ:
@@ -792,6 +875,7 @@ int buff_server_create(buff_ctx* ctx, const buff_server_cfg* cfg, buff_server** 
   sv->slot_busy.assign(sv->cfg.slots, false);
   for (auto& ex : sv->slots) {
     int rc = exec_init(ex, c, true);
+    if (rc == BUFF_OK) rc = exec_reserve(ex, sv->cfg.max_batch, sv->cfg.arena_bytes, ctx_tuning(c));
     if (rc != BUFF_OK) {
       for (auto& e2 : sv->slots) exec_free(e2);
       delete sv;
@@ -838,9 +922,12 @@ int buff_server_wait(buff_server* server, uint64_t ticket, buff_query_result* re
   Server* sv = reinterpret_cast<Server*>(server);
   if (!sv) return fail(BUFF_E_INVALID, "null argument");
   std::unique_lock<std::mutex> lk(sv->mu);
+  if (sv->reqs.find(ticket) == sv->reqs.end())
+    return fail(BUFF_E_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
+  // look the request up again after every wake-up: submissions from other
+  // threads insert into the map meanwhile and may rehash it
+  sv->cv_done.wait(lk, [&] { return sv->reqs.find(ticket)->second.done; });
   auto it = sv->reqs.find(ticket);
-  if (it == sv->reqs.end()) return fail(BUFF_E_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
-  sv->cv_done.wait(lk, [&] { return it->second.done; });
   if (res) *res = it->second.res;
   if (latency_us) *latency_us = (it->second.t_done - it->second.t_submit) * 1e6;
   sv->reqs.erase(it);
