@@ -10,8 +10,9 @@ and at 2 decimals and the uniform column at 2 decimals (its 4-decimal request
 is a reference panic). `value` is GB/s of float64 input: 8 bytes per row per
 operation over the step time.
 
-Per-kernel durations come from HIP events recorded on the library's stream
-during the timed steps. `roofline` is reported for the kernel with the largest
+Per-kernel durations come from HIP events attached to the kernel launches
+(start of the operation's first kernel to end of its last) during a repetition
+of the timed steps. `roofline` is reported for the kernel with the largest
 share of the step, from its algorithmic bytes (HBM bytes an ideal kernel must
 move) over its average duration against 8 TB/s. A cold-cache pass (the
 memory-side cache refilled with 1 GiB of other, clean data before each launch)
@@ -503,8 +504,6 @@ def main():
     for _ in range(args.warmup):
         step()
     ctx.synchronize()
-    ctx.prof_reset()
-    ctx.prof_enable(True)
     log("bench: timing %d steps" % args.steps)
     ctx.synchronize()
     t = time.perf_counter()
@@ -514,6 +513,15 @@ def main():
         log("bench: step %d done" % (i + 1))
     ctx.synchronize()
     dt = time.perf_counter() - t
+    # The same K steps again with per-kernel HIP events (each profiled kernel
+    # launch carries start/stop events that time its execution): launching
+    # with events costs host time between operations, so it stays out of the
+    # timed steps above.
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    for i in range(args.steps):
+        step()
+    ctx.synchronize()
     ctx.prof_enable(False)
     ms_per_step = dt * 1e3 / args.steps
     logical_bytes = 8.0 * n * ops

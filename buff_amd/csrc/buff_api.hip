@@ -22,6 +22,7 @@
 #include "buff_common.h"
 #include "buff_kernels.h"
 #include "buff_runtime.h"
+#include "buff_launch.h"
 
 namespace buffgpu {
 
@@ -166,7 +167,7 @@ int Ctx::scratch(Scratch& s, uint64_t bytes) {
   return BUFF_OK;
 }
 
-void Ctx::prof_begin(const char* name, hipEvent_t* ev) {
+void Ctx::prof_begin(const char* name, hipEvent_t* ev, bool kernels) {
   ev[0] = ev[1] = nullptr;
   if (!prof_on) return;
   for (int i = 0; i < 2; ++i) {
@@ -177,15 +178,33 @@ void Ctx::prof_begin(const char* name, hipEvent_t* ev) {
       prof_pool.pop_back();
     }
   }
-  (void)hipEventRecord(ev[0], stream);
+  if (kernels) {
+    LaunchEvents& le = launch_events();
+    le.start = ev[0];
+    le.stop = ev[1];
+    le.started = false;
+  } else {
+    (void)hipEventRecord(ev[0], stream);
+  }
   (void)name;
 }
 
-void Ctx::prof_end(const char* name, hipEvent_t* ev) {
+void Ctx::prof_end(const char* name, hipEvent_t* ev, bool kernels) {
   if (!ev[0]) return;
-  (void)hipEventRecord(ev[1], stream);
+  if (kernels) {
+    LaunchEvents& le = launch_events();
+    const bool started = le.started;
+    le = LaunchEvents{};
+    if (!started) {   // nothing launched: an empty interval
+      (void)hipEventRecord(ev[0], stream);
+      (void)hipEventRecord(ev[1], stream);
+    }
+  } else {
+    (void)hipEventRecord(ev[1], stream);
+  }
   prof_pending.push_back(ProfRec{name, ev[0], ev[1]});
 }
+
 
 int Ctx::prof_collect() {
   if (prof_pending.empty()) return BUFF_OK;
@@ -267,7 +286,7 @@ int quirk_minmax(Ctx* ctx, const double* x, uint64_t n, int32_t pexp, uint32_t d
   int64_t mn = INT64_MAX, mx = INT64_MIN;
   if (n > 0) {
     const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
-    const uint32_t nwg = (uint32_t)std::min<uint64_t>(ntiles, 1024);
+    const uint32_t nwg = minmax_grid(ntiles);
     BUFF_TRY(ctx->scratch(ctx->s_partial, 3ull * nwg * sizeof(int64_t)));
     BUFF_TRY(ctx->scratch(ctx->s_minmax, sizeof(MinMaxOut)));
     MinMaxOut* dmm = (MinMaxOut*)ctx->s_minmax.ptr;

@@ -16,7 +16,7 @@ namespace buffgpu {
 __global__ __launch_bounds__(256) void k_bminmax(const BEncChunk* __restrict__ chunks, const BItem* __restrict__ items,
                                                  uint32_t nitems, int64_t* __restrict__ part) {
   __shared__ int64_t smn[4], smx[4];
-  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t lane = __lane_id(), wave = wave_in_wg();
   for (uint32_t i = blockIdx.x; i < nitems; i += gridDim.x) {
     const BItem it = items[i];
     const BEncChunk& ck = chunks[it.owner];
@@ -55,8 +55,8 @@ __global__ __launch_bounds__(256) void k_bminmax(const BEncChunk* __restrict__ c
 __global__ __launch_bounds__(256) void k_bminmax_final(const BEncChunk* __restrict__ chunks, uint32_t nchunks,
                                                        const int64_t* __restrict__ part, BMinMax* __restrict__ out) {
   const uint32_t lane = __lane_id();
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
   for (uint64_t k = wave; k < nchunks; k += nwaves) {
     const BEncChunk& ck = chunks[k];
     int64_t mn = INT64_MAX, mx = INT64_MIN;
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void k_bminmax_final(const BEncChunk* __restri
 __global__ __launch_bounds__(256) void k_bencode(const BEncChunk* __restrict__ chunks, const BItem* __restrict__ items,
                                                  uint32_t nitems) {
   __shared__ uint64_t s_off[4][64 * kLdsRow];
-  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t lane = __lane_id(), wave = wave_in_wg();
   uint64_t* so = s_off[wave];
   for (uint32_t i = blockIdx.x; i < nitems; i += gridDim.x) {
     const BItem it = items[i];
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(256) void k_bq_items(const BatchBuffers* __restrict
 __global__ __launch_bounds__(256) void k_bq_reduce(const BatchBuffers* __restrict__ bbp) {
   __shared__ unsigned long long s_part[4][kMaxPlanes + 1];
   const BatchBuffers bb = cload(bbp);
-  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t lane = __lane_id(), wave = wave_in_wg();
   const uint32_t nq = cmem(bb.hdr)->nq;
   for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
     const BQuery q = cload(bb.q, qi);

@@ -148,7 +148,7 @@ int batch_encode(Ctx* ctx, const double* x, const uint64_t* row_off, const uint6
     HIP_TRY(hipMemcpyAsync(d_ck, ck.data(), sizeof(BEncChunk) * nchunks, hipMemcpyHostToDevice, ctx->stream));
     if (nitems) HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(BItem) * nitems, hipMemcpyHostToDevice, ctx->stream));
     {
-      ProfScope ps(ctx, "k_batch_minmax");
+      ProfScope ps(ctx, "k_batch_minmax", false);
       HIP_TRY(launch_batch_minmax(d_ck, nchunks, d_items, nitems, (int64_t*)(sb + o_part), (BMinMax*)(sb + o_mm),
                                   std::max<uint32_t>(1, std::min<uint32_t>(nitems, 4096)), ctx->stream));
     }
@@ -208,7 +208,7 @@ int batch_encode(Ctx* ctx, const double* x, const uint64_t* row_off, const uint6
   if (nchunks) {
     e = hipMemcpyAsync(d_ck, ck.data(), sizeof(BEncChunk) * nchunks, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
-      ProfScope ps(ctx, "k_batch_encode");
+      ProfScope ps(ctx, "k_batch_encode", false);
       e = launch_batch_encode(d_ck, d_items, nitems, std::max<uint32_t>(1, std::min<uint32_t>(nitems, 4096)),
                               ctx->stream);
     }
@@ -563,7 +563,7 @@ int batch_query(Ctx* ctx, Batch* b, const buff_query* qs, uint32_t nq, buff_quer
   }
   int rc;
   {
-    ProfScope ps(ctx, "k_batch_graph");
+    ProfScope ps(ctx, "k_batch_graph", false);
     rc = exec_launch(b->ex, b->chunks, b->views.p, qs, nq, nullptr, tu);
   }
   if (rc == BUFF_OK) rc = exec_finish(b->ex, b->chunks, qs);

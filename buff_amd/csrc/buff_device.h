@@ -65,6 +65,15 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
   return v;
 }
 
+// Wave indices as the compiler can see they are wave-uniform (so tile and
+// plane addresses derived from them stay in scalar registers and loads use a
+// scalar base with a per-lane offset).
+__device__ __forceinline__ uint32_t wave_in_wg() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+__device__ __forceinline__ uint64_t wave_global() {
+  return (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave_in_wg();
+}
+__device__ __forceinline__ uint64_t waves_in_grid() { return (uint64_t)gridDim.x * (blockDim.x >> 6); }
+
 __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
   uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
   uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
@@ -220,7 +229,7 @@ __device__ __forceinline__ void refine_tail(const ChunkView& cv, uint64_t c, uin
   uint32_t h[8];
   const uint32_t src = eqm ? lane : 0u;
 #pragma unroll
-  for (uint32_t j = 0; j < 8; ++j) h[j] = load_tail(cv, min(j, cv.r - 1), c, src);
+  for (uint32_t j = 0; j < 8; ++j) h[j] = j < cv.r ? load_tail(cv, j, c, src) : 0u;   // only the r bit-planes
 #pragma unroll
   for (int j = 7; j >= 0; --j) {
     if ((uint32_t)j >= cv.r) continue;

@@ -28,6 +28,8 @@ struct MatParams {
   double inv_scale;  // exactly 2^-dlen
 };
 
+// Workgroups of the min/max pass over ntiles tiles (all resident at once).
+uint32_t minmax_grid(uint64_t ntiles);
 hipError_t launch_minmax(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t* partial,
                          int64_t* partial2, MinMaxOut* out, uint32_t nwg, bool prefix, hipStream_t s);
 hipError_t launch_encode(const double* x, const ChunkView& cv, int32_t pexp, hipStream_t s, const LaunchShape& ls);

@@ -14,9 +14,15 @@
 
 #include "buff_common.h"
 #include "buff_device.h"
+#include "buff_launch.h"
 #include "buff_kernels.h"
 
 namespace buffgpu {
+
+LaunchEvents& launch_events() {
+  static thread_local LaunchEvents le;
+  return le;
+}
 
 // ----------------------------------------------------------------------------
 // K1: fixed-point conversion + min / max reduction (encode pass 1).
@@ -29,7 +35,7 @@ __global__ __launch_bounds__(256) void k_minmax(const double* __restrict__ x, ui
                                                 uint64_t tpw, int32_t pexp, uint32_t dlen,
                                                 int64_t* __restrict__ partial) {
   const uint32_t lane = __lane_id();
-  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t wave = wave_in_wg();
   const uint64_t t0 = (uint64_t)blockIdx.x * tpw;
   const uint64_t t1 = min(t0 + tpw, ntiles);
   int64_t mn = INT64_MAX, mx = INT64_MIN;
@@ -71,7 +77,7 @@ __global__ __launch_bounds__(256) void k_minmax(const double* __restrict__ x, ui
 __global__ __launch_bounds__(256) void k_minmax_final(const double* __restrict__ x, uint64_t n, int32_t pexp,
                                                       uint32_t dlen, const int64_t* __restrict__ partial,
                                                       uint32_t nwg, MinMaxOut* __restrict__ out) {
-  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t lane = __lane_id(), wave = wave_in_wg();
   __shared__ int64_t smn[4], smx[4];
   int64_t a = INT64_MAX, b = INT64_MIN;
   for (uint32_t i = threadIdx.x; i < nwg; i += blockDim.x) {
@@ -147,10 +153,10 @@ __global__ __launch_bounds__(256) void k_encode(const double* __restrict__ x, ui
                                                 uint16_t* __restrict__ tail) {
   __shared__ uint64_t s_off[4][64 * kLdsRow];
   const uint32_t lane = __lane_id();
-  const uint32_t wslot = threadIdx.x >> 6;
+  const uint32_t wslot = wave_in_wg();
   uint64_t* so = s_off[wslot];
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
   for (uint64_t c = wave; c < ntiles; c += nwaves) {
     uint64_t v[16];
     const bool full = tile_full(n, c);
@@ -199,8 +205,8 @@ template <int MODE, int U>
 __global__ __launch_bounds__(256) void k_filter(ChunkView cv, FilterTarget tg, uint64_t* __restrict__ out,
                                                 uint64_t nwords, unsigned long long* __restrict__ stats) {
   const uint32_t lane = __lane_id();
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
   const uint64_t ngroups = (cv.ntiles + U - 1) / U;
   const bool has_planes = cv.P > 0;
   const Digit d0 = make_digit(tg.digit[0]);
@@ -316,10 +322,10 @@ __global__ void k_touch_bytes(const uint4* __restrict__ in, uint64_t n16, uint32
 // workgroup writes one row of partial sums; k_sum_final folds the rows.
 
 template <int P, int R, int U>
-__global__ __launch_bounds__(256) void k_sum(ChunkView cv, unsigned long long* __restrict__ sums) {
+__global__ __launch_bounds__(256) void k_sum(ChunkView cv, unsigned long long* __restrict__ rows) {
   const uint32_t lane = __lane_id();
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
   const uint64_t ngroups = (cv.ntiles + U - 1) / U;
   constexpr int PP = P > 0 ? P : 1;
   constexpr int RR = R > 0 ? R : 1;
@@ -364,48 +370,62 @@ __global__ __launch_bounds__(256) void k_sum(ChunkView cv, unsigned long long* _
     }
   }
   // wave sums -> workgroup row in LDS -> one partial row per workgroup
-  __shared__ uint64_t rows[16][kMaxPlanes + 1];
-  const uint32_t wslot = threadIdx.x >> 6;
+  // (a last-workgroup fold inside this kernel would need a device-scope
+  // release per workgroup, which writes back the XCD's L2; a second tiny
+  // launch is cheaper)
+  __shared__ uint64_t wrow[16][kMaxPlanes + 1];
+  const uint32_t wslot = wave_in_wg();
 #pragma unroll
   for (int p = 0; p < P; ++p) {
     const uint64_t s = wave_sum_u64(wide[p] + acc[p]);
-    if (lane == 0) rows[wslot][p] = s;
+    if (lane == 0) wrow[wslot][p] = s;
   }
   uint64_t t = 0;
 #pragma unroll
   for (int j = 0; j < R; ++j) t += (uint64_t)tacc[j] << j;
   t = wave_sum_u64(t);
-  if (lane == 0) rows[wslot][P] = t;
+  if (lane == 0) wrow[wslot][P] = t;
   __syncthreads();
   if (threadIdx.x <= (uint32_t)P) {
     uint64_t v = 0;
-    for (uint32_t k = 0; k < (blockDim.x >> 6); ++k) v += rows[k][threadIdx.x];
-    sums[(uint64_t)blockIdx.x * (kMaxPlanes + 1) + threadIdx.x] = v;
+    for (uint32_t k = 0; k < (blockDim.x >> 6); ++k) v += wrow[k][threadIdx.x];
+    rows[(uint64_t)blockIdx.x * (kMaxPlanes + 1) + threadIdx.x] = v;
   }
 }
 
-// Sums the per-workgroup rows of k_sum (one workgroup).
+// Sums the per-workgroup rows of k_sum (one workgroup; each thread keeps its
+// rows' loads in flight together).
+template <int W>
 __global__ __launch_bounds__(256) void k_sum_final(const unsigned long long* __restrict__ rows, uint32_t nrows,
-                                                   uint32_t width, unsigned long long* __restrict__ out) {
+                                                   unsigned long long* __restrict__ out) {
   __shared__ unsigned long long part[256 / 64][kMaxPlanes + 1];
-  const uint32_t lane = __lane_id(), wslot = threadIdx.x >> 6;
-  uint64_t acc[kMaxPlanes + 1];
+  const uint32_t lane = __lane_id(), wslot = wave_in_wg();
+  uint64_t acc[W];
 #pragma unroll
-  for (uint32_t k = 0; k <= kMaxPlanes; ++k) acc[k] = 0;
-  for (uint32_t i = threadIdx.x; i < nrows; i += blockDim.x) {
+  for (int k = 0; k < W; ++k) acc[k] = 0;
+  for (uint32_t i0 = threadIdx.x; i0 < nrows; i0 += 8 * blockDim.x) {
+    uint64_t v[8][W];
 #pragma unroll
-    for (uint32_t k = 0; k <= kMaxPlanes; ++k) acc[k] += rows[(uint64_t)i * (kMaxPlanes + 1) + k];
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t i = i0 + u * blockDim.x;
+#pragma unroll
+      for (int k = 0; k < W; ++k) v[u][k] = i < nrows ? rows[(uint64_t)i * (kMaxPlanes + 1) + k] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int k = 0; k < W; ++k) acc[k] += v[u][k];
   }
 #pragma unroll
-  for (uint32_t k = 0; k <= kMaxPlanes; ++k) {
+  for (int k = 0; k < W; ++k) {
     const uint64_t v = wave_sum_u64(acc[k]);
     if (lane == 0) part[wslot][k] = v;
   }
   __syncthreads();
-  if (threadIdx.x < kMaxPlanes + 1)
-    out[threadIdx.x] = threadIdx.x < width ? part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
-                                                  part[3][threadIdx.x]
-                                            : 0ull;
+  if (threadIdx.x <= kMaxPlanes)
+    out[threadIdx.x] = threadIdx.x < (uint32_t)W ? part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                                                       part[3][threadIdx.x]
+                                                 : 0ull;
 }
 
 // ----------------------------------------------------------------------------
@@ -420,8 +440,8 @@ template <int U>
 __global__ __launch_bounds__(256) void k_max_top(ChunkView cv, uint8_t* __restrict__ tile_max,
                                                  unsigned int* __restrict__ top) {
   const uint32_t lane = __lane_id();
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
   const uint64_t ngroups = (cv.ntiles + U - 1) / U;
   uint32_t best = 0;
   for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
@@ -464,8 +484,8 @@ __global__ __launch_bounds__(256) void k_max_refine(ChunkView cv, const uint8_t*
                                                     const unsigned int* __restrict__ top, uint32_t ntop,
                                                     unsigned long long* __restrict__ best_out) {
   const uint32_t lane = __lane_id();
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
   const bool has_planes = cv.P > 0;
   const uint32_t t = has_planes ? top[ntop] : 0u;
   uint64_t best = 0;
@@ -513,10 +533,10 @@ template <int NP, int NT, int U>
 __global__ __launch_bounds__(256) void k_materialize(ChunkView cv, MatParams mp, double* __restrict__ out) {
   __shared__ double s_val[4][64 * kLdsRow];
   const uint32_t lane = __lane_id();
-  const uint32_t wslot = threadIdx.x >> 6;
+  const uint32_t wslot = wave_in_wg();
   double* sv = s_val[wslot];
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
   const uint64_t ngroups = (cv.ntiles + U - 1) / U;
   const uint32_t lo = cv.r - NT;   // lowest tail bit kept
   constexpr int NPP = NP > 0 ? NP : 1;
@@ -603,8 +623,8 @@ __global__ __launch_bounds__(256) void k_materialize_ref(ChunkView cv, MatParams
 __global__ __launch_bounds__(256) void k_serialize(ChunkView cv, uint8_t flipmask, uint8_t* __restrict__ body,
                                                    uint64_t tail_bytes) {
   const uint32_t lane = __lane_id();
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
   for (uint64_t c = wave; c < cv.ntiles; c += nwaves) {
     const uint64_t r0 = c * kTileRows + 16ull * lane;
     for (uint32_t p = 0; p < cv.P; ++p) {
@@ -637,8 +657,8 @@ __global__ __launch_bounds__(256) void k_serialize(ChunkView cv, uint8_t flipmas
 __global__ __launch_bounds__(256) void k_load(ChunkView cv, uint8_t flipmask, const uint8_t* __restrict__ body,
                                               uint64_t tail_bytes) {
   const uint32_t lane = __lane_id();
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
   for (uint64_t c = wave; c < cv.ntiles; c += nwaves) {
     const uint64_t r0 = c * kTileRows + 16ull * lane;
     for (uint32_t p = 0; p < cv.P; ++p) {
@@ -676,18 +696,23 @@ static inline uint32_t grid_for(uint64_t units, uint32_t waves_per_wg, uint32_t 
   return (uint32_t)wgs;
 }
 
+uint32_t minmax_grid(uint64_t ntiles) {
+  // 1024 workgroups (16 waves per CU) measured faster than the ~1800 that fit
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({ntiles, 1024, resident_wgs(k_minmax, 256)}));
+}
+
 hipError_t launch_minmax(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t* partial,
                          int64_t* partial2, MinMaxOut* out, uint32_t nwg, bool prefix, hipStream_t s) {
   const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
   const uint64_t tpw = (ntiles + nwg - 1) / nwg;
   if (!prefix) {
-    hipLaunchKernelGGL(k_minmax, dim3(nwg), dim3(256), 0, s, x, n, ntiles, tpw, pexp, dlen, partial);
-    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, s, x, n, pexp, dlen, (const int64_t*)partial, nwg,
+    launch_k(k_minmax, dim3(nwg), dim3(256), 0, s, x, n, ntiles, tpw, pexp, dlen, partial);
+    launch_k(k_minmax_final, dim3(1), dim3(256), 0, s, x, n, pexp, dlen, (const int64_t*)partial, nwg,
                        out);
   } else {
-    hipLaunchKernelGGL(k_minmax_prefix, dim3(nwg), dim3(64), 0, s, x, n, ntiles, tpw, pexp, dlen,
+    launch_k(k_minmax_prefix, dim3(nwg), dim3(64), 0, s, x, n, ntiles, tpw, pexp, dlen,
                        (const int64_t*)partial, partial2);
-    hipLaunchKernelGGL(k_minmax_prefix_final, dim3(1), dim3(64), 0, s, (const int64_t*)partial2, nwg, out);
+    launch_k(k_minmax_prefix_final, dim3(1), dim3(64), 0, s, (const int64_t*)partial2, nwg, out);
   }
   return hipGetLastError();
 }
@@ -695,7 +720,7 @@ hipError_t launch_minmax(const double* x, uint64_t n, int32_t pexp, uint32_t dle
 hipError_t launch_encode(const double* x, const ChunkView& cv, int32_t pexp, hipStream_t s, const LaunchShape& ls) {
   // k_encode's LDS holds 4 waves
   const uint32_t wg = grid_for(cv.ntiles, 4, ls.max_wg);
-  hipLaunchKernelGGL(k_encode, dim3(wg), dim3(256), 0, s, x, cv.n, cv.ntiles, pexp, cv.dlen, cv.base, cv.F, cv.P,
+  launch_k(k_encode, dim3(wg), dim3(256), 0, s, x, cv.n, cv.ntiles, pexp, cv.dlen, cv.base, cv.F, cv.P,
                      cv.r, cv.planes, cv.pitch, cv.tail);
   return hipGetLastError();
 }
@@ -703,25 +728,28 @@ hipError_t launch_encode(const double* x, const ChunkView& cv, int32_t pexp, hip
 hipError_t launch_filter(const ChunkView& cv, int op, const FilterTarget& tg, uint64_t* out, uint64_t nwords,
                          unsigned long long* stats, hipStream_t s, const LaunchShape& ls) {
   constexpr int U = 4;
+  const uint32_t block = 64 * ls.waves_per_wg;
   const uint32_t wg = grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, ls.max_wg);
   if (op == 0)
-    hipLaunchKernelGGL((k_filter<0, U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, tg, out, nwords, stats);
+    launch_k((k_filter<0, U>), dim3(std::min(wg, resident_wgs(k_filter<0, U>, block))), dim3(block), 0, s, cv, tg,
+             out, nwords, stats);
   else
-    hipLaunchKernelGGL((k_filter<1, U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, tg, out, nwords, stats);
+    launch_k((k_filter<1, U>), dim3(std::min(wg, resident_wgs(k_filter<1, U>, block))), dim3(block), 0, s, cv, tg,
+             out, nwords, stats);
   return hipGetLastError();
 }
 
 hipError_t launch_fill_bitmap(uint64_t* out, uint64_t nwords, uint64_t n, int ones, hipStream_t s) {
   uint32_t wg = (uint32_t)std::min<uint64_t>((nwords + 255) / 256, 4096);
   if (wg == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_bitmap, dim3(wg), dim3(256), 0, s, out, nwords, n, ones);
+  launch_k(k_fill_bitmap, dim3(wg), dim3(256), 0, s, out, nwords, n, ones);
   return hipGetLastError();
 }
 
 hipError_t launch_fill_bytes(void* out, uint64_t bytes, uint8_t value, hipStream_t s) {
   const uint32_t v = 0x01010101u * value;
   const uint64_t n16 = bytes / 16;
-  if (n16) hipLaunchKernelGGL(k_fill_bytes, dim3(2048), dim3(256), 0, s, (uint4*)out, n16, v);
+  if (n16) launch_k(k_fill_bytes, dim3(2048), dim3(256), 0, s, (uint4*)out, n16, v);
   if (bytes % 16) {
     hipError_t e = hipMemsetAsync((uint8_t*)out + n16 * 16, value, bytes % 16, s);
     if (e != hipSuccess) return e;
@@ -731,18 +759,19 @@ hipError_t launch_fill_bytes(void* out, uint64_t bytes, uint8_t value, hipStream
 
 hipError_t launch_touch_bytes(const void* in, uint64_t bytes, uint32_t* sink, hipStream_t s) {
   const uint64_t n16 = bytes / 16;
-  if (n16) hipLaunchKernelGGL(k_touch_bytes, dim3(2048), dim3(256), 0, s, (const uint4*)in, n16, sink);
+  if (n16) launch_k(k_touch_bytes, dim3(2048), dim3(256), 0, s, (const uint4*)in, n16, sink);
   return hipGetLastError();
 }
 
 template <int P, int R>
 static void launch_sum_pr(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
   constexpr int U = (P + R) <= 3 ? 4 : 2;
-  const uint32_t wg = grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, ls.max_wg);
+  const uint32_t block = 64 * ls.waves_per_wg;
+  const uint32_t wg =
+      std::min(grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, ls.max_wg), resident_wgs(k_sum<P, R, U>, block));
   unsigned long long* rows = sums + (kMaxPlanes + 1);
-  hipLaunchKernelGGL((k_sum<P, R, U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, rows);
-  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, (const unsigned long long*)rows, wg, (uint32_t)(P + 1),
-                     sums);
+  launch_k((k_sum<P, R, U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, rows);
+  launch_k((k_sum_final<P + 1>), dim3(1), dim3(256), 0, s, (const unsigned long long*)rows, wg, sums);
 }
 
 uint64_t sum_scratch_bytes(const LaunchShape& ls) { return (uint64_t)(ls.max_wg + 1) * (kMaxPlanes + 1) * 8; }
@@ -787,12 +816,13 @@ hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* sc
   uint32_t wg1 = 0;
   if (cv.P > 0) {
     constexpr int U = 8;
-    wg1 = grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, std::min<uint32_t>(ls.max_wg, 8192));
-    hipLaunchKernelGGL((k_max_top<U>), dim3(wg1), dim3(64 * ls.waves_per_wg), 0, s, cv, scratch, top);
-    hipLaunchKernelGGL(k_max_top_final, dim3(1), dim3(256), 0, s, top, wg1);
+    wg1 = std::min(grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, std::min<uint32_t>(ls.max_wg, 8192)),
+                   resident_wgs(k_max_top<U>, 64 * ls.waves_per_wg));
+    launch_k((k_max_top<U>), dim3(wg1), dim3(64 * ls.waves_per_wg), 0, s, cv, scratch, top);
+    launch_k(k_max_top_final, dim3(1), dim3(256), 0, s, top, wg1);
   }
-  const uint32_t wg2 = grid_for((cv.ntiles + 63) / 64, 4, 1024);
-  hipLaunchKernelGGL(k_max_refine, dim3(wg2), dim3(256), 0, s, cv, (const uint8_t*)scratch, (const unsigned int*)top,
+  const uint32_t wg2 = std::min(grid_for((cv.ntiles + 63) / 64, 4, 1024), resident_wgs(k_max_refine, 256));
+  launch_k(k_max_refine, dim3(wg2), dim3(256), 0, s, cv, (const uint8_t*)scratch, (const unsigned int*)top,
                      wg1, best);
   return hipGetLastError();
 }
@@ -803,7 +833,7 @@ static void launch_mat_pt(const ChunkView& cv, const MatParams& mp, double* out,
   constexpr int U = (NP + NT) <= 3 ? 2 : 1;
   // k_materialize's LDS holds 4 waves
   const uint32_t wg = grid_for((cv.ntiles + U - 1) / U, 4, ls.max_wg);
-  hipLaunchKernelGGL((k_materialize<NP, NT, U>), dim3(wg), dim3(256), 0, s, cv, mp, out);
+  launch_k((k_materialize<NP, NT, U>), dim3(wg), dim3(256), 0, s, cv, mp, out);
 }
 
 template <int NP>
@@ -827,7 +857,7 @@ hipError_t launch_materialize(const ChunkView& cv, const MatParams& mp, double* 
                               const LaunchShape& ls) {
   if (mp.mode == kMatRefPrecision) {
     const uint32_t wg = (uint32_t)std::min<uint64_t>((cv.n + 255) / 256, 4096);
-    if (wg) hipLaunchKernelGGL(k_materialize_ref, dim3(wg), dim3(256), 0, s, cv, mp, out);
+    if (wg) launch_k(k_materialize_ref, dim3(wg), dim3(256), 0, s, cv, mp, out);
     return hipGetLastError();
   }
   switch (mp.planes) {
@@ -846,14 +876,14 @@ hipError_t launch_materialize(const ChunkView& cv, const MatParams& mp, double* 
 hipError_t launch_serialize(const ChunkView& cv, uint8_t flipmask, uint8_t* body, uint64_t tail_bytes,
                             hipStream_t s) {
   const uint32_t wg = grid_for(cv.ntiles, 4, 2048);
-  hipLaunchKernelGGL(k_serialize, dim3(wg), dim3(256), 0, s, cv, flipmask, body, tail_bytes);
+  launch_k(k_serialize, dim3(wg), dim3(256), 0, s, cv, flipmask, body, tail_bytes);
   return hipGetLastError();
 }
 
 hipError_t launch_load(const ChunkView& cv, uint8_t flipmask, const uint8_t* body, uint64_t tail_bytes,
                        hipStream_t s) {
   const uint32_t wg = grid_for(cv.ntiles, 4, 2048);
-  hipLaunchKernelGGL(k_load, dim3(wg), dim3(256), 0, s, cv, flipmask, body, tail_bytes);
+  launch_k(k_load, dim3(wg), dim3(256), 0, s, cv, flipmask, body, tail_bytes);
   return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 
 #include "buff_common.h"
 #include "buff_device.h"
+#include "buff_launch.h"
 #include "buff_major.h"
 
 namespace buffgpu {
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(256) void k_major_encode(const double* __restrict__
   __shared__ uint64_t s_off[4][64 * kLdsRow];
   __shared__ uint32_t s_cnt[kMaxSparse][4];
   const uint32_t lane = __lane_id();
-  const uint32_t wslot = threadIdx.x >> 6;
+  const uint32_t wslot = wave_in_wg();
   uint64_t* so = s_off[wslot];
   const uint64_t t0 = (uint64_t)blockIdx.x * me.tpw;
   const uint64_t t1 = min(t0 + me.tpw, ntiles);
@@ -328,8 +329,8 @@ __global__ __launch_bounds__(256) void k_major_tileprefix(MajorEnc me, uint64_t 
 // Pass 2b: every tile's staged outlier bytes to prefix[c] + lane offset.
 __global__ __launch_bounds__(256) void k_major_copy(MajorEnc me, uint64_t ntiles) {
   const uint32_t lane = __lane_id();
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
   for (uint64_t c = wave; c < ntiles; c += nwaves) {
 #pragma unroll 1
     for (uint32_t j = 0; j < me.S; ++j) {
@@ -367,8 +368,8 @@ template <int MODE, int S, int U>
 __global__ __launch_bounds__(256) void k_major_filter(MajorView mv, MajorTarget tg, uint32_t sem,
                                                       uint64_t* __restrict__ out, uint64_t nwords) {
   const uint32_t lane = __lane_id();
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
   const ChunkView& dv = mv.dense;
   uint16_t* out16 = reinterpret_cast<uint16_t*>(out);
   const uint64_t nout16 = 4 * nwords;
@@ -474,9 +475,9 @@ __global__ __launch_bounds__(256) void k_major_decode(MajorView mv, double inv_s
   constexpr int NPM = (int)kMaxPlanes - S > 0 ? (int)kMaxPlanes - S : 1;   // dense planes at most (F <= 64)
   __shared__ double s_val[4][64 * kLdsRow];
   const uint32_t lane = __lane_id();
-  double* sv = s_val[threadIdx.x >> 6];
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  double* sv = s_val[wave_in_wg()];
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
   const ChunkView& dv = mv.dense;
   for (uint64_t c = wave; c < dv.ntiles; c += nwaves) {
     // every load of the tile first: masks and prefixes, dense planes and tail,
@@ -604,8 +605,8 @@ __global__ __launch_bounds__(256) void k_major_max_stage(MajorView mv, uint32_t 
                                                          const MajorMaxState* __restrict__ st,
                                                          uint32_t* __restrict__ partial) {
   const uint32_t lane = __lane_id();
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
   const ChunkView& dv = mv.dense;
   const uint32_t ncol = mv.S + dv.P;
   uint32_t acc = 0;
@@ -722,20 +723,20 @@ void major_enc_geometry(uint64_t ntiles, MajorEnc* me) {
 hipError_t launch_major_encode(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t base,
                                const ChunkView& dense, const MajorEnc& me, hipStream_t s) {
   const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
-  hipLaunchKernelGGL(k_major_encode, dim3(me.nwg), dim3(256), 0, s, x, n, ntiles, pexp, dlen, base, dense, me);
+  launch_k(k_major_encode, dim3(me.nwg), dim3(256), 0, s, x, n, ntiles, pexp, dlen, base, dense, me);
   return hipGetLastError();
 }
 
 hipError_t launch_major_scan(const MajorEnc& me, uint64_t ntiles, unsigned long long* totals, hipStream_t s) {
-  hipLaunchKernelGGL(k_major_scan, dim3(1), dim3(256), 0, s, me, ntiles, totals);
+  launch_k(k_major_scan, dim3(1), dim3(256), 0, s, me, ntiles, totals);
   return hipGetLastError();
 }
 
 hipError_t launch_major_place(const MajorEnc& me, uint64_t ntiles, hipStream_t s) {
-  hipLaunchKernelGGL(k_major_tileprefix, dim3(me.nwg), dim3(256), 0, s, me, ntiles);
+  launch_k(k_major_tileprefix, dim3(me.nwg), dim3(256), 0, s, me, ntiles);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_major_copy, dim3(tiles_grid(ntiles, 4096)), dim3(256), 0, s, me, ntiles);
+  launch_k(k_major_copy, dim3(tiles_grid(ntiles, 4096)), dim3(256), 0, s, me, ntiles);
   return hipGetLastError();
 }
 
@@ -745,7 +746,7 @@ static void launch_filter_s(const MajorView& mv, bool semantic, const MajorTarge
   constexpr int U = 2;
   const uint64_t groups = (mv.dense.ntiles + U - 1) / U;
   const uint32_t grid = tiles_grid(groups, max_wg);
-  hipLaunchKernelGGL((k_major_filter<MODE, S, U>), dim3(grid), dim3(256), 0, s, mv, tg, semantic ? 1u : 0u, out,
+  launch_k((k_major_filter<MODE, S, U>), dim3(grid), dim3(256), 0, s, mv, tg, semantic ? 1u : 0u, out,
                      nwords);
 }
 
@@ -775,15 +776,15 @@ template <int SEM>
 static hipError_t launch_decode_sem(const MajorView& mv, double inv_scale, double* out, uint32_t grid,
                                     hipStream_t s) {
   switch (mv.S) {
-    case 0: hipLaunchKernelGGL((k_major_decode<SEM, 0>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 1: hipLaunchKernelGGL((k_major_decode<SEM, 1>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 2: hipLaunchKernelGGL((k_major_decode<SEM, 2>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 3: hipLaunchKernelGGL((k_major_decode<SEM, 3>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 4: hipLaunchKernelGGL((k_major_decode<SEM, 4>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 5: hipLaunchKernelGGL((k_major_decode<SEM, 5>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 6: hipLaunchKernelGGL((k_major_decode<SEM, 6>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 7: hipLaunchKernelGGL((k_major_decode<SEM, 7>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 8: hipLaunchKernelGGL((k_major_decode<SEM, 8>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 0: launch_k((k_major_decode<SEM, 0>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 1: launch_k((k_major_decode<SEM, 1>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 2: launch_k((k_major_decode<SEM, 2>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 3: launch_k((k_major_decode<SEM, 3>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 4: launch_k((k_major_decode<SEM, 4>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 5: launch_k((k_major_decode<SEM, 5>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 6: launch_k((k_major_decode<SEM, 6>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 7: launch_k((k_major_decode<SEM, 7>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 8: launch_k((k_major_decode<SEM, 8>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -804,10 +805,10 @@ hipError_t launch_major_outlier_sums(const MajorView& mv, const uint64_t* n_out,
   if (mv.S == 0) return hipSuccess;
   OutlierCounts oc;
   for (uint32_t j = 0; j < kMaxSparse; ++j) oc.n[j] = j < mv.S ? n_out[j] : 0;
-  hipLaunchKernelGGL(k_major_outsum, dim3(kOutsumBlocks, mv.S), dim3(256), 0, s, mv, oc, scratch);
+  launch_k(k_major_outsum, dim3(kOutsumBlocks, mv.S), dim3(256), 0, s, mv, oc, scratch);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_major_outsum_final, dim3(mv.S), dim3(64), 0, s, scratch, kOutsumBlocks, sums);
+  launch_k(k_major_outsum_final, dim3(mv.S), dim3(64), 0, s, scratch, kOutsumBlocks, sums);
   return hipGetLastError();
 }
 
@@ -829,7 +830,7 @@ hipError_t launch_major_max(const MajorView& mv, MajorMaxState* st, uint8_t* scr
     const uint16_t* in = k >= 2 ? cand[(k - 1) & 1] : nullptr;
     uint16_t* outc = cand[k & 1];
 #define BUFF_MAX_STAGE(KIND, SEM)                                                                            \
-  hipLaunchKernelGGL((k_major_max_stage<KIND, SEM>), dim3(grid), dim3(256), 0, s, mv, k, in, outc, st, partial)
+  launch_k((k_major_max_stage<KIND, SEM>), dim3(grid), dim3(256), 0, s, mv, k, in, outc, st, partial)
     if (kind == kStSparse) {
       if (semantic) BUFF_MAX_STAGE(kStSparse, 1);
       else BUFF_MAX_STAGE(kStSparse, 0);
@@ -845,7 +846,7 @@ hipError_t launch_major_max(const MajorView& mv, MajorMaxState* st, uint8_t* scr
     if (e != hipSuccess) return e;
     const uint32_t major = k < mv.S ? mv.col[k].major : 0;
     const uint32_t shift = k < mv.S ? mv.col[k].shift : k < ncol ? dv.F - 8 * (k - mv.S + 1) : 0;
-    hipLaunchKernelGGL(k_major_max_final, dim3(1), dim3(256), 0, s, partial, grid, kind, k, major, shift,
+    launch_k(k_major_max_final, dim3(1), dim3(256), 0, s, partial, grid, kind, k, major, shift,
                        semantic ? 1u : 0u, st);
     e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -187,7 +187,7 @@ int major_encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, u
   const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
 
   // plain min / max (buff_simd.rs:340-349: `if` / `if`)
-  const uint32_t nwg = (uint32_t)std::min<uint64_t>(ntiles, 1024);
+  const uint32_t nwg = minmax_grid(ntiles);
   BUFF_TRY(ctx->scratch(ctx->s_partial, 3ull * nwg * sizeof(int64_t)));
   BUFF_TRY(ctx->scratch(ctx->s_minmax, sizeof(MinMaxOut)));
   MinMaxOut* dmm = (MinMaxOut*)ctx->s_minmax.ptr;

@@ -81,18 +81,23 @@ struct Ctx {
   int alloc(uint64_t bytes, void** out);
   int release(void* p);
   int scratch(Scratch& s, uint64_t bytes);
-  void prof_begin(const char* name, hipEvent_t* ev);
-  void prof_end(const char* name, hipEvent_t* ev);
+  void prof_begin(const char* name, hipEvent_t* ev, bool kernels);
+  void prof_end(const char* name, hipEvent_t* ev, bool kernels);
   int prof_collect();
   LaunchShape shape(Op op, uint64_t ntiles) const;
 };
 
+// Times one operation while profiling is on. kernels = true: the interval is
+// the execution of the kernels the operation launches through launch_k (first
+// start to last end); false: events recorded on the stream around it (used for
+// graph launches).
 struct ProfScope {
   Ctx* ctx;
   const char* name;
+  bool kernels;
   hipEvent_t ev[2];
-  ProfScope(Ctx* c, const char* n) : ctx(c), name(n) { ctx->prof_begin(name, ev); }
-  ~ProfScope() { ctx->prof_end(name, ev); }
+  ProfScope(Ctx* c, const char* n, bool k = true) : ctx(c), name(n), kernels(k) { ctx->prof_begin(name, ev, kernels); }
+  ~ProfScope() { ctx->prof_end(name, ev, kernels); }
 };
 
 // One sparse (majority / outlier) byte column of a buff-major chunk.
