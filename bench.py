@@ -80,6 +80,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=1 << 27, help="rows per column (default 134,217,728)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--cpu-rows", type=int, default=1 << 25)
     ap.add_argument("--no-config3", action="store_true", help="skip the outlier-heavy (buff vs buff-major) section")
     ap.add_argument("--config3-rows", type=int, default=1 << 26)
@@ -504,6 +505,14 @@ def main():
     for _ in range(args.warmup):
         step()
     ctx.synchronize()
+    # launch-shape autotuner: each column's chunk, at its 10% predicate (kept per size class and byte planes)
+    tuned = {}
+    if not args.no_tune:
+        for c, b in zip(cols, dx):
+            ch = ctx.encode_device(b.ptr, c.n, c.scale)
+            tuned[c.name] = ch.tune(c.preds[0.10])
+            ch.free()
+        log("bench: tuned launch shapes %s" % json.dumps(tuned))
     log("bench: timing %d steps" % args.steps)
     ctx.synchronize()
     t = time.perf_counter()
@@ -633,6 +642,7 @@ def main():
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": kernels[dominant]["frac"],
                      "traffic": pmc_traffic(dominant, n), "traffic_source": TRAFFIC_FILE},
         "kernels": kernels,
+        "launch_shapes": tuned,
         "cold_cache": cold,
         "cold_filters": filters,
     }

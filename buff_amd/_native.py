@@ -78,6 +78,11 @@ class QueryResult(ctypes.Structure):
 BUFF_Q_GT, BUFF_Q_EQ, BUFF_Q_SUM, BUFF_Q_MAX, BUFF_Q_MAT = 0, 1, 2, 3, 4
 
 
+class TuneChoice(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_uint32), ("waves_per_wg", ctypes.c_uint32), ("max_wg", ctypes.c_uint32),
+                ("tiles_per_wave", ctypes.c_uint32), ("best_us", ctypes.c_float), ("default_us", ctypes.c_float)]
+
+
 class ServerCfg(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_uint32), ("slots", ctypes.c_uint32), ("arena_bytes", ctypes.c_uint64)]
 
@@ -128,6 +133,7 @@ SIGNATURES = {
     "buff_batch_arena": [vp, P(vp), P(vp)],
     "buff_batch_tune": [vp, vp, P(Query), u32, u32p, u32p, u32p],
     "buff_batch_set_tuning": [vp, u32, u32, u32],
+    "buff_tune_chunk": [vp, vp, ctypes.c_double, P(TuneChoice)],
     "buff_server_create": [vp, P(ServerCfg), P(vp)],
     "buff_server_add_chunk": [vp, vp, u32p],
     "buff_server_submit": [vp, P(Query), vp, u64p],
@@ -243,6 +249,16 @@ class Chunk:
         v, o, a = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
         check(lib().buff_max_ex(self.ctx.handle, self.handle, flags, ctypes.byref(v), ctypes.byref(o), ctypes.byref(a)))
         return v.value, o.value, a.value
+
+    def tune(self, pred):
+        """Times candidate launch shapes of the range filter (at pred), sum, max and full materialization on
+        this chunk and keeps the fastest for chunks of its size class. Returns one dict per operation."""
+        ch = (TuneChoice * 4)()
+        check(lib().buff_tune_chunk(self.ctx.handle, self.handle, float(pred), ch))
+        names = ("range_filter", "sum", "max", "materialize")
+        return {names[i]: {"waves_per_wg": c.waves_per_wg, "max_wg": c.max_wg, "tiles_per_wave": c.tiles_per_wave,
+                           "best_us": round(c.best_us, 2), "default_us": round(c.default_us, 2)}
+                for i, c in enumerate(ch)}
 
     def sparse_col(self, j):
         """buff-major sparse column j: (majority byte, outlier rows, last outlier row)."""

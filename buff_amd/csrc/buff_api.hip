@@ -222,10 +222,10 @@ int Ctx::prof_collect() {
   return BUFF_OK;
 }
 
-LaunchShape Ctx::shape(Op op, uint64_t ntiles) const {
-  auto it = tuned.find(tune_key(op, ntiles));
+LaunchShape Ctx::shape(Op op, const ChunkView& v) const {
+  auto it = tuned.find(tune_key(op, v.ntiles, v.P));
   if (it != tuned.end()) return it->second;
-  return LaunchShape{4, 2048};
+  return LaunchShape{4, 0, 0};
 }
 
 uint32_t tune_bucket(uint64_t ntiles) {
@@ -358,7 +358,7 @@ int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_
   }
   if (n > 0 && (ch->view.P || ch->view.r)) {
     ProfScope ps(ctx, "k_encode");
-    hipError_t e = launch_encode(x, ch->view, pexp, ctx->stream, ctx->shape(Op::Encode, ch->view.ntiles));
+    hipError_t e = launch_encode(x, ch->view, pexp, ctx->stream, ctx->shape(Op::Encode, ch->view));
     if (e != hipSuccess) {
       (void)ctx->release(ch->mem);
       delete ch;
@@ -509,7 +509,7 @@ int filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags
   }
   ProfScope ps(ctx, op == BUFF_OP_GT ? "k_filter_gt" : "k_filter_eq");
   HIP_TRY(launch_filter(ch->view, op == BUFF_OP_GT ? 0 : 1, tg, words, nwords, stats, ctx->stream,
-                        ctx->shape(op == BUFF_OP_GT ? Op::FilterGt : Op::FilterEq, ch->view.ntiles)));
+                        ctx->shape(op == BUFF_OP_GT ? Op::FilterGt : Op::FilterEq, ch->view)));
   return BUFF_OK;
 }
 
@@ -565,7 +565,7 @@ int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64
   uint64_t ps[kMaxPlanes + 1] = {0};
   const ChunkView& v = ch->view;
   if (ch->n > 0 && (v.P || v.r)) {
-    const LaunchShape ls = ctx->shape(Op::Sum, v.ntiles);
+    const LaunchShape ls = ctx->shape(Op::Sum, v);
     BUFF_TRY(ctx->scratch(ctx->s_sums, sizeof(ps) + sum_scratch_bytes(ls)));
     {
       ProfScope sc(ctx, "k_sum");
@@ -601,7 +601,7 @@ int max_op(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t* m
     HIP_TRY(hipMemsetAsync(sb, 0, 8, ctx->stream));
     {
       ProfScope sc(ctx, "k_max");
-      HIP_TRY(launch_max(v, (unsigned long long*)sb, sb + 256, ctx->stream, ctx->shape(Op::Max, v.ntiles)));
+      HIP_TRY(launch_max(v, (unsigned long long*)sb, sb + 256, ctx->stream, ctx->shape(Op::Max, v)));
     }
     BUFF_TRY(d2h_small(ctx, &best, sb, 8));
   }
@@ -703,7 +703,7 @@ int materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t fl
   BUFF_TRY(mat_plan(ch, precision, flags, &mp));
   if (ch->n == 0) return BUFF_OK;
   ProfScope sc(ctx, "k_materialize");
-  HIP_TRY(launch_materialize(ch->view, mp, out, ctx->stream, ctx->shape(Op::Materialize, ch->view.ntiles)));
+  HIP_TRY(launch_materialize(ch->view, mp, out, ctx->stream, ctx->shape(Op::Materialize, ch->view)));
   return BUFF_OK;
 }
 
@@ -743,6 +743,136 @@ using namespace buffgpu;
 
 // ===========================================================================
 // C ABI
+
+namespace buffgpu {
+namespace {
+
+// Kernel execution time of one launch sequence (dispatch-level events).
+struct OpTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~OpTimer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  int init() {
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    return BUFF_OK;
+  }
+  template <typename F>
+  int time(Ctx* ctx, F&& launch, float* us) {
+    LaunchEvents& le = launch_events();
+    le.start = a;
+    le.stop = b;
+    le.started = false;
+    hipError_t e = launch();
+    const bool started = le.started;
+    le = LaunchEvents{};
+    if (e != hipSuccess) return fail(BUFF_E_DEVICE, "tuning launch: %s", hipGetErrorString(e));
+    if (!started) return fail(BUFF_E_DEVICE, "tuning launch issued no kernel");
+    HIP_TRY(hipEventSynchronize(b));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, a, b));
+    *us = ms * 1e3f;
+    (void)ctx;
+    return BUFF_OK;
+  }
+};
+
+// Times every candidate shape of the range filter, sum, max and full
+// materialization on this chunk (best of three runs each, the memory-side cache
+// evicted before every run) and keeps the fastest per operation for chunks of
+// the same size class and byte planes.
+int tune_chunk(Ctx* ctx, const Chunk* ch, double pred, buff_tune_choice* out) {
+  if (ch->S > 0 || (ch->flags & BUFF_MAJORITY))
+    return fail(BUFF_E_UNSUPPORTED, "launch-shape tuning covers byte_fixed chunks");
+  const ChunkView& v = ch->view;
+  if (ch->n == 0 || (!v.P && !v.r)) return fail(BUFF_E_INVALID, "nothing to tune on an empty chunk");
+  FilterTarget tg{};
+  int fill = 0;
+  if (!filter_plan(ch, BUFF_OP_GT, pred, 0, &tg, &fill))
+    return fail(BUFF_E_INVALID, "the predicate needs no scan on this chunk (all or no rows qualify)");
+  MatParams mp{};
+  BUFF_TRY(mat_plan(ch, -1, 0, &mp));
+  const uint64_t nwords = (ch->n + 63) / 64;
+  const uint64_t evict = 1ull << 30;
+  const LaunchShape probe{4, 0, 0};
+  BUFF_TRY(ctx->scratch(ctx->s_bitmap, nwords * 8));
+  BUFF_TRY(ctx->scratch(ctx->s_sums, 9 * 8 + sum_scratch_bytes(probe)));
+  BUFF_TRY(ctx->scratch(ctx->s_best, 256 + max_scratch_bytes(v.ntiles)));
+  BUFF_TRY(ctx->scratch(ctx->s_values, ch->n * 8));
+  BUFF_TRY(ctx->scratch(ctx->s_sink, 256));
+  void* flush = nullptr;
+  BUFF_TRY(ctx->alloc(evict, &flush));
+  OpTimer tm;
+  int rc = tm.init();
+  const Op ops[4] = {Op::FilterGt, Op::Sum, Op::Max, Op::Materialize};
+  const uint32_t tiles_opts[4][3] = {{2, 4, 8}, {1, 2, 4}, {4, 8, 16}, {0, 0, 0}};
+  for (int oi = 0; oi < 4 && rc == BUFF_OK; ++oi) {
+    const Op op = ops[oi];
+    auto run = [&](const LaunchShape& ls) -> hipError_t {
+      switch (op) {
+        case Op::FilterGt:
+          return launch_filter(v, 0, tg, (uint64_t*)ctx->s_bitmap.ptr, nwords, nullptr, ctx->stream, ls);
+        case Op::Sum:
+          return launch_sum(v, (unsigned long long*)ctx->s_sums.ptr, ctx->stream, ls);
+        case Op::Max: {
+          uint8_t* sb = (uint8_t*)ctx->s_best.ptr;
+          return launch_max(v, (unsigned long long*)sb, sb + 256, ctx->stream, ls);
+        }
+        default:
+          return launch_materialize(v, mp, (double*)ctx->s_values.ptr, ctx->stream, ls);
+      }
+    };
+    auto measure = [&](const LaunchShape& ls, float* best_us) -> int {
+      *best_us = 1e30f;
+      for (int rep = 0; rep < 4; ++rep) {   // first run warms code and descriptors
+        HIP_TRY(launch_touch_bytes(flush, evict, (uint32_t*)ctx->s_sink.ptr, ctx->stream));
+        float us = 0;
+        BUFF_TRY(tm.time(ctx, [&] { return run(ls); }, &us));
+        if (rep > 0) *best_us = std::min(*best_us, us);
+      }
+      return BUFF_OK;
+    };
+    buff_tune_choice ch_out{};
+    ch_out.op = (uint32_t)op;
+    LaunchShape best{4, 0, 0};
+    float best_us = 0;
+    rc = measure(best, &best_us);
+    ch_out.default_us = best_us;
+    for (uint32_t waves : {2u, 4u}) {
+      for (uint32_t cap : {0u, 1024u, 2048u, 4096u, 8192u}) {
+        for (uint32_t t : tiles_opts[oi]) {
+          if (rc != BUFF_OK) break;
+          if (op == Op::Materialize && waves != 4) continue;   // its LDS staging assumes four waves
+          const LaunchShape ls{waves, cap, t};
+          float us = 0;
+          rc = measure(ls, &us);
+          if (rc == BUFF_OK && us < best_us) {
+            best_us = us;
+            best = ls;
+          }
+          if (op == Op::Materialize) break;
+        }
+      }
+    }
+    if (rc != BUFF_OK) break;
+    ctx->tuned[tune_key(op, v.ntiles, v.P)] = best;
+    if (op == Op::FilterGt) ctx->tuned[tune_key(Op::FilterEq, v.ntiles, v.P)] = best;
+    ch_out.waves_per_wg = best.waves_per_wg;
+    ch_out.max_wg = best.max_wg;
+    ch_out.tiles_per_wave = best.tiles_per_wave;
+    ch_out.best_us = best_us;
+    if (out) out[oi] = ch_out;
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)ctx->release(flush);
+  return rc;
+}
+
+}  // namespace
+
+}  // namespace buffgpu
 
 extern "C" {
 
@@ -838,6 +968,14 @@ int buff_dev_touch(buff_ctx* ctx, const void* dptr, uint64_t bytes) {
   ProfScope ps(c, "k_touch_bytes");
   HIP_TRY(launch_touch_bytes(dptr, bytes, (uint32_t*)c->s_sink.ptr, c->stream));
   return BUFF_OK;
+}
+
+int buff_tune_chunk(buff_ctx* ctx, const buff_chunk* chunk, double pred, buff_tune_choice* choices) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  const Chunk* ch = reinterpret_cast<const Chunk*>(chunk);
+  if (!c || !ch) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->ctx != c) return fail(BUFF_E_INVALID, "the chunk belongs to another context");
+  return tune_chunk(c, ch, pred, choices);
 }
 
 int buff_prof_enable(buff_ctx* ctx, int on) {

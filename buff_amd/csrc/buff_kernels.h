@@ -8,8 +8,11 @@ namespace buffgpu {
 
 // Launch geometry of the tile kernels (chosen per op by the autotuner).
 struct LaunchShape {
-  uint32_t waves_per_wg;  // 1, 2, 4 (a workgroup is 64*waves_per_wg threads)
-  uint32_t max_wg;        // grid cap; waves grid-stride over tiles beyond it
+  uint32_t waves_per_wg;    // 1, 2, 4 (a workgroup is 64*waves_per_wg threads)
+  uint32_t max_wg;          // grid cap (0: the workgroups that fit resident for filter, sum and max, else 2048);
+                            // waves grid-stride over tiles beyond it
+  uint32_t tiles_per_wave;  // tiles a wave loads per step (0: the kernel's default; filter 2/4/8, sum 1/2/4,
+                            // max 4/8/16)
 };
 
 enum : uint32_t { kMatFull = 0, kMatSemantic = 1, kMatRefPrecision = 2 };
@@ -38,7 +41,9 @@ hipError_t launch_filter(const ChunkView& cv, int op, const FilterTarget& tg, ui
 hipError_t launch_fill_bitmap(uint64_t* out, uint64_t nwords, uint64_t n, int ones, hipStream_t s);
 hipError_t launch_fill_bytes(void* out, uint64_t bytes, uint8_t value, hipStream_t s);
 hipError_t launch_touch_bytes(const void* in, uint64_t bytes, uint32_t* sink, hipStream_t s);
-// sums: 9 u64 results followed by sum_scratch_bytes(ls) of per-workgroup rows.
+// sums: 9 u64 results followed by sum_scratch_bytes(ls) of per-workgroup rows
+// (at most kMaxSumRows workgroups).
+constexpr uint32_t kMaxSumRows = 8192;
 uint64_t sum_scratch_bytes(const LaunchShape& ls);
 hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls);
 // Two passes; `scratch` holds max_scratch_bytes(ntiles) bytes; the caller

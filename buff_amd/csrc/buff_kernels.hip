@@ -689,6 +689,9 @@ __global__ __launch_bounds__(256) void k_load(ChunkView cv, uint8_t flipmask, co
 // ----------------------------------------------------------------------------
 // Host-side launchers (declared in buff_kernels.h).
 
+// Grid cap of a launch shape: its max_wg, else `dflt`.
+static inline uint32_t cap_of(const LaunchShape& ls, uint32_t dflt) { return ls.max_wg ? ls.max_wg : dflt; }
+
 static inline uint32_t grid_for(uint64_t units, uint32_t waves_per_wg, uint32_t max_wg) {
   uint64_t wgs = (units + waves_per_wg - 1) / waves_per_wg;
   if (wgs < 1) wgs = 1;
@@ -719,23 +722,35 @@ hipError_t launch_minmax(const double* x, uint64_t n, int32_t pexp, uint32_t dle
 
 hipError_t launch_encode(const double* x, const ChunkView& cv, int32_t pexp, hipStream_t s, const LaunchShape& ls) {
   // k_encode's LDS holds 4 waves
-  const uint32_t wg = grid_for(cv.ntiles, 4, ls.max_wg);
+  const uint32_t wg = grid_for(cv.ntiles, 4, cap_of(ls, 2048));
   launch_k(k_encode, dim3(wg), dim3(256), 0, s, x, cv.n, cv.ntiles, pexp, cv.dlen, cv.base, cv.F, cv.P,
                      cv.r, cv.planes, cv.pitch, cv.tail);
   return hipGetLastError();
 }
 
+template <int MODE, int U>
+static void launch_filter_u(const ChunkView& cv, const FilterTarget& tg, uint64_t* out, uint64_t nwords,
+                            unsigned long long* stats, hipStream_t s, const LaunchShape& ls) {
+  const uint32_t block = 64 * ls.waves_per_wg;
+  const uint32_t wg =
+      grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, cap_of(ls, resident_wgs(k_filter<MODE, U>, block)));
+  launch_k((k_filter<MODE, U>), dim3(wg), dim3(block), 0, s, cv, tg, out, nwords, stats);
+}
+
+template <int MODE>
+static void launch_filter_m(const ChunkView& cv, const FilterTarget& tg, uint64_t* out, uint64_t nwords,
+                            unsigned long long* stats, hipStream_t s, const LaunchShape& ls) {
+  switch (ls.tiles_per_wave) {
+    case 2: launch_filter_u<MODE, 2>(cv, tg, out, nwords, stats, s, ls); break;
+    case 8: launch_filter_u<MODE, 8>(cv, tg, out, nwords, stats, s, ls); break;
+    default: launch_filter_u<MODE, 4>(cv, tg, out, nwords, stats, s, ls); break;
+  }
+}
+
 hipError_t launch_filter(const ChunkView& cv, int op, const FilterTarget& tg, uint64_t* out, uint64_t nwords,
                          unsigned long long* stats, hipStream_t s, const LaunchShape& ls) {
-  constexpr int U = 4;
-  const uint32_t block = 64 * ls.waves_per_wg;
-  const uint32_t wg = grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, ls.max_wg);
-  if (op == 0)
-    launch_k((k_filter<0, U>), dim3(std::min(wg, resident_wgs(k_filter<0, U>, block))), dim3(block), 0, s, cv, tg,
-             out, nwords, stats);
-  else
-    launch_k((k_filter<1, U>), dim3(std::min(wg, resident_wgs(k_filter<1, U>, block))), dim3(block), 0, s, cv, tg,
-             out, nwords, stats);
+  if (op == 0) launch_filter_m<0>(cv, tg, out, nwords, stats, s, ls);
+  else launch_filter_m<1>(cv, tg, out, nwords, stats, s, ls);
   return hipGetLastError();
 }
 
@@ -763,18 +778,31 @@ hipError_t launch_touch_bytes(const void* in, uint64_t bytes, uint32_t* sink, hi
   return hipGetLastError();
 }
 
-template <int P, int R>
-static void launch_sum_pr(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
-  constexpr int U = (P + R) <= 3 ? 4 : 2;
+template <int P, int R, int U>
+static void launch_sum_pru(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
   const uint32_t block = 64 * ls.waves_per_wg;
-  const uint32_t wg =
-      std::min(grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, ls.max_wg), resident_wgs(k_sum<P, R, U>, block));
+  const uint32_t wg = std::min<uint32_t>(
+      grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, cap_of(ls, resident_wgs(k_sum<P, R, U>, block))),
+      kMaxSumRows);
   unsigned long long* rows = sums + (kMaxPlanes + 1);
-  launch_k((k_sum<P, R, U>), dim3(wg), dim3(64 * ls.waves_per_wg), 0, s, cv, rows);
+  launch_k((k_sum<P, R, U>), dim3(wg), dim3(block), 0, s, cv, rows);
   launch_k((k_sum_final<P + 1>), dim3(1), dim3(256), 0, s, (const unsigned long long*)rows, wg, sums);
 }
 
-uint64_t sum_scratch_bytes(const LaunchShape& ls) { return (uint64_t)(ls.max_wg + 1) * (kMaxPlanes + 1) * 8; }
+template <int P, int R>
+static void launch_sum_pr(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
+  switch (ls.tiles_per_wave) {
+    case 1: launch_sum_pru<P, R, 1>(cv, sums, s, ls); break;
+    case 2: launch_sum_pru<P, R, 2>(cv, sums, s, ls); break;
+    case 4: launch_sum_pru<P, R, 4>(cv, sums, s, ls); break;
+    default: launch_sum_pru<P, R, (P + R) <= 3 ? 4 : 2>(cv, sums, s, ls); break;
+  }
+}
+
+uint64_t sum_scratch_bytes(const LaunchShape& ls) {
+  (void)ls;
+  return (uint64_t)(kMaxSumRows + 1) * (kMaxPlanes + 1) * 8;
+}
 
 template <int P>
 static hipError_t launch_sum_p(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
@@ -810,15 +838,26 @@ uint64_t max_scratch_bytes(uint64_t ntiles) { return ((ntiles + 255) / 256) * 25
 
 // scratch: per-tile maxima (ntiles bytes, padded), then one top byte per
 // pass-1 workgroup; the caller zeroes `best`.
+template <int U>
+static uint32_t launch_max_top_u(const ChunkView& cv, uint8_t* scratch, unsigned int* top, hipStream_t s,
+                                 const LaunchShape& ls) {
+  const uint32_t block = 64 * ls.waves_per_wg;
+  const uint32_t wg = std::min<uint32_t>(
+      grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, cap_of(ls, resident_wgs(k_max_top<U>, block))), 8192);
+  launch_k((k_max_top<U>), dim3(wg), dim3(block), 0, s, cv, scratch, top);
+  return wg;
+}
+
 hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* scratch, hipStream_t s,
                       const LaunchShape& ls) {
   unsigned int* top = reinterpret_cast<unsigned int*>(scratch + ((cv.ntiles + 255) / 256) * 256);
   uint32_t wg1 = 0;
   if (cv.P > 0) {
-    constexpr int U = 8;
-    wg1 = std::min(grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, std::min<uint32_t>(ls.max_wg, 8192)),
-                   resident_wgs(k_max_top<U>, 64 * ls.waves_per_wg));
-    launch_k((k_max_top<U>), dim3(wg1), dim3(64 * ls.waves_per_wg), 0, s, cv, scratch, top);
+    switch (ls.tiles_per_wave) {
+      case 4: wg1 = launch_max_top_u<4>(cv, scratch, top, s, ls); break;
+      case 16: wg1 = launch_max_top_u<16>(cv, scratch, top, s, ls); break;
+      default: wg1 = launch_max_top_u<8>(cv, scratch, top, s, ls); break;
+    }
     launch_k(k_max_top_final, dim3(1), dim3(256), 0, s, top, wg1);
   }
   const uint32_t wg2 = std::min(grid_for((cv.ntiles + 63) / 64, 4, 1024), resident_wgs(k_max_refine, 256));
@@ -832,7 +871,7 @@ static void launch_mat_pt(const ChunkView& cv, const MatParams& mp, double* out,
                           const LaunchShape& ls) {
   constexpr int U = (NP + NT) <= 3 ? 2 : 1;
   // k_materialize's LDS holds 4 waves
-  const uint32_t wg = grid_for((cv.ntiles + U - 1) / U, 4, ls.max_wg);
+  const uint32_t wg = grid_for((cv.ntiles + U - 1) / U, 4, cap_of(ls, 2048));
   launch_k((k_materialize<NP, NT, U>), dim3(wg), dim3(256), 0, s, cv, mp, out);
 }
 

@@ -768,8 +768,9 @@ static hipError_t launch_filter_mode(const MajorView& mv, bool semantic, const M
 
 hipError_t launch_major_filter(const MajorView& mv, int op, bool semantic, const MajorTarget& tg, uint64_t* out,
                                uint64_t nwords, hipStream_t s, const LaunchShape& ls) {
-  return op == 0 ? launch_filter_mode<0>(mv, semantic, tg, out, nwords, s, ls.max_wg)
-                 : launch_filter_mode<1>(mv, semantic, tg, out, nwords, s, ls.max_wg);
+  const uint32_t cap = ls.max_wg ? ls.max_wg : 2048;
+  return op == 0 ? launch_filter_mode<0>(mv, semantic, tg, out, nwords, s, cap)
+                 : launch_filter_mode<1>(mv, semantic, tg, out, nwords, s, cap);
 }
 
 template <int SEM>
@@ -792,7 +793,7 @@ static hipError_t launch_decode_sem(const MajorView& mv, double inv_scale, doubl
 
 hipError_t launch_major_decode(const MajorView& mv, bool semantic, double inv_scale, double* out, hipStream_t s,
                                const LaunchShape& ls) {
-  const uint32_t grid = tiles_grid(mv.dense.ntiles, ls.max_wg);
+  const uint32_t grid = tiles_grid(mv.dense.ntiles, ls.max_wg ? ls.max_wg : 2048);
   return semantic ? launch_decode_sem<1>(mv, inv_scale, out, grid, s) : launch_decode_sem<0>(mv, inv_scale, out, grid, s);
 }
 
@@ -820,7 +821,7 @@ hipError_t launch_major_max(const MajorView& mv, MajorMaxState* st, uint8_t* scr
   const uint64_t ntiles = dv.ntiles;
   uint16_t* cand[2] = {reinterpret_cast<uint16_t*>(scratch), reinterpret_cast<uint16_t*>(scratch + ntiles * 128)};
   uint32_t* partial = reinterpret_cast<uint32_t*>(scratch + 2 * ntiles * 128);
-  const uint32_t grid = tiles_grid(ntiles, std::min<uint32_t>(ls.max_wg, 4096));
+  const uint32_t grid = tiles_grid(ntiles, std::min<uint32_t>(ls.max_wg ? ls.max_wg : 2048, 4096));
   hipError_t e = launch_fill_bytes(st, sizeof(MajorMaxState), 0, s);
   if (e != hipSuccess) return e;
   const uint32_t ncol = mv.S + dv.P;

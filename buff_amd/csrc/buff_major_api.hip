@@ -241,7 +241,7 @@ int major_encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, u
   if (S == 0) {
     if (ch->view.P || ch->view.r) {
       ProfScope ps(ctx, "k_encode");
-      hipError_t e = launch_encode(x, ch->view, pexp, ctx->stream, ctx->shape(Op::Encode, ch->view.ntiles));
+      hipError_t e = launch_encode(x, ch->view, pexp, ctx->stream, ctx->shape(Op::Encode, ch->view));
       if (e != hipSuccess) {
         free_partial(ctx, ch);
         return fail(BUFF_E_DEVICE, "k_encode launch: %s", hipGetErrorString(e));
@@ -560,7 +560,7 @@ int major_filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t
   const MajorView mv = major_view(ch);
   ProfScope ps(ctx, op == BUFF_OP_GT ? "k_major_filter_gt" : "k_major_filter_eq");
   HIP_TRY(launch_major_filter(mv, op == BUFF_OP_GT ? 0 : 1, semantic, tg, words, nwords, ctx->stream,
-                              ctx->shape(op == BUFF_OP_GT ? Op::FilterGt : Op::FilterEq, ch->view.ntiles)));
+                              ctx->shape(op == BUFF_OP_GT ? Op::FilterGt : Op::FilterEq, ch->view)));
   return BUFF_OK;
 }
 
@@ -570,7 +570,7 @@ int major_sum(Ctx* ctx, const Chunk* ch, uint32_t flags, double* ref_sum, double
   const uint32_t S = ch->S;
   uint64_t dense_ps[kMaxPlanes + 1] = {0};
   uint64_t outsum[kMaxSparse] = {0};
-  const LaunchShape ls = ctx->shape(Op::Sum, v.ntiles);
+  const LaunchShape ls = ctx->shape(Op::Sum, v);
   // [outlier sums | pad | per-block outlier rows (4 KiB) | dense sums + k_sum rows]
   const uint64_t rows_off = 256, dense_off = rows_off + 4096;
   BUFF_TRY(ctx->scratch(ctx->s_sums, dense_off + sizeof(dense_ps) + sum_scratch_bytes(ls)));
@@ -639,7 +639,7 @@ int major_max(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t
   MajorMaxState* st = (MajorMaxState*)sb;
   {
     ProfScope sc(ctx, "k_major_max");
-    HIP_TRY(launch_major_max(major_view(ch), st, sb + 256, ctx->stream, ctx->shape(Op::Max, v.ntiles), semantic));
+    HIP_TRY(launch_major_max(major_view(ch), st, sb + 256, ctx->stream, ctx->shape(Op::Max, v), semantic));
   }
   MajorMaxState hs;
   BUFF_TRY(d2h_small(ctx, &hs, st, sizeof(hs)));
@@ -672,7 +672,7 @@ int major_materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint3
   if (!out) return fail(BUFF_E_INVALID, "null output");
   ProfScope sc(ctx, "k_major_decode");
   HIP_TRY(launch_major_decode(major_view(ch), semantic, std::ldexp(1.0, -(int)ch->dlen), out, ctx->stream,
-                              ctx->shape(Op::Materialize, ch->view.ntiles)));
+                              ctx->shape(Op::Materialize, ch->view)));
   return BUFF_OK;
 }
 

@@ -41,7 +41,10 @@ inline uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 enum class Op : uint32_t { Encode = 0, FilterGt, FilterEq, Sum, Max, Materialize, kCount };
 
 uint32_t tune_bucket(uint64_t ntiles);
-inline uint64_t tune_key(Op op, uint64_t ntiles) { return ((uint64_t)op << 32) | tune_bucket(ntiles); }
+// Launch shapes are kept per (operation, size class, byte planes).
+inline uint64_t tune_key(Op op, uint64_t ntiles, uint32_t planes) {
+  return ((uint64_t)op << 40) | ((uint64_t)planes << 32) | tune_bucket(ntiles);
+}
 
 struct Scratch {
   void* ptr = nullptr;
@@ -84,7 +87,7 @@ struct Ctx {
   void prof_begin(const char* name, hipEvent_t* ev, bool kernels);
   void prof_end(const char* name, hipEvent_t* ev, bool kernels);
   int prof_collect();
-  LaunchShape shape(Op op, uint64_t ntiles) const;
+  LaunchShape shape(Op op, const ChunkView& v) const;
 };
 
 // Times one operation while profiling is on. kernels = true: the interval is

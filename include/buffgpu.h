@@ -164,6 +164,23 @@ int buff_materialize(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, 
 int buff_materialize_device(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags,
                             double* out_dev);
 
+/* ---- launch-shape tuning -----------------------------------------------------
+ * Times candidate launch shapes (waves per workgroup, grid cap, tiles a wave
+ * loads per step) of the range filter (at `pred`), sum, max and full
+ * materialization on this byte_fixed chunk, each run after evicting the
+ * memory-side cache, and keeps the fastest per operation for later calls on
+ * chunks of the same size class and byte-plane count. choices: 4 entries in
+ * that order (may be NULL). No reference counterpart. */
+typedef struct {
+  uint32_t op;             /* internal operation id */
+  uint32_t waves_per_wg;
+  uint32_t max_wg;         /* 0: the workgroups that fit resident */
+  uint32_t tiles_per_wave; /* 0: the kernel's default */
+  float best_us;           /* kernel time of the chosen shape */
+  float default_us;        /* kernel time of the default shape */
+} buff_tune_choice;
+int buff_tune_chunk(buff_ctx* ctx, const buff_chunk* chunk, double pred, buff_tune_choice* choices);
+
 /* ---- batched execution ------------------------------------------------------
  * Many chunks of uneven length and mixed precision encoded in one persistent
  * launch (plus one min/max launch), and queries over them run as one captured
