@@ -442,6 +442,27 @@ def bench_config5(ctx, args, buff_amd):
         sweep.append({"load": frac, "continuous_batching": serve(4096, 2, qps, 60 + int(frac * 100)),
                       "one_launch_per_request": serve(1, 1, qps, 60 + int(frac * 100))})
     res["poisson_sweep"] = sweep
+    # throughput at a stated p99: the highest Poisson rate (bisection, 7 probes) whose p99 latency stays within the
+    # bound, for each mode; a probe also has to keep up with its offered rate
+    slo = []
+    for bound_us in (1000.0, 5000.0):
+        row = {"p99_bound_us": bound_us}
+        for mode, mb, sl, hi in (("continuous_batching", 4096, 2, cap * 1.05),
+                                 ("one_launch_per_request", 1, 1,
+                                  res["saturated"]["one_launch_per_request"]["achieved_qps"] * 1.05)):
+            lo_rate, hi_rate, best = 0.0, hi, None
+            for it in range(7):
+                rate = (lo_rate + hi_rate) / 2
+                r = serve(mb, sl, rate, 200 + it)
+                if r["p99_us"] <= bound_us and r["achieved_qps"] >= 0.97 * rate:
+                    lo_rate, best = rate, r
+                else:
+                    hi_rate = rate
+            row[mode] = best
+        b, o = row["continuous_batching"], row["one_launch_per_request"]
+        row["ratio"] = round(b["achieved_qps"] / o["achieved_qps"], 2) if b and o else None
+        slo.append(row)
+    res["throughput_at_p99"] = slo
     for ch in chunks:
         ch.free()
     xd.free()

@@ -80,7 +80,8 @@ BUFF_Q_GT, BUFF_Q_EQ, BUFF_Q_SUM, BUFF_Q_MAX, BUFF_Q_MAT = 0, 1, 2, 3, 4
 
 class TuneChoice(ctypes.Structure):
     _fields_ = [("op", ctypes.c_uint32), ("waves_per_wg", ctypes.c_uint32), ("max_wg", ctypes.c_uint32),
-                ("tiles_per_wave", ctypes.c_uint32), ("best_us", ctypes.c_float), ("default_us", ctypes.c_float)]
+                ("tiles_per_wave", ctypes.c_uint32), ("prefetch", ctypes.c_uint32), ("best_us", ctypes.c_float),
+                ("default_us", ctypes.c_float)]
 
 
 class ServerCfg(ctypes.Structure):
@@ -257,7 +258,7 @@ class Chunk:
         check(lib().buff_tune_chunk(self.ctx.handle, self.handle, float(pred), ch))
         names = ("range_filter", "sum", "max", "materialize")
         return {names[i]: {"waves_per_wg": c.waves_per_wg, "max_wg": c.max_wg, "tiles_per_wave": c.tiles_per_wave,
-                           "best_us": round(c.best_us, 2), "default_us": round(c.default_us, 2)}
+                           "prefetch": c.prefetch, "best_us": round(c.best_us, 2), "default_us": round(c.default_us, 2)}
                 for i, c in enumerate(ch)}
 
     def sparse_col(self, j):

@@ -841,16 +841,18 @@ int tune_chunk(Ctx* ctx, const Chunk* ch, double pred, buff_tune_choice* out) {
     rc = measure(best, &best_us);
     ch_out.default_us = best_us;
     for (uint32_t waves : {2u, 4u}) {
-      for (uint32_t cap : {0u, 1024u, 2048u, 4096u, 8192u}) {
+      for (uint32_t cap : {0u, 1024u, 2048u, 4096u, 8192u, 16384u}) {
         for (uint32_t t : tiles_opts[oi]) {
-          if (rc != BUFF_OK) break;
-          if (op == Op::Materialize && waves != 4) continue;   // its LDS staging assumes four waves
-          const LaunchShape ls{waves, cap, t};
-          float us = 0;
-          rc = measure(ls, &us);
-          if (rc == BUFF_OK && us < best_us) {
-            best_us = us;
-            best = ls;
+          for (uint32_t pf = 0; pf < (op == Op::FilterGt ? 2u : 1u); ++pf) {
+            if (rc != BUFF_OK) break;
+            if (op == Op::Materialize && waves != 4) continue;   // its LDS staging assumes four waves
+            const LaunchShape ls{waves, cap, t, pf};
+            float us = 0;
+            rc = measure(ls, &us);
+            if (rc == BUFF_OK && us < best_us) {
+              best_us = us;
+              best = ls;
+            }
           }
           if (op == Op::Materialize) break;
         }
@@ -862,6 +864,7 @@ int tune_chunk(Ctx* ctx, const Chunk* ch, double pred, buff_tune_choice* out) {
     ch_out.waves_per_wg = best.waves_per_wg;
     ch_out.max_wg = best.max_wg;
     ch_out.tiles_per_wave = best.tiles_per_wave;
+    ch_out.prefetch = best.prefetch;
     ch_out.best_us = best_us;
     if (out) out[oi] = ch_out;
   }

@@ -13,6 +13,7 @@ struct LaunchShape {
                             // waves grid-stride over tiles beyond it
   uint32_t tiles_per_wave;  // tiles a wave loads per step (0: the kernel's default; filter 2/4/8, sum 1/2/4,
                             // max 4/8/16)
+  uint32_t prefetch;        // filter: load the next step's top plane before refining this one
 };
 
 enum : uint32_t { kMatFull = 0, kMatSemantic = 1, kMatRefPrecision = 2 };

@@ -201,7 +201,10 @@ __global__ __launch_bounds__(256) void k_encode(const double* __restrict__ x, ui
 // bit-plane by bit-plane on the 16-row masks. The result mask is the lane's
 // 16 bits of the tile bitmap.
 
-template <int MODE, int U>
+// PF: the next group's top plane is loaded before this group is compared and
+// refined, so the refinement's dependent loads overlap it (columns that tie on
+// the top byte in most tiles).
+template <int MODE, int U, bool PF>
 __global__ __launch_bounds__(256) void k_filter(ChunkView cv, FilterTarget tg, uint64_t* __restrict__ out,
                                                 uint64_t nwords, unsigned long long* __restrict__ stats) {
   const uint32_t lane = __lane_id();
@@ -212,12 +215,27 @@ __global__ __launch_bounds__(256) void k_filter(ChunkView cv, FilterTarget tg, u
   const Digit d0 = make_digit(tg.digit[0]);
   uint16_t* out16 = reinterpret_cast<uint16_t*>(out);
   const uint64_t nout16 = 4 * nwords;
+  uint4 wn[U];
+  if (PF && has_planes && wave < ngroups) {
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) wn[u] = load_plane(cv, 0, min(wave * U + u, cv.ntiles - 1), lane);
+  }
   for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
     const uint64_t c0 = grp * U;
     uint4 w[U];
     if (has_planes) {
+      if (PF) {
 #pragma unroll
-      for (uint32_t u = 0; u < U; ++u) w[u] = load_plane(cv, 0, min(c0 + u, cv.ntiles - 1), lane);
+        for (uint32_t u = 0; u < U; ++u) w[u] = wn[u];
+        const uint64_t gn = grp + nwaves;
+        if (gn < ngroups) {
+#pragma unroll
+          for (uint32_t u = 0; u < U; ++u) wn[u] = load_plane(cv, 0, min(gn * U + u, cv.ntiles - 1), lane);
+        }
+      } else {
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) w[u] = load_plane(cv, 0, min(c0 + u, cv.ntiles - 1), lane);
+      }
     }
     uint32_t gtm[U], eqm[U];
     bool any = false;
@@ -728,23 +746,30 @@ hipError_t launch_encode(const double* x, const ChunkView& cv, int32_t pexp, hip
   return hipGetLastError();
 }
 
-template <int MODE, int U>
+template <int MODE, int U, bool PF>
 static void launch_filter_u(const ChunkView& cv, const FilterTarget& tg, uint64_t* out, uint64_t nwords,
                             unsigned long long* stats, hipStream_t s, const LaunchShape& ls) {
   const uint32_t block = 64 * ls.waves_per_wg;
   const uint32_t wg =
-      grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, cap_of(ls, resident_wgs(k_filter<MODE, U>, block)));
-  launch_k((k_filter<MODE, U>), dim3(wg), dim3(block), 0, s, cv, tg, out, nwords, stats);
+      grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, cap_of(ls, resident_wgs(k_filter<MODE, U, PF>, block)));
+  launch_k((k_filter<MODE, U, PF>), dim3(wg), dim3(block), 0, s, cv, tg, out, nwords, stats);
+}
+
+template <int MODE, bool PF>
+static void launch_filter_mp(const ChunkView& cv, const FilterTarget& tg, uint64_t* out, uint64_t nwords,
+                             unsigned long long* stats, hipStream_t s, const LaunchShape& ls) {
+  switch (ls.tiles_per_wave) {
+    case 2: launch_filter_u<MODE, 2, PF>(cv, tg, out, nwords, stats, s, ls); break;
+    case 8: launch_filter_u<MODE, 8, PF>(cv, tg, out, nwords, stats, s, ls); break;
+    default: launch_filter_u<MODE, 4, PF>(cv, tg, out, nwords, stats, s, ls); break;
+  }
 }
 
 template <int MODE>
 static void launch_filter_m(const ChunkView& cv, const FilterTarget& tg, uint64_t* out, uint64_t nwords,
                             unsigned long long* stats, hipStream_t s, const LaunchShape& ls) {
-  switch (ls.tiles_per_wave) {
-    case 2: launch_filter_u<MODE, 2>(cv, tg, out, nwords, stats, s, ls); break;
-    case 8: launch_filter_u<MODE, 8>(cv, tg, out, nwords, stats, s, ls); break;
-    default: launch_filter_u<MODE, 4>(cv, tg, out, nwords, stats, s, ls); break;
-  }
+  if (ls.prefetch) launch_filter_mp<MODE, true>(cv, tg, out, nwords, stats, s, ls);
+  else launch_filter_mp<MODE, false>(cv, tg, out, nwords, stats, s, ls);
 }
 
 hipError_t launch_filter(const ChunkView& cv, int op, const FilterTarget& tg, uint64_t* out, uint64_t nwords,

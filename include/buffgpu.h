@@ -176,6 +176,7 @@ typedef struct {
   uint32_t waves_per_wg;
   uint32_t max_wg;         /* 0: the workgroups that fit resident */
   uint32_t tiles_per_wave; /* 0: the kernel's default */
+  uint32_t prefetch;       /* range filter: next step's top plane loaded before refining */
   float best_us;           /* kernel time of the chosen shape */
   float default_us;        /* kernel time of the default shape */
 } buff_tune_choice;

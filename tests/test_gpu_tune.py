@@ -36,7 +36,7 @@ def test_tuned_shapes_keep_results(gpu_ctx, oracle):
         choices = ch.tune(float(np.quantile(x, 0.9, method="lower")))
         assert set(choices) == {"range_filter", "sum", "max", "materialize"}
         for c in choices.values():
-            assert c["waves_per_wg"] in (2, 4) and c["max_wg"] in (0, 1024, 2048, 4096, 8192)
+            assert c["waves_per_wg"] in (2, 4) and c["max_wg"] in (0, 1024, 2048, 4096, 8192, 16384)
             assert 0 < c["best_us"] <= c["default_us"]
         _check(ch, oracle, x, 10 ** 4)
     finally:
