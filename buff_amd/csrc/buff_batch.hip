@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void k_bencode(const BEncChunk* __restrict__ c
         uint32_t by[16];
 #pragma unroll
         for (uint32_t b = 0; b < 16; ++b) by[b] = (uint32_t)((off[b] >> sh) & 0xFF);
-        reinterpret_cast<uint4*>(cv.planes + p * cv.pitch + c * kTileRows)[lane] = pack_lane(by);
+        store_plane_nt(cv.planes + p * cv.pitch + c * kTileRows, lane, pack_lane(by));
       }
       for (uint32_t j = 0; j < cv.r; ++j) {
         uint32_t h = 0;
@@ -175,8 +175,9 @@ constexpr int kPlanesPerRound = 4;   // plane loads in flight per wave and round
 
 // Planes P0 .. NP-1 and the tail bit-planes of tile c, for the rows still in
 // `eq` (gt gains the rows above the target), in rounds whose loads are all in
-// flight together; the tail bit-planes travel with the last round. Lanes without a live row load lane 0's bytes instead, which costs
-// no extra HBM sectors.
+// flight together; the tail bit-planes travel with the last round. Lanes
+// without a live row load the first live lane's bytes instead, which costs no
+// extra HBM lines.
 template <int MODE, int P0, int NP>
 __device__ __forceinline__ void filter_rest(const ChunkView& cv, const FilterTarget& tg, uint64_t c, uint32_t lane,
                                             uint32_t& gt, uint32_t& eq) {
@@ -185,7 +186,7 @@ __device__ __forceinline__ void filter_rest(const ChunkView& cv, const FilterTar
   constexpr int P1 = P0 == 1 ? (NP > 2 ? 2 : NP) : (NP - P0 > kPlanesPerRound ? P0 + kPlanesPerRound : NP);
   constexpr bool last = P1 == NP;
   constexpr int K = P1 > P0 ? P1 - P0 : 1;
-  const uint32_t src = eq ? lane : 0u;
+  const uint32_t src = refine_src(eq, lane);
   const uint32_t r = last ? cv.r : 0u;
   uint4 w[K];
 #pragma unroll
@@ -619,11 +620,11 @@ __device__ __forceinline__ void mat_tile(const ChunkView& cv, const MatParams& m
   double* ot = outv + c * kTileRows + lane;
   if (tile_full(cv.n, c)) {
 #pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) ot[64 * g] = sv[lds_slot(64 * g + lane)];
+    for (uint32_t g = 0; g < 16; ++g) store_nt(ot + 64 * g, sv[lds_slot(64 * g + lane)]);
   } else {
 #pragma unroll
     for (uint32_t g = 0; g < 16; ++g)
-      if (c * kTileRows + 64 * g + lane < cv.n) ot[64 * g] = sv[lds_slot(64 * g + lane)];
+      if (c * kTileRows + 64 * g + lane < cv.n) store_nt(ot + 64 * g, sv[lds_slot(64 * g + lane)]);
   }
   wave_lds_sync();
 }

@@ -124,9 +124,27 @@ __device__ __forceinline__ T cload(const T* base, uint64_t i = 0) {
 
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
+// Plane bytes are streamed once per operation: loads are non-temporal, so they
+// do not displace other lines from the caches.
 __device__ __forceinline__ uint4 load_plane(const ChunkView& cv, uint32_t p, uint64_t c, uint32_t lane) {
-  const u32x4_t v = gmem(reinterpret_cast<const u32x4_t*>(cv.planes + p * cv.pitch + c * kTileRows))[lane];
+  const u32x4_t v =
+      __builtin_nontemporal_load(gmem(reinterpret_cast<const u32x4_t*>(cv.planes + p * cv.pitch + c * kTileRows)) + lane);
   return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+// Streaming stores (written once, read by a later operation).
+__device__ __forceinline__ void store_nt(double* p, double v) { __builtin_nontemporal_store(v, gmem_w(p)); }
+__device__ __forceinline__ void store_plane_nt(uint8_t* tile, uint32_t lane, const uint4& w) {
+  const u32x4_t v = {w.x, w.y, w.z, w.w};
+  __builtin_nontemporal_store(v, gmem_w(reinterpret_cast<u32x4_t*>(tile)) + lane);
+}
+
+// The lane whose bytes a lane loads in a refinement round: its own when it
+// still has a candidate row, else the first lane that has one (a line the
+// round reads anyway, so idle lanes add no HBM traffic).
+__device__ __forceinline__ uint32_t refine_src(uint32_t live, uint32_t lane) {
+  const uint64_t b = __ballot(live != 0);
+  return live ? lane : (b ? (uint32_t)__builtin_ctzll(b) : 0u);
 }
 
 // bit-plane j of tile c, lane's 16 rows (bit b <-> row 16*lane + b)
@@ -203,15 +221,15 @@ __device__ __forceinline__ uint32_t mask_eq(const uint4& w, const Digit& dg) {
 
 __device__ __forceinline__ void load_tile_f64(const double* __restrict__ x, uint64_t n, uint64_t c, uint32_t lane,
                                               bool full, uint64_t (&v)[16]) {
-  const double* xt = x + c * kTileRows + lane;
+  const BUFF_GLOBAL double* xt = gmem(x) + c * kTileRows + lane;
   if (full) {
 #pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) v[g] = __double_as_longlong(xt[64 * g]);
+    for (uint32_t g = 0; g < 16; ++g) v[g] = __double_as_longlong(__builtin_nontemporal_load(xt + 64 * g));
   } else {
 #pragma unroll
     for (uint32_t g = 0; g < 16; ++g) {
       const uint64_t row = c * kTileRows + 64 * g + lane;
-      v[g] = row < n ? __double_as_longlong(xt[64 * g]) : 0;
+      v[g] = row < n ? __double_as_longlong(__builtin_nontemporal_load(xt + 64 * g)) : 0;
     }
   }
 }
@@ -227,7 +245,7 @@ template <int MODE>
 __device__ __forceinline__ void refine_tail(const ChunkView& cv, uint64_t c, uint64_t dt, uint32_t lane, uint32_t& gtm,
                                             uint32_t& eqm) {
   uint32_t h[8];
-  const uint32_t src = eqm ? lane : 0u;
+  const uint32_t src = refine_src(eqm, lane);
 #pragma unroll
   for (uint32_t j = 0; j < 8; ++j) h[j] = j < cv.r ? load_tail(cv, j, c, src) : 0u;   // only the r bit-planes
 #pragma unroll

@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_encode(const double* __restrict__ x, ui
       uint32_t by[16];
 #pragma unroll
       for (uint32_t b = 0; b < 16; ++b) by[b] = (uint32_t)((off[b] >> sh) & 0xFF);
-      reinterpret_cast<uint4*>(planes + p * pitch + c * kTileRows)[lane] = pack_lane(by);
+      store_plane_nt(planes + p * pitch + c * kTileRows, lane, pack_lane(by));
     }
     for (uint32_t j = 0; j < r; ++j) {
       uint32_t h = 0;
@@ -196,8 +196,8 @@ __global__ __launch_bounds__(256) void k_encode(const double* __restrict__ x, ui
 // four rows per dword op (SWAR), give the lane's 16-row masks of rows above
 // (`gt`) and tied with (`eq`) the target. Tiles where some row ties are refined
 // together, one sub-column at a time with all their loads in flight; a lane
-// without ties reads lane 0's bytes of the same tile instead of its own, so a
-// refined tile costs the 64-byte sectors that hold ties. The tail compares
+// without ties reads the bytes of the tile's first lane with ties instead of its
+// own, so a refined tile costs only the lines that hold ties. The tail compares
 // bit-plane by bit-plane on the 16-row masks. The result mask is the lane's
 // 16 bits of the tile bitmap.
 
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void k_filter(ChunkView cv, FilterTarget tg, u
         }
         uint4 wp[U];
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u) wp[u] = load_plane(cv, p, min(c0 + u, cv.ntiles - 1), eqm[u] ? lane : 0u);
+        for (uint32_t u = 0; u < U; ++u) wp[u] = load_plane(cv, p, min(c0 + u, cv.ntiles - 1), refine_src(eqm[u], lane));
         const Digit dp = make_digit(tg.digit[p]);
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(256) void k_filter(ChunkView cv, FilterTarget tg, u
       const uint64_t c = c0 + u;
       const uint64_t idx = c * 64 + lane;
       const uint32_t res = (MODE == 0 ? gtm[u] & lane_valid16(cv.n, c, lane) : eqm[u]);
-      if (c < cv.ntiles && idx < nout16) out16[idx] = (uint16_t)res;
+      if (c < cv.ntiles && idx < nout16) __builtin_nontemporal_store((uint16_t)res, gmem_w(out16) + idx);
     }
   }
 }
@@ -589,11 +589,11 @@ __global__ __launch_bounds__(256) void k_materialize(ChunkView cv, MatParams mp,
       double* ot = out + c * kTileRows + lane;
       if (tile_full(cv.n, c)) {
 #pragma unroll
-        for (uint32_t g = 0; g < 16; ++g) ot[64 * g] = sv[lds_slot(64 * g + lane)];
+        for (uint32_t g = 0; g < 16; ++g) store_nt(ot + 64 * g, sv[lds_slot(64 * g + lane)]);
       } else {
 #pragma unroll
         for (uint32_t g = 0; g < 16; ++g)
-          if (c * kTileRows + 64 * g + lane < cv.n) ot[64 * g] = sv[lds_slot(64 * g + lane)];
+          if (c * kTileRows + 64 * g + lane < cv.n) store_nt(ot + 64 * g, sv[lds_slot(64 * g + lane)]);
       }
       wave_lds_sync();
     }
@@ -684,7 +684,7 @@ __global__ __launch_bounds__(256) void k_load(ChunkView cv, uint8_t flipmask, co
       uint32_t by[16];
 #pragma unroll
       for (uint32_t b = 0; b < 16; ++b) by[b] = r0 + b < cv.n ? (uint32_t)(src[b] ^ flipmask) : 0u;
-      reinterpret_cast<uint4*>(cv.planes + p * cv.pitch + c * kTileRows)[lane] = pack_lane(by);
+      store_plane_nt(cv.planes + p * cv.pitch + c * kTileRows, lane, pack_lane(by));
     }
     if (cv.r) {
       const uint8_t* tsrc = body + cv.P * cv.n + 128ull * cv.r * c + 2ull * cv.r * lane;

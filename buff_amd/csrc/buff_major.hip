@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void k_major_encode(const double* __restrict__
       uint32_t by[16];
 #pragma unroll
       for (uint32_t b = 0; b < 16; ++b) by[b] = (uint32_t)((off[b] >> sh) & 0xFF);
-      reinterpret_cast<uint4*>(dense.planes + p * dense.pitch + c * kTileRows)[lane] = pack_lane(by);
+      store_plane_nt(dense.planes + p * dense.pitch + c * kTileRows, lane, pack_lane(by));
     }
     for (uint32_t j = 0; j < dense.r; ++j) {
       uint32_t h = 0;
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(256) void k_major_filter(MajorView mv, MajorTarget 
       uint4 wp[U];
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        wp[u] = p == 0 ? w0[u] : load_plane(dv, p, min(c0 + u, dv.ntiles - 1), cand[u] ? lane : 0u);
+        wp[u] = p == 0 ? w0[u] : load_plane(dv, p, min(c0 + u, dv.ntiles - 1), refine_src(cand[u], lane));
       const Digit dg = make_digit(tg.digit[S + p]);
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -528,11 +528,11 @@ __global__ __launch_bounds__(256) void k_major_decode(MajorView mv, double inv_s
     double* ot = out + c * kTileRows + lane;
     if (tile_full(dv.n, c)) {
 #pragma unroll
-      for (uint32_t g = 0; g < 16; ++g) ot[64 * g] = sv[lds_slot(64 * g + lane)];
+      for (uint32_t g = 0; g < 16; ++g) store_nt(ot + 64 * g, sv[lds_slot(64 * g + lane)]);
     } else {
 #pragma unroll
       for (uint32_t g = 0; g < 16; ++g)
-        if (c * kTileRows + 64 * g + lane < dv.n) ot[64 * g] = sv[lds_slot(64 * g + lane)];
+        if (c * kTileRows + 64 * g + lane < dv.n) store_nt(ot + 64 * g, sv[lds_slot(64 * g + lane)]);
     }
     wave_lds_sync();
   }
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(256) void k_major_max_stage(MajorView mv, uint32_t 
         cand &= eq;
       } else if (prev < ncol) {
         if (__ballot(cand != 0)) {
-          const uint4 w = load_plane(dv, prev - mv.S, c, cand ? lane : 0u);
+          const uint4 w = load_plane(dv, prev - mv.S, c, refine_src(cand, lane));
           cand &= mask_eq(w, make_digit(chosen));
         }
       } else if (need_tail) {   // the tail narrows the count stage
@@ -648,7 +648,7 @@ __global__ __launch_bounds__(256) void k_major_max_stage(MajorView mv, uint32_t 
       if (SEM && (cand & ~m16)) acc = max(acc, sd.major + 1u);
     } else if (KIND == kStDense) {
       if (__ballot(cand != 0)) {
-        const uint4 w = load_plane(dv, k - mv.S, c, cand ? lane : 0u);
+        const uint4 w = load_plane(dv, k - mv.S, c, refine_src(cand, lane));
         acc = max(acc, masked_max_plus1(w, cand));
       }
     } else if (KIND == kStTail) {
