@@ -103,19 +103,7 @@ __global__ __launch_bounds__(256) void k_bencode(const BEncChunk* __restrict__ c
 #pragma unroll
       for (uint32_t b = 0; b < 16; ++b) off[b] = so[kLdsRow * lane + b];
       wave_lds_sync();
-      for (uint32_t p = 0; p < cv.P; ++p) {
-        const uint32_t sh = cv.F - 8 * (p + 1);
-        uint32_t by[16];
-#pragma unroll
-        for (uint32_t b = 0; b < 16; ++b) by[b] = (uint32_t)((off[b] >> sh) & 0xFF);
-        store_plane_nt(cv.planes + p * cv.pitch + c * kTileRows, lane, pack_lane(by));
-      }
-      for (uint32_t j = 0; j < cv.r; ++j) {
-        uint32_t h = 0;
-#pragma unroll
-        for (uint32_t b = 0; b < 16; ++b) h |= (uint32_t)((off[b] >> j) & 1ull) << b;
-        cv.tail[(c * cv.r + j) * 64 + lane] = (uint16_t)h;
-      }
+      store_tile_planes(off, cv.P, cv.r, cv.planes, cv.pitch, cv.tail, c, lane);
     }
   }
 }

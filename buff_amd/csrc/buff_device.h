@@ -167,6 +167,57 @@ __device__ __forceinline__ uint4 pack_lane(const uint32_t (&by)[16]) {
   return make_uint4(d[0], d[1], d[2], d[3]);
 }
 
+// Byte k of each of 16 words, gathered into the lane's plane layout (dword i
+// byte k' = word 4k' + i): two byte permutes and a shift-or per dword.
+__device__ __forceinline__ uint4 gather_byte16(const uint32_t (&W)[16], uint32_t k) {
+  const uint32_t sel = k | ((k + 4) << 8) | 0x0C0C0000u;   // bytes 0,1 <- byte k of the two sources; 2,3 <- 0
+  uint32_t d[4];
+#pragma unroll
+  for (uint32_t i = 0; i < 4; ++i) {
+    const uint32_t lo = __builtin_amdgcn_perm(W[4 + i], W[i], sel);
+    const uint32_t hi = __builtin_amdgcn_perm(W[12 + i], W[8 + i], sel);
+    d[i] = lo | (hi << 16);
+  }
+  return make_uint4(d[0], d[1], d[2], d[3]);
+}
+
+// Byte planes and tail bit-planes of tile c from the lane's 16 offsets (row
+// 16*lane + b in off[b]; bits above the low 8P + r are ignored). Plane p holds
+// bits [r + 8(P-1-p), +8) of an offset, which is byte P-1-p of (off >> r): one
+// shift by r byte-aligns every plane, and each plane is a byte gather. Tail
+// bit j of the 16 rows is gathered from the low bytes (already in the plane
+// layout) by one mask per dword and a nibble merge.
+__device__ __forceinline__ void store_tile_planes(const uint64_t (&off)[16], uint32_t P, uint32_t r,
+                                                  uint8_t* planes, uint64_t pitch, uint16_t* tail, uint64_t c,
+                                                  uint32_t lane) {
+  uint32_t lo[16], hi[16], low8[16];
+#pragma unroll
+  for (uint32_t b = 0; b < 16; ++b) {
+    const uint64_t a = off[b] >> r;
+    lo[b] = (uint32_t)a;
+    hi[b] = (uint32_t)(a >> 32);
+    low8[b] = (uint32_t)off[b];
+  }
+  uint8_t* tile = planes + c * kTileRows;
+  uint32_t p = 0;
+#pragma unroll 1
+  for (; p + 4 < P; ++p) store_plane_nt(tile + p * pitch, lane, gather_byte16(hi, P - 5 - p));
+#pragma unroll 1
+  for (; p < P; ++p) store_plane_nt(tile + p * pitch, lane, gather_byte16(lo, P - 1 - p));
+  if (r == 0) return;
+  const uint4 L = gather_byte16(low8, 0);
+#pragma unroll 1
+  for (uint32_t j = 0; j < r; ++j) {
+    // bit 8k + i of G: bit j of row 4k + i
+    uint32_t G = (L.x >> j) & 0x01010101u;
+    G |= ((L.y >> j) & 0x01010101u) << 1;
+    G |= ((L.z >> j) & 0x01010101u) << 2;
+    G |= ((L.w >> j) & 0x01010101u) << 3;
+    const uint32_t G2 = G | (G >> 4);   // byte 0: rows 0-7, byte 2: rows 8-15
+    tail[(c * r + j) * 64 + lane] = (uint16_t)__builtin_amdgcn_perm(G2, G2, 0x0C0C0200u);
+  }
+}
+
 // valid-row mask of the lane's 16 rows
 __device__ __forceinline__ uint32_t lane_valid16(uint64_t n, uint64_t c, uint32_t lane) {
   const uint64_t r0 = c * kTileRows + 16ull * lane;

@@ -161,30 +161,23 @@ __global__ __launch_bounds__(256) void k_encode(const double* __restrict__ x, ui
     uint64_t v[16];
     const bool full = tile_full(n, c);
     load_tile_f64(x, n, c, lane, full, v);
+    if (full) {
 #pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) {
-      const int64_t f = fetch_fixed_aligned_bits(v[g], pexp, dlen);
-      const uint64_t o = (uint64_t)f - (uint64_t)base;
-      so[lds_slot(64 * g + lane)] = (full || c * kTileRows + 64 * g + lane < n) ? o : 0;
+      for (uint32_t g = 0; g < 16; ++g)
+        so[lds_slot(64 * g + lane)] = (uint64_t)fetch_fixed_aligned_bits(v[g], pexp, dlen) - (uint64_t)base;
+    } else {
+#pragma unroll
+      for (uint32_t g = 0; g < 16; ++g) {
+        const uint64_t o = (uint64_t)fetch_fixed_aligned_bits(v[g], pexp, dlen) - (uint64_t)base;
+        so[lds_slot(64 * g + lane)] = c * kTileRows + 64 * g + lane < n ? o : 0;
+      }
     }
     wave_lds_sync();
     uint64_t off[16];
 #pragma unroll
     for (uint32_t b = 0; b < 16; ++b) off[b] = so[kLdsRow * lane + b];
     wave_lds_sync();
-    for (uint32_t p = 0; p < P; ++p) {
-      const uint32_t sh = F - 8 * (p + 1);
-      uint32_t by[16];
-#pragma unroll
-      for (uint32_t b = 0; b < 16; ++b) by[b] = (uint32_t)((off[b] >> sh) & 0xFF);
-      store_plane_nt(planes + p * pitch + c * kTileRows, lane, pack_lane(by));
-    }
-    for (uint32_t j = 0; j < r; ++j) {
-      uint32_t h = 0;
-#pragma unroll
-      for (uint32_t b = 0; b < 16; ++b) h |= (uint32_t)((off[b] >> j) & 1ull) << b;
-      tail[(c * r + j) * 64 + lane] = (uint16_t)h;
-    }
+    store_tile_planes(off, P, r, planes, pitch, tail, c, lane);
   }
 }
 

@@ -188,19 +188,7 @@ __global__ __launch_bounds__(256) void k_major_encode(const double* __restrict__
     for (uint32_t b = 0; b < 16; ++b) off[b] = so[kLdsRow * lane + b];
     wave_lds_sync();
     const uint32_t valid = lane_valid16(n, c, lane);
-    for (uint32_t p = 0; p < dense.P; ++p) {
-      const uint32_t sh = dense.F - 8 * (p + 1);
-      uint32_t by[16];
-#pragma unroll
-      for (uint32_t b = 0; b < 16; ++b) by[b] = (uint32_t)((off[b] >> sh) & 0xFF);
-      store_plane_nt(dense.planes + p * dense.pitch + c * kTileRows, lane, pack_lane(by));
-    }
-    for (uint32_t j = 0; j < dense.r; ++j) {
-      uint32_t h = 0;
-#pragma unroll
-      for (uint32_t b = 0; b < 16; ++b) h |= (uint32_t)((off[b] >> j) & 1ull) << b;
-      dense.tail[(c * dense.r + j) * 64 + lane] = (uint16_t)h;
-    }
+    store_tile_planes(off, dense.P, dense.r, dense.planes, dense.pitch, dense.tail, c, lane);
 #pragma unroll 1
     for (uint32_t j = 0; j < me.S; ++j) {
       const uint32_t sh = me.shift[j], mj = me.major[j];

@@ -1,7 +1,8 @@
 """Batched execution on the GPU through the C ABI.
 
 A batch of chunks with uneven lengths and mixed precision (the batched
-workload's shape, scaled down) is encoded with the one-launch batch encoder;
+workload's shape, scaled down, plus majority-value columns whose rows tie on
+their top planes) is encoded with the one-launch batch encoder;
 every chunk must serialize to the oracle's byte_fixed stream. Then mixed query
 batches (range / equality filters, sums, maxima, materializations at full and
 reduced precision, compat and semantic) run as one graph launch each, and every
@@ -33,6 +34,9 @@ def _columns(seed=5, nchunks=48):
     cols.append((np.round(np.linspace(0, 7.9, 4097), 1), 10))
     cols.append((np.array([3.25]), 10 ** 2))
     cols.append((dg.random_walk(1025, 3, 77), 10 ** 3))
+    # majority-value columns: rows tie on the top planes in nearly every tile
+    cols.append((dg.outlier_heavy(150_000, 3, 901), 10 ** 3))
+    cols.append((dg.outlier_heavy(33_333, 3, 902), 10 ** 3))
     return cols
 
 
