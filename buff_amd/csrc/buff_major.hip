@@ -587,9 +587,15 @@ __device__ __forceinline__ uint32_t tail_value(const ChunkView& dv, uint64_t c, 
   return v;
 }
 
+// Stages after the first also keep one flag byte per tile (any candidate
+// left): a stage skips the tiles whose input flag is clear without reading
+// their candidates or column bytes, so once the chosen bytes single out a few
+// rows (an outlier holding the max), later stages touch only those tiles.
 template <int KIND, int SEM>
 __global__ __launch_bounds__(256) void k_major_max_stage(MajorView mv, uint32_t k, const uint16_t* __restrict__ cand_in,
                                                          uint16_t* __restrict__ cand_out,
+                                                         const uint8_t* __restrict__ flag_in,
+                                                         uint8_t* __restrict__ flag_out,
                                                          const MajorMaxState* __restrict__ st,
                                                          uint32_t* __restrict__ partial) {
   const uint32_t lane = __lane_id();
@@ -599,6 +605,10 @@ __global__ __launch_bounds__(256) void k_major_max_stage(MajorView mv, uint32_t 
   const uint32_t ncol = mv.S + dv.P;
   uint32_t acc = 0;
   for (uint64_t c = wave; c < dv.ntiles; c += nwaves) {
+    if (flag_in && !flag_in[c]) {
+      if (flag_out && lane == 0) flag_out[c] = 0;
+      continue;
+    }
     uint32_t cand = cand_in ? (uint32_t)cand_in[c * 64 + lane] : lane_valid16(dv.n, c, lane);
     uint32_t h[8];
     const bool need_tail = (KIND == kStTail || KIND == kStCount) && __ballot(cand != 0);
@@ -626,7 +636,11 @@ __global__ __launch_bounds__(256) void k_major_max_stage(MajorView mv, uint32_t 
         for (uint32_t b = 0; b < 16; ++b) eq |= (uint32_t)(tail_value(dv, c, lane, b, h) == chosen) << b;
         cand &= eq;
       }
-      if (KIND != kStCount) cand_out[c * 64 + lane] = (uint16_t)cand;
+      if (KIND != kStCount) {
+        cand_out[c * 64 + lane] = (uint16_t)cand;
+        const bool any = __ballot(cand != 0) != 0;
+        if (flag_out && lane == 0) flag_out[c] = any ? 1 : 0;
+      }
     }
     if (KIND == kStSparse) {
       const SparseDev& sd = mv.col[k];
@@ -801,7 +815,7 @@ hipError_t launch_major_outlier_sums(const MajorView& mv, const uint64_t* n_out,
   return hipGetLastError();
 }
 
-uint64_t major_max_scratch_bytes(uint64_t ntiles) { return 2 * ntiles * 128 + 4096 * 4 + 256; }
+uint64_t major_max_scratch_bytes(uint64_t ntiles) { return 2 * ntiles * 128 + 4096 * 4 + 2 * ntiles + 512; }
 
 hipError_t launch_major_max(const MajorView& mv, MajorMaxState* st, uint8_t* scratch, hipStream_t s,
                             const LaunchShape& ls, bool semantic) {
@@ -809,6 +823,7 @@ hipError_t launch_major_max(const MajorView& mv, MajorMaxState* st, uint8_t* scr
   const uint64_t ntiles = dv.ntiles;
   uint16_t* cand[2] = {reinterpret_cast<uint16_t*>(scratch), reinterpret_cast<uint16_t*>(scratch + ntiles * 128)};
   uint32_t* partial = reinterpret_cast<uint32_t*>(scratch + 2 * ntiles * 128);
+  uint8_t* flags[2] = {scratch + 2 * ntiles * 128 + 4096 * 4, scratch + 2 * ntiles * 128 + 4096 * 4 + ntiles};
   const uint32_t grid = tiles_grid(ntiles, std::min<uint32_t>(ls.max_wg ? ls.max_wg : 2048, 4096));
   hipError_t e = launch_fill_bytes(st, sizeof(MajorMaxState), 0, s);
   if (e != hipSuccess) return e;
@@ -818,8 +833,10 @@ hipError_t launch_major_max(const MajorView& mv, MajorMaxState* st, uint8_t* scr
     const uint32_t kind = k < mv.S ? kStSparse : k < ncol ? kStDense : k == ncol ? kStTail : kStCount;
     const uint16_t* in = k >= 2 ? cand[(k - 1) & 1] : nullptr;
     uint16_t* outc = cand[k & 1];
+    const uint8_t* fin = k >= 2 ? flags[(k - 1) & 1] : nullptr;
+    uint8_t* fout = k >= 1 && kind != kStCount ? flags[k & 1] : nullptr;
 #define BUFF_MAX_STAGE(KIND, SEM)                                                                            \
-  launch_k((k_major_max_stage<KIND, SEM>), dim3(grid), dim3(256), 0, s, mv, k, in, outc, st, partial)
+  launch_k((k_major_max_stage<KIND, SEM>), dim3(grid), dim3(256), 0, s, mv, k, in, outc, fin, fout, st, partial)
     if (kind == kStSparse) {
       if (semantic) BUFF_MAX_STAGE(kStSparse, 1);
       else BUFF_MAX_STAGE(kStSparse, 0);
