@@ -124,6 +124,10 @@ SIGNATURES = {
     "buff_sum": [vp, vp, f64p, f64p, u64p],
     "buff_max": [vp, vp, f64p, u64p],
     "buff_max_ex": [vp, vp, u32, f64p, u64p, u64p],
+    "buff_filter_cond": [vp, vp, cint, f64, u32, u32p, u32p, u64, u64p],
+    "buff_filter_cond_device": [vp, vp, cint, f64, u32, vp, vp],
+    "buff_materialize_cond": [vp, vp, u32, u32p, f64p, u64, u64p],
+    "buff_materialize_cond_device": [vp, vp, u32, vp, vp, u64, u64p],
     "buff_materialize": [vp, vp, i32, u32, f64p],
     "buff_materialize_device": [vp, vp, i32, u32, vp],
     "buff_batch_encode_device": [vp, vp, u64p, u64p, u32, u32, P(vp)],
@@ -231,6 +235,44 @@ class Chunk:
     def filter_device(self, op, pred, words_dev, flags=0, stats_dev=None):
         opc = BUFF_OP_GT if op in ("range", "gt", BUFF_OP_GT) else BUFF_OP_EQ
         check(lib().buff_filter_device(self.ctx.handle, self.handle, opc, float(pred), flags, words_dev, stats_dev))
+
+    def filter_cond(self, op, pred, cond_words, flags=0):
+        """The filter over the rows set in cond_words (u32 words, the layout filter returns).
+        Returns (u32 words, count)."""
+        n = self.n
+        cw = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
+        c = np.asarray(cond_words, dtype=np.uint32)[: (n + 31) // 32]
+        cw[: len(c)] = c
+        words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
+        cnt = ctypes.c_uint64()
+        opc = BUFF_OP_GT if op in ("range", "gt", BUFF_OP_GT) else BUFF_OP_EQ
+        check(lib().buff_filter_cond(self.ctx.handle, self.handle, opc, float(pred), flags, ptr(cw, ctypes.c_uint32),
+                                     ptr(words, ctypes.c_uint32), len(words), ctypes.byref(cnt)))
+        return words[: (n + 31) // 32], cnt.value
+
+    def filter_cond_device(self, op, pred, cond_dev, words_dev, flags=0):
+        opc = BUFF_OP_GT if op in ("range", "gt", BUFF_OP_GT) else BUFF_OP_EQ
+        check(lib().buff_filter_cond_device(self.ctx.handle, self.handle, opc, float(pred), flags, cond_dev,
+                                            words_dev))
+
+    def materialize_cond(self, cond_words, flags=0):
+        """Values of the rows set in cond_words (u32 words), in row order."""
+        n = self.n
+        cw = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
+        c = np.asarray(cond_words, dtype=np.uint32)[: (n + 31) // 32]
+        cw[: len(c)] = c
+        out = np.zeros(max(n, 1), dtype=np.float64)
+        rows = ctypes.c_uint64()
+        check(lib().buff_materialize_cond(self.ctx.handle, self.handle, flags, ptr(cw, ctypes.c_uint32),
+                                          ptr(out, ctypes.c_double), n, ctypes.byref(rows)))
+        return out[: rows.value]
+
+    def materialize_cond_device(self, cond_dev, out_dev, cap, flags=0):
+        """Returns the number of condition rows (their values are at out_dev)."""
+        rows = ctypes.c_uint64()
+        check(lib().buff_materialize_cond_device(self.ctx.handle, self.handle, flags, cond_dev, out_dev, cap,
+                                                 ctypes.byref(rows)))
+        return rows.value
 
     def sum(self):
         """Returns (reference f64 sum, exact sum, per-sub-column u64 sums[9])."""

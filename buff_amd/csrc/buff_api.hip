@@ -20,6 +20,7 @@
 
 #include "../../include/buffgpu.h"
 #include "buff_common.h"
+#include "buff_cond.h"
 #include "buff_kernels.h"
 #include "buff_runtime.h"
 #include "buff_launch.h"
@@ -531,6 +532,152 @@ int filter_host(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, 
   for (uint64_t w : host) cnt += (uint64_t)__builtin_popcountll(w);
   if (count) *count = cnt;
   if (words && need32) memcpy(words, host.data(), need32 * 4);
+  return BUFF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Filter -> project on the rows of a condition bitmap (ceil(n/64) u64 words,
+// the layout filter_device writes).
+
+// The plan of a conditional filter. buff_range_filter_condition
+// (buff_simd.rs:4683-4921) differs from buff_range_filter only where it
+// returns early: a predicate below the column base gives no rows (4706-4709),
+// and with F < 8 the target is its low byte (`fixed_target as u8`, 4718). The
+// equality form has no reference counterpart: buff_filter's equality on the
+// condition rows. fill: 0 = no rows, 1 = every condition row.
+static bool cond_plan(const Chunk* ch, int op, double pred, uint32_t flags, FilterTarget* tg, int* fill) {
+  const ChunkView& v = ch->view;
+  if (op == BUFF_OP_GT && !(flags & BUFF_SEMANTIC)) {
+    const int32_t pexp = precision_exp_of_prec(prec_of_scale(ch->scale));
+    const int64_t fixed = fetch_fixed_aligned_bits([&] { uint64_t u; memcpy(&u, &pred, 8); return u; }(), pexp,
+                                                   ch->dlen);
+    if (fixed < v.base) {
+      *fill = 0;
+      return false;
+    }
+    if (v.F > 0 && v.F < 8) {
+      const uint64_t t8 = ((uint64_t)fixed - (uint64_t)v.base) & 0xFF;
+      if (t8 >= (1ull << v.F)) {
+        *fill = 0;
+        return false;
+      }
+      memset(tg, 0, sizeof(*tg));
+      tg->tail = t8;
+      return true;
+    }
+  }
+  return filter_plan(ch, op, pred, flags, tg, fill);
+}
+
+int filter_cond_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, const uint64_t* cond,
+                       uint64_t* words) {
+  if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  if (op != BUFF_OP_GT && op != BUFF_OP_EQ) return fail(BUFF_E_INVALID, "unknown filter op %d", op);
+  const uint64_t nwords = (ch->n + 63) / 64;
+  if (nwords == 0) return BUFF_OK;
+  if (!words || !cond) return fail(BUFF_E_INVALID, "null bitmap");
+  if (ch->S > 0) return fail(BUFF_E_UNSUPPORTED, "conditional filters take byte_fixed chunks");
+  FilterTarget tg;
+  int fill = 0;
+  if (!cond_plan(ch, op, pred, flags, &tg, &fill)) {
+    ProfScope ps(ctx, "k_cond_fill");
+    HIP_TRY(launch_cond_fill(words, cond, nwords, ch->n, fill, ctx->stream));
+    return BUFF_OK;
+  }
+  ProfScope ps(ctx, op == BUFF_OP_GT ? "k_filter_cond_gt" : "k_filter_cond_eq");
+  HIP_TRY(launch_filter_cond(ch->view, op == BUFF_OP_GT ? 0 : 1, tg, cond, words, nwords, ctx->stream,
+                             ctx->shape(op == BUFF_OP_GT ? Op::FilterGt : Op::FilterEq, ch->view)));
+  return BUFF_OK;
+}
+
+// Host condition words (ceil(n/32) u32, BitVec<u32> order) -> ctx->s_condin.
+static int upload_cond(Ctx* ctx, const Chunk* ch, const uint32_t* cond_words, const uint64_t** out) {
+  const uint64_t nwords64 = (ch->n + 63) / 64, need32 = (ch->n + 31) / 32;
+  BUFF_TRY(ctx->scratch(ctx->s_condin, nwords64 * 8));
+  std::vector<uint64_t> host(nwords64, 0);
+  if (need32) memcpy(host.data(), cond_words, need32 * 4);
+  HIP_TRY(hipMemcpyAsync(ctx->s_condin.ptr, host.data(), nwords64 * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));   // the host vector goes away
+  *out = (const uint64_t*)ctx->s_condin.ptr;
+  return BUFF_OK;
+}
+
+int filter_cond_host(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, const uint32_t* cond_words,
+                     uint32_t* words, uint64_t nwords32, uint64_t* count) {
+  if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  const uint64_t need32 = (ch->n + 31) / 32, nwords64 = (ch->n + 63) / 64;
+  if (need32 && !cond_words) return fail(BUFF_E_INVALID, "null condition");
+  if (words && nwords32 < need32)
+    return fail(BUFF_E_SMALL_BUFFER, "bitmap needs %llu u32 words", (unsigned long long)need32);
+  std::vector<uint64_t> host(nwords64);
+  if (nwords64) {
+    const uint64_t* cond = nullptr;
+    BUFF_TRY(upload_cond(ctx, ch, cond_words, &cond));
+    BUFF_TRY(ctx->scratch(ctx->s_bitmap, nwords64 * 8));
+    BUFF_TRY(filter_cond_device(ctx, ch, op, pred, flags, cond, (uint64_t*)ctx->s_bitmap.ptr));
+    HIP_TRY(hipMemcpyAsync(host.data(), ctx->s_bitmap.ptr, nwords64 * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  uint64_t cnt = 0;
+  for (uint64_t w : host) cnt += (uint64_t)__builtin_popcountll(w);
+  if (count) *count = cnt;
+  if (words && need32) memcpy(words, host.data(), need32 * 4);
+  return BUFF_OK;
+}
+
+// buff_decode_condition (buff_simd.rs:1528-1712): the values of the condition
+// rows in row order. Default: the reference's tail placement (F >= 8 and
+// F % 8 != 0: the tail bits OR-ed in shifted left by the tail width,
+// 1693/1701) and its panic on an empty condition (`unwrap` of the first row).
+// BUFF_SEMANTIC: the stored values; an empty condition gives no rows.
+int materialize_cond_device(Ctx* ctx, const Chunk* ch, uint32_t flags, const uint64_t* cond, double* out,
+                            uint64_t cap, uint64_t* rows) {
+  if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->S > 0) return fail(BUFF_E_UNSUPPORTED, "conditional decode takes byte_fixed chunks");
+  const bool semantic = (flags & BUFF_SEMANTIC) != 0;
+  uint64_t total = 0;
+  if (ch->n) {
+    if (!cond || (cap && !out)) return fail(BUFF_E_INVALID, "null argument");
+    const uint64_t ntiles = ch->view.ntiles;
+    BUFF_TRY(ctx->scratch(ctx->s_cond, cond_scratch_bytes(ntiles)));
+    uint8_t* scr = (uint8_t*)ctx->s_cond.ptr;
+    {
+      ProfScope ps(ctx, "k_cond_decode");
+      HIP_TRY(launch_cond_decode(ch->view, std::ldexp(1.0, -(int)ch->dlen), !semantic, cond, out, cap, scr,
+                                 ctx->stream));
+    }
+    BUFF_TRY(d2h_small(ctx, &total, scr + cond_total_offset(ntiles), 8));
+  }
+  if (rows) *rows = total;
+  if (total == 0 && !semantic) return fail(BUFF_E_REF_PANIC, "empty condition: the reference unwraps the first row");
+  if (total > cap)
+    return fail(BUFF_E_SMALL_BUFFER, "%llu condition rows, room for %llu", (unsigned long long)total,
+                (unsigned long long)cap);
+  return BUFF_OK;
+}
+
+int materialize_cond_host(Ctx* ctx, const Chunk* ch, uint32_t flags, const uint32_t* cond_words, double* out,
+                          uint64_t cap, uint64_t* rows) {
+  if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->S > 0) return fail(BUFF_E_UNSUPPORTED, "conditional decode takes byte_fixed chunks");
+  if (ch->n && !cond_words) return fail(BUFF_E_INVALID, "null condition");
+  const uint64_t* cond = nullptr;
+  if (ch->n) {
+    BUFF_TRY(upload_cond(ctx, ch, cond_words, &cond));
+    BUFF_TRY(ctx->scratch(ctx->s_values, ch->n * 8));
+  }
+  uint64_t total = 0;
+  const int rc = materialize_cond_device(ctx, ch, flags, cond, (double*)ctx->s_values.ptr, ch->n, &total);
+  if (rows) *rows = total;
+  if (rc != BUFF_OK) return rc;
+  if (total > cap)
+    return fail(BUFF_E_SMALL_BUFFER, "%llu condition rows, room for %llu", (unsigned long long)total,
+                (unsigned long long)cap);
+  if (total) {
+    if (!out) return fail(BUFF_E_INVALID, "null output");
+    HIP_TRY(hipMemcpyAsync(out, ctx->s_values.ptr, total * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
   return BUFF_OK;
 }
 
@@ -1104,6 +1251,30 @@ int buff_filter_device(buff_ctx* ctx, const buff_chunk* chunk, int op, double pr
                        uint64_t* words_dev, uint64_t* stats_dev) {
   return filter_device(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), op, pred, flags,
                        words_dev, (unsigned long long*)stats_dev);
+}
+
+int buff_filter_cond(buff_ctx* ctx, const buff_chunk* chunk, int op, double pred, uint32_t flags,
+                     const uint32_t* cond_words, uint32_t* words, uint64_t nwords, uint64_t* count) {
+  return filter_cond_host(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), op, pred, flags,
+                          cond_words, words, nwords, count);
+}
+
+int buff_filter_cond_device(buff_ctx* ctx, const buff_chunk* chunk, int op, double pred, uint32_t flags,
+                            const uint64_t* cond_dev, uint64_t* words_dev) {
+  return filter_cond_device(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), op, pred, flags,
+                            cond_dev, words_dev);
+}
+
+int buff_materialize_cond(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, const uint32_t* cond_words,
+                          double* out, uint64_t cap, uint64_t* rows) {
+  return materialize_cond_host(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), flags,
+                               cond_words, out, cap, rows);
+}
+
+int buff_materialize_cond_device(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, const uint64_t* cond_dev,
+                                 double* out_dev, uint64_t cap, uint64_t* rows) {
+  return materialize_cond_device(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), flags,
+                                 cond_dev, out_dev, cap, rows);
 }
 
 int buff_sum(buff_ctx* ctx, const buff_chunk* chunk, double* ref_sum, double* exact_sum, uint64_t* plane_sums) {

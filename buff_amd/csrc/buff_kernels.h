@@ -40,6 +40,12 @@ hipError_t launch_encode(const double* x, const ChunkView& cv, int32_t pexp, hip
 hipError_t launch_filter(const ChunkView& cv, int op, const FilterTarget& tg, uint64_t* out, uint64_t nwords,
                          unsigned long long* stats, hipStream_t s, const LaunchShape& ls);
 hipError_t launch_fill_bitmap(uint64_t* out, uint64_t nwords, uint64_t n, int ones, hipStream_t s);
+// Filter restricted to the rows set in `cond` (ceil(n/64) u64 words, the bitmap layout).
+hipError_t launch_filter_cond(const ChunkView& cv, int op, const FilterTarget& tg, const uint64_t* cond,
+                              uint64_t* out, uint64_t nwords, hipStream_t s, const LaunchShape& ls);
+// out = cond masked to the n rows (copy = 1) or zeros (copy = 0).
+hipError_t launch_cond_fill(uint64_t* out, const uint64_t* cond, uint64_t nwords, uint64_t n, int copy,
+                            hipStream_t s);
 hipError_t launch_fill_bytes(void* out, uint64_t bytes, uint8_t value, hipStream_t s);
 hipError_t launch_touch_bytes(const void* in, uint64_t bytes, uint32_t* sink, hipStream_t s);
 // sums: 9 u64 results followed by sum_scratch_bytes(ls) of per-workgroup rows

@@ -197,9 +197,14 @@ __global__ __launch_bounds__(256) void k_encode(const double* __restrict__ x, ui
 // PF: the next group's top plane is loaded before this group is compared and
 // refined, so the refinement's dependent loads overlap it (columns that tie on
 // the top byte in most tiles).
-template <int MODE, int U, bool PF>
+// COND: only the rows set in `cond` (the bitmap layout of `out`) take part; a
+// group of tiles without any such row is written as zeros without reading its
+// planes.
+template <int MODE, int U, bool PF, bool COND = false>
 __global__ __launch_bounds__(256) void k_filter(ChunkView cv, FilterTarget tg, uint64_t* __restrict__ out,
-                                                uint64_t nwords, unsigned long long* __restrict__ stats) {
+                                                uint64_t nwords, unsigned long long* __restrict__ stats,
+                                                const uint16_t* __restrict__ cond) {
+  static_assert(!(COND && PF), "conditional filters run without the prefetch");
   const uint32_t lane = __lane_id();
   const uint64_t wave = wave_global();
   const uint64_t nwaves = waves_in_grid();
@@ -215,6 +220,25 @@ __global__ __launch_bounds__(256) void k_filter(ChunkView cv, FilterTarget tg, u
   }
   for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
     const uint64_t c0 = grp * U;
+    uint32_t cm[U];
+    if (COND) {
+      bool anyc = false;
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint64_t idx = (c0 + u) * 64 + lane;
+        const bool in = c0 + u < cv.ntiles && idx < nout16;
+        cm[u] = (uint32_t)cond[in ? idx : 0] & (in ? 0xFFFFu : 0u);   // the address stays inside the bitmap
+        anyc |= cm[u] != 0;
+      }
+      if (!__ballot(anyc)) {
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+          const uint64_t idx = (c0 + u) * 64 + lane;
+          if (c0 + u < cv.ntiles && idx < nout16) __builtin_nontemporal_store((uint16_t)0, gmem_w(out16) + idx);
+        }
+        continue;
+      }
+    }
     uint4 w[U];
     if (has_planes) {
       if (PF) {
@@ -235,12 +259,13 @@ __global__ __launch_bounds__(256) void k_filter(ChunkView cv, FilterTarget tg, u
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {
       const uint32_t valid = lane_valid16(cv.n, c0 + u, lane);
+      const uint32_t live = COND ? valid & cm[u] : valid;
       if (has_planes) {
-        gtm[u] = MODE == 0 ? mask_gt(w[u], d0) : 0u;
-        eqm[u] = mask_eq(w[u], d0) & valid;
+        gtm[u] = MODE == 0 ? mask_gt(w[u], d0) & live : 0u;
+        eqm[u] = mask_eq(w[u], d0) & live;
       } else {
         gtm[u] = 0;
-        eqm[u] = valid;
+        eqm[u] = live;
       }
       any |= eqm[u] != 0;
     }
@@ -745,7 +770,8 @@ static void launch_filter_u(const ChunkView& cv, const FilterTarget& tg, uint64_
   const uint32_t block = 64 * ls.waves_per_wg;
   const uint32_t wg =
       grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, cap_of(ls, resident_wgs(k_filter<MODE, U, PF>, block)));
-  launch_k((k_filter<MODE, U, PF>), dim3(wg), dim3(block), 0, s, cv, tg, out, nwords, stats);
+  launch_k((k_filter<MODE, U, PF>), dim3(wg), dim3(block), 0, s, cv, tg, out, nwords, stats,
+           static_cast<const uint16_t*>(nullptr));
 }
 
 template <int MODE, bool PF>
@@ -769,6 +795,42 @@ hipError_t launch_filter(const ChunkView& cv, int op, const FilterTarget& tg, ui
                          unsigned long long* stats, hipStream_t s, const LaunchShape& ls) {
   if (op == 0) launch_filter_m<0>(cv, tg, out, nwords, stats, s, ls);
   else launch_filter_m<1>(cv, tg, out, nwords, stats, s, ls);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter_cond(const ChunkView& cv, int op, const FilterTarget& tg, const uint64_t* cond,
+                              uint64_t* out, uint64_t nwords, hipStream_t s, const LaunchShape& ls) {
+  const uint32_t block = 64 * ls.waves_per_wg;
+  const uint16_t* c16 = reinterpret_cast<const uint16_t*>(cond);
+  if (op == 0) {
+    const uint32_t wg = grid_for((cv.ntiles + 3) / 4, ls.waves_per_wg,
+                                 cap_of(ls, resident_wgs(k_filter<0, 4, false, true>, block)));
+    launch_k((k_filter<0, 4, false, true>), dim3(wg), dim3(block), 0, s, cv, tg, out, nwords,
+             static_cast<unsigned long long*>(nullptr), c16);
+  } else {
+    const uint32_t wg = grid_for((cv.ntiles + 3) / 4, ls.waves_per_wg,
+                                 cap_of(ls, resident_wgs(k_filter<1, 4, false, true>, block)));
+    launch_k((k_filter<1, 4, false, true>), dim3(wg), dim3(block), 0, s, cv, tg, out, nwords,
+             static_cast<unsigned long long*>(nullptr), c16);
+  }
+  return hipGetLastError();
+}
+
+// Rows of the condition (mode 1, masked to the n rows) or none (mode 0).
+__global__ void k_cond_fill(uint64_t* __restrict__ out, const uint64_t* __restrict__ cond, uint64_t nwords,
+                            uint64_t n, int copy) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t w = copy ? cond[i] : 0ull;
+    if (copy && i == nwords - 1 && (n & 63)) w &= (1ull << (n & 63)) - 1;
+    out[i] = w;
+  }
+}
+
+hipError_t launch_cond_fill(uint64_t* out, const uint64_t* cond, uint64_t nwords, uint64_t n, int copy,
+                            hipStream_t s) {
+  uint32_t wg = (uint32_t)std::min<uint64_t>((nwords + 255) / 256, 4096);
+  if (wg == 0) return hipSuccess;
+  launch_k(k_cond_fill, dim3(wg), dim3(256), 0, s, out, cond, nwords, n, copy);
   return hipGetLastError();
 }
 

@@ -143,6 +143,36 @@ int buff_filter(buff_ctx* ctx, const buff_chunk* chunk, int op, double pred, uin
  * entering sub-column k >= 1 (index k) and the tail (index planes). */
 int buff_filter_device(buff_ctx* ctx, const buff_chunk* chunk, int op, double pred, uint32_t flags,
                        uint64_t* words_dev, uint64_t* stats_dev);
+/* ---- filter -> project: queries over the rows of a condition bitmap --------
+ * A condition is a bitmap in the layout the filters write (bit i = row i; the
+ * _device forms take ceil(n/64) u64 words, the host forms ceil(n/32) u32
+ * words), e.g. the result of an earlier filter: predicates chain, and a
+ * filter's rows can be materialized. byte_fixed chunks only (a buff-major
+ * chunk is BUFF_E_UNSUPPORTED).
+ *
+ * buff_range_filter_condition (buff_simd.rs:4683-4921): x > pred over the
+ * condition rows (the reference takes their row ids as an iterator); rows
+ * outside the condition are 0. Default reproduces its early returns: a
+ * predicate below the column base gives no rows, and with F < 8 the target is
+ * its low byte. BUFF_SEMANTIC: x > pred on every condition row. op BUFF_OP_EQ
+ * has no reference counterpart: buff_filter's equality on the condition rows.
+ * *count (host form) = rows set in the result. */
+int buff_filter_cond(buff_ctx* ctx, const buff_chunk* chunk, int op, double pred, uint32_t flags,
+                     const uint32_t* cond_words, uint32_t* words, uint64_t nwords, uint64_t* count);
+int buff_filter_cond_device(buff_ctx* ctx, const buff_chunk* chunk, int op, double pred, uint32_t flags,
+                            const uint64_t* cond_dev, uint64_t* words_dev);
+/* buff_decode_condition (buff_simd.rs:1528-1712): the values of the condition
+ * rows in row order, at most cap of them; *rows = the condition's row count
+ * (BUFF_E_SMALL_BUFFER when it exceeds cap). Default reproduces the reference:
+ * when F >= 8 and F % 8 != 0 the tail bits are OR-ed in shifted left by the
+ * tail width (1693, 1701), and an empty condition is its panic
+ * (BUFF_E_REF_PANIC). BUFF_SEMANTIC: the stored values; an empty condition
+ * gives 0 rows. The device form reads the row count back (one host sync). */
+int buff_materialize_cond(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, const uint32_t* cond_words,
+                          double* out, uint64_t cap, uint64_t* rows);
+int buff_materialize_cond_device(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, const uint64_t* cond_dev,
+                                 double* out_dev, uint64_t cap, uint64_t* rows);
+
 /* byte_fixed_sum (split_double.rs:2127-2208). ref_sum reproduces the reference
  * f64 fold bit for bit; exact_sum = (n*base + sum(off)) / 2^dlen rounded once;
  * plane_sums (nullable, 9 u64) are the exact per-sub-column byte sums (tail last). */

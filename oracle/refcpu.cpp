@@ -1141,6 +1141,264 @@ FilterOut buff_simd_equal_filter(const uint8_t* bytes, size_t blen, uint64_t sca
   return fo;
 }
 
+// buff_simd.rs:4683-4921 (buff_range_filter_condition): x > pred over the
+// rows the iterator yields (`rows`, ascending). The result keeps no other row.
+FilterOut buff_range_filter_condition(const uint8_t* bytes, size_t blen, uint64_t scale, double pred,
+                                      const uint64_t* rows, size_t nrows) {
+  BitPackReader bp(bytes, blen);
+  Header h = read_header(bp);
+  size_t len = h.len;
+  size_t remain = (size_t)(uint32_t)(h.dlen + h.ilen);   // `(dlen+ilen) as usize`: the u32 sum wraps first
+  Bound bound = bound_for(scale, h.dlen);
+  BitVec rb1(len), res(len);
+  FilterOut fo;
+  int64_t fixed_part = bound.fixed(pred);
+  if (fixed_part < h.base) {            // 4706-4709: prints len, returns the empty bitmap
+    fo.res = std::move(res);
+    return fo;
+  }
+  uint64_t fixed_target = (uint64_t)fixed_part - (uint64_t)h.base;
+  uint8_t cur_tar = 0;
+  size_t ri = 0;
+  auto next_row = [&](size_t* out) {
+    if (ri >= nrows) return false;
+    *out = (size_t)rows[ri++];
+    return true;
+  };
+  if (remain < 8) {                     // 4714-4752
+    size_t dec_cur = 0, dec_pre = 0, idx;
+    cur_tar = (uint8_t)fixed_target;    // `fixed_target as u8`
+    bool have = next_row(&idx);
+    if (have) {
+      dec_cur = idx;
+      if (dec_cur != 0) bp.skip(dec_cur * remain);
+      uint8_t dec = bp.read_bits(remain);
+      if (dec > cur_tar) res.set(dec_cur);
+      have = next_row(&idx);
+      dec_pre = dec_cur;
+    }
+    while (have) {
+      dec_cur = idx;
+      size_t delta = dec_cur - dec_pre;
+      if (delta != 1) bp.skip((delta - 1) * remain);
+      uint8_t dec = bp.read_bits(remain);
+      if (dec > cur_tar) res.set(dec_cur);
+      have = next_row(&idx);
+      dec_pre = dec_cur;
+    }
+    remain = 0;
+  } else {                              // 4753-4800
+    remain -= 8;
+    size_t dec_cur = 0, dec_pre = 0, idx;
+    cur_tar = (uint8_t)(fixed_target >> (remain & 63));
+    bool have = next_row(&idx);
+    if (have) {
+      dec_cur = idx;
+      if (dec_cur != 0) bp.skip_n_byte(dec_cur);
+      uint8_t dec = bp.read_byte();
+      if (dec > cur_tar) res.set(dec_cur);
+      else if (dec == cur_tar) rb1.set(dec_cur);
+      have = next_row(&idx);
+      dec_pre = dec_cur;
+    }
+    while (have) {
+      dec_cur = idx;
+      size_t delta = dec_cur - dec_pre;
+      if (delta != 1) bp.skip_n_byte(delta - 1);
+      uint8_t dec = bp.read_byte();
+      if (dec > cur_tar) res.set(dec_cur);
+      else if (dec == cur_tar) rb1.set(dec_cur);
+      have = next_row(&idx);
+      dec_pre = dec_cur;
+    }
+    if (rb1.cardinality() == 0) {
+      fo.count = res.cardinality();
+      fo.res = std::move(res);
+      return fo;
+    }
+    if (len - dec_pre > 1) bp.skip_n_byte(len - dec_pre - 1);
+  }
+  while (remain > 0) {                  // 4805-4918
+    if (remain >= 8) {
+      remain -= 8;
+      if (rb1.cardinality() != 0) {
+        BitVec cur_rb(len);
+        BVIter it(rb1);
+        size_t dec_cur = 0, dec_pre = 0, idx;
+        cur_tar = (uint8_t)(fixed_target >> (remain & 63));
+        bool have = it.next(&idx);
+        if (have) {
+          dec_cur = idx;
+          if (dec_cur != 0) bp.skip_n_byte(dec_cur);
+          uint8_t dec = bp.read_byte();
+          if (dec > cur_tar) res.set(dec_cur);
+          else if (dec == cur_tar) cur_rb.set(dec_cur);
+          have = it.next(&idx);
+          dec_pre = dec_cur;
+        }
+        while (have) {
+          dec_cur = idx;
+          size_t delta = dec_cur - dec_pre;
+          if (delta != 1) bp.skip_n_byte(delta - 1);
+          uint8_t dec = bp.read_byte();
+          if (dec > cur_tar) res.set(dec_cur);
+          else if (dec == cur_tar) cur_rb.set(dec_cur);
+          have = it.next(&idx);
+          dec_pre = dec_cur;
+        }
+        if (len - dec_pre > 1) bp.skip_n_byte(len - dec_pre - 1);
+        rb1 = cur_rb;
+      } else {
+        bp.skip_n_byte(len);
+        break;
+      }
+    } else {
+      uint8_t sh = (uint8_t)(BYTE_BITS - remain);
+      cur_tar = (uint8_t)((uint8_t)((uint8_t)fixed_target << sh) >> sh);
+      bp.finish_read_byte();
+      if (rb1.cardinality() != 0) {
+        BVIter it(rb1);
+        size_t dec_cur = 0, dec_pre = 0, idx;
+        bool have = it.next(&idx);
+        if (have) {
+          dec_cur = idx;
+          if (dec_cur != 0) bp.skip(dec_cur * remain);
+          uint8_t dec = bp.read_bits(remain);
+          if (dec > cur_tar) res.set(dec_cur);
+          have = it.next(&idx);
+          dec_pre = dec_cur;
+        }
+        while (have) {
+          dec_cur = idx;
+          size_t delta = dec_cur - dec_pre;
+          if (delta != 1) bp.skip((delta - 1) * remain);
+          uint8_t dec = bp.read_bits(remain);
+          if (dec > cur_tar) res.set(dec_cur);
+          have = it.next(&idx);
+          dec_pre = dec_cur;
+        }
+        remain = 0;
+      } else {
+        break;
+      }
+    }
+  }
+  fo.count = res.cardinality();
+  fo.res = std::move(res);
+  return fo;
+}
+
+// buff_simd.rs:1528-1712 (buff_decode_condition): the values of the rows the
+// iterator yields (`rows`, ascending), in that order. An empty iterator is the
+// reference's unwrap of None. When F % 8 != 0 the tail bits are OR-ed in
+// shifted left by the tail width (1693, 1701), not in place.
+std::vector<double> buff_decode_condition(const uint8_t* bytes, size_t blen, uint64_t scale, const uint64_t* rows,
+                                          size_t nrows) {
+  (void)scale;   // 1529-1533 build a PrecisionBound from it that the decode never reads
+  BitPackReader bp(bytes, blen);
+  Header h = read_header(bp);
+  size_t len = h.len;
+  std::vector<double> out;
+  std::vector<uint64_t> fixed_vec;
+  const double dec_scl = powi(2.0, (int32_t)h.dlen);
+  size_t remain = (size_t)(uint32_t)(h.dlen + h.ilen);   // `(dlen+ilen) as usize`: the u32 sum wraps first
+  if (nrows == 0) throw RefPanic("iter.next().unwrap() on an empty condition");
+  auto value = [&](uint64_t off) { return (double)(int64_t)((uint64_t)h.base + off) / dec_scl; };
+  size_t pre, delta;
+  if (remain < 8) {                     // 1559-1577
+    pre = rows[0];
+    bp.skip(pre * remain);
+    uint8_t cur = bp.read_bits(remain);
+    out.push_back(value(cur));
+    for (size_t i = 1; i < nrows; ++i) {
+      size_t entry = rows[i];
+      delta = entry - pre;
+      if (delta != 1) bp.skip((delta - 1) * remain);
+      cur = bp.read_bits(remain);
+      out.push_back(value(cur));
+      pre = entry;
+    }
+    return out;
+  }
+  remain -= 8;
+  if (remain == 0) {                    // 1583-1602
+    pre = rows[0];
+    bp.skip_n_byte(pre);
+    uint8_t cur = bp.read_byte();
+    out.push_back(value(cur));
+    for (size_t i = 1; i < nrows; ++i) {
+      size_t entry = rows[i];
+      delta = entry - pre;
+      if (delta != 1) bp.skip_n_byte(delta - 1);
+      cur = bp.read_byte();
+      out.push_back(value(cur));
+      pre = entry;
+    }
+    if (len - pre > 1) bp.skip_n_byte(len - pre - 1);
+    return out;
+  }
+  pre = rows[0];                        // 1604-1623
+  bp.skip_n_byte(pre);
+  uint8_t cur = bp.read_byte();
+  fixed_vec.push_back((uint64_t)cur << (remain & 63));
+  for (size_t i = 1; i < nrows; ++i) {
+    size_t entry = rows[i];
+    delta = entry - pre;
+    if (delta != 1) bp.skip_n_byte(delta - 1);
+    cur = bp.read_byte();
+    fixed_vec.push_back((uint64_t)cur << (remain & 63));
+    pre = entry;
+  }
+  if (len - pre > 1) bp.skip_n_byte(len - pre - 1);
+  while (remain >= 8) {                 // 1627-1678
+    remain -= 8;
+    if (remain == 0) {
+      pre = rows[0];
+      bp.skip_n_byte(pre);
+      cur = bp.read_byte();
+      out.push_back(value((uint64_t)cur | fixed_vec[0]));
+      for (size_t i = 1; i < nrows; ++i) {
+        size_t entry = rows[i];
+        delta = entry - pre;
+        if (delta != 1) bp.skip_n_byte(delta - 1);
+        cur = bp.read_byte();
+        out.push_back(value((uint64_t)cur | fixed_vec[i]));
+        pre = entry;
+      }
+    } else {
+      pre = rows[0];
+      bp.skip_n_byte(pre);
+      cur = bp.read_byte();
+      fixed_vec[0] |= (uint64_t)cur << (remain & 63);
+      for (size_t i = 1; i < nrows; ++i) {
+        size_t entry = rows[i];
+        delta = entry - pre;
+        if (delta != 1) bp.skip_n_byte(delta - 1);
+        cur = bp.read_byte();
+        fixed_vec[i] |= (uint64_t)cur << (remain & 63);
+        pre = entry;
+      }
+      if (len - pre > 1) bp.skip_n_byte(len - pre - 1);
+    }
+  }
+  if (remain > 0) {                     // 1685-1706
+    bp.finish_read_byte();
+    pre = rows[0];
+    bp.skip(pre * remain);
+    uint8_t c = bp.read_bits(remain);
+    out.push_back(value(fixed_vec[0] | ((uint64_t)c << (remain & 63))));
+    for (size_t i = 1; i < nrows; ++i) {
+      size_t entry = rows[i];
+      delta = entry - pre;
+      if (delta != 1) bp.skip((delta - 1) * remain);
+      c = bp.read_bits(remain);
+      out.push_back(value(fixed_vec[i] | ((uint64_t)c << (remain & 63))));
+      pre = entry;
+    }
+  }
+  return out;
+}
+
 int copy_filter(const FilterOut& fo, uint32_t* words, uint64_t nwords_cap, uint64_t* count, uint64_t* cand) {
   if (words) {
     if (fo.res.storage.size() > nwords_cap) return REF_SMALL_BUFFER;
@@ -1920,6 +2178,27 @@ int ref_filter(const uint8_t* s, uint64_t len, uint64_t scale, int op, double pr
   try {
     FilterOut fo = op == 0 ? buff_range_filter(s, len, scale, pred) : buff_equal_filter(s, len, scale, pred);
     return copy_filter(fo, words, nwords_cap, count, cand);
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+// buff_range_filter_condition over the ascending row ids `rows`.
+int ref_filter_cond(const uint8_t* s, uint64_t len, uint64_t scale, double pred, const uint64_t* rows, uint64_t nrows,
+                    uint32_t* words, uint64_t nwords_cap, uint64_t* count) {
+  try {
+    FilterOut fo = buff_range_filter_condition(s, len, scale, pred, rows, nrows);
+    return copy_filter(fo, words, nwords_cap, count, nullptr);
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+// buff_decode_condition over the ascending row ids `rows`.
+int ref_decode_cond(const uint8_t* s, uint64_t len, uint64_t scale, const uint64_t* rows, uint64_t nrows, double* out,
+                    uint64_t cap, uint64_t* n_out) {
+  try {
+    std::vector<double> v = buff_decode_condition(s, len, scale, rows, nrows);
+    *n_out = v.size();
+    if (v.size() > cap) return REF_SMALL_BUFFER;
+    if (!v.empty()) std::memcpy(out, v.data(), v.size() * 8);
+    return REF_OK;
   } catch (const RefPanic&) { return REF_PANIC; }
 }
 

@@ -157,6 +157,36 @@ def filter_bits(stream, scale, op, pred, semantic=False):
     return off == tm
 
 
+def filter_cond_bits(stream, scale, op, pred, cond, semantic=False):
+    """Rows of `cond` the conditional filter reports. buff_range_filter_condition
+    (buff_simd.rs:4683-4921) returns no rows for a predicate below the base and
+    compares against the target's low byte when F < 8; otherwise (and with
+    semantic) it is the plain filter on the condition rows."""
+    h = parse(stream)
+    cond = np.asarray(cond, bool)
+    if op == "range" and not semantic:
+        fp = fixed_pred(pred, scale, h["dlen"])
+        if fp < h["base"]:
+            return np.zeros(h["n"], bool)
+        if 0 < h["F"] < 8:
+            return (h["off"] > np.uint64((fp - h["base"]) & 0xFF)) & cond
+    return filter_bits(stream, scale, op, pred, semantic=semantic) & cond
+
+
+def decode_cond(stream, cond, semantic=False):
+    """Values of the `cond` rows in row order (buff_decode_condition,
+    buff_simd.rs:1528-1712): offset o = planes * 2^r + tail, but the reference
+    ORs the tail bits in at bit r when F >= 8 and r > 0: (planes | tail) * 2^r."""
+    h = parse(stream)
+    rows = np.flatnonzero(np.asarray(cond, bool))
+    off = h["off"][rows].astype(np.uint64)
+    F, r = h["F"], h["F"] % 8
+    if not semantic and F >= 8 and r:
+        rr = np.uint64(r)
+        off = ((off >> rr) | (off & np.uint64((1 << r) - 1))) << rr
+    return (np.int64(h["base"]) + off.astype(np.int64)).astype(np.float64) / float(1 << h["dlen"])
+
+
 def to_words(mask):
     """bool rows -> BitVec<u32> words."""
     n = len(mask)

@@ -179,6 +179,34 @@ def test_decode_and_queries_match_closed_form(oracle, name, x, scale):
     assert mval == cf.decode(s).max()
 
 
+@pytest.mark.parametrize("name,x,scale", list(_cases()))
+def test_condition_queries_match_closed_form(oracle, name, x, scale):
+    # buff_range_filter_condition / buff_decode_condition over random ascending
+    # row sets (empty, sparse, dense, every row), against the closed forms: a
+    # slip in the row skipping (skip_n_byte / skip by delta-1) or in the tail
+    # placement shows as a wrong row or value
+    s = oracle.encode(x, scale)
+    n = len(x)
+    rng = np.random.default_rng(11)
+    preds = list(rng.choice(x, 4)) + [float(x.min()) - 1, float(x.max()) + 1, float(np.median(x))]
+    for frac in (0.0, 0.003, 0.3, 1.0):
+        cond = rng.random(n) < frac
+        if frac == 0.003 and n:
+            cond[-1] = True   # the last row: the reference's final skip is 0
+        rows = np.flatnonzero(cond)
+        for pred in preds:
+            words, cnt = oracle.filter_cond(s, scale, pred, rows)
+            m = cf.filter_cond_bits(s, scale, "range", pred, cond)
+            assert np.array_equal(words, cf.to_words(m)), (name, frac, pred)
+            assert cnt == int(m.sum())
+        if len(rows):
+            got = oracle.decode_cond(s, scale, rows)
+            assert np.array_equal(got.view(np.uint64), cf.decode_cond(s, cond).view(np.uint64)), (name, frac)
+        else:
+            with pytest.raises(oracle.RefPanic):
+                oracle.decode_cond(s, scale, rows)
+
+
 def test_quirk_max_changes_width(oracle):
     # Q1: row 0 is the unique maximum; the `else if` loop never records it
     x = np.array([100.0, 1.0, 2.0, 3.0, 1.5])
