@@ -16,7 +16,9 @@ of the timed steps. `roofline` is reported for the kernel with the largest
 share of the step, from its algorithmic bytes (HBM bytes an ideal kernel must
 move) over its average duration against 8 TB/s. A cold-cache pass (the
 memory-side cache refilled with 1 GiB of other, clean data before each launch)
-is reported beside it.
+is reported beside it. `filter_project` times the conditional filter and the
+gather of a filter's rows (buff_range_filter_condition, buff_decode_condition)
+on the walk column, cold.
 
 Further sections of the same JSON line (each skippable):
   config3 — an outlier-heavy column of 67,108,864 rows in buff and buff-major:
@@ -168,6 +170,59 @@ def major_filter_rows(x, info, majors, pred, dlen):
         sh = np.uint64(F - 8 * (S + p + 1))
         cand &= ((off >> sh) & np.uint64(0xFF)).astype(np.int64) == ((t >> int(sh)) & 0xFF)
     return rows, int(cand.sum()) if info.tail_bits else 0
+
+
+def bench_filter_project(ctx, col, ch, info, words, out, flush, buff_amd):
+    """Filter -> project on the walk column, each launch after a 1 GiB cache
+    eviction (HIP events, mean of 3): the range filter at 10% restricted to an
+    earlier filter's rows (x above the median: the condition holds half the
+    rows), and the values of the rows of a 10% and a 0.1% filter gathered in row
+    order (BUFF_SEMANTIC). Algorithmic bytes: condition bitmap + result bitmap +
+    top plane for the filter; condition bitmap + every plane and tail bit of the
+    tiles holding condition rows + 8 bytes per value for the gather."""
+    n = info.n
+    nw = (n + 63) // 64
+    cond = ctx.alloc(nw * 8)
+    res = {}
+
+    def timed(fn, kname, trials=3):
+        ts = []
+        for _ in range(trials):
+            ctx.prof_reset()
+            flush.touch()
+            ctx.synchronize()
+            ctx.prof_enable(True)
+            fn()
+            ctx.synchronize()
+            ctx.prof_enable(False)
+            launches, ms = ctx.prof_query(kname)
+            ts.append(ms * 1e3 / max(launches, 1))
+        return float(np.mean(ts))
+
+    def entry(us, b, extra):
+        g = b / (us * 1e-6) / 1e9
+        d = {"avg_us": round(us, 2), "algorithmic_bytes": round(b), "achieved_GBps": round(g, 1),
+             "frac": round(g / HBM_PEAK_GBPS, 4)}
+        d.update(extra)
+        return d
+
+    ch.filter_device("range", col.preds[0.50], cond.ptr)
+    us = timed(lambda: ch.filter_cond_device("range", col.preds[0.10], cond.ptr, words.ptr), "k_filter_cond_gt")
+    res["filter_10pct_within_50pct"] = entry(us, n / 8.0 + n / 8.0 + n, {})
+    per_row = info.planes + info.tail_bits / 8.0
+    for sel in (0.10, 0.001):
+        ch.filter_device("range", col.preds[sel], cond.ptr)
+        bits = cond.download(np.uint64, nw)
+        rows = int(np.unpackbits(bits.view(np.uint8)).sum())
+        tb = np.concatenate([bits, np.zeros((-nw) % 16, np.uint64)]).reshape(-1, 16)   # 16 words = one tile
+        tiles = int(np.count_nonzero(tb.any(axis=1)))
+        us = timed(lambda: ch.materialize_cond_device(cond.ptr, out.ptr, n, flags=buff_amd.BUFF_SEMANTIC),
+                   "k_cond_decode")
+        b = n / 8.0 + tiles * 1024 * per_row + 8.0 * rows
+        res["gather_%g%%" % (sel * 100)] = entry(us, b, {"rows": rows, "tiles_with_rows": tiles,
+                                                         "GBps_f64_out": round(8.0 * rows / (us * 1e-6) / 1e9, 1)})
+    cond.free()
+    return res
 
 
 def bench_config3(ctx, args, buff_amd, datagen):
@@ -643,6 +698,7 @@ def main():
                        "achieved_GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4),
                        "note": "10% filter" if k == "k_filter_gt" else ("full precision" if k == "k_materialize"
                                                                        else "")}
+    filter_project = bench_filter_project(ctx, cols[0], keep[0], infos[0], words, out, flush, buff_amd)
     for ch in keep:
         ch.free()
 
@@ -669,6 +725,7 @@ def main():
         "launch_shapes": tuned,
         "cold_cache": cold,
         "cold_filters": filters,
+        "filter_project": filter_project,
     }
 
     if not args.no_config3:

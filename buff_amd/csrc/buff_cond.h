@@ -8,9 +8,8 @@
 
 namespace buffgpu {
 
-constexpr uint64_t kCondMaxTilesPerWg = 4096;   // tiles one gather workgroup scans (LDS prefix)
+constexpr uint32_t kCondTilesPerWg = 64;   // tiles one count / gather workgroup owns
 
-void cond_geometry(uint64_t ntiles, uint32_t* nwg, uint64_t* tpw);
 // Device scratch the decode needs, and where in it the total row count lands (u64).
 uint64_t cond_scratch_bytes(uint64_t ntiles);
 uint64_t cond_total_offset(uint64_t ntiles);
