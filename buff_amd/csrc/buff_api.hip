@@ -544,7 +544,9 @@ int filter_host(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, 
 // returns early: a predicate below the column base gives no rows (4706-4709),
 // and with F < 8 the target is its low byte (`fixed_target as u8`, 4718). The
 // equality form has no reference counterpart: buff_filter's equality on the
-// condition rows. fill: 0 = no rows, 1 = every condition row.
+// condition rows. buff_range_filter_majority_condition (buff_simd.rs:1279-1526)
+// is the same on a buff-major chunk without sparse columns. fill: 0 = no rows,
+// 1 = every condition row.
 static bool cond_plan(const Chunk* ch, int op, double pred, uint32_t flags, FilterTarget* tg, int* fill) {
   const ChunkView& v = ch->view;
   if (op == BUFF_OP_GT && !(flags & BUFF_SEMANTIC)) {
@@ -576,10 +578,21 @@ int filter_cond_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t 
   const uint64_t nwords = (ch->n + 63) / 64;
   if (nwords == 0) return BUFF_OK;
   if (!words || !cond) return fail(BUFF_E_INVALID, "null bitmap");
-  if (ch->S > 0) return fail(BUFF_E_UNSUPPORTED, "conditional filters take byte_fixed chunks");
   FilterTarget tg;
   int fill = 0;
-  if (!cond_plan(ch, op, pred, flags, &tg, &fill)) {
+  bool run = true;
+  if (ch->flags & BUFF_MAJORITY) {
+    if (ch->F < 8) return fail(BUFF_E_UNSUPPORTED, "buff-major chunk with F < 8: its column-count byte holds tail bits");
+    if (ch->S > 0) {
+      // buff_range_filter_majority_condition (buff_simd.rs:1313-1315) skips every
+      // column of a stream with sparse columns: no rows
+      if (op != BUFF_OP_GT || (flags & BUFF_SEMANTIC))
+        return fail(BUFF_E_UNSUPPORTED, "conditional filters of buff-major chunks with sparse columns");
+      run = false;
+      fill = 0;
+    }
+  }
+  if (!run || !cond_plan(ch, op, pred, flags, &tg, &fill)) {
     ProfScope ps(ctx, "k_cond_fill");
     HIP_TRY(launch_cond_fill(words, cond, nwords, ch->n, fill, ctx->stream));
     return BUFF_OK;
@@ -633,8 +646,21 @@ int filter_cond_host(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t fl
 int materialize_cond_device(Ctx* ctx, const Chunk* ch, uint32_t flags, const uint64_t* cond, double* out,
                             uint64_t cap, uint64_t* rows) {
   if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
-  if (ch->S > 0) return fail(BUFF_E_UNSUPPORTED, "conditional decode takes byte_fixed chunks");
   const bool semantic = (flags & BUFF_SEMANTIC) != 0;
+  const bool major = (ch->flags & BUFF_MAJORITY) != 0;
+  if (major) {
+    if (ch->F < 8) return fail(BUFF_E_UNSUPPORTED, "buff-major chunk with F < 8: its column-count byte holds tail bits");
+    if (ch->S > 0) {
+      // buff_major_decode_condition (buff_simd.rs:1738-1740) decodes nothing
+      // from a stream with sparse columns
+      if (semantic) return fail(BUFF_E_UNSUPPORTED, "conditional decode of buff-major chunks with sparse columns");
+      if (rows) *rows = 0;
+      return BUFF_OK;
+    }
+  }
+  // buff_major_decode_condition places the tail bits correctly and reads an
+  // empty condition as no rows
+  const bool compat = !semantic && !major;
   uint64_t total = 0;
   if (ch->n) {
     if (!cond || (cap && !out)) return fail(BUFF_E_INVALID, "null argument");
@@ -643,13 +669,13 @@ int materialize_cond_device(Ctx* ctx, const Chunk* ch, uint32_t flags, const uin
     uint8_t* scr = (uint8_t*)ctx->s_cond.ptr;
     {
       ProfScope ps(ctx, "k_cond_decode");
-      HIP_TRY(launch_cond_decode(ch->view, std::ldexp(1.0, -(int)ch->dlen), !semantic, cond, out, cap, scr,
+      HIP_TRY(launch_cond_decode(ch->view, std::ldexp(1.0, -(int)ch->dlen), compat, cond, out, cap, scr,
                                  ctx->stream));
     }
     BUFF_TRY(d2h_small(ctx, &total, scr + cond_total_offset(ntiles), 8));
   }
   if (rows) *rows = total;
-  if (total == 0 && !semantic) return fail(BUFF_E_REF_PANIC, "empty condition: the reference unwraps the first row");
+  if (total == 0 && compat) return fail(BUFF_E_REF_PANIC, "empty condition: the reference unwraps the first row");
   if (total > cap)
     return fail(BUFF_E_SMALL_BUFFER, "%llu condition rows, room for %llu", (unsigned long long)total,
                 (unsigned long long)cap);
@@ -659,7 +685,6 @@ int materialize_cond_device(Ctx* ctx, const Chunk* ch, uint32_t flags, const uin
 int materialize_cond_host(Ctx* ctx, const Chunk* ch, uint32_t flags, const uint32_t* cond_words, double* out,
                           uint64_t cap, uint64_t* rows) {
   if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
-  if (ch->S > 0) return fail(BUFF_E_UNSUPPORTED, "conditional decode takes byte_fixed chunks");
   if (ch->n && !cond_words) return fail(BUFF_E_INVALID, "null condition");
   const uint64_t* cond = nullptr;
   if (ch->n) {

@@ -147,8 +147,13 @@ int buff_filter_device(buff_ctx* ctx, const buff_chunk* chunk, int op, double pr
  * A condition is a bitmap in the layout the filters write (bit i = row i; the
  * _device forms take ceil(n/64) u64 words, the host forms ceil(n/32) u32
  * words), e.g. the result of an earlier filter: predicates chain, and a
- * filter's rows can be materialized. byte_fixed chunks only (a buff-major
- * chunk is BUFF_E_UNSUPPORTED).
+ * filter's rows can be materialized. On a buff-major chunk these are
+ * buff_range_filter_majority_condition (buff_simd.rs:1279-1526) and
+ * buff_major_decode_condition (1714-1914): with sparse columns the reference
+ * answers nothing (no rows, no values; BUFF_SEMANTIC and the equality form are
+ * BUFF_E_UNSUPPORTED there); without, they equal the byte_fixed forms except
+ * that the decode places the tail bits correctly and an empty condition gives
+ * no values. F < 8 buff-major chunks are BUFF_E_UNSUPPORTED.
  *
  * buff_range_filter_condition (buff_simd.rs:4683-4921): x > pred over the
  * condition rows (the reference takes their row ids as an iterator); rows

@@ -1143,8 +1143,11 @@ FilterOut buff_simd_equal_filter(const uint8_t* bytes, size_t blen, uint64_t sca
 
 // buff_simd.rs:4683-4921 (buff_range_filter_condition): x > pred over the
 // rows the iterator yields (`rows`, ascending). The result keeps no other row.
+// major: buff_range_filter_majority_condition (buff_simd.rs:1279-1526), the
+// same after the column-count byte, and no rows when there are sparse columns
+// (1313-1315 log "not implemented" and skip everything).
 FilterOut buff_range_filter_condition(const uint8_t* bytes, size_t blen, uint64_t scale, double pred,
-                                      const uint64_t* rows, size_t nrows) {
+                                      const uint64_t* rows, size_t nrows, bool major = false) {
   BitPackReader bp(bytes, blen);
   Header h = read_header(bp);
   size_t len = h.len;
@@ -1159,6 +1162,10 @@ FilterOut buff_range_filter_condition(const uint8_t* bytes, size_t blen, uint64_
   }
   uint64_t fixed_target = (uint64_t)fixed_part - (uint64_t)h.base;
   uint8_t cur_tar = 0;
+  if (major && bp.read(8) > 0) {        // 1308-1315
+    fo.res = std::move(res);
+    return fo;
+  }
   size_t ri = 0;
   auto next_row = [&](size_t* out) {
     if (ri >= nrows) return false;
@@ -1292,8 +1299,11 @@ FilterOut buff_range_filter_condition(const uint8_t* bytes, size_t blen, uint64_
 // iterator yields (`rows`, ascending), in that order. An empty iterator is the
 // reference's unwrap of None. When F % 8 != 0 the tail bits are OR-ed in
 // shifted left by the tail width (1693, 1701), not in place.
+// major: buff_major_decode_condition (buff_simd.rs:1714-1914): after the
+// column-count byte, nothing when there are sparse columns (1738-1740); the
+// tail bits in place; an empty condition gives no values instead of a panic.
 std::vector<double> buff_decode_condition(const uint8_t* bytes, size_t blen, uint64_t scale, const uint64_t* rows,
-                                          size_t nrows) {
+                                          size_t nrows, bool major = false) {
   (void)scale;   // 1529-1533 build a PrecisionBound from it that the decode never reads
   BitPackReader bp(bytes, blen);
   Header h = read_header(bp);
@@ -1302,7 +1312,11 @@ std::vector<double> buff_decode_condition(const uint8_t* bytes, size_t blen, uin
   std::vector<uint64_t> fixed_vec;
   const double dec_scl = powi(2.0, (int32_t)h.dlen);
   size_t remain = (size_t)(uint32_t)(h.dlen + h.ilen);   // `(dlen+ilen) as usize`: the u32 sum wraps first
-  if (nrows == 0) throw RefPanic("iter.next().unwrap() on an empty condition");
+  if (major) {
+    if (bp.read(8) > 0 || nrows == 0) return out;
+  } else if (nrows == 0) {
+    throw RefPanic("iter.next().unwrap() on an empty condition");
+  }
   auto value = [&](uint64_t off) { return (double)(int64_t)((uint64_t)h.base + off) / dec_scl; };
   size_t pre, delta;
   if (remain < 8) {                     // 1559-1577
@@ -1385,14 +1399,15 @@ std::vector<double> buff_decode_condition(const uint8_t* bytes, size_t blen, uin
     bp.finish_read_byte();
     pre = rows[0];
     bp.skip(pre * remain);
+    const uint32_t tsh = major ? 0u : (uint32_t)(remain & 63);   // 1693/1701 shift; 1895/1909 do not
     uint8_t c = bp.read_bits(remain);
-    out.push_back(value(fixed_vec[0] | ((uint64_t)c << (remain & 63))));
+    out.push_back(value(fixed_vec[0] | ((uint64_t)c << tsh)));
     for (size_t i = 1; i < nrows; ++i) {
       size_t entry = rows[i];
       delta = entry - pre;
       if (delta != 1) bp.skip((delta - 1) * remain);
       c = bp.read_bits(remain);
-      out.push_back(value(fixed_vec[i] | ((uint64_t)c << (remain & 63))));
+      out.push_back(value(fixed_vec[i] | ((uint64_t)c << tsh)));
       pre = entry;
     }
   }
@@ -2181,20 +2196,20 @@ int ref_filter(const uint8_t* s, uint64_t len, uint64_t scale, int op, double pr
   } catch (const RefPanic&) { return REF_PANIC; }
 }
 
-// buff_range_filter_condition over the ascending row ids `rows`.
+// buff_range_filter_condition (major: _majority_condition) over the ascending row ids `rows`.
 int ref_filter_cond(const uint8_t* s, uint64_t len, uint64_t scale, double pred, const uint64_t* rows, uint64_t nrows,
-                    uint32_t* words, uint64_t nwords_cap, uint64_t* count) {
+                    int major, uint32_t* words, uint64_t nwords_cap, uint64_t* count) {
   try {
-    FilterOut fo = buff_range_filter_condition(s, len, scale, pred, rows, nrows);
+    FilterOut fo = buff_range_filter_condition(s, len, scale, pred, rows, nrows, major != 0);
     return copy_filter(fo, words, nwords_cap, count, nullptr);
   } catch (const RefPanic&) { return REF_PANIC; }
 }
 
-// buff_decode_condition over the ascending row ids `rows`.
-int ref_decode_cond(const uint8_t* s, uint64_t len, uint64_t scale, const uint64_t* rows, uint64_t nrows, double* out,
-                    uint64_t cap, uint64_t* n_out) {
+// buff_decode_condition (major: buff_major_decode_condition) over the ascending row ids `rows`.
+int ref_decode_cond(const uint8_t* s, uint64_t len, uint64_t scale, const uint64_t* rows, uint64_t nrows, int major,
+                    double* out, uint64_t cap, uint64_t* n_out) {
   try {
-    std::vector<double> v = buff_decode_condition(s, len, scale, rows, nrows);
+    std::vector<double> v = buff_decode_condition(s, len, scale, rows, nrows, major != 0);
     *n_out = v.size();
     if (v.size() > cap) return REF_SMALL_BUFFER;
     if (!v.empty()) std::memcpy(out, v.data(), v.size() * 8);

@@ -62,9 +62,9 @@ def lib():
             "ref_filter": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_double,
                                           u32p, ctypes.c_uint64, u64p, u64p]),
             "ref_filter_cond": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, u64p,
-                                               ctypes.c_uint64, u32p, ctypes.c_uint64, u64p]),
-            "ref_decode_cond": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, u64p, ctypes.c_uint64, f64p,
-                                               ctypes.c_uint64, u64p]),
+                                               ctypes.c_uint64, ctypes.c_int, u32p, ctypes.c_uint64, u64p]),
+            "ref_decode_cond": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, u64p, ctypes.c_uint64,
+                                               ctypes.c_int, f64p, ctypes.c_uint64, u64p]),
             "ref_simd_filter": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
                                                ctypes.c_double, u32p, ctypes.c_uint64, u64p]),
             "ref_sum": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, f64p, u64p]),
@@ -198,26 +198,28 @@ def buff_filter(stream, scale, op, pred):
     return words[: (n + 31) // 32], cnt.value, cand
 
 
-def filter_cond(stream, scale, pred, rows):
-    """buff_range_filter_condition over ascending row ids. Returns (u32 words, count)."""
+def filter_cond(stream, scale, pred, rows, major=False):
+    """buff_range_filter_condition (major: buff_range_filter_majority_condition) over ascending
+    row ids. Returns (u32 words, count)."""
     s = np.ascontiguousarray(stream, dtype=np.uint8)
     n = header(s)[1]
     r = np.ascontiguousarray(rows, dtype=np.uint64)
     words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
     cnt = ctypes.c_uint64()
     _check(lib().ref_filter_cond(_ptr(s, ctypes.c_uint8), len(s), scale, float(pred), _ptr(r, ctypes.c_uint64), len(r),
-                                 _ptr(words, ctypes.c_uint32), len(words), ctypes.byref(cnt)))
+                                 int(major), _ptr(words, ctypes.c_uint32), len(words), ctypes.byref(cnt)))
     return words[: (n + 31) // 32], cnt.value
 
 
-def decode_cond(stream, scale, rows):
-    """buff_decode_condition over ascending row ids: the rows' values in order."""
+def decode_cond(stream, scale, rows, major=False):
+    """buff_decode_condition (major: buff_major_decode_condition) over ascending row ids: the
+    rows' values in order."""
     s = np.ascontiguousarray(stream, dtype=np.uint8)
     r = np.ascontiguousarray(rows, dtype=np.uint64)
     out = np.zeros(max(len(r), 1), dtype=np.float64)
     got = ctypes.c_uint64()
     _check(lib().ref_decode_cond(_ptr(s, ctypes.c_uint8), len(s), scale, _ptr(r, ctypes.c_uint64), len(r),
-                                 _ptr(out, ctypes.c_double), len(out), ctypes.byref(got)))
+                                 int(major), _ptr(out, ctypes.c_double), len(out), ctypes.byref(got)))
     return out[: got.value]
 
 

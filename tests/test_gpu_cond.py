@@ -158,14 +158,33 @@ def test_all_rows_is_the_plain_path_at_scale(gpu_ctx):
         ch.free()
 
 
-def test_majority_chunk_is_unsupported(gpu_ctx):
-    x = dg.outlier_heavy(10_000, 3, 6)
-    ch = gpu_ctx.encode(x, 10 ** 3, flags=buff_amd.BUFF_MAJORITY)
+@pytest.mark.parametrize("case", ["config3", "no_majority", "all_sparse"])
+def test_majority_forms_match_oracle(gpu_ctx, oracle, case):
+    # buff_range_filter_majority_condition / buff_major_decode_condition: nothing
+    # when the chunk has sparse columns, the byte_fixed forms (tail in place, no
+    # panic on an empty condition) when it has none
+    from tests import majority_cases as mc
+
+    build, scale = mc.CASES[case]
+    x = build()
+    s = oracle.major_encode(x, scale)
+    ch = gpu_ctx.encode(x, scale, flags=buff_amd.BUFF_MAJORITY)
     try:
-        cw = cf.to_words(np.ones(len(x), bool))
-        with pytest.raises(N.Unsupported):
-            ch.filter_cond("range", 100.0, cw)
-        with pytest.raises(N.Unsupported):
-            ch.materialize_cond(cw)
+        sparse = ch.info().sparse_cols > 0
+        for cname, cond in _conds(len(x), 13):
+            cw = cf.to_words(cond)
+            rows = np.flatnonzero(cond)
+            for pred in mc.predicates(x)[:6]:
+                words, cnt = ch.filter_cond("range", pred, cw)
+                ref, rcnt = oracle.filter_cond(s, scale, pred, rows, major=True)
+                assert np.array_equal(words, ref) and cnt == rcnt, (case, cname, pred)
+            got = ch.materialize_cond(cw)
+            ref = oracle.decode_cond(s, scale, rows, major=True)
+            assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), (case, cname)
+            if sparse:
+                with pytest.raises(N.Unsupported):
+                    ch.materialize_cond(cw, flags=N.BUFF_SEMANTIC)
+                with pytest.raises(N.Unsupported):
+                    ch.filter_cond("equal", 100.0, cw)
     finally:
         ch.free()
