@@ -128,6 +128,8 @@ SIGNATURES = {
     "buff_filter_cond_device": [vp, vp, cint, f64, u32, vp, vp],
     "buff_materialize_cond": [vp, vp, u32, u32p, f64p, u64, u64p],
     "buff_materialize_cond_device": [vp, vp, u32, vp, vp, u64, u64p],
+    "buff_max_range": [vp, vp, u64, u64, u64, u32, f64p, u64, u64p, u32p, u64],
+    "buff_max_range_device": [vp, vp, u64, u64, u64, u32, vp, u64, u64p, vp],
     "buff_materialize": [vp, vp, i32, u32, f64p],
     "buff_materialize_device": [vp, vp, i32, u32, vp],
     "buff_batch_encode_device": [vp, vp, u64p, u64p, u32, u32, P(vp)],
@@ -254,6 +256,24 @@ class Chunk:
         opc = BUFF_OP_GT if op in ("range", "gt", BUFF_OP_GT) else BUFF_OP_EQ
         check(lib().buff_filter_cond_device(self.ctx.handle, self.handle, opc, float(pred), flags, cond_dev,
                                             words_dev))
+
+    def max_range(self, s, e, window, flags=0):
+        """buff_max_range: (window max values, argmax u32 words)."""
+        n = self.n
+        cap = max((e - s + window - 1) // window, 1) if window and e >= s else 1
+        vals = np.zeros(cap, dtype=np.float64)
+        nwin = ctypes.c_uint64()
+        words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
+        check(lib().buff_max_range(self.ctx.handle, self.handle, s, e, window, flags, ptr(vals, ctypes.c_double), cap,
+                                   ctypes.byref(nwin), ptr(words, ctypes.c_uint32), len(words)))
+        return vals[: nwin.value], words[: (n + 31) // 32]
+
+    def max_range_device(self, s, e, window, vals_dev, cap, argmax_dev=None, flags=0):
+        """Returns the number of windows (values at vals_dev)."""
+        nwin = ctypes.c_uint64()
+        check(lib().buff_max_range_device(self.ctx.handle, self.handle, s, e, window, flags, vals_dev, cap,
+                                          ctypes.byref(nwin), argmax_dev))
+        return nwin.value
 
     def materialize_cond(self, cond_words, flags=0):
         """Values of the rows set in cond_words (u32 words), in row order."""

@@ -24,6 +24,7 @@
 #include "buff_kernels.h"
 #include "buff_runtime.h"
 #include "buff_launch.h"
+#include "buff_wmax.h"
 
 namespace buffgpu {
 
@@ -706,6 +707,76 @@ int materialize_cond_host(Ctx* ctx, const Chunk* ch, uint32_t flags, const uint3
   return BUFF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Windowed max: buff_max_range (buff_simd.rs:5096-5334).
+
+// The reference's outcome on [s, e) in windows of W rows: the maxima of the
+// full windows (its max_vec; a partial last window is dropped when F <= 8 and
+// is a panic when F > 8, as is an empty range), and the argmax rows, which only
+// its tail stage fills (F < 8, or F > 8 with a tail). BUFF_SEMANTIC: every
+// window including a partial last one, and the argmax rows in every case.
+int max_range_device(Ctx* ctx, const Chunk* ch, uint64_t s, uint64_t e, uint64_t W, uint32_t flags, double* vals,
+                     uint64_t cap, uint64_t* nwin, uint64_t* argmax) {
+  if (!ctx || !ch || !nwin) return fail(BUFF_E_INVALID, "null argument");
+  if (W == 0 || s > e || e > ch->n)
+    return fail(BUFF_E_INVALID, "window %llu over rows [%llu, %llu) of %llu", (unsigned long long)W,
+                (unsigned long long)s, (unsigned long long)e, (unsigned long long)ch->n);
+  if (ch->flags & BUFF_MAJORITY) return fail(BUFF_E_UNSUPPORTED, "windowed max takes byte_fixed chunks");
+  const bool semantic = (flags & BUFF_SEMANTIC) != 0;
+  const ChunkView& v = ch->view;
+  const uint64_t L = e - s;
+  if (!semantic && v.F > 8 && (L == 0 || L % W))
+    return fail(BUFF_E_REF_PANIC, "the reference asserts that windows of %llu rows tile %llu rows",
+                (unsigned long long)W, (unsigned long long)L);
+  const WmaxLayout lay = wmax_layout(s, e, W);
+  const uint64_t k = semantic ? lay.nwin : L / W;
+  const bool want = semantic || v.F < 8 || (v.F > 8 && v.r > 0);
+  *nwin = k;
+  if (k > cap)
+    return fail(BUFF_E_SMALL_BUFFER, "%llu windows, room for %llu", (unsigned long long)k, (unsigned long long)cap);
+  if (k && !vals) return fail(BUFF_E_INVALID, "null output");
+  const uint64_t nwords = (ch->n + 63) / 64;
+  if (lay.ntiles == 0) {
+    if (argmax && nwords) HIP_TRY(launch_fill_bitmap(argmax, nwords, ch->n, 0, ctx->stream));
+    return BUFF_OK;
+  }
+  BUFF_TRY(ctx->scratch(ctx->s_wmax, lay.bytes));
+  uint8_t* scr = (uint8_t*)ctx->s_wmax.ptr;
+  ProfScope ps(ctx, "k_wmax");
+  HIP_TRY(launch_wmax(v, s, e, W, lay, scr, ctx->stream));
+  if (argmax) HIP_TRY(launch_wmax_argmax(v, s, e, W, want, scr, lay, argmax, ctx->stream));
+  HIP_TRY(launch_wmax_values(scr, lay, k, v.base, std::ldexp(1.0, -(int)ch->dlen), vals, ctx->stream));
+  return BUFF_OK;
+}
+
+int max_range_host(Ctx* ctx, const Chunk* ch, uint64_t s, uint64_t e, uint64_t W, uint32_t flags, double* vals,
+                   uint64_t cap, uint64_t* nwin, uint32_t* argmax_words, uint64_t nwords32) {
+  if (!ctx || !ch || !nwin) return fail(BUFF_E_INVALID, "null argument");
+  const uint64_t need32 = (ch->n + 31) / 32, nwords64 = (ch->n + 63) / 64;
+  if (argmax_words && nwords32 < need32)
+    return fail(BUFF_E_SMALL_BUFFER, "bitmap needs %llu u32 words", (unsigned long long)need32);
+  if (W == 0 || s > e || e > ch->n)
+    return fail(BUFF_E_INVALID, "window %llu over rows [%llu, %llu) of %llu", (unsigned long long)W,
+                (unsigned long long)s, (unsigned long long)e, (unsigned long long)ch->n);
+  const uint64_t nall = e > s ? (e - s + W - 1) / W : 0;
+  BUFF_TRY(ctx->scratch(ctx->s_values, std::max<uint64_t>(nall, 1) * 8));
+  if (argmax_words && nwords64) BUFF_TRY(ctx->scratch(ctx->s_bitmap, nwords64 * 8));
+  BUFF_TRY(max_range_device(ctx, ch, s, e, W, flags, (double*)ctx->s_values.ptr, nall, nwin,
+                            argmax_words && nwords64 ? (uint64_t*)ctx->s_bitmap.ptr : nullptr));
+  if (*nwin > cap)
+    return fail(BUFF_E_SMALL_BUFFER, "%llu windows, room for %llu", (unsigned long long)*nwin, (unsigned long long)cap);
+  if (*nwin) {
+    if (!vals) return fail(BUFF_E_INVALID, "null output");
+    HIP_TRY(hipMemcpyAsync(vals, ctx->s_values.ptr, *nwin * 8, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  std::vector<uint64_t> host(argmax_words ? nwords64 : 0);
+  if (argmax_words && nwords64)
+    HIP_TRY(hipMemcpyAsync(host.data(), ctx->s_bitmap.ptr, nwords64 * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (argmax_words && need32) memcpy(argmax_words, host.data(), need32 * 4);
+  return BUFF_OK;
+}
+
 // split_double.rs:2127-2208 fold of the per-sub-column sums.
 double reference_sum_fold(const Chunk* ch, const uint64_t* ps) {
   const uint32_t len = (uint32_t)ch->n, ilen = ch->ilen, dlen = ch->dlen;
@@ -1300,6 +1371,19 @@ int buff_materialize_cond_device(buff_ctx* ctx, const buff_chunk* chunk, uint32_
                                  double* out_dev, uint64_t cap, uint64_t* rows) {
   return materialize_cond_device(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), flags,
                                  cond_dev, out_dev, cap, rows);
+}
+
+int buff_max_range(buff_ctx* ctx, const buff_chunk* chunk, uint64_t s, uint64_t e, uint64_t window, uint32_t flags,
+                   double* max_values, uint64_t cap, uint64_t* nwindows, uint32_t* argmax_words, uint64_t nwords) {
+  return max_range_host(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), s, e, window, flags,
+                        max_values, cap, nwindows, argmax_words, nwords);
+}
+
+int buff_max_range_device(buff_ctx* ctx, const buff_chunk* chunk, uint64_t s, uint64_t e, uint64_t window,
+                          uint32_t flags, double* max_values_dev, uint64_t cap, uint64_t* nwindows,
+                          uint64_t* argmax_dev) {
+  return max_range_device(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), s, e, window, flags,
+                          max_values_dev, cap, nwindows, argmax_dev);
 }
 
 int buff_sum(buff_ctx* ctx, const buff_chunk* chunk, double* ref_sum, double* exact_sum, uint64_t* plane_sums) {

@@ -68,7 +68,7 @@ struct Ctx {
   std::multimap<uint64_t, void*> free_blocks;
   void* pinned = nullptr;
 
-  Scratch s_partial, s_minmax, s_stream, s_bitmap, s_sums, s_best, s_values, s_sink, s_major, s_cond, s_condin;
+  Scratch s_partial, s_minmax, s_stream, s_bitmap, s_sums, s_best, s_values, s_sink, s_major, s_cond, s_condin, s_wmax;
 
   bool prof_on = false;
   std::vector<hipEvent_t> prof_pool;

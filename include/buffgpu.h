@@ -178,6 +178,25 @@ int buff_materialize_cond(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags
 int buff_materialize_cond_device(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, const uint64_t* cond_dev,
                                  double* out_dev, uint64_t cap, uint64_t* rows);
 
+/* buff_max_range (buff_simd.rs:5096-5334): the max of every window of `window`
+ * rows of [s, e), as the reference's f64 (max offset + base) / 2^dlen, in window
+ * order (*nwindows of them, at most cap; BUFF_E_SMALL_BUFFER beyond), and
+ * (argmax_words, nullable: ceil(n/32) u32 words) the rows holding their
+ * window's max. Default reproduces the reference: only full windows are
+ * reported; when F > 8 a range the windows do not tile exactly (or an empty
+ * one) is its panic (BUFF_E_REF_PANIC); the argmax rows are filled only where
+ * its tail stage runs (F < 8, or F > 8 with F % 8 != 0), and for F < 8 they
+ * include the rows of a partial last window. BUFF_SEMANTIC: every window,
+ * including a partial last one, and the argmax rows in every case. window = 0
+ * or a range outside [0, n] is BUFF_E_INVALID; byte_fixed chunks only. The
+ * device form writes values and ceil(n/64) u64 argmax words to device memory
+ * (argmax_dev nullable). */
+int buff_max_range(buff_ctx* ctx, const buff_chunk* chunk, uint64_t s, uint64_t e, uint64_t window, uint32_t flags,
+                   double* max_values, uint64_t cap, uint64_t* nwindows, uint32_t* argmax_words, uint64_t nwords);
+int buff_max_range_device(buff_ctx* ctx, const buff_chunk* chunk, uint64_t s, uint64_t e, uint64_t window,
+                          uint32_t flags, double* max_values_dev, uint64_t cap, uint64_t* nwindows,
+                          uint64_t* argmax_dev);
+
 /* byte_fixed_sum (split_double.rs:2127-2208). ref_sum reproduces the reference
  * f64 fold bit for bit; exact_sum = (n*base + sum(off)) / 2^dlen rounded once;
  * plane_sums (nullable, 9 u64) are the exact per-sub-column byte sums (tail last). */

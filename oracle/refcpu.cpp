@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1414,6 +1415,186 @@ std::vector<double> buff_decode_condition(const uint8_t* bytes, size_t blen, uin
   return out;
 }
 
+// croaring Bitmap::remove_range(a..b) on the sorted row set.
+void remove_range(std::set<uint32_t>& b, uint32_t lo, uint32_t hi) {
+  if (lo >= hi) return;
+  b.erase(b.lower_bound(lo), b.lower_bound(hi));
+}
+
+// buff_simd.rs:5096-5334 (buff_max_range): the max of each window of `window`
+// rows of [s, e) (max_vec, stored offsets) and the croaring bitmap `res` of the
+// rows holding it, which only the tail stage (or an F < 8 column) fills. u32
+// arithmetic wraps as in release Rust; Vec::get_mut().unwrap(), assert_eq! and
+// division by zero are panics.
+struct MaxRangeOut {
+  std::vector<uint64_t> maxv;
+  std::set<uint32_t> res;
+};
+MaxRangeOut buff_max_range(const uint8_t* bytes, size_t blen, uint32_t s, uint32_t e, uint32_t window) {
+  BitPackReader bp(bytes, blen);
+  Header h = read_header(bp);
+  const uint32_t len = h.len;
+  uint32_t remain = h.dlen + h.ilen;
+  std::set<uint32_t> rb1;
+  MaxRangeOut o;
+  uint64_t max = 0;
+  uint8_t byte_max = 0;
+  uint32_t cur_s = s;
+  auto at = [&](uint32_t cs) -> uint64_t& {
+    if (window == 0) throw RefPanic("attempt to divide by zero");
+    const size_t k = (size_t)((uint32_t)(cs - s) / window);
+    if (k >= o.maxv.size()) throw RefPanic("max_vec.get_mut(..).unwrap() on None");
+    return o.maxv[k];
+  };
+  auto check = [&](uint32_t cs) {
+    if (window == 0) throw RefPanic("attempt to divide by zero");
+    if ((uint32_t)(cs - s) / window + 1 != (uint32_t)(e - s) / window) throw RefPanic("assert_eq! in buff_max_range");
+  };
+  if (remain < 8) {                                                   // 5118-5143
+    bp.skip((size_t)(uint32_t)(s * remain));
+    for (uint32_t i = s; i < e; ++i) {
+      if (i == (uint32_t)(cur_s + window)) {
+        cur_s = i;
+        o.maxv.push_back(max);
+        max = 0;
+      }
+      const uint64_t cur = bp.read_bits(remain);
+      if (cur > max) {
+        max = cur;
+        remove_range(o.res, cur_s, i);
+        o.res.insert(i);
+      } else if (cur == max) {
+        o.res.insert(i);
+      }
+    }
+    if (e == (uint32_t)(cur_s + window)) {
+      cur_s = e;
+      o.maxv.push_back(max);
+      max = 0;
+    }
+    remain = 0;
+  } else {                                                            // 5144-5175
+    remain -= 8;
+    bp.skip_n_byte(s);
+    const uint32_t cnt = e - s;
+    const uint8_t* chunk = bp.read_n_byte(cnt);
+    uint32_t i = s;
+    for (uint32_t k = 0; k < cnt; ++k) {
+      const uint8_t c = chunk[k];
+      if (i == (uint32_t)(cur_s + window)) {
+        cur_s = i;
+        o.maxv.push_back((uint64_t)byte_max << (remain & 63));
+        byte_max = 0;
+      }
+      if (c > byte_max) {
+        byte_max = c;
+        remove_range(rb1, cur_s, i);
+        rb1.insert(i);
+      } else if (c == byte_max) {
+        rb1.insert(i);
+      }
+      i += 1;
+    }
+    if (i == (uint32_t)(cur_s + window)) {
+      cur_s = i;
+      o.maxv.push_back((uint64_t)byte_max << (remain & 63));
+      byte_max = 0;
+    }
+    bp.skip_n_byte((uint32_t)(len - e));
+  }
+  while (remain > 0) {                                                // 5177-5331
+    byte_max = 0;
+    uint32_t cs = s;
+    if (remain >= 8) {
+      remain -= 8;
+      uint32_t dec_cur = 0, dec_pre = 0;
+      std::set<uint32_t> cur_rb;
+      auto it = rb1.begin();
+      if (it != rb1.end()) {
+        dec_cur = *it;
+        if (dec_cur != 0) bp.skip_n_byte(dec_cur);
+        const uint8_t dec = bp.read_byte();
+        if (dec > byte_max) {
+          byte_max = dec;
+          cur_rb.insert(dec_cur);
+        } else if (dec == byte_max) {
+          cur_rb.insert(dec_cur);
+        }
+        ++it;
+        dec_pre = dec_cur;
+      }
+      for (; it != rb1.end(); ++it) {
+        dec_cur = *it;
+        if (dec_cur >= (uint32_t)(cs + window)) {
+          at(cs) |= (uint64_t)byte_max << (remain & 63);
+          byte_max = 0;
+          cs = cs + window;
+        }
+        const uint32_t delta = dec_cur - dec_pre;
+        if (delta != 1) bp.skip_n_byte((uint32_t)(delta - 1));
+        const uint8_t dec = bp.read_byte();
+        if (dec > byte_max) {
+          byte_max = dec;
+          remove_range(cur_rb, cs, dec_cur);
+          cur_rb.insert(dec_cur);
+        } else if (dec == byte_max) {
+          cur_rb.insert(dec_cur);
+        }
+        dec_pre = dec_cur;
+      }
+      check(cs);
+      at(cs) |= (uint64_t)byte_max << (remain & 63);
+      byte_max = 0;
+      cs = cs + window;
+      if ((uint32_t)(len - dec_pre) > 1) bp.skip_n_byte((uint32_t)(len - dec_pre - 1));
+      rb1 = cur_rb;
+    } else {
+      byte_max = 0;
+      bp.finish_read_byte();
+      uint32_t dec_cur = 0, dec_pre = 0;
+      auto it = rb1.begin();
+      if (it != rb1.end()) {
+        dec_cur = *it;
+        if (dec_cur != 0) bp.skip((size_t)(uint32_t)(dec_cur * remain));
+        const uint8_t dec = bp.read_bits(remain);
+        if (dec > byte_max) {
+          byte_max = dec;
+          o.res.insert(dec_cur);
+        } else if (dec == byte_max) {
+          o.res.insert(dec_cur);
+        }
+        ++it;
+        dec_pre = dec_cur;
+      }
+      for (; it != rb1.end(); ++it) {
+        dec_cur = *it;
+        if (dec_cur >= (uint32_t)(cs + window)) {
+          at(cs) |= (uint64_t)byte_max;
+          byte_max = 0;
+          cs = cs + window;
+        }
+        const uint32_t delta = dec_cur - dec_pre;
+        if (delta != 1) bp.skip((size_t)(uint32_t)((delta - 1) * remain));
+        const uint8_t dec = bp.read_bits(remain);
+        if (dec > byte_max) {
+          byte_max = dec;
+          remove_range(o.res, cs, dec_cur);
+          o.res.insert(dec_cur);
+        } else if (dec == byte_max) {
+          o.res.insert(dec_cur);
+        }
+        dec_pre = dec_cur;
+      }
+      check(cs);
+      at(cs) |= (uint64_t)byte_max;
+      byte_max = 0;
+      cs = cs + window;
+      remain = 0;
+    }
+  }
+  return o;
+}
+
 int copy_filter(const FilterOut& fo, uint32_t* words, uint64_t nwords_cap, uint64_t* count, uint64_t* cand) {
   if (words) {
     if (fo.res.storage.size() > nwords_cap) return REF_SMALL_BUFFER;
@@ -2238,6 +2419,30 @@ int ref_sum(const uint8_t* s, uint64_t len, uint64_t scale, double* sum, uint64_
 int ref_max(const uint8_t* s, uint64_t len, uint64_t scale, double* mx, uint64_t* max_off, uint64_t* argmax_count) {
   try {
     *mx = byte_fixed_max(s, len, scale, max_off, argmax_count);
+    return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+// buff_max_range: the window maxima (stored offsets, and as the reference's
+// f64 `(x as i64 + base) / 2^dlen`) and the argmax rows as BitVec<u32> words.
+int ref_max_range(const uint8_t* s, uint64_t len, uint32_t rs, uint32_t re, uint32_t window, uint64_t* offs,
+                  double* vals, uint64_t cap, uint64_t* nwin, uint32_t* words, uint64_t nwords_cap) {
+  try {
+    MaxRangeOut o = buff_max_range(s, len, rs, re, window);
+    BitPackReader bp(s, len);
+    Header h = read_header(bp);
+    *nwin = o.maxv.size();
+    if (o.maxv.size() > cap || (size_t)(h.len + 31) / 32 > nwords_cap) return REF_SMALL_BUFFER;
+    const double dec_scl = powi(2.0, (int32_t)h.dlen);
+    for (size_t k = 0; k < o.maxv.size(); ++k) {
+      offs[k] = o.maxv[k];
+      vals[k] = (double)(int64_t)(o.maxv[k] + (uint64_t)h.base) / dec_scl;
+    }
+    std::memset(words, 0, 4 * ((size_t)(h.len + 31) / 32));
+    for (uint32_t r : o.res) {
+      if (r >= h.len) throw RefPanic("argmax row past the column");
+      words[r / 32] |= 1u << (r % 32);
+    }
     return REF_OK;
   } catch (const RefPanic&) { return REF_PANIC; }
 }

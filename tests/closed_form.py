@@ -187,6 +187,38 @@ def decode_cond(stream, cond, semantic=False):
     return (np.int64(h["base"]) + off.astype(np.int64)).astype(np.float64) / float(1 << h["dlen"])
 
 
+class Panic(Exception):
+    """The reference panics on this input."""
+
+
+def max_range(stream, s, e, window, semantic=False):
+    """buff_max_range (buff_simd.rs:5096-5334) from its arithmetic: the max stored
+    offset of each window of `window` rows of [s, e) and the rows holding it.
+    The reference keeps only full windows, panics when F > 8 and the windows do
+    not tile [s, e) exactly (or it is empty), and fills the argmax rows only in
+    its tail stage (F < 8, or F > 8 with a tail); semantic: every window, every
+    argmax row. Returns (max offsets, argmax row mask)."""
+    h = parse(stream)
+    off, F, n = h["off"], h["F"], h["n"]
+    L = e - s
+    nall = -(-L // window) if L > 0 else 0
+    seg = off[s:e]
+    wm = np.array([int(seg[k * window:(k + 1) * window].max()) for k in range(nall)], dtype=np.uint64)
+    rows = np.arange(n)
+    inr = (rows >= s) & (rows < e)
+    widx = np.where(inr, (rows - s) // window, 0)
+    arg = np.zeros(n, bool)
+    if nall:
+        arg[inr] = off[inr] == wm[widx[inr]]
+    if semantic:
+        return wm, arg
+    if F > 8 and (L == 0 or L % window):
+        raise Panic("assert_eq! / get_mut().unwrap()")
+    if F == 8 or (F > 8 and F % 8 == 0):
+        arg[:] = False
+    return wm[: L // window], arg
+
+
 def to_words(mask):
     """bool rows -> BitVec<u32> words."""
     n = len(mask)

@@ -207,6 +207,36 @@ def test_condition_queries_match_closed_form(oracle, name, x, scale):
                 oracle.decode_cond(s, scale, rows)
 
 
+@pytest.mark.parametrize("name,x,scale", list(_cases()) + [("F8", np.round(np.linspace(0, 7.9, 3001), 1), 10)])
+def test_max_range_matches_closed_form(oracle, name, x, scale):
+    # buff_max_range over windows that tile [s, e) exactly, that leave a partial
+    # last window (a panic when F > 8), one row, one window, and an empty range
+    s_ = oracle.encode(x, scale)
+    n = len(x)
+    F = cf.parse(s_)["F"]
+    rng = np.random.default_rng(17)
+    spans = [(0, n, n), (0, n, 1), (1, n, max(1, (n - 1) // 3)), (n // 4, n // 4 + 0, 5)]
+    for _ in range(6):
+        a = int(rng.integers(0, n))
+        b = int(rng.integers(a, n + 1))
+        w = int(rng.choice([1, 3, 64, 100, 1000, max(1, b - a)]))
+        spans.append((a, b, w))
+        if b > a:
+            spans.append((a, a + (b - a) // w * w, w))   # exact tiling
+    for a, b, w in spans:
+        try:
+            want_m, want_arg = cf.max_range(s_, a, b, w)
+        except cf.Panic:
+            with pytest.raises(oracle.RefPanic):
+                oracle.max_range(s_, a, b, w)
+            continue
+        offs, vals, words = oracle.max_range(s_, a, b, w)
+        assert np.array_equal(offs, want_m), (name, a, b, w, F)
+        assert np.array_equal(words, cf.to_words(want_arg)), (name, a, b, w, F)
+        h = cf.parse(s_)
+        assert np.array_equal(vals, (want_m.astype(np.int64) + h["base"]).astype(np.float64) / float(1 << h["dlen"]))
+
+
 def test_quirk_max_changes_width(oracle):
     # Q1: row 0 is the unique maximum; the `else if` loop never records it
     x = np.array([100.0, 1.0, 2.0, 3.0, 1.5])
