@@ -18,7 +18,8 @@ move) over its average duration against 8 TB/s. A cold-cache pass (the
 memory-side cache refilled with 1 GiB of other, clean data before each launch)
 is reported beside it. `filter_project` times the conditional filter and the
 gather of a filter's rows (buff_range_filter_condition, buff_decode_condition)
-on the walk column, cold.
+on the walk column, cold; `window_max` times buff_max_range over the walk
+column at two window sizes.
 
 Further sections of the same JSON line (each skippable):
   config3 — an outlier-heavy column of 67,108,864 rows in buff and buff-major:
@@ -222,6 +223,34 @@ def bench_filter_project(ctx, col, ch, info, words, out, flush, buff_amd):
         res["gather_%g%%" % (sel * 100)] = entry(us, b, {"rows": rows, "tiles_with_rows": tiles,
                                                          "GBps_f64_out": round(8.0 * rows / (us * 1e-6) / 1e9, 1)})
     cond.free()
+    return res
+
+
+def bench_window_max(ctx, col, ch, info, words, out, flush, buff_amd):
+    """buff_max_range on the whole walk column, each launch after a 1 GiB cache
+    eviction (HIP events, mean of 3), with the argmax bitmap (BUFF_SEMANTIC):
+    windows of 1000 and of 2^20 rows. Algorithmic bytes: every plane and tail
+    bit once, the argmax bitmap and 8 bytes per window maximum."""
+    n = info.n
+    res = {}
+    for w in (1000, 1 << 20):
+        ts = []
+        k = 0
+        for _ in range(3):
+            ctx.prof_reset()
+            flush.touch()
+            ctx.synchronize()
+            ctx.prof_enable(True)
+            k = ch.max_range_device(0, n, w, out.ptr, n, argmax_dev=words.ptr, flags=buff_amd.BUFF_SEMANTIC)
+            ctx.synchronize()
+            ctx.prof_enable(False)
+            launches, ms = ctx.prof_query("k_wmax")
+            ts.append(ms * 1e3 / max(launches, 1))
+        us = float(np.mean(ts))
+        b = n * (info.planes + info.tail_bits / 8.0) + n / 8.0 + 8.0 * k
+        g = b / (us * 1e-6) / 1e9
+        res["window_%d" % w] = {"windows": k, "avg_us": round(us, 2), "algorithmic_bytes": round(b),
+                                "achieved_GBps": round(g, 1), "frac": round(g / HBM_PEAK_GBPS, 4)}
     return res
 
 
@@ -699,6 +728,7 @@ def main():
                        "note": "10% filter" if k == "k_filter_gt" else ("full precision" if k == "k_materialize"
                                                                        else "")}
     filter_project = bench_filter_project(ctx, cols[0], keep[0], infos[0], words, out, flush, buff_amd)
+    window_max = bench_window_max(ctx, cols[0], keep[0], infos[0], words, out, flush, buff_amd)
     for ch in keep:
         ch.free()
 
@@ -726,6 +756,7 @@ def main():
         "cold_cache": cold,
         "cold_filters": filters,
         "filter_project": filter_project,
+        "window_max": window_max,
     }
 
     if not args.no_config3:

@@ -162,16 +162,11 @@ __global__ __launch_bounds__(64) void k_cond_index(const uint16_t* __restrict__ 
 // F >= 8 and r > 0 the tail field is OR-ed in shifted left by r, as the
 // reference does (buff_simd.rs:1693, 1701); otherwise in place.
 template <int NP>
-__device__ __forceinline__ uint64_t cond_offset(const uint4 (&w)[NP > 0 ? NP : 1], const uint32_t (&h)[8], uint32_t r,
-                                                uint32_t b, bool compat) {
-  uint64_t v = 0;
-#pragma unroll
-  for (int p = 0; p < NP; ++p) v = (v << 8) | byte_at(w[p], b);
-  uint32_t t = 0;
-#pragma unroll
-  for (uint32_t j = 0; j < 8; ++j) t |= ((h[j] >> b) & 1u) << j;   // h[j] = 0 for j >= r
-  if (NP == 0) return t;
-  return compat ? (v | t) << r : (v << r) | t;
+__device__ __forceinline__ uint64_t cond_offset(const RowBytes<NP>& rb, uint32_t r, uint32_t b, bool compat) {
+  const uint64_t X = row_bytes_x<NP>(rb, b);
+  if (NP == 0 || !compat) return X >> (8u - r);
+  const uint64_t v = X >> 8, t = (X & 0xFFu) >> (8u - r);
+  return (v | t) << r;
 }
 
 // Waves take the compacted tiles in turn, so clustered condition rows spread
@@ -191,19 +186,15 @@ __global__ __launch_bounds__(64) void k_cond_gather(ChunkView cv, double inv_sca
     const unsigned long long o = cmem(row_off)[i];
     const uint32_t live = cond_lane(cond, cv.n, c, lane);
     const uint32_t src = refine_src(live, lane);
-    uint4 w[NP > 0 ? NP : 1];
-#pragma unroll
-    for (int p = 0; p < NP; ++p) w[p] = load_plane(cv, p, c, src);
-    uint32_t h[8];
-#pragma unroll
-    for (uint32_t j = 0; j < 8; ++j) h[j] = j < cv.r ? load_tail(cv, j, c, src) : 0u;
+    RowBytes<NP> rb;
+    load_row_bytes<NP>(cv, c, src, rb);
     const uint32_t cntl = (uint32_t)__popc(live);
     uint32_t pos = wave_excl_scan_u32(cntl, lane);
     const uint32_t tot = wave_total_u32(cntl);
 #pragma unroll
     for (uint32_t b = 0; b < 16; ++b) {
       if ((live >> b) & 1u) {
-        const uint64_t off = cond_offset<NP>(w, h, cv.r, b, compat != 0);
+        const uint64_t off = cond_offset<NP>(rb, cv.r, b, compat != 0);
         sv[pos++] = (double)(int64_t)((uint64_t)cv.base + off) * inv_scale;
       }
     }

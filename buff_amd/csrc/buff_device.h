@@ -218,6 +218,65 @@ __device__ __forceinline__ void store_tile_planes(const uint64_t (&off)[16], uin
   }
 }
 
+// The lane's 16 rows of tile c as whole offsets (planes then r tail bits),
+// assembled with byte permutes: the tail bit-planes are first spread into the
+// plane layout as one byte per row holding the bits at its top (bit j of the
+// tail at bit 8 - r + j), so row b's NP plane bytes and that byte form one
+// integer X = planes * 2^8 + tail * 2^(8 - r), and the offset is X >> (8 - r).
+// `src`: the lane whose bytes are loaded (a lane's own, or another live lane's).
+template <int NP>
+struct RowBytes {
+  uint4 w[NP > 0 ? NP : 1];
+  uint32_t T[4];   // tail byte of row 4k + i in byte k of T[i]
+};
+
+template <int NP>
+__device__ __forceinline__ void load_row_bytes(const ChunkView& cv, uint64_t c, uint32_t src, RowBytes<NP>& rb) {
+#pragma unroll
+  for (int p = 0; p < NP; ++p) rb.w[p] = load_plane(cv, p, c, src);
+  const uint32_t r = cv.r;
+  uint32_t h[7];
+#pragma unroll
+  for (uint32_t j = 0; j < 7; ++j) h[j] = j < r ? load_tail(cv, j, c, src) : 0u;
+  rb.T[0] = rb.T[1] = rb.T[2] = rb.T[3] = 0u;
+#pragma unroll
+  for (uint32_t j = 0; j < 7; ++j) {
+    if (j >= r) break;
+    const uint32_t x = h[j];
+    // nibble k of the 16 row bits -> byte k
+    const uint32_t y = (x & 0xFu) | ((x << 4) & 0xF00u) | ((x << 8) & 0xF0000u) | ((x << 12) & 0xF000000u);
+    const uint32_t sh = 8u - r + j;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) rb.T[i] |= ((y >> i) & 0x01010101u) << sh;
+  }
+}
+
+// X of row b (see above): byte 0 the tail byte, bytes 1.. the planes from the last.
+template <int NP>
+__device__ __forceinline__ uint64_t row_bytes_x(const RowBytes<NP>& rb, uint32_t b) {
+  const uint32_t i = b & 3, k = b >> 2;
+  const uint32_t sel = k | ((k + 4) << 8) | 0x0C0C0000u;
+  auto byte_src = [&](int m) -> uint32_t {   // source dword of byte m of X
+    if (m == 0) return i == 0 ? rb.T[0] : i == 1 ? rb.T[1] : i == 2 ? rb.T[2] : rb.T[3];
+    const int p = NP - m;
+    if (p < 0) return 0u;
+    const uint4& q = rb.w[p < NP ? p : 0];
+    return i == 0 ? q.x : i == 1 ? q.y : i == 2 ? q.z : q.w;
+  };
+  uint64_t X = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (2 * q > NP) break;
+    X |= (uint64_t)__builtin_amdgcn_perm(byte_src(2 * q + 1), byte_src(2 * q), sel) << (16 * q);
+  }
+  return X;
+}
+
+template <int NP>
+__device__ __forceinline__ uint64_t row_offset(const RowBytes<NP>& rb, uint32_t b, uint32_t r) {
+  return row_bytes_x<NP>(rb, b) >> (8u - r);
+}
+
 // valid-row mask of the lane's 16 rows
 __device__ __forceinline__ uint32_t lane_valid16(uint64_t n, uint64_t c, uint32_t lane) {
   const uint64_t r0 = c * kTileRows + 16ull * lane;

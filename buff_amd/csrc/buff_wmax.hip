@@ -6,7 +6,8 @@
 // window's max offset. Here every tile of [s, e) decodes its rows' offsets once
 // (planes + tail), parks them in LDS and reduces every window segment it holds:
 //   k_wmax_tiles   windows that lie inside one tile are final; every tile also
-//                  emits (window, max) for its first and last window segment;
+//                  emits (window, max) for its first and last window segment
+//                  (k_wmax_tiles_wide: windows of a tile or more, in registers);
 //   k_wmax_reduce  the emitted pairs are sorted by window: a reduce-by-key per
 //                  block of 256 pairs writes the windows that end inside the
 //                  block and passes on the two boundary runs, until one block
@@ -24,22 +25,10 @@ namespace buffgpu {
 // Offsets of the lane's 16 rows of tile c: NP planes, then r tail bits.
 template <int NP>
 __device__ __forceinline__ void lane_offsets(const ChunkView& cv, uint64_t c, uint32_t lane, uint64_t (&off)[16]) {
-  uint4 w[NP > 0 ? NP : 1];
+  RowBytes<NP> rb;
+  load_row_bytes<NP>(cv, c, lane, rb);
 #pragma unroll
-  for (int p = 0; p < NP; ++p) w[p] = load_plane(cv, p, c, lane);
-  uint32_t h[8];
-#pragma unroll
-  for (uint32_t j = 0; j < 8; ++j) h[j] = j < cv.r ? load_tail(cv, j, c, lane) : 0u;
-#pragma unroll
-  for (uint32_t b = 0; b < 16; ++b) {
-    uint64_t v = 0;
-#pragma unroll
-    for (int p = 0; p < NP; ++p) v = (v << 8) | byte_at(w[p], b);
-    uint32_t t = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < 8; ++j) t |= ((h[j] >> b) & 1u) << j;
-    off[b] = NP > 0 ? (v << cv.r) | t : (uint64_t)t;
-  }
+  for (uint32_t b = 0; b < 16; ++b) off[b] = row_offset<NP>(rb, b, cv.r);
 }
 
 // One wave per tile of [s, e): tiles t_first + blockIdx.x.
@@ -103,6 +92,49 @@ __global__ __launch_bounds__(64) void k_wmax_tiles(ChunkView cv, uint64_t s, uin
   }
 }
 
+// Windows of at least one tile (W >= 1024): a tile holds at most two window
+// segments, so each lane keeps its rows' max for both and the wave reduces
+// them; no LDS. One wave per tile, four per workgroup.
+template <int NP>
+__global__ __launch_bounds__(256) void k_wmax_tiles_wide(ChunkView cv, uint64_t s, uint64_t e, uint64_t W,
+                                                         uint64_t t_first, uint64_t ntiles,
+                                                         unsigned long long* __restrict__ wmax,
+                                                         uint32_t* __restrict__ ekey,
+                                                         unsigned long long* __restrict__ eval) {
+  const uint32_t lane = __lane_id();
+  const uint64_t k = wave_global();
+  if (k >= ntiles) return;
+  const uint64_t c = t_first + k;
+  const uint64_t r0 = c * kTileRows, lr0 = r0 + 16ull * lane;
+  uint64_t off[16];
+  lane_offsets<NP>(cv, c, lane, off);
+  const uint64_t a = (s > r0 ? s : r0), b = (e < r0 + kTileRows ? e : r0 + kTileRows);   // rows [a, b)
+  const uint64_t fw = (a - s) / W, lw = (b - 1 - s) / W;
+  const uint64_t split = s + (fw + 1) * W;   // first row of window fw + 1
+  uint64_t m0 = 0, m1 = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 16; ++j) {
+    const uint64_t row = lr0 + j;
+    if (row < a || row >= b) continue;
+    if (row < split) m0 = max(m0, off[j]);
+    else m1 = max(m1, off[j]);
+  }
+  m0 = wave_max_u64(m0);
+  m1 = wave_max_u64(m1);
+  if (lane == 0) {
+    const uint64_t ws0 = s + fw * W, we0 = ws0 + W < e ? ws0 + W : e;
+    if (ws0 >= r0 && we0 <= r0 + kTileRows) wmax[fw] = m0;
+    ekey[2 * k] = (uint32_t)fw;
+    eval[2 * k] = m0;
+    ekey[2 * k + 1] = (uint32_t)lw;
+    eval[2 * k + 1] = lw == fw ? m0 : m1;
+    if (lw != fw) {
+      const uint64_t ws1 = s + lw * W, we1 = ws1 + W < e ? ws1 + W : e;
+      if (ws1 >= r0 && we1 <= r0 + kTileRows) wmax[lw] = m1;
+    }
+  }
+}
+
 // Reduce-by-key over pairs sorted by window (one block of 256 pairs per
 // workgroup). A run of equal windows that touches neither end of its block is
 // complete: its max is written. The runs at the block's ends go to the next
@@ -155,7 +187,9 @@ __global__ __launch_bounds__(256) void k_wmax_argmax(ChunkView cv, uint64_t s, u
                                                      const unsigned long long* __restrict__ wmax,
                                                      uint16_t* __restrict__ out16, uint64_t nout16) {
   const uint32_t lane = __lane_id();
-  for (uint64_t c = wave_global(); c < cv.ntiles; c += waves_in_grid()) {
+  {
+    const uint64_t c = wave_global();   // one tile per wave
+    if (c >= cv.ntiles) return;
     const uint64_t idx = c * 64 + lane;
     const uint64_t r0 = c * kTileRows, lr0 = r0 + 16ull * lane;
     uint32_t bits = 0;
@@ -224,7 +258,11 @@ static hipError_t launch_wmax_np(const ChunkView& cv, uint64_t s, uint64_t e, ui
   uint32_t* kb[2] = {reinterpret_cast<uint32_t*>(scratch + L.k[0]), reinterpret_cast<uint32_t*>(scratch + L.k[1])};
   unsigned long long* vb[2] = {reinterpret_cast<unsigned long long*>(scratch + L.v[0]),
                                reinterpret_cast<unsigned long long*>(scratch + L.v[1])};
-  launch_k(k_wmax_tiles<NP>, dim3((uint32_t)L.ntiles), dim3(64), 0, st, cv, s, e, W, L.t_first, wmax, kb[0], vb[0]);
+  if (W >= kTileRows)
+    launch_k(k_wmax_tiles_wide<NP>, dim3((uint32_t)((L.ntiles + 3) / 4)), dim3(256), 0, st, cv, s, e, W, L.t_first,
+             L.ntiles, wmax, kb[0], vb[0]);
+  else
+    launch_k(k_wmax_tiles<NP>, dim3((uint32_t)L.ntiles), dim3(64), 0, st, cv, s, e, W, L.t_first, wmax, kb[0], vb[0]);
   uint64_t nin = 2 * L.ntiles;
   uint32_t cur = 0;
   while (true) {
@@ -261,7 +299,7 @@ hipError_t launch_wmax_argmax(const ChunkView& cv, uint64_t s, uint64_t e, uint6
   const unsigned long long* wmax = reinterpret_cast<const unsigned long long*>(scratch + L.wmax);
   uint16_t* o16 = reinterpret_cast<uint16_t*>(out);
   const uint64_t nout16 = 4 * ((cv.n + 63) / 64);
-  const uint32_t wg = (uint32_t)std::min<uint64_t>((cv.ntiles + 3) / 4, 2048);
+  const uint32_t wg = (uint32_t)((cv.ntiles + 3) / 4);
   const uint32_t w = want && L.nwin ? 1u : 0u;
 #define BUFF_ARG(NP) launch_k(k_wmax_argmax<NP>, dim3(wg), dim3(256), 0, st, cv, s, e, W, w, wmax, o16, nout16)
   switch (cv.P) {
