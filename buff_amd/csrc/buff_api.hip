@@ -721,10 +721,23 @@ int max_range_device(Ctx* ctx, const Chunk* ch, uint64_t s, uint64_t e, uint64_t
   if (W == 0 || s > e || e > ch->n)
     return fail(BUFF_E_INVALID, "window %llu over rows [%llu, %llu) of %llu", (unsigned long long)W,
                 (unsigned long long)s, (unsigned long long)e, (unsigned long long)ch->n);
-  if (ch->flags & BUFF_MAJORITY) return fail(BUFF_E_UNSUPPORTED, "windowed max takes byte_fixed chunks");
   const bool semantic = (flags & BUFF_SEMANTIC) != 0;
   const ChunkView& v = ch->view;
   const uint64_t L = e - s;
+  if (ch->flags & BUFF_MAJORITY) {
+    // buff_max_majority_range (buff_simd.rs:2863-3114): with sparse columns it
+    // logs an error and finds no window (2894-2896); without, it is
+    // buff_max_range after the column-count byte, and the chunk's planes are the
+    // byte_fixed layout.
+    if (ch->F < 8) return fail(BUFF_E_UNSUPPORTED, "buff-major chunk with F < 8: its column-count byte holds tail bits");
+    if (ch->S > 0) {
+      if (semantic) return fail(BUFF_E_UNSUPPORTED, "windowed max of buff-major chunks with sparse columns");
+      *nwin = 0;
+      const uint64_t nwords = (ch->n + 63) / 64;
+      if (argmax && nwords) HIP_TRY(launch_fill_bitmap(argmax, nwords, ch->n, 0, ctx->stream));
+      return BUFF_OK;
+    }
+  }
   if (!semantic && v.F > 8 && (L == 0 || L % W))
     return fail(BUFF_E_REF_PANIC, "the reference asserts that windows of %llu rows tile %llu rows",
                 (unsigned long long)W, (unsigned long long)L);

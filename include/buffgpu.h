@@ -188,9 +188,12 @@ int buff_materialize_cond_device(buff_ctx* ctx, const buff_chunk* chunk, uint32_
  * its tail stage runs (F < 8, or F > 8 with F % 8 != 0), and for F < 8 they
  * include the rows of a partial last window. BUFF_SEMANTIC: every window,
  * including a partial last one, and the argmax rows in every case. window = 0
- * or a range outside [0, n] is BUFF_E_INVALID; byte_fixed chunks only. The
- * device form writes values and ceil(n/64) u64 argmax words to device memory
- * (argmax_dev nullable). */
+ * or a range outside [0, n] is BUFF_E_INVALID. A buff-major chunk is
+ * buff_max_majority_range (buff_simd.rs:2863-3114): with sparse columns no
+ * window and no argmax row (its log-and-return; BUFF_SEMANTIC there and F < 8
+ * are BUFF_E_UNSUPPORTED), without them the byte_fixed result. The device form
+ * writes values and ceil(n/64) u64 argmax words to device memory (argmax_dev
+ * nullable). */
 int buff_max_range(buff_ctx* ctx, const buff_chunk* chunk, uint64_t s, uint64_t e, uint64_t window, uint32_t flags,
                    double* max_values, uint64_t cap, uint64_t* nwindows, uint32_t* argmax_words, uint64_t nwords);
 int buff_max_range_device(buff_ctx* ctx, const buff_chunk* chunk, uint64_t s, uint64_t e, uint64_t window,

@@ -1430,9 +1430,13 @@ struct MaxRangeOut {
   std::vector<uint64_t> maxv;
   std::set<uint32_t> res;
 };
-MaxRangeOut buff_max_range(const uint8_t* bytes, size_t blen, uint32_t s, uint32_t e, uint32_t window) {
+// major: buff_max_majority_range (buff_simd.rs:2863-3114), the same after the
+// column-count byte, and nothing when there are sparse columns (2894-2896).
+MaxRangeOut buff_max_range(const uint8_t* bytes, size_t blen, uint32_t s, uint32_t e, uint32_t window,
+                           bool major = false) {
   BitPackReader bp(bytes, blen);
   Header h = read_header(bp);
+  if (major && bp.read(8) > 0) return MaxRangeOut{};
   const uint32_t len = h.len;
   uint32_t remain = h.dlen + h.ilen;
   std::set<uint32_t> rb1;
@@ -2423,12 +2427,12 @@ int ref_max(const uint8_t* s, uint64_t len, uint64_t scale, double* mx, uint64_t
   } catch (const RefPanic&) { return REF_PANIC; }
 }
 
-// buff_max_range: the window maxima (stored offsets, and as the reference's
+// buff_max_range (major: buff_max_majority_range): the window maxima (stored offsets, and as the reference's
 // f64 `(x as i64 + base) / 2^dlen`) and the argmax rows as BitVec<u32> words.
-int ref_max_range(const uint8_t* s, uint64_t len, uint32_t rs, uint32_t re, uint32_t window, uint64_t* offs,
+int ref_max_range(const uint8_t* s, uint64_t len, uint32_t rs, uint32_t re, uint32_t window, int major, uint64_t* offs,
                   double* vals, uint64_t cap, uint64_t* nwin, uint32_t* words, uint64_t nwords_cap) {
   try {
-    MaxRangeOut o = buff_max_range(s, len, rs, re, window);
+    MaxRangeOut o = buff_max_range(s, len, rs, re, window, major != 0);
     BitPackReader bp(s, len);
     Header h = read_header(bp);
     *nwin = o.maxv.size();

@@ -66,7 +66,7 @@ def lib():
             "ref_decode_cond": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, u64p, ctypes.c_uint64,
                                                ctypes.c_int, f64p, ctypes.c_uint64, u64p]),
             "ref_max_range": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
-                                             u64p, f64p, ctypes.c_uint64, u64p, u32p, ctypes.c_uint64]),
+                                             ctypes.c_int, u64p, f64p, ctypes.c_uint64, u64p, u32p, ctypes.c_uint64]),
             "ref_simd_filter": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
                                                ctypes.c_double, u32p, ctypes.c_uint64, u64p]),
             "ref_sum": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, f64p, u64p]),
@@ -225,9 +225,9 @@ def decode_cond(stream, scale, rows, major=False):
     return out[: got.value]
 
 
-def max_range(stream, s, e, window):
-    """buff_max_range over rows [s, e) in windows of `window` rows: (window max offsets, their f64
-    values, argmax u32 words)."""
+def max_range(stream, s, e, window, major=False):
+    """buff_max_range (major: buff_max_majority_range) over rows [s, e) in windows of `window` rows:
+    (window max offsets, their f64 values, argmax u32 words)."""
     st = np.ascontiguousarray(stream, dtype=np.uint8)
     n = header(st)[1]
     cap = max((e - s) // max(window, 1) + 2, 1) if e >= s else 1
@@ -235,7 +235,7 @@ def max_range(stream, s, e, window):
     vals = np.zeros(cap, dtype=np.float64)
     nwin = ctypes.c_uint64()
     words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
-    _check(lib().ref_max_range(_ptr(st, ctypes.c_uint8), len(st), s, e, window, _ptr(offs, ctypes.c_uint64),
+    _check(lib().ref_max_range(_ptr(st, ctypes.c_uint8), len(st), s, e, window, int(major), _ptr(offs, ctypes.c_uint64),
                                _ptr(vals, ctypes.c_double), cap, ctypes.byref(nwin), _ptr(words, ctypes.c_uint32),
                                len(words)))
     k = nwin.value

@@ -91,3 +91,33 @@ def test_max_range_at_scale(gpu_ctx, window):
         assert np.array_equal(words, cf.to_words(arg))
     finally:
         ch.free()
+
+
+@pytest.mark.parametrize("case", ["config3", "no_majority", "all_sparse", "no_outliers"])
+def test_majority_form_matches_oracle(gpu_ctx, oracle, case):
+    # buff_max_majority_range: no window and no argmax row when the chunk has
+    # sparse columns, the byte_fixed result when it has none
+    from tests import majority_cases as mc
+
+    build, scale = mc.CASES[case]
+    x = build()
+    s = oracle.major_encode(x, scale)
+    ch = gpu_ctx.encode(x, scale, flags=N.BUFF_MAJORITY)
+    try:
+        sparse = ch.info().sparse_cols > 0
+        for a, b, w in _spans(len(x), 3 + len(x)):
+            try:
+                offs, vals, words = oracle.max_range(s, a, b, w, major=True)
+            except oracle.RefPanic:
+                with pytest.raises(N.RefPanic):
+                    ch.max_range(a, b, w)
+                continue
+            gv, gw = ch.max_range(a, b, w)
+            assert np.array_equal(gv.view(np.uint64), vals.view(np.uint64)), (case, a, b, w)
+            assert np.array_equal(gw, words), (case, a, b, w)
+            if sparse:
+                assert len(gv) == 0 and not gw.any()
+                with pytest.raises(N.Unsupported):
+                    ch.max_range(a, b, w, flags=N.BUFF_SEMANTIC)
+    finally:
+        ch.free()

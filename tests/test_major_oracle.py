@@ -206,3 +206,25 @@ def test_condition_forms(case):
         got = R.decode_cond(s, scale, rows, major=True)
         want = np.zeros(0) if plain is None else cf.decode_cond(plain, cond, semantic=True)
         assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), (name, frac)
+
+
+def test_max_range_majority(case):
+    """buff_max_majority_range (buff_simd.rs:2863-3114): nothing with sparse
+    columns; without, buff_max_range on the stream without its column-count byte."""
+    name, x, scale, s, h = case
+    n = h["n"]
+    plain = np.concatenate([s[:20], s[21:]]) if h["S"] == 0 else None
+    for a, b, w in ((0, n, n), (0, n - n % 100, 100), (5, 5 + 1000 * 7, 7), (3, n, 1000)):
+        if plain is None:
+            offs, vals, words = R.max_range(s, a, b, w, major=True)
+            assert len(offs) == 0 and not words.any(), name
+            continue
+        try:
+            want = R.max_range(plain, a, b, w)
+        except R.RefPanic:
+            with pytest.raises(R.RefPanic):
+                R.max_range(s, a, b, w, major=True)
+            continue
+        got = R.max_range(s, a, b, w, major=True)
+        for g, wv in zip(got, want):
+            assert np.array_equal(g, wv), (name, a, b, w)
