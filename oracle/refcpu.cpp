@@ -18,6 +18,7 @@
 
 #include <immintrin.h>
 
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -2302,6 +2303,8 @@ double buff_max_majority(const uint8_t* bytes, size_t blen, uint64_t scale, uint
   return (double)(int64_t)(max + (uint64_t)h.base) / powi(2.0, (int32_t)h.dlen);
 }
 
+#include "refslice.inc"
+
 }  // namespace
 
 // ===========================================================================
@@ -2531,6 +2534,73 @@ int ref_snappy_decompress(const uint8_t* in, uint64_t n, uint8_t* out, uint64_t 
     if (!v.empty()) std::memcpy(out, v.data(), v.size());
     return REF_OK;
   } catch (const snappy_ref::Error&) { return REF_PANIC; }
+}
+
+// ---- buff-slice (buff_slice.rs:222-1970, buff_simd.rs:3736-4025) ----------
+int ref_slice_encode(const double* data, uint64_t n, uint64_t scale, uint8_t* out, uint64_t cap, uint64_t* out_len) {
+  try {
+    std::vector<uint8_t> v = buff_slice_encode(data, n, scale);
+    *out_len = v.size();
+    if (v.size() > cap) return REF_SMALL_BUFFER;
+    if (!v.empty()) std::memcpy(out, v.data(), v.size());
+    return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+// kind 0: buff_slice_decode, 1: buff_slice_decode_bs, 2: buff_slice_decode_condition (rows).
+int ref_slice_decode(const uint8_t* s, uint64_t len, int kind, const uint64_t* rows, uint64_t nrows, double* out,
+                     uint64_t cap, uint64_t* n_out) {
+  try {
+    std::vector<double> v = kind == 0 ? buff_slice_decode(s, len)
+                            : kind == 1 ? buff_slice_decode_bs(s, len)
+                                        : buff_slice_decode_condition(s, len, rows, nrows);
+    *n_out = v.size();
+    if (v.size() > cap) return REF_SMALL_BUFFER;
+    if (!v.empty()) std::memcpy(out, v.data(), v.size() * 8);
+    return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+// scalar 0: buff_slice_buffsum, 1: buff_slice_sum.
+int ref_slice_sum(const uint8_t* s, uint64_t len, int scalar, double* sum) {
+  try {
+    *sum = scalar ? buff_slice_sum(s, len) : buff_slice_buffsum(s, len);
+    return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+// scalar 0: buff_slice_buffmax, 1: buff_slice_max. words: ceil(n/32) u32.
+int ref_slice_max(const uint8_t* s, uint64_t len, int scalar, double* mx, uint64_t* count, uint32_t* words,
+                  uint64_t nwords_cap) {
+  try {
+    SliceMaxOut o = scalar ? buff_slice_max(s, len) : buff_slice_buffmax(s, len);
+    *mx = o.max;
+    *count = o.res.cardinality();
+    if (o.res.storage.size() > nwords_cap) return REF_SMALL_BUFFER;
+    if (!o.res.storage.empty()) std::memcpy(words, o.res.storage.data(), o.res.storage.size() * 4);
+    return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+int ref_slice_max_range(const uint8_t* s, uint64_t len, uint32_t rs, uint32_t re, uint32_t window, double* vals,
+                        uint64_t cap, uint64_t* nwin, uint32_t* words, uint64_t nwords_cap) {
+  try {
+    SliceMaxRangeOut o = buff_slice_max_range(s, len, rs, re, window);
+    *nwin = o.max_vec.size();
+    if (o.max_vec.size() > cap || o.res.storage.size() > nwords_cap) return REF_SMALL_BUFFER;
+    if (!o.max_vec.empty()) std::memcpy(vals, o.max_vec.data(), o.max_vec.size() * 8);
+    if (!o.res.storage.empty()) std::memcpy(words, o.res.storage.data(), o.res.storage.size() * 4);
+    return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+// kind: SliceFilterKind; rows/nrows: the condition of kinds 4 and 5.
+int ref_slice_filter(const uint8_t* s, uint64_t len, uint64_t scale, double pred, int kind, const uint64_t* rows,
+                     uint64_t nrows, uint32_t* words, uint64_t nwords_cap, uint64_t* count) {
+  try {
+    return copy_filter(buff_slice_filter(s, len, scale, pred, kind, rows, nrows), words, nwords_cap, count,
+                       nullptr);
+  } catch (const RefPanic&) { return REF_PANIC; }
 }
 
 }  // extern "C"

@@ -80,6 +80,16 @@ def lib():
                                                 u32p, ctypes.c_uint64, u64p]),
             "ref_major_sum": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, f64p, u64p]),
             "ref_major_max": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, f64p, u64p, u64p]),
+            "ref_slice_encode": (ctypes.c_int, [f64p, ctypes.c_uint64, ctypes.c_uint64, u8p, ctypes.c_uint64, u64p]),
+            "ref_slice_decode": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_int, u64p, ctypes.c_uint64, f64p,
+                                                ctypes.c_uint64, u64p]),
+            "ref_slice_sum": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_int, f64p]),
+            "ref_slice_max": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_int, f64p, u64p, u32p, ctypes.c_uint64]),
+            "ref_slice_max_range": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                                   ctypes.c_uint32, f64p, ctypes.c_uint64, u64p, u32p,
+                                                   ctypes.c_uint64]),
+            "ref_slice_filter": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
+                                                u64p, ctypes.c_uint64, u32p, ctypes.c_uint64, u64p]),
             "ref_snappy_compress": (ctypes.c_int, [u8p, ctypes.c_uint64, u8p, ctypes.c_uint64, u64p]),
             "ref_snappy_decompress": (ctypes.c_int, [u8p, ctypes.c_uint64, u8p, ctypes.c_uint64, u64p]),
         }
@@ -358,3 +368,82 @@ def snappy_decompress(data, size):
     _check(lib().ref_snappy_decompress(_ptr(d, ctypes.c_uint8), len(d), _ptr(out, ctypes.c_uint8), len(out),
                                        ctypes.byref(n)))
     return out[: n.value].tobytes()
+
+
+# ---- buff-slice (buff_slice.rs:222-1970; buff_simd.rs:3736-4025) -------------
+
+SLICE_KINDS = {"range": 0, "equal": 1, "range_nosimd": 2, "equal_nosimd": 3, "range_cond": 4, "smaller_cond": 5,
+               "range_with_slice": 6, "equal_with_slice": 7}
+
+
+def slice_encode(x, scale):
+    """buff_slice_encode: the byte-slice stream."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    cap = 20 + len(x) * 8 + 8
+    out = np.zeros(cap, dtype=np.uint8)
+    ln = ctypes.c_uint64()
+    _check(lib().ref_slice_encode(_ptr(x, ctypes.c_double), len(x), scale, _ptr(out, ctypes.c_uint8), cap,
+                                  ctypes.byref(ln)))
+    return out[: ln.value].copy()
+
+
+def slice_decode(stream, kind="buff", rows=None):
+    """buff_slice_decode (kind "buff"), buff_slice_decode_bs ("bs") or
+    buff_slice_decode_condition ("cond", ascending rows)."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    n = header(s)[1]
+    r = np.ascontiguousarray(rows if rows is not None else [], dtype=np.uint64)
+    cap = max(len(r) if kind == "cond" else n, 1)
+    out = np.zeros(cap, dtype=np.float64)
+    got = ctypes.c_uint64()
+    _check(lib().ref_slice_decode(_ptr(s, ctypes.c_uint8), len(s), {"buff": 0, "bs": 1, "cond": 2}[kind],
+                                  _ptr(r, ctypes.c_uint64), len(r), _ptr(out, ctypes.c_double), cap,
+                                  ctypes.byref(got)))
+    return out[: got.value]
+
+
+def slice_sum(stream, scalar=False):
+    """buff_slice_buffsum (scalar: buff_slice_sum): the f64 running sum."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    out = ctypes.c_double()
+    _check(lib().ref_slice_sum(_ptr(s, ctypes.c_uint8), len(s), int(scalar), ctypes.byref(out)))
+    return out.value
+
+
+def slice_max(stream, scalar=False):
+    """buff_slice_buffmax (scalar: buff_slice_max): (max value, argmax count, argmax u32 words)."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    n = header(s)[1]
+    words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
+    mx = ctypes.c_double()
+    cnt = ctypes.c_uint64()
+    _check(lib().ref_slice_max(_ptr(s, ctypes.c_uint8), len(s), int(scalar), ctypes.byref(mx), ctypes.byref(cnt),
+                               _ptr(words, ctypes.c_uint32), len(words)))
+    return mx.value, cnt.value, words[: (n + 31) // 32]
+
+
+def slice_max_range(stream, s, e, window):
+    """buff_slice_max_range: (window max values, argmax u32 words)."""
+    st = np.ascontiguousarray(stream, dtype=np.uint8)
+    n = header(st)[1]
+    cap = max((e - s) // max(window, 1) + 2, 1) if e >= s else 1
+    vals = np.zeros(cap, dtype=np.float64)
+    nwin = ctypes.c_uint64()
+    words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
+    _check(lib().ref_slice_max_range(_ptr(st, ctypes.c_uint8), len(st), s, e, window, _ptr(vals, ctypes.c_double),
+                                     cap, ctypes.byref(nwin), _ptr(words, ctypes.c_uint32), len(words)))
+    return vals[: nwin.value], words[: (n + 31) // 32]
+
+
+def slice_filter(stream, scale, kind, pred, rows=None):
+    """One of the byte-slice filters (SLICE_KINDS; the *_with_slice kinds read a
+    buff_simd256 stream): (u32 words, printed count)."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    n = header(s)[1]
+    r = np.ascontiguousarray(rows if rows is not None else [], dtype=np.uint64)
+    words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
+    cnt = ctypes.c_uint64()
+    _check(lib().ref_slice_filter(_ptr(s, ctypes.c_uint8), len(s), scale, float(pred), SLICE_KINDS[kind],
+                                  _ptr(r, ctypes.c_uint64), len(r), _ptr(words, ctypes.c_uint32), len(words),
+                                  ctypes.byref(cnt)))
+    return words[: (n + 31) // 32], cnt.value

@@ -546,3 +546,162 @@ def major_max(h):
     v = (mx + h["base"]) & ((1 << 64) - 1)
     v = v - (1 << 64) if v >= 1 << 63 else v
     return float(np.float64(np.int64(v))) / 2.0 ** h["dlen"], mx, argmax
+
+
+# ---- byte-slice (buff_slice.rs:222-1970) --------------------------------------
+# The stream is byte_fixed's 20-byte header followed by ceil(F/8) planes (one
+# plane when F == 0) of the F-bit offset shifted left to a whole number of
+# bytes, MSB plane first, every byte XOR 0x80. A filter compares the top
+# min(planes, 4) bytes of that left-aligned offset with the same bytes of the
+# target taken modulo 2^F (only those planes are read); the AVX2 forms cover
+# the first 32*floor(n/32) rows.
+
+def slice_planes(F):
+    return max(1, -(-F // 8))
+
+
+def slice_encode(x, scale):
+    """Independent byte-slice encoder (uint8 stream)."""
+    _, dlen = dlen_of_scale(scale)
+    f = fixed(x, dlen)
+    n = len(f)
+    mn, mx = quirk_minmax(f)
+    F = width(mn, mx)
+    off = (f.astype(np.uint64) - np.uint64(mn & ((1 << 64) - 1))) if n else np.zeros(0, np.uint64)
+    ns = slice_planes(F)
+    pad = 8 * ns - F
+    left = (off % np.uint64(1 << F) if F < 64 else off) << np.uint64(pad)
+    hdr = np.zeros(20, dtype=np.uint8)
+    hdr[0:8] = np.frombuffer(np.int64(mn).tobytes(), dtype=np.uint8)
+    hdr[8:12] = np.frombuffer(np.uint32(n).tobytes(), dtype=np.uint8)
+    hdr[12:16] = np.frombuffer(np.uint32((F - dlen) & 0xFFFFFFFF).tobytes(), dtype=np.uint8)
+    hdr[16:20] = np.frombuffer(np.uint32(dlen).tobytes(), dtype=np.uint8)
+    parts = [hdr]
+    for k in range(ns):
+        parts.append(((left >> np.uint64(8 * (ns - 1 - k))) & np.uint64(0xFF)).astype(np.uint8) ^ np.uint8(0x80))
+    return np.concatenate(parts)
+
+
+def slice_parse(stream):
+    """Header fields and the F-bit offsets of a byte-slice stream."""
+    s = np.asarray(stream, dtype=np.uint8)
+    base = int(np.frombuffer(s[0:8].tobytes(), dtype=np.int64)[0])
+    n = int(np.frombuffer(s[8:12].tobytes(), dtype=np.uint32)[0])
+    ilen = int(np.frombuffer(s[12:16].tobytes(), dtype=np.uint32)[0])
+    dlen = int(np.frombuffer(s[16:20].tobytes(), dtype=np.uint32)[0])
+    F = (ilen + dlen) & 0xFFFFFFFF
+    ns = slice_planes(F)
+    left = np.zeros(n, dtype=np.uint64)
+    for k in range(ns):
+        b = s[20 + k * n: 20 + (k + 1) * n] ^ np.uint8(0x80)
+        left = (left << np.uint64(8)) | b.astype(np.uint64)
+    off = left >> np.uint64(8 * ns - F)
+    return dict(base=base, n=n, ilen=ilen, dlen=dlen, F=F, ns=ns, off=off)
+
+
+def _values(h):
+    v = (np.uint64(h["base"] & ((1 << 64) - 1)) + h["off"]).view(np.int64)
+    return v.astype(np.float64) / 2.0 ** h["dlen"]
+
+
+def slice_decode(stream, kind="buff", rows=None):
+    """buff_slice_decode; "bs" / "cond" (the four-plane readers) panic unless 1 <= F <= 32."""
+    h = slice_parse(stream)
+    if kind != "buff" and not 1 <= h["F"] <= 32:
+        raise Panic("bit length greater than 32 is not supported yet.")
+    v = _values(h)
+    if kind == "cond":
+        rows = np.asarray(rows, dtype=np.int64)
+        if len(rows) and rows.max() >= h["n"]:
+            raise Panic("chunk.get(elem).unwrap()")
+        return v[rows]
+    return v
+
+
+def slice_max(stream, scalar=False):
+    """(max value, argmax mask) of buff_slice_buffmax / buff_slice_max; the f64
+    values are exact here (|base + off| < 2^53)."""
+    h = slice_parse(stream)
+    if scalar and not 1 <= h["F"] <= 32:
+        raise Panic("bit length greater than 32 is not supported yet.")
+    v = _values(h)
+    if h["n"] == 0:
+        return -np.finfo(np.float64).max, np.zeros(0, bool)
+    return float(v.max()), v == v.max()
+
+
+def slice_max_range(stream, s, e, window):
+    """buff_slice_max_range: a panic unless 1 <= F <= 32 and the windows tile a
+    non-empty [s, e); then every window's max and the rows holding it."""
+    h = slice_parse(stream)
+    if not 1 <= h["F"] <= 32:
+        raise Panic("bit length greater than 32 is not supported yet.")
+    L = e - s
+    if window == 0 or L <= 0 or L % window:
+        raise Panic("assert_eq! on the window count")
+    v = _values(h)
+    seg = v[s:e].reshape(-1, window)
+    wm = seg.max(axis=1)
+    arg = np.zeros(h["n"], bool)
+    arg[s:e] = (seg == wm[:, None]).reshape(-1)
+    return wm, arg
+
+
+def slice_filter_bits(stream, scale, kind, pred, cond=None, semantic=False):
+    """The row mask of one byte-slice filter kind (oracle.refcpu.SLICE_KINDS);
+    the *_with_slice kinds take a buff_simd256 (flipped byte_fixed) stream.
+    semantic: the intended comparison on every row."""
+    with_slice = kind.endswith("with_slice")
+    h = parse(stream, flip=True) if with_slice else slice_parse(stream)
+    n, F, base = h["n"], h["F"], h["base"]
+    off = h["off"]
+    is_cond = kind.endswith("_cond")
+    keep = np.ones(n, bool) if cond is None else np.asarray(cond, bool)
+    fp = fixed_pred(pred, scale, h["dlen"])
+    if fp < base:
+        if semantic:
+            return (np.ones(n, bool) if kind.startswith("range") else np.zeros(n, bool)) & keep
+        return np.zeros(n, bool) if is_cond else np.ones(n, bool)
+    t = fp - base
+    if semantic:
+        if F < 64 and t >= 1 << F:
+            return np.zeros(n, bool) if not kind.startswith("smaller") else keep.copy()
+        tt = np.uint64(t)
+        res = off > tt if kind.startswith("range") else off < tt if kind.startswith("smaller") else off == tt
+        return res & keep
+    nosimd = kind.endswith("nosimd")
+    if F == 0 or (with_slice and F < 8):
+        if nosimd and n == 0:
+            return np.zeros(0, bool)
+        raise Panic("no slice to read")
+    if not nosimd and n < 32:
+        raise Panic("len - BYTE_WORD wraps")
+    tm = t % (1 << F)
+    if with_slice:
+        P, r = F // 8, F % 8
+        used = min(P, 4)
+        sh = F - 8 * used
+        hi_o, hi_t = off >> np.uint64(sh), np.uint64(tm >> sh)
+        lo_o, lo_t = off & np.uint64((1 << r) - 1), np.uint64(tm & ((1 << r) - 1))
+        if kind.startswith("range"):
+            res = (hi_o > hi_t) | ((hi_o == hi_t) & (lo_o > lo_t) if r else False)
+        else:
+            res = (hi_o == hi_t) & (lo_o == lo_t) if r else np.zeros(n, bool)
+    else:
+        ns = slice_planes(F)
+        used = min(ns, 4)
+        sh = 8 * ns - F        # left-align, then keep the top `used` bytes
+        left_o = off << np.uint64(sh)
+        left_t = tm << sh
+        drop = 8 * (ns - used)
+        a, b = left_o >> np.uint64(drop), np.uint64(left_t >> drop)
+        if kind.startswith("range"):
+            res = a > b
+        elif kind.startswith("smaller"):
+            res = a < b
+        else:
+            res = a == b
+    res = np.asarray(res, bool) & keep
+    if not nosimd:
+        res[32 * (n // 32):] = False
+    return res
