@@ -9,7 +9,11 @@ from ._native import (  # noqa: F401
     BUFF_MAJORITY,
     BUFF_OP_EQ,
     BUFF_OP_GT,
+    BUFF_OP_LT,
+    BUFF_SCALAR,
     BUFF_SEMANTIC,
+    BUFF_SLICE,
+    BUFF_WITH_SLICE,
     BuffError,
     Batch,
     Chunk,
@@ -21,9 +25,11 @@ from ._native import (  # noqa: F401
     Unsupported,
     lib,
 )
+from .buff_slice import BuffSliceCompress  # noqa: F401
 from .split_double import SplitBDDoubleCompress  # noqa: F401
 
 __all__ = [
-    "BUFF_FLIP", "BUFF_MAJORITY", "BUFF_OP_EQ", "BUFF_OP_GT", "BUFF_SEMANTIC", "Batch", "BuffError", "Chunk", "Context", "DeviceBuffer",
-    "OutOfMemory", "RefPanic", "Server", "Unsupported", "SplitBDDoubleCompress", "lib",
+    "BUFF_FLIP", "BUFF_MAJORITY", "BUFF_OP_EQ", "BUFF_OP_GT", "BUFF_OP_LT", "BUFF_SCALAR", "BUFF_SEMANTIC",
+    "BUFF_SLICE", "BUFF_WITH_SLICE", "Batch", "BuffError", "Chunk", "Context", "DeviceBuffer",
+    "OutOfMemory", "RefPanic", "Server", "Unsupported", "SplitBDDoubleCompress", "BuffSliceCompress", "lib",
 ]

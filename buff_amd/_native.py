@@ -23,8 +23,23 @@ BUFF_E_SMALL_BUFFER = 6
 BUFF_FLIP = 0x1
 BUFF_SEMANTIC = 0x2
 BUFF_MAJORITY = 0x4
+BUFF_SLICE = 0x8
+BUFF_SCALAR = 0x10
+BUFF_WITH_SLICE = 0x20
 BUFF_OP_GT = 0
 BUFF_OP_EQ = 1
+BUFF_OP_LT = 2
+
+
+def _opc(op):
+    """Filter op name -> BUFF_OP_* ("range"/"gt", "equal"/"eq", "smaller"/"lt")."""
+    if op in ("range", "gt", BUFF_OP_GT):
+        return BUFF_OP_GT
+    if op in ("equal", "eq", BUFF_OP_EQ):
+        return BUFF_OP_EQ
+    if op in ("smaller", "lt", BUFF_OP_LT):
+        return BUFF_OP_LT
+    raise ValueError("unknown filter op %r" % (op,))
 
 
 class BuffError(RuntimeError):
@@ -122,6 +137,7 @@ SIGNATURES = {
     "buff_filter": [vp, vp, cint, f64, u32, u32p, u64, u64p],
     "buff_filter_device": [vp, vp, cint, f64, u32, vp, vp],
     "buff_sum": [vp, vp, f64p, f64p, u64p],
+    "buff_sum_ex": [vp, vp, u32, f64p, f64p, u64p],
     "buff_max": [vp, vp, f64p, u64p],
     "buff_max_ex": [vp, vp, u32, f64p, u64p, u64p],
     "buff_filter_cond": [vp, vp, cint, f64, u32, u32p, u32p, u64, u64p],
@@ -149,6 +165,19 @@ SIGNATURES = {
     "buff_server_stats": [vp, u64p, u64p, u64p],
     "buff_server_destroy": [vp],
     "buff_sd_byte_fixed_encode": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
+    "buff_sd_buff_slice_encode": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
+    "buff_sd_buff_slice_decode": [P(SplitDouble), u8p, u64, f64p, u64],
+    "buff_sd_buff_slice_decode_bs": [P(SplitDouble), u8p, u64, f64p, u64],
+    "buff_sd_buff_slice_range_filter": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
+    "buff_sd_buff_slice_equal_filter": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
+    "buff_sd_buff_slice_range_filter_nosimd": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
+    "buff_sd_buff_slice_equal_filter_nosimd": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
+    "buff_sd_buff_simd_range_filter_with_slice": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
+    "buff_sd_buff_simd_equal_filter_with_slice": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
+    "buff_sd_buff_slice_buffsum": [P(SplitDouble), u8p, u64, f64p],
+    "buff_sd_buff_slice_sum": [P(SplitDouble), u8p, u64, f64p],
+    "buff_sd_buff_slice_buffmax": [P(SplitDouble), u8p, u64, f64p, u64p],
+    "buff_sd_buff_slice_max": [P(SplitDouble), u8p, u64, f64p, u64p],
     "buff_sd_buff_simd256_encode": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
     "buff_sd_byte_fixed_decode": [P(SplitDouble), u8p, u64, f64p, u64],
     "buff_sd_buff_simd256_decode": [P(SplitDouble), u8p, u64, f64p, u64],
@@ -229,13 +258,13 @@ class Chunk:
         n = self.n
         words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
         cnt = ctypes.c_uint64()
-        opc = BUFF_OP_GT if op in ("range", "gt", BUFF_OP_GT) else BUFF_OP_EQ
+        opc = _opc(op)
         check(lib().buff_filter(self.ctx.handle, self.handle, opc, float(pred), flags, ptr(words, ctypes.c_uint32),
                                 len(words), ctypes.byref(cnt)))
         return words[: (n + 31) // 32], cnt.value
 
     def filter_device(self, op, pred, words_dev, flags=0, stats_dev=None):
-        opc = BUFF_OP_GT if op in ("range", "gt", BUFF_OP_GT) else BUFF_OP_EQ
+        opc = _opc(op)
         check(lib().buff_filter_device(self.ctx.handle, self.handle, opc, float(pred), flags, words_dev, stats_dev))
 
     def filter_cond(self, op, pred, cond_words, flags=0):
@@ -247,13 +276,13 @@ class Chunk:
         cw[: len(c)] = c
         words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
         cnt = ctypes.c_uint64()
-        opc = BUFF_OP_GT if op in ("range", "gt", BUFF_OP_GT) else BUFF_OP_EQ
+        opc = _opc(op)
         check(lib().buff_filter_cond(self.ctx.handle, self.handle, opc, float(pred), flags, ptr(cw, ctypes.c_uint32),
                                      ptr(words, ctypes.c_uint32), len(words), ctypes.byref(cnt)))
         return words[: (n + 31) // 32], cnt.value
 
     def filter_cond_device(self, op, pred, cond_dev, words_dev, flags=0):
-        opc = BUFF_OP_GT if op in ("range", "gt", BUFF_OP_GT) else BUFF_OP_EQ
+        opc = _opc(op)
         check(lib().buff_filter_cond_device(self.ctx.handle, self.handle, opc, float(pred), flags, cond_dev,
                                             words_dev))
 
@@ -294,11 +323,12 @@ class Chunk:
                                                  ctypes.byref(rows)))
         return rows.value
 
-    def sum(self):
+    def sum(self, flags=0):
         """Returns (reference f64 sum, exact sum, per-sub-column u64 sums[9])."""
         r, e = ctypes.c_double(), ctypes.c_double()
         ps = np.zeros(9, dtype=np.uint64)
-        check(lib().buff_sum(self.ctx.handle, self.handle, ctypes.byref(r), ctypes.byref(e), ptr(ps, ctypes.c_uint64)))
+        check(lib().buff_sum_ex(self.ctx.handle, self.handle, flags, ctypes.byref(r), ctypes.byref(e),
+                                ptr(ps, ctypes.c_uint64)))
         return r.value, e.value, ps
 
     def max(self):

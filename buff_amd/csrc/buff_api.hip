@@ -322,6 +322,9 @@ int quirk_minmax(Ctx* ctx, const double* x, uint64_t n, int32_t pexp, uint32_t d
 
 int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_t flags, Chunk** out) {
   if (!ctx || !out) return fail(BUFF_E_INVALID, "null argument");
+  const bool slice = (flags & BUFF_SLICE) != 0;
+  if (slice && (flags & (BUFF_FLIP | BUFF_MAJORITY)))
+    return fail(BUFF_E_INVALID, "BUFF_SLICE excludes BUFF_FLIP and BUFF_MAJORITY");
   if (flags & BUFF_MAJORITY) return major_encode_device(ctx, x, n, scale, flags, out);
   if (n > 0xFFFFFFFFull) return fail(BUFF_E_INVALID, "a chunk holds at most 2^32-1 rows (n is stored as u32)");
   if (n && !x) return fail(BUFF_E_INVALID, "null input");
@@ -336,7 +339,8 @@ int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_
   BUFF_TRY(quirk_minmax(ctx, x, n, pexp, dlen, &mn, &mx));
   const int64_t delta = (int64_t)((uint64_t)mx - (uint64_t)mn);
   const uint32_t F = width_of_delta(delta);
-  if (F == 0 && n >= 1 && !(flags & BUFF_SEMANTIC))
+  // buff_slice_encode writes a constant column as one 0x80 plane (buff_slice.rs:372-379)
+  if (F == 0 && n >= 1 && !(flags & BUFF_SEMANTIC) && !slice)
     return fail(BUFF_E_REF_PANIC, "all %llu values map to one fixed-point value (F = 0): the reference panics",
                 (unsigned long long)n);
   if (F > 63) return fail(BUFF_E_UNSUPPORTED, "bit width %u exceeds 63", F);
@@ -346,7 +350,7 @@ int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_
   ch->ctx = ctx;
   ch->n = n;
   ch->scale = scale;
-  ch->flags = flags & BUFF_FLIP;
+  ch->flags = flags & (BUFF_FLIP | BUFF_SLICE);
   ch->dlen = dlen;
   ch->ilen = (uint32_t)((uint64_t)F - dlen);
   ch->F = F;
@@ -374,6 +378,7 @@ int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_
 int chunk_serialize(Ctx* ctx, const Chunk* ch, uint8_t* dst, uint64_t* len) {
   if (!ctx || !ch || !len) return fail(BUFF_E_INVALID, "null argument");
   if (ch->flags & BUFF_MAJORITY) return major_serialize(ctx, ch, dst, len);
+  if (ch->flags & BUFF_SLICE) return slice_serialize(ctx, ch, dst, len);
   const uint64_t total = stream_bytes_of(ch->n, ch->view.P, ch->view.r);
   const uint64_t need = *len;
   *len = total;
@@ -398,7 +403,10 @@ int chunk_serialize(Ctx* ctx, const Chunk* ch, uint8_t* dst, uint64_t* len) {
 
 int chunk_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_t flags, Chunk** out) {
   if (!ctx || !s || !out) return fail(BUFF_E_INVALID, "null argument");
+  if ((flags & BUFF_SLICE) && (flags & (BUFF_FLIP | BUFF_MAJORITY)))
+    return fail(BUFF_E_INVALID, "BUFF_SLICE excludes BUFF_FLIP and BUFF_MAJORITY");
   if (flags & BUFF_MAJORITY) return major_load(ctx, s, len, scale, flags, out);
+  if (flags & BUFF_SLICE) return slice_load(ctx, s, len, scale, flags, out);
   if (len < kStreamHeader) return fail(BUFF_E_INVALID, "stream of %llu bytes has no 20-byte header",
                                        (unsigned long long)len);
   uint32_t hdr[5];
@@ -454,6 +462,15 @@ int chunk_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_
   return BUFF_OK;
 }
 
+// fetch_fixed_aligned of a predicate at the chunk's scale and dlen
+// (prec_double.rs:190-205 with the operators' PrecisionBound).
+int64_t fixed_of_pred(const Chunk* ch, double pred) {
+  const int32_t pexp = precision_exp_of_prec(prec_of_scale(ch->scale));
+  uint64_t u;
+  memcpy(&u, &pred, 8);
+  return fetch_fixed_aligned_bits(u, pexp, ch->dlen);
+}
+
 // Predicate -> per-plane target, or a constant answer. Returns true when the
 // kernel must run; otherwise *fill is 0 (no rows) or 1 (all rows).
 bool filter_plan(const Chunk* ch, int op, double pred, uint32_t flags, FilterTarget* tg, int* fill) {
@@ -498,6 +515,8 @@ bool filter_plan(const Chunk* ch, int op, double pred, uint32_t flags, FilterTar
 int filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, uint64_t* words,
                   unsigned long long* stats) {
   if (!ctx || !ch || !words) return fail(BUFF_E_INVALID, "null argument");
+  const int rs = slice_filter_dispatch(ctx, ch, op, pred, flags, false, nullptr, words);
+  if (rs >= 0) return rs;
   if (op != BUFF_OP_GT && op != BUFF_OP_EQ) return fail(BUFF_E_INVALID, "unknown filter op %d", op);
   const uint64_t nwords = (ch->n + 63) / 64;
   if (nwords == 0) return BUFF_OK;
@@ -575,6 +594,8 @@ static bool cond_plan(const Chunk* ch, int op, double pred, uint32_t flags, Filt
 int filter_cond_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, const uint64_t* cond,
                        uint64_t* words) {
   if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  const int rs = slice_filter_dispatch(ctx, ch, op, pred, flags, true, cond, words);
+  if (rs >= 0) return rs;
   if (op != BUFF_OP_GT && op != BUFF_OP_EQ) return fail(BUFF_E_INVALID, "unknown filter op %d", op);
   const uint64_t nwords = (ch->n + 63) / 64;
   if (nwords == 0) return BUFF_OK;
@@ -647,6 +668,13 @@ int filter_cond_host(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t fl
 int materialize_cond_device(Ctx* ctx, const Chunk* ch, uint32_t flags, const uint64_t* cond, double* out,
                             uint64_t cap, uint64_t* rows) {
   if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->flags & BUFF_SLICE) {
+    // buff_slice_decode_condition (buff_slice.rs:1266-1347): the stored values,
+    // no rows for an empty condition, and the four-plane reader's panic
+    if (!(flags & BUFF_SEMANTIC) && (ch->F == 0 || ch->F > 32))
+      return fail(BUFF_E_REF_PANIC, "bit length greater than 32 is not supported yet (F = %u)", ch->F);
+    flags |= BUFF_SEMANTIC;
+  }
   const bool semantic = (flags & BUFF_SEMANTIC) != 0;
   const bool major = (ch->flags & BUFF_MAJORITY) != 0;
   if (major) {
@@ -721,6 +749,12 @@ int max_range_device(Ctx* ctx, const Chunk* ch, uint64_t s, uint64_t e, uint64_t
   if (W == 0 || s > e || e > ch->n)
     return fail(BUFF_E_INVALID, "window %llu over rows [%llu, %llu) of %llu", (unsigned long long)W,
                 (unsigned long long)s, (unsigned long long)e, (unsigned long long)ch->n);
+  if (ch->flags & BUFF_SLICE) {
+    // buff_slice_max_range (buff_slice.rs:930-1098): every window and every
+    // argmax row once its panics are ruled out
+    BUFF_TRY(slice_max_range_check(ch, s, e, W, flags));
+    flags |= BUFF_SEMANTIC;
+  }
   const bool semantic = (flags & BUFF_SEMANTIC) != 0;
   const ChunkView& v = ch->view;
   const uint64_t L = e - s;
@@ -950,6 +984,10 @@ int mat_plan(const Chunk* ch, int32_t precision, uint32_t flags, MatParams* out)
 
 int materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* out) {
   if (!ctx || !ch || (!out && ch->n)) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->flags & BUFF_SLICE) {
+    BUFF_TRY(slice_materialize_check(ch, precision, flags));
+    precision = -1;
+  }
   if (ch->flags & BUFF_MAJORITY) {
     if (precision >= 0)
       return fail(BUFF_E_UNSUPPORTED, "the reference has no reduced-precision decode of a buff-major stream");
@@ -1308,6 +1346,7 @@ int buff_chunk_info_get(const buff_chunk* chunk, buff_chunk_info* info) {
     // the snappy-compressed bitmaps are only sized by buff_chunk_serialize(dst = NULL)
     info->stream_bytes = ch->S ? 0 : (ch->F >= 8 ? info->stream_bytes + 1 : info->stream_bytes);
   }
+  if (ch->flags & BUFF_SLICE) info->stream_bytes = slice_stream_bytes(ch->n, ch->F);
   info->device_bytes = ch->device_bytes;
   info->sparse_cols = ch->S;
   info->reserved = 0;
@@ -1399,18 +1438,30 @@ int buff_max_range_device(buff_ctx* ctx, const buff_chunk* chunk, uint64_t s, ui
                           max_values_dev, cap, nwindows, argmax_dev);
 }
 
-int buff_sum(buff_ctx* ctx, const buff_chunk* chunk, double* ref_sum, double* exact_sum, uint64_t* plane_sums) {
-  return sum_op(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), ref_sum, exact_sum, plane_sums);
+int buff_sum_ex(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, double* ref_sum, double* exact_sum,
+                uint64_t* plane_sums) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  const Chunk* ch = reinterpret_cast<const Chunk*>(chunk);
+  if (!c || !ch) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->flags & BUFF_SLICE) return slice_sum(c, ch, flags, ref_sum, exact_sum, plane_sums);
+  return sum_op(c, ch, ref_sum, exact_sum, plane_sums);
 }
 
-int buff_max(buff_ctx* ctx, const buff_chunk* chunk, double* max_value, uint64_t* max_off) {
-  return max_op(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), 0, max_value, max_off, nullptr);
+int buff_sum(buff_ctx* ctx, const buff_chunk* chunk, double* ref_sum, double* exact_sum, uint64_t* plane_sums) {
+  return buff_sum_ex(ctx, chunk, 0, ref_sum, exact_sum, plane_sums);
 }
 
 int buff_max_ex(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, double* max_value, uint64_t* max_off,
                 uint64_t* argmax_count) {
-  return max_op(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), flags, max_value, max_off,
-                argmax_count);
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  const Chunk* ch = reinterpret_cast<const Chunk*>(chunk);
+  if (!c || !ch) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->flags & BUFF_SLICE) return slice_max(c, ch, flags, max_value, max_off, argmax_count);
+  return max_op(c, ch, flags, max_value, max_off, argmax_count);
+}
+
+int buff_max(buff_ctx* ctx, const buff_chunk* chunk, double* max_value, uint64_t* max_off) {
+  return buff_max_ex(ctx, chunk, 0, max_value, max_off, nullptr);
 }
 
 int buff_materialize(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags, double* out) {

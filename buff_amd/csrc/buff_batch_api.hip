@@ -892,7 +892,8 @@ int buff_server_add_chunk(buff_server* server, buff_chunk* chunk, uint32_t* id) 
   if (!sv || !chunk) return fail(BUFF_E_INVALID, "null argument");
   Chunk* ch = reinterpret_cast<Chunk*>(chunk);
   if (ch->ctx != sv->ctx) return fail(BUFF_E_INVALID, "the chunk belongs to another context");
-  if (ch->flags & BUFF_MAJORITY) return fail(BUFF_E_UNSUPPORTED, "served chunks use the byte_fixed layout");
+  if (ch->flags & (BUFF_MAJORITY | BUFF_SLICE))
+    return fail(BUFF_E_UNSUPPORTED, "served chunks use the byte_fixed layout");
   std::lock_guard<std::mutex> lk(sv->mu);
   if (id) *id = (uint32_t)sv->chunks.size();
   sv->chunks.push_back(ch);

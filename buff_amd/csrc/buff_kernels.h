@@ -40,6 +40,12 @@ hipError_t launch_encode(const double* x, const ChunkView& cv, int32_t pexp, hip
 hipError_t launch_filter(const ChunkView& cv, int op, const FilterTarget& tg, uint64_t* out, uint64_t nwords,
                          unsigned long long* stats, hipStream_t s, const LaunchShape& ls);
 hipError_t launch_fill_bitmap(uint64_t* out, uint64_t nwords, uint64_t n, int ones, hipStream_t s);
+// Byte-slice stream: the right-padded last plane <-> the resident tail bit-planes.
+hipError_t launch_slice_tail_serialize(const ChunkView& cv, uint8_t* plane, hipStream_t s);
+hipError_t launch_slice_tail_load(const ChunkView& cv, const uint8_t* plane, hipStream_t s);
+// out = (inv_a ? ~a : a) & (inv_b ? ~b : b) & cond over rows < cover (null a/b/cond: all rows).
+hipError_t launch_bitmap_logic(uint64_t* out, const uint64_t* a, const uint64_t* b, const uint64_t* cond,
+                               uint64_t nwords, uint64_t cover, int inv_a, int inv_b, hipStream_t s);
 // Filter restricted to the rows set in `cond` (ceil(n/64) u64 words, the bitmap layout).
 hipError_t launch_filter_cond(const ChunkView& cv, int op, const FilterTarget& tg, const uint64_t* cond,
                               uint64_t* out, uint64_t nwords, hipStream_t s, const LaunchShape& ls);

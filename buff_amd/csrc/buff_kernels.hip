@@ -723,6 +723,60 @@ __global__ __launch_bounds__(256) void k_load(ChunkView cv, uint8_t flipmask, co
 }
 
 // ----------------------------------------------------------------------------
+// Byte-slice stream (buff_slice.rs:298-437): the P full planes are the byte_fixed
+// planes XOR 0x80 (k_serialize / k_load with flipmask 0x80 and no tail); the r
+// tail bits of row i become one more plane, byte i = (tail << (8 - r)) ^ 0x80.
+
+__global__ __launch_bounds__(256) void k_slice_tail_serialize(ChunkView cv, uint8_t* __restrict__ plane) {
+  const uint32_t lane = __lane_id();
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
+  for (uint64_t c = wave; c < cv.ntiles; c += nwaves) {
+    const uint64_t r0 = c * kTileRows + 16ull * lane;
+    uint32_t h[8];
+    for (uint32_t j = 0; j < 8; ++j) h[j] = j < cv.r ? load_tail(cv, j, c, lane) : 0u;
+#pragma unroll
+    for (uint32_t b = 0; b < 16; ++b) {
+      uint32_t tv = 0;
+      for (uint32_t j = 0; j < cv.r; ++j) tv |= ((h[j] >> b) & 1u) << j;
+      if (r0 + b < cv.n) plane[r0 + b] = (uint8_t)((tv << (8 - cv.r)) ^ 0x80u);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_slice_tail_load(ChunkView cv, const uint8_t* __restrict__ plane) {
+  const uint32_t lane = __lane_id();
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
+  for (uint64_t c = wave; c < cv.ntiles; c += nwaves) {
+    const uint64_t r0 = c * kTileRows + 16ull * lane;
+    uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (uint32_t b = 0; b < 16; ++b) {
+      const uint32_t tv = r0 + b < cv.n ? ((uint32_t)(plane[r0 + b] ^ 0x80u) >> (8 - cv.r)) : 0u;
+      for (uint32_t j = 0; j < 8; ++j) h[j] |= ((tv >> j) & 1u) << b;
+    }
+    for (uint32_t j = 0; j < cv.r; ++j) cv.tail[(c * cv.r + j) * 64 + lane] = (uint16_t)h[j];
+  }
+}
+
+// out[w] = (a[w] or all rows, complemented when inv_a) & (b[w] or its
+// complement when inv_b; absent: all rows) & (cond[w] if given), restricted to
+// rows < cover. Element-wise, so out may alias a, b or cond.
+__global__ void k_bitmap_logic(uint64_t* out, const uint64_t* a, const uint64_t* b, const uint64_t* cond,
+                               uint64_t nwords, uint64_t cover, int inv_a, int inv_b) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t x = a ? a[i] : ~0ull;
+    if (inv_a) x = ~x;
+    if (b) x &= inv_b ? ~b[i] : b[i];
+    if (cond) x &= cond[i];
+    const uint64_t r0 = i * 64;
+    x &= r0 + 64 <= cover ? ~0ull : (r0 < cover ? (1ull << (cover - r0)) - 1 : 0ull);
+    out[i] = x;
+  }
+}
+
+// ----------------------------------------------------------------------------
 // Host-side launchers (declared in buff_kernels.h).
 
 // Grid cap of a launch shape: its max_wg, else `dflt`.
@@ -1003,6 +1057,25 @@ hipError_t launch_load(const ChunkView& cv, uint8_t flipmask, const uint8_t* bod
                        hipStream_t s) {
   const uint32_t wg = grid_for(cv.ntiles, 4, 2048);
   launch_k(k_load, dim3(wg), dim3(256), 0, s, cv, flipmask, body, tail_bytes);
+  return hipGetLastError();
+}
+
+hipError_t launch_slice_tail_serialize(const ChunkView& cv, uint8_t* plane, hipStream_t s) {
+  const uint32_t wg = grid_for(cv.ntiles, 4, 2048);
+  launch_k(k_slice_tail_serialize, dim3(wg), dim3(256), 0, s, cv, plane);
+  return hipGetLastError();
+}
+
+hipError_t launch_slice_tail_load(const ChunkView& cv, const uint8_t* plane, hipStream_t s) {
+  const uint32_t wg = grid_for(cv.ntiles, 4, 2048);
+  launch_k(k_slice_tail_load, dim3(wg), dim3(256), 0, s, cv, plane);
+  return hipGetLastError();
+}
+
+hipError_t launch_bitmap_logic(uint64_t* out, const uint64_t* a, const uint64_t* b, const uint64_t* cond,
+                               uint64_t nwords, uint64_t cover, int inv_a, int inv_b, hipStream_t s) {
+  const uint32_t wg = (uint32_t)std::min<uint64_t>((nwords + 255) / 256, 2048);
+  if (wg) launch_k(k_bitmap_logic, dim3(wg), dim3(256), 0, s, out, a, b, cond, nwords, cover, inv_a, inv_b);
   return hipGetLastError();
 }
 

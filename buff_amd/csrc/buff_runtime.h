@@ -68,7 +68,7 @@ struct Ctx {
   std::multimap<uint64_t, void*> free_blocks;
   void* pinned = nullptr;
 
-  Scratch s_partial, s_minmax, s_stream, s_bitmap, s_sums, s_best, s_values, s_sink, s_major, s_cond, s_condin, s_wmax;
+  Scratch s_partial, s_minmax, s_stream, s_bitmap, s_sums, s_best, s_values, s_sink, s_major, s_cond, s_condin, s_wmax, s_slice;
 
   bool prof_on = false;
   std::vector<hipEvent_t> prof_pool;
@@ -118,7 +118,7 @@ struct Chunk {
   Ctx* ctx = nullptr;
   uint64_t n = 0;
   uint64_t scale = 0;
-  uint32_t flags = 0;  // BUFF_FLIP | BUFF_MAJORITY
+  uint32_t flags = 0;  // BUFF_FLIP | BUFF_MAJORITY | BUFF_SLICE
   uint32_t dlen = 0;
   uint32_t ilen = 0;
   uint32_t F = 0;      // full stored width; `view` holds the low F - 8*S bits
@@ -145,6 +145,24 @@ int filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags
 int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64_t* plane_sums);
 int max_op(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t* max_off, uint64_t* argmax);
 int materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* out);
+int filter_cond_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, const uint64_t* cond,
+                       uint64_t* words);
+int materialize_cond_device(Ctx* ctx, const Chunk* ch, uint32_t flags, const uint64_t* cond, double* out,
+                            uint64_t cap, uint64_t* rows);
+int max_range_device(Ctx* ctx, const Chunk* ch, uint64_t s, uint64_t e, uint64_t W, uint32_t flags, double* vals,
+                     uint64_t cap, uint64_t* nwin, uint64_t* argmax);
+int64_t fixed_of_pred(const Chunk* ch, double pred);
+
+// byte-slice chunks and the _with_slice filters (buff_slice_api.hip)
+uint64_t slice_stream_bytes(uint64_t n, uint32_t F);
+int slice_serialize(Ctx* ctx, const Chunk* ch, uint8_t* dst, uint64_t* len);
+int slice_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_t flags, Chunk** out);
+int slice_filter_dispatch(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, bool conditional,
+                          const uint64_t* cond, uint64_t* words);
+int slice_materialize_check(const Chunk* ch, int32_t precision, uint32_t flags);
+int slice_sum(Ctx* ctx, const Chunk* ch, uint32_t flags, double* ref_sum, double* exact_sum, uint64_t* plane_sums);
+int slice_max(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t* max_off, uint64_t* argmax);
+int slice_max_range_check(const Chunk* ch, uint64_t s, uint64_t e, uint64_t W, uint32_t flags);
 
 // buff-major (buff_major_api.hip)
 int major_encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_t flags, Chunk** out);

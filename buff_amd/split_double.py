@@ -131,3 +131,11 @@ class SplitBDDoubleCompress:
         N.check(N.lib().buff_sd_buff_max_majority(ctypes.byref(self._sd), N.ptr(b, ctypes.c_uint8), len(b),
                                                   ctypes.byref(m)))
         return m.value
+
+    # ---- buff-simd "_with_slice" filters (buff_simd.rs:3736-4025) -----------------
+
+    def buff_simd_range_filter_with_slice(self, b, pred):  # buff_simd.rs:3736
+        return self._filter("buff_sd_buff_simd_range_filter_with_slice", b, pred)
+
+    def buff_simd_equal_filter_with_slice(self, b, pred):  # buff_simd.rs:3888
+        return self._filter("buff_sd_buff_simd_equal_filter_with_slice", b, pred)

@@ -56,10 +56,24 @@ extern "C" {
  * byte_fixed_encode. Accepted by buff_encode, buff_encode_device and
  * buff_chunk_load; every query dispatches on it. */
 #define BUFF_MAJORITY 0x4u
+/* Byte-slice layout of buff_slice_encode (buff_slice.rs:298-437): byte_fixed's
+ * header and offsets, every sub-column a full byte XOR 0x80 (the last one
+ * right-padded; one 0x80 plane when all values are equal). Only the stream
+ * differs; the resident form is byte_fixed's. Queries of such a chunk follow
+ * BuffSliceCompress (buff_slice.rs:222-1970). Excludes BUFF_FLIP and BUFF_MAJORITY. */
+#define BUFF_SLICE 0x8u
+/* Byte-slice queries: the scalar driver's variants (run_buff_slice_scalar_encoding_decoding,
+ * buff_slice.rs:103-171): the nosimd filters, decode_bs, buff_slice_sum,
+ * buff_slice_max instead of the AVX2 / "buff way" ones. */
+#define BUFF_SCALAR 0x10u
+/* Filters of a byte_fixed chunk: buff_simd_range_filter_with_slice /
+ * buff_simd_equal_filter_with_slice (buff_simd.rs:3736-4025). */
+#define BUFF_WITH_SLICE 0x20u
 
 /* filter operators */
 #define BUFF_OP_GT 0 /* x > pred (buff_range_filter, buff_simd.rs:4507-4679) */
 #define BUFF_OP_EQ 1 /* x == pred (buff_equal_filter, buff_simd.rs:4925-5093) */
+#define BUFF_OP_LT 2 /* x < pred: buff_slice_range_smaller_filter_condition (buff_slice.rs:1553-1662) */
 
 typedef struct buff_ctx buff_ctx;
 typedef struct buff_chunk buff_chunk;
@@ -204,6 +218,11 @@ int buff_max_range_device(buff_ctx* ctx, const buff_chunk* chunk, uint64_t s, ui
  * f64 fold bit for bit; exact_sum = (n*base + sum(off)) / 2^dlen rounded once;
  * plane_sums (nullable, 9 u64) are the exact per-sub-column byte sums (tail last). */
 int buff_sum(buff_ctx* ctx, const buff_chunk* chunk, double* ref_sum, double* exact_sum, uint64_t* plane_sums);
+/* buff_sum with flags. A BUFF_SLICE chunk is buff_slice_buffsum (buff_slice.rs:
+ * 702-814; BUFF_SCALAR: buff_slice_sum, 1101-1182, a panic unless 1 <= F <= 32):
+ * ref_sum = exact_sum, the f64 running sum's value within its rounding. */
+int buff_sum_ex(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, double* ref_sum, double* exact_sum,
+                uint64_t* plane_sums);
 /* byte_fixed_max (split_double.rs:3919-4105): the printed max value and the max
  * stored offset. */
 int buff_max(buff_ctx* ctx, const buff_chunk* chunk, double* max_value, uint64_t* max_off);
@@ -391,6 +410,43 @@ int buff_sd_buff_equal_filter_majority(const buff_split_double* sd, const uint8_
                                        uint32_t* words, uint64_t nwords, uint64_t* count);
 int buff_sd_buff_sum_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* sum);
 int buff_sd_buff_max_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* max_value);
+
+/* byte-slice (BuffSliceCompress, buff_slice.rs:222-1970; its fields are
+ * SplitBDDoubleCompress's {chunksize, batchsize, scale}, buff_slice.rs:222-232):
+ * :298 buff_slice_encode (out == NULL: size query), :439 buff_slice_decode,
+ * :1184 buff_slice_decode_bs, :1348 buff_slice_range_filter, :1665
+ * buff_slice_equal_filter, :1753 buff_slice_range_filter_nosimd, :1873
+ * buff_slice_equal_filter_nosimd, :702 buff_slice_buffsum, :1101
+ * buff_slice_sum, :548 buff_slice_buffmax, :816 buff_slice_max; and over a
+ * buff_simd256 stream buff_simd.rs:3736 buff_simd_range_filter_with_slice /
+ * :3888 buff_simd_equal_filter_with_slice. Sums: the exact sum rounded once
+ * (the reference's f64 running sum agrees within 1e-12 relative on the tested
+ * columns). Maxima: the printed value and the printed argmax count. */
+typedef buff_split_double buff_slice_compress;
+int buff_sd_buff_slice_encode(const buff_slice_compress* sc, const double* data, uint64_t n, uint8_t* out,
+                              uint64_t cap, uint64_t* out_len);
+int buff_sd_buff_slice_decode(const buff_slice_compress* sc, const uint8_t* bytes, uint64_t len, double* out,
+                              uint64_t cap);
+int buff_sd_buff_slice_decode_bs(const buff_slice_compress* sc, const uint8_t* bytes, uint64_t len, double* out,
+                                 uint64_t cap);
+int buff_sd_buff_slice_range_filter(const buff_slice_compress* sc, const uint8_t* bytes, uint64_t len, double pred,
+                                    uint32_t* words, uint64_t nwords, uint64_t* count);
+int buff_sd_buff_slice_equal_filter(const buff_slice_compress* sc, const uint8_t* bytes, uint64_t len, double pred,
+                                    uint32_t* words, uint64_t nwords, uint64_t* count);
+int buff_sd_buff_slice_range_filter_nosimd(const buff_slice_compress* sc, const uint8_t* bytes, uint64_t len,
+                                           double pred, uint32_t* words, uint64_t nwords, uint64_t* count);
+int buff_sd_buff_slice_equal_filter_nosimd(const buff_slice_compress* sc, const uint8_t* bytes, uint64_t len,
+                                           double pred, uint32_t* words, uint64_t nwords, uint64_t* count);
+int buff_sd_buff_simd_range_filter_with_slice(const buff_split_double* sd, const uint8_t* bytes, uint64_t len,
+                                              double pred, uint32_t* words, uint64_t nwords, uint64_t* count);
+int buff_sd_buff_simd_equal_filter_with_slice(const buff_split_double* sd, const uint8_t* bytes, uint64_t len,
+                                              double pred, uint32_t* words, uint64_t nwords, uint64_t* count);
+int buff_sd_buff_slice_buffsum(const buff_slice_compress* sc, const uint8_t* bytes, uint64_t len, double* sum);
+int buff_sd_buff_slice_sum(const buff_slice_compress* sc, const uint8_t* bytes, uint64_t len, double* sum);
+int buff_sd_buff_slice_buffmax(const buff_slice_compress* sc, const uint8_t* bytes, uint64_t len, double* max_value,
+                               uint64_t* argmax_count);
+int buff_sd_buff_slice_max(const buff_slice_compress* sc, const uint8_t* bytes, uint64_t len, double* max_value,
+                           uint64_t* argmax_count);
 
 #ifdef __cplusplus
 }
