@@ -19,7 +19,8 @@ memory-side cache refilled with 1 GiB of other, clean data before each launch)
 is reported beside it. `filter_project` times the conditional filter and the
 gather of a filter's rows (buff_range_filter_condition, buff_decode_condition)
 on the walk column, cold; `window_max` times buff_max_range over the walk
-column at two window sizes.
+column at two window sizes; `slice` times the walk column as a byte-slice
+chunk (buff_slice_encode layout): encode, filters, decode, sum, stream write.
 
 Further sections of the same JSON line (each skippable):
   config3 — an outlier-heavy column of 67,108,864 rows in buff and buff-major:
@@ -251,6 +252,55 @@ def bench_window_max(ctx, col, ch, info, words, out, flush, buff_amd):
         g = b / (us * 1e-6) / 1e9
         res["window_%d" % w] = {"windows": k, "avg_us": round(us, 2), "algorithmic_bytes": round(b),
                                 "achieved_GBps": round(g, 1), "frac": round(g / HBM_PEAK_GBPS, 4)}
+    return res
+
+
+def bench_slice(ctx, col, xdev, words, out, flush, buff_amd):
+    """The walk column as a byte-slice chunk (BUFF_SLICE), each launch after a
+    1 GiB cache eviction (HIP events, mean of 3): encode, the AVX2-form range
+    filter at 10% and equality on a stored value, full decode, sum, and the
+    slice stream kernels (planes XOR 0x80 plus the right-padded last plane).
+    The resident form is byte_fixed's, so these are its kernels with the slice
+    targets; algorithmic bytes as for byte_fixed, the stream write n*ceil(F/8)."""
+    n = col.n
+    ch = ctx.encode_device(xdev.ptr, n, col.scale, flags=buff_amd.BUFF_SLICE)
+    info = ch.info()
+    per_row = info.planes + info.tail_bits / 8.0
+    ns = max(1, -(-info.width // 8))
+
+    def timed(fn, kname, trials=3):
+        ts = []
+        for _ in range(trials):
+            ctx.prof_reset()
+            flush.touch()
+            ctx.synchronize()
+            ctx.prof_enable(True)
+            r = fn()
+            ctx.synchronize()
+            ctx.prof_enable(False)
+            if r is not None and hasattr(r, "free"):
+                r.free()
+            launches, ms = ctx.prof_query(kname)
+            ts.append(ms * 1e3 / max(launches, 1))
+        return float(np.mean(ts))
+
+    def entry(us, b):
+        g = b / (us * 1e-6) / 1e9
+        return {"avg_us": round(us, 2), "algorithmic_bytes": round(b), "achieved_GBps": round(g, 1),
+                "frac": round(g / HBM_PEAK_GBPS, 4), "GBps_f64_input": round(8.0 * n / (us * 1e-6) / 1e9, 1)}
+
+    res = {"rows": n, "F": info.width, "stream_bytes": info.stream_bytes}
+    res["encode"] = entry(timed(lambda: ctx.encode_device(xdev.ptr, n, col.scale, flags=buff_amd.BUFF_SLICE),
+                                "k_encode"), 8.0 * n + n * info.width / 8.0)
+    res["range_10pct"] = entry(timed(lambda: ch.filter_device("range", col.preds[0.10], words.ptr), "k_filter_gt"),
+                               n * (1 + 1 / 8.0))
+    res["equal_stored_value"] = entry(timed(lambda: ch.filter_device("equal", col.preds[0.50], words.ptr),
+                                            "k_filter_eq"), n * (1 + 1 / 8.0))
+    res["decode"] = entry(timed(lambda: ch.materialize_device(out.ptr, -1), "k_materialize"),
+                          n * (info.width / 8.0 + 8.0))
+    res["sum"] = entry(timed(lambda: ch.sum(), "k_sum"), n * per_row)
+    res["stream_write"] = entry(timed(lambda: ch.serialize(), "k_serialize"), n * per_row + n * ns)
+    ch.free()
     return res
 
 
@@ -729,6 +779,7 @@ def main():
                                                                        else "")}
     filter_project = bench_filter_project(ctx, cols[0], keep[0], infos[0], words, out, flush, buff_amd)
     window_max = bench_window_max(ctx, cols[0], keep[0], infos[0], words, out, flush, buff_amd)
+    slice_sec = bench_slice(ctx, cols[0], dx[0], words, out, flush, buff_amd)
     for ch in keep:
         ch.free()
 
@@ -757,6 +808,7 @@ def main():
         "cold_filters": filters,
         "filter_project": filter_project,
         "window_max": window_max,
+        "slice": slice_sec,
     }
 
     if not args.no_config3:
