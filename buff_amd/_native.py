@@ -31,6 +31,20 @@ BUFF_OP_EQ = 1
 BUFF_OP_LT = 2
 
 
+def construct_file_iterator_skip_newline(path, skip_val=0, delim=b","):
+    """client.rs:163-182 on a file: its values (read on the host, parsed on the GPU)."""
+    try:
+        size = os.path.getsize(path)
+    except OSError:
+        size = 0   # the library reports the file it cannot open
+    cap = size // 2 + 1
+    out = np.zeros(max(cap, 1), dtype=np.float64)
+    n = ctypes.c_uint64()
+    check(lib().buff_construct_file_iterator_skip_newline(os.fsencode(path), skip_val, ord(delim),
+                                                          ptr(out, ctypes.c_double), cap, ctypes.byref(n)))
+    return out[: n.value]
+
+
 def _opc(op):
     """Filter op name -> BUFF_OP_* ("range"/"gt", "equal"/"eq", "smaller"/"lt")."""
     if op in ("range", "gt", BUFF_OP_GT):
@@ -164,6 +178,9 @@ SIGNATURES = {
     "buff_server_run_trace": [vp, P(Query), f64p, P(vp), u32, P(QueryResult), f64p],
     "buff_server_stats": [vp, u64p, u64p, u64p],
     "buff_server_destroy": [vp],
+    "buff_csv_parse": [vp, u8p, u64, ctypes.c_uint8, u32, f64p, u64, u64p],
+    "buff_csv_parse_device": [vp, vp, u64, ctypes.c_uint8, u32, vp, u64, u64p],
+    "buff_construct_file_iterator_skip_newline": [ctypes.c_char_p, u32, ctypes.c_uint8, f64p, u64, u64p],
     "buff_sd_byte_fixed_encode": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
     "buff_sd_buff_slice_encode": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
     "buff_sd_buff_slice_decode": [P(SplitDouble), u8p, u64, f64p, u64],
@@ -568,6 +585,23 @@ class Context:
         h = ctypes.c_void_p()
         check(lib().buff_encode_device(self.handle, x_dev, n, scale, flags, ctypes.byref(h)))
         return Chunk(self, h.value)
+
+    def csv_parse(self, text, delim=b",", skip_val=0):
+        """construct_file_iterator_skip_newline::<f64> over `text` (bytes): the values."""
+        t = np.frombuffer(bytes(text), dtype=np.uint8)
+        cap = len(t) // 2 + 1
+        out = np.zeros(max(cap, 1), dtype=np.float64)
+        n = ctypes.c_uint64()
+        check(lib().buff_csv_parse(self.handle, ptr(t, ctypes.c_uint8) if len(t) else None, len(t), ord(delim),
+                                   skip_val, ptr(out, ctypes.c_double), cap, ctypes.byref(n)))
+        return out[: n.value]
+
+    def csv_parse_device(self, text_dev, length, out_dev, cap, delim=b",", skip_val=0):
+        """Parses text already in device memory into device memory; returns the value count."""
+        n = ctypes.c_uint64()
+        check(lib().buff_csv_parse_device(self.handle, text_dev, length, ord(delim), skip_val, out_dev, cap,
+                                          ctypes.byref(n)))
+        return n.value
 
     def load(self, stream, scale, flags=0):
         s = np.ascontiguousarray(stream, dtype=np.uint8)

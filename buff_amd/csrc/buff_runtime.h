@@ -68,7 +68,9 @@ struct Ctx {
   std::multimap<uint64_t, void*> free_blocks;
   void* pinned = nullptr;
 
-  Scratch s_partial, s_minmax, s_stream, s_bitmap, s_sums, s_best, s_values, s_sink, s_major, s_cond, s_condin, s_wmax, s_slice;
+  Scratch s_partial, s_minmax, s_stream, s_bitmap, s_sums, s_best, s_values, s_sink, s_major, s_cond, s_condin, s_wmax, s_slice, s_csv, s_csvtab,
+      s_csvtmp, s_csvslow;
+  bool csv_tab_ready = false;  // s_csvtab holds the table of 5^q
 
   bool prof_on = false;
   std::vector<hipEvent_t> prof_pool;
@@ -152,6 +154,10 @@ int materialize_cond_device(Ctx* ctx, const Chunk* ch, uint32_t flags, const uin
 int max_range_device(Ctx* ctx, const Chunk* ch, uint64_t s, uint64_t e, uint64_t W, uint32_t flags, double* vals,
                      uint64_t cap, uint64_t* nwin, uint64_t* argmax);
 int64_t fixed_of_pred(const Chunk* ch, double pred);
+
+// CSV ingest (buff_csv.hip)
+int csv_parse_device(Ctx* ctx, const uint8_t* text, uint64_t len, uint8_t delim, uint32_t skip, double* out,
+                     uint64_t cap, uint64_t* count);
 
 // byte-slice chunks and the _with_slice filters (buff_slice_api.hip)
 uint64_t slice_stream_bytes(uint64_t n, uint32_t F);

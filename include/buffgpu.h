@@ -411,6 +411,26 @@ int buff_sd_buff_equal_filter_majority(const buff_split_double* sd, const uint8_
 int buff_sd_buff_sum_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* sum);
 int buff_sd_buff_max_majority(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* max_value);
 
+/* ---- CSV ingest ---------------------------------------------------------------
+ * construct_file_iterator_skip_newline::<f64>(file, skip_val, delim)
+ * (database/src/client.rs:163-182), the reader every benchmark driver starts
+ * with: lines split on '\n' (a trailing "\r" removed), items split on delim
+ * (ASCII, not CR), the first skip_val items of each line skipped, and every item
+ * f64::from_str accepts kept in file order, correctly rounded; other items are
+ * dropped. *count = values found (BUFF_E_SMALL_BUFFER when above cap).
+ * BUFF_E_UNSUPPORTED: a byte >= 0x80 (the reference drops lines that are not
+ * UTF-8), or an item whose rounding the 128-bit power of 5 leaves open.
+ * The _device form parses text already in device memory into device memory
+ * (e.g. straight into buff_encode_device); buff_csv_parse takes host buffers;
+ * buff_construct_file_iterator_skip_newline reads the file (BUFF_E_INVALID
+ * where the reference returns Err(())). */
+int buff_csv_parse_device(buff_ctx* ctx, const uint8_t* text_dev, uint64_t len, uint8_t delim, uint32_t skip_val,
+                          double* out_dev, uint64_t cap, uint64_t* count);
+int buff_csv_parse(buff_ctx* ctx, const uint8_t* text, uint64_t len, uint8_t delim, uint32_t skip_val, double* out,
+                   uint64_t cap, uint64_t* count);
+int buff_construct_file_iterator_skip_newline(const char* file, uint32_t skip_val, uint8_t delim, double* out,
+                                              uint64_t cap, uint64_t* count);
+
 /* byte-slice (BuffSliceCompress, buff_slice.rs:222-1970; its fields are
  * SplitBDDoubleCompress's {chunksize, batchsize, scale}, buff_slice.rs:222-232):
  * :298 buff_slice_encode (out == NULL: size query), :439 buff_slice_decode,
