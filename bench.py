@@ -20,7 +20,9 @@ is reported beside it. `filter_project` times the conditional filter and the
 gather of a filter's rows (buff_range_filter_condition, buff_decode_condition)
 on the walk column, cold; `window_max` times buff_max_range over the walk
 column at two window sizes; `slice` times the walk column as a byte-slice
-chunk (buff_slice_encode layout): encode, filters, decode, sum, stream write.
+chunk (buff_slice_encode layout): encode, filters, decode, sum, stream write;
+`ingest` parses the reference's CSV data file (repeated 40 times, resident in
+HBM) into f64 values on the device.
 
 Further sections of the same JSON line (each skippable):
   config3 — an outlier-heavy column of 67,108,864 rows in buff and buff-major:
@@ -301,6 +303,49 @@ def bench_slice(ctx, col, xdev, words, out, flush, buff_amd):
     res["sum"] = entry(timed(lambda: ch.sum(), "k_sum"), n * per_row)
     res["stream_write"] = entry(timed(lambda: ch.serialize(), "k_serialize"), n * per_row + n * ns)
     ch.free()
+    return res
+
+
+def bench_ingest(ctx, flush):
+    """CSV ingest (construct_file_iterator_skip_newline::<f64>) on the device: the
+    reference's data file (500 lines x 1000 items) repeated 40 times, resident
+    in HBM, parsed into f64 values, each run after a 1 GiB cache eviction (HIP
+    events, mean of 3). Algorithmic bytes: the text once plus 8 bytes per
+    value; the kernels read the text twice (the item scan and the parse)."""
+    import gzip
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                        "randomwalkdatasample1k-1k.csv.gz")
+    one = gzip.open(path).read()
+    text = b"\n".join([one] * 40)
+    n_text = len(text)
+    tb = ctx.alloc(n_text)
+    tb.upload(np.frombuffer(text, dtype=np.uint8))
+    cap = 40 * 500 * 1000
+    vb = ctx.alloc(cap * 8)
+    kernels = ("k_csv_count", "k_csv_parse")
+    ts = {k: [] for k in kernels}
+    n = 0
+    for _ in range(3):
+        ctx.prof_reset()
+        flush.touch()
+        ctx.synchronize()
+        ctx.prof_enable(True)
+        n = ctx.csv_parse_device(tb.ptr, n_text, vb.ptr, cap)
+        ctx.synchronize()
+        ctx.prof_enable(False)
+        for k in kernels:
+            launches, ms = ctx.prof_query(k)
+            ts[k].append(ms * 1e3 / max(launches, 1))
+    us = {k: float(np.mean(v)) for k, v in ts.items()}
+    total = sum(us.values())
+    b = n_text + 8.0 * n
+    g = b / (total * 1e-6) / 1e9
+    res = {"text_bytes": n_text, "values": n, "kernel_us": {k: round(v, 2) for k, v in us.items()},
+           "avg_us": round(total, 2), "algorithmic_bytes": round(b), "achieved_GBps": round(g, 1),
+           "frac": round(g / HBM_PEAK_GBPS, 4), "text_GBps": round(n_text / (total * 1e-6) / 1e9, 1),
+           "values_per_s": round(n / (total * 1e-6), 1)}
+    tb.free()
+    vb.free()
     return res
 
 
@@ -780,6 +825,7 @@ def main():
     filter_project = bench_filter_project(ctx, cols[0], keep[0], infos[0], words, out, flush, buff_amd)
     window_max = bench_window_max(ctx, cols[0], keep[0], infos[0], words, out, flush, buff_amd)
     slice_sec = bench_slice(ctx, cols[0], dx[0], words, out, flush, buff_amd)
+    ingest = bench_ingest(ctx, flush)
     for ch in keep:
         ch.free()
 
@@ -809,6 +855,7 @@ def main():
         "filter_project": filter_project,
         "window_max": window_max,
         "slice": slice_sec,
+        "ingest": ingest,
     }
 
     if not args.no_config3:

@@ -32,6 +32,7 @@
 
 #include "buff_csv.h"
 #include "buff_device.h"
+#include "buff_launch.h"
 
 namespace buffgpu {
 
@@ -457,10 +458,8 @@ __global__ __launch_bounds__(256) void k_csv_parse(const uint8_t* __restrict__ t
   uint8_t by[16];
   load16(text, len, at, by);
   const LaneFacts f = lane_facts(by, at, len, delim);
-  uint64_t dummy_u = 0;
   int64_t dummy_i = 0;
   Seg dummy_s{0, 0};
-  const uint64_t ex_tok = wg_exclusive<uint64_t>(f.ntok, 0ull, SumOp{}, lds_u, &dummy_u);
   const int64_t ex_delim = wg_exclusive<int64_t>(f.last_delim, -1ll, MaxOp{}, lds_i, &dummy_i);
   const Seg ex_seg =
       wg_exclusive<Seg>(Seg{f.has_nl, f.since}, Seg{0, 0}, SegOp{}, lds_s, &dummy_s);
@@ -734,8 +733,9 @@ static inline CsvPre* pre_of(uint8_t* s, uint64_t nb) {
 hipError_t launch_csv_count(const uint8_t* text, uint64_t len, uint8_t delim, uint8_t* scratch, uint64_t nblocks,
                             hipStream_t s) {
   unsigned long long* totals = reinterpret_cast<unsigned long long*>(scratch + csv_totals_offset(nblocks));
-  k_csv_blocks<<<dim3((uint32_t)nblocks), dim3(256), 0, s>>>(text, len, delim, blk_of(scratch));
-  k_csv_scan<<<dim3(1), dim3(256), 0, s>>>(blk_of(scratch), pre_of(scratch, nblocks), nblocks, 0, totals);
+  launch_k(k_csv_blocks, dim3((uint32_t)nblocks), dim3(256), 0, s, text, len, (uint32_t)delim, blk_of(scratch));
+  launch_k(k_csv_scan, dim3(1), dim3(256), 0, s, (const CsvBlock*)blk_of(scratch), pre_of(scratch, nblocks), nblocks, 0,
+           totals);
   return hipGetLastError();
 }
 
@@ -743,23 +743,24 @@ hipError_t launch_csv_parse(const uint8_t* text, uint64_t len, uint8_t delim, ui
                             uint8_t* scratch, uint64_t nblocks, double* dst, CsvSlow* slow, uint64_t slow_cap,
                             hipStream_t s) {
   unsigned long long* totals = reinterpret_cast<unsigned long long*>(scratch + csv_totals_offset(nblocks));
-  k_csv_parse<<<dim3((uint32_t)nblocks), dim3(256), 0, s>>>(text, len, delim, skip, tab, pre_of(scratch, nblocks),
-                                                            dst, slow, slow_cap, totals);
-  k_csv_scan<<<dim3(1), dim3(256), 0, s>>>(blk_of(scratch), pre_of(scratch, nblocks), nblocks, 1, totals);
+  launch_k(k_csv_parse, dim3((uint32_t)nblocks), dim3(256), 0, s, text, len, (uint32_t)delim, skip, tab,
+           pre_of(scratch, nblocks), dst, slow, slow_cap, totals);
+  launch_k(k_csv_scan, dim3(1), dim3(256), 0, s, (const CsvBlock*)blk_of(scratch), pre_of(scratch, nblocks), nblocks, 1,
+           totals);
   return hipGetLastError();
 }
 
 hipError_t launch_csv_slow(const uint8_t* text, const CsvSlow* slow, uint64_t count, uint8_t* scratch,
                            uint64_t nblocks, double* dst, hipStream_t s) {
   unsigned long long* totals = reinterpret_cast<unsigned long long*>(scratch + csv_totals_offset(nblocks));
-  if (count) k_csv_slow<<<dim3((uint32_t)((count + 63) / 64)), dim3(64), 0, s>>>(text, slow, count, dst, totals);
+  if (count) launch_k(k_csv_slow, dim3((uint32_t)((count + 63) / 64)), dim3(64), 0, s, text, slow, count, dst, totals);
   return hipGetLastError();
 }
 
 hipError_t launch_csv_compact(const uint8_t* scratch, uint64_t nblocks, const double* src, double* out,
                               hipStream_t s) {
-  k_csv_compact<<<dim3((uint32_t)nblocks), dim3(256), 0, s>>>(pre_of(const_cast<uint8_t*>(scratch), nblocks), src,
-                                                              out);
+  launch_k(k_csv_compact, dim3((uint32_t)nblocks), dim3(256), 0, s,
+           (const CsvPre*)pre_of(const_cast<uint8_t*>(scratch), nblocks), src, out);
   return hipGetLastError();
 }
 
