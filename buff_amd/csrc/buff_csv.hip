@@ -289,23 +289,11 @@ __device__ __forceinline__ void load16(const uint8_t* __restrict__ text, uint64_
   }
 }
 
-// Exclusive scans over the 256 lanes of a workgroup (Hillis-Steele in LDS).
-template <class T, class Op>
-__device__ T wg_exclusive(T x, T ident, Op op, T* lds, T* total) {
-  const uint32_t t = threadIdx.x;
-  lds[t] = x;
-  __syncthreads();
-  for (uint32_t d = 1; d < 256; d <<= 1) {
-    const T y = t >= d ? lds[t - d] : ident;
-    __syncthreads();
-    lds[t] = t >= d ? op(y, lds[t]) : lds[t];
-    __syncthreads();
-  }
-  const T ex = t ? lds[t - 1] : ident;
-  if (total) *total = lds[255];
-  __syncthreads();
-  return ex;
+// Lane shuffles of the scanned types.
+__device__ __forceinline__ uint64_t shfl_up_t(uint64_t v, unsigned d) {
+  return (uint64_t)__shfl_up((unsigned long long)v, d, 64);
 }
+__device__ __forceinline__ int64_t shfl_up_t(int64_t v, unsigned d) { return (int64_t)__shfl_up((long long)v, d, 64); }
 
 struct Seg {  // items of a run: reset = it contains a newline, v = items since the last one
   uint32_t reset, v;
@@ -319,6 +307,36 @@ struct SumOp {
 struct MaxOp {
   __device__ int64_t operator()(int64_t a, int64_t b) const { return a > b ? a : b; }
 };
+__device__ __forceinline__ Seg shfl_up_t(Seg v, unsigned d) {
+  return Seg{(uint32_t)__shfl_up((unsigned)v.reset, d, 64), (uint32_t)__shfl_up((unsigned)v.v, d, 64)};
+}
+
+// Exclusive scan over the 256 lanes of a workgroup (in lane order; `op` need
+// not commute): shuffles within each wave, then the four wave totals from LDS.
+// *total gets the workgroup's combined value.
+template <class T, class Op>
+__device__ T wg_exclusive(T x, T ident, Op op, T* lds4, T* total) {
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  T incl = x;
+#pragma unroll
+  for (unsigned d = 1; d < 64; d <<= 1) {
+    const T y = shfl_up_t(incl, d);
+    if (lane >= d) incl = op(y, incl);
+  }
+  T ex_in_wave = shfl_up_t(incl, 1);
+  if (lane == 0) ex_in_wave = ident;
+  if (lane == 63) lds4[wave] = incl;
+  __syncthreads();
+  T before = ident, all = ident;
+#pragma unroll
+  for (unsigned w = 0; w < 4; ++w) {
+    if (w < wave) before = op(before, lds4[w]);
+    all = op(all, lds4[w]);
+  }
+  __syncthreads();
+  if (total) *total = all;
+  return op(before, ex_in_wave);
+}
 
 // Per-lane facts of 16 bytes at absolute position `at`.
 struct LaneFacts {
@@ -352,9 +370,9 @@ __device__ LaneFacts lane_facts(const uint8_t* by, uint64_t at, uint64_t len, ui
 
 __global__ __launch_bounds__(256) void k_csv_blocks(const uint8_t* __restrict__ text, uint64_t len, uint32_t delim,
                                                     CsvBlock* __restrict__ blk) {
-  __shared__ uint64_t lds_u[256];
-  __shared__ int64_t lds_i[256];
-  __shared__ Seg lds_s[256];
+  __shared__ uint64_t lds_u[4];
+  __shared__ int64_t lds_i[4];
+  __shared__ Seg lds_s[4];
   const uint64_t b = blockIdx.x;
   const uint64_t at = b * kCsvBlock + 16ull * threadIdx.x;
   uint8_t by[16];
@@ -383,12 +401,15 @@ __global__ __launch_bounds__(256) void k_csv_blocks(const uint8_t* __restrict__ 
 }
 
 // mode 0: item base, line index, previous delimiter, totals[0..1];
-// mode 1: parsed-item base, totals[2].
+// mode 1: parsed-item base, totals[2]. Each lane takes 16 consecutive blocks
+// (one pass over its blocks, a workgroup scan of the lane totals, a second pass
+// writing the prefixes), 4096 blocks per step.
 __global__ __launch_bounds__(256) void k_csv_scan(const CsvBlock* __restrict__ blk, CsvPre* __restrict__ pre,
                                                   uint64_t nblocks, int mode, unsigned long long* __restrict__ totals) {
-  __shared__ uint64_t lds_u[256];
-  __shared__ int64_t lds_i[256];
-  __shared__ Seg lds_s[256];
+  constexpr uint32_t kPer = 16;
+  __shared__ uint64_t lds_u[4];
+  __shared__ int64_t lds_i[4];
+  __shared__ Seg lds_s[4];
   __shared__ uint32_t any_nonascii;
   if (threadIdx.x == 0) any_nonascii = 0;
   __syncthreads();
@@ -396,36 +417,54 @@ __global__ __launch_bounds__(256) void k_csv_scan(const CsvBlock* __restrict__ b
   Seg carry_seg{0, 0};
   int64_t carry_delim = -1;
   uint32_t nonascii = 0;
-  for (uint64_t base = 0; base < nblocks; base += 256) {
-    const uint64_t i = base + threadIdx.x;
-    const bool live = i < nblocks;
+  for (uint64_t base = 0; base < nblocks; base += 256ull * kPer) {
+    const uint64_t i0 = base + (uint64_t)threadIdx.x * kPer;
+    const uint64_t i1 = i0 + kPer < nblocks ? i0 + kPer : nblocks;
     if (mode == 0) {
-      const CsvBlock c = live ? blk[i] : CsvBlock{0, 0, 0, 0, -1};
-      nonascii |= c.nonascii;
+      uint64_t tok = 0;
+      Seg seg{0, 0};
+      int64_t dl = -1;
+      for (uint64_t i = i0; i < i1; ++i) {
+        const CsvBlock c = blk[i];
+        tok += c.ntok;
+        seg = SegOp{}(seg, Seg{c.has_nl, c.has_nl ? c.since : c.ntok});
+        dl = c.last_delim > dl ? c.last_delim : dl;
+        nonascii |= c.nonascii;
+      }
       uint64_t tsum = 0;
       int64_t dtot = -1;
       Seg stot{0, 0};
-      const uint64_t ex_tok = wg_exclusive<uint64_t>(c.ntok, 0ull, SumOp{}, lds_u, &tsum);
-      const int64_t ex_delim = wg_exclusive<int64_t>(c.last_delim, -1ll, MaxOp{}, lds_i, &dtot);
-      const Seg ex_seg = wg_exclusive<Seg>(Seg{c.has_nl, c.has_nl ? c.since : c.ntok}, Seg{0, 0}, SegOp{}, lds_s,
-                                           &stot);
-      if (live) {
+      const uint64_t ex_tok = wg_exclusive<uint64_t>(tok, 0ull, SumOp{}, lds_u, &tsum);
+      const int64_t ex_delim = wg_exclusive<int64_t>(dl, -1ll, MaxOp{}, lds_i, &dtot);
+      const Seg ex_seg = wg_exclusive<Seg>(seg, Seg{0, 0}, SegOp{}, lds_s, &stot);
+      uint64_t t = carry_tok + ex_tok;
+      Seg sg = SegOp{}(carry_seg, ex_seg);
+      int64_t pd = ex_delim > carry_delim ? ex_delim : carry_delim;
+      for (uint64_t i = i0; i < i1; ++i) {
+        const CsvBlock c = blk[i];
         CsvPre p;
-        p.tok_base = carry_tok + ex_tok;
+        p.tok_base = t;
         p.ok_base = 0;
-        p.line_idx = ex_seg.reset ? ex_seg.v : carry_seg.v + ex_seg.v;
+        p.line_idx = sg.v;
         p.ok = 0;
-        p.prev_delim = ex_delim > carry_delim ? ex_delim : carry_delim;
+        p.prev_delim = pd;
         pre[i] = p;
+        t += c.ntok;
+        sg = SegOp{}(sg, Seg{c.has_nl, c.has_nl ? c.since : c.ntok});
+        pd = c.last_delim > pd ? c.last_delim : pd;
       }
       carry_tok += tsum;
       carry_seg = SegOp{}(carry_seg, stot);
       carry_delim = dtot > carry_delim ? dtot : carry_delim;
     } else {
-      const uint64_t ok = live ? pre[i].ok : 0;
+      uint64_t ok = 0;
+      for (uint64_t i = i0; i < i1; ++i) ok += pre[i].ok;
       uint64_t tsum = 0;
-      const uint64_t ex = wg_exclusive<uint64_t>(ok, 0ull, SumOp{}, lds_u, &tsum);
-      if (live) pre[i].ok_base = carry_tok + ex;
+      uint64_t acc = carry_tok + wg_exclusive<uint64_t>(ok, 0ull, SumOp{}, lds_u, &tsum);
+      for (uint64_t i = i0; i < i1; ++i) {
+        pre[i].ok_base = acc;
+        acc += pre[i].ok;
+      }
       carry_tok += tsum;
     }
   }
@@ -449,14 +488,30 @@ __global__ __launch_bounds__(256) void k_csv_parse(const uint8_t* __restrict__ t
                                                    CsvPre* __restrict__ pre, double* __restrict__ dst,
                                                    CsvSlow* __restrict__ slow, uint64_t slow_cap,
                                                    unsigned long long* __restrict__ totals) {
-  __shared__ uint64_t lds_u[256];
-  __shared__ int64_t lds_i[256];
-  __shared__ Seg lds_s[256];
+  __shared__ uint64_t lds_u[4];
+  __shared__ int64_t lds_i[4];
+  __shared__ Seg lds_s[4];
   const uint64_t b = blockIdx.x;
   const CsvPre bp = pre[b];
   const uint64_t at = b * kCsvBlock + 16ull * threadIdx.x;
   uint8_t by[16];
   load16(text, len, at, by);
+  // the block and the 512 bytes before it in LDS: items are parsed from there
+  constexpr uint32_t kHalo = 512;
+  __shared__ __align__(16) uint8_t txt[kHalo + kCsvBlock];
+  const uint64_t blk0 = b * kCsvBlock;
+  {
+    uint32_t w4[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 16; ++j) w4[j / 4] |= (uint32_t)by[j] << (8 * (j % 4));
+    *reinterpret_cast<uint4*>(txt + kHalo + 16 * threadIdx.x) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+    if (b > 0 && threadIdx.x < kHalo / 16)
+      *reinterpret_cast<uint4*>(txt + 16 * threadIdx.x) =
+          *reinterpret_cast<const uint4*>(text + blk0 - kHalo + 16 * threadIdx.x);
+  }
+  __syncthreads();
+  const uint64_t staged = b > 0 ? blk0 - kHalo : 0;  // first byte held in LDS
+  const uint8_t* lds_at = txt + kHalo;                 // LDS address of byte blk0
   const LaneFacts f = lane_facts(by, at, len, delim);
   int64_t dummy_i = 0;
   Seg dummy_s{0, 0};
@@ -468,33 +523,41 @@ __global__ __launch_bounds__(256) void k_csv_parse(const uint8_t* __restrict__ t
   int64_t prev = prev0;
   __shared__ double vals[256 * 16];  // this lane's values at [16 * lane, +16)
   uint32_t okmask = 0, nok = 0, undec = 0;
+  // the lane's item ends first, then one item per step: a wave takes as many
+  // parse steps as its busiest lane has items, not one per byte position
+  uint32_t ends = 0;
+#pragma unroll
   for (int j = 0; j < 16; ++j) {
     const uint64_t pos = at + j;
-    if (pos >= len) break;
-    const uint32_t c = by[j];
-    const bool dl = csv_delim(c, delim);
-    const bool eof = pos + 1 == len && !dl;
-    if (dl || eof) {
-      const uint64_t start = (uint64_t)(prev + 1);
-      uint64_t end = eof ? len : pos;
-      if (c == '\n' && end > start && text[end - 1] == '\r') end -= 1;  // lines() drops "\r\n"
-      if (li >= skip) {
-        double v = 0.0;
-        const int r = parse_item(text + start, end - start, tab, &v);
-        if (r == 1) {
-          vals[16 * threadIdx.x + j] = v;
-          okmask |= 1u << j;
-          ++nok;
-        } else if (r == 2) {  // a valid item: its exact value is filled in by k_csv_slow
-          vals[16 * threadIdx.x + j] = 0.0;
-          okmask |= 1u << j;
-          undec |= 1u << j;
-          ++nok;
-        }
+    if (pos < len && (csv_delim(by[j], delim) || pos + 1 == len)) ends |= 1u << j;
+  }
+  const uint8_t* lane_txt = txt + kHalo + 16 * threadIdx.x;
+  while (ends) {
+    const int j = __builtin_ctz(ends);
+    ends &= ends - 1;
+    const uint64_t pos = at + j;
+    const uint32_t c = lane_txt[j];
+    const bool eof = pos + 1 == len && !csv_delim(c, delim);
+    const uint64_t start = (uint64_t)(prev + 1);
+    uint64_t end = eof ? len : pos;
+    if (c == '\n' && end > start && lds_at[(int64_t)(end - 1) - (int64_t)blk0] == '\r') end -= 1;  // "\r\n"
+    if (li >= skip) {
+      double v = 0.0;
+      const uint8_t* src = start >= staged ? lds_at + ((int64_t)start - (int64_t)blk0) : text + start;
+      const int r = parse_item(src, end - start, tab, &v);
+      if (r == 1) {
+        vals[16 * threadIdx.x + j] = v;
+        okmask |= 1u << j;
+        ++nok;
+      } else if (r == 2) {  // a valid item: its exact value is filled in by k_csv_slow
+        vals[16 * threadIdx.x + j] = 0.0;
+        okmask |= 1u << j;
+        undec |= 1u << j;
+        ++nok;
       }
-      prev = (int64_t)pos;
-      li = c == '\n' ? 0 : li + 1;
     }
+    prev = (int64_t)pos;
+    li = c == '\n' ? 0 : li + 1;
   }
   uint64_t ok_tot = 0;
   const uint64_t rank = wg_exclusive<uint64_t>(nok, 0ull, SumOp{}, lds_u, &ok_tot);
