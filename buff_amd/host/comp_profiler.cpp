@@ -3,62 +3,32 @@
 //
 //   comp_profiler <input_file> <method> <int_scale> <pred>
 //
-// method "buff" runs the sequence of run_buff_encoding_decoding_mybitvec
-// (buff_simd.rs:167-243): encode, decode, range filter, equality filter, sum,
-// max. "buff-simd" runs run_buff_simd_encoding_decoding (buff_simd.rs:28-95):
-// the flipped-layout encode, decode and filters; its fifth and sixth timings
-// are the reference's fused "BS way" filters, which compute the same bitmaps
-// as the plain ones and are timed here with the same GPU filters. The input is
-// read like construct_file_iterator_skip_newline (client.rs:163-182): every
-// comma-separated field of every line that parses as f64. The last line is the
-// reference's `Performance:` line: compression ratio and MiB/s of the bincode
-// Segment size per operator (buff_simd.rs:232-242).
+// The input is read by construct_file_iterator_skip_newline::<f64>(file, 0, ',')
+// (client.rs:163-182), parsed on the GPU. Each method times the six operations
+// of its reference driver and ends with the reference's `Performance:` line:
+// compression ratio and MiB/s of the bincode Segment size per operation.
+//   buff              run_buff_encoding_decoding_mybitvec (buff_simd.rs:167-243):
+//                     encode, decode, range filter, equality filter, sum, max;
+//   buff-simd         run_buff_simd_encoding_decoding (buff_simd.rs:28-95): the
+//                     flipped layout's encode, decode and filters, then the
+//                     "BS way" _with_slice filters;
+//   buff-slice        run_buff_slice_encoding_decoding (buff_slice.rs:30-101):
+//                     slice encode, decode, range and equality filters, buffsum,
+//                     buffmax;
+//   buff-slice-scalar run_buff_slice_scalar_encoding_decoding (buff_slice.rs:
+//                     103-171): encode, decode_bs, the nosimd filters, sum, max;
+//   buff-major        run_buff_majority_encoding_decoding (buff_simd.rs:246-311):
+//                     majority encode, decode, filters, sum, max.
 #include <chrono>
-#include <cerrno>
 #include <cstdio>
 #include <cstdlib>
-#include <cstring>
-#include <fstream>
 #include <functional>
-#include <iostream>
-#include <sstream>
 #include <string>
 #include <vector>
 
 #include "split_double.hpp"
 
 namespace {
-
-// Rust's str::parse::<f64> accepts an optional sign, digits with an optional
-// dot and exponent, "inf"/"infinity"/"nan" (any case); no surrounding spaces.
-bool parse_f64(const std::string& s, double* out) {
-  if (s.empty()) return false;
-  for (char ch : s)
-    if (ch == ' ' || ch == '\t' || ch == '\r' || ch == '\n' || ch == '_') return false;
-  if (s.size() > 1 && (s[0] == '0') && (s[1] == 'x' || s[1] == 'X')) return false;
-  char* end = nullptr;
-  errno = 0;
-  const double v = std::strtod(s.c_str(), &end);
-  if (end != s.c_str() + s.size()) return false;
-  *out = v;
-  return true;
-}
-
-std::vector<double> read_csv(const char* path) {
-  std::ifstream f(path);
-  if (!f) throw std::runtime_error(std::string("cannot open ") + path);
-  std::vector<double> v;
-  std::string line;
-  while (std::getline(f, line)) {
-    std::stringstream ss(line);
-    std::string field;
-    while (std::getline(ss, field, ',')) {
-      double x;
-      if (parse_f64(field, &x)) v.push_back(x);
-    }
-  }
-  return v;
-}
 
 double timed_ns(const std::function<void()>& fn) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -71,7 +41,8 @@ double timed_ns(const std::function<void()>& fn) {
 
 int main(int argc, char** argv) {
   if (argc < 5) {
-    std::fprintf(stderr, "usage: %s <input_file> <buff|buff-simd> <int_scale> <pred>\n", argv[0]);
+    std::fprintf(stderr, "usage: %s <input_file> <buff|buff-simd|buff-slice|buff-slice-scalar|buff-major> "
+                         "<int_scale> <pred>\n", argv[0]);
     return 2;
   }
   const char* file = argv[1];
@@ -80,35 +51,50 @@ int main(int argc, char** argv) {
   const double pred = std::strtod(argv[4], nullptr);
   std::printf("ARGS: %s, %s, %llu, %s, \n", file, method.c_str(), (unsigned long long)scl, argv[4]);
   try {
-    buff::Segment seg{read_csv(file)};
+    buff::Segment seg{buff::construct_file_iterator_skip_newline(file, 0, ',')};
     const double org_size = (double)seg.byte_size();
-    buff::SplitBDDoubleCompress comp(10, 10, scl);
-    const bool simd = method == "buff-simd";
-    if (!simd && method != "buff") throw std::runtime_error("Compression not supported yet.");
-    std::vector<uint8_t> compressed;
+    buff::BuffSliceCompress comp(10, 10, scl);
+    std::vector<uint8_t> c;
     double d[6];
-    d[0] = timed_ns([&] { compressed = simd ? comp.buff_simd256_encode(seg) : comp.byte_fixed_encode(seg); });
-    const double comp_size = (double)compressed.size();
-    d[1] = timed_ns([&] {
-      if (simd) comp.buff_simd256_decode(compressed);
-      else comp.byte_fixed_decode(compressed);
-    });
-    d[2] = timed_ns([&] {
-      if (simd) comp.buff_simd_range_filter(compressed, pred);
-      else comp.buff_range_filter(compressed, pred);
-    });
-    d[3] = timed_ns([&] {
-      if (simd) comp.buff_simd_equal_filter(compressed, pred);
-      else comp.buff_equal_filter(compressed, pred);
-    });
-    if (simd) {
-      d[4] = timed_ns([&] { comp.buff_simd_range_filter(compressed, pred); });
-      d[5] = timed_ns([&] { comp.buff_simd_equal_filter(compressed, pred); });
+    if (method == "buff") {
+      d[0] = timed_ns([&] { c = comp.byte_fixed_encode(seg); });
+      d[1] = timed_ns([&] { comp.byte_fixed_decode(c); });
+      d[2] = timed_ns([&] { comp.buff_range_filter(c, pred); });
+      d[3] = timed_ns([&] { comp.buff_equal_filter(c, pred); });
+      d[4] = timed_ns([&] { std::printf("sum is: %.17g\n", comp.byte_fixed_sum(c)); });
+      d[5] = timed_ns([&] { comp.byte_fixed_max(c); });
+    } else if (method == "buff-simd") {
+      d[0] = timed_ns([&] { c = comp.buff_simd256_encode(seg); });
+      d[1] = timed_ns([&] { comp.buff_simd256_decode(c); });
+      d[2] = timed_ns([&] { comp.buff_simd_range_filter(c, pred); });
+      d[3] = timed_ns([&] { comp.buff_simd_equal_filter(c, pred); });
+      d[4] = timed_ns([&] { comp.buff_simd_range_filter_with_slice(c, pred); });
+      d[5] = timed_ns([&] { comp.buff_simd_equal_filter_with_slice(c, pred); });
+    } else if (method == "buff-slice") {
+      d[0] = timed_ns([&] { c = comp.buff_slice_encode(seg); });
+      d[1] = timed_ns([&] { comp.buff_slice_decode(c); });
+      d[2] = timed_ns([&] { comp.buff_slice_range_filter(c, pred); });
+      d[3] = timed_ns([&] { comp.buff_slice_equal_filter(c, pred); });
+      d[4] = timed_ns([&] { comp.buff_slice_buffsum(c); });
+      d[5] = timed_ns([&] { comp.buff_slice_buffmax(c); });
+    } else if (method == "buff-slice-scalar") {
+      d[0] = timed_ns([&] { c = comp.buff_slice_encode(seg); });
+      d[1] = timed_ns([&] { comp.buff_slice_decode_bs(c); });
+      d[2] = timed_ns([&] { comp.buff_slice_range_filter_nosimd(c, pred); });
+      d[3] = timed_ns([&] { comp.buff_slice_equal_filter_nosimd(c, pred); });
+      d[4] = timed_ns([&] { comp.buff_slice_sum(c); });
+      d[5] = timed_ns([&] { comp.buff_slice_max(c); });
+    } else if (method == "buff-major") {
+      d[0] = timed_ns([&] { c = comp.buff_encode_majority(seg); });
+      d[1] = timed_ns([&] { comp.buff_decode_majority(c); });
+      d[2] = timed_ns([&] { comp.buff_range_filter_majority(c, pred); });
+      d[3] = timed_ns([&] { comp.buff_equal_filter_majority(c, pred); });
+      d[4] = timed_ns([&] { std::printf("sum is: %.17g\n", comp.buff_sum_majority(c)); });
+      d[5] = timed_ns([&] { comp.buff_max_majority(c); });
     } else {
-      d[4] = timed_ns([&] { std::printf("sum is: %.17g\n", comp.byte_fixed_sum(compressed)); });
-      d[5] = timed_ns([&] { comp.byte_fixed_max(compressed); });
+      throw buff::Panic(BUFF_E_REF_PANIC, "Compression not supported yet.");
     }
-    std::printf("Performance:%s,%llu,%s,%.17g", file, (unsigned long long)scl, argv[4], comp_size / org_size);
+    std::printf("Performance:%s,%llu,%s,%.17g", file, (unsigned long long)scl, argv[4], (double)c.size() / org_size);
     for (double ns : d) std::printf(",%.17g", 1e9 * org_size / ns / 1024.0 / 1024.0);
     std::printf("\n");
   } catch (const buff::Panic& e) {

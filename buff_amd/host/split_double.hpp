@@ -122,10 +122,55 @@ class SplitBDDoubleCompress {
     return m;
   }
 
- private:
+  // buff_simd.rs:3736 / 3888: the "BS way" filters over a flipped stream
+  void buff_simd_range_filter_with_slice(std::vector<uint8_t> bytes, double pred) const {
+    std::printf("Number of qualified int items:%llu\n",
+                (unsigned long long)filter(bytes, pred, buff_sd_buff_simd_range_filter_with_slice).count);
+  }
+  void buff_simd_equal_filter_with_slice(std::vector<uint8_t> bytes, double pred) const {
+    std::printf("Number of qualified int items:%llu\n",
+                (unsigned long long)filter(bytes, pred, buff_sd_buff_simd_equal_filter_with_slice).count);
+  }
+
+  // buff-major (buff_simd.rs:317-2861)
+  std::vector<uint8_t> buff_encode_majority(const Segment& seg) const {
+    uint64_t len = 0;
+    check(buff_sd_buff_encode_majority(&sd_, seg.data.data(), seg.data.size(), nullptr, 0, &len));
+    std::vector<uint8_t> out(len);
+    check(buff_sd_buff_encode_majority(&sd_, seg.data.data(), seg.data.size(), out.data(), out.size(), &len));
+    out.resize(len);
+    return out;
+  }
+  std::vector<double> buff_decode_majority(std::vector<uint8_t> bytes) const {
+    std::vector<double> out(rows(bytes));
+    uint64_t n = 0;
+    check(buff_sd_buff_decode_majority(&sd_, bytes.data(), bytes.size(), out.data(), out.size(), &n));
+    out.resize(n);
+    return out;
+  }
+  void buff_range_filter_majority(std::vector<uint8_t> bytes, double pred) const {
+    std::printf("Number of qualified items:%llu\n",
+                (unsigned long long)filter(bytes, pred, buff_sd_buff_range_filter_majority).count);
+  }
+  void buff_equal_filter_majority(std::vector<uint8_t> bytes, double pred) const {
+    std::printf("Number of qualified items:%llu\n",
+                (unsigned long long)filter(bytes, pred, buff_sd_buff_equal_filter_majority).count);
+  }
+  double buff_sum_majority(std::vector<uint8_t> bytes) const {
+    double s = 0;
+    check(buff_sd_buff_sum_majority(&sd_, bytes.data(), bytes.size(), &s));
+    return s;
+  }
+  void buff_max_majority(std::vector<uint8_t> bytes) const {
+    double m = 0;
+    check(buff_sd_buff_max_majority(&sd_, bytes.data(), bytes.size(), &m));
+    std::printf("Max value:%.17g\n", m);
+  }
+
+ protected:
   buff_split_double sd_;
 
-  static uint64_t rows(const std::vector<uint8_t>& b) {
+  static uint64_t rows(const std::vector<uint8_t>& b) {  // the u32 row count of the header
     if (b.size() < 20) throw Error(BUFF_E_INVALID, "stream shorter than its 20-byte header");
     return (uint64_t)b[8] | ((uint64_t)b[9] << 8) | ((uint64_t)b[10] << 16) | ((uint64_t)b[11] << 24);
   }
@@ -148,5 +193,86 @@ class SplitBDDoubleCompress {
     return bm;
   }
 };
+
+// database/src/compress/buff_slice.rs:222-1970 (BuffSliceCompress: the same
+// three fields). Printing methods print what the reference prints.
+class BuffSliceCompress : public SplitBDDoubleCompress {
+ public:
+  using SplitBDDoubleCompress::SplitBDDoubleCompress;
+
+  std::vector<uint8_t> buff_slice_encode(const Segment& seg) const {  // :298
+    uint64_t len = 0;
+    check(buff_sd_buff_slice_encode(&sd_, seg.data.data(), seg.data.size(), nullptr, 0, &len));
+    std::vector<uint8_t> out(len);
+    check(buff_sd_buff_slice_encode(&sd_, seg.data.data(), seg.data.size(), out.data(), out.size(), &len));
+    out.resize(len);
+    return out;
+  }
+  std::vector<double> buff_slice_decode(std::vector<uint8_t> bytes) const {  // :439
+    std::vector<double> out(rows(bytes));
+    check(buff_sd_buff_slice_decode(&sd_, bytes.data(), bytes.size(), out.data(), out.size()));
+    return out;
+  }
+  std::vector<double> buff_slice_decode_bs(std::vector<uint8_t> bytes) const {  // :1184
+    std::vector<double> out(rows(bytes));
+    check(buff_sd_buff_slice_decode_bs(&sd_, bytes.data(), bytes.size(), out.data(), out.size()));
+    return out;
+  }
+  void buff_slice_range_filter(std::vector<uint8_t> bytes, double pred) const {  // :1348
+    print_count(filter(bytes, pred, buff_sd_buff_slice_range_filter));
+  }
+  void buff_slice_equal_filter(std::vector<uint8_t> bytes, double pred) const {  // :1665
+    print_count(filter(bytes, pred, buff_sd_buff_slice_equal_filter));
+  }
+  void buff_slice_range_filter_nosimd(std::vector<uint8_t> bytes, double pred) const {  // :1753
+    print_count(filter(bytes, pred, buff_sd_buff_slice_range_filter_nosimd));
+  }
+  void buff_slice_equal_filter_nosimd(std::vector<uint8_t> bytes, double pred) const {  // :1873
+    print_count(filter(bytes, pred, buff_sd_buff_slice_equal_filter_nosimd));
+  }
+  void buff_slice_buffsum(std::vector<uint8_t> bytes) const {  // :702
+    double s = 0;
+    check(buff_sd_buff_slice_buffsum(&sd_, bytes.data(), bytes.size(), &s));
+    std::printf("sum is:%.17g\n", s);
+  }
+  void buff_slice_sum(std::vector<uint8_t> bytes) const {  // :1101
+    double s = 0;
+    check(buff_sd_buff_slice_sum(&sd_, bytes.data(), bytes.size(), &s));
+    std::printf("sum is:%.17g\n", s);
+  }
+  void buff_slice_buffmax(std::vector<uint8_t> bytes) const {  // :548
+    max_line(bytes, buff_sd_buff_slice_buffmax);
+  }
+  void buff_slice_max(std::vector<uint8_t> bytes) const {  // :816
+    max_line(bytes, buff_sd_buff_slice_max);
+  }
+
+ private:
+  static void print_count(const Bitmap& bm) {
+    std::printf("Number of qualified items:%llu\n", (unsigned long long)bm.count);
+  }
+  template <typename Fn>
+  void max_line(const std::vector<uint8_t>& bytes, Fn fn) const {
+    double m = 0;
+    uint64_t cnt = 0;
+    check(fn(&sd_, bytes.data(), bytes.size(), &m, &cnt));
+    std::printf("Number of qualified max items:%llu\nMax value:%.17g\n", (unsigned long long)cnt, m);
+  }
+};
+
+// database/src/client.rs:163-182, construct_file_iterator_skip_newline::<f64>,
+// parsed on the GPU; the reference's Err(()) (a file it cannot open) is thrown.
+inline std::vector<double> construct_file_iterator_skip_newline(const char* file, uint32_t skip_val, char delim) {
+  std::FILE* f = std::fopen(file, "rb");
+  if (!f) throw Error(BUFF_E_INVALID, std::string("cannot open ") + file);
+  std::fseek(f, 0, SEEK_END);
+  const long size = std::ftell(f);
+  std::fclose(f);
+  std::vector<double> out((size > 0 ? (uint64_t)size / 2 : 0) + 1);
+  uint64_t n = 0;
+  check(buff_construct_file_iterator_skip_newline(file, skip_val, (uint8_t)delim, out.data(), out.size(), &n));
+  out.resize(n);
+  return out;
+}
 
 }  // namespace buff

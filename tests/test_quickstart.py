@@ -55,7 +55,7 @@ def test_gpu_quickstart(gpu_ctx, oracle, data):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("method", ["buff", "buff-simd"])
+@pytest.mark.parametrize("method", ["buff", "buff-simd", "buff-slice", "buff-slice-scalar", "buff-major"])
 def test_comp_profiler_cli(tmp_path, oracle, data, method):
     """The C++ host mirror + CLI prints what the reference driver prints."""
     from tests.conftest import ROOT
@@ -67,10 +67,34 @@ def test_comp_profiler_cli(tmp_path, oracle, data, method):
     out = subprocess.run([exe, str(csv), method, str(SCALE), str(PRED)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     s = oracle.encode(data, SCALE)
+    perf = [ln for ln in out.stdout.splitlines() if ln.startswith("Performance:")]
+    assert len(perf) == 1 and len(perf[0].split(",")) == 10
+    if method.startswith("buff-slice"):
+        sl = oracle.slice_encode(data, SCALE)
+        kinds = ("range_nosimd", "equal_nosimd") if method.endswith("scalar") else ("range", "equal")
+        for kind in kinds:
+            assert "Number of qualified items:%d" % oracle.slice_filter(sl, SCALE, kind, PRED)[1] in out.stdout
+        mx, cnt, _ = oracle.slice_max(sl, method.endswith("scalar"))
+        assert "Number of qualified max items:%d" % cnt in out.stdout
+        m = re.search(r"Max value:(\S+)", out.stdout)
+        assert m and float(m.group(1)) == mx
+        m = re.search(r"sum is:(\S+)", out.stdout)
+        vals = oracle.slice_decode(sl)
+        assert m and abs(float(m.group(1)) - oracle.slice_sum(sl)) <= 1e-12 * np.abs(vals).sum()
+        assert abs(float(perf[0].split(",")[3]) - len(sl) / (31 + 8 * len(data))) < 1e-12
+        return
+    if method == "buff-major":
+        mj = oracle.major_encode(data, SCALE)
+        assert "Number of qualified items:%d" % oracle.major_filter(mj, SCALE, "range", PRED)[1] in out.stdout
+        assert abs(float(perf[0].split(",")[3]) - len(mj) / (31 + 8 * len(data))) < 1e-12
+        return
     c_range = oracle.buff_filter(s, SCALE, "range", PRED)[1]
     c_eq = oracle.buff_filter(s, SCALE, "equal", PRED)[1]
     assert "Number of qualified items:%d" % c_range in out.stdout
     assert "Number of qualified int items:%d" % c_eq in out.stdout
+    if method == "buff-simd":
+        wsl = oracle.slice_filter(oracle.encode(data, SCALE, flip=True), SCALE, "range_with_slice", PRED)[1]
+        assert "Number of qualified int items:%d" % wsl in out.stdout
     if method == "buff":
         m = re.search(r"sum is: (\S+)", out.stdout)
         assert m and float(m.group(1)) == oracle.byte_fixed_sum(s, SCALE)[0]
