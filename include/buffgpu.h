@@ -137,8 +137,9 @@ int buff_encode(buff_ctx* ctx, const double* x_host, uint64_t n, uint64_t scale,
 int buff_chunk_info_get(const buff_chunk* chunk, buff_chunk_info* info);
 /* Writes the reference byte stream. dst == NULL: only *len is set. */
 int buff_chunk_serialize(buff_ctx* ctx, const buff_chunk* chunk, uint8_t* dst, uint64_t* len);
-/* Parses a reference byte stream (BUFF_FLIP: a buff_simd256 stream) into the
- * resident form. `scale` is the caller's SplitBDDoubleCompress scale. */
+/* Parses a reference byte stream (BUFF_FLIP: a buff_simd256 stream; BUFF_SLICE:
+ * a buff_slice_encode stream) into the resident form. `scale` is the caller's
+ * SplitBDDoubleCompress scale. */
 int buff_chunk_load(buff_ctx* ctx, const uint8_t* stream, uint64_t len, uint64_t scale, uint32_t flags,
                     buff_chunk** out);
 int buff_chunk_free(buff_chunk* chunk);
@@ -149,7 +150,16 @@ int buff_chunk_sparse_col(const buff_chunk* chunk, uint32_t j, uint32_t* major, 
 
 /* ---- queries on a resident chunk ------------------------------------------ */
 /* Filter into host words: ceil(n/32) u32 words; *count = set bits (the count the
- * reference prints). */
+ * reference prints). On a BUFF_SLICE chunk these and the calls below follow
+ * BuffSliceCompress: buff_filter is buff_slice_range_filter / _equal_filter
+ * (BUFF_SCALAR: the _nosimd forms), buff_filter_cond is
+ * buff_slice_range_filter_condition (BUFF_OP_GT) or
+ * buff_slice_range_smaller_filter_condition (BUFF_OP_LT), buff_materialize is
+ * buff_slice_decode (BUFF_SCALAR: decode_bs), buff_materialize_cond is
+ * buff_slice_decode_condition, buff_sum_ex / buff_max_ex are buffsum / buffmax
+ * (BUFF_SCALAR: sum / max), buff_max_range is buff_slice_max_range. With
+ * BUFF_WITH_SLICE on a byte_fixed chunk, buff_filter is
+ * buff_simd_range_filter_with_slice / buff_simd_equal_filter_with_slice. */
 int buff_filter(buff_ctx* ctx, const buff_chunk* chunk, int op, double pred, uint32_t flags, uint32_t* words,
                 uint64_t nwords, uint64_t* count);
 /* Filter into device memory (ceil(n/64) u64 words, 8-byte aligned); no host sync.
