@@ -873,7 +873,31 @@ def main():
         result["config5"] = bench_config5(ctx, args, buff_amd)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_rows)
+        result["slice"]["cpu_port"] = cpu_slice(1 << 24)
     print(json.dumps(result), flush=True)
+
+
+def cpu_slice(rows):
+    """The oracle's restatement of the byte-slice path on one core, on a bounded
+    sample of the walk column: buff_slice_encode, buff_slice_range_filter at
+    10%, buff_slice_decode. GB/s of float64 input per operation."""
+    from buff_amd import datagen
+    from oracle import refcpu
+
+    refcpu.build()
+    x = datagen.random_walk(rows, 4, 2)
+    res = {"rows": rows, "cores": 1, "kind": "port"}
+    t = time.perf_counter()
+    s = refcpu.slice_encode(x, 10 ** 4)
+    res["encode_GBps"] = round(8.0 * rows / (time.perf_counter() - t) / 1e9, 4)
+    pred = float(np.quantile(x, 0.90, method="lower"))
+    t = time.perf_counter()
+    refcpu.slice_filter(s, 10 ** 4, "range", pred)
+    res["range_10pct_GBps"] = round(8.0 * rows / (time.perf_counter() - t) / 1e9, 4)
+    t = time.perf_counter()
+    refcpu.slice_decode(s)
+    res["decode_GBps"] = round(8.0 * rows / (time.perf_counter() - t) / 1e9, 4)
+    return res
 
 
 def cpu_baseline(rows):
