@@ -45,12 +45,12 @@ uint64_t csv_scratch_bytes(uint64_t nblocks);
 hipError_t launch_csv_count(const uint8_t* text, uint64_t len, uint8_t delim, uint8_t* scratch, uint64_t nblocks,
                             hipStream_t s);
 hipError_t launch_csv_parse(const uint8_t* text, uint64_t len, uint8_t delim, uint32_t skip, const Pow5* tab,
-                            uint8_t* scratch, uint64_t nblocks, double* dst, CsvSlow* slow, uint64_t slow_cap,
-                            hipStream_t s);
+                            uint8_t* scratch, uint64_t nblocks, double* dst, uint8_t* okb, CsvSlow* slow,
+                            uint64_t slow_cap, hipStream_t s);
 hipError_t launch_csv_slow(const uint8_t* text, const CsvSlow* slow, uint64_t count, uint8_t* scratch,
                            uint64_t nblocks, double* dst, hipStream_t s);
-hipError_t launch_csv_compact(const uint8_t* scratch, uint64_t nblocks, const double* src, double* out,
-                              hipStream_t s);
+hipError_t launch_csv_compact(const uint8_t* scratch, uint64_t nblocks, const uint8_t* okb, const double* src,
+                              double* out, hipStream_t s);
 // totals pointer inside the scratch
 uint64_t csv_totals_offset(uint64_t nblocks);
 void csv_pow5_table(Pow5* out);  // host: the 651 entries, q = -342 .. 308

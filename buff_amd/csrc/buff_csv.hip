@@ -421,11 +421,16 @@ __global__ __launch_bounds__(256) void k_csv_scan(const CsvBlock* __restrict__ b
     const uint64_t i0 = base + (uint64_t)threadIdx.x * kPer;
     const uint64_t i1 = i0 + kPer < nblocks ? i0 + kPer : nblocks;
     if (mode == 0) {
+      // the lane's blocks in registers: all loads issued before any is used
+      CsvBlock cb[kPer];
+#pragma unroll
+      for (uint32_t q = 0; q < kPer; ++q) cb[q] = i0 + q < i1 ? blk[i0 + q] : CsvBlock{0, 0, 0, 0, -1};
       uint64_t tok = 0;
       Seg seg{0, 0};
       int64_t dl = -1;
-      for (uint64_t i = i0; i < i1; ++i) {
-        const CsvBlock c = blk[i];
+#pragma unroll
+      for (uint32_t q = 0; q < kPer; ++q) {
+        const CsvBlock& c = cb[q];
         tok += c.ntok;
         seg = SegOp{}(seg, Seg{c.has_nl, c.has_nl ? c.since : c.ntok});
         dl = c.last_delim > dl ? c.last_delim : dl;
@@ -440,15 +445,18 @@ __global__ __launch_bounds__(256) void k_csv_scan(const CsvBlock* __restrict__ b
       uint64_t t = carry_tok + ex_tok;
       Seg sg = SegOp{}(carry_seg, ex_seg);
       int64_t pd = ex_delim > carry_delim ? ex_delim : carry_delim;
-      for (uint64_t i = i0; i < i1; ++i) {
-        const CsvBlock c = blk[i];
-        CsvPre p;
-        p.tok_base = t;
-        p.ok_base = 0;
-        p.line_idx = sg.v;
-        p.ok = 0;
-        p.prev_delim = pd;
-        pre[i] = p;
+#pragma unroll
+      for (uint32_t q = 0; q < kPer; ++q) {
+        const CsvBlock& c = cb[q];
+        if (i0 + q < i1) {
+          CsvPre p;
+          p.tok_base = t;
+          p.ok_base = 0;
+          p.line_idx = sg.v;
+          p.ok = 0;
+          p.prev_delim = pd;
+          pre[i0 + q] = p;
+        }
         t += c.ntok;
         sg = SegOp{}(sg, Seg{c.has_nl, c.has_nl ? c.since : c.ntok});
         pd = c.last_delim > pd ? c.last_delim : pd;
@@ -457,13 +465,18 @@ __global__ __launch_bounds__(256) void k_csv_scan(const CsvBlock* __restrict__ b
       carry_seg = SegOp{}(carry_seg, stot);
       carry_delim = dtot > carry_delim ? dtot : carry_delim;
     } else {
+      uint32_t oks[kPer];
+#pragma unroll
+      for (uint32_t q = 0; q < kPer; ++q) oks[q] = i0 + q < i1 ? pre[i0 + q].ok : 0u;
       uint64_t ok = 0;
-      for (uint64_t i = i0; i < i1; ++i) ok += pre[i].ok;
+#pragma unroll
+      for (uint32_t q = 0; q < kPer; ++q) ok += oks[q];
       uint64_t tsum = 0;
       uint64_t acc = carry_tok + wg_exclusive<uint64_t>(ok, 0ull, SumOp{}, lds_u, &tsum);
-      for (uint64_t i = i0; i < i1; ++i) {
-        pre[i].ok_base = acc;
-        acc += pre[i].ok;
+#pragma unroll
+      for (uint32_t q = 0; q < kPer; ++q) {
+        if (i0 + q < i1) pre[i0 + q].ok_base = acc;
+        acc += oks[q];
       }
       carry_tok += tsum;
     }
@@ -486,8 +499,8 @@ __global__ __launch_bounds__(256) void k_csv_scan(const CsvBlock* __restrict__ b
 __global__ __launch_bounds__(256) void k_csv_parse(const uint8_t* __restrict__ text, uint64_t len, uint32_t delim,
                                                    uint32_t skip, const Pow5* __restrict__ tab,
                                                    CsvPre* __restrict__ pre, double* __restrict__ dst,
-                                                   CsvSlow* __restrict__ slow, uint64_t slow_cap,
-                                                   unsigned long long* __restrict__ totals) {
+                                                   uint8_t* __restrict__ okb, CsvSlow* __restrict__ slow,
+                                                   uint64_t slow_cap, unsigned long long* __restrict__ totals) {
   __shared__ uint64_t lds_u[4];
   __shared__ int64_t lds_i[4];
   __shared__ Seg lds_s[4];
@@ -513,18 +526,19 @@ __global__ __launch_bounds__(256) void k_csv_parse(const uint8_t* __restrict__ t
   const uint64_t staged = b > 0 ? blk0 - kHalo : 0;  // first byte held in LDS
   const uint8_t* lds_at = txt + kHalo;                 // LDS address of byte blk0
   const LaneFacts f = lane_facts(by, at, len, delim);
+  uint64_t dummy_u = 0;
   int64_t dummy_i = 0;
   Seg dummy_s{0, 0};
+  const uint64_t ex_tok = wg_exclusive<uint64_t>(f.ntok, 0ull, SumOp{}, lds_u, &dummy_u);
   const int64_t ex_delim = wg_exclusive<int64_t>(f.last_delim, -1ll, MaxOp{}, lds_i, &dummy_i);
-  const Seg ex_seg =
-      wg_exclusive<Seg>(Seg{f.has_nl, f.since}, Seg{0, 0}, SegOp{}, lds_s, &dummy_s);
+  const Seg ex_seg = wg_exclusive<Seg>(Seg{f.has_nl, f.since}, Seg{0, 0}, SegOp{}, lds_s, &dummy_s);
   uint32_t li = ex_seg.reset ? ex_seg.v : bp.line_idx + ex_seg.v;
-  const int64_t prev0 = ex_delim > bp.prev_delim ? ex_delim : bp.prev_delim;
-  int64_t prev = prev0;
-  __shared__ double vals[256 * 16];  // this lane's values at [16 * lane, +16)
-  uint32_t okmask = 0, nok = 0, undec = 0;
+  int64_t prev = ex_delim > bp.prev_delim ? ex_delim : bp.prev_delim;
+  uint64_t k = bp.tok_base + ex_tok;  // this lane's first item
+  uint32_t nok = 0;
   // the lane's item ends first, then one item per step: a wave takes as many
-  // parse steps as its busiest lane has items, not one per byte position
+  // parse steps as its busiest lane has items, not one per byte position.
+  // Each value goes to its item's slot; okb marks the items that parsed.
   uint32_t ends = 0;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
@@ -541,56 +555,33 @@ __global__ __launch_bounds__(256) void k_csv_parse(const uint8_t* __restrict__ t
     const uint64_t start = (uint64_t)(prev + 1);
     uint64_t end = eof ? len : pos;
     if (c == '\n' && end > start && lds_at[(int64_t)(end - 1) - (int64_t)blk0] == '\r') end -= 1;  // "\r\n"
+    uint8_t ok = 0;
     if (li >= skip) {
       double v = 0.0;
       const uint8_t* src = start >= staged ? lds_at + ((int64_t)start - (int64_t)blk0) : text + start;
       const int r = parse_item(src, end - start, tab, &v);
       if (r == 1) {
-        vals[16 * threadIdx.x + j] = v;
-        okmask |= 1u << j;
-        ++nok;
+        dst[k] = v;
+        ok = 1;
       } else if (r == 2) {  // a valid item: its exact value is filled in by k_csv_slow
-        vals[16 * threadIdx.x + j] = 0.0;
-        okmask |= 1u << j;
-        undec |= 1u << j;
-        ++nok;
+        ok = 1;
+        const unsigned long long e = atomicAdd(&totals[3], 1ull);
+        if (e < slow_cap) {
+          slow[e] = CsvSlow{start, end, k};
+        } else {
+          atomicAdd(&totals[4], 1ull);
+          atomicMin(&totals[5], (unsigned long long)start);
+        }
       }
     }
+    okb[k] = ok;
+    nok += ok;
+    ++k;
     prev = (int64_t)pos;
     li = c == '\n' ? 0 : li + 1;
   }
   uint64_t ok_tot = 0;
-  const uint64_t rank = wg_exclusive<uint64_t>(nok, 0ull, SumOp{}, lds_u, &ok_tot);
-  double* o = dst + bp.tok_base + rank;
-  for (int j = 0; j < 16; ++j)
-    if (okmask & (1u << j)) *o++ = vals[16 * threadIdx.x + j];
-  if (undec) {  // walk the lane's items again for the spans and slots of the undecided ones
-    int64_t pv = prev0;
-    uint64_t slot = bp.tok_base + rank;
-    for (int j = 0; j < 16; ++j) {
-      const uint64_t pos = at + j;
-      if (pos >= len) break;
-      const uint32_t c = by[j];
-      const bool dl = csv_delim(c, delim);
-      if (!(dl || pos + 1 == len)) continue;
-      if (okmask & (1u << j)) {
-        if (undec & (1u << j)) {
-          const uint64_t start = (uint64_t)(pv + 1);
-          uint64_t end = dl ? pos : len;
-          if (c == '\n' && end > start && text[end - 1] == '\r') end -= 1;
-          const unsigned long long e = atomicAdd(&totals[3], 1ull);
-          if (e < slow_cap) {
-            slow[e] = CsvSlow{start, end, slot};
-          } else {
-            atomicAdd(&totals[4], 1ull);
-            atomicMin(&totals[5], (unsigned long long)start);
-          }
-        }
-        ++slot;
-      }
-      pv = (int64_t)pos;
-    }
-  }
+  (void)wg_exclusive<uint64_t>(nok, 0ull, SumOp{}, lds_u, &ok_tot);
   if (threadIdx.x == 0) pre[b].ok = (uint32_t)ok_tot;
 }
 
@@ -773,10 +764,22 @@ __global__ __launch_bounds__(64) void k_csv_slow(const uint8_t* __restrict__ tex
   }
 }
 
-__global__ __launch_bounds__(256) void k_csv_compact(const CsvPre* __restrict__ pre, const double* __restrict__ src,
+// Moves block b's parsed values (items [tok_base, tok_base + ntok), marked in
+// okb) to out[ok_base ...] in item order: each lane takes 17 consecutive items.
+__global__ __launch_bounds__(256) void k_csv_compact(const CsvBlock* __restrict__ blk, const CsvPre* __restrict__ pre,
+                                                     const uint8_t* __restrict__ okb, const double* __restrict__ src,
                                                      double* __restrict__ out) {
+  __shared__ uint64_t lds_u[4];
+  constexpr uint32_t kPer = (kCsvBlock + 1 + 255) / 256;  // at most kCsvBlock + 1 items end in a block
   const CsvPre bp = pre[blockIdx.x];
-  for (uint32_t j = threadIdx.x; j < bp.ok; j += 256) out[bp.ok_base + j] = src[bp.tok_base + j];
+  const uint32_t ntok = blk[blockIdx.x].ntok;
+  const uint32_t i0 = threadIdx.x * kPer, i1 = i0 + kPer < ntok ? i0 + kPer : ntok;
+  uint64_t cnt = 0;
+  for (uint32_t i = i0; i < i1; ++i) cnt += okb[bp.tok_base + i];
+  uint64_t tot = 0;
+  uint64_t r = bp.ok_base + wg_exclusive<uint64_t>(cnt, 0ull, SumOp{}, lds_u, &tot);
+  for (uint32_t i = i0; i < i1; ++i)
+    if (okb[bp.tok_base + i]) out[r++] = src[bp.tok_base + i];
 }
 
 // ---- launchers ------------------------------------------------------------------------
@@ -803,11 +806,11 @@ hipError_t launch_csv_count(const uint8_t* text, uint64_t len, uint8_t delim, ui
 }
 
 hipError_t launch_csv_parse(const uint8_t* text, uint64_t len, uint8_t delim, uint32_t skip, const Pow5* tab,
-                            uint8_t* scratch, uint64_t nblocks, double* dst, CsvSlow* slow, uint64_t slow_cap,
-                            hipStream_t s) {
+                            uint8_t* scratch, uint64_t nblocks, double* dst, uint8_t* okb, CsvSlow* slow,
+                            uint64_t slow_cap, hipStream_t s) {
   unsigned long long* totals = reinterpret_cast<unsigned long long*>(scratch + csv_totals_offset(nblocks));
   launch_k(k_csv_parse, dim3((uint32_t)nblocks), dim3(256), 0, s, text, len, (uint32_t)delim, skip, tab,
-           pre_of(scratch, nblocks), dst, slow, slow_cap, totals);
+           pre_of(scratch, nblocks), dst, okb, slow, slow_cap, totals);
   launch_k(k_csv_scan, dim3(1), dim3(256), 0, s, (const CsvBlock*)blk_of(scratch), pre_of(scratch, nblocks), nblocks, 1,
            totals);
   return hipGetLastError();
@@ -820,10 +823,11 @@ hipError_t launch_csv_slow(const uint8_t* text, const CsvSlow* slow, uint64_t co
   return hipGetLastError();
 }
 
-hipError_t launch_csv_compact(const uint8_t* scratch, uint64_t nblocks, const double* src, double* out,
-                              hipStream_t s) {
-  launch_k(k_csv_compact, dim3((uint32_t)nblocks), dim3(256), 0, s,
-           (const CsvPre*)pre_of(const_cast<uint8_t*>(scratch), nblocks), src, out);
+hipError_t launch_csv_compact(const uint8_t* scratch, uint64_t nblocks, const uint8_t* okb, const double* src,
+                              double* out, hipStream_t s) {
+  uint8_t* sc = const_cast<uint8_t*>(scratch);
+  launch_k(k_csv_compact, dim3((uint32_t)nblocks), dim3(256), 0, s, (const CsvBlock*)blk_of(sc),
+           (const CsvPre*)pre_of(sc, nblocks), okb, src, out);
   return hipGetLastError();
 }
 
@@ -889,12 +893,15 @@ int csv_parse_device(Ctx* ctx, const uint8_t* text, uint64_t len, uint8_t delim,
   // room for the items the 19-digit rounding leaves open (more than 19 digits
   // close to a rounding boundary: rare); they are decided exactly by k_csv_slow
   const uint64_t slow_cap = std::max<uint64_t>(4096, items / 64);
-  BUFF_TRY(ctx->scratch(ctx->s_csvslow, slow_cap * sizeof(CsvSlow)));
+  // [queue of undecided items | one ok byte per item]
+  const uint64_t slow_bytes = (slow_cap * sizeof(CsvSlow) + 255) / 256 * 256;
+  BUFF_TRY(ctx->scratch(ctx->s_csvslow, slow_bytes + items));
   CsvSlow* slow = (CsvSlow*)ctx->s_csvslow.ptr;
+  uint8_t* okb = (uint8_t*)ctx->s_csvslow.ptr + slow_bytes;
   {
     ProfScope ps(ctx, "k_csv_parse");
-    HIP_TRY(launch_csv_parse(text, len, delim, skip, (const Pow5*)ctx->s_csvtab.ptr, scr, nb, dst, slow, slow_cap,
-                             ctx->stream));
+    HIP_TRY(launch_csv_parse(text, len, delim, skip, (const Pow5*)ctx->s_csvtab.ptr, scr, nb, dst, okb, slow,
+                             slow_cap, ctx->stream));
   }
   BUFF_TRY(d2h_small(ctx, tot, scr + csv_totals_offset(nb), sizeof(tot)));
   if (tot[3] && !tot[4]) {
@@ -912,13 +919,13 @@ int csv_parse_device(Ctx* ctx, const uint8_t* text, uint64_t len, uint8_t delim,
   if (ok && !out) return fail(BUFF_E_INVALID, "null output");
   if (ok < items || !direct) {
     const double* src = dst;
-    if (direct) {  // the values sit in out, compacted per block: move them through scratch
+    if (direct) {  // the values sit in out at their items' slots: move them through scratch
       BUFF_TRY(ctx->scratch(ctx->s_csvtmp, items * 8));
       HIP_TRY(hipMemcpyAsync(ctx->s_csvtmp.ptr, out, items * 8, hipMemcpyDeviceToDevice, ctx->stream));
       src = (const double*)ctx->s_csvtmp.ptr;
     }
     ProfScope ps(ctx, "k_csv_compact");
-    HIP_TRY(launch_csv_compact(scr, nb, src, out, ctx->stream));
+    HIP_TRY(launch_csv_compact(scr, nb, okb, src, out, ctx->stream));
   }
   return BUFF_OK;
 }
