@@ -42,3 +42,19 @@ def test_binding_covers_every_declared_symbol():
     declared = set(_declared())
     bound = set(_native.SIGNATURES) | {"buff_last_error"}
     assert declared <= bound, declared - bound
+
+
+def test_binding_constants_match_the_header():
+    # every flag, op and status code the header defines has the same value in
+    # the Python binding (a mismatch would pass the wrong bits through the ABI)
+    from buff_amd import _native as N
+
+    text = open(os.path.join(ROOT, "include", "buffgpu.h")).read()
+    defines = dict(re.findall(r"^#define\s+(BUFF_\w+)\s+(0x[0-9a-fA-F]+|\d+)u?\b", text, flags=re.M))
+    assert {"BUFF_SLICE", "BUFF_SCALAR", "BUFF_WITH_SLICE", "BUFF_OP_LT", "BUFF_E_REF_PANIC"} <= set(defines)
+    for name, value in defines.items():
+        if hasattr(N, name):
+            assert getattr(N, name) == int(value, 0), name
+    for name in ("BUFF_FLIP", "BUFF_SEMANTIC", "BUFF_MAJORITY", "BUFF_SLICE", "BUFF_SCALAR", "BUFF_WITH_SLICE",
+                 "BUFF_OP_GT", "BUFF_OP_EQ", "BUFF_OP_LT"):
+        assert hasattr(N, name), name
