@@ -607,9 +607,16 @@ int filter_cond_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t 
     if (ch->F < 8) return fail(BUFF_E_UNSUPPORTED, "buff-major chunk with F < 8: its column-count byte holds tail bits");
     if (ch->S > 0) {
       // buff_range_filter_majority_condition (buff_simd.rs:1313-1315) skips every
-      // column of a stream with sparse columns: no rows
-      if (op != BUFF_OP_GT || (flags & BUFF_SEMANTIC))
-        return fail(BUFF_E_UNSUPPORTED, "conditional filters of buff-major chunks with sparse columns");
+      // column of a stream with sparse columns: no rows. The intended answer
+      // (BUFF_SEMANTIC) and the equality form, which has no reference
+      // counterpart, are the chunk's own filter on the condition rows.
+      if (op == BUFF_OP_EQ || (flags & BUFF_SEMANTIC)) {
+        BUFF_TRY(ctx->scratch(ctx->s_slice, nwords * 8));
+        BUFF_TRY(major_filter_device(ctx, ch, op, pred, flags, (uint64_t*)ctx->s_slice.ptr));
+        HIP_TRY(launch_bitmap_logic(words, (const uint64_t*)ctx->s_slice.ptr, nullptr, cond, nwords, ch->n, 0, 0,
+                                    ctx->stream));
+        return BUFF_OK;
+      }
       run = false;
       fill = 0;
     }

@@ -184,7 +184,12 @@ def test_majority_forms_match_oracle(gpu_ctx, oracle, case):
             if sparse:
                 with pytest.raises(N.Unsupported):
                     ch.materialize_cond(cw, flags=N.BUFF_SEMANTIC)
-                with pytest.raises(N.Unsupported):
-                    ch.filter_cond("equal", 100.0, cw)
+            # the intended answer and the equality form: the chunk's own filter on the condition rows
+            for pred in mc.predicates(x)[:4]:
+                for op, flags in (("range", N.BUFF_SEMANTIC), ("equal", 0), ("equal", N.BUFF_SEMANTIC)):
+                    full, _ = ch.filter(op, pred, flags=flags)
+                    got, cnt = ch.filter_cond(op, pred, cw, flags=flags)
+                    assert np.array_equal(got, full & cw), (case, cname, pred, op, flags)
+                    assert cnt == int(np.unpackbits(got.view(np.uint8)).sum())
     finally:
         ch.free()
