@@ -688,9 +688,30 @@ int materialize_cond_device(Ctx* ctx, const Chunk* ch, uint32_t flags, const uin
     if (ch->F < 8) return fail(BUFF_E_UNSUPPORTED, "buff-major chunk with F < 8: its column-count byte holds tail bits");
     if (ch->S > 0) {
       // buff_major_decode_condition (buff_simd.rs:1738-1740) decodes nothing
-      // from a stream with sparse columns
-      if (semantic) return fail(BUFF_E_UNSUPPORTED, "conditional decode of buff-major chunks with sparse columns");
-      if (rows) *rows = 0;
+      // from a stream with sparse columns; BUFF_SEMANTIC: the stored values of
+      // the condition rows (the whole column decoded, then gathered)
+      if (!semantic) {
+        if (rows) *rows = 0;
+        return BUFF_OK;
+      }
+      uint64_t total = 0;
+      if (ch->n) {
+        if (!cond || (cap && !out)) return fail(BUFF_E_INVALID, "null argument");
+        BUFF_TRY(ctx->scratch(ctx->s_condval, ch->n * 8));
+        BUFF_TRY(major_materialize_device(ctx, ch, -1, BUFF_SEMANTIC, (double*)ctx->s_condval.ptr, nullptr));
+        const uint64_t ntiles = (ch->n + kTileRows - 1) / kTileRows;
+        BUFF_TRY(ctx->scratch(ctx->s_cond, cond_scratch_bytes(ntiles)));
+        uint8_t* scr = (uint8_t*)ctx->s_cond.ptr;
+        {
+          ProfScope ps(ctx, "k_cond_decode");
+          HIP_TRY(launch_cond_values((const double*)ctx->s_condval.ptr, ch->n, cond, out, cap, scr, ctx->stream));
+        }
+        BUFF_TRY(d2h_small(ctx, &total, scr + cond_total_offset(ntiles), 8));
+      }
+      if (rows) *rows = total;
+      if (total > cap)
+        return fail(BUFF_E_SMALL_BUFFER, "%llu condition rows, room for %llu", (unsigned long long)total,
+                    (unsigned long long)cap);
       return BUFF_OK;
     }
   }

@@ -16,6 +16,10 @@ uint64_t cond_total_offset(uint64_t ntiles);
 // Writes the condition rows' values (at most `cap`) to out in row order; the
 // row count is left at scratch + cond_total_offset(ntiles). compat: the
 // reference's tail placement when F >= 8 and F % 8 != 0.
+// The condition rows of n decoded doubles `vals`, in row order (same scratch as
+// launch_cond_decode for ceil(n/1024) tiles; the row count at cond_total_offset).
+hipError_t launch_cond_values(const double* vals, uint64_t n, const uint64_t* cond, double* out, uint64_t cap,
+                              uint8_t* scratch, hipStream_t s);
 hipError_t launch_cond_decode(const ChunkView& cv, double inv_scale, bool compat, const uint64_t* cond, double* out,
                               uint64_t cap, uint8_t* scratch, hipStream_t s);
 

@@ -208,6 +208,35 @@ __global__ __launch_bounds__(64) void k_cond_gather(ChunkView cv, double inv_sca
 // Scratch layout (each region 256-byte aligned): off_rows (nwg + 1 u64),
 // row_off (ntiles u64), off_tiles (nwg + 1 u32), wg_rows, wg_tiles (nwg u32
 // each), list (ntiles u32), tile_cnt (ntiles u16).
+// The condition rows of an already decoded column `vals` (n doubles), in row
+// order: the same tile list and row offsets as k_cond_gather.
+__global__ __launch_bounds__(64) void k_cond_gather_values(const double* __restrict__ vals, uint64_t n,
+                                                            const uint16_t* __restrict__ cond,
+                                                            const uint32_t* __restrict__ ntiles_live,
+                                                            const uint32_t* __restrict__ list,
+                                                            const unsigned long long* __restrict__ row_off,
+                                                            double* __restrict__ out, uint64_t cap) {
+  __shared__ double sv[kTileRows];
+  const uint32_t lane = __lane_id();
+  const uint32_t total = *cmem(ntiles_live);
+  for (uint64_t i = wave_global(); i < total; i += waves_in_grid()) {
+    const uint64_t c = cmem(list)[i];
+    const unsigned long long o = cmem(row_off)[i];
+    const uint32_t live = cond_lane(cond, n, c, lane);
+    const uint32_t cntl = (uint32_t)__popc(live);
+    uint32_t pos = wave_excl_scan_u32(cntl, lane);
+    const uint32_t tot = wave_total_u32(cntl);
+    const uint64_t r0 = c * kTileRows + 16ull * lane;
+#pragma unroll
+    for (uint32_t b = 0; b < 16; ++b)
+      if ((live >> b) & 1u) sv[pos++] = vals[r0 + b];
+    wave_lds_sync();
+    for (uint32_t k = lane; k < tot; k += 64)
+      if (o + k < cap) __builtin_nontemporal_store(sv[k], gmem_w(out) + o + k);
+    wave_lds_sync();
+  }
+}
+
 struct CondLayout {
   uint32_t nwg;
   uint64_t off_rows, row_off, off_tiles, wg_rows, wg_tiles, list, tile_cnt, bytes;
@@ -239,6 +268,30 @@ uint64_t cond_scratch_bytes(uint64_t ntiles) { return cond_layout(ntiles).bytes;
 uint64_t cond_total_offset(uint64_t ntiles) {
   const CondLayout L = cond_layout(ntiles);
   return L.off_rows + 8ull * L.nwg;
+}
+
+hipError_t launch_cond_values(const double* vals, uint64_t n, const uint64_t* cond, double* out, uint64_t cap,
+                              uint8_t* scratch, hipStream_t s) {
+  const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
+  if (ntiles == 0) return hipSuccess;
+  const CondLayout L = cond_layout(ntiles);
+  unsigned long long* off_rows = reinterpret_cast<unsigned long long*>(scratch + L.off_rows);
+  unsigned long long* row_off = reinterpret_cast<unsigned long long*>(scratch + L.row_off);
+  uint32_t* off_tiles = reinterpret_cast<uint32_t*>(scratch + L.off_tiles);
+  uint32_t* wg_rows = reinterpret_cast<uint32_t*>(scratch + L.wg_rows);
+  uint32_t* wg_tiles = reinterpret_cast<uint32_t*>(scratch + L.wg_tiles);
+  uint32_t* list = reinterpret_cast<uint32_t*>(scratch + L.list);
+  uint16_t* tile_cnt = reinterpret_cast<uint16_t*>(scratch + L.tile_cnt);
+  launch_k(k_cond_count, dim3(L.nwg), dim3(256), 0, s, cond, n, ntiles, tile_cnt, wg_rows, wg_tiles);
+  launch_k(k_cond_scan, dim3(1), dim3(256), 0, s, static_cast<const uint32_t*>(wg_rows),
+           static_cast<const uint32_t*>(wg_tiles), L.nwg, off_rows, off_tiles);
+  launch_k(k_cond_index, dim3(L.nwg), dim3(64), 0, s, static_cast<const uint16_t*>(tile_cnt), ntiles,
+           static_cast<const unsigned long long*>(off_rows), static_cast<const uint32_t*>(off_tiles), list, row_off);
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, 5120);
+  launch_k(k_cond_gather_values, dim3(grid), dim3(64), 0, s, vals, n, reinterpret_cast<const uint16_t*>(cond),
+           static_cast<const uint32_t*>(off_tiles + L.nwg), static_cast<const uint32_t*>(list),
+           static_cast<const unsigned long long*>(row_off), out, cap);
+  return hipGetLastError();
 }
 
 hipError_t launch_cond_decode(const ChunkView& cv, double inv_scale, bool compat, const uint64_t* cond, double* out,

@@ -69,7 +69,7 @@ struct Ctx {
   void* pinned = nullptr;
 
   Scratch s_partial, s_minmax, s_stream, s_bitmap, s_sums, s_best, s_values, s_sink, s_major, s_cond, s_condin, s_wmax, s_slice, s_csv, s_csvtab,
-      s_csvtmp, s_csvslow;
+      s_csvtmp, s_csvslow, s_condval;
   bool csv_tab_ready = false;  // s_csvtab holds the table of 5^q
 
   bool prof_on = false;

@@ -174,9 +174,9 @@ int buff_filter_device(buff_ctx* ctx, const buff_chunk* chunk, int op, double pr
  * filter's rows can be materialized. On a buff-major chunk these are
  * buff_range_filter_majority_condition (buff_simd.rs:1279-1526) and
  * buff_major_decode_condition (1714-1914): with sparse columns the reference
- * answers nothing (no rows, no values); there BUFF_SEMANTIC and the equality
- * form of the filter are the chunk's own filter on the condition rows, and the
- * BUFF_SEMANTIC decode is BUFF_E_UNSUPPORTED; without, they equal the byte_fixed forms except
+ * answers nothing (no rows, no values); there BUFF_SEMANTIC (and the filter's
+ * equality form) answer with the chunk's own filter and stored values on the
+ * condition rows; without, they equal the byte_fixed forms except
  * that the decode places the tail bits correctly and an empty condition gives
  * no values. F < 8 buff-major chunks are BUFF_E_UNSUPPORTED.
  *

@@ -181,9 +181,10 @@ def test_majority_forms_match_oracle(gpu_ctx, oracle, case):
             got = ch.materialize_cond(cw)
             ref = oracle.decode_cond(s, scale, rows, major=True)
             assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), (case, cname)
-            if sparse:
-                with pytest.raises(N.Unsupported):
-                    ch.materialize_cond(cw, flags=N.BUFF_SEMANTIC)
+            # the intended values: the semantic full decode on the condition rows
+            full_vals = ch.materialize(flags=N.BUFF_SEMANTIC)
+            sem = ch.materialize_cond(cw, flags=N.BUFF_SEMANTIC)
+            assert np.array_equal(sem.view(np.uint64), full_vals[cond].view(np.uint64)), (case, cname, sparse)
             # the intended answer and the equality form: the chunk's own filter on the condition rows
             for pred in mc.predicates(x)[:4]:
                 for op, flags in (("range", N.BUFF_SEMANTIC), ("equal", 0), ("equal", N.BUFF_SEMANTIC)):
