@@ -151,6 +151,96 @@ def _fixed(x, dlen):
     return np.trunc(np.ldexp(x, dlen)).astype(np.int64)
 
 
+def stored_offsets(x, info):
+    """The offsets a plain chunk stores: (trunc(x * 2^dlen) - base) mod 2^F (prec_double.rs:190-205;
+    split_double.rs:369-414 keeps the low F bits)."""
+    F = info.width
+    off = (_fixed(x, info.dlen) - info.base).astype(np.uint64)
+    return off & np.uint64((1 << F) - 1) if F < 64 else off
+
+
+def _words64(mask):
+    b = np.packbits(mask, bitorder="little")
+    return np.concatenate([b, np.zeros((-len(b)) % 8, np.uint8)]).view(np.uint64)
+
+
+def verify_plain(ctx, ch, x, preds, mat_precisions, words, out, buff_amd):
+    """Checks a plain chunk's answers against numpy on its stored offsets: the
+    base, range-filter bitmaps and counts at each predicate, the exact sum and
+    the reference fold (within 1e-12 of the summed magnitudes; without the
+    base term when F % 8 == 0), the max, and the materialized doubles at each
+    precision (BUFF_SEMANTIC: the top R stored bits). Returns the mismatches."""
+    from fractions import Fraction
+
+    bad = []
+    info = ch.info()
+    n, F, dlen, mn = info.n, info.width, info.dlen, info.base
+    f = _fixed(x, dlen)
+    if mn != int(f.min()):
+        bad.append("base")
+    off = stored_offsets(x, info)
+    del f
+    nw = (n + 63) // 64
+    for pred in preds:
+        fp = int(_fixed(np.array([pred]), dlen)[0])
+        if fp < mn:
+            m = np.ones(n, bool)
+        else:
+            t = fp - mn if F < 8 else (fp - mn) & ((1 << F) - 1)
+            m = off > np.uint64(t)
+        ch.filter_device("range", pred, words.ptr)
+        if not np.array_equal(words.download(np.uint64, nw), _words64(m)[:nw]):
+            bad.append("range filter at %r" % pred)
+    so = int(np.sum(off, dtype=np.uint64))
+    ref, exact, _ = ch.sum()
+    if exact != float(Fraction(n * mn + so, 1 << dlen)):
+        bad.append("exact sum")
+    want = float(Fraction((0 if (F >= 8 and F % 8 == 0) else n * mn) + so, 1 << dlen))
+    if abs(ref - want) > 1e-12 * (n * abs(mn) + so) / 2.0 ** dlen:
+        bad.append("reference sum")
+    v, o = ch.max()
+    mo = int(off.max()) if n else 0
+    if o != mo or v != float(mo + mn) / 2.0 ** dlen:
+        bad.append("max")
+    for p in mat_precisions:
+        ch.materialize_device(out.ptr, p, flags=buff_amd.BUFF_SEMANTIC)
+        R = precision_R(info, p)
+        sh = np.uint64(F - R)
+        want_v = (((off >> sh) << sh).astype(np.int64) + mn).astype(np.float64) / 2.0 ** dlen
+        if not np.array_equal(out.download(np.float64, n).view(np.uint64), want_v.view(np.uint64)):
+            bad.append("materialize at precision %d" % p)
+    return bad
+
+
+def verify_major(ch, x, preds, words, out, buff_amd):
+    """Checks a buff-major chunk's intended answers (BUFF_SEMANTIC) against numpy
+    on trunc(x * 2^dlen): filter bitmaps and counts, the exact sum, the max and
+    the full decode. Returns the mismatches."""
+    from fractions import Fraction
+
+    bad = []
+    info = ch.info()
+    n, dlen = info.n, info.dlen
+    f = _fixed(x, dlen)
+    nw = (n + 63) // 64
+    for op, pred in preds:
+        fp = int(_fixed(np.array([pred]), dlen)[0])
+        m = f > fp if op == "range" else f == fp
+        ch.filter_device(op, pred, words.ptr, flags=buff_amd.BUFF_SEMANTIC)
+        if not np.array_equal(words.download(np.uint64, nw), _words64(m)[:nw]):
+            bad.append("%s filter at %r" % (op, pred))
+    _, exact, _ = ch.sum(flags=buff_amd.BUFF_SEMANTIC)
+    if exact != float(Fraction(int(np.sum(f, dtype=np.int64)), 1 << dlen)):
+        bad.append("exact sum")
+    if ch.max_ex(flags=buff_amd.BUFF_SEMANTIC)[0] != float(f.max()) / 2.0 ** dlen:
+        bad.append("max")
+    ch.materialize_device(out.ptr, -1, flags=buff_amd.BUFF_SEMANTIC)
+    if not np.array_equal(out.download(np.float64, n).view(np.uint64),
+                          (f.astype(np.float64) / 2.0 ** dlen).view(np.uint64)):
+        bad.append("decode")
+    return bad
+
+
 def major_filter_rows(x, info, majors, pred, dlen):
     """Rows a buff-major range filter must touch: its candidates entering each
     dense plane and the tail (the compat rules of buff_simd.rs:863-1277)."""
@@ -430,6 +520,16 @@ def bench_config3(ctx, args, buff_amd, datagen):
                 g = b / (us * 1e-6) / 1e9
                 fres["ops"][name]["roofline"] = {"algorithmic_bytes": round(b), "achieved_GBps": round(g, 1),
                                                  "frac": round(g / HBM_PEAK_GBPS, 4)}
+        # the answers, against numpy
+        if fmt == "buff":
+            bad = verify_plain(ctx, ch, x, [pred10], [-1], words, out, buff_amd)
+        else:
+            bad = verify_major(ch, x, [("range", pred10), ("equal", in_major), ("equal", outlier)], words, out,
+                               buff_amd)
+        if bad:
+            log("bench: WRONG RESULTS (%s, outlier-heavy): %s" % (fmt, bad))
+            sys.exit(1)
+        fres["verified"] = True
         ch.free()
         res[fmt] = fres
     for b in (xb, words, out):
@@ -504,6 +604,45 @@ def bench_config4(ctx, args, buff_amd):
         bt.append(time.perf_counter() - t)
     bt.sort()
     assert all(r.status == 0 for r in out)
+    # the batch's answers (the last run's, still in its arena) against numpy on a sample of the queries
+    t = time.time()
+    sample = np.random.default_rng(7).choice(nq, min(nq, 2048), replace=False)
+    bms = {int(j): batch.bitmap(out[j], int(lens[qs[j][1]])) for j in sample
+           if qs[j][0] in (buff_amd._native.BUFF_Q_GT, buff_amd._native.BUFF_Q_EQ)}
+
+    def check(j):
+        from fractions import Fraction
+        N = buff_amd._native
+        op, c, pred, _, _ = qs[j]
+        r = out[j]
+        info = batch.chunks[c].info()
+        x = xh[int(row_off[c]):int(row_off[c + 1])]
+        off = stored_offsets(x, info)
+        n, F, dlen, mn = info.n, info.width, info.dlen, info.base
+        if op in (N.BUFF_Q_GT, N.BUFF_Q_EQ):
+            fp = int(_fixed(np.array([pred]), dlen)[0])
+            if fp < mn:
+                m = np.full(n, op == N.BUFF_Q_GT)
+            else:
+                tt = fp - mn if F < 8 else (fp - mn) & ((1 << F) - 1)
+                m = off > np.uint64(tt) if op == N.BUFF_Q_GT else off == np.uint64(tt)
+            if op == N.BUFF_Q_EQ and F >= 8 and F % 8 == 0:
+                m = np.zeros(n, bool)   # the equality answer stays in rb1 (buff_simd.rs:4987-5091)
+            w = bms[j]
+            return r.count == int(m.sum()) and np.array_equal(w, _words64(m).view(np.uint32)[: len(w)])
+        if op == N.BUFF_Q_SUM:
+            so = int(np.sum(off, dtype=np.uint64))
+            return r.exact == float(Fraction(n * mn + so, 1 << dlen))
+        mo = int(off.max())
+        return r.max_off == mo and r.value == float(mo + mn) / 2.0 ** dlen
+
+    with ThreadPoolExecutor(16) as ex:
+        ok = list(ex.map(check, [int(j) for j in sample]))
+    if not all(ok):
+        log("bench: WRONG RESULTS (batched): queries %s" % [int(j) for j, v in zip(sample, ok) if not v][:20])
+        sys.exit(1)
+    res["verified_queries"] = len(sample)
+    log("bench: %d batched answers verified against numpy in %.1fs" % (len(sample), time.time() - t))
     res["batched"] = {"queries": nq, "seconds": round(bt[len(bt) // 2], 5), "best_seconds": round(bt[0], 5),
                       "queries_per_s": round(nq / bt[len(bt) // 2], 1),
                       "GBps_f64_scanned": round(scanned / bt[len(bt) // 2] / 1e9, 1)}
@@ -576,6 +715,23 @@ def bench_config5(ctx, args, buff_amd):
            "mix": "60% range filter, 25% sum, 15% materialize at 1-3 decimals (semantic)"}
     nreq = args.config5_requests
 
+    # filter counts and exact sums of every pool column, for checking the served answers
+    from fractions import Fraction
+    qpreds = (99.5, 100.0, 100.5, 1e6)
+
+    def pool_answers(i):
+        info = chunks[i].info()
+        off = stored_offsets(xh[poff[i]:poff[i + 1]], info)
+        n, F, dlen, mn = info.n, info.width, info.dlen, info.base
+        counts = {}
+        for pred in qpreds:
+            fp = int(_fixed(np.array([pred]), dlen)[0])
+            counts[pred] = n if fp < mn else int(np.count_nonzero(off > np.uint64((fp - mn) & ((1 << F) - 1))))
+        return counts, float(Fraction(n * mn + int(np.sum(off, dtype=np.uint64)), 1 << dlen))
+
+    with ThreadPoolExecutor(16) as ex:
+        answers = list(ex.map(pool_answers, range(min(K, len(chunks)))))
+
     def trace(seed):
         r = np.random.default_rng(seed)
         qs, rows = [], 0
@@ -583,7 +739,7 @@ def bench_config5(ctx, args, buff_amd):
             k = int(r.integers(0, len(chunks)))
             u = r.random()
             if u < 0.6:
-                qs.append((N.BUFF_Q_GT, k, float(r.choice([99.5, 100.0, 100.5, 1e6])), -1, 0))
+                qs.append((N.BUFF_Q_GT, k, float(r.choice(qpreds)), -1, 0))
             elif u < 0.85:
                 qs.append((N.BUFF_Q_SUM, k, 0.0, -1, 0))
             else:
@@ -609,6 +765,12 @@ def bench_config5(ctx, args, buff_amd):
         finally:
             srv.destroy()
         assert all(o.status == 0 for o in out)
+        wrong = [i for i, (q, o) in enumerate(zip(qs, out))
+                 if (q[0] == N.BUFF_Q_GT and o.count != answers[q[1] % K][0][q[2]]) or
+                 (q[0] == N.BUFF_Q_SUM and o.exact != answers[q[1] % K][1])]
+        if wrong:
+            log("bench: WRONG RESULTS (served, max_batch %d): requests %s" % (max_batch, wrong[:20]))
+            sys.exit(1)
         return {"offered_qps": qps or None, "achieved_qps": round(nreq / wall, 1),
                 "p50_us": round(float(np.percentile(lat, 50)), 1), "p99_us": round(float(np.percentile(lat, 99)), 1),
                 "GBps_f64_scanned": round(rows * 8 / wall / 1e9, 1), "batches": int(batches - b0),
@@ -739,9 +901,20 @@ def main():
     logical_bytes = 8.0 * n * ops
     value = logical_bytes / (dt / args.steps) / 1e9
 
-    # ---- per-kernel accounting over the timed region ----------------------------
+    # ---- the step's answers, checked against numpy (no line is printed if any is wrong)
     keep = []
-    step(record=keep)   # chunks for byte accounting (outside the timed region)
+    step(record=keep)
+    t = time.time()
+    bad = []
+    for c, ch, x in zip(cols, keep, (x_walk, x_uni)):
+        bad += ["%s: %s" % (c.name, b) for b in verify_plain(ctx, ch, x, [c.preds[s_] for s_ in (0.001, 0.10, 0.50)],
+                                                             c.mat_precisions, words, out, buff_amd)]
+    if bad:
+        log("bench: WRONG RESULTS: %s" % bad)
+        sys.exit(1)
+    log("bench: step results verified against numpy in %.1fs" % (time.time() - t))
+
+    # ---- per-kernel accounting over the timed region ----------------------------
     infos = [ch.info() for ch in keep]
     stats_dev = ctx.alloc(9 * 8)
     filt = {}

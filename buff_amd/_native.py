@@ -152,6 +152,7 @@ SIGNATURES = {
     "buff_filter_device": [vp, vp, cint, f64, u32, vp, vp],
     "buff_sum": [vp, vp, f64p, f64p, u64p],
     "buff_sum_ex": [vp, vp, u32, f64p, f64p, u64p],
+    "buff_sum_precision": [vp, vp, i32, u32, f64p, f64p, u64p],
     "buff_max": [vp, vp, f64p, u64p],
     "buff_max_ex": [vp, vp, u32, f64p, u64p, u64p],
     "buff_filter_cond": [vp, vp, cint, f64, u32, u32p, u32p, u64, u64p],
@@ -205,6 +206,7 @@ SIGNATURES = {
     "buff_sd_byte_fixed_sum": [P(SplitDouble), u8p, u64, f64p],
     "buff_sd_byte_fixed_max": [P(SplitDouble), u8p, u64, f64p],
     "buff_sd_byte_residue_decode_with_precision": [P(SplitDouble), u8p, u64, u32, f64p, u64],
+    "buff_sd_byte_residue_sum_with_precision": [P(SplitDouble), u8p, u64, u32, f64p],
     "buff_sd_buff_encode_majority": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
     "buff_sd_buff_decode_majority": [P(SplitDouble), u8p, u64, f64p, u64, u64p],
     "buff_sd_buff_range_filter_majority": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
@@ -346,6 +348,15 @@ class Chunk:
         ps = np.zeros(9, dtype=np.uint64)
         check(lib().buff_sum_ex(self.ctx.handle, self.handle, flags, ctypes.byref(r), ctypes.byref(e),
                                 ptr(ps, ctypes.c_uint64)))
+        return r.value, e.value, ps
+
+    def sum_precision(self, precision, flags=0):
+        """byte_residue_sum_with_precision: returns (reference f64 sum, exact sum of the values truncated to
+        the bits read, u64 sums read [9])."""
+        r, e = ctypes.c_double(), ctypes.c_double()
+        ps = np.zeros(9, dtype=np.uint64)
+        check(lib().buff_sum_precision(self.ctx.handle, self.handle, int(precision), flags, ctypes.byref(r),
+                                       ctypes.byref(e), ptr(ps, ctypes.c_uint64)))
         return r.value, e.value, ps
 
     def max(self):

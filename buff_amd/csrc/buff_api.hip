@@ -959,6 +959,109 @@ int precision_bits(const Chunk* ch, int32_t precision, uint32_t* R) {
   return BUFF_OK;
 }
 
+// split_double.rs:2463-2626 fold of byte_residue_sum_with_precision over the
+// sums it reads: one term per whole sub-column (returning without the base term
+// when the R-bit read ends on a byte boundary), then the R % 8-bit fields.
+double precision_sum_fold(const Chunk* ch, uint32_t R, const uint64_t* sums) {
+  const uint32_t ilen = ch->ilen;
+  uint32_t remain = R;
+  int32_t processed = 0;
+  double sum_base = (double)(uint32_t)ch->n * (double)ch->view.base;
+  double sum = 0.0;
+  sum_base = sum_base / powi2((int32_t)ch->dlen);
+  uint32_t k = 0;
+  while (remain >= 8) {
+    remain -= 8;
+    processed += 8;
+    sum = sum + (double)sums[k++] / powi2((int32_t)((uint32_t)processed - ilen));
+    if (remain == 0) return sum;
+  }
+  if (remain > 0) {
+    processed += (int32_t)remain;
+    sum = sum + (double)sums[k] / powi2((int32_t)((uint32_t)processed - ilen));
+  }
+  sum += sum_base;
+  return sum;
+}
+
+// byte_residue_sum_with_precision (split_double.rs:2409-2629) on a plain chunk
+// (the stream branch of ilen <= 15, 2576-2624). The R-bit read rule selects
+// R / 8 whole sub-columns, summed by k_sum over a view of just those planes,
+// and R % 8 = q bits per row: the whole tail when q equals the stored tail
+// width (summed in the same launch), else k_tail_fields (the fields the
+// reference reads from the packed tail, and the rows' top q tail bits).
+// ref_sum replays the reference's fold; exact_sum is the exact sum of the
+// values truncated to their top min(R, F) stored bits, rounded once. With
+// BUFF_SEMANTIC, ref_sum = exact_sum and the reference's panics and ilen > 15
+// branch do not apply (more decimals than stored: every bit).
+int sum_precision_op(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* ref_sum,
+                     double* exact_sum, uint64_t* sums_out) {
+  if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  if (precision < 0) return fail(BUFF_E_INVALID, "precision %d < 0", precision);
+  if (ch->flags & (BUFF_MAJORITY | BUFF_SLICE))
+    return fail(BUFF_E_UNSUPPORTED, "the reference has no reduced-precision sum of a %s stream",
+                (ch->flags & BUFF_SLICE) ? "byte-slice" : "buff-major");
+  const bool semantic = (flags & BUFF_SEMANTIC) != 0;
+  const ChunkView& v = ch->view;
+  uint32_t R = 0;
+  int rc = precision_bits(ch, precision, &R);
+  if (rc == BUFF_E_REF_PANIC && semantic) {
+    R = v.F;
+  } else if (rc != BUFF_OK) {
+    return rc;
+  }
+  if (!semantic) {
+    if (ch->ilen > 15)
+      return fail(BUFF_E_UNSUPPORTED, "ilen %u > 15: the reference parses an outlier section here", ch->ilen);
+    if (R / 8 > v.P) {
+      if (kStreamHeader + (uint64_t)(R / 8) * ch->n > stream_bytes_of(ch->n, v.P, v.r))
+        return fail(BUFF_E_REF_PANIC, "the reference reads %u sub-columns of a %u-plane stream", R / 8, v.P);
+      return fail(BUFF_E_UNSUPPORTED, "the reference would read %u whole bytes of a %u-plane stream", R / 8, v.P);
+    }
+  }
+  const uint32_t Rs = std::min(R, v.F);   // bits the exact (truncated) sum keeps
+  // R % 8 != 0 only when R / 8 == P (R <= F): the q bits come from the tail.
+  const uint32_t q = semantic ? Rs % 8 : R % 8;
+  const uint32_t nplanes = std::min<uint32_t>((semantic ? Rs : R) / 8, v.P);
+  uint64_t ps[kMaxPlanes + 1] = {0};
+  uint64_t fields[2] = {0, 0};   // reference fields, exact top-q tail bits
+  if (ch->n > 0) {
+    ChunkView pv = v;
+    pv.P = nplanes;
+    pv.r = (q > 0 && q == v.r) ? v.r : 0;
+    if (pv.P || pv.r) {
+      const LaunchShape ls = ctx->shape(Op::Sum, pv);
+      BUFF_TRY(ctx->scratch(ctx->s_sums, sizeof(ps) + sum_scratch_bytes(ls)));
+      ProfScope sc(ctx, "k_sum");
+      HIP_TRY(launch_sum(pv, (unsigned long long*)ctx->s_sums.ptr, ctx->stream, ls));
+    }
+    if (q > 0 && q < v.r) {
+      BUFF_TRY(ctx->scratch(ctx->s_best, 256));
+      HIP_TRY(hipMemsetAsync(ctx->s_best.ptr, 0, 16, ctx->stream));
+      ProfScope sc(ctx, "k_tail_fields");
+      HIP_TRY(launch_tail_fields(v, q, (unsigned long long*)ctx->s_best.ptr, ctx->stream));
+    }
+    if (pv.P || pv.r) BUFF_TRY(d2h_small(ctx, ps, ctx->s_sums.ptr, sizeof(ps)));
+    if (q > 0 && q < v.r) BUFF_TRY(d2h_small(ctx, fields, ctx->s_best.ptr, sizeof(fields)));
+  }
+  // sums in read order: the whole sub-columns, then the q-bit part
+  uint64_t sums[kMaxPlanes + 1] = {0};
+  for (uint32_t k = 0; k < nplanes; ++k) sums[k] = ps[k];
+  uint64_t top_q = 0;
+  if (q > 0) {
+    sums[nplanes] = (q == v.r) ? ps[nplanes] : fields[0];
+    top_q = (q == v.r) ? ps[nplanes] : fields[1];
+  }
+  if (sums_out) memcpy(sums_out, sums, sizeof(sums));
+  __int128 tot = (__int128)ch->n * (__int128)v.base;
+  for (uint32_t k = 0; k < std::min<uint32_t>(Rs / 8, v.P); ++k) tot += (__int128)ps[k] << (v.F - 8 * (k + 1));
+  if (Rs % 8) tot += (__int128)top_q << (v.F - Rs);
+  const double exact = (double)tot / powi2((int32_t)ch->dlen);
+  if (exact_sum) *exact_sum = exact;
+  if (ref_sum) *ref_sum = semantic ? exact : precision_sum_fold(ch, R, sums);
+  return BUFF_OK;
+}
+
 // The materialization recipe of a plain chunk: full precision
 // (split_double.rs:1243-1350), the reference's reduced-precision read
 // (1612-1893) or, with BUFF_SEMANTIC, the top R stored bits re-scaled.
@@ -1479,6 +1582,12 @@ int buff_sum(buff_ctx* ctx, const buff_chunk* chunk, double* ref_sum, double* ex
   return buff_sum_ex(ctx, chunk, 0, ref_sum, exact_sum, plane_sums);
 }
 
+int buff_sum_precision(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags, double* ref_sum,
+                       double* exact_sum, uint64_t* sums) {
+  return sum_precision_op(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<const Chunk*>(chunk), precision, flags,
+                          ref_sum, exact_sum, sums);
+}
+
 int buff_max_ex(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, double* max_value, uint64_t* max_off,
                 uint64_t* argmax_count) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
@@ -1600,6 +1709,17 @@ int buff_sd_byte_fixed_sum(const buff_split_double* sd, const uint8_t* bytes, ui
   buff_chunk* ch = nullptr;
   BUFF_TRY(sd_load(sd, bytes, len, 0, &c, &ch));
   int rc = buff_sum(reinterpret_cast<buff_ctx*>(c), ch, s, nullptr, nullptr);
+  (void)buff_chunk_free(ch);
+  return rc;
+}
+
+int buff_sd_byte_residue_sum_with_precision(const buff_split_double* sd, const uint8_t* bytes, uint64_t len,
+                                            uint32_t precision, double* s) {
+  if (precision > 0x7FFFFFFFu) return fail(BUFF_E_REF_PANIC, "precision %u is not in PRECISION_MAP", precision);
+  Ctx* c = nullptr;
+  buff_chunk* ch = nullptr;
+  BUFF_TRY(sd_load(sd, bytes, len, 0, &c, &ch));
+  int rc = buff_sum_precision(reinterpret_cast<buff_ctx*>(c), ch, (int32_t)precision, 0, s, nullptr, nullptr);
   (void)buff_chunk_free(ch);
   return rc;
 }

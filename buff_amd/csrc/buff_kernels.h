@@ -59,6 +59,10 @@ hipError_t launch_touch_bytes(const void* in, uint64_t bytes, uint32_t* sink, hi
 constexpr uint32_t kMaxSumRows = 8192;
 uint64_t sum_scratch_bytes(const LaunchShape& ls);
 hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls);
+// Tail part of the reduced-precision sum reading q < r bits per row: out[0] +=
+// the sum of the n q-bit fields the reference reads from the packed tail,
+// out[1] += the sum of every row's top q tail bits (the caller zeroes out[0..1]).
+hipError_t launch_tail_fields(const ChunkView& cv, uint32_t q, unsigned long long* out, hipStream_t s);
 // Two passes; `scratch` holds max_scratch_bytes(ntiles) bytes; the caller
 // zeroes `best`.
 uint64_t max_scratch_bytes(uint64_t ntiles);

@@ -465,6 +465,59 @@ __global__ __launch_bounds__(256) void k_sum_final(const unsigned long long* __r
 }
 
 // ----------------------------------------------------------------------------
+// K5b: the tail part of the reduced-precision sum
+// (byte_residue_sum_with_precision, split_double.rs:2611-2622) when it reads
+// q < r bits per row. The reference then reads n q-bit fields from the start of
+// the packed r-bit tail: field i holds stream tail bits [i*q, i*q + q), i.e. tail
+// bit b of row i (stream bit i*r + b) lands in field bit (i*r + b) mod q when
+// i*r + b < n*q. out[0] accumulates the sum of those fields; out[1] the sum of
+// every row's top q tail bits (tail >> (r - q)), the exact truncated value.
+// One lane per 16 rows of a tile (its r bit-plane words), grid-stride over
+// tiles; one atomic add per workgroup and value.
+__global__ __launch_bounds__(256) void k_tail_fields(ChunkView cv, uint32_t q, unsigned long long* __restrict__ out) {
+  const uint32_t lane = __lane_id();
+  const uint32_t r = cv.r;
+  const uint64_t limit = cv.n * q;   // stream bits the reference reads
+  uint64_t ref = 0, sem = 0;
+  for (uint64_t c = wave_global(); c < cv.ntiles; c += waves_in_grid()) {
+    uint32_t h[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) h[j] = j < (int)r ? cv.tail[(c * r + j) * 64 + lane] : 0u;
+    for (uint32_t j = r - q; j < r; ++j) sem += (uint64_t)__popc(h[j]) << (j - (r - q));
+    const uint64_t row0 = c * kTileRows + 16ull * lane;
+    uint64_t pos = row0 * r;             // stream bit of (row0, bit 0)
+    uint32_t m = (uint32_t)(pos % q);    // its field bit
+    for (int k = 0; k < 16; ++k) {
+      for (uint32_t b = 0; b < r; ++b) {
+        if (pos < limit && ((h[b] >> k) & 1u)) ref += 1ull << m;
+        ++pos;
+        m = (m + 1 == q) ? 0u : m + 1;
+      }
+    }
+  }
+  __shared__ uint64_t part[2][4];
+  ref = wave_sum_u64(ref);
+  sem = wave_sum_u64(sem);
+  if (lane == 0) {
+    part[0][wave_in_wg()] = ref;
+    part[1][wave_in_wg()] = sem;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    uint64_t v = 0;
+    for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) v += part[threadIdx.x][w];
+    if (v) atomicAdd(out + threadIdx.x, (unsigned long long)v);
+  }
+}
+
+hipError_t launch_tail_fields(const ChunkView& cv, uint32_t q, unsigned long long* out, hipStream_t s) {
+  if (q == 0 || q >= cv.r || cv.r > 7) return hipErrorInvalidValue;
+  const uint32_t wg = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((cv.ntiles + 3) / 4, 2048));
+  launch_k(k_tail_fields, dim3(wg), dim3(256), 0, s, cv, q, out);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
 // K6: max stored offset (split_double.rs:3919-4105). The reference narrows the
 // candidates byte by byte; its result is the largest stored offset, whose top
 // byte is the largest top-plane byte. Pass 1 streams the top plane once and

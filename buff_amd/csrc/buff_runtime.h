@@ -145,6 +145,8 @@ int mat_plan(const Chunk* ch, int32_t precision, uint32_t flags, MatParams* out)
 int filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, uint64_t* words,
                   unsigned long long* stats);
 int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64_t* plane_sums);
+int sum_precision_op(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* ref_sum,
+                     double* exact_sum, uint64_t* sums);
 int max_op(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t* max_off, uint64_t* argmax);
 int materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* out);
 int filter_cond_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, const uint64_t* cond,

@@ -114,6 +114,12 @@ class SplitBDDoubleCompress {
     check(buff_sd_byte_fixed_sum(&sd_, bytes.data(), bytes.size(), &s));
     return s;
   }
+  // split_double.rs:2409 byte_residue_sum_with_precision (byte_fixed layout)
+  double byte_residue_sum_with_precision(std::vector<uint8_t> bytes, uint32_t precision) const {
+    double s = 0;
+    check(buff_sd_byte_residue_sum_with_precision(&sd_, bytes.data(), bytes.size(), precision, &s));
+    return s;
+  }
   // split_double.rs:3919 byte_fixed_max: prints the max like the reference
   void byte_fixed_max(std::vector<uint8_t> bytes) const { std::printf("Max value:%.17g\n", max_value(std::move(bytes))); }
   double max_value(std::vector<uint8_t> bytes) const {

@@ -91,6 +91,13 @@ class SplitBDDoubleCompress:
                                                ctypes.byref(s)))
         return s.value
 
+    def byte_residue_sum_with_precision(self, b, precision):  # split_double.rs:2409
+        b = _u8(b)
+        s = ctypes.c_double()
+        N.check(N.lib().buff_sd_byte_residue_sum_with_precision(ctypes.byref(self._sd), N.ptr(b, ctypes.c_uint8),
+                                                                len(b), int(precision), ctypes.byref(s)))
+        return s.value
+
     def byte_fixed_max(self, b):  # split_double.rs:3919
         b = _u8(b)
         m = ctypes.c_double()

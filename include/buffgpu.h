@@ -234,6 +234,22 @@ int buff_sum(buff_ctx* ctx, const buff_chunk* chunk, double* ref_sum, double* ex
  * ref_sum = exact_sum, the f64 running sum's value within its rounding. */
 int buff_sum_ex(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, double* ref_sum, double* exact_sum,
                 uint64_t* plane_sums);
+/* byte_residue_sum_with_precision (split_double.rs:2409-2629) on a plain chunk
+ * (the stream branch taken when ilen <= 15; larger ilen is BUFF_E_UNSUPPORTED,
+ * as are buff-major and byte-slice chunks). The read width R (the rule of
+ * 2439-2462, shared with the reduced-precision decode) selects R/8 whole
+ * sub-columns and, when R % 8 = q > 0, q-bit fields read from the packed tail
+ * (reading fields of q < r bits at stride q, as the reference does). ref_sum
+ * reproduces the reference f64 fold bit for bit, including the missing base
+ * term when R is a positive multiple of 8; exact_sum is the exact sum of the
+ * values truncated to their top min(R, F) stored bits, rounded once; sums
+ * (nullable, 9 u64) are the u64 sums read, in read order. Reference panics
+ * (precision not in PRECISION_MAP, dlen below the requested bits, reads past
+ * the stream) are BUFF_E_REF_PANIC. BUFF_SEMANTIC: ref_sum = exact_sum and
+ * those panics and the ilen > 15 branch do not apply (more decimals than
+ * stored keep every bit). */
+int buff_sum_precision(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags, double* ref_sum,
+                       double* exact_sum, uint64_t* sums);
 /* byte_fixed_max (split_double.rs:3919-4105): the printed max value and the max
  * stored offset. */
 int buff_max(buff_ctx* ctx, const buff_chunk* chunk, double* max_value, uint64_t* max_off);
@@ -407,6 +423,10 @@ int buff_sd_byte_fixed_max(const buff_split_double* sd, const uint8_t* bytes, ui
  * stream (the ilen <= 15 branch; larger ilen is BUFF_E_UNSUPPORTED). */
 int buff_sd_byte_residue_decode_with_precision(const buff_split_double* sd, const uint8_t* bytes, uint64_t len,
                                                uint32_t precision, double* out, uint64_t cap);
+/* split_double.rs:2409 byte_residue_sum_with_precision on a byte_fixed stream
+ * (the ilen <= 15 branch; larger ilen is BUFF_E_UNSUPPORTED). */
+int buff_sd_byte_residue_sum_with_precision(const buff_split_double* sd, const uint8_t* bytes, uint64_t len,
+                                            uint32_t precision, double* sum);
 /* buff-major: buff_simd.rs:317 buff_encode_majority (out == NULL: size query),
  * :633 buff_decode_majority (*n_out = values written; 0 when the stream ends in
  * sparse columns), :863 buff_range_filter_majority, :1916

@@ -887,6 +887,67 @@ std::vector<double> decode_with_precision(const uint8_t* bytes, size_t blen, uin
   return out;
 }
 
+// split_double.rs:2409-2629 (byte_residue_sum_with_precision), the branch taken
+// when the stored ilen <= 15 (2576-2624: the plain byte layout, identical to the
+// byte_fixed stream). The read width R follows the same rule as the
+// reduced-precision decode (2439-2462). `plane_sums` receives the u64 sum of
+// every sub-column read (index k) and, when R % 8 != 0, the sum of the R % 8-bit
+// fields read_bits returns (index R / 8). With ilen > 15 the reference parses an
+// outlier section that a byte_fixed stream does not have: UNSUPPORTED.
+double sum_with_precision(const uint8_t* bytes, size_t blen, uint64_t scale, uint32_t precision,
+                          bool* unsupported, uint64_t* plane_sums) {
+  (void)scale;
+  *unsupported = false;
+  BitPackReader bp(bytes, blen);
+  Header h = read_header(bp);
+  uint32_t remain = h.ilen + h.dlen;
+  int32_t processed = 0;
+  double sum_base = (double)h.len * (double)h.base;
+  uint64_t sum_fixed = 0;
+  double sum = 0.0;
+  double dec_scl = powi(2.0, (int32_t)h.dlen);
+  sum_base = sum_base / dec_scl;
+  if (precision == 0) {
+    remain = h.ilen;
+    uint32_t byte_needed = h.ilen / 8;
+    if (h.ilen % 8 > 0) { byte_needed += 1; remain = byte_needed * 8; }
+  } else {
+    const uint32_t bits_needed = (uint32_t)precision_map((int32_t)precision);
+    if (!(h.dlen >= bits_needed)) throw RefPanic("assert!(dlen >= bits_needed)");
+    remain = h.ilen + bits_needed;
+    const uint32_t fixed_byte = (h.dlen + h.ilen) / 8;
+    uint32_t byte_needed = (h.ilen + bits_needed) / 8;
+    const uint32_t extra_bits = (h.ilen + bits_needed) % 8;
+    if (byte_needed < fixed_byte && extra_bits > 0) { byte_needed += 1; remain = byte_needed * 8; }
+  }
+  if (h.ilen > 15) { *unsupported = true; return 0.0; }
+  int k = 0;
+  // 2577-2593 and 2594-2610: one whole sub-column per step; the loop returns
+  // without the base term as soon as the width is used up on a byte boundary
+  while (remain >= 8) {
+    remain -= 8;
+    processed += 8;
+    dec_scl = powi(2.0, (int32_t)((uint32_t)processed - h.ilen));
+    const uint8_t* chunk = bp.read_n_byte(h.len);
+    for (uint32_t i = 0; i < h.len; ++i) sum_fixed += chunk[i];
+    if (plane_sums && k < 9) plane_sums[k] = sum_fixed;
+    ++k;
+    sum = sum + (double)sum_fixed / dec_scl;
+    sum_fixed = 0;
+    if (remain == 0) return sum;
+  }
+  if (remain > 0) {   // 2611-2622
+    if (processed >= 8) bp.finish_read_byte();
+    processed += (int32_t)remain;
+    dec_scl = powi(2.0, (int32_t)((uint32_t)processed - h.ilen));
+    for (uint32_t i = 0; i < h.len; ++i) sum_fixed += bp.read_bits(remain);
+    if (plane_sums && k < 9) plane_sums[k] = sum_fixed;
+    sum = sum + (double)sum_fixed / dec_scl;
+  }
+  sum += sum_base;
+  return sum;
+}
+
 // ---------------------------------------------------------------------------
 // AVX2 "buff-simd" query path — the reference's fast CPU path, timed as the
 // CPU baseline. database/src/simd/vectorize_query.rs:168-222 and
@@ -2464,6 +2525,16 @@ int ref_decode_with_precision(const uint8_t* s, uint64_t len, uint64_t scale, ui
     if (v.size() > cap) return REF_SMALL_BUFFER;
     std::memcpy(out, v.data(), v.size() * 8);
     return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+int ref_sum_with_precision(const uint8_t* s, uint64_t len, uint64_t scale, uint32_t precision, double* sum,
+                           uint64_t* plane_sums) {
+  try {
+    if (plane_sums) std::memset(plane_sums, 0, 9 * 8);
+    bool unsup = false;
+    *sum = sum_with_precision(s, len, scale, precision, &unsup, plane_sums);
+    return unsup ? REF_UNSUPPORTED : REF_OK;
   } catch (const RefPanic&) { return REF_PANIC; }
 }
 

@@ -15,7 +15,10 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "librefcpu.so")
+# BUFF_ORACLE_LIB: load another build of refcpu.cpp (the planted-mutation test
+# points it at mutated copies); nothing is built then.
+_LIB_OVERRIDE = os.environ.get("BUFF_ORACLE_LIB")
+_LIB_PATH = _LIB_OVERRIDE or os.path.join(_HERE, "librefcpu.so")
 
 REF_OK, REF_PANIC, REF_UNSUPPORTED, REF_SMALL_BUFFER = 0, 4, 5, 6
 
@@ -29,9 +32,11 @@ class RefUnsupported(RuntimeError):
 
 
 def build(force=False):
-    """Compile librefcpu.so from refcpu.cpp with the committed Makefile."""
-    if force or not os.path.exists(_LIB_PATH):
-        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
+    """Compile librefcpu.so from refcpu.cpp with the committed Makefile (make
+    rebuilds it when a source is newer)."""
+    if _LIB_OVERRIDE:
+        return
+    subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
 
 
 _lib = None
@@ -73,6 +78,8 @@ def lib():
             "ref_max": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, f64p, u64p, u64p]),
             "ref_decode_with_precision": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
                                                          f64p, ctypes.c_uint64, u64p]),
+            "ref_sum_with_precision": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, f64p,
+                                                      u64p]),
             "ref_major_encode": (ctypes.c_int, [f64p, ctypes.c_uint64, ctypes.c_uint64, u8p, ctypes.c_uint64, u64p,
                                                 i64p]),
             "ref_major_decode": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_uint64, f64p, ctypes.c_uint64, u64p]),
@@ -292,6 +299,16 @@ def decode_with_precision(stream, scale, precision):
                                            _ptr(out, ctypes.c_double), len(out), ctypes.byref(got)))
     return out[: got.value]
 
+
+def sum_with_precision(stream, scale, precision):
+    """byte_residue_sum_with_precision -> (f64 sum, u64 sums of the sub-columns read and of the
+    R % 8-bit fields, index by read order)."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    out = ctypes.c_double()
+    ps = np.zeros(9, dtype=np.uint64)
+    _check(lib().ref_sum_with_precision(_ptr(s, ctypes.c_uint8), len(s), scale, precision, ctypes.byref(out),
+                                        _ptr(ps, ctypes.c_uint64)))
+    return out.value, ps
 
 
 # --- buff-major (buff_simd.rs:317-2861) -------------------------------------

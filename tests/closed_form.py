@@ -263,6 +263,51 @@ def precision_R(h, precision):
     return R
 
 
+def sum_precision(stream, precision):
+    """What byte_residue_sum_with_precision reads and returns on a plain stream,
+    from the stream's bytes: the R-bit read rule gives R // 8 whole n-byte
+    sub-columns after the header, then (R % 8 = q > 0) n q-bit LSB-first fields
+    taken from the next byte on. Each sub-column k weighs 2^(ilen - 8(k+1)), the
+    fields 2^(ilen - R), and the n * base / 2^dlen term is left out when the read
+    ends on a byte boundary (R >= 8, R % 8 == 0). Returns (sums, exact value as
+    a Fraction); raises Panic when the reads run past the stream."""
+    from fractions import Fraction
+
+    s = np.asarray(stream, dtype=np.uint8)
+    h = parse(stream)
+    n = h["n"]
+    R = precision_R(h, precision)
+    ilen_s = h["F"] - h["dlen"]           # ilen read as i32 (it wraps below 0)
+    nplanes, q = R // 8, R % 8
+    if 20 + nplanes * n > len(s):
+        raise Panic("whole sub-columns past the stream")
+    sums = [int(s[20 + k * n: 20 + (k + 1) * n].astype(np.uint64).sum()) for k in range(nplanes)]
+    val = sum(Fraction(v) * Fraction(2) ** (ilen_s - 8 * (k + 1)) for k, v in enumerate(sums))
+    if q:
+        start = 20 + nplanes * n
+        nb = (n * q + 7) // 8
+        if start + nb > len(s):
+            raise Panic("fields past the stream")
+        bits = np.unpackbits(s[start:start + nb], bitorder="little")[: n * q].reshape(n, q).astype(np.uint64)
+        fs = int((bits << np.arange(q, dtype=np.uint64)[None, :]).sum())
+        sums.append(fs)
+        val += Fraction(fs) * Fraction(2) ** (ilen_s - R)
+    if not (R >= 8 and q == 0):
+        val += Fraction(n * h["base"], 1 << h["dlen"])
+    return sums, val
+
+
+def sum_semantic_precision(stream, precision):
+    """Exact sum of the values truncated to their top min(R, F) stored bits."""
+    from fractions import Fraction
+
+    h = parse(stream)
+    R = min(precision_R(h, precision), h["F"])
+    sh = np.uint64(h["F"] - R)
+    off = (h["off"] >> sh) << sh
+    return Fraction(h["n"] * h["base"] + int(off.astype(object).sum()), 1 << h["dlen"])
+
+
 def decode_semantic_precision(stream, precision):
     """Values truncated to the top R stored bits, re-scaled."""
     h = parse(stream)

@@ -6,10 +6,12 @@ their top planes) is encoded with the one-launch batch encoder;
 every chunk must serialize to the oracle's byte_fixed stream. Then mixed query
 batches (range / equality filters, sums, maxima, materializations at full and
 reduced precision, compat and semantic) run as one graph launch each, and every
-result must equal the single-chunk call on the same chunk: bitmaps, counts, the
-reference f64 sum bit for bit, exact sums, max offsets and values, materialized
-doubles bit for bit, and the same error status. Graph replays with different
-batches, and the autotuner's choice, must not change any answer.
+result must equal the oracle's answer on the chunk's reference stream (the
+closed forms for BUFF_SEMANTIC) and the single-chunk call on the same chunk:
+bitmaps, counts, the reference f64 sum bit for bit, exact sums, max offsets and
+values, materialized doubles bit for bit, and the same error status. Graph
+replays with different batches, and the autotuner's choice, must not change any
+answer. The 4096-chunk batch at its real size is in test_gpu_realsize.py.
 """
 import numpy as np
 import pytest
@@ -109,12 +111,65 @@ def _check(b, cols, qs, res):
             assert np.array_equal(got.view(np.int64), want.view(np.int64)), (k, prec, flags)
 
 
-def test_batch_queries_match_single_calls(batch):
+def _check_oracle(b, cols, streams, qs, res, oracle):
+    """Every batched result against the oracle on the chunk's reference stream
+    (BUFF_SEMANTIC answers against the closed forms)."""
+    from tests import closed_form as cf
+
+    for (op, k, pred, prec, flags), r in zip(qs, res):
+        if k >= len(cols):
+            continue
+        s, scale = streams[k], cols[k][1]
+        n = len(cols[k][0])
+        if op in (N.BUFF_Q_GT, N.BUFF_Q_EQ):
+            kind = "range" if op == N.BUFF_Q_GT else "equal"
+            if flags & N.BUFF_SEMANTIC:
+                m = cf.filter_bits(s, scale, kind, pred, semantic=True)
+                w_ref, c_ref = cf.to_words(m), int(m.sum())
+            else:
+                w_ref, c_ref, _ = oracle.buff_filter(s, scale, kind, pred)
+            assert r.status == 0 and r.count == c_ref, ("oracle", op, k, pred)
+            assert np.array_equal(b.bitmap(r, n), w_ref), ("oracle", op, k, pred)
+        elif op == N.BUFF_Q_SUM:
+            s_ref, _ = oracle.byte_fixed_sum(s, scale)
+            tot, _ = cf.exact_sum(s)
+            assert r.status == 0 and (r.value == s_ref or (np.isnan(s_ref) and np.isnan(r.value))), ("oracle", k)
+            assert abs(r.exact - float(tot)) <= 1e-12 * max(1.0, abs(float(tot))), ("oracle", k)
+        elif op == N.BUFF_Q_MAX:
+            m_ref, off_ref, _ = oracle.byte_fixed_max(s, scale)
+            assert r.status == 0 and r.max_off == off_ref and r.value == m_ref, ("oracle", k)
+        else:
+            if flags & N.BUFF_SEMANTIC:
+                try:
+                    want = cf.decode(s) if prec < 0 else cf.decode_semantic_precision(s, prec)
+                except (ValueError, KeyError):
+                    want = cf.decode(s)   # more decimals than stored: every bit
+            else:
+                try:
+                    want = oracle.decode(s, scale) if prec < 0 else oracle.decode_with_precision(s, scale, prec)
+                except oracle.RefPanic:
+                    assert r.status == N.BUFF_E_REF_PANIC, ("oracle", k, prec)
+                    continue
+                except oracle.RefUnsupported:
+                    assert r.status == N.BUFF_E_UNSUPPORTED, ("oracle", k, prec)
+                    continue
+                h = cf.parse(s)
+                if prec >= 0 and cf.precision_R(h, prec) // 8 > h["P"]:
+                    assert r.status == N.BUFF_E_UNSUPPORTED, ("oracle", k, prec)
+                    continue
+            assert r.status == 0, ("oracle", k, prec, flags, r.status)
+            assert np.array_equal(b.values(r, n).view(np.uint64), want.view(np.uint64)), ("oracle", k, prec, flags)
+
+
+def test_batch_queries_match_single_calls(batch, oracle):
     b, cols = batch
+    streams = [oracle.encode(x, scale) for x, scale in cols]
     for seed in (1, 2):
         qs = _mix(cols, seed)
         print("batch", seed, len(qs), flush=True)
-        _check(b, cols, qs, b.query(qs))
+        res = b.query(qs)
+        _check(b, cols, qs, res)
+        _check_oracle(b, cols, streams, qs, res, oracle)
 
 
 def test_graph_replay_and_tuning(batch):

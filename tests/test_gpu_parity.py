@@ -25,6 +25,8 @@ def _cases():
     yield "tail_only", np.round(rng.integers(0, 5, 3001) * 0.1, 1), 10
     yield "F16_base", np.round(np.linspace(100, 2147.9, 5000), 1), 10
     yield "F8", np.round(np.linspace(0, 7.9, 4097), 1), 10
+    yield "F8_base", np.round(np.linspace(100, 107.9, 4097), 1), 10
+    yield "walk4_r6", dg.random_walk(200_001, 4, 15) * 64, 10 ** 4   # F = 30: q < r reads at p = 3
     yield "first_is_max", np.concatenate([[50.0], np.round(np.linspace(0, 10, 70_000), 2)]), 10 ** 2
     yield "negative", dg.random_walk(33_333, 3, 9) - 500, 10 ** 3
     yield "p6_ilen_wraps", dg.random_walk(20_000, 6, 10) * 1e-3, 10 ** 6
@@ -129,6 +131,52 @@ def test_reduced_precision(gpu_ctx, oracle, name, x, scale):
             sem_want = cf.decode(ref)
         got = ch.materialize(p, flags=buff_amd.BUFF_SEMANTIC)
         assert np.array_equal(got.view(np.uint64), sem_want.view(np.uint64)), (name, "semantic", p)
+    ch.free()
+
+
+@pytest.mark.parametrize("name,x,scale", CASES, ids=IDS)
+def test_sum_with_precision(gpu_ctx, oracle, name, x, scale):
+    # byte_residue_sum_with_precision: the reference fold bit for bit (or the
+    # same panic / unsupported branch), the u64 sums it reads, and the exact
+    # sum of the truncated values against the closed form
+    import buff_amd
+    from buff_amd.split_double import SplitBDDoubleCompress
+
+    ref = oracle.encode(x, scale)
+    h = cf.parse(ref)
+    ch = gpu_ctx.load(ref, scale)
+    sd = SplitBDDoubleCompress(10, 10, scale)
+    for p in range(0, 7):
+        try:
+            want, ps_want = oracle.sum_with_precision(ref, scale, p)
+            err = None
+        except oracle.RefPanic:
+            err = buff_amd.RefPanic
+        except oracle.RefUnsupported:
+            err = buff_amd.Unsupported
+        if err is None and cf.precision_R(h, p) // 8 > h["P"]:
+            err = buff_amd.Unsupported   # whole bytes read past the planes of a tiny column
+        if err is not None:
+            with pytest.raises(err):
+                ch.sum_precision(p)
+            with pytest.raises(err):
+                sd.byte_residue_sum_with_precision(ref, p)
+        else:
+            got, exact, ps = ch.sum_precision(p)
+            assert got == want or (np.isnan(got) and np.isnan(want)), (name, p, got, want)
+            R = cf.precision_R(h, p)
+            k = R // 8 + (1 if R % 8 else 0)
+            assert list(ps[:k]) == list(ps_want[:k]), (name, p)
+            sem = cf.sum_semantic_precision(ref, p)
+            assert abs(exact - float(sem)) <= 1e-12 * max(1.0, abs(float(sem))), (name, p)
+            assert sd.byte_residue_sum_with_precision(ref, p) == got
+        # BUFF_SEMANTIC: the exact truncated sum (every bit when the reference would panic)
+        try:
+            sem = cf.sum_semantic_precision(ref, p)
+        except (ValueError, KeyError):
+            sem = cf.exact_sum(ref)[0]
+        r2, e2, _ = ch.sum_precision(p, flags=buff_amd.BUFF_SEMANTIC)
+        assert r2 == e2 and abs(e2 - float(sem)) <= 1e-12 * max(1.0, abs(float(sem))), (name, "semantic", p)
     ch.free()
 
 

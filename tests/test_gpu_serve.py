@@ -40,8 +40,9 @@ def _requests(lens, seed, n):
 def served(gpu_ctx):
     rng = np.random.default_rng(3)
     lens = [int(v) for v in np.exp(rng.uniform(np.log(2000), np.log(300_000), 20))]
-    chunks = [gpu_ctx.encode(dg.outlier_heavy(n, 3, 40 + i), 10 ** 3) for i, n in enumerate(lens)]
-    yield chunks, lens
+    xs = [dg.outlier_heavy(n, 3, 40 + i) for i, n in enumerate(lens)]
+    chunks = [gpu_ctx.encode(x, 10 ** 3) for x in xs]
+    yield chunks, lens, xs
     for c in chunks:
         c.free()
 
@@ -81,8 +82,32 @@ def _same(ctx, lens, q, want, r, buf):
         assert np.array_equal(vals.view(np.int64), want.view(np.int64)), q
 
 
-def test_served_equals_alone_threads(gpu_ctx, served):
-    chunks, lens = served
+def _oracle_same(oracle, streams, lens, q, r, buf):
+    """A served result against the oracle on the chunk's reference stream (the
+    semantic reduced-precision decode against the closed form)."""
+    from tests import closed_form as cf
+
+    op, k, pred, prec, flags = q
+    s = streams[k]
+    assert r.status == 0, (q, r.status)
+    if op == N.BUFF_Q_GT:
+        w_ref, c_ref, _ = oracle.buff_filter(s, 10 ** 3, "range", pred)
+        assert r.count == c_ref, ("oracle", q)
+        if buf is not None:
+            words = buf.download(np.uint32, ((lens[k] + 63) // 64) * 2)[: (lens[k] + 31) // 32]
+            assert np.array_equal(words, w_ref), ("oracle", q)
+    elif op == N.BUFF_Q_SUM:
+        s_ref, _ = oracle.byte_fixed_sum(s, 10 ** 3)
+        tot, _ = cf.exact_sum(s)
+        assert r.value == s_ref and abs(r.exact - float(tot)) <= 1e-12 * abs(float(tot)), ("oracle", q)
+    elif buf is not None:
+        want = cf.decode_semantic_precision(s, prec)
+        vals = buf.download(np.float64, lens[k])
+        assert np.array_equal(vals.view(np.uint64), want.view(np.uint64)), ("oracle", q)
+
+
+def test_served_equals_alone_threads(gpu_ctx, served, oracle):
+    chunks, lens, xs = served
     reqs = _requests(lens, 1, 400)
     alone = [_alone(chunks, q) for q in reqs]
     bufs = _outs(gpu_ctx, lens, reqs)
@@ -107,8 +132,10 @@ def test_served_equals_alone_threads(gpu_ctx, served):
             t.join()
         batches, served_n, largest = srv.stats()
         assert served_n == len(reqs) and batches < len(reqs) and largest > 1
+        streams = [oracle.encode(x, 10 ** 3) for x in xs]
         for q, want, r, b in zip(reqs, alone, results, bufs):
             _same(gpu_ctx, lens, q, want, r, b)
+            _oracle_same(oracle, streams, lens, q, r, b)
     finally:
         srv.destroy()
         for b in bufs:
@@ -116,8 +143,8 @@ def test_served_equals_alone_threads(gpu_ctx, served):
                 b.free()
 
 
-def test_served_trace_and_memory_admission(gpu_ctx, served):
-    chunks, lens = served
+def test_served_trace_and_memory_admission(gpu_ctx, served, oracle):
+    chunks, lens, xs = served
     reqs = _requests(lens, 2, 300)
     alone = [_alone(chunks, q) for q in reqs]
     bufs = _outs(gpu_ctx, lens, reqs)
@@ -129,8 +156,10 @@ def test_served_trace_and_memory_admission(gpu_ctx, served):
         arrival = np.cumsum(np.random.default_rng(9).exponential(1 / 20000.0, len(reqs)))
         res, lat = srv.run_trace(reqs, arrival, [b.ptr if b else 0 for b in bufs])
         assert np.all(lat > 0)
+        streams = [oracle.encode(x, 10 ** 3) for x in xs]
         for q, want, r, b in zip(reqs, alone, res, bufs):
             _same(gpu_ctx, lens, q, want, r, b)
+            _oracle_same(oracle, streams, lens, q, r, b)
         # the same requests without output buffers: only counts and sums come back
         res2, _ = srv.run_trace(reqs, arrival)
         for q, want, r in zip(reqs, alone, res2):

@@ -137,6 +137,9 @@ def _cases():
     yield "tiny_F", np.round(rng.integers(0, 5, 1000) * 0.1, 1), 10       # F < 8
     yield "F16", np.round(np.linspace(0, 2047.9, 2000), 1), 10        # F = 16: F % 8 == 0
     yield "F16_base", np.round(np.linspace(100, 2147.9, 2000), 1), 10  # F = 16 with a non-zero base
+    yield "F8", np.round(np.linspace(0, 7.9, 3001), 1), 10               # F = 8: one plane, no tail
+    yield "F8_base", np.round(np.linspace(100, 107.9, 3001), 1), 10      # F = 8 with a non-zero base
+    yield "F8_negbase", np.round(np.linspace(-307.9, -300, 1999), 1), 10  # F = 8, negative base
     yield "desc_first_max", np.concatenate([[50.0], np.linspace(0, 10, 3000)]), 10 ** 2   # Q1: row 0 is the unique max
     yield "neg_walk", dg.random_walk(2500, 3, 9) - 500, 10 ** 3
     yield "p6", dg.random_walk(3000, 6, 10) * 1e-3, 10 ** 6   # F < dlen: ilen wraps
@@ -207,7 +210,7 @@ def test_condition_queries_match_closed_form(oracle, name, x, scale):
                 oracle.decode_cond(s, scale, rows)
 
 
-@pytest.mark.parametrize("name,x,scale", list(_cases()) + [("F8", np.round(np.linspace(0, 7.9, 3001), 1), 10)])
+@pytest.mark.parametrize("name,x,scale", list(_cases()))
 def test_max_range_matches_closed_form(oracle, name, x, scale):
     # buff_max_range over windows that tile [s, e) exactly, that leave a partial
     # last window (a panic when F > 8), one row, one window, and an empty range
@@ -283,6 +286,45 @@ def test_precision_decode(oracle, name, x, scale):
         if R == h["F"]:
             got = oracle.decode_with_precision(s, scale, p)
             assert np.array_equal(got.view(np.uint64), cf.decode(s).view(np.uint64))
+
+
+def _sum_tolerance(sums, h, R):
+    """1e-12 of the summed magnitudes of the fold's terms (the f64 fold rounds
+    each of its at most 10 terms and partial sums once)."""
+    from fractions import Fraction
+
+    ilen_s = h["F"] - h["dlen"]
+    mag = sum(Fraction(v) * Fraction(2) ** (ilen_s - 8 * (k + 1)) for k, v in enumerate(sums))
+    mag += abs(Fraction(h["n"] * h["base"], 1 << h["dlen"]))
+    return 1e-12 * float(mag) + 2.0 ** -1000
+
+
+@pytest.mark.parametrize("name,x,scale", list(_cases()))
+def test_sum_with_precision(oracle, name, x, scale):
+    # byte_residue_sum_with_precision (split_double.rs:2409-2629) against the
+    # closed form read from the stream bytes: the sub-columns and q-bit fields
+    # the R rule selects, each weighed by its bit position, the base term left
+    # out when the read ends on a byte boundary
+    s = oracle.encode(x, scale)
+    h = cf.parse(s)
+    for p in range(0, 7):
+        try:
+            sums, want = cf.sum_precision(s, p)
+        except (ValueError, KeyError, cf.Panic):
+            with pytest.raises(oracle.RefPanic):
+                oracle.sum_with_precision(s, scale, p)
+            continue
+        if h["ilen"] > 15:
+            with pytest.raises(oracle.RefUnsupported):
+                oracle.sum_with_precision(s, scale, p)
+            continue
+        got, ps = oracle.sum_with_precision(s, scale, p)
+        assert [int(v) for v in ps[: len(sums)]] == sums, (name, p)
+        assert abs(got - float(want)) <= _sum_tolerance(sums, h, cf.precision_R(h, p)), (name, p, got, float(want))
+        R = cf.precision_R(h, p)
+        if R == h["F"] and h["F"] >= 8:
+            # the whole width: byte_fixed_sum's answer
+            assert got == oracle.byte_fixed_sum(s, scale)[0], (name, p)
 
 
 @pytest.mark.parametrize("name,x,scale", list(_cases()))
