@@ -176,8 +176,12 @@ SIGNATURES = {
     "buff_server_add_chunk": [vp, vp, u32p],
     "buff_server_submit": [vp, P(Query), vp, u64p],
     "buff_server_wait": [vp, u64, P(QueryResult), f64p],
+    "buff_server_poll": [vp, u64, P(cint)],
+    "buff_server_release": [vp, vp],
+    "buff_server_arena_stats": [vp, u64p, u64p, u64p],
     "buff_server_run_trace": [vp, P(Query), f64p, P(vp), u32, P(QueryResult), f64p],
     "buff_server_stats": [vp, u64p, u64p, u64p],
+    "buff_trace_per_call": [vp, P(vp), u32, P(Query), f64p, P(vp), u32, P(QueryResult), f64p],
     "buff_server_destroy": [vp],
     "buff_csv_parse": [vp, u8p, u64, ctypes.c_uint8, u32, f64p, u64, u64p],
     "buff_csv_parse_device": [vp, vp, u64, ctypes.c_uint8, u32, vp, u64, u64p],
@@ -503,10 +507,26 @@ class Server:
         return t.value
 
     def wait(self, ticket):
+        """Returns (result, latency_us). A filter or materialization submitted without out_dev has its output
+        at result.out_offset (a device address) until release(result.out_offset)."""
         r = QueryResult()
         lat = ctypes.c_double()
         check(lib().buff_server_wait(self.handle, ticket, ctypes.byref(r), ctypes.byref(lat)))
         return r, lat.value
+
+    def poll(self, ticket):
+        d = ctypes.c_int()
+        check(lib().buff_server_poll(self.handle, ticket, ctypes.byref(d)))
+        return bool(d.value)
+
+    def release(self, out_addr):
+        check(lib().buff_server_release(self.handle, ctypes.c_void_p(out_addr)))
+
+    def arena_stats(self):
+        """(bytes held now, most ever held, arena bytes)."""
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().buff_server_arena_stats(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     def run_trace(self, qs, arrival_s, outs=None):
         """Open-loop replay: request i submitted at arrival_s[i] seconds. Returns (results, latencies_us)."""
@@ -622,6 +642,21 @@ class Context:
 
     def alloc(self, nbytes):
         return DeviceBuffer(self, nbytes)
+
+    def trace_per_call(self, chunks, qs, arrival_s, outs=None):
+        """Open-loop replay of requests as plain single-chunk calls, one at a time (no batching).
+        Returns (results, latencies_us)."""
+        arr, n = make_queries(qs)
+        hs = (ctypes.c_void_p * max(len(chunks), 1))(*[c.handle for c in chunks])
+        t = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        res = (QueryResult * max(n, 1))()
+        lat = np.zeros(max(n, 1), dtype=np.float64)
+        o = None
+        if outs is not None:
+            o = (ctypes.c_void_p * max(n, 1))(*[ctypes.c_void_p(v) if v else None for v in outs])
+        check(lib().buff_trace_per_call(self.handle, hs, len(chunks), arr, ptr(t, ctypes.c_double), o, n, res,
+                                        ptr(lat, ctypes.c_double)))
+        return [res[i] for i in range(n)], lat[:n]
 
     def set_batch_tuning(self, grid, tiles_per_item, first_planes):
         """Pins the batch shape (grid, tiles per item, 1 or 8 first filter planes)

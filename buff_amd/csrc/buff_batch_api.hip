@@ -467,8 +467,9 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
   for (uint32_t k = 0; k < nq; ++k) {
     if (ex.res_host[k].status != BUFF_OK) continue;
     const uint64_t addr = (outs && outs[k]) ? outs[k] : (uint64_t)ex.arena.p + (arena_at[k] == ~0ull ? 0 : arena_at[k]);
+    const bool has_out = qs[k].op == BUFF_Q_GT || qs[k].op == BUFF_Q_EQ || qs[k].op == BUFF_Q_MAT;
     ex.hq[k].out = addr;
-    ex.res_host[k].out_offset = addr;
+    ex.res_host[k].out_offset = has_out ? addr : 0;
   }
   BUFF_TRY(pin_reserve(ex, nq));
   BatchBuffers bb;
@@ -643,7 +644,8 @@ int batch_tune(Ctx* ctx, Batch* b, const buff_query* qs, uint32_t nq, BatchTunin
 
 struct Req {
   buff_query q;
-  uint64_t out = 0;
+  uint64_t out = 0;             // the caller's output address (0: the server arena)
+  uint64_t arena_off = ~0ull;   // the arena range holding its output, until buff_server_release
   double t_submit = 0, t_done = 0;
   buff_query_result res{};
   bool done = false;
@@ -660,6 +662,14 @@ struct Server {
   std::vector<std::vector<buff_query>> slot_qs;
   std::vector<std::vector<Chunk*>> slot_chunks;   // the chunk list a slot's batch was planned on
   std::vector<bool> slot_busy;
+  // Output memory of requests without an address: one device arena of
+  // cfg.arena_bytes; a request is admitted when a range of it is free, and the
+  // range stays held (its bitmap or values readable) until buff_server_release.
+  DevBuf arena;
+  std::map<uint64_t, uint64_t> arena_free;   // offset -> bytes, coalesced
+  std::map<uint64_t, uint64_t> arena_held;   // offset -> bytes
+  bool arena_blocked = false;                // the oldest pending request waits for arena memory
+  uint64_t arena_peak = 0, arena_in_use = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<uint64_t> pending;
@@ -675,6 +685,48 @@ namespace {
 
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit range of the server arena (caller holds sv->mu).
+bool arena_take(Server* sv, uint64_t need, uint64_t* off) {
+  for (auto it = sv->arena_free.begin(); it != sv->arena_free.end(); ++it) {
+    if (it->second < need) continue;
+    const uint64_t o = it->first, sz = it->second;
+    sv->arena_free.erase(it);
+    if (sz > need) sv->arena_free[o + need] = sz - need;
+    sv->arena_held[o] = need;
+    sv->arena_in_use += need;
+    sv->arena_peak = std::max(sv->arena_peak, sv->arena_in_use);
+    *off = o;
+    return true;
+  }
+  return false;
+}
+
+// Returns a held range to the free list, merged with its neighbours (caller holds sv->mu).
+bool arena_give(Server* sv, uint64_t off) {
+  auto h = sv->arena_held.find(off);
+  if (h == sv->arena_held.end()) return false;
+  uint64_t o = off, sz = h->second;
+  sv->arena_held.erase(h);
+  sv->arena_in_use -= sz;
+  auto next = sv->arena_free.find(o + sz);
+  if (next != sv->arena_free.end()) {
+    sz += next->second;
+    sv->arena_free.erase(next);
+  }
+  auto prev = sv->arena_free.lower_bound(o);
+  if (prev != sv->arena_free.begin()) {
+    --prev;
+    if (prev->first + prev->second == o) {
+      o = prev->first;
+      sz += prev->second;
+      sv->arena_free.erase(prev);
+    }
+  }
+  sv->arena_free[o] = sz;
+  sv->arena_blocked = false;
+  return true;
 }
 
 void server_loop(Server* sv) {
@@ -697,6 +749,11 @@ void server_loop(Server* sv) {
         Req& r = sv->reqs[sv->slot_tickets[s][k]];
         r.res = sv->slots[s].res_host[k];
         if (rc != BUFF_OK) r.res.status = rc;
+        if (r.res.status != BUFF_OK && r.arena_off != ~0ull) {   // no output to hold
+          (void)arena_give(sv, r.arena_off);
+          r.arena_off = ~0ull;
+          r.res.out_offset = 0;
+        }
         r.t_done = t;
         r.done = true;
       }
@@ -704,13 +761,25 @@ void server_loop(Server* sv) {
       sv->cv_done.notify_all();
       progressed = true;
     }
-    if (sv->stop && sv->pending.empty()) {
-      bool any = false;
-      for (uint32_t s = 0; s < nslots; ++s) any |= sv->slot_busy[s];
-      if (!any) break;
+    bool any_busy = false;
+    for (uint32_t s = 0; s < nslots; ++s) any_busy |= sv->slot_busy[s];
+    if (sv->stop && (sv->pending.empty() || (sv->arena_blocked && !any_busy))) {
+      if (!any_busy) {
+        // shutting down: requests still waiting for arena memory will not get it
+        const double t = now_s();
+        for (uint64_t id : sv->pending) {
+          Req& r = sv->reqs[id];
+          r.res.status = fail(BUFF_E_INVALID, "server destroyed while the request waited for arena memory");
+          r.t_done = t;
+          r.done = true;
+        }
+        sv->pending.clear();
+        sv->cv_done.notify_all();
+        break;
+      }
     }
     // launches: a free slot takes what fits
-    for (uint32_t i = 0; i < nslots && !sv->pending.empty(); ++i) {
+    for (uint32_t i = 0; i < nslots && !sv->pending.empty() && !sv->arena_blocked; ++i) {
       const uint32_t s = (rr + i) % nslots;
       if (sv->slot_busy[s]) continue;
       std::vector<uint64_t>& tk = sv->slot_tickets[s];
@@ -718,19 +787,32 @@ void server_loop(Server* sv) {
       std::vector<uint64_t> outs;
       tk.clear();
       qv.clear();
-      uint64_t arena = 0;
       while (!sv->pending.empty() && tk.size() < sv->cfg.max_batch) {
         const uint64_t id = sv->pending.front();
         Req& r = sv->reqs[id];
         const uint64_t need =
             (r.out || r.q.chunk >= sv->chunks.size()) ? 0 : arena_need(sv->chunks[r.q.chunk], r.q);
-        if (!tk.empty() && arena + need > sv->cfg.arena_bytes) break;   // waits for memory to free up
-        arena += need;
+        if (need > sv->cfg.arena_bytes) {   // can never be admitted
+          sv->pending.pop_front();
+          r.res.status = fail(BUFF_E_OOM, "the request's output (%llu bytes) exceeds the server arena (%llu bytes)",
+                              (unsigned long long)need, (unsigned long long)sv->cfg.arena_bytes);
+          r.t_done = now_s();
+          r.done = true;
+          sv->cv_done.notify_all();
+          continue;
+        }
+        uint64_t off = 0;
+        if (need && !arena_take(sv, need, &off)) {   // waits until a held range is released
+          sv->arena_blocked = true;
+          break;
+        }
+        if (need) r.arena_off = off;
         sv->pending.pop_front();
         tk.push_back(id);
         qv.push_back(r.q);
-        outs.push_back(r.out);
+        outs.push_back(r.out ? r.out : (need ? (uint64_t)sv->arena.p + off : 0));
       }
+      if (tk.empty()) break;
       const bool vdirty = sv->views_dirty;
       sv->views_dirty = false;
       sv->slot_chunks[s] = sv->chunks;
@@ -754,6 +836,8 @@ void server_loop(Server* sv) {
         const double t = now_s();
         for (uint64_t id : tk) {
           Req& r = sv->reqs[id];
+          if (r.arena_off != ~0ull) (void)arena_give(sv, r.arena_off);
+          r.arena_off = ~0ull;
           r.res.status = rc;
           r.t_done = t;
           r.done = true;
@@ -776,7 +860,7 @@ void server_loop(Server* sv) {
         std::this_thread::yield();   // poll the in-flight batches
         lk.lock();
       } else {
-        sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty(); });
+        sv->cv_work.wait(lk, [&] { return sv->stop || (!sv->pending.empty() && !sv->arena_blocked); });
       }
     }
   }
@@ -873,14 +957,17 @@ int buff_server_create(buff_ctx* ctx, const buff_server_cfg* cfg, buff_server** 
   sv->slot_qs.resize(sv->cfg.slots);
   sv->slot_chunks.resize(sv->cfg.slots);
   sv->slot_busy.assign(sv->cfg.slots, false);
+  int rc = reserve(c, sv->arena, sv->cfg.arena_bytes);
+  sv->arena_free[0] = sv->arena.bytes;
   for (auto& ex : sv->slots) {
-    int rc = exec_init(ex, c, true);
-    if (rc == BUFF_OK) rc = exec_reserve(ex, sv->cfg.max_batch, sv->cfg.arena_bytes, ctx_tuning(c));
-    if (rc != BUFF_OK) {
-      for (auto& e2 : sv->slots) exec_free(e2);
-      delete sv;
-      return rc;
-    }
+    if (rc == BUFF_OK) rc = exec_init(ex, c, true);
+    if (rc == BUFF_OK) rc = exec_reserve(ex, sv->cfg.max_batch, 0, ctx_tuning(c));
+  }
+  if (rc != BUFF_OK) {
+    for (auto& e2 : sv->slots) exec_free(e2);
+    drop(c, sv->arena);
+    delete sv;
+    return rc;
   }
   sv->worker = std::thread(server_loop, sv);
   *out = reinterpret_cast<buff_server*>(sv);
@@ -919,6 +1006,39 @@ int buff_server_submit(buff_server* server, const buff_query* q, void* out_dev, 
   return BUFF_OK;
 }
 
+int buff_server_poll(buff_server* server, uint64_t ticket, int* done) {
+  Server* sv = reinterpret_cast<Server*>(server);
+  if (!sv || !done) return fail(BUFF_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return fail(BUFF_E_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
+  *done = it->second.done ? 1 : 0;
+  return BUFF_OK;
+}
+
+int buff_server_release(buff_server* server, const void* out) {
+  Server* sv = reinterpret_cast<Server*>(server);
+  if (!sv) return fail(BUFF_E_INVALID, "null argument");
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    const uint64_t a = (uint64_t)out, base = (uint64_t)sv->arena.p;
+    if (a < base || a >= base + sv->arena.bytes || !arena_give(sv, a - base))
+      return fail(BUFF_E_INVALID, "%p is not a held output of this server's arena", out);
+  }
+  sv->cv_work.notify_one();
+  return BUFF_OK;
+}
+
+int buff_server_arena_stats(buff_server* server, uint64_t* in_use, uint64_t* peak, uint64_t* capacity) {
+  Server* sv = reinterpret_cast<Server*>(server);
+  if (!sv) return fail(BUFF_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (in_use) *in_use = sv->arena_in_use;
+  if (peak) *peak = sv->arena_peak;
+  if (capacity) *capacity = sv->arena.bytes;
+  return BUFF_OK;
+}
+
 int buff_server_wait(buff_server* server, uint64_t ticket, buff_query_result* res, double* latency_us) {
   Server* sv = reinterpret_cast<Server*>(server);
   if (!sv) return fail(BUFF_E_INVALID, "null argument");
@@ -945,8 +1065,67 @@ int buff_server_run_trace(buff_server* server, const buff_query* qs, const doubl
     while (now_s() - t0 < arrival_s[i]) std::this_thread::yield();
     BUFF_TRY(buff_server_submit(server, &qs[i], outs ? outs[i] : nullptr, &tickets[i]));
   }
-  for (uint32_t i = 0; i < n; ++i)
-    BUFF_TRY(buff_server_wait(server, tickets[i], res ? &res[i] : nullptr, latency_us ? &latency_us[i] : nullptr));
+  for (uint32_t i = 0; i < n; ++i) {
+    buff_query_result r{};
+    BUFF_TRY(buff_server_wait(server, tickets[i], &r, latency_us ? &latency_us[i] : nullptr));
+    // an output the server arena holds goes back as soon as it is complete (its address is not returned)
+    if (r.status == BUFF_OK && !(outs && outs[i]) && (qs[i].op == BUFF_Q_GT || qs[i].op == BUFF_Q_EQ ||
+                                                      qs[i].op == BUFF_Q_MAT)) {
+      BUFF_TRY(buff_server_release(server, (const void*)r.out_offset));
+      r.out_offset = 0;
+    }
+    if (res) res[i] = r;
+  }
+  return BUFF_OK;
+}
+
+int buff_trace_per_call(buff_ctx* ctx, buff_chunk* const* chunks, uint32_t nchunks, const buff_query* qs,
+                        const double* arrival_s, void* const* outs, uint32_t n, buff_query_result* res,
+                        double* latency_us) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || (n && (!qs || !arrival_s)) || (nchunks && !chunks)) return fail(BUFF_E_INVALID, "null argument");
+  // output memory for requests without an address, sized before the replay starts
+  uint64_t max_rows = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (qs[i].chunk >= nchunks) return fail(BUFF_E_INVALID, "request %u: chunk %u outside the list", i, qs[i].chunk);
+    max_rows = std::max<uint64_t>(max_rows, reinterpret_cast<const Chunk*>(chunks[qs[i].chunk])->n);
+  }
+  BUFF_TRY(c->scratch(c->s_values, std::max<uint64_t>(max_rows * 8, 256)));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const double t0 = now_s();
+  for (uint32_t i = 0; i < n; ++i) {
+    while (now_s() - t0 < arrival_s[i]) std::this_thread::yield();
+    const buff_query& q = qs[i];
+    const Chunk* ch = reinterpret_cast<const Chunk*>(chunks[q.chunk]);
+    void* out = (outs && outs[i]) ? outs[i] : c->s_values.ptr;
+    buff_query_result r{};
+    switch (q.op) {
+      case BUFF_Q_GT:
+      case BUFF_Q_EQ:
+        r.status = filter_device(c, ch, q.op == BUFF_Q_GT ? BUFF_OP_GT : BUFF_OP_EQ, q.pred, q.flags,
+                                 (uint64_t*)out, nullptr);
+        r.out_offset = (outs && outs[i]) ? (uint64_t)out : 0;
+        break;
+      case BUFF_Q_SUM:
+        r.status = sum_op(c, ch, &r.value, &r.exact, nullptr);
+        break;
+      case BUFF_Q_MAX:
+        r.status = max_op(c, ch, q.flags, &r.value, &r.max_off, nullptr);
+        break;
+      case BUFF_Q_MAT:
+        r.status = materialize_device(c, ch, q.precision, q.flags, (double*)out);
+        r.out_offset = (outs && outs[i]) ? (uint64_t)out : 0;
+        break;
+      default:
+        r.status = fail(BUFF_E_INVALID, "request %u: unknown op %u", i, q.op);
+    }
+    if (r.status == BUFF_OK) {
+      const hipError_t e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) return fail(BUFF_E_DEVICE, "per-call replay: %s", hipGetErrorString(e));
+    }
+    if (latency_us) latency_us[i] = (now_s() - t0 - arrival_s[i]) * 1e6;
+    if (res) res[i] = r;
+  }
   return BUFF_OK;
 }
 
@@ -971,6 +1150,7 @@ int buff_server_destroy(buff_server* server) {
   if (sv->worker.joinable()) sv->worker.join();
   for (auto& ex : sv->slots) exec_free(ex);
   drop(sv->ctx, sv->views);
+  drop(sv->ctx, sv->arena);
   delete sv;
   return BUFF_OK;
 }

@@ -352,31 +352,62 @@ int buff_batch_set_tuning(buff_ctx* ctx, uint32_t grid, uint32_t tiles_per_item,
  * A request loop over resident chunks with continuous batching: requests that
  * arrive while batches run on the GPU are planned into the next batch as soon as
  * one of `slots` in-flight batch slots is free (each slot has its own stream, so
- * batches overlap on the device) and while the batch has room: max_batch
- * requests and arena_bytes of output memory for requests without an output
- * address (the rest wait for the next batch). Results equal the single-chunk
- * calls'. No reference counterpart (the reference has no query server). */
+ * batches overlap on the device) and while the batch has room (max_batch
+ * requests). Admission by memory: a filter or materialization submitted
+ * without an output address gets its output range in the server's arena
+ * (arena_bytes of device memory); requests are admitted in arrival order, and
+ * the oldest waits (holding back the ones behind it) until enough of the arena
+ * is free. Such a range stays held, and the output at the result's out_offset
+ * stays valid, until the client returns it with buff_server_release. A request
+ * whose output is larger than the whole arena fails with BUFF_E_OOM. Results
+ * equal the single-chunk calls'. No reference counterpart (the reference has
+ * no query server). */
 typedef struct buff_server buff_server;
 typedef struct {
   uint32_t max_batch;   /* requests per batch (0: 4096) */
   uint32_t slots;       /* batches in flight (0: 2) */
-  uint64_t arena_bytes; /* output memory per batch for requests without out_dev (0: 1 GiB) */
+  uint64_t arena_bytes; /* the server's output arena for requests without out_dev (0: 1 GiB) */
 } buff_server_cfg;
 
 int buff_server_create(buff_ctx* ctx, const buff_server_cfg* cfg, buff_server** out);
 /* Makes a resident byte_fixed chunk of the same context servable; *id is the
- * chunk index requests name. The caller keeps ownership (free it after the server). */
+ * chunk index requests name. The caller keeps ownership (free it after the
+ * server). Calls from several client threads may overlap; every call on the
+ * server is serialized on its lock. */
 int buff_server_add_chunk(buff_server* server, buff_chunk* chunk, uint32_t* id);
 /* Queues one request (buff_query.chunk = server chunk id). out_dev: where a
  * filter writes its ceil(n/64) bitmap words or a materialization its n doubles
- * (NULL: server memory, overwritten by later batches). */
+ * (NULL: a range of the server arena, held until buff_server_release). */
 int buff_server_submit(buff_server* server, const buff_query* q, void* out_dev, uint64_t* ticket);
-/* Blocks until the request completes; latency_us = completion - submission. */
+/* Blocks until the request completes; latency_us = completion - submission.
+ * result->out_offset is the device address of a filter's bitmap or a
+ * materialization's values (out_dev, or the held arena range); 0 for sums and
+ * maxima and for failed requests. The ticket is spent afterwards. */
 int buff_server_wait(buff_server* server, uint64_t ticket, buff_query_result* result, double* latency_us);
+/* *done = 1 when the request has completed (it can then be waited without blocking). */
+int buff_server_poll(buff_server* server, uint64_t ticket, int* done);
+/* Returns a completed request's arena output range (its out_offset) to the
+ * arena; requests waiting for arena memory are admitted as ranges return. */
+int buff_server_release(buff_server* server, const void* out);
+/* Arena bytes held now, the most ever held at once, and the arena's size. */
+int buff_server_arena_stats(buff_server* server, uint64_t* in_use, uint64_t* peak, uint64_t* capacity);
 /* Submits request i at arrival_s[i] seconds after the call (a native open-loop
- * client), then waits for all of them. outs may be NULL. */
+ * client), then waits for all of them in order, returning each arena output
+ * range as soon as its request completes (results[i].out_offset is then 0;
+ * give outs[i] to keep an output). outs may be NULL. */
 int buff_server_run_trace(buff_server* server, const buff_query* qs, const double* arrival_s, void* const* outs,
                           uint32_t n, buff_query_result* results, double* latency_us);
+/* The same open-loop replay without a server: each request, in arrival order,
+ * is the plain single-chunk call on the context's stream (buff_filter_device,
+ * buff_sum, buff_max, buff_materialize_device; one query per launch) followed
+ * by a stream synchronization; latency_us[i] = completion - arrival. Filters
+ * write their bitmap but no count (results[i].count = 0). Outputs of requests
+ * without outs[i] go to one context buffer that each call overwrites. chunks:
+ * the resident chunks requests name by index. The baseline that continuous
+ * batching is measured against. */
+int buff_trace_per_call(buff_ctx* ctx, buff_chunk* const* chunks, uint32_t nchunks, const buff_query* qs,
+                        const double* arrival_s, void* const* outs, uint32_t n, buff_query_result* results,
+                        double* latency_us);
 /* Batches launched, requests served, largest batch; returns a sticky device error if any. */
 int buff_server_stats(buff_server* server, uint64_t* batches, uint64_t* queries, uint64_t* largest_batch);
 int buff_server_destroy(buff_server* server);

@@ -59,6 +59,18 @@ def _width(mn, mx):
     return int(math.ceil(math.log2(float(d)))) if d else 0
 
 
+def _u64_sum(a):
+    """Exact sum of a uint64 array (two 32-bit halves, so no partial sum wraps)."""
+    a = np.asarray(a, dtype=np.uint64)
+    return (int(np.sum(a >> np.uint64(32), dtype=np.uint64)) << 32) + int(np.sum(a & np.uint64(0xFFFFFFFF),
+                                                                                 dtype=np.uint64))
+
+
+def _exact_fixed_sum(f):
+    mn = int(f.min())
+    return len(f) * mn + _u64_sum((f - mn).astype(np.uint64))
+
+
 def _words(mask):
     b = np.packbits(mask, bitorder="little")
     b = np.concatenate([b, np.zeros((-len(b)) % 4, np.uint8)])
@@ -108,12 +120,12 @@ def _check_plain_chunk(ch, x, scale, preds, mat_precisions):
             m = f == fp
             assert c == int(m.sum()) and np.array_equal(w, _words(m)), ("equal", pred)
     # sum: exact, and the reference fold within 1e-12 of the summed magnitudes
-    tot = int(np.sum(f, dtype=np.int64))
+    tot = _exact_fixed_sum(f)
     ref, exact, ps = ch.sum()
     assert exact == float(Fraction(tot, 1 << dlen))
     for k in range(P):
         assert int(ps[k]) == int(np.sum((off >> np.uint64(F - 8 * (k + 1))) & np.uint64(0xFF), dtype=np.uint64))
-    mag = (n * abs(mn) + int(np.sum(off, dtype=np.uint64))) / 2.0 ** dlen
+    mag = (n * abs(mn) + _u64_sum(off)) / 2.0 ** dlen
     want = float(Fraction(tot, 1 << dlen)) if F % 8 else float(Fraction(tot - n * mn, 1 << dlen))
     assert abs(ref - want) <= 1e-12 * mag
     # max
@@ -197,7 +209,7 @@ def test_config3_outlier_heavy_64M(gpu_ctx, oracle):
             w, c = major.filter(op, pred, flags=buff_amd.BUFF_SEMANTIC)
             assert c == int(m.sum()) and np.array_equal(w, _words(m)), ("semantic", op, pred)
         r, e, _ = major.sum(flags=buff_amd.BUFF_SEMANTIC)
-        assert e == float(Fraction(int(np.sum(f, dtype=np.int64)), 1 << dlen))
+        assert e == float(Fraction(_exact_fixed_sum(f), 1 << dlen))
         v, o, _ = major.max_ex(flags=buff_amd.BUFF_SEMANTIC)
         assert v == float(f.max()) / 2.0 ** dlen
         got = major.materialize(-1, flags=buff_amd.BUFF_SEMANTIC)
@@ -285,7 +297,7 @@ def test_config4_batch_4096_chunks(gpu_ctx, oracle):
                 if r.count != int(m.sum()) or not np.array_equal(bms[j], _words(m)):
                     return "filter"
             elif op == N.BUFF_Q_SUM:
-                so = int(np.sum(off, dtype=np.uint64))
+                so = _u64_sum(off)
                 if r.exact != float(Fraction(n * mn + so, 1 << dlen)):
                     return "exact sum"
                 mag = (n * abs(mn) + so) / 2.0 ** dlen

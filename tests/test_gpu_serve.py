@@ -175,3 +175,84 @@ def test_served_trace_and_memory_admission(gpu_ctx, served, oracle):
         for b in bufs:
             if b:
                 b.free()
+
+
+def _d2h(ctx, addr, dtype, count):
+    out = np.zeros(count, dtype=dtype)
+    N.check(N.lib().buff_memcpy_d2h(ctx.handle, out.ctypes.data, addr, out.nbytes))
+    return out
+
+
+def test_served_arena_outputs_held_until_release(gpu_ctx, served, oracle):
+    # admission by memory: an arena with room for one bitmap; a served bitmap stays
+    # readable until it is released, and the next request waits for that release
+    chunks, lens, xs = served
+    k = int(np.argmax(lens))
+    n = lens[k]
+    nb = ((n + 63) // 64) * 8
+    arena = (nb + 255) // 256 * 256
+    stream = oracle.encode(xs[k], 10 ** 3)
+    srv = buff_amd.Server(gpu_ctx, max_batch=64, slots=2, arena_bytes=arena)
+    try:
+        for c in chunks:
+            srv.add_chunk(c)
+        t1 = srv.submit(N.BUFF_Q_GT, k, 100.0)
+        r1, _ = srv.wait(t1)
+        assert r1.status == 0 and r1.out_offset
+        w_ref, c_ref, _ = oracle.buff_filter(stream, 10 ** 3, "range", 100.0)
+        assert r1.count == c_ref
+        t2 = srv.submit(N.BUFF_Q_GT, k, 100.4)
+        t3 = srv.submit(N.BUFF_Q_SUM, k)   # behind t2 in arrival order
+        import time
+        time.sleep(0.2)
+        assert not srv.poll(t2) and not srv.poll(t3), "admitted without arena memory"
+        assert srv.arena_stats()[0] == arena
+        got = _d2h(gpu_ctx, r1.out_offset, np.uint32, (n + 31) // 32)
+        assert np.array_equal(got, w_ref), "a held bitmap was overwritten"
+        srv.release(r1.out_offset)
+        r2, _ = srv.wait(t2)
+        r3, _ = srv.wait(t3)
+        assert r2.status == 0 and r3.status == 0 and r3.out_offset == 0
+        w2, c2, _ = oracle.buff_filter(stream, 10 ** 3, "range", 100.4)
+        assert r2.count == c2 and np.array_equal(_d2h(gpu_ctx, r2.out_offset, np.uint32, (n + 31) // 32), w2)
+        srv.release(r2.out_offset)
+        with pytest.raises(N.BuffError):
+            srv.release(r2.out_offset)   # already returned
+        # an output larger than the whole arena can never be admitted
+        t4 = srv.submit(N.BUFF_Q_MAT, k, 0.0, 2, N.BUFF_SEMANTIC)
+        r4, _ = srv.wait(t4)
+        assert r4.status == N.BUFF_E_OOM and r4.out_offset == 0
+        in_use, peak, cap = srv.arena_stats()
+        assert in_use == 0 and peak == arena and cap == arena
+    finally:
+        srv.destroy()
+
+
+def test_trace_per_call_baseline(gpu_ctx, served, oracle):
+    # the plain one-call-per-request replay (the baseline continuous batching is
+    # measured against) answers like the single-chunk calls
+    chunks, lens, xs = served
+    reqs = _requests(lens, 3, 120)
+    bufs = _outs(gpu_ctx, lens, reqs)
+    try:
+        arrival = np.cumsum(np.random.default_rng(4).exponential(1 / 5000.0, len(reqs)))
+        res, lat = gpu_ctx.trace_per_call(chunks, reqs, arrival, [b.ptr if b else 0 for b in bufs])
+        assert np.all(lat > 0)
+        streams = [oracle.encode(x, 10 ** 3) for x in xs]
+        for q, r, b in zip(reqs, res, bufs):
+            assert r.status == 0
+            if q[0] == N.BUFF_Q_GT:
+                w_ref, _, _ = oracle.buff_filter(streams[q[1]], 10 ** 3, "range", q[2])
+                words = b.download(np.uint32, ((lens[q[1]] + 63) // 64) * 2)[: (lens[q[1]] + 31) // 32]
+                assert np.array_equal(words, w_ref), q
+            elif q[0] == N.BUFF_Q_SUM:
+                assert (r.value, r.exact) == tuple(chunks[q[1]].sum()[:2]), q
+            else:
+                from tests import closed_form as cf
+                want = cf.decode_semantic_precision(streams[q[1]], q[3])
+                vals = b.download(np.float64, lens[q[1]])
+                assert np.array_equal(vals.view(np.uint64), want.view(np.uint64)), q
+    finally:
+        for b in bufs:
+            if b:
+                b.free()
