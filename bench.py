@@ -171,6 +171,33 @@ def _words64(mask):
     return np.concatenate([b, np.zeros((-len(b)) % 8, np.uint8)]).view(np.uint64)
 
 
+def filter_line_bytes(off, info, fp):
+    """HBM bytes a range filter must move at 128-byte line granularity: every
+    line of the top plane and of the result bitmap, then on each later plane the
+    lines holding a row still tied with the target (128 consecutive rows share a
+    line of a plane), and the tail bit-planes of the tiles with tied rows left
+    (one line per bit-plane and 1024-row tile)."""
+    n, F, P, r = info.n, info.width, info.planes, info.tail_bits
+    ntiles = (n + 1023) // 1024
+    b = ntiles * 1024.0 * (1 if P else 0) + ((n + 63) // 64) * 8.0
+    t = fp - info.base
+    if t < 0 or not P:
+        return b
+    t &= (1 << F) - 1
+    cand = None
+    for p in range(P):
+        sh = np.uint64(F - 8 * (p + 1))
+        tb = np.uint64((t >> int(sh)) & 0xFF)
+        if cand is None:
+            cand = np.flatnonzero(((off >> sh) & np.uint64(0xFF)) == tb)
+        else:
+            b += 128.0 * len(np.unique(cand // 128))
+            cand = cand[((off[cand] >> sh) & np.uint64(0xFF)) == tb]
+    if r and len(cand):
+        b += 128.0 * r * len(np.unique(cand // 1024))
+    return b
+
+
 def verify_plain(ctx, ch, x, preds, mat_precisions, words, out, buff_amd):
     """Checks a plain chunk's answers against numpy on its stored offsets: the
     base, range-filter bitmaps and counts at each predicate, the exact sum and
@@ -247,6 +274,48 @@ def verify_major(ch, x, preds, words, out, buff_amd):
                           (f.astype(np.float64) / 2.0 ** dlen).view(np.uint64)):
         bad.append("decode")
     return bad
+
+
+def major_range_count(off, info, majors, fp):
+    """Rows buff_range_filter_majority (buff_simd.rs:863-1277) reports for a
+    fixed-point predicate fp, from the stored offsets of a buff-major chunk:
+    sparse column 0 as a plain byte; a later sparse column adds its outliers
+    above the target byte, and its non-outlier rows when the majority byte is
+    above it (target 0 with majority 0 keeps the candidates' outliers and ends
+    the scan); then the dense planes and the tail on the remaining candidates."""
+    n, F, S, base = info.n, info.width, info.sparse_cols, info.base
+    if fp < base:
+        return n
+    t = fp - base
+    cand = np.ones(n, bool)
+    res = np.zeros(n, bool)
+    for j in range(S):
+        sh = np.uint64(F - 8 * (j + 1))
+        b = (off >> sh) & np.uint64(0xFF)
+        tb = np.uint64((t >> int(sh)) & 0xFF)
+        mask = b != np.uint64(majors[j])
+        if j == 0:
+            res |= cand & (b > tb)
+            cand &= b == tb
+        elif tb == 0 and majors[j] == 0:
+            res |= cand & mask
+            cand[:] = False
+        else:
+            res |= mask & (b > tb)
+            if majors[j] > tb:
+                res |= cand & ~mask
+            cand &= b == tb
+    for p in range(info.planes):
+        sh = np.uint64(F - 8 * (S + p + 1))
+        b = (off >> sh) & np.uint64(0xFF)
+        tb = np.uint64((t >> int(sh)) & 0xFF)
+        res |= cand & (b > tb)
+        cand &= b == tb
+    r = info.tail_bits
+    if r:
+        lowm = np.uint64((1 << r) - 1)
+        res |= cand & ((off & lowm) > np.uint64(t & ((1 << r) - 1)))
+    return int(np.count_nonzero(res))
 
 
 def major_filter_rows(x, info, majors, pred, dlen):
@@ -680,7 +749,8 @@ def bench_config4(ctx, args, buff_amd):
 
 def bench_config5(ctx_unused, args, buff_amd):
     """Served form: outlier-heavy chunks (95% N(100,0.5), 5% +-10^U[6,9], 3
-    decimals) with lengths log-uniform in [2^20, 2^26], encoded resident until
+    decimals) with lengths log-uniform in [2^20, 2^26], encoded in the buff-major
+    layout (buff_encode_majority, the layout the outlier-heavy column defines) and kept resident until
     the compressed bytes reach the target (the chunk data repeats a pool of 64
     distinct generated columns), in a context of its own with a stated HBM
     budget. A Poisson request stream (60% range filters at a random predicate,
@@ -718,7 +788,7 @@ def bench_config5(ctx_unused, args, buff_amd):
     while resident < target:
         i = len(chunks) % K
         try:
-            ch = ctx.encode_device(xd.ptr + 8 * int(poff[i]), int(plens[i]), 10 ** 3)
+            ch = ctx.encode_device(xd.ptr + 8 * int(poff[i]), int(plens[i]), 10 ** 3, flags=buff_amd.BUFF_MAJORITY)
         except N.OutOfMemory:
             break   # the HBM budget admits no more chunks
         chunks.append(ch)
@@ -728,7 +798,7 @@ def bench_config5(ctx_unused, args, buff_amd):
     log("bench: served store %d chunks, %.1f GB resident, %d rows, built in %.1fs" % (
         len(chunks), resident / 1e9, sum(lens), time.time() - t0))
     arena_bytes = 8 << 30
-    res = {"workload": "outlier-heavy chunks, lengths log-uniform in [2^20, 2^26], 3 decimals, byte_fixed layout",
+    res = {"workload": "outlier-heavy chunks, lengths log-uniform in [2^20, 2^26], 3 decimals, buff-major layout",
            "chunks": len(chunks), "resident_bytes": int(resident), "rows": int(sum(lens)),
            "mix": "60% range filter, 25% sum, 15% materialize at 1-3 decimals (semantic)",
            "hbm_budget_bytes": budget, "server_arena_bytes": arena_bytes}
@@ -738,14 +808,14 @@ def bench_config5(ctx_unused, args, buff_amd):
     qpreds = (99.5, 100.0, 100.5, 1e6)
 
     def pool_answers(i):
+        # the chunks store every offset bit (the buff-major encoder's min/max is exact)
         info = chunks[i].info()
-        off = stored_offsets(xh[poff[i]:poff[i + 1]], info)
-        n, F, dlen, mn = info.n, info.width, info.dlen, info.base
+        off = (_fixed(xh[poff[i]:poff[i + 1]], info.dlen) - info.base).astype(np.uint64)
+        majors = [chunks[i].sparse_col(j)[0] for j in range(info.sparse_cols)]
         counts = {}
         for pred in qpreds:
-            fp = int(_fixed(np.array([pred]), dlen)[0])
-            counts[pred] = n if fp < mn else int(np.count_nonzero(off > np.uint64((fp - mn) & ((1 << F) - 1))))
-        return counts, float(Fraction(n * mn + u64_sum(off), 1 << dlen))
+            counts[pred] = major_range_count(off, info, majors, int(_fixed(np.array([pred]), info.dlen)[0]))
+        return counts, float(Fraction(info.n * info.base + u64_sum(off), 1 << info.dlen))
 
     with ThreadPoolExecutor(16) as ex:
         answers = list(ex.map(pool_answers, range(min(K, len(chunks)))))
@@ -1000,6 +1070,13 @@ def main():
                   "k_materialize": [algorithmic_bytes(i, "k_materialize") for i in infos]}
     cold = {}
     trials = 3
+    line_bytes = {}
+    for c, info, x in zip(cols, infos, (x_walk, x_uni)):
+        off = stored_offsets(x, info)
+        for sel in (0.001, 0.10, 0.50):
+            line_bytes[(c.name, sel)] = filter_line_bytes(off, info, int(_fixed(np.array([c.preds[sel]]),
+                                                                                info.dlen)[0]))
+        del off
     # each range filter alone, cold: per column and selectivity
     filters = {}
     for c, ch in zip(cols, keep):
@@ -1016,9 +1093,13 @@ def main():
             if launches:
                 us = total_ms / launches * 1e3
                 gbps = filt[(c.name, sel)] / (us * 1e-6) / 1e9
+                lb = line_bytes[(c.name, sel)]
                 filters["%s@%g%%" % (c.name, sel * 100)] = {"avg_us": round(us, 2), "achieved_GBps": round(gbps, 1),
                                                             "frac": round(gbps / HBM_PEAK_GBPS, 4),
-                                                            "algorithmic_bytes": round(filt[(c.name, sel)])}
+                                                            "algorithmic_bytes": round(filt[(c.name, sel)]),
+                                                            "line_granular_bytes": round(lb),
+                                                            "frac_of_line_granular_roofline":
+                                                                round(lb / (us * 1e-6) / 1e9 / HBM_PEAK_GBPS, 4)}
     ctx.prof_reset()
     for trial in range(trials):
         for c, ch in zip(cols, keep):

@@ -1120,7 +1120,7 @@ int materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t fl
     precision = -1;
   }
   if (ch->flags & BUFF_MAJORITY) {
-    if (precision >= 0)
+    if (precision >= 0 && !(flags & BUFF_SEMANTIC))
       return fail(BUFF_E_UNSUPPORTED, "the reference has no reduced-precision decode of a buff-major stream");
     if (ch->S > 0) return major_materialize_device(ctx, ch, precision, flags, out, nullptr);
   }
@@ -1268,7 +1268,7 @@ int tune_chunk(Ctx* ctx, const Chunk* ch, double pred, buff_tune_choice* out) {
     for (uint32_t waves : {2u, 4u}) {
       for (uint32_t cap : {0u, 1024u, 2048u, 4096u, 8192u, 16384u}) {
         for (uint32_t t : tiles_opts[oi]) {
-          for (uint32_t pf = 0; pf < (op == Op::FilterGt ? 2u : 1u); ++pf) {
+          for (uint32_t pf = 0; pf < ((op == Op::FilterGt || op == Op::Max) ? 2u : 1u); ++pf) {
             if (rc != BUFF_OK) break;
             if (op == Op::Materialize && waves != 4) continue;   // its LDS staging assumes four waves
             const LaunchShape ls{waves, cap, t, pf};

@@ -13,6 +13,7 @@
 
 #include "buff_common.h"
 #include "buff_kernels.h"
+#include "buff_major.h"
 
 namespace buffgpu {
 
@@ -57,9 +58,9 @@ struct BQuery {
   uint32_t chunk;        // index into the batch's chunk views
   uint32_t fill;         // filters: 0 = run, 1 = no rows, 2 = all rows (decided on the host)
   uint32_t item0, nitems;
-  uint32_t pad;
-  FilterTarget tg;       // filters
-  MatParams mp;          // materialize
+  uint32_t sparse;       // buff-major chunk with sparse columns: S (bits 0-7), BUFF_SEMANTIC (bit 8); else 0
+  FilterTarget tg;       // filters (on a buff-major chunk: the target byte of every byte column, sparse first)
+  MatParams mp;          // materialize (buff-major: mode kMatSemantic keeps the top F - drop bits)
   uint64_t out;          // filters: device address of the ceil(n/64) bitmap words;
                          // materialize: device address of the n doubles
   uint64_t top;          // max: byte offset of its per-tile top bytes in the max arena
@@ -85,6 +86,7 @@ struct BatchBuffers {
   BResult* host_res;                 // pinned host: nq results, written by the last kernel
   const BHeader* hdr;
   const ChunkView* views;
+  const MajorView* majors;           // per chunk, read only by queries whose chunk has sparse columns
   const BQuery* q;
   BItem* items;                      // written by the batch's first kernel
   uint8_t* tile_top;                 // max: per-tile top byte
@@ -108,8 +110,11 @@ struct BatchTuning {
 constexpr uint64_t kHostQueries = 256;   // byte offset of the queries in host_in
 
 // Kernels a batch needs: the scan (filters, sums, maxima and their fold), the
-// max refinement, the materialization.
-enum : uint32_t { kBqPartScan = 1, kBqPartMax = 2, kBqPartMat = 4 };
+// max refinement, the materialization, and the same for queries on buff-major
+// chunks with sparse columns (kernels of their own, so the plain kernels keep
+// their register budget).
+enum : uint32_t { kBqPartScan = 1, kBqPartMax = 2, kBqPartMat = 4, kBqPartMajor = 8, kBqPartMajorMat = 16 };
+constexpr uint32_t kBqSparseSem = 1u << 8;
 
 // The launch sequence of one batch (captured in a graph by the host). The
 // kernels read every buffer address and count from `bb` (device memory), so one

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "buff_batch.h"
+#include "buff_major_device.h"
 #include "buff_common.h"
 #include "buff_device.h"
 
@@ -391,6 +392,7 @@ __global__ __launch_bounds__(256) void k_bq_scan(const BatchBuffers* __restrict_
   for (uint32_t i = ws.first; i < nscan; i += ws.stride) {
     const BItem it = cload(bb.items, i);
     const BQuery q = cload(bb.q, it.owner);
+    if (q.sparse) continue;   // k_bq_major
     const ChunkView cv = cload(bb.views, it.chunk);
     const uint32_t op = q.op;
     unsigned long long* dst = bb.item_acc + (uint64_t)i * (kMaxPlanes + 1);
@@ -658,6 +660,7 @@ __global__ __launch_bounds__(256) void k_bq_mat(const BatchBuffers* __restrict__
   for (uint32_t i = nscan + ws.first; i < nitems; i += ws.stride) {
     const BItem it = cload(bb.items, i);
     const BQuery q = cload(bb.q, it.owner);
+    if (q.sparse) continue;   // k_bq_major_mat
     const ChunkView cv = cload(bb.views, it.chunk);
     const MatParams& mp = q.mp;
     double* outv = reinterpret_cast<double*>(q.out);
@@ -682,11 +685,185 @@ __global__ __launch_bounds__(256) void k_bq_mat(const BatchBuffers* __restrict__
   }
 }
 
+// ---- queries on buff-major chunks with sparse columns -------------------------
+// Filters run the single-chunk filter's tile function (major_filter_group:
+// the reference's rules, or with BUFF_SEMANTIC the intended answer); sums add
+// up the dense planes and tail like a plain chunk and each sparse column's
+// outlier bytes (the host adds major * (n - outliers) and folds the columns);
+// materializations build the offsets with major_tile_offsets.
+
+template <int MODE, int S>
+__device__ __forceinline__ uint32_t major_filter_item(const MajorPrefix<S>& mv, const BQuery& q, const BItem& it,
+                                                      uint32_t lane, bool sem) {
+  BUFF_GLOBAL uint16_t* out16 = gmem_w(reinterpret_cast<uint16_t*>(q.out));
+  MajorTarget tg;
+#pragma unroll
+  for (uint32_t k = 0; k < kMaxPlanes; ++k) tg.digit[k] = q.tg.digit[k];
+  tg.tail = q.tg.tail;
+  uint32_t count = 0;
+  #pragma unroll 1
+  for (uint64_t c = it.c0; c < it.c1; ++c) {
+    uint32_t res[1];
+    major_filter_group<MODE, S, 1>(mv, tg, sem, c, lane, res);
+    filter_store<MODE>(mv.dense, out16, c, lane, res[0], count);
+  }
+  return count;
+}
+
+// Per-lane outlier byte sums of every sparse column over tiles [c0, c1).
+template <int S>
+__device__ __forceinline__ void major_outlier_sums(const MajorPrefix<S>& mv, const BItem& it, uint32_t lane,
+                                                   uint32_t (&osum)[S]) {
+  #pragma unroll 1
+  for (uint64_t c = it.c0; c < it.c1; ++c) {
+    uint32_t m[S], pre[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      m[j] = mv.col[j].mask16[c * 64 + lane];
+      pre[j] = mv.col[j].prefix[c];
+    }
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const uint32_t excl = wave_excl_scan_u32(__popc(m[j]), lane);
+      if (m[j]) osum[j] += packed_sum(load_packed(mv.col[j].outl, (uint64_t)pre[j] + excl), m[j]);
+    }
+  }
+}
+
+// One scan item of a query on a buff-major chunk with S sparse columns (only
+// the view's first S columns are read into registers).
+template <int S>
+__device__ __forceinline__ void major_scan_item(const BatchBuffers& bb, const BQuery& q, const BItem& it, uint32_t i,
+                                                uint32_t lane) {
+  const MajorPrefix<S> mv = cload(reinterpret_cast<const MajorPrefix<S>*>(bb.majors + it.chunk));
+  const bool sem = (q.sparse & kBqSparseSem) != 0;
+  unsigned long long* dst = bb.item_acc + (uint64_t)i * (kMaxPlanes + 1);
+  if (q.op == kBqGt || q.op == kBqEq) {
+    const uint32_t cnt = q.op == kBqGt ? major_filter_item<0, S>(mv, q, it, lane, sem)
+                                       : major_filter_item<1, S>(mv, q, it, lane, sem);
+    const uint32_t tot = wave_total_u32(cnt);
+    if (lane == 0) dst[0] = tot;
+  } else if (q.op == kBqSum) {
+    const ChunkView& cv = mv.dense;
+    uint32_t acc[kMaxPlanes + 1];
+#pragma unroll
+    for (uint32_t k = 0; k <= kMaxPlanes; ++k) acc[k] = 0;
+    switch (cv.P) {
+      case 0: sum_item<0>(cv, it, lane, acc); break;
+      case 1: sum_item<1>(cv, it, lane, acc); break;
+      case 2: sum_item<2>(cv, it, lane, acc); break;
+      case 3: sum_item<3>(cv, it, lane, acc); break;
+      case 4: sum_item<4>(cv, it, lane, acc); break;
+      case 5: sum_item<5>(cv, it, lane, acc); break;
+      default: sum_item<6>(cv, it, lane, acc); break;
+    }
+    uint32_t osum[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) osum[j] = 0;
+    major_outlier_sums<S>(mv, it, lane, osum);
+    // slots: dense planes 0..P-1, outlier sums P..P+S-1, tail kMaxPlanes
+#pragma unroll
+    for (uint32_t k = 0; k < kMaxPlanes; ++k) {
+      uint32_t v = k < cv.P ? acc[k] : 0u;
+#pragma unroll
+      for (int j = 0; j < S; ++j) v = k == cv.P + j ? osum[j] : v;
+      const uint32_t tot = wave_total_u32(v);
+      if (lane == 0) dst[k] = tot;
+    }
+    const uint32_t tail = wave_total_u32(acc[kMaxPlanes]);
+    if (lane == 0) dst[kMaxPlanes] = tail;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bq_major(const BatchBuffers* __restrict__ bbp) {
+  const BatchBuffers bb = cload(bbp);
+  const WaveSlot ws = wave_slot();
+  const uint32_t lane = ws.lane;
+  const uint32_t nscan = cmem(bb.hdr)->nscan;
+  #pragma unroll 1
+  for (uint32_t i = ws.first; i < nscan; i += ws.stride) {
+    const BItem it = cload(bb.items, i);
+    const BQuery q = cload(bb.q, it.owner);
+    switch (q.sparse & 0xFFu) {
+      case 0: break;
+      case 1: major_scan_item<1>(bb, q, it, i, lane); break;
+      case 2: major_scan_item<2>(bb, q, it, i, lane); break;
+      case 3: major_scan_item<3>(bb, q, it, i, lane); break;
+      case 4: major_scan_item<4>(bb, q, it, i, lane); break;
+      case 5: major_scan_item<5>(bb, q, it, i, lane); break;
+      case 6: major_scan_item<6>(bb, q, it, i, lane); break;
+      default: major_scan_item<7>(bb, q, it, i, lane); break;
+    }
+  }
+}
+
+template <int SEM, int S>
+__device__ __forceinline__ void major_mat_item(const BatchBuffers& bb, const BQuery& q, const BItem& it, uint32_t lane,
+                                               double* sv) {
+  const MajorPrefix<S> mv = cload(reinterpret_cast<const MajorPrefix<S>*>(bb.majors + it.chunk));
+  const ChunkView& dv = mv.dense;
+  double* outv = reinterpret_cast<double*>(q.out);
+  const uint32_t drop = SEM ? q.mp.drop : 0u;
+  #pragma unroll 1
+  for (uint64_t c = it.c0; c < it.c1; ++c) {
+    uint64_t v[16];
+    major_tile_offsets<SEM, S>(mv, c, lane, v);
+#pragma unroll
+    for (uint32_t b = 0; b < 16; ++b) {
+      const uint64_t o = (v[b] >> drop) << drop;
+      sv[kLdsRow * lane + b] = (double)(int64_t)((uint64_t)dv.base + o) * q.mp.inv_scale;
+    }
+    wave_lds_sync();
+    double* ot = outv + c * kTileRows + lane;
+    if (tile_full(dv.n, c)) {
+#pragma unroll
+      for (uint32_t g = 0; g < 16; ++g) store_nt(ot + 64 * g, sv[lds_slot(64 * g + lane)]);
+    } else {
+#pragma unroll
+      for (uint32_t g = 0; g < 16; ++g)
+        if (c * kTileRows + 64 * g + lane < dv.n) store_nt(ot + 64 * g, sv[lds_slot(64 * g + lane)]);
+    }
+    wave_lds_sync();
+  }
+}
+
+template <int SEM>
+__device__ __forceinline__ void major_mat_item_s(const BatchBuffers& bb, const BQuery& q, const BItem& it,
+                                                 uint32_t lane, double* sv) {
+  switch (q.sparse & 0xFFu) {
+    case 1: major_mat_item<SEM, 1>(bb, q, it, lane, sv); break;
+    case 2: major_mat_item<SEM, 2>(bb, q, it, lane, sv); break;
+    case 3: major_mat_item<SEM, 3>(bb, q, it, lane, sv); break;
+    case 4: major_mat_item<SEM, 4>(bb, q, it, lane, sv); break;
+    case 5: major_mat_item<SEM, 5>(bb, q, it, lane, sv); break;
+    case 6: major_mat_item<SEM, 6>(bb, q, it, lane, sv); break;
+    default: major_mat_item<SEM, 7>(bb, q, it, lane, sv); break;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bq_major_mat(const BatchBuffers* __restrict__ bbp) {
+  __shared__ double s_val[4][64 * kLdsRow];
+  const BatchBuffers bb = cload(bbp);
+  const WaveSlot ws = wave_slot();
+  const uint32_t lane = ws.lane;
+  const uint32_t nscan = cmem(bb.hdr)->nscan, nitems = cmem(bb.hdr)->nitems;
+  double* sv = s_val[ws.local];
+  #pragma unroll 1
+  for (uint32_t i = nscan + ws.first; i < nitems; i += ws.stride) {
+    const BItem it = cload(bb.items, i);
+    const BQuery q = cload(bb.q, it.owner);
+    if (!q.sparse) continue;
+    if (q.sparse & kBqSparseSem) major_mat_item_s<1>(bb, q, it, lane, sv);
+    else major_mat_item_s<0>(bb, q, it, lane, sv);
+  }
+}
+
 hipError_t launch_batch_queries(const BatchBuffers* bb, const BatchTuning& tu, uint32_t parts, hipStream_t s) {
   hipLaunchKernelGGL(k_bq_fetch, dim3(64), dim3(256), 0, s, bb);
   hipLaunchKernelGGL(k_bq_items, dim3(1024), dim3(256), 0, s, bb);
   if (parts & kBqPartScan) {
     hipLaunchKernelGGL(k_bq_scan, dim3(tu.grid), dim3(256), 0, s, bb, tu);
+    if (parts & kBqPartMajor) hipLaunchKernelGGL(k_bq_major, dim3(tu.grid), dim3(256), 0, s, bb);
     hipLaunchKernelGGL(k_bq_reduce, dim3(1024), dim3(256), 0, s, bb);
   }
   if (parts & kBqPartMax) {
@@ -695,6 +872,8 @@ hipError_t launch_batch_queries(const BatchBuffers* bb, const BatchTuning& tu, u
   }
   // four resident workgroups per CU fit the LDS staging of k_bq_mat
   if (parts & kBqPartMat) hipLaunchKernelGGL(k_bq_mat, dim3(tu.grid < 1024 ? tu.grid : 1024), dim3(256), 0, s, bb);
+  if (parts & kBqPartMajorMat)
+    hipLaunchKernelGGL(k_bq_major_mat, dim3(tu.grid < 1024 ? tu.grid : 1024), dim3(256), 0, s, bb);
   hipLaunchKernelGGL(k_bq_publish, dim3(16), dim3(256), 0, s, bb);
   return hipGetLastError();
 }

@@ -312,9 +312,10 @@ int exec_reserve(QueryExec& ex, uint32_t max_q, uint64_t arena_bytes, const Batc
   BUFF_TRY(reserve(ctx, ex.arena, arena_bytes));
   BUFF_TRY(reserve(ctx, ex.bbdev, sizeof(BatchBuffers)));
   BUFF_TRY(pin_reserve(ex, max_q));
-  const uint32_t all_parts[4] = {kBqPartScan, kBqPartScan | kBqPartMax, kBqPartScan | kBqPartMat,
-                                 kBqPartScan | kBqPartMax | kBqPartMat};
-  for (uint32_t parts : all_parts) {
+  // every combination of kernels a batch can need (the scan always runs)
+  for (uint32_t extra = 0; extra < 16; ++extra) {
+    const uint32_t parts = kBqPartScan | ((extra & 1) ? kBqPartMax : 0u) | ((extra & 2) ? kBqPartMat : 0u) |
+                           ((extra & 4) ? kBqPartMajor : 0u) | ((extra & 8) ? kBqPartMajorMat : 0u);
     hipGraphExec_t gx = nullptr;
     BUFF_TRY(graph_for(ex, tu, parts, &gx));
   }
@@ -370,8 +371,8 @@ uint64_t arena_need(const Chunk* ch, const buff_query& in) {
 
 // Plans and launches nq queries (asynchronously on ex.stream); outs[k] is the
 // query's output address or 0 for the arena. Results come from exec_finish.
-int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* views_dev, const buff_query* qs,
-                uint32_t nq, const uint64_t* outs, const BatchTuning& tu) {
+int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* views_dev, const void* majors_dev,
+                const buff_query* qs, uint32_t nq, const uint64_t* outs, const BatchTuning& tu) {
   Ctx* ctx = ex.ctx;
   const uint64_t Tmax = std::min<uint32_t>(std::max<uint32_t>(tu.tiles_per_item, 1), kMaxTilesPerItem);
   ex.hq.assign(nq, BQuery{});
@@ -400,30 +401,68 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
       arena_at[k] = arena_off;
       arena_off += need;
     }
+    const bool sparse = ch->S > 0;   // buff-major with sparse columns: k_bq_major / k_bq_major_mat
+    const bool sem = (in.flags & BUFF_SEMANTIC) != 0;
+    const uint32_t sparse_word = sparse ? (ch->S | (sem ? kBqSparseSem : 0u)) : 0u;
     switch (in.op) {
       case BUFF_Q_GT:
       case BUFF_Q_EQ: {
         q.op = in.op == BUFF_Q_GT ? kBqGt : kBqEq;
-        int fill = 0;
-        const bool run = filter_plan(ch, in.op == BUFF_Q_GT ? BUFF_OP_GT : BUFF_OP_EQ, in.pred, in.flags, &q.tg, &fill);
-        q.fill = run ? 0 : (fill ? 2 : 1);
+        const int op = in.op == BUFF_Q_GT ? BUFF_OP_GT : BUFF_OP_EQ;
+        if (sparse) {
+          MajorTarget mt;
+          const int fill = major_filter_plan(ch, op, in.pred, in.flags, &mt);
+          q.fill = fill < 0 ? 0 : (fill ? 2 : 1);
+          q.sparse = fill < 0 ? sparse_word : 0u;   // a host-decided fill needs no sparse column
+          for (uint32_t d = 0; d < kMaxPlanes; ++d) q.tg.digit[d] = mt.digit[d];
+          q.tg.tail = mt.tail;
+        } else {
+          int fill = 0;
+          const bool run = filter_plan(ch, op, in.pred, in.flags, &q.tg, &fill);
+          q.fill = run ? 0 : (fill ? 2 : 1);
+        }
         break;
       }
       case BUFF_Q_SUM:
         q.op = kBqSum;
-        tiles = tiles && (v.P || v.r);
+        q.sparse = sparse_word;
+        tiles = tiles && (sparse || v.P || v.r);
         break;
       case BUFF_Q_MAX:
+        if (sparse) {
+          r.status = fail(BUFF_E_UNSUPPORTED, "query %u: batched max of a buff-major chunk with sparse columns "
+                          "(the single-chunk buff_max answers it)", k);
+          tiles = false;
+          break;
+        }
         q.op = kBqMax;
         tiles = tiles && (v.P || v.r);
         q.top = top_off;
         top_off += v.ntiles;
         break;
       case BUFF_Q_MAT: {
-        const int rc = mat_plan(ch, in.precision, in.flags, &q.mp);
+        int rc = BUFF_OK;
+        if ((ch->flags & BUFF_MAJORITY) && in.precision >= 0 && !sem)
+          rc = fail(BUFF_E_UNSUPPORTED, "query %u: the reference has no reduced-precision decode of a buff-major "
+                    "stream", k);
+        if (rc == BUFF_OK && sparse) {
+          if (!sem)
+            for (uint32_t j = 0; j < ch->S && rc == BUFF_OK; ++j)
+              if (ch->sp[j].n_out == 0)
+                rc = fail(BUFF_E_REF_PANIC, "query %u: sparse column %u has no outliers (buff_simd.rs:683)", k, j);
+          memset(&q.mp, 0, sizeof(q.mp));
+          q.mp.mode = sem ? kMatSemantic : kMatFull;
+          q.mp.inv_scale = std::ldexp(1.0, -(int)ch->dlen);
+          if (rc == BUFF_OK) rc = major_precision_drop(ch, in.precision, &q.mp.drop);
+          tiles = tiles && major_decoded_rows(ch, in.flags) > 0;
+          q.sparse = sparse_word;
+        } else if (rc == BUFF_OK) {
+          rc = mat_plan(ch, in.precision, in.flags, &q.mp);
+        }
         if (rc != BUFF_OK) {
           r.status = rc;
           q.op = kBqSum;
+          q.sparse = 0;
           tiles = false;
           break;
         }
@@ -475,6 +514,7 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
   BatchBuffers bb;
   bb.hdr = (const BHeader*)ex.hdr.p;
   bb.views = (const ChunkView*)views_dev;
+  bb.majors = (const MajorView*)majors_dev;
   bb.q = (const BQuery*)ex.q.p;
   bb.items = (BItem*)ex.items.p;
   bb.tile_top = (uint8_t*)ex.top.p;
@@ -495,7 +535,8 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
   uint32_t parts = 0;
   for (const BQuery& q : ex.hq) {
     if (!q.nitems) continue;
-    parts |= q.op == kBqMat ? kBqPartMat : kBqPartScan;
+    if (q.sparse) parts |= q.op == kBqMat ? kBqPartMajorMat : kBqPartMajor;
+    else parts |= q.op == kBqMat ? kBqPartMat : kBqPartScan;
     if (q.op == kBqMax) parts |= kBqPartMax;
   }
   parts |= kBqPartScan;   // the fold also writes the results of queries without tiles
@@ -523,6 +564,12 @@ int exec_finish(QueryExec& ex, const std::vector<Chunk*>& chunks, const buff_que
         uint64_t ps[kMaxPlanes + 1] = {0};
         for (uint32_t p = 0; p < ch->view.P; ++p) ps[p] = br.v[p];
         ps[ch->view.P] = br.v[kMaxPlanes];
+        if (ch->S > 0) {   // dense planes, then the outlier sums of the sparse columns (k_bq_major)
+          uint64_t outsum[kMaxSparse] = {0};
+          for (uint32_t j = 0; j < ch->S; ++j) outsum[j] = br.v[ch->view.P + j];
+          major_sum_fold(ch, outsum, ps, &r.value, &r.exact, nullptr);
+          break;
+        }
         r.value = reference_sum_fold(ch, ps);
         __int128 tot = (__int128)ch->n * (__int128)ch->view.base;
         for (uint32_t p = 0; p < ch->view.P; ++p) tot += (__int128)ps[p] << (ch->view.F - 8 * (p + 1));
@@ -565,7 +612,7 @@ int batch_query(Ctx* ctx, Batch* b, const buff_query* qs, uint32_t nq, buff_quer
   int rc;
   {
     ProfScope ps(ctx, "k_batch_graph", false);
-    rc = exec_launch(b->ex, b->chunks, b->views.p, qs, nq, nullptr, tu);
+    rc = exec_launch(b->ex, b->chunks, b->views.p, nullptr, qs, nq, nullptr, tu);
   }
   if (rc == BUFF_OK) rc = exec_finish(b->ex, b->chunks, qs);
   if (graph_ms) {
@@ -655,7 +702,7 @@ struct Server {
   Ctx* ctx = nullptr;
   buff_server_cfg cfg{};
   std::vector<Chunk*> chunks;
-  DevBuf views;
+  DevBuf views, majors;   // per chunk: its (dense) view; the sparse columns of buff-major chunks
   bool views_dirty = false;
   std::vector<QueryExec> slots;
   std::vector<std::vector<uint64_t>> slot_tickets;
@@ -821,15 +868,23 @@ void server_loop(Server* sv) {
       int rc = BUFF_OK;
       if (vdirty) {
         std::vector<ChunkView> v(chunks.size());
-        for (size_t k = 0; k < v.size(); ++k) v[k] = chunks[k]->view;
+        std::vector<MajorView> mj(chunks.size());
+        for (size_t k = 0; k < v.size(); ++k) {
+          v[k] = chunks[k]->view;
+          if (chunks[k]->S > 0) mj[k] = major_view(chunks[k]);
+          else memset(&mj[k], 0, sizeof(MajorView));
+        }
         for (uint32_t o = 0; o < nslots; ++o) (void)hipStreamSynchronize(sv->slots[o].stream);
         rc = ensure(sv->ctx, sv->views, sizeof(ChunkView) * std::max<size_t>(v.size(), 1));
+        if (rc == BUFF_OK) rc = ensure(sv->ctx, sv->majors, sizeof(MajorView) * std::max<size_t>(mj.size(), 1));
         if (rc == BUFF_OK && !v.empty() &&
-            hipMemcpy(sv->views.p, v.data(), sizeof(ChunkView) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
+            (hipMemcpy(sv->views.p, v.data(), sizeof(ChunkView) * v.size(), hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(sv->majors.p, mj.data(), sizeof(MajorView) * mj.size(), hipMemcpyHostToDevice) != hipSuccess))
           rc = fail(BUFF_E_DEVICE, "server views upload");
       }
       if (rc == BUFF_OK)
-        rc = exec_launch(sv->slots[s], chunks, sv->views.p, qv.data(), (uint32_t)qv.size(), outs.data(), tu);
+        rc = exec_launch(sv->slots[s], chunks, sv->views.p, sv->majors.p, qv.data(), (uint32_t)qv.size(),
+                         outs.data(), tu);
       lk.lock();
       if (rc != BUFF_OK) {
         sv->error = rc;
@@ -979,8 +1034,8 @@ int buff_server_add_chunk(buff_server* server, buff_chunk* chunk, uint32_t* id) 
   if (!sv || !chunk) return fail(BUFF_E_INVALID, "null argument");
   Chunk* ch = reinterpret_cast<Chunk*>(chunk);
   if (ch->ctx != sv->ctx) return fail(BUFF_E_INVALID, "the chunk belongs to another context");
-  if (ch->flags & (BUFF_MAJORITY | BUFF_SLICE))
-    return fail(BUFF_E_UNSUPPORTED, "served chunks use the byte_fixed layout");
+  if (ch->flags & BUFF_SLICE)
+    return fail(BUFF_E_UNSUPPORTED, "served chunks use the byte_fixed or buff-major layout");
   std::lock_guard<std::mutex> lk(sv->mu);
   if (id) *id = (uint32_t)sv->chunks.size();
   sv->chunks.push_back(ch);
@@ -1150,6 +1205,7 @@ int buff_server_destroy(buff_server* server) {
   if (sv->worker.joinable()) sv->worker.join();
   for (auto& ex : sv->slots) exec_free(ex);
   drop(sv->ctx, sv->views);
+  drop(sv->ctx, sv->majors);
   drop(sv->ctx, sv->arena);
   delete sv;
   return BUFF_OK;

@@ -612,6 +612,98 @@ __global__ __launch_bounds__(256) void k_max_refine(ChunkView cv, const uint8_t*
   if (have && lane == 0) atomicMax(best_out, (unsigned long long)best);
 }
 
+// One pass (the default form): every wave streams the top plane of its tiles
+// and keeps (top, best), the largest top byte it has seen and the largest
+// offset among its tiles whose top byte reached `top`. Such a tile is narrowed
+// as the reference narrows its candidates: the rows holding the tile's top
+// byte, then those holding the largest next byte among them, ..., then the tail
+// bits from the highest (refinement loads redirected to lanes that still hold
+// candidates, so they touch only lines with candidate rows), stopping as soon
+// as the prefix falls below best's. A wave's best is the exact max of its
+// tiles; each workgroup folds its waves' into *best_out with one atomic.
+template <int U>
+__global__ __launch_bounds__(256) void k_max_fused(ChunkView cv, unsigned long long* __restrict__ best_out) {
+  const uint32_t lane = __lane_id();
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
+  const uint64_t ngroups = (cv.ntiles + U - 1) / U;
+  const uint32_t P = cv.P, r = cv.r, F = cv.F;
+  uint32_t top = 0;
+  uint64_t best = 0;
+  bool have = false;
+  for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
+    const uint64_t c0 = grp * U;
+    uint4 w[U];
+    if (P > 0) {
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) w[u] = load_plane(cv, 0, min(c0 + u, cv.ntiles - 1), lane);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint64_t c = c0 + u;
+      if (c >= cv.ntiles) break;
+      uint32_t mb = 0;
+      if (P > 0)
+        mb = __builtin_amdgcn_readfirstlane(wave_max_u32(
+            max(max(max_byte4(w[u].x), max_byte4(w[u].y)), max(max_byte4(w[u].z), max_byte4(w[u].w)))));
+      if (have && mb < top) continue;   // wave-uniform
+      const bool first_top = !have || mb > top;
+      uint32_t cand = P > 0 ? mask_eq(w[u], make_digit(mb)) : 0xFFFFu;
+      uint64_t pre = mb;
+      uint32_t bits = P > 0 ? 8u : 0u;
+      bool beat = first_top;   // the prefix is already above best's
+      bool lost = false;       // ... or below it: the tile cannot hold the max
+      for (uint32_t p = 1; p < P && !lost; ++p) {
+        const uint4 wp = load_plane(cv, p, c, refine_src(cand, lane));
+        const uint32_t pb = __builtin_amdgcn_readfirstlane(wave_max_u32(masked_max_plus1(wp, cand))) - 1u;
+        pre = (pre << 8) | pb;
+        bits += 8;
+        if (!beat) {
+          const uint64_t bp = best >> (F - bits);
+          lost = pre < bp;
+          beat = pre > bp;
+        }
+        cand &= mask_eq(wp, make_digit(pb));
+      }
+      if (!lost && r > 0) {
+        const uint32_t src = refine_src(cand, lane);
+        uint32_t h[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) h[j] = j < r ? load_tail(cv, j, c, src) : 0u;
+#pragma unroll
+        for (int j = 7; j >= 0; --j) {
+          if ((uint32_t)j >= r || lost) continue;
+          const uint32_t one = __ballot((cand & h[j]) != 0) != 0 ? 1u : 0u;
+          pre = (pre << 1) | one;
+          bits += 1;
+          if (!beat) {
+            const uint64_t bp = best >> (F - bits);
+            lost = pre < bp;
+            beat = pre > bp;
+          }
+          cand &= one ? h[j] : ~h[j];
+        }
+      }
+      if (lost) continue;
+      if (first_top) {
+        top = mb;
+        best = pre;
+        have = true;
+      } else if (pre > best) {
+        best = pre;
+      }
+    }
+  }
+  __shared__ unsigned long long wb[16];
+  if (lane == 0) wb[wave_in_wg()] = best;   // 0 for a wave without tiles: no larger than the max
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long b = 0;
+    for (uint32_t k = 0; k < (blockDim.x >> 6); ++k) b = max(b, wb[k]);
+    atomicMax(best_out, b);
+  }
+}
+
 // ----------------------------------------------------------------------------
 // K7: materialization. Full (split_double.rs:1243-1350) and semantic reduced
 // precision (top R bits kept, re-scaled). A lane builds the values of its 16
@@ -1035,8 +1127,25 @@ static uint32_t launch_max_top_u(const ChunkView& cv, uint8_t* scratch, unsigned
   return wg;
 }
 
+template <int U>
+static void launch_max_fused_u(const ChunkView& cv, unsigned long long* best, hipStream_t s, const LaunchShape& ls) {
+  const uint32_t block = 64 * ls.waves_per_wg;
+  const uint32_t wg =
+      grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, cap_of(ls, resident_wgs(k_max_fused<U>, block)));
+  launch_k((k_max_fused<U>), dim3(wg), dim3(block), 0, s, cv, best);
+}
+
 hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* scratch, hipStream_t s,
                       const LaunchShape& ls) {
+  if (ls.prefetch == 0) {   // one pass
+    switch (ls.tiles_per_wave) {
+      case 4: launch_max_fused_u<4>(cv, best, s, ls); break;
+      case 16: launch_max_fused_u<16>(cv, best, s, ls); break;
+      default: launch_max_fused_u<8>(cv, best, s, ls); break;
+    }
+    return hipGetLastError();
+  }
+  // two passes: every tile's top byte, then the tiles holding the largest
   unsigned int* top = reinterpret_cast<unsigned int*>(scratch + ((cv.ntiles + 255) / 256) * 256);
   uint32_t wg1 = 0;
   if (cv.P > 0) {

@@ -84,8 +84,9 @@ hipError_t launch_major_scan(const MajorEnc& me, uint64_t ntiles, unsigned long 
 hipError_t launch_major_place(const MajorEnc& me, uint64_t ntiles, hipStream_t s);
 hipError_t launch_major_filter(const MajorView& mv, int op, bool semantic, const MajorTarget& tg, uint64_t* out,
                                uint64_t nwords, hipStream_t s, const LaunchShape& ls);
-hipError_t launch_major_decode(const MajorView& mv, bool semantic, double inv_scale, double* out, hipStream_t s,
-                               const LaunchShape& ls);
+// semantic: the true offsets, truncated to their top F - drop bits (drop = 0: every bit).
+hipError_t launch_major_decode(const MajorView& mv, bool semantic, uint32_t drop, double inv_scale, double* out,
+                               hipStream_t s, const LaunchShape& ls);
 // sums[j] = sum of the outlier bytes of sparse column j; scratch: major_sum_scratch_bytes().
 uint64_t major_sum_scratch_bytes(uint32_t S);
 hipError_t launch_major_outlier_sums(const MajorView& mv, const uint64_t* n_out, unsigned long long* sums,

@@ -15,126 +15,9 @@
 #include "buff_device.h"
 #include "buff_launch.h"
 #include "buff_major.h"
+#include "buff_major_device.h"
 
 namespace buffgpu {
-
-// The lane's packed outlier bytes (k-th outlier of the lane = byte k) as four
-// dwords, from a 20-byte window of aligned loads at its compacted position.
-struct Packed16 {
-  uint32_t d0, d1, d2, d3;
-};
-__device__ __forceinline__ Packed16 load_packed(const uint8_t* outl, uint64_t pos) {
-  const uint32_t* w = reinterpret_cast<const uint32_t*>(outl + (pos & ~3ull));
-  const uint32_t sh = (uint32_t)(pos & 3);
-  const uint32_t a = w[0], b = w[1], c = w[2], d = w[3], e = w[4];
-  Packed16 p;
-  p.d0 = __builtin_amdgcn_alignbyte(b, a, sh);
-  p.d1 = __builtin_amdgcn_alignbyte(c, b, sh);
-  p.d2 = __builtin_amdgcn_alignbyte(d, c, sh);
-  p.d3 = __builtin_amdgcn_alignbyte(e, d, sh);
-  return p;
-}
-// Row masks (bit b <-> row 16*lane + b) of one sparse column against target byte
-// t: majority rows all at once, outlier rows from the packed bytes. The lane's
-// k-th outlier (k-th set bit of m16, ascending rows) is byte k of its packed
-// window; k runs with a wave-uniform bound in an unrolled loop, so every step
-// reads a fixed register and byte (no per-lane select of the byte). Only the
-// rows in `want` keep their results.
-__device__ __forceinline__ void sparse_cmp(const Packed16& pk, uint32_t m16, uint32_t want, uint32_t major, uint32_t t,
-                                           uint32_t& gt, uint32_t& eq) {
-  const uint32_t nm = ~m16 & 0xFFFFu;
-  const uint32_t d[4] = {pk.d0, pk.d1, pk.d2, pk.d3};
-  uint32_t go = 0, eo = 0, rem = m16;
-#pragma unroll
-  for (uint32_t k = 0; k < 16; ++k) {
-    if (!__ballot((rem & want) != 0)) break;
-    const uint32_t pos = (uint32_t)__builtin_ctz(rem | 0x10000u);   // 16 once the lane has no outlier left
-    rem &= rem - 1u;
-    const uint32_t v = (d[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-    go |= (uint32_t)(v > t) << pos;
-    eo |= (uint32_t)(v == t) << pos;
-  }
-  gt = (major > t ? nm : 0u) | (go & want);
-  eq = (major == t ? nm : 0u) | (eo & want);
-}
-
-// Replaces byte b (row 16*lane + b: dword b%4, byte b/4) of a lane's 16 bytes
-// when `on` (written with selects: every lane runs the same instructions).
-__device__ __forceinline__ void set_byte(uint32_t& d0, uint32_t& d1, uint32_t& d2, uint32_t& d3, uint32_t b,
-                                         uint32_t v, bool on) {
-  const uint32_t sh = 8 * ((b >> 2) & 3);
-  const uint32_t keep = ~(0xFFu << sh), val = v << sh;
-  const uint32_t i = b & 3;
-  d0 = on && i == 0 ? (d0 & keep) | val : d0;
-  d1 = on && i == 1 ? (d1 & keep) | val : d1;
-  d2 = on && i == 2 ? (d2 & keep) | val : d2;
-  d3 = on && i == 3 ? (d3 & keep) | val : d3;
-}
-
-// The lane's 16 bytes of a sparse column: the majority byte, with the outlier
-// bytes of the rows in `want` (a subset of m16) patched in. The outliers are
-// walked by rank as in sparse_cmp.
-__device__ __forceinline__ uint4 patch_outliers(const Packed16& pk, uint32_t m16, uint32_t want, uint32_t major) {
-  const uint32_t M = 0x01010101u * major;
-  uint32_t o0 = M, o1 = M, o2 = M, o3 = M;
-  const uint32_t d[4] = {pk.d0, pk.d1, pk.d2, pk.d3};
-  uint32_t rem = m16;
-#pragma unroll
-  for (uint32_t k = 0; k < 16; ++k) {
-    if (!__ballot((rem & want) != 0)) break;
-    const uint32_t pos = (uint32_t)__builtin_ctz(rem | 0x10000u);
-    rem &= rem - 1u;
-    const uint32_t v = (d[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-    set_byte(o0, o1, o2, o3, pos, v, (want >> pos) & 1u);
-  }
-  return make_uint4(o0, o1, o2, o3);
-}
-
-// max over the outlier rows in `want` of (byte + 1); 0 when there is none.
-__device__ __forceinline__ uint32_t sparse_max_plus1(const Packed16& pk, uint32_t m16, uint32_t want) {
-  const uint32_t d[4] = {pk.d0, pk.d1, pk.d2, pk.d3};
-  uint32_t rem = m16, best = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < 16; ++k) {
-    if (!__ballot((rem & want) != 0)) break;
-    const uint32_t pos = (uint32_t)__builtin_ctz(rem | 0x10000u);
-    rem &= rem - 1u;
-    const uint32_t v = (d[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-    best = (want >> pos) & 1u ? max(best, v + 1u) : best;
-  }
-  return best;
-}
-
-// The outlier window of column sd for the lane in tile c (every lane of the
-// wave calls it together: the exclusive scan).
-__device__ __forceinline__ Packed16 lane_outliers(const SparseDev& sd, uint64_t c, uint32_t lane, uint32_t m16) {
-  const uint32_t excl = wave_excl_scan_u32(__popc(m16), lane);
-  return m16 ? load_packed(sd.outl, (uint64_t)sd.prefix[c] + excl) : Packed16{0, 0, 0, 0};
-}
-
-// Masks and outlier windows of every sparse column of tile c, with all their
-// loads issued before any is consumed (the columns are independent).
-struct SparseLanes {
-  uint32_t m[kMaxSparse];
-  Packed16 pk[kMaxSparse];
-};
-
-__device__ __forceinline__ void load_sparse(const MajorView& mv, uint64_t c, uint32_t lane, SparseLanes& sl) {
-  uint32_t pre[kMaxSparse];
-#pragma unroll
-  for (uint32_t j = 0; j < kMaxSparse; ++j) {
-    sl.m[j] = j < mv.S ? (uint32_t)mv.col[j].mask16[c * 64 + lane] : 0u;
-    pre[j] = j < mv.S ? mv.col[j].prefix[c] : 0u;
-  }
-#pragma unroll
-  for (uint32_t j = 0; j < kMaxSparse; ++j) {
-    sl.pk[j] = Packed16{0, 0, 0, 0};
-    if (j < mv.S) {
-      const uint32_t excl = wave_excl_scan_u32(__popc(sl.m[j]), lane);
-      if (sl.m[j]) sl.pk[j] = load_packed(mv.col[j].outl, (uint64_t)pre[j] + excl);
-    }
-  }
-}
 
 // ----------------------------------------------------------------------------
 // Encode (buff_simd.rs:451-555). Pass 1 reads the float64 column once and, per
@@ -362,105 +245,28 @@ __global__ __launch_bounds__(256) void k_major_filter(MajorView mv, MajorTarget 
   uint16_t* out16 = reinterpret_cast<uint16_t*>(out);
   const uint64_t nout16 = 4 * nwords;
   const uint64_t ngroups = (dv.ntiles + U - 1) / U;
-  const bool all_outliers = MODE == 0 && !sem;   // the range rules read every outlier byte
   for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
     const uint64_t c0 = grp * U;
-    uint32_t m[U][S], pre[U][S];
-    uint4 w0[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint64_t c = min(c0 + u, dv.ntiles - 1);
-#pragma unroll
-      for (int j = 0; j < S; ++j) {
-        m[u][j] = mv.col[j].mask16[c * 64 + lane];
-        pre[u][j] = mv.col[j].prefix[c];
-      }
-      w0[u] = dv.P ? load_plane(dv, 0, c, lane) : make_uint4(0, 0, 0, 0);
-    }
-    Packed16 pk[U][S];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-#pragma unroll
-      for (int j = 0; j < S; ++j) {
-        const uint32_t excl = wave_excl_scan_u32(__popc(m[u][j]), lane);
-        pk[u][j] = m[u][j] ? load_packed(mv.col[j].outl, (uint64_t)pre[u][j] + excl) : Packed16{0, 0, 0, 0};
-      }
-    }
-    uint32_t res[U], cand[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      res[u] = 0;
-      cand[u] = lane_valid16(dv.n, c0 + u, lane);
-#pragma unroll
-      for (int j = 0; j < S; ++j) {
-        const uint32_t m16 = m[u][j];
-        const uint32_t major = mv.col[j].major;
-        const uint32_t t = tg.digit[j];
-        uint32_t gt, eq;
-        sparse_cmp(pk[u][j], m16, all_outliers ? m16 : (m16 & cand[u]), major, t, gt, eq);
-        if (MODE == 1) {
-          cand[u] &= eq;
-          continue;
-        }
-        if (sem || j == 0) {
-          res[u] |= cand[u] & gt;
-          cand[u] &= eq;
-        } else if (t == 0 && major == 0) {
-          res[u] |= cand[u] & m16;
-          cand[u] = 0;
-        } else {
-          res[u] |= m16 & gt;
-          if (major > t) res[u] |= cand[u] & ~m16;
-          cand[u] &= eq;
-        }
-      }
-    }
-    for (uint32_t p = 0; p < dv.P; ++p) {
-      bool live = false;
-#pragma unroll
-      for (int u = 0; u < U; ++u) live |= cand[u] != 0;
-      if (!__ballot(live)) break;
-      uint4 wp[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        wp[u] = p == 0 ? w0[u] : load_plane(dv, p, min(c0 + u, dv.ntiles - 1), refine_src(cand[u], lane));
-      const Digit dg = make_digit(tg.digit[S + p]);
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (MODE == 0) res[u] |= cand[u] & mask_gt(wp[u], dg);
-        cand[u] &= mask_eq(wp[u], dg);
-      }
-    }
-    if (dv.r) {
-      bool live = false;
-#pragma unroll
-      for (int u = 0; u < U; ++u) live |= cand[u] != 0;
-      if (__ballot(live)) {
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (c0 + u < dv.ntiles) refine_tail<MODE>(dv, c0 + u, tg.tail, lane, res[u], cand[u]);
-      }
-    }
+    uint32_t res[U];
+    major_filter_group<MODE, S, U>(mv, tg, sem != 0, c0, lane, res);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t c = c0 + u;
-      const uint32_t r = MODE == 0 ? (res[u] & lane_valid16(dv.n, c, lane)) : cand[u];
       const uint64_t idx = c * 64 + lane;
-      if (c < dv.ntiles && idx < nout16) out16[idx] = (uint16_t)r;
+      if (c < dv.ntiles && idx < nout16) out16[idx] = (uint16_t)res[u];
     }
   }
 }
 
 // ----------------------------------------------------------------------------
-// Decode: buff_decode_majority (buff_simd.rs:633-861). Column 0 places its byte;
-// a later sparse column ORs outlier bytes at its shift and the majority byte at
-// twice its shift, and only for rows up to its last outlier (SEM: every row gets
-// its byte once). Values are staged in LDS and written coalesced.
+// Decode: buff_decode_majority (buff_simd.rs:633-861), the offsets built by
+// major_tile_offsets (SEM: the true offsets, truncated to their top F - drop
+// bits for a reduced-precision read). Values are staged in LDS and written
+// coalesced.
 
 template <int SEM, int S>
-__global__ __launch_bounds__(256) void k_major_decode(MajorView mv, double inv_scale, double* __restrict__ out) {
-  constexpr int SS = S > 0 ? S : 1;
-  constexpr int NPM = (int)kMaxPlanes - S > 0 ? (int)kMaxPlanes - S : 1;   // dense planes at most (F <= 64)
+__global__ __launch_bounds__(256) void k_major_decode(MajorView mv, double inv_scale, uint32_t drop,
+                                                      double* __restrict__ out) {
   __shared__ double s_val[4][64 * kLdsRow];
   const uint32_t lane = __lane_id();
   double* sv = s_val[wave_in_wg()];
@@ -468,49 +274,12 @@ __global__ __launch_bounds__(256) void k_major_decode(MajorView mv, double inv_s
   const uint64_t nwaves = waves_in_grid();
   const ChunkView& dv = mv.dense;
   for (uint64_t c = wave; c < dv.ntiles; c += nwaves) {
-    // every load of the tile first: masks and prefixes, dense planes and tail,
-    // then the outlier windows (they need the masks' scan)
-    uint32_t m[SS], pre[SS];
-#pragma unroll
-    for (int j = 0; j < S; ++j) {
-      m[j] = mv.col[j].mask16[c * 64 + lane];
-      pre[j] = mv.col[j].prefix[c];
-    }
-    uint4 wp[NPM];
-#pragma unroll
-    for (int p = 0; p < NPM; ++p) wp[p] = (uint32_t)p < dv.P ? load_plane(dv, p, c, lane) : make_uint4(0, 0, 0, 0);
-    uint32_t h[7];
-#pragma unroll
-    for (uint32_t j = 0; j < 7; ++j) h[j] = j < dv.r ? load_tail(dv, j, c, lane) : 0u;
-    uint4 ws[SS];
-#pragma unroll
-    for (int j = 0; j < S; ++j) {
-      const uint32_t excl = wave_excl_scan_u32(__popc(m[j]), lane);
-      const Packed16 pk = m[j] ? load_packed(mv.col[j].outl, (uint64_t)pre[j] + excl) : Packed16{0, 0, 0, 0};
-      ws[j] = patch_outliers(pk, m[j], m[j], mv.col[j].major);
-    }
-    const uint64_t row0 = c * kTileRows + 16ull * lane;
+    uint64_t v[16];
+    major_tile_offsets<SEM, S>(mv, c, lane, v);
 #pragma unroll
     for (uint32_t b = 0; b < 16; ++b) {
-      uint64_t v = 0;
-#pragma unroll
-      for (int j = 0; j < S; ++j) {
-        const uint32_t sh = mv.col[j].shift;
-        const uint64_t byte = (uint64_t)byte_at(ws[j], b) << sh;
-        if (SEM || j == 0) {
-          v |= byte;
-        } else {
-          const uint64_t twice = ((uint64_t)mv.col[j].major << sh) << sh;
-          const uint64_t t = ((m[j] >> b) & 1u) ? byte : twice;
-          v |= row0 + b <= mv.col[j].last ? t : 0ull;
-        }
-      }
-#pragma unroll
-      for (int p = 0; p < NPM; ++p)
-        if ((uint32_t)p < dv.P) v |= (uint64_t)byte_at(wp[p], b) << (dv.F - 8 * (p + 1));
-#pragma unroll
-      for (uint32_t j = 0; j < 7; ++j) v |= (uint64_t)((h[j] >> b) & 1u) << j;
-      sv[kLdsRow * lane + b] = (double)(int64_t)((uint64_t)dv.base + v) * inv_scale;
+      const uint64_t o = SEM ? (v[b] >> drop) << drop : v[b];   // SEM: the top F - drop bits
+      sv[kLdsRow * lane + b] = (double)(int64_t)((uint64_t)dv.base + o) * inv_scale;
     }
     wave_lds_sync();
     double* ot = out + c * kTileRows + lane;
@@ -776,27 +545,29 @@ hipError_t launch_major_filter(const MajorView& mv, int op, bool semantic, const
 }
 
 template <int SEM>
-static hipError_t launch_decode_sem(const MajorView& mv, double inv_scale, double* out, uint32_t grid,
+static hipError_t launch_decode_sem(const MajorView& mv, double inv_scale, uint32_t drop, double* out, uint32_t grid,
                                     hipStream_t s) {
   switch (mv.S) {
-    case 0: launch_k((k_major_decode<SEM, 0>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 1: launch_k((k_major_decode<SEM, 1>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 2: launch_k((k_major_decode<SEM, 2>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 3: launch_k((k_major_decode<SEM, 3>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 4: launch_k((k_major_decode<SEM, 4>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 5: launch_k((k_major_decode<SEM, 5>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 6: launch_k((k_major_decode<SEM, 6>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 7: launch_k((k_major_decode<SEM, 7>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
-    case 8: launch_k((k_major_decode<SEM, 8>), dim3(grid), dim3(256), 0, s, mv, inv_scale, out); break;
+    case 0: launch_k((k_major_decode<SEM, 0>), dim3(grid), dim3(256), 0, s, mv, inv_scale, drop, out); break;
+    case 1: launch_k((k_major_decode<SEM, 1>), dim3(grid), dim3(256), 0, s, mv, inv_scale, drop, out); break;
+    case 2: launch_k((k_major_decode<SEM, 2>), dim3(grid), dim3(256), 0, s, mv, inv_scale, drop, out); break;
+    case 3: launch_k((k_major_decode<SEM, 3>), dim3(grid), dim3(256), 0, s, mv, inv_scale, drop, out); break;
+    case 4: launch_k((k_major_decode<SEM, 4>), dim3(grid), dim3(256), 0, s, mv, inv_scale, drop, out); break;
+    case 5: launch_k((k_major_decode<SEM, 5>), dim3(grid), dim3(256), 0, s, mv, inv_scale, drop, out); break;
+    case 6: launch_k((k_major_decode<SEM, 6>), dim3(grid), dim3(256), 0, s, mv, inv_scale, drop, out); break;
+    case 7: launch_k((k_major_decode<SEM, 7>), dim3(grid), dim3(256), 0, s, mv, inv_scale, drop, out); break;
+    case 8: launch_k((k_major_decode<SEM, 8>), dim3(grid), dim3(256), 0, s, mv, inv_scale, drop, out); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_major_decode(const MajorView& mv, bool semantic, double inv_scale, double* out, hipStream_t s,
-                               const LaunchShape& ls) {
+hipError_t launch_major_decode(const MajorView& mv, bool semantic, uint32_t drop, double inv_scale, double* out,
+                               hipStream_t s, const LaunchShape& ls) {
+  if (drop && !semantic) return hipErrorInvalidValue;
   const uint32_t grid = tiles_grid(mv.dense.ntiles, ls.max_wg ? ls.max_wg : 2048);
-  return semantic ? launch_decode_sem<1>(mv, inv_scale, out, grid, s) : launch_decode_sem<0>(mv, inv_scale, out, grid, s);
+  return semantic ? launch_decode_sem<1>(mv, inv_scale, drop, out, grid, s)
+                  : launch_decode_sem<0>(mv, inv_scale, 0, out, grid, s);
 }
 
 static constexpr uint32_t kOutsumBlocks = 64;

@@ -45,24 +45,6 @@ void vote(const std::vector<uint64_t>& head, uint32_t rsf, uint8_t* major, int32
   *freq = f;
 }
 
-MajorView major_view(const Chunk* ch) {
-  MajorView mv;
-  memset(&mv, 0, sizeof(mv));
-  mv.dense = ch->view;
-  mv.S = ch->S;
-  mv.F = ch->F;
-  for (uint32_t j = 0; j < ch->S; ++j) {
-    const SparseCol& c = ch->sp[j];
-    mv.col[j].mask16 = c.mask16;
-    mv.col[j].prefix = c.prefix;
-    mv.col[j].outl = c.outl;
-    mv.col[j].last = c.last;
-    mv.col[j].major = c.major;
-    mv.col[j].shift = c.shift;
-  }
-  return mv;
-}
-
 uint64_t mask_bytes(uint64_t ntiles) { return ntiles * 64 * 2; }
 uint64_t prefix_bytes(uint64_t ntiles) { return round_up((ntiles + 1) * 4, 256); }
 
@@ -161,6 +143,25 @@ void free_partial(Ctx* ctx, Chunk* ch) {
 }
 
 }  // namespace
+
+MajorView major_view(const Chunk* ch) {
+  MajorView mv;
+  memset(&mv, 0, sizeof(mv));
+  mv.dense = ch->view;
+  mv.S = ch->S;
+  mv.F = ch->F;
+  for (uint32_t j = 0; j < ch->S; ++j) {
+    const SparseCol& c = ch->sp[j];
+    mv.col[j].mask16 = c.mask16;
+    mv.col[j].prefix = c.prefix;
+    mv.col[j].outl = c.outl;
+    mv.col[j].last = c.last;
+    mv.col[j].major = c.major;
+    mv.col[j].shift = c.shift;
+  }
+  return mv;
+}
+
 
 void major_release(Chunk* ch) {
   if (ch->mem_sparse) (void)ch->ctx->release(ch->mem_sparse);
@@ -531,32 +532,35 @@ int major_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_
 // ---------------------------------------------------------------------------
 // queries (S > 0; a buff-major chunk without sparse columns uses the plain path)
 
-int major_filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, uint64_t* words) {
-  const uint64_t nwords = (ch->n + 63) / 64;
-  if (nwords == 0) return BUFF_OK;
+int major_filter_plan(const Chunk* ch, int op, double pred, uint32_t flags, MajorTarget* tg) {
   const int prec = prec_of_scale(ch->scale);
   const int32_t pexp = precision_exp_of_prec(prec);
   uint64_t pu;
   memcpy(&pu, &pred, 8);
   const int64_t fixed = fetch_fixed_aligned_bits(pu, pexp, ch->dlen);
   const bool semantic = (flags & BUFF_SEMANTIC) != 0;
-  int fill = -1;
-  if (fixed < ch->view.base) {   // buff_simd.rs:886-889 / 1939-1942
-    fill = op == BUFF_OP_GT ? 1 : 0;
-  }
+  if (fixed < ch->view.base) return op == BUFF_OP_GT ? 1 : 0;   // buff_simd.rs:886-889 / 1939-1942
   const uint64_t t = (uint64_t)fixed - (uint64_t)ch->view.base;
-  if (fill < 0 && semantic && ch->F < 64 && t >= (1ull << ch->F)) fill = 0;
+  if (semantic && ch->F < 64 && t >= (1ull << ch->F)) return 0;
   // equality answers come from the tail stage only (buff_simd.rs:2110-2150)
-  if (fill < 0 && op == BUFF_OP_EQ && !semantic && ch->view.r == 0) fill = 0;
+  if (op == BUFF_OP_EQ && !semantic && ch->view.r == 0) return 0;
+  memset(tg, 0, sizeof(*tg));
+  for (uint32_t k = 0; k < ch->F / 8; ++k) tg->digit[k] = (uint32_t)((t >> (ch->F - 8 * (k + 1))) & 0xFF);
+  tg->tail = ch->view.r ? (t & ((1ull << ch->view.r) - 1)) : 0;
+  return -1;
+}
+
+int major_filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, uint64_t* words) {
+  const uint64_t nwords = (ch->n + 63) / 64;
+  if (nwords == 0) return BUFF_OK;
+  const bool semantic = (flags & BUFF_SEMANTIC) != 0;
+  MajorTarget tg;
+  const int fill = major_filter_plan(ch, op, pred, flags, &tg);
   if (fill >= 0) {
     ProfScope ps(ctx, "k_fill_bitmap");
     HIP_TRY(launch_fill_bitmap(words, nwords, ch->n, fill, ctx->stream));
     return BUFF_OK;
   }
-  MajorTarget tg;
-  memset(&tg, 0, sizeof(tg));
-  for (uint32_t k = 0; k < ch->F / 8; ++k) tg.digit[k] = (uint32_t)((t >> (ch->F - 8 * (k + 1))) & 0xFF);
-  tg.tail = ch->view.r ? (t & ((1ull << ch->view.r) - 1)) : 0;
   const MajorView mv = major_view(ch);
   ProfScope ps(ctx, op == BUFF_OP_GT ? "k_major_filter_gt" : "k_major_filter_eq");
   HIP_TRY(launch_major_filter(mv, op == BUFF_OP_GT ? 0 : 1, semantic, tg, words, nwords, ctx->stream,
@@ -564,29 +568,14 @@ int major_filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t
   return BUFF_OK;
 }
 
-int major_sum(Ctx* ctx, const Chunk* ch, uint32_t flags, double* ref_sum, double* exact_sum, uint64_t* col_sums) {
-  (void)flags;
+// buff_sum_majority's fold (buff_simd.rs:2311-2439) of the outlier byte sums of
+// the sparse columns (outsum[j]) and the dense planes' and tail's sums
+// (dense_ps[p], tail at index P): each sparse column sums to its outliers plus
+// major * (n - outliers).
+void major_sum_fold(const Chunk* ch, const uint64_t* outsum, const uint64_t* dense_ps, double* ref_sum,
+                    double* exact_sum, uint64_t* col_sums) {
   const ChunkView& v = ch->view;
   const uint32_t S = ch->S;
-  uint64_t dense_ps[kMaxPlanes + 1] = {0};
-  uint64_t outsum[kMaxSparse] = {0};
-  const LaunchShape ls = ctx->shape(Op::Sum, v);
-  // [outlier sums | pad | per-block outlier rows (4 KiB) | dense sums + k_sum rows]
-  const uint64_t rows_off = 256, dense_off = rows_off + 4096;
-  BUFF_TRY(ctx->scratch(ctx->s_sums, dense_off + sizeof(dense_ps) + sum_scratch_bytes(ls)));
-  uint8_t* sb = (uint8_t*)ctx->s_sums.ptr;
-  const MajorView mv = major_view(ch);
-  uint64_t n_out[kMaxSparse];
-  for (uint32_t j = 0; j < S; ++j) n_out[j] = ch->sp[j].n_out;
-  const bool dense = ch->n > 0 && (v.P || v.r);
-  {
-    ProfScope sc(ctx, "k_major_sum");
-    HIP_TRY(launch_major_outlier_sums(mv, n_out, (unsigned long long*)sb, (unsigned long long*)(sb + rows_off),
-                                      ctx->stream));
-    if (dense) HIP_TRY(launch_sum(v, (unsigned long long*)(sb + dense_off), ctx->stream, ls));
-  }
-  BUFF_TRY(d2h_small(ctx, outsum, sb, 8ull * S));
-  if (dense) BUFF_TRY(d2h_small(ctx, dense_ps, sb + dense_off, sizeof(dense_ps)));
   // byte columns in stream order, then the tail
   uint64_t cs[kMaxPlanes + 2] = {0};
   uint32_t k = 0;
@@ -628,6 +617,32 @@ int major_sum(Ctx* ctx, const Chunk* ch, uint32_t flags, double* ref_sum, double
     tot += (__int128)tail_sum;
     *exact_sum = (double)tot / powi2((int32_t)ch->dlen);
   }
+}
+
+int major_sum(Ctx* ctx, const Chunk* ch, uint32_t flags, double* ref_sum, double* exact_sum, uint64_t* col_sums) {
+  (void)flags;
+  const ChunkView& v = ch->view;
+  const uint32_t S = ch->S;
+  uint64_t dense_ps[kMaxPlanes + 1] = {0};
+  uint64_t outsum[kMaxSparse] = {0};
+  const LaunchShape ls = ctx->shape(Op::Sum, v);
+  // [outlier sums | pad | per-block outlier rows (4 KiB) | dense sums + k_sum rows]
+  const uint64_t rows_off = 256, dense_off = rows_off + 4096;
+  BUFF_TRY(ctx->scratch(ctx->s_sums, dense_off + sizeof(dense_ps) + sum_scratch_bytes(ls)));
+  uint8_t* sb = (uint8_t*)ctx->s_sums.ptr;
+  const MajorView mv = major_view(ch);
+  uint64_t n_out[kMaxSparse];
+  for (uint32_t j = 0; j < S; ++j) n_out[j] = ch->sp[j].n_out;
+  const bool dense = ch->n > 0 && (v.P || v.r);
+  {
+    ProfScope sc(ctx, "k_major_sum");
+    HIP_TRY(launch_major_outlier_sums(mv, n_out, (unsigned long long*)sb, (unsigned long long*)(sb + rows_off),
+                                      ctx->stream));
+    if (dense) HIP_TRY(launch_sum(v, (unsigned long long*)(sb + dense_off), ctx->stream, ls));
+  }
+  BUFF_TRY(d2h_small(ctx, outsum, sb, 8ull * S));
+  if (dense) BUFF_TRY(d2h_small(ctx, dense_ps, sb + dense_off, sizeof(dense_ps)));
+  major_sum_fold(ch, outsum, dense_ps, ref_sum, exact_sum, col_sums);
   return BUFF_OK;
 }
 
@@ -655,11 +670,27 @@ uint64_t major_decoded_rows(const Chunk* ch, uint32_t flags) {
   return ch->F == 8 * ch->S ? 0 : ch->n;   // buff_simd.rs:788-845 push values only below the sparse columns
 }
 
+// The bits a BUFF_SEMANTIC reduced-precision read drops from a buff-major
+// chunk's offsets: F - R, with R from the read rule of split_double.rs:1642-1664
+// (every bit when more decimals are asked than stored).
+int major_precision_drop(const Chunk* ch, int32_t precision, uint32_t* drop) {
+  *drop = 0;
+  if (precision < 0) return BUFF_OK;
+  uint32_t R = 0;
+  const int rc = precision_bits(ch, precision, &R);
+  if (rc == BUFF_E_REF_PANIC) R = ch->F;
+  else if (rc != BUFF_OK) return rc;
+  *drop = ch->F - std::min(R, ch->F);
+  return BUFF_OK;
+}
+
 int major_materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* out,
                              uint64_t* rows) {
-  if (precision >= 0)
-    return fail(BUFF_E_UNSUPPORTED, "the reference has no reduced-precision decode of a buff-major stream");
   const bool semantic = (flags & BUFF_SEMANTIC) != 0;
+  if (precision >= 0 && !semantic)
+    return fail(BUFF_E_UNSUPPORTED, "the reference has no reduced-precision decode of a buff-major stream");
+  uint32_t drop = 0;
+  BUFF_TRY(major_precision_drop(ch, precision, &drop));
   if (!semantic) {
     for (uint32_t j = 0; j < ch->S; ++j)
       if (ch->sp[j].n_out == 0)
@@ -671,7 +702,7 @@ int major_materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint3
   if (r == 0) return BUFF_OK;
   if (!out) return fail(BUFF_E_INVALID, "null output");
   ProfScope sc(ctx, "k_major_decode");
-  HIP_TRY(launch_major_decode(major_view(ch), semantic, std::ldexp(1.0, -(int)ch->dlen), out, ctx->stream,
+  HIP_TRY(launch_major_decode(major_view(ch), semantic, drop, std::ldexp(1.0, -(int)ch->dlen), out, ctx->stream,
                               ctx->shape(Op::Materialize, ch->view)));
   return BUFF_OK;
 }

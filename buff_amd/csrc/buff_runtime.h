@@ -11,6 +11,7 @@
 
 #include "buff_common.h"
 #include "buff_kernels.h"
+#include "buff_major.h"
 
 namespace buffgpu {
 
@@ -142,6 +143,8 @@ int quirk_minmax(Ctx* ctx, const double* x, uint64_t n, int32_t pexp, uint32_t d
 bool filter_plan(const Chunk* ch, int op, double pred, uint32_t flags, FilterTarget* tg, int* fill);
 double reference_sum_fold(const Chunk* ch, const uint64_t* ps);
 int mat_plan(const Chunk* ch, int32_t precision, uint32_t flags, MatParams* out);
+// Stored bits a reduced-precision read keeps (split_double.rs:1642-1664, u32 arithmetic).
+int precision_bits(const Chunk* ch, int32_t precision, uint32_t* R);
 int filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags, uint64_t* words,
                   unsigned long long* stats);
 int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64_t* plane_sums);
@@ -182,6 +185,13 @@ int major_max(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t
 int major_materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* out,
                              uint64_t* rows);
 uint64_t major_decoded_rows(const Chunk* ch, uint32_t flags);
+int major_precision_drop(const Chunk* ch, int32_t precision, uint32_t* drop);
+MajorView major_view(const Chunk* ch);
+// Filter targets of a buff-major chunk; returns -1 when the filter must run, else
+// the fill of every row (1: all rows, 0: none) decided without reading the chunk.
+int major_filter_plan(const Chunk* ch, int op, double pred, uint32_t flags, MajorTarget* tg);
+void major_sum_fold(const Chunk* ch, const uint64_t* outsum, const uint64_t* dense_ps, double* ref_sum,
+                    double* exact_sum, uint64_t* col_sums);
 void major_release(Chunk* ch);
 
 }  // namespace buffgpu

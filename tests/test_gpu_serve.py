@@ -256,3 +256,134 @@ def test_trace_per_call_baseline(gpu_ctx, served, oracle):
         for b in bufs:
             if b:
                 b.free()
+
+
+# ---- buff-major chunks (the served workload's layout) ---------------------------
+
+@pytest.fixture(scope="module")
+def served_major(gpu_ctx, oracle):
+    rng = np.random.default_rng(8)
+    lens = [int(v) for v in np.exp(rng.uniform(np.log(5000), np.log(300_000), 10))]
+    xs = [dg.outlier_heavy(n, 3, 70 + i) for i, n in enumerate(lens)]
+    xs.append(dg.random_walk(40_000, 3, 99))   # no majority byte: a buff-major stream without sparse columns
+    lens.append(40_000)
+    chunks = [gpu_ctx.encode(x, 10 ** 3, flags=buff_amd.BUFF_MAJORITY) for x in xs]
+    streams = [c.serialize() for c in chunks]
+    for x, s in zip(xs, streams):
+        assert np.array_equal(s, oracle.major_encode(x, 10 ** 3))
+    yield chunks, lens, xs, streams
+    for c in chunks:
+        c.free()
+
+
+def _major_requests(lens, seed, n):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        k = int(rng.integers(0, len(lens)))
+        u = rng.random()
+        sem = N.BUFF_SEMANTIC if rng.random() < 0.4 else 0
+        pred = float(rng.choice([99.2, 99.8, 100.0, 100.3, 100.4, 101.0, -5e6, 3e8, 1e10]))
+        if u < 0.45:
+            out.append((N.BUFF_Q_GT, k, pred, -1, sem))
+        elif u < 0.55:
+            out.append((N.BUFF_Q_EQ, k, pred, -1, sem))
+        elif u < 0.8:
+            out.append((N.BUFF_Q_SUM, k, 0.0, -1, 0))
+        elif u < 0.95:
+            out.append((N.BUFF_Q_MAT, k, 0.0, int(rng.integers(1, 4)), N.BUFF_SEMANTIC))
+        else:
+            out.append((N.BUFF_Q_MAT, k, 0.0, -1, sem))
+    return out
+
+
+def _major_want(oracle, x, s, q):
+    """The oracle's answer (compat) or numpy's on the true offsets (BUFF_SEMANTIC)."""
+    from tests import closed_form as cf
+
+    op, k, pred, prec, flags = q
+    hdr = np.frombuffer(np.asarray(s[:20], dtype=np.uint8).tobytes(), dtype=np.uint32)
+    h = {"ilen": int(hdr[3]), "dlen": int(hdr[4])}   # header fields (the body is the buff-major layout)
+    dlen = h["dlen"]
+    F = (h["ilen"] + dlen) & 0xFFFFFFFF
+    f = cf.fixed(x, dlen)
+    base = int(f.min())
+    if op in (N.BUFF_Q_GT, N.BUFF_Q_EQ):
+        kind = "range" if op == N.BUFF_Q_GT else "equal"
+        if flags & N.BUFF_SEMANTIC:
+            fp = cf.fixed_pred(pred, 10 ** 3, dlen)
+            m = np.full(len(x), kind == "range") if fp < base else (f > fp if kind == "range" else f == fp)
+            return cf.to_words(m), int(m.sum())
+        w, c = oracle.major_filter(s, 10 ** 3, kind, pred)[:2]
+        return w, c
+    if op == N.BUFF_Q_SUM:
+        return oracle.major_sum(s, 10 ** 3)[0], sum(int(v) for v in f.astype(object))
+    if prec < 0 and not flags & N.BUFF_SEMANTIC:
+        return oracle.major_decode(s, 10 ** 3)
+    try:
+        R = F if prec < 0 else min(cf.precision_R(h, prec), F)
+    except (ValueError, KeyError):
+        R = F   # more decimals than stored: every bit
+    sh = F - R
+    off = (f - base).astype(np.uint64)
+    return ((((off >> np.uint64(sh)) << np.uint64(sh)).astype(np.int64) + base).astype(np.float64) / 2.0 ** dlen)
+
+
+def test_served_buff_major_matches_oracle(gpu_ctx, served_major, oracle):
+    # buff-major chunks served with continuous batching: every answer equals the
+    # single-chunk call's and the oracle's (numpy's on the true offsets for
+    # BUFF_SEMANTIC); a batched max on a chunk with sparse columns is refused
+    from fractions import Fraction
+
+    chunks, lens, xs, streams = served_major
+    reqs = _major_requests(lens, 21, 300)
+    reqs.append((N.BUFF_Q_MAX, 0, 0.0, -1, 0))
+    bufs = _outs(gpu_ctx, lens, [(N.BUFF_Q_GT if q[0] == N.BUFF_Q_EQ else q[0],) + q[1:] for q in reqs])
+    srv = buff_amd.Server(gpu_ctx, max_batch=64, slots=2)
+    try:
+        for c in chunks:
+            srv.add_chunk(c)
+        res, _ = srv.run_trace(reqs, np.zeros(len(reqs)), [b.ptr if b else 0 for b in bufs])
+        batches, _, largest = srv.stats()
+        assert largest > 1
+        for q, r, b in zip(reqs, res, bufs):
+            op, k, pred, prec, flags = q
+            n = lens[k]
+            if op == N.BUFF_Q_MAX:
+                assert r.status == (N.BUFF_E_UNSUPPORTED if chunks[k].info().sparse_cols else 0), q
+                continue
+            assert r.status == 0, (q, r.status)
+            want = _major_want(oracle, xs[k], streams[k], q)
+            if op in (N.BUFF_Q_GT, N.BUFF_Q_EQ):
+                words = b.download(np.uint32, ((n + 63) // 64) * 2)[: (n + 31) // 32]
+                assert r.count == want[1] and np.array_equal(words, want[0]), q
+                alone = chunks[k].filter("range" if op == N.BUFF_Q_GT else "equal", pred, flags=flags)
+                assert np.array_equal(alone[0], words), q
+            elif op == N.BUFF_Q_SUM:
+                ref, tot = want
+                dlen = chunks[k].info().dlen
+                assert r.value == ref, (q, r.value, ref)
+                assert r.exact == float(Fraction(tot, 1 << int(dlen))), q
+            else:
+                got = b.download(np.float64, n)[: len(want)]
+                assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), q
+                alone = chunks[k].materialize(prec, flags)
+                assert np.array_equal(alone.view(np.uint64), want.view(np.uint64)), q
+    finally:
+        srv.destroy()
+        for b in bufs:
+            if b:
+                b.free()
+
+
+def test_buff_major_semantic_reduced_precision(gpu_ctx, served_major, oracle):
+    # single-chunk materialization of a buff-major chunk at 1..3 decimals
+    # (BUFF_SEMANTIC: the top R stored bits); the reference has no such decode
+    chunks, lens, xs, streams = served_major
+    for k in (0, 3, len(chunks) - 1):
+        for p in (0, 1, 2, 3, 5):
+            want = _major_want(oracle, xs[k], streams[k], (N.BUFF_Q_MAT, k, 0.0, p, N.BUFF_SEMANTIC))
+            got = chunks[k].materialize(p, flags=N.BUFF_SEMANTIC)
+            assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), (k, p)
+        with pytest.raises(N.Unsupported):
+            chunks[k].materialize(2)
