@@ -153,6 +153,7 @@ SIGNATURES = {
     "buff_sum": [vp, vp, f64p, f64p, u64p],
     "buff_sum_ex": [vp, vp, u32, f64p, f64p, u64p],
     "buff_sum_precision": [vp, vp, i32, u32, f64p, f64p, u64p],
+    "buff_sum_noalloc": [vp, vp, f64p, f64p, u64p],
     "buff_max": [vp, vp, f64p, u64p],
     "buff_max_ex": [vp, vp, u32, f64p, u64p, u64p],
     "buff_filter_cond": [vp, vp, cint, f64, u32, u32p, u32p, u64, u64p],
@@ -352,6 +353,15 @@ class Chunk:
         ps = np.zeros(9, dtype=np.uint64)
         check(lib().buff_sum_ex(self.ctx.handle, self.handle, flags, ctypes.byref(r), ctypes.byref(e),
                                 ptr(ps, ctypes.c_uint64)))
+        return r.value, e.value, ps
+
+    def sum_noalloc(self):
+        """buff_sum without device allocation (the library's static scratch): (reference sum, exact sum,
+        per-sub-column u64 sums[9])."""
+        r, e = ctypes.c_double(), ctypes.c_double()
+        ps = np.zeros(9, dtype=np.uint64)
+        check(lib().buff_sum_noalloc(self.ctx.handle, self.handle, ctypes.byref(r), ctypes.byref(e),
+                                     ptr(ps, ctypes.c_uint64)))
         return r.value, e.value, ps
 
     def sum_precision(self, precision, flags=0):

@@ -234,6 +234,17 @@ int buff_sum(buff_ctx* ctx, const buff_chunk* chunk, double* ref_sum, double* ex
  * ref_sum = exact_sum, the f64 running sum's value within its rounding. */
 int buff_sum_ex(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, double* ref_sum, double* exact_sum,
                 uint64_t* plane_sums);
+/* buff_sum without allocating: no reference counterpart; for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). The same results as buff_sum on a plain
+ * (byte_fixed) chunk; buff-major and byte-slice chunks are BUFF_E_UNSUPPORTED.
+ * The partial sums live in a 589,896-byte device array of the library (one per
+ * GPU, held until the process ends) that covers every column length (the sum's
+ * grid is capped); the call allocates nothing and returns after its work is
+ * done. All calls on one GPU share that array: calls that overlap on one GPU
+ * race and must be serialized by the caller. */
+int buff_sum_noalloc(buff_ctx* ctx, const buff_chunk* chunk, double* ref_sum, double* exact_sum,
+                     uint64_t* plane_sums);
 /* byte_residue_sum_with_precision (split_double.rs:2409-2629) on a plain chunk
  * (the stream branch taken when ilen <= 15; larger ilen is BUFF_E_UNSUPPORTED,
  * as are buff-major and byte-slice chunks). The read width R (the rule of
