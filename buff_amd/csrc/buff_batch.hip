@@ -701,11 +701,13 @@ __device__ __forceinline__ uint32_t major_filter_item(const MajorPrefix<S>& mv, 
   for (uint32_t k = 0; k < kMaxPlanes; ++k) tg.digit[k] = q.tg.digit[k];
   tg.tail = q.tg.tail;
   uint32_t count = 0;
+  // two tiles in flight per step (the second may lie past the item: computed, not stored)
   #pragma unroll 1
-  for (uint64_t c = it.c0; c < it.c1; ++c) {
-    uint32_t res[1];
-    major_filter_group<MODE, S, 1>(mv, tg, sem, c, lane, res);
+  for (uint64_t c = it.c0; c < it.c1; c += 2) {
+    uint32_t res[2];
+    major_filter_group<MODE, S, 2>(mv, tg, sem, c, lane, res);
     filter_store<MODE>(mv.dense, out16, c, lane, res[0], count);
+    if (c + 1 < it.c1) filter_store<MODE>(mv.dense, out16, c + 1, lane, res[1], count);
   }
   return count;
 }

@@ -1115,21 +1115,43 @@ int buff_server_run_trace(buff_server* server, const buff_query* qs, const doubl
   Server* sv = reinterpret_cast<Server*>(server);
   if (!sv || (n && (!qs || !arrival_s))) return fail(BUFF_E_INVALID, "null argument");
   std::vector<uint64_t> tickets(n);
-  const double t0 = now_s();
-  for (uint32_t i = 0; i < n; ++i) {
-    while (now_s() - t0 < arrival_s[i]) std::this_thread::yield();
-    BUFF_TRY(buff_server_submit(server, &qs[i], outs ? outs[i] : nullptr, &tickets[i]));
-  }
-  for (uint32_t i = 0; i < n; ++i) {
+  uint32_t drained = 0;   // requests [0, drained) are collected
+  // collects request `drained` (waiting for it when `block`), returning its arena
+  // output range at once: outputs are not handed back, so the arena recycles
+  // while later requests are still being submitted
+  auto drain_one = [&](bool block, bool* took) -> int {
+    *took = false;
+    if (drained >= n) return BUFF_OK;
+    if (!block) {
+      int done = 0;
+      BUFF_TRY(buff_server_poll(server, tickets[drained], &done));
+      if (!done) return BUFF_OK;
+    }
+    const uint32_t i = drained;
     buff_query_result r{};
     BUFF_TRY(buff_server_wait(server, tickets[i], &r, latency_us ? &latency_us[i] : nullptr));
-    // an output the server arena holds goes back as soon as it is complete (its address is not returned)
     if (r.status == BUFF_OK && !(outs && outs[i]) && (qs[i].op == BUFF_Q_GT || qs[i].op == BUFF_Q_EQ ||
                                                       qs[i].op == BUFF_Q_MAT)) {
       BUFF_TRY(buff_server_release(server, (const void*)r.out_offset));
       r.out_offset = 0;
     }
     if (res) res[i] = r;
+    ++drained;
+    *took = true;
+    return BUFF_OK;
+  };
+  const double t0 = now_s();
+  for (uint32_t i = 0; i < n; ++i) {
+    while (now_s() - t0 < arrival_s[i]) {
+      bool took = false;
+      if (drained < i) BUFF_TRY(drain_one(false, &took));
+      if (!took) std::this_thread::yield();
+    }
+    BUFF_TRY(buff_server_submit(server, &qs[i], outs ? outs[i] : nullptr, &tickets[i]));
+  }
+  while (drained < n) {
+    bool took = false;
+    BUFF_TRY(drain_one(true, &took));
   }
   return BUFF_OK;
 }

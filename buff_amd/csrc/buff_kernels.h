@@ -14,7 +14,7 @@ struct LaunchShape {
   uint32_t tiles_per_wave;  // tiles a wave loads per step (0: the kernel's default; filter 2/4/8, sum 1/2/4,
                             // max 4/8/16)
   uint32_t prefetch;        // filter: load the next step's top plane before refining this one;
-                            // max: 1 = the two-pass form (0 = one pass)
+                            // max: 1 = the one-pass form (0 = two passes)
 };
 
 enum : uint32_t { kMatFull = 0, kMatSemantic = 1, kMatRefPrecision = 2 };
@@ -64,8 +64,8 @@ hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t
 // the sum of the n q-bit fields the reference reads from the packed tail,
 // out[1] += the sum of every row's top q tail bits (the caller zeroes out[0..1]).
 hipError_t launch_tail_fields(const ChunkView& cv, uint32_t q, unsigned long long* out, hipStream_t s);
-// One pass (ls.prefetch == 0), or two passes using `scratch`
-// (max_scratch_bytes(ntiles) bytes); the caller zeroes `best`.
+// Two passes using `scratch` (max_scratch_bytes(ntiles) bytes), or one pass
+// (ls.prefetch == 1); the caller zeroes `best`.
 uint64_t max_scratch_bytes(uint64_t ntiles);
 hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* scratch, hipStream_t s,
                       const LaunchShape& ls);

@@ -612,7 +612,8 @@ __global__ __launch_bounds__(256) void k_max_refine(ChunkView cv, const uint8_t*
   if (have && lane == 0) atomicMax(best_out, (unsigned long long)best);
 }
 
-// One pass (the default form): every wave streams the top plane of its tiles
+// One pass (a tuner choice; the two-pass form below is the default): every
+// wave streams the top plane of its tiles
 // and keeps (top, best), the largest top byte it has seen and the largest
 // offset among its tiles whose top byte reached `top`. Such a tile is narrowed
 // as the reference narrows its candidates: the rows holding the tile's top
@@ -1137,7 +1138,7 @@ static void launch_max_fused_u(const ChunkView& cv, unsigned long long* best, hi
 
 hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* scratch, hipStream_t s,
                       const LaunchShape& ls) {
-  if (ls.prefetch == 0) {   // one pass
+  if (ls.prefetch == 1) {   // one pass
     switch (ls.tiles_per_wave) {
       case 4: launch_max_fused_u<4>(cv, best, s, ls); break;
       case 16: launch_max_fused_u<16>(cv, best, s, ls); break;

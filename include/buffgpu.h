@@ -273,7 +273,9 @@ int buff_max_ex(buff_ctx* ctx, const buff_chunk* chunk, uint32_t flags, double* 
  * byte_residue_decode_with_precision (split_double.rs:1612-1893) at `precision`
  * decimals. out: n doubles (host). A buff-major chunk decodes as
  * buff_decode_majority (buff_simd.rs:633-861) and writes decoded_rows values
- * (buff_chunk_info); it has no reduced-precision form (BUFF_E_UNSUPPORTED). */
+ * (buff_chunk_info); the reference has no reduced-precision decode of it
+ * (BUFF_E_UNSUPPORTED), but with BUFF_SEMANTIC a precision >= 0 gives the true
+ * values truncated to their top R stored bits (R from the same read rule). */
 int buff_materialize(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags, double* out);
 int buff_materialize_device(buff_ctx* ctx, const buff_chunk* chunk, int32_t precision, uint32_t flags,
                             double* out_dev);
@@ -290,7 +292,9 @@ typedef struct {
   uint32_t waves_per_wg;
   uint32_t max_wg;         /* 0: the workgroups that fit resident */
   uint32_t tiles_per_wave; /* 0: the kernel's default */
-  uint32_t prefetch;       /* range filter: next step's top plane loaded before refining */
+  uint32_t prefetch;       /* range filter: next step's top plane loaded before refining; max: 1 = the
+                              one-pass form (0: the top bytes of every tile, then only the tiles holding
+                              the largest) */
   float best_us;           /* kernel time of the chosen shape */
   float default_us;        /* kernel time of the default shape */
 } buff_tune_choice;
@@ -381,10 +385,12 @@ typedef struct {
 } buff_server_cfg;
 
 int buff_server_create(buff_ctx* ctx, const buff_server_cfg* cfg, buff_server** out);
-/* Makes a resident byte_fixed chunk of the same context servable; *id is the
- * chunk index requests name. The caller keeps ownership (free it after the
- * server). Calls from several client threads may overlap; every call on the
- * server is serialized on its lock. */
+/* Makes a resident byte_fixed or buff-major chunk of the same context
+ * servable; *id is the chunk index requests name. The caller keeps ownership
+ * (free it after the server). Calls from several client threads may overlap;
+ * every call on the server is serialized on its lock. On a buff-major chunk
+ * with sparse columns, filters, sums and materializations are batched like the
+ * others; a max there is BUFF_E_UNSUPPORTED (buff_max answers it). */
 int buff_server_add_chunk(buff_server* server, buff_chunk* chunk, uint32_t* id);
 /* Queues one request (buff_query.chunk = server chunk id). out_dev: where a
  * filter writes its ceil(n/64) bitmap words or a materialization its n doubles
