@@ -54,6 +54,18 @@ __device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t lane
   return s - v;
 }
 
+// Max over the 64 lanes with the same DPP steps (every lane gets it).
+__device__ __forceinline__ uint32_t wave_max_u32_dpp(uint32_t v) {
+  uint32_t s = v;
+  s = max(s, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x111, 0xf, 0xf, false));   // row_shr:1
+  s = max(s, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x112, 0xf, 0xf, false));   // row_shr:2
+  s = max(s, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x114, 0xf, 0xf, false));   // row_shr:4
+  s = max(s, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x118, 0xf, 0xf, false));   // row_shr:8
+  s = max(s, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x142, 0xa, 0xf, false));   // row_bcast:15
+  s = max(s, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s, 0x143, 0xc, 0xf, false));   // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)s, 63);
+}
+
 __device__ __forceinline__ uint32_t wave_total_u32(uint32_t v) {
   const uint32_t incl = wave_excl_scan_u32(v, 0) + v;
   return (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);

@@ -70,6 +70,9 @@ struct MajorMaxState {
   uint32_t chosen[16];
   unsigned long long acc;
   unsigned long long argmax;
+  uint32_t top1;       // list path: max outlier byte of column 0, plus 1 (0: none)
+  uint32_t count;      // list path: rows that hold column 0's chosen byte (set when taken)
+  uint32_t fallback;   // list path not taken: the staged kernels must run
 };
 
 // Pass 1 (x read once); me.nwg / me.tpw set by major_enc_geometry().
@@ -91,6 +94,17 @@ hipError_t launch_major_decode(const MajorView& mv, bool semantic, uint32_t drop
 uint64_t major_sum_scratch_bytes(uint32_t S);
 hipError_t launch_major_outlier_sums(const MajorView& mv, const uint64_t* n_out, unsigned long long* sums,
                                      unsigned long long* scratch, hipStream_t s);
+// Max by a list of the rows that hold column 0's chosen byte (S >= 1, n >= 1;
+// n_out0 = outliers of column 0). The same answer as the staged max; when the
+// majority byte is column 0's chosen byte or more than kMaxListRows rows hold
+// it, st->fallback is set and the staged max must run. Scratch:
+// kMaxListScratch + ntiles bytes.
+constexpr uint32_t kMaxListRows = 8192;
+constexpr uint64_t kListCounterBytes = 64 * 128;
+constexpr uint32_t kTopBlocks = 1024;   // workgroups of the first list pass (one partial max each)
+constexpr uint64_t kMaxListScratch = kListCounterBytes + 4ull * kTopBlocks + 16ull * kMaxListRows;
+hipError_t launch_major_max_list(const MajorView& mv, uint64_t n_out0, MajorMaxState* st, uint8_t* scratch,
+                                 hipStream_t s, bool semantic);
 // Staged max over the byte columns and the tail; scratch: major_max_scratch_bytes(ntiles).
 uint64_t major_max_scratch_bytes(uint64_t ntiles);
 hipError_t launch_major_max(const MajorView& mv, MajorMaxState* st, uint8_t* scratch, hipStream_t s,

@@ -473,6 +473,291 @@ __global__ __launch_bounds__(256) void k_major_max_final(const uint32_t* __restr
 }
 
 // ----------------------------------------------------------------------------
+// Max by a row list. Column 0's chosen byte depends only on its outlier bytes
+// (and whether any row is not an outlier), so it is a reduction over the
+// compacted bytes. When it is not the majority byte, the candidates of every
+// later stage are among the rows whose column-0 outlier byte equals it: outliers
+// hold the extremes, so these are few. One pass over the tiles collects each
+// such row's byte in every column and its tail; one workgroup then runs the
+// stage rules of k_major_max_final on that list.
+
+struct MaxRow {
+  uint8_t byte[kMaxPlanes];   // sparse columns, then dense planes (S + P <= 8)
+  uint8_t tail;
+  uint8_t outlier;            // bit j: an outlier of sparse column j
+  uint16_t pad;
+  uint32_t row;
+};
+static_assert(sizeof(MaxRow) == 16, "MaxRow is one dwordx4");
+
+// Column 0's chosen byte (the rule of k_major_max_final for stage 0).
+__device__ __forceinline__ uint32_t list_chosen0(uint32_t top1, uint32_t major, bool semantic, bool has_major) {
+  if (!semantic) return top1 ? max(major, top1 - 1) : major;
+  const uint32_t v = max(top1, has_major ? major + 1 : 0u);
+  return v ? v - 1 : 0u;
+}
+
+// The collect pass appends rows to kListRegions regions (a counter each, 128
+// bytes apart, and kMaxListRows / kListRegions rows): hit rows turn up in many
+// waves at once, and same-address device atomics serialize.
+constexpr uint32_t kListRegions = 64;
+constexpr uint32_t kListRegionRows = kMaxListRows / kListRegions;
+constexpr uint32_t kListCounterStride = 32;   // u32s: 128 bytes
+static_assert(kListRegionRows * kListRegions == kMaxListRows, "list regions");
+
+// Per tile, column 0's largest outlier byte (0 for a tile without outliers:
+// never column 0's chosen byte on the list path, which is then above the
+// majority byte, or every row is an outlier) into tilemax; per workgroup the
+// max over its tiles, plus 1 (0: no outlier), into partial[blockIdx.x]. A
+// tile's outlier bytes are contiguous in the compacted array; lane i reads byte
+// i (and i + 64, ...). Workgroup 0 also clears the collect pass's counters.
+constexpr int kTopTiles = 16;   // tiles per wave step, their loads in flight together
+
+__global__ __launch_bounds__(256) void k_major_max_tiles(SparseDev c0, uint64_t ntiles,
+                                                         uint8_t* __restrict__ tilemax,
+                                                         uint32_t* __restrict__ partial,
+                                                         uint32_t* __restrict__ counters) {
+  constexpr int U = kTopTiles;
+  __shared__ uint32_t part[4];
+  if (blockIdx.x == 0 && threadIdx.x < kListRegions) counters[threadIdx.x * kListCounterStride] = 0;
+  const uint32_t lane = __lane_id();
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
+  uint32_t best = 0;
+  const uint64_t ngroups = (ntiles + U - 1) / U;
+  for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
+    const uint64_t cb = grp * U;
+    uint32_t p[U + 1];
+#pragma unroll
+    for (int u = 0; u <= U; ++u) p[u] = c0.prefix[min(cb + u, ntiles)];   // prefix[ntiles] = outliers
+    uint32_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = lane < p[u + 1] - p[u] ? (uint32_t)c0.outl[p[u] + lane] + 1u : 0u;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      for (uint32_t i = lane + 64; i < p[u + 1] - p[u]; i += 64) v[u] = max(v[u], (uint32_t)c0.outl[p[u] + i] + 1u);
+      const uint32_t m = wave_max_u32_dpp(v[u]);
+      best = max(best, m);
+      if (lane == (uint32_t)u && cb + u < ntiles) tilemax[cb + u] = (uint8_t)(m ? m - 1 : 0);
+    }
+  }
+  if (lane == 0) part[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = max(max(part[0], part[1]), max(part[2], part[3]));
+}
+
+// max over the top pass's partials (at most kTopBlocks; lane i reads 16 of them
+// as dwordx4s), every lane of the wave
+__device__ __forceinline__ uint32_t list_top1(const uint32_t* __restrict__ partial, uint32_t nparts) {
+  const uint32_t lane = __lane_id();
+  uint32_t v = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < kTopBlocks / 256; ++q) {
+    const uint32_t i = 4 * (lane + 64 * q);
+    if (i < nparts) {
+      const uint4 w = reinterpret_cast<const uint4*>(partial)[lane + 64 * q];
+      v = max(v, max(max(w.x, i + 1 < nparts ? w.y : 0u), max(i + 2 < nparts ? w.z : 0u, i + 3 < nparts ? w.w : 0u)));
+    }
+  }
+  return wave_max_u32_dpp(v);
+}
+
+// Tiles per wave of the collect pass (lane u < kCollectTiles reads tilemax of one).
+constexpr uint32_t kCollectTiles = 16;
+
+// The rows of the tiles whose column-0 max is the chosen byte: per hit tile,
+// every column's masks, the dense planes and the tail in one round of loads,
+// the outlier windows in a second.
+template <int S>
+__global__ __launch_bounds__(256) void k_major_max_collect(MajorView mv, const uint8_t* __restrict__ tilemax,
+                                                           const uint32_t* __restrict__ partial, uint32_t nparts,
+                                                           uint32_t* __restrict__ counters, MaxRow* __restrict__ rows,
+                                                           uint32_t semantic, uint32_t has_major) {
+  constexpr int NPM = (int)kMaxPlanes - S > 0 ? (int)kMaxPlanes - S : 1;
+  const uint32_t lane = __lane_id();
+  const uint64_t wave = wave_global();
+  const uint64_t nwaves = waves_in_grid();
+  const ChunkView& dv = mv.dense;
+  const uint32_t chosen0 = list_chosen0(list_top1(partial, nparts), mv.col[0].major, semantic != 0, has_major != 0);
+  if (chosen0 == mv.col[0].major) return;   // the majority rows are candidates: staged max
+  const uint32_t region = (uint32_t)(wave % kListRegions);
+  uint32_t* counter = counters + region * kListCounterStride;
+  MaxRow* rrows = rows + region * kListRegionRows;
+  const uint64_t ngroups = (dv.ntiles + kCollectTiles - 1) / kCollectTiles;
+  for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
+    const uint64_t cb = grp * kCollectTiles;
+    const bool mine = lane < kCollectTiles && cb + lane < dv.ntiles;
+    uint64_t hit_tiles = __ballot(mine && tilemax[cb + lane] == chosen0);
+    while (hit_tiles) {   // rare: the tiles holding the chosen byte, one at a time
+      const uint64_t c = cb + (uint32_t)__builtin_ctzll(hit_tiles);
+      hit_tiles &= hit_tiles - 1;
+      uint32_t m[S], pre[S];
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        m[j] = mv.col[j].mask16[c * 64 + lane];
+        pre[j] = mv.col[j].prefix[c];
+      }
+      uint4 wp[NPM];
+#pragma unroll
+      for (int p = 0; p < NPM; ++p) wp[p] = (uint32_t)p < dv.P ? load_plane(dv, p, c, lane) : make_uint4(0, 0, 0, 0);
+      uint32_t h[8];
+#pragma unroll
+      for (uint32_t j = 0; j < 8; ++j) h[j] = j < dv.r ? load_tail(dv, j, c, lane) : 0u;
+      Packed16 pk[S];
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const uint32_t excl = wave_excl_scan_u32(__popc(m[j]), lane);
+        pk[j] = m[j] ? load_packed(mv.col[j].outl, (uint64_t)pre[j] + excl) : Packed16{0, 0, 0, 0};
+      }
+      uint32_t gt_unused, hit;
+      sparse_cmp(pk[0], m[0], m[0], mv.col[0].major, chosen0, gt_unused, hit);   // chosen0 != major: outliers only
+      uint4 ws[S];
+#pragma unroll
+      for (int j = 1; j < S; ++j) ws[j] = patch_outliers(pk[j], m[j], m[j] & hit, mv.col[j].major);
+      uint32_t left = hit;
+      while (left) {
+        const uint32_t b = (uint32_t)__builtin_ctz(left);
+        left &= left - 1u;
+        MaxRow rec;
+#pragma unroll
+        for (uint32_t j = 0; j < kMaxPlanes; ++j) rec.byte[j] = 0;
+        rec.byte[0] = (uint8_t)chosen0;
+        rec.outlier = 1;
+#pragma unroll
+        for (int j = 1; j < S; ++j) {
+          rec.byte[j] = (uint8_t)byte_at(ws[j], b);
+          rec.outlier |= (uint8_t)(((m[j] >> b) & 1u) << j);
+        }
+#pragma unroll
+        for (int p = 0; p < NPM; ++p)
+          if (S + p < (int)kMaxPlanes && (uint32_t)p < dv.P) rec.byte[S + p] = (uint8_t)byte_at(wp[p], b);
+        uint32_t t = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) t |= j < dv.r ? ((h[j] >> b) & 1u) << j : 0u;
+        rec.tail = (uint8_t)t;
+        rec.pad = 0;
+        rec.row = (uint32_t)(c * kTileRows + 16ull * lane + b);
+        const uint32_t idx = atomicAdd(counter, 1u);
+        if (idx < kListRegionRows) rrows[idx] = rec;
+      }
+    }
+  }
+}
+
+// One workgroup of 1024 threads; every thread holds up to kListPerThread rows.
+constexpr uint32_t kListPerThread = kMaxListRows / 1024;
+
+// Max over the workgroup; slots alternate between calls, so one barrier a call
+// suffices (a wave writes a slot again only after every wave has passed the
+// next call's barrier, i.e. has finished reading it).
+__device__ __forceinline__ uint32_t block_max_1024(uint32_t v, uint32_t (&red)[2][16], uint32_t& parity) {
+  v = wave_max_u32_dpp(v);
+  uint32_t* slot = red[parity];
+  parity ^= 1u;
+  if (__lane_id() == 0) slot[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint32_t r = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) r = max(r, slot[i]);
+  return r;
+}
+
+// A list row as loaded: x, y = the column bytes (byte k of the pair is column k),
+// z = tail | outlier bits << 8.
+__device__ __forceinline__ uint32_t row_byte(const uint4& r, uint32_t k) {
+  return ((k < 4 ? r.x : r.y) >> (8 * (k & 3))) & 0xFFu;
+}
+
+// Writes every field of st the host reads: acc, argmax, count, fallback and the
+// chosen bytes (st is not cleared before the list path).
+__global__ __launch_bounds__(1024) void k_major_max_finish(MajorView mv, const uint32_t* __restrict__ partial,
+                                                           uint32_t nparts, MajorMaxState* __restrict__ st,
+                                                           const uint32_t* __restrict__ counters,
+                                                           const MaxRow* __restrict__ rows, uint32_t semantic,
+                                                           uint32_t has_major) {
+  __shared__ uint32_t red[2][16];
+  __shared__ uint32_t s_cnt[kListRegions];
+  uint32_t parity = 0;
+  const ChunkView& dv = mv.dense;
+  const uint32_t S = mv.S, ncol = mv.S + dv.P;
+  const uint32_t top1 = list_top1(partial, nparts);
+  const uint32_t chosen0 = list_chosen0(top1, mv.col[0].major, semantic != 0, has_major != 0);
+  // region counts: a region past its rows (or no row at all) means the staged max
+  const uint32_t rc = threadIdx.x < kListRegions ? counters[threadIdx.x * kListCounterStride] : 0u;
+  if (threadIdx.x < kListRegions) s_cnt[threadIdx.x] = rc;
+  const uint32_t total = wave_total_u32(min(rc, kListRegionRows + 1));   // the first wave holds every region
+  const bool over = __syncthreads_or(rc > kListRegionRows) != 0;
+  const bool none = __syncthreads_or(threadIdx.x == 0 && total == 0) != 0;
+  if (threadIdx.x == 0) {
+    st->top1 = top1;
+    st->count = total;
+    st->argmax = 0;
+    st->fallback = chosen0 == mv.col[0].major || over || none ? 1u : 0u;
+  }
+  if (chosen0 == mv.col[0].major || over || none) return;
+  const uint4* rv = reinterpret_cast<const uint4*>(rows);
+  uint4 r[kListPerThread];
+  uint32_t active = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < kListPerThread; ++q) {
+    const uint32_t slot = threadIdx.x + 1024 * q;   // region slot / kListRegionRows, row slot % kListRegionRows
+    const bool valid = slot % kListRegionRows < s_cnt[slot / kListRegionRows];
+    r[q] = valid ? rv[slot] : make_uint4(0, 0, 0, 0);
+    active |= valid ? 1u << q : 0u;
+  }
+  unsigned long long acc = (unsigned long long)chosen0 << mv.col[0].shift;
+  if (threadIdx.x == 0) st->chosen[0] = chosen0;
+  for (uint32_t k = 1; k < ncol; ++k) {
+    const bool sparse = k < S;
+    uint32_t v = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kListPerThread; ++q) {
+      const bool outl = (r[q].z >> (8 + k)) & 1u;
+      if (((active >> q) & 1u) && (!sparse || semantic || outl)) v = max(v, row_byte(r[q], k) + 1u);
+    }
+    v = block_max_1024(v, red, parity);
+    uint32_t chosen;
+    bool merge = true;
+    if (sparse && !semantic) {
+      chosen = v ? max(mv.col[k].major, v - 1) : mv.col[k].major;
+      merge = v != 0;
+    } else {
+      chosen = v ? v - 1 : 0u;
+    }
+    const uint32_t shift = sparse ? mv.col[k].shift : dv.F - 8 * (k - S + 1);
+    if (merge) acc |= (unsigned long long)chosen << shift;
+    if (threadIdx.x == 0) st->chosen[k] = chosen;
+#pragma unroll
+    for (uint32_t q = 0; q < kListPerThread; ++q)
+      if (row_byte(r[q], k) != chosen) active &= ~(1u << q);
+  }
+  if (dv.r) {
+    uint32_t v = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kListPerThread; ++q)
+      if ((active >> q) & 1u) v = max(v, (r[q].z & 0xFFu) + 1u);
+    v = block_max_1024(v, red, parity);
+    const uint32_t chosen = v ? v - 1 : 0u;
+    acc |= chosen;
+    if (threadIdx.x == 0) st->chosen[ncol] = chosen;
+#pragma unroll
+    for (uint32_t q = 0; q < kListPerThread; ++q)
+      if ((r[q].z & 0xFFu) != chosen) active &= ~(1u << q);
+    // rows left: the count stage's argmax
+    const uint32_t nleft = wave_total_u32((uint32_t)__popc(active));
+    uint32_t* slot = red[parity];
+    if (__lane_id() == 0) slot[threadIdx.x >> 6] = nleft;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t tot = 0;
+      for (int i = 0; i < 16; ++i) tot += slot[i];
+      st->argmax = tot;
+    }
+  }
+  if (threadIdx.x == 0) st->acc = acc;
+}
+
+// ----------------------------------------------------------------------------
 // launchers
 
 static inline uint32_t tiles_grid(uint64_t ntiles, uint32_t max_wg) {
@@ -583,6 +868,39 @@ hipError_t launch_major_outlier_sums(const MajorView& mv, const uint64_t* n_out,
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   launch_k(k_major_outsum_final, dim3(mv.S), dim3(64), 0, s, scratch, kOutsumBlocks, sums);
+  return hipGetLastError();
+}
+
+hipError_t launch_major_max_list(const MajorView& mv, uint64_t n_out0, MajorMaxState* st, uint8_t* scratch,
+                                 hipStream_t s, bool semantic) {
+  const ChunkView& dv = mv.dense;
+  if (mv.S < 1 || mv.S > kMaxSparse || dv.n == 0) return hipErrorInvalidValue;
+  const uint32_t has_major = n_out0 < dv.n ? 1u : 0u;
+  const uint32_t sem = semantic ? 1u : 0u;
+  uint32_t* counters = reinterpret_cast<uint32_t*>(scratch);
+  uint32_t* partial = reinterpret_cast<uint32_t*>(scratch + kListCounterBytes);
+  MaxRow* rows = reinterpret_cast<MaxRow*>(scratch + kListCounterBytes + kTopBlocks * 4);
+  uint8_t* tilemax = scratch + kMaxListScratch;
+  const uint64_t groups = (dv.ntiles + kTopTiles - 1) / kTopTiles;   // one wave each
+  const uint32_t g1 = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((groups + 3) / 4, 1), kTopBlocks);
+  launch_k(k_major_max_tiles, dim3(g1), dim3(256), 0, s, mv.col[0], dv.ntiles, tilemax, partial, counters);
+  const uint64_t cgroups = (dv.ntiles + kCollectTiles - 1) / kCollectTiles;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((cgroups + 3) / 4, 1), 4096);
+#define BUFF_COLLECT(SS)                                                                               \
+  launch_k((k_major_max_collect<SS>), dim3(grid), dim3(256), 0, s, mv, tilemax, partial, g1, counters, rows, sem, \
+           has_major)
+  switch (mv.S) {
+    case 1: BUFF_COLLECT(1); break;
+    case 2: BUFF_COLLECT(2); break;
+    case 3: BUFF_COLLECT(3); break;
+    case 4: BUFF_COLLECT(4); break;
+    case 5: BUFF_COLLECT(5); break;
+    case 6: BUFF_COLLECT(6); break;
+    case 7: BUFF_COLLECT(7); break;
+    default: BUFF_COLLECT(8); break;
+  }
+#undef BUFF_COLLECT
+  launch_k(k_major_max_finish, dim3(1), dim3(1024), 0, s, mv, partial, g1, st, counters, rows, sem, has_major);
   return hipGetLastError();
 }
 

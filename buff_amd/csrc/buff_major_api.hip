@@ -649,15 +649,27 @@ int major_sum(Ctx* ctx, const Chunk* ch, uint32_t flags, double* ref_sum, double
 int major_max(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t* max_off, uint64_t* argmax) {
   const bool semantic = (flags & BUFF_SEMANTIC) != 0;
   const ChunkView& v = ch->view;
-  BUFF_TRY(ctx->scratch(ctx->s_best, 256 + major_max_scratch_bytes(v.ntiles)));
+  BUFF_TRY(ctx->scratch(ctx->s_best,
+                        256 + std::max<uint64_t>(major_max_scratch_bytes(v.ntiles), kMaxListScratch + v.ntiles)));
   uint8_t* sb = (uint8_t*)ctx->s_best.ptr;
   MajorMaxState* st = (MajorMaxState*)sb;
-  {
-    ProfScope sc(ctx, "k_major_max");
-    HIP_TRY(launch_major_max(major_view(ch), st, sb + 256, ctx->stream, ctx->shape(Op::Max, v), semantic));
-  }
+  const MajorView mv = major_view(ch);
   MajorMaxState hs;
-  BUFF_TRY(d2h_small(ctx, &hs, st, sizeof(hs)));
+  hs.fallback = 1;
+  if (ch->S >= 1 && ch->n > 0) {   // the rows holding column 0's chosen byte, when they are few
+    {
+      ProfScope sc(ctx, "k_major_max");
+      HIP_TRY(launch_major_max_list(mv, ch->sp[0].n_out, st, sb + 256, ctx->stream, semantic));
+    }
+    BUFF_TRY(d2h_small(ctx, &hs, st, sizeof(hs)));
+  }
+  if (hs.fallback) {
+    {
+      ProfScope sc(ctx, "k_major_max");
+      HIP_TRY(launch_major_max(mv, st, sb + 256, ctx->stream, ctx->shape(Op::Max, v), semantic));
+    }
+    BUFF_TRY(d2h_small(ctx, &hs, st, sizeof(hs)));
+  }
   if (max_off) *max_off = hs.acc;
   if (argmax) *argmax = hs.argmax;
   // buff_simd.rs:2855: (max as i64 + base_int) as f64 / 2^dlen

@@ -67,6 +67,26 @@ def case_max_quirks(n=20000, seed=12, below=False):
     return from_offsets(off)
 
 
+def case_max_in_majority(n=30000, seed=18):
+    """F = 24; every column-0 outlier byte is under the majority byte, so the
+    majority rows hold the max (the staged max, not the row list)."""
+    rng = np.random.default_rng(seed)
+    c0 = bytes_col(rng, n, 0x80, 0.05, 0, 0x80)
+    c1 = bytes_col(rng, n, 0x30, 0.05)
+    c2 = rng.integers(0, 256, n)
+    return from_offsets((c0 << 16) | (c1 << 8) | c2)
+
+
+def case_max_many_ties(n=180000, seed=19):
+    """F = 26; about 10,800 rows share the top column-0 outlier byte, more than
+    the max's row list holds (the staged max runs instead)."""
+    rng = np.random.default_rng(seed)
+    c0 = np.where(rng.random(n) < 0.06, 0xF0, 0x40)
+    c1 = bytes_col(rng, n, 0x30, 0.05)
+    c2 = rng.integers(0, 1024, n)
+    return from_offsets((c0 << 18) | (c1 << 10) | c2)
+
+
 def case_all_sparse(n=12000, seed=13):
     """F = 16 with both byte columns sparse: decode returns nothing
     (buff_simd.rs:788-845), the sum keeps its base term."""
@@ -126,6 +146,8 @@ CASES = {
     "all_sparse": (case_all_sparse, SCALE),
     "no_outliers": (case_no_outliers, SCALE),
     "no_majority": (case_no_majority, SCALE),
+    "max_in_majority": (case_max_in_majority, SCALE),
+    "max_many_ties": (case_max_many_ties, SCALE),
 }
 
 

@@ -96,7 +96,8 @@ def test_queries_match_oracle(gpu_ctx, oracle, built, name):
         loaded.free()
 
 
-@pytest.mark.parametrize("name", ["config3", "major_zero", "max_quirk_below", "all_sparse", "no_outliers"])
+@pytest.mark.parametrize("name", ["config3", "major_zero", "max_quirk_below", "all_sparse", "no_outliers",
+                                  "max_in_majority", "max_many_ties"])
 def test_semantic_answers(gpu_ctx, built, name):
     """BUFF_SEMANTIC: the intended answers on the true offsets."""
     x, scale, ref = built[name]
