@@ -578,7 +578,9 @@ def bench_config3(ctx, args, buff_amd, datagen):
                                  "kernels_us": kern}
         # algorithmic bytes of the two ops the headline metric names
         S, P, r = info.sparse_cols, info.planes, info.tail_bits
-        sparse_b = S * n / 8.0 + sum(n_out) + S * ((n + 1023) // 1024) * 4.0
+        # sparse columns as the filter reads them: per outlier its byte and its
+        # u16 row in the tile, per tile the prefix count
+        sparse_b = 3.0 * sum(n_out) + S * ((n + 1023) // 1024) * 4.0
         if fmt == "buff-major":
             rows_p, rows_t = major_filter_rows(x, info, majors, pred10, dlen)
             fb = sparse_b + rows_p + rows_t * r / 8.0 + n / 8.0

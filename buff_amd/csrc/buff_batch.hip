@@ -694,7 +694,7 @@ __global__ __launch_bounds__(256) void k_bq_mat(const BatchBuffers* __restrict__
 
 template <int MODE, int S>
 __device__ __forceinline__ uint32_t major_filter_item(const MajorPrefix<S>& mv, const BQuery& q, const BItem& it,
-                                                      uint32_t lane, bool sem) {
+                                                      uint32_t lane, bool sem, unsigned long long* dep) {
   BUFF_GLOBAL uint16_t* out16 = gmem_w(reinterpret_cast<uint16_t*>(q.out));
   MajorTarget tg;
 #pragma unroll
@@ -705,29 +705,29 @@ __device__ __forceinline__ uint32_t major_filter_item(const MajorPrefix<S>& mv, 
   #pragma unroll 1
   for (uint64_t c = it.c0; c < it.c1; c += 2) {
     uint32_t res[2];
-    major_filter_group<MODE, S, 2>(mv, tg, sem, c, lane, res);
+    major_filter_group<MODE, S, 2>(mv, tg, sem, c, lane, dep, res);
     filter_store<MODE>(mv.dense, out16, c, lane, res[0], count);
     if (c + 1 < it.c1) filter_store<MODE>(mv.dense, out16, c + 1, lane, res[1], count);
   }
   return count;
 }
 
-// Per-lane outlier byte sums of every sparse column over tiles [c0, c1).
+// Per-lane outlier byte sums of every sparse column over tiles [c0, c1): their
+// outlier bytes are contiguous, [prefix[c0], prefix[c1]) of the compacted array,
+// read as aligned dwords with the bytes outside the range masked off (the
+// column's buffer is padded past its last byte).
 template <int S>
 __device__ __forceinline__ void major_outlier_sums(const MajorPrefix<S>& mv, const BItem& it, uint32_t lane,
                                                    uint32_t (&osum)[S]) {
-  #pragma unroll 1
-  for (uint64_t c = it.c0; c < it.c1; ++c) {
-    uint32_t m[S], pre[S];
 #pragma unroll
-    for (int j = 0; j < S; ++j) {
-      m[j] = mv.col[j].mask16[c * 64 + lane];
-      pre[j] = mv.col[j].prefix[c];
-    }
-#pragma unroll
-    for (int j = 0; j < S; ++j) {
-      const uint32_t excl = wave_excl_scan_u32(__popc(m[j]), lane);
-      if (m[j]) osum[j] += packed_sum(load_packed(mv.col[j].outl, (uint64_t)pre[j] + excl), m[j]);
+  for (int j = 0; j < S; ++j) {
+    const uint32_t a = mv.col[j].prefix[it.c0], b = mv.col[j].prefix[it.c1];
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(mv.col[j].outl);
+    for (uint32_t k = (a >> 2) + lane; 4 * k < b; k += 64) {
+      uint32_t v = w[k];
+      if (4 * k < a) v &= 0xFFFFFFFFu << (8 * (a - 4 * k));        // bytes before a
+      if (4 * k + 4 > b) v &= 0xFFFFFFFFu >> (8 * (4 * k + 4 - b));  // bytes from b on
+      osum[j] = __builtin_amdgcn_sad_u8(v, 0u, osum[j]);
     }
   }
 }
@@ -736,13 +736,13 @@ __device__ __forceinline__ void major_outlier_sums(const MajorPrefix<S>& mv, con
 // the view's first S columns are read into registers).
 template <int S>
 __device__ __forceinline__ void major_scan_item(const BatchBuffers& bb, const BQuery& q, const BItem& it, uint32_t i,
-                                                uint32_t lane) {
+                                                uint32_t lane, unsigned long long* dep) {
   const MajorPrefix<S> mv = cload(reinterpret_cast<const MajorPrefix<S>*>(bb.majors + it.chunk));
   const bool sem = (q.sparse & kBqSparseSem) != 0;
   unsigned long long* dst = bb.item_acc + (uint64_t)i * (kMaxPlanes + 1);
   if (q.op == kBqGt || q.op == kBqEq) {
-    const uint32_t cnt = q.op == kBqGt ? major_filter_item<0, S>(mv, q, it, lane, sem)
-                                       : major_filter_item<1, S>(mv, q, it, lane, sem);
+    const uint32_t cnt = q.op == kBqGt ? major_filter_item<0, S>(mv, q, it, lane, sem, dep)
+                                       : major_filter_item<1, S>(mv, q, it, lane, sem, dep);
     const uint32_t tot = wave_total_u32(cnt);
     if (lane == 0) dst[0] = tot;
   } else if (q.op == kBqSum) {
@@ -778,6 +778,7 @@ __device__ __forceinline__ void major_scan_item(const BatchBuffers& bb, const BQ
 }
 
 __global__ __launch_bounds__(256) void k_bq_major(const BatchBuffers* __restrict__ bbp) {
+  __shared__ unsigned long long s_dep[4][64];
   const BatchBuffers bb = cload(bbp);
   const WaveSlot ws = wave_slot();
   const uint32_t lane = ws.lane;
@@ -788,13 +789,13 @@ __global__ __launch_bounds__(256) void k_bq_major(const BatchBuffers* __restrict
     const BQuery q = cload(bb.q, it.owner);
     switch (q.sparse & 0xFFu) {
       case 0: break;
-      case 1: major_scan_item<1>(bb, q, it, i, lane); break;
-      case 2: major_scan_item<2>(bb, q, it, i, lane); break;
-      case 3: major_scan_item<3>(bb, q, it, i, lane); break;
-      case 4: major_scan_item<4>(bb, q, it, i, lane); break;
-      case 5: major_scan_item<5>(bb, q, it, i, lane); break;
-      case 6: major_scan_item<6>(bb, q, it, i, lane); break;
-      default: major_scan_item<7>(bb, q, it, i, lane); break;
+      case 1: major_scan_item<1>(bb, q, it, i, lane, s_dep[threadIdx.x >> 6]); break;
+      case 2: major_scan_item<2>(bb, q, it, i, lane, s_dep[threadIdx.x >> 6]); break;
+      case 3: major_scan_item<3>(bb, q, it, i, lane, s_dep[threadIdx.x >> 6]); break;
+      case 4: major_scan_item<4>(bb, q, it, i, lane, s_dep[threadIdx.x >> 6]); break;
+      case 5: major_scan_item<5>(bb, q, it, i, lane, s_dep[threadIdx.x >> 6]); break;
+      case 6: major_scan_item<6>(bb, q, it, i, lane, s_dep[threadIdx.x >> 6]); break;
+      default: major_scan_item<7>(bb, q, it, i, lane, s_dep[threadIdx.x >> 6]); break;
     }
   }
 }

@@ -9,7 +9,11 @@
 //     64c + t holds rows 1024c + 16t .. +15, the 16 rows lane t owns in tile c),
 //   * the outlier bytes in row order, compacted,
 //   * per tile c the number of outliers in tiles before c (prefix[c]), so a wave
-//     finds the position of its lanes' outlier bytes with one exclusive scan.
+//     finds the position of its lanes' outlier bytes with one exclusive scan,
+//   * per outlier its row in its tile (u16, in the order of the bytes): the
+//     filters give outlier k of a tile to lane k, which compares its byte and
+//     ORs the result bits into the owning lane's LDS word (no per-lane walk of
+//     the bitmap).
 // The low F - 8S bits are an ordinary resident chunk (buff_common.h).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -25,6 +29,7 @@ struct SparseDev {
   const uint16_t* mask16;  // ntiles * 64 words
   const uint32_t* prefix;  // ntiles + 1 (prefix[ntiles] = outliers)
   const uint8_t* outl;     // compacted outlier bytes
+  const uint16_t* pos;     // row in its tile (0..1023) of each outlier, in the order of outl
   uint64_t last;           // row of the column's last outlier
   uint32_t major;          // majority byte
   uint32_t shift;          // F - 8(j+1)
@@ -85,6 +90,9 @@ hipError_t launch_major_scan(const MajorEnc& me, uint64_t ntiles, unsigned long 
 // Pass 2: tile prefixes (per workgroup range), then the outlier bytes moved to
 // their final place (grid-stride over tiles).
 hipError_t launch_major_place(const MajorEnc& me, uint64_t ntiles, hipStream_t s);
+// pos of one sparse column from its bitmap and tile prefixes.
+hipError_t launch_major_positions(const uint16_t* mask16, const uint32_t* prefix, uint64_t ntiles, uint16_t* pos,
+                                  hipStream_t s);
 hipError_t launch_major_filter(const MajorView& mv, int op, bool semantic, const MajorTarget& tg, uint64_t* out,
                                uint64_t nwords, hipStream_t s, const LaunchShape& ls);
 // semantic: the true offsets, truncated to their top F - drop bits (drop = 0: every bit).

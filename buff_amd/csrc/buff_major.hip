@@ -235,9 +235,11 @@ __global__ __launch_bounds__(256) void k_major_copy(MajorEnc me, uint64_t ntiles
 // masks / tile prefixes / first dense plane, then the outlier windows, then the
 // candidates' later planes and tail. S (sparse columns) is a template argument so
 // the per-column state stays in registers.
-template <int MODE, int S, int U>
+template <int MODE, int S, int U, bool PF>
 __global__ __launch_bounds__(256) void k_major_filter(MajorView mv, MajorTarget tg, uint32_t sem,
                                                       uint64_t* __restrict__ out, uint64_t nwords) {
+  __shared__ unsigned long long s_dep[4][64];
+  unsigned long long* dep = s_dep[wave_in_wg()];
   const uint32_t lane = __lane_id();
   const uint64_t wave = wave_global();
   const uint64_t nwaves = waves_in_grid();
@@ -248,12 +250,32 @@ __global__ __launch_bounds__(256) void k_major_filter(MajorView mv, MajorTarget 
   for (uint64_t grp = wave; grp < ngroups; grp += nwaves) {
     const uint64_t c0 = grp * U;
     uint32_t res[U];
-    major_filter_group<MODE, S, U>(mv, tg, sem != 0, c0, lane, res);
+    major_filter_group<MODE, S, U, PF>(mv, tg, sem != 0, c0, lane, dep, res);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t c = c0 + u;
       const uint64_t idx = c * 64 + lane;
       if (c < dv.ntiles && idx < nout16) out16[idx] = (uint16_t)res[u];
+    }
+  }
+}
+
+// Row positions of one sparse column: lane t of tile c writes the rows of its
+// outliers (16t + b for the set bits b of its bitmap word) from its exclusive
+// position on.
+__global__ __launch_bounds__(256) void k_major_positions(const uint16_t* __restrict__ mask16,
+                                                         const uint32_t* __restrict__ prefix, uint64_t ntiles,
+                                                         uint16_t* __restrict__ pos) {
+  const uint32_t lane = __lane_id();
+  const uint64_t nwaves = waves_in_grid();
+  for (uint64_t c = wave_global(); c < ntiles; c += nwaves) {
+    uint32_t m = mask16[c * 64 + lane];
+    const uint32_t excl = wave_excl_scan_u32(__popc(m), lane);
+    uint64_t at = (uint64_t)prefix[c] + excl;
+    while (m) {
+      const uint32_t b = (uint32_t)__builtin_ctz(m);
+      m &= m - 1u;
+      pos[at++] = (uint16_t)(16 * lane + b);
     }
   }
 }
@@ -796,14 +818,45 @@ hipError_t launch_major_place(const MajorEnc& me, uint64_t ntiles, hipStream_t s
   return hipGetLastError();
 }
 
+hipError_t launch_major_positions(const uint16_t* mask16, const uint32_t* prefix, uint64_t ntiles, uint16_t* pos,
+                                  hipStream_t s) {
+  if (ntiles == 0) return hipSuccess;
+  launch_k(k_major_positions, dim3(tiles_grid(ntiles, 4096)), dim3(256), 0, s, mask16, prefix, ntiles, pos);
+  return hipGetLastError();
+}
+
+// Workgroups of 256 threads of kernel k that fit on the device at once (0 if
+// unknown): a grid no larger than that runs in one round, with no second round
+// of stragglers.
+template <typename K>
+static uint32_t resident_wgs(K k) {
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) == hipSuccess && hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && per_cu > 0 && cus > 0)
+    return (uint32_t)(per_cu * cus);
+  return 0;
+}
+
+template <int MODE, int S, bool PF>
+static void launch_filter_pf(const MajorView& mv, bool semantic, const MajorTarget& tg, uint64_t* out,
+                             uint64_t nwords, hipStream_t s, uint32_t max_wg) {
+  constexpr int U = 2;
+  const uint64_t groups = (mv.dense.ntiles + U - 1) / U;
+  static const uint32_t fit = resident_wgs(k_major_filter<MODE, S, U, PF>);   // once per kernel
+  const uint32_t grid = tiles_grid(groups, fit ? std::min(max_wg, fit) : max_wg);
+  launch_k((k_major_filter<MODE, S, U, PF>), dim3(grid), dim3(256), 0, s, mv, tg, semantic ? 1u : 0u, out, nwords);
+}
+
 template <int MODE, int S>
 static void launch_filter_s(const MajorView& mv, bool semantic, const MajorTarget& tg, uint64_t* out,
                             uint64_t nwords, hipStream_t s, uint32_t max_wg) {
-  constexpr int U = 2;
-  const uint64_t groups = (mv.dense.ntiles + U - 1) / U;
-  const uint32_t grid = tiles_grid(groups, max_wg);
-  launch_k((k_major_filter<MODE, S, U>), dim3(grid), dim3(256), 0, s, mv, tg, semantic ? 1u : 0u, out,
-                     nwords);
+  // the target's sparse bytes all majority bytes: most rows reach the dense planes
+  bool pf = mv.dense.P >= 2;
+  for (uint32_t j = 0; j < mv.S; ++j) pf = pf && tg.digit[j] == mv.col[j].major;
+  if (pf)
+    launch_filter_pf<MODE, S, true>(mv, semantic, tg, out, nwords, s, max_wg);
+  else
+    launch_filter_pf<MODE, S, false>(mv, semantic, tg, out, nwords, s, max_wg);
 }
 
 template <int MODE>

@@ -64,10 +64,11 @@ int sparse_alloc(Ctx* ctx, Chunk* ch) {
   return BUFF_OK;
 }
 
-// Outlier bytes of every column, each at a 256-byte aligned offset.
+// Outlier bytes and row positions of every column, each at a 256-byte aligned offset.
 int outlier_alloc(Ctx* ctx, Chunk* ch) {
   uint64_t tot = 0;
-  for (uint32_t j = 0; j < ch->S; ++j) tot += round_up(ch->sp[j].n_out + kOutlierPad, 256);
+  for (uint32_t j = 0; j < ch->S; ++j)
+    tot += round_up(ch->sp[j].n_out + kOutlierPad, 256) + round_up(2 * ch->sp[j].n_out + kOutlierPad, 256);
   void* p = nullptr;
   BUFF_TRY(ctx->alloc(std::max<uint64_t>(tot, 256), &p));
   ch->mem_outl = p;
@@ -76,8 +77,20 @@ int outlier_alloc(Ctx* ctx, Chunk* ch) {
   for (uint32_t j = 0; j < ch->S; ++j) {
     ch->sp[j].outl = (uint8_t*)p + off;
     off += round_up(ch->sp[j].n_out + kOutlierPad, 256);
+    ch->sp[j].pos = (uint16_t*)((uint8_t*)p + off);
+    off += round_up(2 * ch->sp[j].n_out + kOutlierPad, 256);
   }
   return BUFF_OK;
+}
+
+// The row positions of every sparse column (after its bitmap and prefixes).
+hipError_t build_positions(Ctx* ctx, Chunk* ch) {
+  for (uint32_t j = 0; j < ch->S; ++j) {
+    const hipError_t e =
+        launch_major_positions(ch->sp[j].mask16, ch->sp[j].prefix, ch->view.ntiles, ch->sp[j].pos, ctx->stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 // Encode scratch in ctx->s_major: staging slots and tile counts of every column,
@@ -155,6 +168,7 @@ MajorView major_view(const Chunk* ch) {
     mv.col[j].mask16 = c.mask16;
     mv.col[j].prefix = c.prefix;
     mv.col[j].outl = c.outl;
+    mv.col[j].pos = c.pos;
     mv.col[j].last = c.last;
     mv.col[j].major = c.major;
     mv.col[j].shift = c.shift;
@@ -291,6 +305,7 @@ int major_encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, u
   {
     ProfScope ps(ctx, "k_major_place");
     e = launch_major_place(me, ch->view.ntiles, ctx->stream);
+    if (e == hipSuccess) e = build_positions(ctx, ch);
   }
   if (e != hipSuccess) {
     free_partial(ctx, ch);
@@ -508,6 +523,7 @@ int major_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_
     if (e == hipSuccess && cols[j].n_out)
       e = hipMemcpyAsync(ch->sp[j].outl, cols[j].outl, cols[j].n_out, hipMemcpyHostToDevice, ctx->stream);
   }
+  if (e == hipSuccess) e = build_positions(ctx, ch);
   if (e == hipSuccess && body) {
     rc = ctx->scratch(ctx->s_stream, body);
     if (rc == BUFF_OK) {

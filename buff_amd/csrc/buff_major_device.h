@@ -133,34 +133,75 @@ __device__ __forceinline__ void load_sparse(const MajorView& mv, uint64_t c, uin
   }
 }
 
+// One sparse column's outliers of tile c against target byte t, deposited in
+// row order. Outlier k of the tile (k = 0..cnt-1, in row order) goes to lane
+// k % 64, which ORs (1 << b) | (byte > t) << (16 + b) | (byte == t) << (32 + b)
+// into dep[L] for its row 16L + b. Lane L then reads its word: the outlier rows
+// of its 16 (bits 0-15), those above t (16-31) and those equal (32-47). The
+// first outlier of each lane arrives preloaded (row0 / byte0, has0); further
+// ones (a tile with more than 64) are loaded here.
+__device__ __forceinline__ uint64_t deposit_outliers(unsigned long long* dep, const SparseDev& sd, uint32_t p0,
+                                                     uint32_t cnt, uint32_t lane, uint32_t row0, uint32_t byte0,
+                                                     bool has0, uint32_t t) {
+  dep[lane] = 0;
+  wave_lds_sync();
+  auto put = [&](uint32_t row, uint32_t byte) {
+    const uint32_t one = 1u << (row & 15u);
+    const uint32_t lo = one | (byte > t ? one << 16 : 0u), hi = byte == t ? one : 0u;
+    atomicOr(&dep[row >> 4], ((unsigned long long)hi << 32) | lo);
+  };
+  if (has0) put(row0, byte0);
+  for (uint32_t k = lane + 64; k < cnt; k += 64) put(sd.pos[p0 + k], sd.outl[p0 + k]);
+  wave_lds_sync();
+  const uint64_t w = dep[lane];
+  wave_lds_sync();   // every lane has read its word before the next column clears it
+  return w;
+}
+
 // Tiles [c0, c0 + U) of a buff-major filter (buff_simd.rs:863-1277 range,
 // 1916-2292 equality; with sem the intended answer on the true offsets):
 // res[u] is the lane's 16-row result mask of tile c0 + u (tiles past the
-// chunk's last read its last tile and are not meant to be stored).
-template <int MODE, int S, int U, typename MV>
+// chunk's last are computed empty and are not meant to be stored). dep: the
+// wave's 64 LDS words. Every load of the first round (tile prefixes, the first
+// dense plane, each lane's first outlier row and byte of every column) is in
+// flight together.
+//
+// PF (the target's sparse bytes are the majority bytes, so most rows stay
+// candidates into the dense planes): the second dense plane and a tail of at
+// most kPfTail bits are loaded in the first round as well, in full, instead of
+// for the candidates' lines in later dependent rounds.
+constexpr uint32_t kPfTail = 2;
+
+template <int MODE, int S, int U, bool PF = false, typename MV>
 __device__ __forceinline__ void major_filter_group(const MV& mv, const MajorTarget& tg, bool sem, uint64_t c0,
-                                                   uint32_t lane, uint32_t (&res)[U]) {
+                                                   uint32_t lane, unsigned long long* dep, uint32_t (&res)[U]) {
   const ChunkView& dv = mv.dense;
   const bool all_outliers = MODE == 0 && !sem;   // the range rules read every outlier byte
-  uint32_t m[U][S], pre[U][S];
-  uint4 w0[U];
+  uint32_t p0[U][S], cnt[U][S], row0[U][S], byte0[U][S];
+  uint4 w0[U], w1[U];
+  uint32_t hp[U][kPfTail];
+  const bool pf_tail = PF && dv.r <= kPfTail;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
+    const bool in = c0 + u < dv.ntiles;
     const uint64_t c = min(c0 + u, dv.ntiles - 1);
 #pragma unroll
     for (int j = 0; j < S; ++j) {
-      m[u][j] = mv.col[j].mask16[c * 64 + lane];
-      pre[u][j] = mv.col[j].prefix[c];
+      p0[u][j] = mv.col[j].prefix[c];
+      cnt[u][j] = in ? mv.col[j].prefix[c + 1] - p0[u][j] : 0u;
     }
     w0[u] = dv.P ? load_plane(dv, 0, c, lane) : make_uint4(0, 0, 0, 0);
+    w1[u] = PF && dv.P > 1 ? load_plane(dv, 1, c, lane) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (uint32_t j = 0; j < kPfTail; ++j) hp[u][j] = pf_tail && j < dv.r ? load_tail(dv, j, c, lane) : 0u;
   }
-  Packed16 pk[U][S];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
 #pragma unroll
     for (int j = 0; j < S; ++j) {
-      const uint32_t excl = wave_excl_scan_u32(__popc(m[u][j]), lane);
-      pk[u][j] = m[u][j] ? load_packed(mv.col[j].outl, (uint64_t)pre[u][j] + excl) : Packed16{0, 0, 0, 0};
+      const bool has = lane < cnt[u][j];
+      row0[u][j] = has ? mv.col[j].pos[p0[u][j] + lane] : 0u;
+      byte0[u][j] = has ? mv.col[j].outl[p0[u][j] + lane] : 0u;
     }
   }
   uint32_t cand[U];
@@ -170,11 +211,16 @@ __device__ __forceinline__ void major_filter_group(const MV& mv, const MajorTarg
     cand[u] = lane_valid16(dv.n, c0 + u, lane);
 #pragma unroll
     for (int j = 0; j < S; ++j) {
-      const uint32_t m16 = m[u][j];
+      if (!all_outliers && !__ballot(cand[u] != 0)) break;   // nothing left for this tile to narrow or add
       const uint32_t major = mv.col[j].major;
       const uint32_t t = tg.digit[j];
-      uint32_t gt, eq;
-      sparse_cmp(pk[u][j], m16, all_outliers ? m16 : (m16 & cand[u]), major, t, gt, eq);
+      const uint64_t w = deposit_outliers(dep, mv.col[j], p0[u][j], cnt[u][j], lane, row0[u][j], byte0[u][j],
+                                          lane < cnt[u][j], t);
+      const uint32_t m16 = (uint32_t)w & 0xFFFFu;
+      const uint32_t nm = ~m16 & 0xFFFFu;
+      const uint32_t want = all_outliers ? m16 : (m16 & cand[u]);
+      const uint32_t gt = (major > t ? nm : 0u) | ((uint32_t)(w >> 16) & want);
+      const uint32_t eq = (major == t ? nm : 0u) | ((uint32_t)(w >> 32) & want);
       if (MODE == 1) {
         cand[u] &= eq;
         continue;
@@ -200,7 +246,9 @@ __device__ __forceinline__ void major_filter_group(const MV& mv, const MajorTarg
     uint4 wp[U];
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      wp[u] = p == 0 ? w0[u] : load_plane(dv, p, min(c0 + u, dv.ntiles - 1), refine_src(cand[u], lane));
+      wp[u] = p == 0             ? w0[u]
+              : (PF && p == 1) ? w1[u]
+                               : load_plane(dv, p, min(c0 + u, dv.ntiles - 1), refine_src(cand[u], lane));
     const Digit dg = make_digit(tg.digit[S + p]);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -214,8 +262,23 @@ __device__ __forceinline__ void major_filter_group(const MV& mv, const MajorTarg
     for (int u = 0; u < U; ++u) live |= cand[u] != 0;
     if (__ballot(live)) {
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (c0 + u < dv.ntiles) refine_tail<MODE>(dv, c0 + u, tg.tail, lane, res[u], cand[u]);
+      for (int u = 0; u < U; ++u) {
+        if (c0 + u >= dv.ntiles) continue;
+        if (pf_tail) {   // the prefetched bit-planes, most significant first
+#pragma unroll
+          for (int j = (int)kPfTail - 1; j >= 0; --j) {
+            if ((uint32_t)j >= dv.r) continue;
+            if ((tg.tail >> j) & 1) {
+              cand[u] &= hp[u][j];
+            } else {
+              if (MODE == 0) res[u] |= cand[u] & hp[u][j];
+              cand[u] &= ~hp[u][j];
+            }
+          }
+        } else {
+          refine_tail<MODE>(dv, c0 + u, tg.tail, lane, res[u], cand[u]);
+        }
+      }
     }
   }
 #pragma unroll

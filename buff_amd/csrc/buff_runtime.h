@@ -115,6 +115,7 @@ struct SparseCol {
   uint16_t* mask16 = nullptr;
   uint32_t* prefix = nullptr;
   uint8_t* outl = nullptr;
+  uint16_t* pos = nullptr;  // row in tile of each outlier (n_out u16)
 };
 
 struct Chunk {
