@@ -741,8 +741,20 @@ def bench_config4(ctx, args, buff_amd):
             ch.max()
     ctx.synchronize()
     single = time.perf_counter() - t
-    res["one_call_per_query"] = {"seconds": round(single, 5), "queries_per_s": round(nq / single, 1)}
-    res["batched_speedup"] = round(single / bt[len(bt) // 2], 2)
+    res["one_call_per_query_python"] = {"seconds": round(single, 5), "queries_per_s": round(nq / single, 1),
+                                        "note": "ctypes calls from a Python loop"}
+    # the same calls from a native loop (buff_trace_per_call, every arrival at 0:
+    # each query is its single-chunk call followed by a stream synchronization)
+    ctx.trace_per_call(batch.chunks, qs[:16], np.zeros(16))   # first launches of every kernel shape
+    ctx.synchronize()
+    t = time.perf_counter()
+    _, lat = ctx.trace_per_call(batch.chunks, qs, np.zeros(len(qs)))
+    native = time.perf_counter() - t
+    res["one_call_per_query"] = {"seconds": round(native, 5), "queries_per_s": round(nq / native, 1),
+                                 "last_completion_s": round(float(lat[-1]) * 1e-6, 5),
+                                 "note": "native loop: one single-chunk call per query, then a stream sync"}
+    res["batched_speedup"] = round(native / bt[len(bt) // 2], 2)
+    res["batched_speedup_vs_python_loop"] = round(single / bt[len(bt) // 2], 2)
     words.free()
     batch.free()
     xd.free()
@@ -1137,6 +1149,10 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 3),
+        "value_is": "logical GB/s: 8 bytes of float64 input per row per operation over the step; hbm_GBps is "
+                    "the step's algorithmic HBM bytes over the same time",
+        "hbm_GBps": round(sum(per_step.values()) / (ms_per_step * 1e-3) / 1e9, 1),
+        "hbm_frac": round(sum(per_step.values()) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -121,6 +121,16 @@ class SplitDouble(ctypes.Structure):
     _fields_ = [("chunksize", ctypes.c_uint64), ("batchsize", ctypes.c_uint64), ("scale", ctypes.c_uint64)]
 
 
+class SegmentDesc(ctypes.Structure):
+    """buff_segment: the fields of Segment<T> (segment.rs:36-44)."""
+    _fields_ = [("method", ctypes.c_int32), ("dictionary_id", ctypes.c_uint32), ("model", ctypes.c_void_p),
+                ("model_len", ctypes.c_uint64), ("timestamp_secs", ctypes.c_uint64),
+                ("timestamp_nanos", ctypes.c_uint32), ("signal", ctypes.c_uint64), ("elem_bytes", ctypes.c_uint32),
+                ("data", ctypes.c_void_p), ("n", ctypes.c_uint64), ("has_time_lapse", ctypes.c_int32),
+                ("lapse", ctypes.c_void_p), ("n_lapse", ctypes.c_uint64), ("has_prev_offset", ctypes.c_int32),
+                ("prev_secs", ctypes.c_uint64), ("prev_nanos", ctypes.c_uint32), ("consumed", ctypes.c_uint64)]
+
+
 P = ctypes.POINTER
 vp = ctypes.c_void_p
 u8p, u32p, u64p, f64p = P(ctypes.c_uint8), P(ctypes.c_uint32), P(ctypes.c_uint64), P(ctypes.c_double)
@@ -218,6 +228,10 @@ SIGNATURES = {
     "buff_sd_buff_equal_filter_majority": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
     "buff_sd_buff_sum_majority": [P(SplitDouble), u8p, u64, f64p],
     "buff_sd_buff_max_majority": [P(SplitDouble), u8p, u64, f64p],
+    "buff_segment_byte_size": [P(SegmentDesc), u64p],
+    "buff_segment_serialize": [P(SegmentDesc), u8p, u64, u64p],
+    "buff_segment_parse": [u8p, u64, u32, P(SegmentDesc)],
+    "buff_segment_encode": [vp, u8p, u64, u32, u64, u32, P(vp)],
 }
 
 _lib = None
@@ -627,6 +641,14 @@ class Context:
         check(lib().buff_encode_device(self.handle, x_dev, n, scale, flags, ctypes.byref(h)))
         return Chunk(self, h.value)
 
+    def encode_segment(self, seg_bytes, scale, flags=0, elem_bytes=8):
+        """byte_fixed_encode on a serialized Segment<T> (buff_segment_encode)."""
+        b = np.frombuffer(bytes(seg_bytes), dtype=np.uint8)
+        h = ctypes.c_void_p()
+        check(lib().buff_segment_encode(self.handle, ptr(b, ctypes.c_uint8) if len(b) else None, len(b), elem_bytes,
+                                        scale, flags, ctypes.byref(h)))
+        return Chunk(self, h.value)
+
     def csv_parse(self, text, delim=b",", skip_val=0):
         """construct_file_iterator_skip_newline::<f64> over `text` (bytes): the values."""
         t = np.frombuffer(bytes(text), dtype=np.uint8)
@@ -704,3 +726,81 @@ class Context:
         if self.handle:
             check(lib().buff_ctx_destroy(self.handle))
             self.handle = None
+
+
+def _segment_desc(seg, keep):
+    """A SegmentDesc over the fields of `seg` (a buff_amd.segment.Segment); `keep`
+    collects the buffers the descriptor points into."""
+    d = SegmentDesc()
+    d.method = seg.method
+    d.dictionary_id = seg.dictionary_id
+    model = bytes(seg.model)
+    mb = ctypes.create_string_buffer(model, max(len(model), 1))
+    keep.append(mb)
+    d.model = ctypes.cast(mb, ctypes.c_void_p) if model else None
+    d.model_len = len(model)
+    d.timestamp_secs, d.timestamp_nanos = seg.timestamp
+    d.signal = seg.signal
+    d.elem_bytes = seg.elem_bytes
+    arr = np.ascontiguousarray(seg.data, dtype="<f8" if seg.elem_bytes == 8 else "<f4")
+    keep.append(arr)
+    d.data = arr.ctypes.data if len(arr) else None
+    d.n = len(arr)
+    if seg.time_lapse is not None:
+        rec = np.zeros(len(seg.time_lapse), dtype=[("secs", "<u8"), ("nanos", "<u4")])
+        for i, (sec, ns) in enumerate(seg.time_lapse):
+            rec[i] = (sec, ns)
+        keep.append(rec)
+        d.has_time_lapse = 1
+        d.lapse = rec.ctypes.data if len(rec) else None
+        d.n_lapse = len(rec)
+    if seg.prev_seg_offset is not None:
+        d.has_prev_offset = 1
+        d.prev_secs, d.prev_nanos = seg.prev_seg_offset
+    return d
+
+
+def segment_serialize(seg):
+    """Segment::convert_to_bytes (buff_segment_serialize)."""
+    keep = []
+    d = _segment_desc(seg, keep)
+    n = ctypes.c_uint64()
+    check(lib().buff_segment_serialize(ctypes.byref(d), None, 0, ctypes.byref(n)))
+    out = np.zeros(max(n.value, 1), dtype=np.uint8)
+    check(lib().buff_segment_serialize(ctypes.byref(d), ptr(out, ctypes.c_uint8), n.value, ctypes.byref(n)))
+    return bytes(out[: n.value])
+
+
+def segment_byte_size(seg):
+    """Segment::get_byte_size (buff_segment_byte_size)."""
+    keep = []
+    d = _segment_desc(seg, keep)
+    n = ctypes.c_uint64()
+    check(lib().buff_segment_byte_size(ctypes.byref(d), ctypes.byref(n)))
+    return n.value
+
+
+def segment_parse(b, elem_bytes=8):
+    """Segment::convert_from_bytes (buff_segment_parse): the fields as a dict."""
+    buf = np.frombuffer(bytes(b), dtype=np.uint8)
+    d = SegmentDesc()
+    check(lib().buff_segment_parse(ptr(buf, ctypes.c_uint8) if len(buf) else None, len(buf), elem_bytes,
+                                   ctypes.byref(d)))
+    base = buf.ctypes.data
+
+    def at(p, k):
+        return bytes(buf[p - base: p - base + k]) if k else b""
+
+    dt = np.dtype("<f8" if elem_bytes == 8 else "<f4")
+    data = np.frombuffer(at(d.data, d.n * elem_bytes), dtype=dt).copy() if d.n else np.zeros(0, dt)
+    lapse = None
+    if d.has_time_lapse:
+        raw = at(d.lapse, 12 * d.n_lapse)
+        lapse = [(int.from_bytes(raw[12 * i: 12 * i + 8], "little"), int.from_bytes(raw[12 * i + 8: 12 * i + 12],
+                                                                                    "little"))
+                 for i in range(d.n_lapse)]
+    return {"method": d.method, "dictionary_id": d.dictionary_id,
+            "model": at(d.model, d.model_len) if d.model_len else b"",
+            "ts_secs": d.timestamp_secs, "ts_nanos": d.timestamp_nanos, "signal": d.signal, "data": data,
+            "time_lapse": lapse, "prev_offset": (d.prev_secs, d.prev_nanos) if d.has_prev_offset else None,
+            "consumed": d.consumed}

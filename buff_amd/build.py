@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbuffgpu.so")
 CLI = os.path.join(HERE, "comp_profiler")
 SOURCES = ["buff_kernels.hip", "buff_major.hip", "buff_batch.hip", "buff_cond.hip", "buff_wmax.hip", "buff_api.hip",
-           "buff_major_api.hip", "buff_batch_api.hip", "buff_slice_api.hip", "buff_csv.hip", "buff_noalloc.hip"]
+           "buff_major_api.hip", "buff_batch_api.hip", "buff_slice_api.hip", "buff_csv.hip", "buff_noalloc.hip", "buff_segment.hip"]
 ARCH = os.environ.get("BUFF_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -547,6 +547,46 @@ int buff_sd_buff_slice_buffmax(const buff_slice_compress* sc, const uint8_t* byt
 int buff_sd_buff_slice_max(const buff_slice_compress* sc, const uint8_t* bytes, uint64_t len, double* max_value,
                            uint64_t* argmax_count);
 
+/* ---- the Segment container ---------------------------------------------------
+ * Segment<T> (segment.rs:36-44) as bincode 1.x writes it (segment.rs:84-101):
+ * little-endian, u64 lengths, u32 enum variants, u8 Option tags, fields in
+ * declaration order. With method, time_lapse and prev_seg_offset None a
+ * Segment<f64> is 31 + 8n bytes. Host code only (buff_segment_encode aside). */
+typedef struct {
+  int32_t method;             /* -1 None; Methods (methods/mod.rs:40-45): 0 Fourier, 1 Kernel(id),
+                                 2 SparseLearning(id), 3 DeepLearning(model) */
+  uint32_t dictionary_id;     /* Kernel / SparseLearning */
+  const char* model;          /* DeepLearning: model_len UTF-8 bytes */
+  uint64_t model_len;
+  uint64_t timestamp_secs;    /* timestamp: SystemTime since UNIX_EPOCH */
+  uint32_t timestamp_nanos;
+  uint64_t signal;            /* SignalId */
+  uint32_t elem_bytes;        /* T: 8 = f64, 4 = f32 */
+  const void* data;           /* data: n values of T, little-endian */
+  uint64_t n;
+  int32_t has_time_lapse;     /* time_lapse: Option<Vec<Duration>> */
+  const uint8_t* lapse;       /* n_lapse records of 12 bytes: u64 secs, u32 nanos */
+  uint64_t n_lapse;
+  int32_t has_prev_offset;    /* prev_seg_offset: Option<Duration> */
+  uint64_t prev_secs;
+  uint32_t prev_nanos;
+  uint64_t consumed;          /* buff_segment_parse: bytes the Segment took (trailing bytes are allowed) */
+} buff_segment;
+/* Segment::get_byte_size (segment.rs:90-95). */
+int buff_segment_byte_size(const buff_segment* seg, uint64_t* size);
+/* Segment::convert_to_bytes (segment.rs:84-89). dst == NULL: only *len is set;
+ * BUFF_E_SMALL_BUFFER (with *len) when cap is short. */
+int buff_segment_serialize(const buff_segment* seg, uint8_t* dst, uint64_t cap, uint64_t* len);
+/* Segment::convert_from_bytes (segment.rs:96-101): BUFF_E_INVALID where it
+ * returns Err (early end, an Option tag above 1, a Methods variant above 3, a
+ * model name that is not UTF-8, a time whose seconds overflow). The pointers
+ * (model, data, lapse) point into `bytes`. */
+int buff_segment_parse(const uint8_t* bytes, uint64_t len, uint32_t elem_bytes, buff_segment* seg);
+/* byte_fixed_encode (split_double.rs:298) on a serialized Segment<T>: its data,
+ * as float64, encoded into the resident form (flags as buff_encode). */
+int buff_segment_encode(buff_ctx* ctx, const uint8_t* bytes, uint64_t len, uint32_t elem_bytes, uint64_t scale,
+                        uint32_t flags, buff_chunk** out);
+
 #ifdef __cplusplus
 }
 #endif
