@@ -35,6 +35,17 @@ def outlier_heavy(n, decimals, seed, frac=0.05):
     return np.round(x, decimals)
 
 
+def positive_spikes(n, decimals, seed, frac=0.05):
+    """95% N(100, 0.5) and `frac` of rows 100 + 10^U[3,7]: offsets from the
+    minimum are small except the spikes', so the top byte columns are mostly
+    zero (the residue format's outlier columns)."""
+    rng = np.random.default_rng(seed)
+    x = rng.normal(100.0, 0.5, n)
+    out = rng.random(n) < frac
+    x[out] = 100.0 + 10.0 ** rng.uniform(3.0, 7.0, int(out.sum()))
+    return np.round(x, decimals)
+
+
 def percentile_pred(x, frac_above):
     """A predicate with about `frac_above` of the rows strictly above it."""
     q = np.quantile(x, 1.0 - frac_above, method="lower")

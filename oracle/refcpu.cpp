@@ -2365,6 +2365,7 @@ double buff_max_majority(const uint8_t* bytes, size_t blen, uint64_t scale, uint
 }
 
 #include "refslice.inc"
+#include "refrapg.inc"
 
 }  // namespace
 
@@ -2671,6 +2672,59 @@ int ref_slice_filter(const uint8_t* s, uint64_t len, uint64_t scale, double pred
   try {
     return copy_filter(buff_slice_filter(s, len, scale, pred, kind, rows, nrows), words, nwords_cap, count,
                        nullptr);
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+
+// RAPG residue format (split_double.rs:624-954, 1353-1893, 2210-2629). precision
+// < 0: the full-width reads. Call ref_residue_encode with out==NULL for the size.
+int ref_residue_encode(const double* x, uint64_t n, uint64_t scale, uint8_t* out, uint64_t cap, uint64_t* out_len,
+                       uint32_t* cols) {
+  try {
+    std::vector<uint8_t> s = byte_residue_encode(x, n, scale, cols);
+    *out_len = s.size();
+    if (!out) return REF_OK;
+    if (s.size() > cap) return REF_SMALL_BUFFER;
+    std::memcpy(out, s.data(), s.size());
+    return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+int ref_residue_decode(const uint8_t* s, uint64_t len, int32_t precision, double* out, uint64_t cap, uint64_t* n_out) {
+  try {
+    std::vector<double> v = byte_residue_decode(s, len, precision);
+    *n_out = v.size();
+    if (v.size() > cap) return REF_SMALL_BUFFER;
+    std::memcpy(out, v.data(), v.size() * 8);
+    return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+int ref_residue_sum(const uint8_t* s, uint64_t len, int32_t precision, double* sum, uint64_t* plane_sums,
+                    uint32_t* cols) {
+  try {
+    if (plane_sums) std::memset(plane_sums, 0, 9 * 8);
+    *sum = byte_residue_sum(s, len, precision, plane_sums, cols);
+    return REF_OK;
+  } catch (const RefPanic&) { return REF_PANIC; }
+}
+
+int ref_roaring_serialize(const uint32_t* vals, uint64_t n, uint8_t* out, uint64_t cap, uint64_t* out_len) {
+  std::vector<uint8_t> s = roaring_ref::serialize(std::vector<uint32_t>(vals, vals + n));
+  *out_len = s.size();
+  if (!out) return REF_OK;
+  if (s.size() > cap) return REF_SMALL_BUFFER;
+  std::memcpy(out, s.data(), s.size());
+  return REF_OK;
+}
+
+int ref_roaring_deserialize(const uint8_t* p, uint64_t n, uint32_t* vals, uint64_t cap, uint64_t* n_out) {
+  try {
+    std::vector<uint32_t> v = roaring_ref::deserialize(p, n);
+    *n_out = v.size();
+    if (v.size() > cap) return REF_SMALL_BUFFER;
+    if (!v.empty()) std::memcpy(vals, v.data(), v.size() * 4);
+    return REF_OK;
   } catch (const RefPanic&) { return REF_PANIC; }
 }
 

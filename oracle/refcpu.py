@@ -99,6 +99,12 @@ def lib():
                                                 u64p, ctypes.c_uint64, u32p, ctypes.c_uint64, u64p]),
             "ref_snappy_compress": (ctypes.c_int, [u8p, ctypes.c_uint64, u8p, ctypes.c_uint64, u64p]),
             "ref_snappy_decompress": (ctypes.c_int, [u8p, ctypes.c_uint64, u8p, ctypes.c_uint64, u64p]),
+            "ref_residue_encode": (ctypes.c_int, [f64p, ctypes.c_uint64, ctypes.c_uint64, u8p, ctypes.c_uint64, u64p,
+                                                  u32p]),
+            "ref_residue_decode": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_int32, f64p, ctypes.c_uint64, u64p]),
+            "ref_residue_sum": (ctypes.c_int, [u8p, ctypes.c_uint64, ctypes.c_int32, f64p, u64p, u32p]),
+            "ref_roaring_serialize": (ctypes.c_int, [u32p, ctypes.c_uint64, u8p, ctypes.c_uint64, u64p]),
+            "ref_roaring_deserialize": (ctypes.c_int, [u8p, ctypes.c_uint64, u32p, ctypes.c_uint64, u64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -309,6 +315,66 @@ def sum_with_precision(stream, scale, precision):
     _check(lib().ref_sum_with_precision(_ptr(s, ctypes.c_uint8), len(s), scale, precision, ctypes.byref(out),
                                         _ptr(ps, ctypes.c_uint64)))
     return out.value, ps
+
+
+# --- RAPG residue format (split_double.rs:624-954, 1353-1893, 2210-2629) ------
+
+def residue_encode(x, scale):
+    """byte_residue_encode -> (uint8 stream, outlier columns)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = ctypes.c_uint64()
+    cols = ctypes.c_uint32()
+    L = lib()
+    _check(L.ref_residue_encode(_ptr(x, ctypes.c_double), len(x), scale, None, 0, ctypes.byref(n), ctypes.byref(cols)))
+    out = np.zeros(max(n.value, 1), dtype=np.uint8)
+    _check(L.ref_residue_encode(_ptr(x, ctypes.c_double), len(x), scale, _ptr(out, ctypes.c_uint8), n.value,
+                                ctypes.byref(n), ctypes.byref(cols)))
+    return out[: n.value], cols.value
+
+
+def residue_decode(stream, precision=-1):
+    """byte_residue_decode (precision < 0) / byte_residue_decode_with_precision -> float64 values."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    n = header(s)[1]
+    out = np.zeros(max(n, 1), dtype=np.float64)
+    got = ctypes.c_uint64()
+    _check(lib().ref_residue_decode(_ptr(s, ctypes.c_uint8), len(s), precision, _ptr(out, ctypes.c_double), len(out),
+                                    ctypes.byref(got)))
+    return out[: got.value]
+
+
+def residue_sum(stream, precision=-1):
+    """byte_residue_sum (precision < 0) / byte_residue_sum_with_precision -> (f64 sum, u64 sums of the
+    columns read in order (outlier columns: their outlier bytes), outlier columns)."""
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    out = ctypes.c_double()
+    ps = np.zeros(9, dtype=np.uint64)
+    cols = ctypes.c_uint32()
+    _check(lib().ref_residue_sum(_ptr(s, ctypes.c_uint8), len(s), precision, ctypes.byref(out),
+                                 _ptr(ps, ctypes.c_uint64), ctypes.byref(cols)))
+    return out.value, ps, cols.value
+
+
+def roaring_serialize(values):
+    """croaring Bitmap of `values` (ascending), run_optimize(), serialize() -> bytes."""
+    v = np.ascontiguousarray(values, dtype=np.uint32)
+    n = ctypes.c_uint64()
+    L = lib()
+    L.ref_roaring_serialize(_ptr(v, ctypes.c_uint32), len(v), None, 0, ctypes.byref(n))
+    out = np.zeros(max(n.value, 1), dtype=np.uint8)
+    _check(L.ref_roaring_serialize(_ptr(v, ctypes.c_uint32), len(v), _ptr(out, ctypes.c_uint8), n.value,
+                                   ctypes.byref(n)))
+    return bytes(out[: n.value])
+
+
+def roaring_deserialize(b, cap=1 << 22):
+    """Bitmap::deserialize -> ascending uint32 values."""
+    s = np.frombuffer(bytes(b), dtype=np.uint8)
+    out = np.zeros(cap, dtype=np.uint32)
+    n = ctypes.c_uint64()
+    _check(lib().ref_roaring_deserialize(_ptr(s, ctypes.c_uint8) if len(s) else None, len(s),
+                                         _ptr(out, ctypes.c_uint32), cap, ctypes.byref(n)))
+    return out[: n.value]
 
 
 # --- buff-major (buff_simd.rs:317-2861) -------------------------------------

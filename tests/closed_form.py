@@ -750,3 +750,171 @@ def slice_filter_bits(stream, scale, kind, pred, cond=None, semantic=False):
     if not nosimd:
         res[32 * (n // 32):] = False
     return res
+
+
+# --- RAPG residue format (split_double.rs:624-954, 1353-1893, 2210-2629) -------
+# Written from the format, independently of oracle/refrapg.inc.
+
+def roaring_values(b):
+    """The values of a Roaring portable serialization (ascending)."""
+    b = bytes(b)
+    at = 0
+
+    def u16():
+        nonlocal at
+        if at + 2 > len(b):
+            raise ValueError("truncated")
+        at += 2
+        return int.from_bytes(b[at - 2: at], "little")
+
+    def u32():
+        nonlocal at
+        if at + 4 > len(b):
+            raise ValueError("truncated")
+        at += 4
+        return int.from_bytes(b[at - 4: at], "little")
+
+    cookie = u32()
+    if cookie & 0xFFFF == 12347:
+        size = (cookie >> 16) + 1
+        nflag = (size + 7) // 8
+        flags = b[at: at + nflag]
+        at += nflag
+        run = [bool((flags[i // 8] >> (i % 8)) & 1) for i in range(size)]
+    elif cookie == 12346:
+        size = u32()
+        run = [False] * size
+    else:
+        raise ValueError("bad cookie")
+    heads = [(u16(), u16() + 1) for _ in range(size)]
+    if not any(run) or size >= 4:
+        at += 4 * size
+    out = []
+    for (key, card), r in zip(heads, run):
+        hi = key << 16
+        if r:
+            for _ in range(u16()):
+                s, l = u16(), u16()
+                out.extend(hi | v for v in range(s, s + l + 1))
+        elif card > 4096:
+            words = np.frombuffer(b[at: at + 8192], dtype="<u8")
+            at += 8192
+            bits = np.unpackbits(words.view(np.uint8), bitorder="little")
+            out.extend(hi | int(v) for v in np.flatnonzero(bits))
+        else:
+            out.extend(hi | u16() for _ in range(card))
+    return out
+
+
+def rapg_parse(stream):
+    """Fields of a byte_residue_encode stream: header, outlier columns (rows,
+    bytes, bitmap bytes), the byte after them, the dense planes and tail, and
+    the offsets they add up to."""
+    s = bytes(stream)
+    base = int.from_bytes(s[0:8], "little", signed=True)
+    n, ilen, dlen = (int.from_bytes(s[8 + 4 * i: 12 + 4 * i], "little") for i in range(3))
+    F = (ilen + dlen) & 0xFFFFFFFF
+    h = {"base": base, "n": n, "ilen": ilen, "dlen": dlen, "F": F, "cols": [], "S": 0}
+    pos = 20
+    if ilen > 15:
+        S = s[20]
+        h["S"] = S
+        pos = 21
+        for _ in range(S):
+            bl = int.from_bytes(s[pos: pos + 4], "little")
+            bm = s[pos + 4: pos + 4 + bl]
+            pos += 4 + bl
+            no = int.from_bytes(s[pos: pos + 4], "little")
+            ob = np.frombuffer(s[pos + 4: pos + 4 + no], dtype=np.uint8)
+            pos += 4 + no
+            h["cols"].append({"bitmap": bm, "rows": np.array(roaring_values(bm), dtype=np.int64), "bytes": ob})
+        if S:
+            h["pad"] = s[pos]
+            pos += 1
+    h["planes_at"] = pos
+    rem = F - 8 * h["S"]
+    P, r = rem // 8, rem % 8
+    h["P"], h["r"] = P, r
+    h["planes"] = [np.frombuffer(s[pos + k * n: pos + (k + 1) * n], dtype=np.uint8) for k in range(P)]
+    tail_at = pos + P * n
+    h["tail_at"] = tail_at
+    off = np.zeros(n, dtype=np.uint64)
+    for j, c in enumerate(h["cols"]):
+        col = np.zeros(n, dtype=np.uint64)
+        col[c["rows"]] = c["bytes"]
+        off |= col << np.uint64(F - 8 * (j + 1))
+    for k, pl in enumerate(h["planes"]):
+        off |= pl.astype(np.uint64) << np.uint64(rem - 8 * (k + 1))
+    if r:
+        bits = np.unpackbits(np.frombuffer(s[tail_at:], dtype=np.uint8), bitorder="little")
+        idx = np.arange(n)[:, None] * r + np.arange(r)[None, :]
+        off |= (bits[idx].astype(np.uint64) << np.arange(r, dtype=np.uint64)).sum(axis=1).astype(np.uint64)
+    h["off"] = off
+    h["stream_len"] = len(s)
+    return h
+
+
+def rapg_fields(stream, start, n, q):
+    """n LSB-first q-bit fields of the stream from byte `start` on."""
+    bits = np.unpackbits(np.frombuffer(bytes(stream)[start:], dtype=np.uint8), bitorder="little")
+    if n * q > len(bits):
+        raise Panic("fields run past the stream")
+    idx = np.arange(n)[:, None] * q + np.arange(q)[None, :]
+    return (bits[idx].astype(np.uint64) << np.arange(q, dtype=np.uint64)).sum(axis=1).astype(np.uint64)
+
+
+def rapg_read(stream, precision=-1):
+    """What byte_residue_decode[_with_precision] / byte_residue_sum[_with_precision]
+    read: R bits (every bit for precision < 0; else the read rule), the outlier
+    columns at R - 8(j+1), then whole planes, then R - 8S mod 8 = q-bit fields
+    from the byte after the last plane read. Returns (h, R, read values as u64
+    offsets, column u64 sums in read order, q)."""
+    h = rapg_parse(stream)
+    R = h["F"] if precision < 0 else precision_R(h, precision)
+    S, n = h["S"], h["n"]
+    rem = R - 8 * S
+    k, q = rem // 8, rem % 8
+    if k > h["P"]:
+        raise Panic("planes run past the stream")
+    v = np.zeros(n, dtype=np.uint64)
+    sums = []
+    for j, c in enumerate(h["cols"]):
+        col = np.zeros(n, dtype=np.uint64)
+        col[c["rows"]] = c["bytes"]
+        v |= col << np.uint64(R - 8 * (j + 1))
+        sums.append(int(c["bytes"].astype(np.uint64).sum()))
+    for p in range(k):
+        v |= h["planes"][p].astype(np.uint64) << np.uint64(rem - 8 * (p + 1))
+        sums.append(int(h["planes"][p].astype(np.uint64).sum()))
+    if q:
+        f = rapg_fields(stream, h["planes_at"] + k * n, n, q)
+        v |= f
+        sums.append(int(f.sum()))
+    return h, R, v, sums, q
+
+
+def rapg_decode(stream, precision=-1):
+    """byte_residue_decode[_with_precision] from the read model: (base + v) / 2^dlen
+    (wrapping i64), nothing when the outlier columns take every bit read."""
+    h, R, v, _, q = rapg_read(stream, precision)
+    if h["S"] and R - 8 * h["S"] == 0:
+        return np.zeros(0)
+    f = (v + np.uint64(h["base"] & 0xFFFFFFFFFFFFFFFF)).view(np.int64)
+    return f.astype(np.float64) / 2.0 ** h["dlen"]
+
+
+def rapg_sum(stream, precision=-1):
+    """byte_residue_sum[_with_precision] from the read model: (column sums in read
+    order, exact value as a Fraction, whether the base term is added). Unlike the
+    decode, the sum scales the R-bit read by 2^(ilen - R)."""
+    from fractions import Fraction
+
+    h, R, v, sums, q = rapg_read(stream, precision)
+    rem = R - 8 * h["S"]
+    with_base = not (rem >= 8 and rem % 8 == 0)   # the plane loop returns when the width ends on a byte
+    # byte k of the read weighs 2^(ilen - bits read through it): the R-bit value v weighs 2^(ilen - R)
+    sv = int(v.astype(object).sum()) if len(v) else 0
+    tot = Fraction(sv) * Fraction(2) ** (h["ilen"] - R) if sv else Fraction(0)
+    if with_base:
+        tot += Fraction(h["n"] * h["base"], 2 ** h["dlen"])
+    return sums, tot, with_base

@@ -59,6 +59,25 @@ MUTATIONS = [
 ]
 
 
+# mistakes planted in the RAPG restatement (oracle/refrapg.inc), pinned by tests/test_rapg_oracle.py
+RAPG_MUTATIONS = [
+    # CRoaring's run_optimize keeps an array whose run form is the same size (2 + 4 runs == 2 card + 2)
+    ("roaring_run_ties_become_runs", "if ((c.type == 0 && as_run < 2 * card + 2)",
+     "if ((c.type == 0 && as_run <= 2 * card + 2)", 0),
+    # the offset table is also written with runs once there are four or more containers
+    ("roaring_no_offsets_with_runs", "if (!hasrun || size >= kNoOffsetThreshold) {\n    for (auto& c : cs) {",
+     "if (!hasrun) {\n    for (auto& c : cs) {", 0),
+    # byte_residue_encode's min / max is `if` / `if` (split_double.rs:642-651), not byte_fixed's `else if`
+    ("residue_encode_else_if", "    if (f > mx) mx = f;\n    fixed_vec.push_back(f);",
+     "    else if (f > mx) mx = f;\n    fixed_vec.push_back(f);", 0),
+    # the sum's early return on a byte boundary (split_double.rs:2271-2273) given the base term
+    ("residue_sum_return_adds_base", "if (returned) return s;", "if (false) return s;", 0),
+    # an outlier column's byte ORed one byte too high (split_double.rs:1411)
+    ("residue_decode_outlier_shift", "fixed_vec[i] |= (uint64_t)outlier_value << (remain & 63);",
+     "fixed_vec[i] |= (uint64_t)outlier_value << ((remain + 8) & 63);", 0),
+]
+
+
 def _mutate(src, old, new, occ):
     pos = -1
     for _ in range(occ + 1):
@@ -68,14 +87,21 @@ def _mutate(src, old, new, occ):
     return src[:pos] + new + src[pos + len(old):]
 
 
-def _build_and_pin(tmp, name, old, new, occ):
-    with open(SRC) as f:
-        src = f.read()
+def _build_and_pin(tmp, name, old, new, occ, target=SRC, pins="test_oracle_pins.py"):
     d = os.path.join(tmp, name)
     os.makedirs(d)
     cpp = os.path.join(d, "refcpu.cpp")
+    with open(SRC) as f:
+        src = f.read()
+    if target == SRC:
+        src = _mutate(src, old, new, occ)
+    else:   # an included file: the mutated copy sits beside the copy that includes it
+        with open(target) as f:
+            inc = f.read()
+        with open(os.path.join(d, os.path.basename(target)), "w") as f:
+            f.write(_mutate(inc, old, new, occ))
     with open(cpp, "w") as f:
-        f.write(_mutate(src, old, new, occ))
+        f.write(src)
     # the mutated copy includes the oracle's other sources from their own directory
     so = os.path.join(d, "librefcpu.so")
     b = subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-fno-fast-math", "-ffp-contract=off",
@@ -83,16 +109,21 @@ def _build_and_pin(tmp, name, old, new, occ):
     if b.returncode != 0:
         return name, "build failed: " + b.stderr[-2000:]
     env = dict(os.environ, BUFF_ORACLE_LIB=so)
-    r = subprocess.run([sys.executable, "-m", "pytest", os.path.join(ROOT, "tests", "test_oracle_pins.py"), "-x",
+    r = subprocess.run([sys.executable, "-m", "pytest", os.path.join(ROOT, "tests", pins), "-x",
                         "-q", "-p", "no:cacheprovider", "-m", "not gpu"], cwd=ROOT, env=env, capture_output=True,
                        text=True, timeout=600)
     return name, (r.returncode, r.stdout[-3000:])
 
 
+RAPG_SRC = os.path.join(ROOT, "oracle", "refrapg.inc")
+
+
 def test_mutations_are_caught():
+    jobs = [m + (SRC, "test_oracle_pins.py") for m in MUTATIONS] + \
+           [m + (RAPG_SRC, "test_rapg_oracle.py") for m in RAPG_MUTATIONS]
     with tempfile.TemporaryDirectory() as tmp:
         with ThreadPoolExecutor(4) as ex:
-            results = list(ex.map(lambda m: _build_and_pin(tmp, *m), MUTATIONS))
+            results = list(ex.map(lambda m: _build_and_pin(tmp, *m), jobs))
     survivors = []
     for name, res in results:
         assert not isinstance(res, str), (name, res)
@@ -103,7 +134,8 @@ def test_mutations_are_caught():
     assert not survivors, survivors
 
 
-@pytest.mark.parametrize("name,old,new,occ", MUTATIONS)
-def test_mutation_anchor_present(name, old, new, occ):
-    with open(SRC) as f:
+@pytest.mark.parametrize("name,old,new,occ,src", [m + (SRC,) for m in MUTATIONS] +
+                         [m + (RAPG_SRC,) for m in RAPG_MUTATIONS])
+def test_mutation_anchor_present(name, old, new, occ, src):
+    with open(src) as f:
         _mutate(f.read(), old, new, occ)
