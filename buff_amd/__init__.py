@@ -14,6 +14,7 @@ from ._native import (  # noqa: F401
     BUFF_SEMANTIC,
     BUFF_SLICE,
     BUFF_WITH_SLICE,
+    BUFF_RESIDUE,
     BuffError,
     Batch,
     Chunk,
@@ -30,6 +31,6 @@ from .split_double import SplitBDDoubleCompress  # noqa: F401
 
 __all__ = [
     "BUFF_FLIP", "BUFF_MAJORITY", "BUFF_OP_EQ", "BUFF_OP_GT", "BUFF_OP_LT", "BUFF_SCALAR", "BUFF_SEMANTIC",
-    "BUFF_SLICE", "BUFF_WITH_SLICE", "Batch", "BuffError", "Chunk", "Context", "DeviceBuffer",
+    "BUFF_SLICE", "BUFF_WITH_SLICE", "BUFF_RESIDUE", "Batch", "BuffError", "Chunk", "Context", "DeviceBuffer",
     "OutOfMemory", "RefPanic", "Server", "Unsupported", "SplitBDDoubleCompress", "BuffSliceCompress", "lib",
 ]

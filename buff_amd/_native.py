@@ -26,6 +26,7 @@ BUFF_MAJORITY = 0x4
 BUFF_SLICE = 0x8
 BUFF_SCALAR = 0x10
 BUFF_WITH_SLICE = 0x20
+BUFF_RESIDUE = 0x40
 BUFF_OP_GT = 0
 BUFF_OP_EQ = 1
 BUFF_OP_LT = 2
@@ -220,7 +221,10 @@ SIGNATURES = {
     "buff_sd_buff_simd_equal_filter": [P(SplitDouble), u8p, u64, f64, u32p, u64, u64p],
     "buff_sd_byte_fixed_sum": [P(SplitDouble), u8p, u64, f64p],
     "buff_sd_byte_fixed_max": [P(SplitDouble), u8p, u64, f64p],
-    "buff_sd_byte_residue_decode_with_precision": [P(SplitDouble), u8p, u64, u32, f64p, u64],
+    "buff_sd_byte_residue_decode_with_precision": [P(SplitDouble), u8p, u64, u32, f64p, u64, u64p],
+    "buff_sd_byte_residue_encode": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
+    "buff_sd_byte_residue_decode": [P(SplitDouble), u8p, u64, f64p, u64, u64p],
+    "buff_sd_byte_residue_sum": [P(SplitDouble), u8p, u64, f64p],
     "buff_sd_byte_residue_sum_with_precision": [P(SplitDouble), u8p, u64, u32, f64p],
     "buff_sd_buff_encode_majority": [P(SplitDouble), f64p, u64, u8p, u64, u64p],
     "buff_sd_buff_decode_majority": [P(SplitDouble), u8p, u64, f64p, u64, u64p],

@@ -325,7 +325,9 @@ int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_
   const bool slice = (flags & BUFF_SLICE) != 0;
   if (slice && (flags & (BUFF_FLIP | BUFF_MAJORITY)))
     return fail(BUFF_E_INVALID, "BUFF_SLICE excludes BUFF_FLIP and BUFF_MAJORITY");
-  if (flags & BUFF_MAJORITY) return major_encode_device(ctx, x, n, scale, flags, out);
+  if ((flags & BUFF_RESIDUE) && (flags & (BUFF_FLIP | BUFF_MAJORITY | BUFF_SLICE)))
+    return fail(BUFF_E_INVALID, "BUFF_RESIDUE excludes BUFF_FLIP, BUFF_MAJORITY and BUFF_SLICE");
+  if (flags & (BUFF_MAJORITY | BUFF_RESIDUE)) return major_encode_device(ctx, x, n, scale, flags, out);
   if (n > 0xFFFFFFFFull) return fail(BUFF_E_INVALID, "a chunk holds at most 2^32-1 rows (n is stored as u32)");
   if (n && !x) return fail(BUFF_E_INVALID, "null input");
   const int prec = scale == 0 ? 0 : prec_of_scale(scale);
@@ -377,6 +379,7 @@ int encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_
 
 int chunk_serialize(Ctx* ctx, const Chunk* ch, uint8_t* dst, uint64_t* len) {
   if (!ctx || !ch || !len) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->flags & BUFF_RESIDUE) return residue_serialize(ctx, ch, dst, len);
   if (ch->flags & BUFF_MAJORITY) return major_serialize(ctx, ch, dst, len);
   if (ch->flags & BUFF_SLICE) return slice_serialize(ctx, ch, dst, len);
   const uint64_t total = stream_bytes_of(ch->n, ch->view.P, ch->view.r);
@@ -405,6 +408,9 @@ int chunk_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_
   if (!ctx || !s || !out) return fail(BUFF_E_INVALID, "null argument");
   if ((flags & BUFF_SLICE) && (flags & (BUFF_FLIP | BUFF_MAJORITY)))
     return fail(BUFF_E_INVALID, "BUFF_SLICE excludes BUFF_FLIP and BUFF_MAJORITY");
+  if ((flags & BUFF_RESIDUE) && (flags & (BUFF_FLIP | BUFF_MAJORITY | BUFF_SLICE)))
+    return fail(BUFF_E_INVALID, "BUFF_RESIDUE excludes BUFF_FLIP, BUFF_MAJORITY and BUFF_SLICE");
+  if (flags & BUFF_RESIDUE) return residue_load(ctx, s, len, scale, flags, out);
   if (flags & BUFF_MAJORITY) return major_load(ctx, s, len, scale, flags, out);
   if (flags & BUFF_SLICE) return slice_load(ctx, s, len, scale, flags, out);
   if (len < kStreamHeader) return fail(BUFF_E_INVALID, "stream of %llu bytes has no 20-byte header",
@@ -518,6 +524,8 @@ int filter_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t flags
   const int rs = slice_filter_dispatch(ctx, ch, op, pred, flags, false, nullptr, words);
   if (rs >= 0) return rs;
   if (op != BUFF_OP_GT && op != BUFF_OP_EQ) return fail(BUFF_E_INVALID, "unknown filter op %d", op);
+  if ((ch->flags & BUFF_RESIDUE) && !(flags & BUFF_SEMANTIC))
+    return fail(BUFF_E_UNSUPPORTED, "the reference has no filter of a residue stream (BUFF_SEMANTIC answers it)");
   const uint64_t nwords = (ch->n + 63) / 64;
   if (nwords == 0) return BUFF_OK;
   if (ch->S > 0) return major_filter_device(ctx, ch, op, pred, flags, words);
@@ -603,6 +611,7 @@ int filter_cond_device(Ctx* ctx, const Chunk* ch, int op, double pred, uint32_t 
   FilterTarget tg;
   int fill = 0;
   bool run = true;
+  if (ch->flags & BUFF_RESIDUE) return fail(BUFF_E_UNSUPPORTED, "conditions and windows of residue chunks");
   if (ch->flags & BUFF_MAJORITY) {
     if (ch->F < 8) return fail(BUFF_E_UNSUPPORTED, "buff-major chunk with F < 8: its column-count byte holds tail bits");
     if (ch->S > 0) {
@@ -683,6 +692,7 @@ int materialize_cond_device(Ctx* ctx, const Chunk* ch, uint32_t flags, const uin
     flags |= BUFF_SEMANTIC;
   }
   const bool semantic = (flags & BUFF_SEMANTIC) != 0;
+  if (ch->flags & BUFF_RESIDUE) return fail(BUFF_E_UNSUPPORTED, "conditions and windows of residue chunks");
   const bool major = (ch->flags & BUFF_MAJORITY) != 0;
   if (major) {
     if (ch->F < 8) return fail(BUFF_E_UNSUPPORTED, "buff-major chunk with F < 8: its column-count byte holds tail bits");
@@ -786,6 +796,7 @@ int max_range_device(Ctx* ctx, const Chunk* ch, uint64_t s, uint64_t e, uint64_t
   const bool semantic = (flags & BUFF_SEMANTIC) != 0;
   const ChunkView& v = ch->view;
   const uint64_t L = e - s;
+  if (ch->flags & BUFF_RESIDUE) return fail(BUFF_E_UNSUPPORTED, "conditions and windows of residue chunks");
   if (ch->flags & BUFF_MAJORITY) {
     // buff_max_majority_range (buff_simd.rs:2863-3114): with sparse columns it
     // logs an error and finds no window (2894-2896); without, it is
@@ -879,6 +890,7 @@ double reference_sum_fold(const Chunk* ch, const uint64_t* ps) {
 
 int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64_t* plane_sums) {
   if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
+  if (ch->flags & BUFF_RESIDUE) return residue_sum_op(ctx, ch, -1, 0, ref_sum, exact_sum, plane_sums);
   if (ch->S > 0) return major_sum(ctx, ch, 0, ref_sum, exact_sum, plane_sums);
   uint64_t ps[kMaxPlanes + 1] = {0};
   const ChunkView& v = ch->view;
@@ -905,6 +917,8 @@ int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64
 int max_op(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t* max_off, uint64_t* argmax) {
   if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
   if (argmax) *argmax = 0;
+  if ((ch->flags & BUFF_RESIDUE) && !(flags & BUFF_SEMANTIC))
+    return fail(BUFF_E_UNSUPPORTED, "the reference has no max of a residue stream (BUFF_SEMANTIC answers it)");
   // buff-major chunks report the tail-stage ties (buff_simd.rs:2856); a chunk
   // without sparse columns runs the same stages over its planes
   if (ch->S > 0 || ((ch->flags & BUFF_MAJORITY) && argmax && ch->n > 0))
@@ -998,6 +1012,7 @@ int sum_precision_op(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flag
                      double* exact_sum, uint64_t* sums_out) {
   if (!ctx || !ch) return fail(BUFF_E_INVALID, "null argument");
   if (precision < 0) return fail(BUFF_E_INVALID, "precision %d < 0", precision);
+  if (ch->flags & BUFF_RESIDUE) return residue_sum_op(ctx, ch, precision, flags, ref_sum, exact_sum, sums_out);
   if (ch->flags & (BUFF_MAJORITY | BUFF_SLICE))
     return fail(BUFF_E_UNSUPPORTED, "the reference has no reduced-precision sum of a %s stream",
                 (ch->flags & BUFF_SLICE) ? "byte-slice" : "buff-major");
@@ -1083,7 +1098,7 @@ int mat_plan(const Chunk* ch, int32_t precision, uint32_t flags, MatParams* out)
       return rc;
     }
     if (!semantic) {
-      if (ch->ilen > 15)
+      if (ch->ilen > 15 && !(ch->flags & BUFF_RESIDUE))
         return fail(BUFF_E_UNSUPPORTED, "ilen %u > 15: the reference parses an outlier section here", ch->ilen);
       if (R / 8 > v.P) {
         // The reference reads R/8 whole n-byte sub-columns: past the end of the
@@ -1119,8 +1134,10 @@ int materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t fl
     BUFF_TRY(slice_materialize_check(ch, precision, flags));
     precision = -1;
   }
+  if ((ch->flags & BUFF_RESIDUE) && ch->S > 0 && !(flags & BUFF_SEMANTIC))
+    return residue_materialize_device(ctx, ch, precision, out, nullptr);
   if (ch->flags & BUFF_MAJORITY) {
-    if (precision >= 0 && !(flags & BUFF_SEMANTIC))
+    if (precision >= 0 && !(flags & BUFF_SEMANTIC) && !(ch->flags & BUFF_RESIDUE))
       return fail(BUFF_E_UNSUPPORTED, "the reference has no reduced-precision decode of a buff-major stream");
     if (ch->S > 0) return major_materialize_device(ctx, ch, precision, flags, out, nullptr);
   }
@@ -1138,7 +1155,9 @@ int materialize_host(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flag
   if (!out) return fail(BUFF_E_INVALID, "null output");
   BUFF_TRY(ctx->scratch(ctx->s_values, ch->n * 8));
   BUFF_TRY(materialize_device(ctx, ch, precision, flags, (double*)ctx->s_values.ptr));
-  const uint64_t rows = (ch->flags & BUFF_MAJORITY) ? major_decoded_rows(ch, flags) : ch->n;
+  const uint64_t rows = ((ch->flags & BUFF_RESIDUE) && !(flags & BUFF_SEMANTIC)) ? residue_decoded_rows(ch, precision)
+                        : (ch->flags & BUFF_MAJORITY) ? major_decoded_rows(ch, flags)
+                                                      : ch->n;
   if (rows) HIP_TRY(hipMemcpyAsync(out, ctx->s_values.ptr, rows * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   return BUFF_OK;
@@ -1481,7 +1500,9 @@ int buff_chunk_info_get(const buff_chunk* chunk, buff_chunk_info* info) {
   info->device_bytes = ch->device_bytes;
   info->sparse_cols = ch->S;
   info->reserved = 0;
-  info->decoded_rows = (ch->flags & BUFF_MAJORITY) ? major_decoded_rows(ch, 0) : ch->n;
+  info->decoded_rows = (ch->flags & BUFF_RESIDUE)  ? residue_decoded_rows(ch, -1)
+                       : (ch->flags & BUFF_MAJORITY) ? major_decoded_rows(ch, 0)
+                                                     : ch->n;
   return BUFF_OK;
 }
 
@@ -1668,10 +1689,45 @@ int buff_sd_buff_simd256_decode(const buff_split_double* sd, const uint8_t* byte
   return sd_decode(sd, bytes, len, BUFF_FLIP, -1, out, cap);
 }
 
+// a residue stream decoded: values written to out (cap doubles), their count to n_out
+static int sd_residue_decode(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, int32_t prec,
+                             double* out, uint64_t cap, uint64_t* n_out) {
+  Ctx* c = nullptr;
+  buff_chunk* ch = nullptr;
+  BUFF_TRY(sd_load(sd, bytes, len, BUFF_RESIDUE, &c, &ch));
+  const Chunk* k = reinterpret_cast<Chunk*>(ch);
+  const uint64_t rows = residue_decoded_rows(k, prec);
+  int rc = BUFF_OK;
+  if (cap < rows) rc = fail(BUFF_E_SMALL_BUFFER, "output needs %llu doubles", (unsigned long long)rows);
+  if (rc == BUFF_OK) rc = buff_materialize(reinterpret_cast<buff_ctx*>(c), ch, prec, 0, out);
+  if (rc == BUFF_OK && n_out) *n_out = rows;
+  (void)buff_chunk_free(ch);
+  return rc;
+}
+
+int buff_sd_byte_residue_encode(const buff_split_double* sd, const double* data, uint64_t n, uint8_t* out,
+                                uint64_t cap, uint64_t* out_len) {
+  return sd_encode(sd, data, n, BUFF_RESIDUE, out, cap, out_len);
+}
+
+int buff_sd_byte_residue_decode(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* out,
+                                uint64_t cap, uint64_t* n_out) {
+  return sd_residue_decode(sd, bytes, len, -1, out, cap, n_out);
+}
+
 int buff_sd_byte_residue_decode_with_precision(const buff_split_double* sd, const uint8_t* bytes, uint64_t len,
-                                               uint32_t precision, double* out, uint64_t cap) {
+                                               uint32_t precision, double* out, uint64_t cap, uint64_t* n_out) {
   if (precision > 0x7FFFFFFFu) return fail(BUFF_E_REF_PANIC, "precision %u is not in PRECISION_MAP", precision);
-  return sd_decode(sd, bytes, len, 0, (int32_t)precision, out, cap);
+  return sd_residue_decode(sd, bytes, len, (int32_t)precision, out, cap, n_out);
+}
+
+int buff_sd_byte_residue_sum(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* s) {
+  Ctx* c = nullptr;
+  buff_chunk* ch = nullptr;
+  BUFF_TRY(sd_load(sd, bytes, len, BUFF_RESIDUE, &c, &ch));
+  int rc = buff_sum(reinterpret_cast<buff_ctx*>(c), ch, s, nullptr, nullptr);
+  (void)buff_chunk_free(ch);
+  return rc;
 }
 
 static int sd_filter(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, uint32_t flags, int op,
@@ -1718,7 +1774,7 @@ int buff_sd_byte_residue_sum_with_precision(const buff_split_double* sd, const u
   if (precision > 0x7FFFFFFFu) return fail(BUFF_E_REF_PANIC, "precision %u is not in PRECISION_MAP", precision);
   Ctx* c = nullptr;
   buff_chunk* ch = nullptr;
-  BUFF_TRY(sd_load(sd, bytes, len, 0, &c, &ch));
+  BUFF_TRY(sd_load(sd, bytes, len, BUFF_RESIDUE, &c, &ch));
   int rc = buff_sum_precision(reinterpret_cast<buff_ctx*>(c), ch, (int32_t)precision, 0, s, nullptr, nullptr);
   (void)buff_chunk_free(ch);
   return rc;

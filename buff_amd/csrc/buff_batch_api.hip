@@ -1034,7 +1034,7 @@ int buff_server_add_chunk(buff_server* server, buff_chunk* chunk, uint32_t* id) 
   if (!sv || !chunk) return fail(BUFF_E_INVALID, "null argument");
   Chunk* ch = reinterpret_cast<Chunk*>(chunk);
   if (ch->ctx != sv->ctx) return fail(BUFF_E_INVALID, "the chunk belongs to another context");
-  if (ch->flags & BUFF_SLICE)
+  if (ch->flags & (BUFF_SLICE | BUFF_RESIDUE))
     return fail(BUFF_E_UNSUPPORTED, "served chunks use the byte_fixed or buff-major layout");
   std::lock_guard<std::mutex> lk(sv->mu);
   if (id) *id = (uint32_t)sv->chunks.size();

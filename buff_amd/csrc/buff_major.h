@@ -95,8 +95,10 @@ hipError_t launch_major_positions(const uint16_t* mask16, const uint32_t* prefix
                                   hipStream_t s);
 hipError_t launch_major_filter(const MajorView& mv, int op, bool semantic, const MajorTarget& tg, uint64_t* out,
                                uint64_t nwords, hipStream_t s, const LaunchShape& ls);
-// semantic: the true offsets, truncated to their top F - drop bits (drop = 0: every bit).
-hipError_t launch_major_decode(const MajorView& mv, bool semantic, uint32_t drop, double inv_scale, double* out,
+// mode 0: buff_decode_majority's values; 1: the true offsets truncated to their
+// top F - drop bits (drop = 0: every bit); 2: the true offsets' top F - drop
+// bits as a number (off >> drop), the residue format's reduced-precision read.
+hipError_t launch_major_decode(const MajorView& mv, int mode, uint32_t drop, double inv_scale, double* out,
                                hipStream_t s, const LaunchShape& ls);
 // sums[j] = sum of the outlier bytes of sparse column j; scratch: major_sum_scratch_bytes().
 uint64_t major_sum_scratch_bytes(uint32_t S);

@@ -300,7 +300,9 @@ __global__ __launch_bounds__(256) void k_major_decode(MajorView mv, double inv_s
     major_tile_offsets<SEM, S>(mv, c, lane, v);
 #pragma unroll
     for (uint32_t b = 0; b < 16; ++b) {
-      const uint64_t o = SEM ? (v[b] >> drop) << drop : v[b];   // SEM: the top F - drop bits
+      // SEM 1: the top F - drop bits in place; SEM 2: the top F - drop bits as an
+      // (F - drop)-bit number (the residue format's reduced-precision read)
+      const uint64_t o = SEM == 1 ? (v[b] >> drop) << drop : SEM == 2 ? v[b] >> drop : v[b];
       sv[kLdsRow * lane + b] = (double)(int64_t)((uint64_t)dv.base + o) * inv_scale;
     }
     wave_lds_sync();
@@ -900,12 +902,13 @@ static hipError_t launch_decode_sem(const MajorView& mv, double inv_scale, uint3
   return hipGetLastError();
 }
 
-hipError_t launch_major_decode(const MajorView& mv, bool semantic, uint32_t drop, double inv_scale, double* out,
+hipError_t launch_major_decode(const MajorView& mv, int mode, uint32_t drop, double inv_scale, double* out,
                                hipStream_t s, const LaunchShape& ls) {
-  if (drop && !semantic) return hipErrorInvalidValue;
+  if (drop && mode == 0) return hipErrorInvalidValue;
   const uint32_t grid = tiles_grid(mv.dense.ntiles, ls.max_wg ? ls.max_wg : 2048);
-  return semantic ? launch_decode_sem<1>(mv, inv_scale, drop, out, grid, s)
-                  : launch_decode_sem<0>(mv, inv_scale, 0, out, grid, s);
+  return mode == 1 ? launch_decode_sem<1>(mv, inv_scale, drop, out, grid, s)
+       : mode == 2 ? launch_decode_sem<2>(mv, inv_scale, drop, out, grid, s)
+                   : launch_decode_sem<0>(mv, inv_scale, 0, out, grid, s);
 }
 
 static constexpr uint32_t kOutsumBlocks = 64;

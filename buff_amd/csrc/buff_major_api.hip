@@ -14,6 +14,7 @@
 #include "buff_common.h"
 #include "buff_major.h"
 #include "buff_runtime.h"
+#include "roaring_portable.h"
 #include "snappy_raw.h"
 
 namespace buffgpu {
@@ -137,7 +138,7 @@ Chunk* new_major_chunk(Ctx* ctx, uint64_t n, uint64_t scale, uint32_t flags, uin
   ch->ctx = ctx;
   ch->n = n;
   ch->scale = scale;
-  ch->flags = (flags & BUFF_FLIP) | BUFF_MAJORITY;
+  ch->flags = (flags & (BUFF_FLIP | BUFF_RESIDUE)) | BUFF_MAJORITY;
   ch->dlen = dlen;
   ch->ilen = ilen;
   ch->F = F;
@@ -183,8 +184,39 @@ void major_release(Chunk* ch) {
   ch->mem_sparse = ch->mem_outl = nullptr;
 }
 
+// split_double.rs:680-716: the leading-zero histogram of the first 2000
+// offsets (clz - (64 - F), usize, counted below 32), summed byte by byte over
+// the ilen / 8 leading bytes; a byte becomes an outlier column while fewer than
+// (2000 as f32 * 0.1f32) as u32 = 200 samples have a nonzero bit up to it. The
+// histogram has 32 slots: reading slot 32 panics.
+int residue_columns(const std::vector<uint64_t>& head, uint32_t F, uint64_t ilen64, uint32_t* S) {
+  const uint64_t head_len = 64 - (uint64_t)F;
+  uint32_t counter[32] = {0};
+  for (uint64_t v : head) {
+    const uint64_t lz = (uint64_t)(v ? __builtin_clzll(v) : 64) - head_len;
+    if (lz < 32) counter[lz] += 1;
+  }
+  const uint32_t limit = (uint32_t)((float)kSample * 0.1f);
+  const uint64_t adjusted = (ilen64 / 8) * 8;
+  uint32_t sum = 0, cols = 0;
+  for (uint64_t i = 0; i < adjusted; ++i) {
+    if (i >= 32) return fail(BUFF_E_REF_PANIC, "byte_residue_encode reads bits_counter[%llu] of 32", (unsigned long long)i);
+    sum += counter[i];
+    if (i % 8 == 7) {
+      if (sum < limit) cols += 1;
+      else break;
+    }
+  }
+  if (cols > kMaxSparse) return fail(BUFF_E_UNSUPPORTED, "%u outlier columns", cols);
+  *S = cols;
+  return BUFF_OK;
+}
+
 // ---------------------------------------------------------------------------
 // buff_simd.rs:317-630 (buff_encode_majority)
+
+int major_build(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_t flags, int32_t pexp, uint32_t dlen,
+                int64_t mn, uint32_t F, uint32_t S, const uint8_t* majors, Chunk** out);
 
 int major_encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_t flags, Chunk** out) {
   if (!ctx || !out) return fail(BUFF_E_INVALID, "null argument");
@@ -194,9 +226,12 @@ int major_encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, u
   const int dl = precision_map(prec);
   if (dl < 0)
     return fail(BUFF_E_REF_PANIC, "precision %d (scale %llu) is not in PRECISION_MAP", prec, (unsigned long long)scale);
-  if (n < kSample)
+  const bool residue = (flags & BUFF_RESIDUE) != 0;
+  if (n < kSample && !residue)
     return fail(BUFF_E_REF_PANIC, "buff-major samples the first %llu rows (fixed_vec[0..SAMPLE]); the column has %llu",
                 (unsigned long long)kSample, (unsigned long long)n);
+  if (residue && n == 0)   // min/max stay at their initial values, ilen wraps above 15, the sample slice panics
+    return fail(BUFF_E_REF_PANIC, "byte_residue_encode of an empty column samples fixed_vec[0..SAMPLE]");
   const uint32_t dlen = (uint32_t)dl;
   const int32_t pexp = precision_exp_of_prec(prec);
   const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
@@ -218,6 +253,16 @@ int major_encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, u
   const uint32_t F = width_of_delta((int64_t)((uint64_t)mx - (uint64_t)mn));
   if (F > 63) return fail(BUFF_E_UNSUPPORTED, "bit width %u exceeds 63", F);
 
+  // byte_residue_encode (split_double.rs:624-954): ilen = fixed_len - dec_len in
+  // usize (wrapping for a constant column); with ilen > 15 the leading byte
+  // columns whose top byte is nonzero in fewer than 10% of the first 2000
+  // offsets become outlier columns (majority byte 0), else the byte_fixed layout
+  const uint64_t ilen64 = (uint64_t)F - dlen;
+  if (residue && ilen64 <= 15)
+    return major_build(ctx, x, n, scale, flags, pexp, dlen, mn, F, 0, nullptr, out);
+  if (residue && n < kSample)
+    return fail(BUFF_E_REF_PANIC, "byte_residue_encode samples fixed_vec[0..SAMPLE] (ilen %llu > 15); the column "
+                "has %llu rows", (unsigned long long)ilen64, (unsigned long long)n);
   // the vote reads the offsets of the first 2000 rows
   std::vector<double> head_x(kSample);
   HIP_TRY(hipMemcpyAsync(head_x.data(), x, kSample * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -228,24 +273,34 @@ int major_encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, u
     memcpy(&u, &head_x[i], 8);
     head[i] = (uint64_t)fetch_fixed_aligned_bits(u, pexp, dlen) - (uint64_t)mn;
   }
-  uint8_t majors[kMaxSparse];
+  uint8_t majors[kMaxSparse] = {0};
   uint32_t S = 0;
-  uint32_t rsf = F >= 8 ? F - 8 : 0;
-  uint8_t m;
-  int32_t fq;
-  vote(head, rsf, &m, &fq);
-  if (fq > kMajorLimit) {
-    majors[S++] = m;
-    while (rsf > 7) {
-      rsf -= 8;
-      vote(head, rsf, &m, &fq);
-      if (fq > kMajorLimit) majors[S++] = m;
-      else break;
+  if (residue) {
+    BUFF_TRY(residue_columns(head, F, ilen64, &S));
+  } else {
+    uint32_t rsf = F >= 8 ? F - 8 : 0;
+    uint8_t m;
+    int32_t fq;
+    vote(head, rsf, &m, &fq);
+    if (fq > kMajorLimit) {
+      majors[S++] = m;
+      while (rsf > 7) {
+        rsf -= 8;
+        vote(head, rsf, &m, &fq);
+        if (fq > kMajorLimit) majors[S++] = m;
+        else break;
+      }
     }
+    if (S > 0 && F < 8)
+      return fail(BUFF_E_REF_PANIC, "F = %u < 8 with a majority byte: `remain -= 8` underflows (buff_simd.rs:465)", F);
   }
-  if (S > 0 && F < 8)
-    return fail(BUFF_E_REF_PANIC, "F = %u < 8 with a majority byte: `remain -= 8` underflows (buff_simd.rs:465)", F);
+  return major_build(ctx, x, n, scale, flags, pexp, dlen, mn, F, S, majors, out);
+}
 
+// The resident chunk of S sparse columns with the given majority bytes (the
+// dense planes and tail below them) from the float64 column x.
+int major_build(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, uint32_t flags, int32_t pexp, uint32_t dlen,
+                int64_t mn, uint32_t F, uint32_t S, const uint8_t* majors, Chunk** out) {
   Chunk* ch = new_major_chunk(ctx, n, scale, flags, dlen, (uint32_t)((uint64_t)F - dlen), mn, F, S);
   if (!ch) return fail(BUFF_E_OOM, "host allocation");
   int rc = chunk_alloc(ctx, ch);
@@ -730,8 +785,353 @@ int major_materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint3
   if (r == 0) return BUFF_OK;
   if (!out) return fail(BUFF_E_INVALID, "null output");
   ProfScope sc(ctx, "k_major_decode");
-  HIP_TRY(launch_major_decode(major_view(ch), semantic, drop, std::ldexp(1.0, -(int)ch->dlen), out, ctx->stream,
+  HIP_TRY(launch_major_decode(major_view(ch), semantic ? 1 : 0, drop, std::ldexp(1.0, -(int)ch->dlen), out, ctx->stream,
                               ctx->shape(Op::Materialize, ch->view)));
+  return BUFF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// RAPG residue format (split_double.rs:624-954 byte_residue_encode, 1353-1893
+// byte_residue_decode[_with_precision], 2210-2629 byte_residue_sum
+// [_with_precision]) on the same resident form, every majority byte 0.
+//
+// Stream: header; with ilen > 15 (header ilen, u32) a count byte S, per
+// outlier column {u32 size, portable bitmap, u32 count, bytes}, and when S > 0
+// one zero byte; then the remaining planes and the LSB-first tail, as
+// byte_fixed_encode writes them (ilen <= 15: the byte_fixed stream itself).
+
+namespace {
+
+bool residue_section(const Chunk* ch) { return ch->ilen > 15; }
+
+}  // namespace
+
+int residue_serialize(Ctx* ctx, const Chunk* ch, uint8_t* dst, uint64_t* len) {
+  if (!ctx || !ch || !len) return fail(BUFF_E_INVALID, "null argument");
+  const uint64_t need = *len;
+  const uint64_t body = dense_body_bytes(ch);
+  std::vector<std::vector<uint8_t>> bms(ch->S);
+  uint64_t total = kStreamHeader + body;
+  if (residue_section(ch)) total += 1 + (ch->S ? 1 : 0);
+  for (uint32_t j = 0; j < ch->S; ++j) {
+    std::vector<uint8_t> binary;
+    BUFF_TRY(bitmap_words(ctx, ch, j, &binary));
+    roaring::serialize(reinterpret_cast<const uint32_t*>(binary.data() + 8), ch->n, &bms[j]);
+    total += 4 + bms[j].size() + 4 + ch->sp[j].n_out;
+  }
+  *len = total;
+  if (!dst) return BUFF_OK;
+  if (need < total) return fail(BUFF_E_SMALL_BUFFER, "stream needs %llu bytes", (unsigned long long)total);
+  const uint64_t ubase = (uint64_t)ch->view.base;
+  const uint32_t hdr[5] = {(uint32_t)ubase, (uint32_t)(ubase >> 32), (uint32_t)ch->n, ch->ilen, ch->dlen};
+  memcpy(dst, hdr, 20);
+  uint64_t pos = kStreamHeader;
+  if (residue_section(ch)) dst[pos++] = (uint8_t)ch->S;
+  for (uint32_t j = 0; j < ch->S; ++j) {
+    const uint32_t bl = (uint32_t)bms[j].size(), no = (uint32_t)ch->sp[j].n_out;
+    memcpy(dst + pos, &bl, 4);
+    memcpy(dst + pos + 4, bms[j].data(), bl);
+    pos += 4 + bl;
+    memcpy(dst + pos, &no, 4);
+    pos += 4;
+    if (no) HIP_TRY(hipMemcpyAsync(dst + pos, ch->sp[j].outl, no, hipMemcpyDeviceToHost, ctx->stream));
+    pos += no;
+  }
+  if (ch->S) dst[pos++] = 0;
+  if (body) {
+    BUFF_TRY(ctx->scratch(ctx->s_stream, body));
+    {
+      ProfScope ps(ctx, "k_serialize");
+      HIP_TRY(launch_serialize(ch->view, 0, (uint8_t*)ctx->s_stream.ptr, (ch->n * ch->view.r + 7) / 8, ctx->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(dst + pos, ctx->s_stream.ptr, body, hipMemcpyDeviceToHost, ctx->stream));
+    pos += body;
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (pos != total) return fail(BUFF_E_DEVICE, "internal: wrote %llu of %llu stream bytes", (unsigned long long)pos,
+                                (unsigned long long)total);
+  return BUFF_OK;
+}
+
+int residue_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_t flags, Chunk** out) {
+  if (!ctx || !s || !out) return fail(BUFF_E_INVALID, "null argument");
+  if (len < kStreamHeader) return fail(BUFF_E_INVALID, "residue stream of %llu bytes", (unsigned long long)len);
+  uint32_t hdr[5];
+  memcpy(hdr, s, 20);
+  const int64_t base = (int64_t)((uint64_t)hdr[0] | ((uint64_t)hdr[1] << 32));
+  const uint64_t n = hdr[2];
+  const uint32_t ilen = hdr[3], dlen = hdr[4];
+  const uint32_t F = dlen + ilen;   // u32, as the readers add them
+  if (F > 63) return fail(BUFF_E_INVALID, "stored width dlen+ilen = %u is not a BUFF stream width", F);
+  struct Col {
+    std::vector<uint32_t> words;
+    const uint8_t* outl;
+    uint64_t n_out;
+  };
+  std::vector<Col> cols;
+  uint64_t pos = kStreamHeader;
+  uint32_t S = 0;
+  if (ilen > 15) {
+    if (len < kStreamHeader + 1) return fail(BUFF_E_INVALID, "residue stream without its column count");
+    S = s[pos++];
+    if (S > F / 8 || S > kMaxSparse) return fail(BUFF_E_INVALID, "%u outlier columns in a %u-bit stream", S, F);
+    cols.resize(S);
+    for (uint32_t j = 0; j < S; ++j) {
+      uint32_t bl, no;
+      if (pos + 4 > len) return fail(BUFF_E_INVALID, "truncated outlier column %u", j);
+      memcpy(&bl, s + pos, 4);
+      pos += 4;
+      if (bl > len - pos) return fail(BUFF_E_INVALID, "outlier column %u bitmap runs past the stream", j);
+      uint64_t card = 0;
+      const char* why = "";
+      if (!roaring::deserialize(s + pos, bl, n, &cols[j].words, &card, &why))
+        return fail(BUFF_E_INVALID, "outlier column %u bitmap: %s", j, why);
+      pos += bl;
+      if (pos + 4 > len) return fail(BUFF_E_INVALID, "truncated outlier column %u", j);
+      memcpy(&no, s + pos, 4);
+      pos += 4;
+      if (no > len - pos) return fail(BUFF_E_INVALID, "outlier column %u bytes run past the stream", j);
+      if (no != card)
+        return fail(BUFF_E_UNSUPPORTED, "outlier column %u: %llu bitmap rows, %u bytes (the readers pair them in order)",
+                    j, (unsigned long long)card, no);
+      cols[j].outl = s + pos;
+      cols[j].n_out = no;
+      pos += no;
+    }
+    if (S) pos += 1;   // the zero byte finish_write_byte leaves
+  }
+  Chunk* ch = new_major_chunk(ctx, n, scale, flags | BUFF_RESIDUE, dlen, ilen, base, F, S);
+  if (!ch) return fail(BUFF_E_OOM, "host allocation");
+  const uint64_t body = dense_body_bytes(ch);
+  if (pos > len || body > len - pos) {
+    delete ch;
+    return fail(BUFF_E_INVALID, "stream is %llu bytes, its layout needs %llu", (unsigned long long)len,
+                (unsigned long long)(pos + body));
+  }
+  int rc = chunk_alloc(ctx, ch);
+  for (uint32_t j = 0; j < S; ++j) {
+    ch->sp[j].major = 0;
+    ch->sp[j].shift = F - 8 * (j + 1);
+    ch->sp[j].n_out = cols[j].n_out;
+  }
+  if (rc == BUFF_OK && S) rc = sparse_alloc(ctx, ch);
+  if (rc == BUFF_OK && S) rc = outlier_alloc(ctx, ch);
+  if (rc != BUFF_OK) {
+    free_partial(ctx, ch);
+    return rc;
+  }
+  const uint64_t ntiles = ch->view.ntiles, nw = (n + 31) / 32;
+  std::vector<std::vector<uint32_t>> pre(S);
+  for (uint32_t j = 0; j < S; ++j) {   // tile prefixes and last outlier rows
+    pre[j].assign(ntiles + 1, 0);
+    uint64_t run = 0, last = ~0ull;
+    for (uint64_t c = 0; c < ntiles; ++c) {
+      pre[j][c] = (uint32_t)run;
+      for (uint64_t w = 32 * c; w < std::min<uint64_t>(32 * c + 32, nw); ++w) {
+        const uint32_t v = cols[j].words[w];
+        run += (uint64_t)__builtin_popcount(v);
+        if (v) last = 32 * w + 31 - (uint64_t)__builtin_clz(v);
+      }
+    }
+    pre[j][ntiles] = (uint32_t)run;
+    ch->sp[j].last = last;
+  }
+  hipError_t e = hipSuccess;
+  for (uint32_t j = 0; j < S && e == hipSuccess; ++j) {
+    e = hipMemsetAsync(ch->sp[j].mask16, 0, mask_bytes(ntiles), ctx->stream);
+    if (e == hipSuccess && nw)
+      e = hipMemcpyAsync(ch->sp[j].mask16, cols[j].words.data(), 4 * nw, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(ch->sp[j].prefix, pre[j].data(), 4 * (ntiles + 1), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && cols[j].n_out)
+      e = hipMemcpyAsync(ch->sp[j].outl, cols[j].outl, cols[j].n_out, hipMemcpyHostToDevice, ctx->stream);
+  }
+  if (e == hipSuccess && S) e = build_positions(ctx, ch);
+  if (e == hipSuccess && body) {
+    rc = ctx->scratch(ctx->s_stream, body);
+    if (rc == BUFF_OK) {
+      e = hipMemcpyAsync(ctx->s_stream.ptr, s + pos, body, hipMemcpyHostToDevice, ctx->stream);
+      if (e == hipSuccess) {
+        ProfScope ps(ctx, "k_load");
+        e = launch_load(ch->view, 0, (const uint8_t*)ctx->s_stream.ptr, (n * ch->view.r + 7) / 8, ctx->stream);
+      }
+    }
+  }
+  if (e == hipSuccess && rc == BUFF_OK) e = hipStreamSynchronize(ctx->stream);   // host buffers may go away
+  if (e != hipSuccess) rc = fail(BUFF_E_DEVICE, "residue upload: %s", hipGetErrorString(e));
+  if (rc != BUFF_OK) {
+    free_partial(ctx, ch);
+    return rc;
+  }
+  *out = ch;
+  return BUFF_OK;
+}
+
+namespace {
+
+// What a reference read of R bits takes from a residue chunk with outlier
+// columns: the S columns, k whole planes and q-bit fields after them. Reads
+// past the stored planes (R > F) read whatever bytes follow: a panic where they
+// run past the stream, else not reproduced. q-bit fields that are not the
+// rows' tail values (0 < q < r) are fields of the packed tail.
+struct ResidueRead {
+  uint32_t R, k, q;
+  bool fields;   // 0 < q < r: the misaligned tail fields
+};
+
+int residue_read(const Chunk* ch, int32_t precision, ResidueRead* rr) {
+  uint32_t R = ch->F;
+  if (precision >= 0) BUFF_TRY(precision_bits(ch, precision, &R));
+  const ChunkView& v = ch->view;
+  const uint32_t rem = R - 8 * ch->S;   // R >= 8 S: R >= ilen >= 8 S
+  rr->R = R;
+  rr->k = rem / 8;
+  rr->q = rem % 8;
+  rr->fields = false;
+  if (rr->k > v.P || (rr->k == v.P && rr->q > v.r)) {
+    const uint64_t avail = dense_body_bytes(ch);
+    const uint64_t reads = (uint64_t)rr->k * ch->n + (ch->n * rr->q + 7) / 8;
+    if (reads > avail) return fail(BUFF_E_REF_PANIC, "the reference reads %u bits of a %u-bit residue stream", R, ch->F);
+    return fail(BUFF_E_UNSUPPORTED, "the reference would read %u bits of a %u-bit residue stream", R, ch->F);
+  }
+  if (rr->q > 0 && rr->q != v.r) {
+    if (rr->k != v.P) return fail(BUFF_E_UNSUPPORTED, "q-bit fields of plane bytes");
+    rr->fields = true;
+  }
+  return BUFF_OK;
+}
+
+}  // namespace
+
+uint64_t residue_decoded_rows(const Chunk* ch, int32_t precision) {
+  if (ch->S == 0) return ch->n;
+  ResidueRead rr;
+  if (residue_read(ch, precision, &rr) != BUFF_OK) return 0;
+  return (rr.k || rr.q) ? ch->n : 0;   // nothing is pushed unless a plane or the tail is read
+}
+
+// byte_residue_decode (precision < 0) / byte_residue_decode_with_precision on a
+// residue chunk with outlier columns: (base + (off >> (F - R))) / 2^dlen for
+// every row (mis-scaled below F bits, as the reference), nothing when the
+// columns take every bit read; a column without outliers panics
+// (outlier_iter.next().unwrap(), split_double.rs:1405).
+int residue_materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, double* out, uint64_t* rows) {
+  ResidueRead rr;
+  BUFF_TRY(residue_read(ch, precision, &rr));
+  for (uint32_t j = 0; j < ch->S; ++j)
+    if (ch->sp[j].n_out == 0)
+      return fail(BUFF_E_REF_PANIC, "outlier column %u has no outliers: outlier_iter.next().unwrap() panics "
+                  "(split_double.rs:1405)", j);
+  if (rr.fields)
+    return fail(BUFF_E_UNSUPPORTED, "a %u-bit read ends in %u-bit fields of the %u-bit packed tail", rr.R, rr.q,
+                ch->view.r);
+  const uint64_t r = (rr.k || rr.q) ? ch->n : 0;
+  if (rows) *rows = r;
+  if (r == 0) return BUFF_OK;
+  if (!out) return fail(BUFF_E_INVALID, "null output");
+  ProfScope sc(ctx, "k_major_decode");
+  HIP_TRY(launch_major_decode(major_view(ch), 2, ch->F - rr.R, std::ldexp(1.0, -(int)ch->dlen), out, ctx->stream,
+                              ctx->shape(Op::Materialize, ch->view)));
+  return BUFF_OK;
+}
+
+// byte_residue_sum (precision < 0) / byte_residue_sum_with_precision on a
+// residue chunk: the outlier byte sums of the columns, the sums of the k planes
+// read, the q-bit fields (the tail, or the packed tail's q-bit fields), folded in
+// the reference's order (each term / 2^(bits read - ilen); no base term when the
+// plane loop ends on a byte, split_double.rs:2271-2273). sums_out: the u64 sums in
+// read order. exact_sum: the exact sum of the values truncated to their top R
+// bits, base included, rounded once.
+int residue_sum_op(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* ref_sum, double* exact_sum,
+                   uint64_t* sums_out) {
+  const bool semantic = (flags & BUFF_SEMANTIC) != 0;
+  ResidueRead rr;
+  const int rc = residue_read(ch, precision, &rr);
+  if (rc != BUFF_OK) {
+    if (!semantic) return rc;
+    rr = ResidueRead{ch->F, ch->view.P, ch->view.r, false};   // what cannot be read: every bit
+  }
+  const ChunkView& v = ch->view;
+  uint64_t ps[kMaxPlanes + 1] = {0};
+  uint64_t osum[kMaxSparse] = {0};
+  uint64_t fields[2] = {0, 0};
+  if (ch->n > 0) {
+    const LaunchShape ls = ctx->shape(Op::Sum, v);
+    const uint64_t rows_off = 256, dense_off = rows_off + 4096;
+    BUFF_TRY(ctx->scratch(ctx->s_sums, dense_off + sizeof(ps) + sum_scratch_bytes(ls)));
+    uint8_t* sb = (uint8_t*)ctx->s_sums.ptr;
+    ChunkView pv = v;
+    pv.P = rr.k;
+    pv.r = (rr.q > 0 && !rr.fields) ? v.r : 0;
+    {
+      ProfScope sc(ctx, "k_residue_sum");
+      if (ch->S) {
+        uint64_t n_out[kMaxSparse];
+        for (uint32_t j = 0; j < ch->S; ++j) n_out[j] = ch->sp[j].n_out;
+        HIP_TRY(launch_major_outlier_sums(major_view(ch), n_out, (unsigned long long*)sb,
+                                          (unsigned long long*)(sb + rows_off), ctx->stream));
+      }
+      if (pv.P || pv.r) HIP_TRY(launch_sum(pv, (unsigned long long*)(sb + dense_off), ctx->stream, ls));
+    }
+    if (rr.fields) {
+      BUFF_TRY(ctx->scratch(ctx->s_best, 256));
+      HIP_TRY(launch_fill_bytes(ctx->s_best.ptr, 16, 0, ctx->stream));
+      ProfScope sc(ctx, "k_tail_fields");
+      HIP_TRY(launch_tail_fields(v, rr.q, (unsigned long long*)ctx->s_best.ptr, ctx->stream));
+    }
+    if (ch->S) BUFF_TRY(d2h_small(ctx, osum, sb, 8ull * ch->S));
+    if (pv.P || pv.r) BUFF_TRY(d2h_small(ctx, ps, sb + dense_off, sizeof(ps)));
+    if (rr.fields) BUFF_TRY(d2h_small(ctx, fields, ctx->s_best.ptr, sizeof(fields)));
+  }
+  // sums in read order: columns, whole planes, the q-bit part
+  uint64_t sums[2 * kMaxPlanes + 1] = {0};
+  uint32_t m = 0;
+  for (uint32_t j = 0; j < ch->S; ++j) sums[m++] = osum[j];
+  for (uint32_t k = 0; k < rr.k; ++k) sums[m++] = ps[k];
+  uint64_t top_q = 0;
+  if (rr.q > 0) {
+    sums[m++] = rr.fields ? fields[0] : ps[rr.k];
+    top_q = rr.fields ? fields[1] : ps[rr.k];
+  }
+  if (sums_out) memcpy(sums_out, sums, sizeof(uint64_t) * (kMaxPlanes + 1));
+  // exact: the top R bits of every offset (columns and planes whole, q top tail bits)
+  __int128 tot = (__int128)ch->n * (__int128)v.base;
+  for (uint32_t j = 0; j < ch->S; ++j) tot += (__int128)osum[j] << ch->sp[j].shift;
+  for (uint32_t k = 0; k < rr.k; ++k) tot += (__int128)ps[k] << (v.F - 8 * (k + 1));
+  if (rr.q) tot += (__int128)top_q << (v.r - rr.q);
+  const double exact = (double)tot / powi2((int32_t)ch->dlen);
+  if (exact_sum) *exact_sum = exact;
+  if (!ref_sum) return BUFF_OK;
+  if (semantic) {
+    *ref_sum = exact;
+    return BUFF_OK;
+  }
+  // split_double.rs:2245-2304 / 2467-2533
+  const uint32_t ilen = ch->ilen;
+  uint32_t remain = rr.R;
+  int32_t processed = 0;
+  double sum = 0.0;
+  const double sum_base = (double)(uint32_t)ch->n * (double)v.base / powi2((int32_t)ch->dlen);
+  uint32_t i = 0;
+  for (uint32_t j = 0; j < ch->S; ++j) {
+    remain -= 8;
+    processed += 8;
+    sum = sum + (double)sums[i++] / powi2((int32_t)((uint32_t)processed - ilen));
+  }
+  while (remain >= 8) {
+    remain -= 8;
+    processed += 8;
+    sum = sum + (double)sums[i++] / powi2((int32_t)((uint32_t)processed - ilen));
+    if (remain == 0) {
+      *ref_sum = sum;
+      return BUFF_OK;
+    }
+  }
+  if (remain > 0) {
+    processed += (int32_t)remain;
+    sum = sum + (double)sums[i] / powi2((int32_t)((uint32_t)processed - ilen));
+  }
+  *ref_sum = sum + sum_base;
   return BUFF_OK;
 }
 

@@ -186,6 +186,13 @@ int major_max(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t
 int major_materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* out,
                              uint64_t* rows);
 uint64_t major_decoded_rows(const Chunk* ch, uint32_t flags);
+// RAPG residue format (BUFF_RESIDUE chunks: the buff-major resident form, majority bytes 0).
+int residue_serialize(Ctx* ctx, const Chunk* ch, uint8_t* dst, uint64_t* len);
+int residue_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_t flags, Chunk** out);
+uint64_t residue_decoded_rows(const Chunk* ch, int32_t precision);
+int residue_materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, double* out, uint64_t* rows);
+int residue_sum_op(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* ref_sum, double* exact_sum,
+                   uint64_t* sums_out);
 int major_precision_drop(const Chunk* ch, int32_t precision, uint32_t* drop);
 MajorView major_view(const Chunk* ch);
 // Filter targets of a buff-major chunk; returns -1 when the filter must run, else

@@ -75,11 +75,29 @@ class SplitBDDoubleCompress {
     check(buff_sd_buff_simd256_decode(&sd_, bytes.data(), bytes.size(), out.data(), out.size()));
     return out;
   }
-  // split_double.rs:1612 byte_residue_decode_with_precision (byte_fixed layout)
+  // split_double.rs:624 byte_residue_encode (the RAPG residue stream)
+  std::vector<uint8_t> byte_residue_encode(const Segment& seg) const {
+    uint64_t len = 0;
+    check(buff_sd_byte_residue_encode(&sd_, seg.data.data(), seg.data.size(), nullptr, 0, &len));
+    std::vector<uint8_t> out(len);
+    check(buff_sd_byte_residue_encode(&sd_, seg.data.data(), seg.data.size(), out.data(), out.size(), &len));
+    return out;
+  }
+  // split_double.rs:1353 byte_residue_decode
+  std::vector<double> byte_residue_decode(std::vector<uint8_t> bytes) const {
+    std::vector<double> out(rows(bytes));
+    uint64_t got = 0;
+    check(buff_sd_byte_residue_decode(&sd_, bytes.data(), bytes.size(), out.data(), out.size(), &got));
+    out.resize(got);
+    return out;
+  }
+  // split_double.rs:1612 byte_residue_decode_with_precision
   std::vector<double> byte_residue_decode_with_precision(std::vector<uint8_t> bytes, uint32_t precision) const {
     std::vector<double> out(rows(bytes));
+    uint64_t got = 0;
     check(buff_sd_byte_residue_decode_with_precision(&sd_, bytes.data(), bytes.size(), precision, out.data(),
-                                                     out.size()));
+                                                     out.size(), &got));
+    out.resize(got);
     return out;
   }
 
@@ -114,7 +132,13 @@ class SplitBDDoubleCompress {
     check(buff_sd_byte_fixed_sum(&sd_, bytes.data(), bytes.size(), &s));
     return s;
   }
-  // split_double.rs:2409 byte_residue_sum_with_precision (byte_fixed layout)
+  // split_double.rs:2210 byte_residue_sum
+  double byte_residue_sum(std::vector<uint8_t> bytes) const {
+    double s = 0;
+    check(buff_sd_byte_residue_sum(&sd_, bytes.data(), bytes.size(), &s));
+    return s;
+  }
+  // split_double.rs:2409 byte_residue_sum_with_precision
   double byte_residue_sum_with_precision(std::vector<uint8_t> bytes, uint32_t precision) const {
     double s = 0;
     check(buff_sd_byte_residue_sum_with_precision(&sd_, bytes.data(), bytes.size(), precision, &s));

@@ -60,8 +60,29 @@ class SplitBDDoubleCompress:
     def buff_simd256_decode(self, b):  # buff_simd.rs:3252
         return self._decode("buff_sd_buff_simd256_decode", b)
 
+    def byte_residue_encode(self, x):  # split_double.rs:624
+        return self._encode("buff_sd_byte_residue_encode", x)
+
+    def _residue_decode(self, fn, b, *extra):
+        b = _u8(b)
+        out = np.zeros(max(_stream_n(b), 1), dtype=np.float64)
+        got = ctypes.c_uint64()
+        N.check(getattr(N.lib(), fn)(ctypes.byref(self._sd), N.ptr(b, ctypes.c_uint8), len(b), *extra,
+                                     N.ptr(out, ctypes.c_double), len(out), ctypes.byref(got)))
+        return out[: got.value]
+
+    def byte_residue_decode(self, b):  # split_double.rs:1353
+        return self._residue_decode("buff_sd_byte_residue_decode", b)
+
     def byte_residue_decode_with_precision(self, b, precision):  # split_double.rs:1612
-        return self._decode("buff_sd_byte_residue_decode_with_precision", b, precision)
+        return self._residue_decode("buff_sd_byte_residue_decode_with_precision", b, precision)
+
+    def byte_residue_sum(self, b):  # split_double.rs:2210
+        b = _u8(b)
+        s = ctypes.c_double()
+        N.check(N.lib().buff_sd_byte_residue_sum(ctypes.byref(self._sd), N.ptr(b, ctypes.c_uint8), len(b),
+                                                 ctypes.byref(s)))
+        return s.value
 
     def _filter(self, fn, b, pred):
         b = _u8(b)

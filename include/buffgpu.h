@@ -69,6 +69,19 @@ extern "C" {
 /* Filters of a byte_fixed chunk: buff_simd_range_filter_with_slice /
  * buff_simd_equal_filter_with_slice (buff_simd.rs:3736-4025). */
 #define BUFF_WITH_SLICE 0x20u
+/* RAPG residue layout of byte_residue_encode (split_double.rs:624-954): header
+ * from the true minimum and maximum; with ilen > 15 a count byte and the
+ * leading byte columns whose top byte is nonzero in fewer than 10% of the first
+ * 2000 offsets stored as {u32 size, CRoaring portable bitmap of the rows whose
+ * byte is nonzero (run-optimized), u32 count, those bytes}, a zero byte, then
+ * the remaining planes and tail as in byte_fixed_encode (with ilen <= 15 the
+ * byte_fixed layout itself). Accepted by buff_encode, buff_encode_device and
+ * buff_chunk_load; such a chunk answers buff_materialize (full and reduced
+ * precision: byte_residue_decode[_with_precision]), buff_sum and
+ * buff_sum_precision (byte_residue_sum[_with_precision]); filters and maxima
+ * (no reference counterpart) only with BUFF_SEMANTIC. Excludes BUFF_FLIP,
+ * BUFF_MAJORITY and BUFF_SLICE. */
+#define BUFF_RESIDUE 0x40u
 
 /* filter operators */
 #define BUFF_OP_GT 0 /* x > pred (buff_range_filter, buff_simd.rs:4507-4679) */
@@ -467,12 +480,20 @@ int buff_sd_buff_simd_equal_filter(const buff_split_double* sd, const uint8_t* b
 int buff_sd_byte_fixed_sum(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* sum);
 /* split_double.rs:3919 byte_fixed_max (prints the max; returned here) */
 int buff_sd_byte_fixed_max(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* max_value);
-/* split_double.rs:1612 byte_residue_decode_with_precision on a byte_fixed
- * stream (the ilen <= 15 branch; larger ilen is BUFF_E_UNSUPPORTED). */
+/* RAPG residue streams (BUFF_RESIDUE; with ilen <= 15 the byte_fixed stream):
+ * split_double.rs:624 byte_residue_encode (out == NULL: size query), :1353
+ * byte_residue_decode, :1612 byte_residue_decode_with_precision (n_out, may be
+ * NULL: values written, 0 when the outlier columns take every bit read), :2210
+ * byte_residue_sum, :2409 byte_residue_sum_with_precision. A reduced-precision
+ * decode of a stream with outlier columns whose read ends in fields of the
+ * packed tail is BUFF_E_UNSUPPORTED. */
+int buff_sd_byte_residue_encode(const buff_split_double* sd, const double* data, uint64_t n, uint8_t* out,
+                                uint64_t cap, uint64_t* out_len);
+int buff_sd_byte_residue_decode(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* out,
+                                uint64_t cap, uint64_t* n_out);
 int buff_sd_byte_residue_decode_with_precision(const buff_split_double* sd, const uint8_t* bytes, uint64_t len,
-                                               uint32_t precision, double* out, uint64_t cap);
-/* split_double.rs:2409 byte_residue_sum_with_precision on a byte_fixed stream
- * (the ilen <= 15 branch; larger ilen is BUFF_E_UNSUPPORTED). */
+                                               uint32_t precision, double* out, uint64_t cap, uint64_t* n_out);
+int buff_sd_byte_residue_sum(const buff_split_double* sd, const uint8_t* bytes, uint64_t len, double* sum);
 int buff_sd_byte_residue_sum_with_precision(const buff_split_double* sd, const uint8_t* bytes, uint64_t len,
                                             uint32_t precision, double* sum);
 /* buff-major: buff_simd.rs:317 buff_encode_majority (out == NULL: size query),

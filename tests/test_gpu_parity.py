@@ -159,7 +159,10 @@ def test_sum_with_precision(gpu_ctx, oracle, name, x, scale):
         if err is not None:
             with pytest.raises(err):
                 ch.sum_precision(p)
-            with pytest.raises(err):
+            # the reference-shaped entry point parses its input as a residue
+            # stream: a byte_fixed stream with ilen > 15 has no outlier section,
+            # its first plane byte is taken for the column count and the parse fails
+            with pytest.raises(err if h["ilen"] <= 15 else buff_amd.BuffError):
                 sd.byte_residue_sum_with_precision(ref, p)
         else:
             got, exact, ps = ch.sum_precision(p)
