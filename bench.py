@@ -90,6 +90,8 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=1 << 25)
     ap.add_argument("--no-config3", action="store_true", help="skip the outlier-heavy (buff vs buff-major) section")
     ap.add_argument("--config3-rows", type=int, default=1 << 26)
+    ap.add_argument("--no-residue", action="store_true", help="skip the RAPG residue-format section")
+    ap.add_argument("--residue-rows", type=int, default=1 << 26)
     ap.add_argument("--no-config4", action="store_true", help="skip the batched-form section")
     ap.add_argument("--config4-chunks", type=int, default=4096)
     ap.add_argument("--config4-queries", type=int, default=16384)
@@ -613,6 +615,100 @@ def bench_config3(ctx, args, buff_amd, datagen):
         res[fmt] = fres
     for b in (xb, words, out):
         b.free()
+    return res
+
+
+def bench_residue(ctx, args, buff_amd, datagen):
+    """RAPG residue format (the p_scan driver's codec, pscan/main.rs): a column of
+    95% N(100, 0.5) and 1% 100 + 10^U[3,7] spikes, 3 decimals, seed 11, so the
+    top byte columns are mostly zero. byte_residue_encode on the GPU, then
+    byte_residue_decode and byte_residue_sum at full precision and at 0..3
+    decimals, each timed with the library's HIP events (median of 5). The answers
+    are checked against numpy (full decode = the fixed grid, sums against the
+    exact integer sums) and the oracle's restatement is timed on one core on a
+    sample."""
+    n = args.residue_rows
+    scale, dlen = 1000, 11
+    x = datagen.positive_spikes(n, 3, 11, 0.01)
+    xb = ctx.alloc(x.nbytes)
+    xb.upload(x)
+    out = ctx.alloc(n * 8)
+    reps = 5
+    res = {"workload": "residue (RAPG) column, %d float64 rows: 95%% N(100,0.5), 1%% 100 + 10^U[3,7], 3 decimals"
+           % n, "rows": n}
+
+    def timed(fn, kernels):
+        per = []
+        for _ in range(reps):
+            ctx.synchronize()
+            ctx.prof_reset()
+            ctx.prof_enable(True)
+            fn()
+            ctx.synchronize()
+            ctx.prof_enable(False)
+            per.append(sum(ctx.prof_query(k)[1] for k in kernels) * 1e3)
+        return float(np.median(per))
+
+    enc_k = ("k_minmax", "k_major_encode", "k_major_scan", "k_major_place", "k_encode")
+    us = timed(lambda: ctx.encode_device(xb.ptr, n, scale, flags=buff_amd.BUFF_RESIDUE).free(), enc_k)
+    ch = ctx.encode_device(xb.ptr, n, scale, flags=buff_amd.BUFF_RESIDUE)
+    info = ch.info()
+    n_out = [ch.sparse_col(j)[1] for j in range(info.sparse_cols)]
+    res.update({"F": info.width, "outlier_cols": info.sparse_cols, "outliers": n_out, "planes": info.planes,
+                "tail_bits": info.tail_bits, "stream_bytes": None, "device_bytes": info.device_bytes})
+    res["encode"] = {"device_us": round(us, 2), "GBps_f64_input": round(8.0 * n / (us * 1e-6) / 1e9, 1)}
+    dense_b = n * (info.planes + info.tail_bits / 8.0)
+    sparse_b = float(sum(n_out))
+    ops = {}
+    for p in (-1, 0, 1, 2, 3):
+        us = timed(lambda: ch.materialize_device(out.ptr, p), ("k_major_decode", "k_materialize"))
+        ops["decode" if p < 0 else "decode_p%d" % p] = {"device_us": round(us, 2),
+                                                       "GBps_f64_output": round(8.0 * n / (us * 1e-6) / 1e9, 1)}
+        us = timed((lambda: ch.sum()) if p < 0 else (lambda: ch.sum_precision(p)),
+                   ("k_residue_sum", "k_tail_fields"))
+        e = {"device_us": round(us, 2), "GBps_f64_input": round(8.0 * n / (us * 1e-6) / 1e9, 1)}
+        if p < 0:   # every outlier byte, every plane and the tail
+            b = sparse_b + dense_b + info.sparse_cols * ((n + 1023) // 1024) * 4.0
+            e.update({"algorithmic_bytes": round(b), "achieved_GBps": round(b / (us * 1e-6) / 1e9, 1),
+                      "frac": round(b / (us * 1e-6) / 1e9 / HBM_PEAK_GBPS, 4)})
+        ops["sum" if p < 0 else "sum_p%d" % p] = e
+    res["ops"] = ops
+    # the answers, against numpy
+    f = np.trunc(np.ldexp(x, dlen)).astype(np.int64)
+    ch.materialize_device(out.ptr, -1)
+    got = out.download(np.float64, n)
+    bad = []
+    if not np.array_equal(got, f / 2.0 ** dlen):
+        bad.append("full decode")
+    _, exact, _ = ch.sum()
+    tot = u64_sum((f - f.min()).astype(np.uint64)) + n * int(f.min())
+    if abs(exact - tot / 2.0 ** dlen) > 1e-12 * abs(tot / 2.0 ** dlen):
+        bad.append("exact sum")
+    if bad:
+        log("bench: WRONG RESULTS (residue): %s" % bad)
+        sys.exit(1)
+    res["verified"] = True
+    res["stream_bytes"] = len(ch.serialize())
+    ch.free()
+    for b in (xb, out):
+        b.free()
+    if not args.no_cpu:   # the oracle on one core, on a 4M-row sample of the same column
+        from oracle import refcpu
+
+        refcpu.build()
+        m = 1 << 22
+        xs = x[:m]
+        cpu = {"rows": m, "cores": 1, "kind": "port"}
+        t = time.perf_counter()
+        st, _ = refcpu.residue_encode(xs, scale)
+        cpu["encode_GBps"] = round(8.0 * m / (time.perf_counter() - t) / 1e9, 4)
+        t = time.perf_counter()
+        refcpu.residue_decode(st)
+        cpu["decode_GBps"] = round(8.0 * m / (time.perf_counter() - t) / 1e9, 4)
+        t = time.perf_counter()
+        refcpu.residue_sum(st)
+        cpu["sum_GBps"] = round(8.0 * m / (time.perf_counter() - t) / 1e9, 4)
+        res["cpu_port"] = cpu
     return res
 
 
@@ -1181,6 +1277,9 @@ def main():
             b.free()
         log("bench: outlier-heavy section (%d rows)" % args.config3_rows)
         result["config3"] = bench_config3(ctx, args, buff_amd, datagen)
+    if not args.no_residue:
+        log("bench: residue section (%d rows)" % args.residue_rows)
+        result["residue"] = bench_residue(ctx, args, buff_amd, datagen)
     if not args.no_config4:
         log("bench: batched section (%d chunks, %d queries)" % (args.config4_chunks, args.config4_queries))
         result["config4"] = bench_config4(ctx, args, buff_amd)
