@@ -330,28 +330,44 @@ __device__ __forceinline__ void major_tile_offsets(const MV& mv, uint64_t c, uin
     ws[j] = patch_outliers(pk, m[j], m[j], mv.col[j].major);
   }
   const uint64_t row0 = c * kTileRows + 16ull * lane;
+  // one column at a time over the lane's 16 rows, each behind a wave-uniform
+  // guard (absent planes and tail bits cost no instructions); the tail bits
+  // gather in 32-bit words
 #pragma unroll
-  for (uint32_t b = 0; b < 16; ++b) {
-    uint64_t x = 0;
+  for (uint32_t b = 0; b < 16; ++b) v[b] = 0;
 #pragma unroll
-    for (int j = 0; j < S; ++j) {
-      const uint32_t sh = mv.col[j].shift;
+  for (int j = 0; j < S; ++j) {
+    const uint32_t sh = mv.col[j].shift;
+#pragma unroll
+    for (uint32_t b = 0; b < 16; ++b) {
       const uint64_t byte = (uint64_t)byte_at(ws[j], b) << sh;
       if (SEM || j == 0) {
-        x |= byte;
+        v[b] |= byte;
       } else {
         const uint64_t twice = ((uint64_t)mv.col[j].major << sh) << sh;
         const uint64_t t = ((m[j] >> b) & 1u) ? byte : twice;
-        x |= row0 + b <= mv.col[j].last ? t : 0ull;
+        v[b] |= row0 + b <= mv.col[j].last ? t : 0ull;
       }
     }
-#pragma unroll
-    for (int p = 0; p < NPM; ++p)
-      if ((uint32_t)p < dv.P) x |= (uint64_t)byte_at(wp[p], b) << (dv.F - 8 * (p + 1));
-#pragma unroll
-    for (uint32_t j = 0; j < 7; ++j) x |= (uint64_t)((h[j] >> b) & 1u) << j;
-    v[b] = x;
   }
+#pragma unroll
+  for (int p = 0; p < NPM; ++p) {
+    if ((uint32_t)p >= dv.P) break;
+    const uint32_t sh = dv.F - 8 * (p + 1);
+#pragma unroll
+    for (uint32_t b = 0; b < 16; ++b) v[b] |= (uint64_t)byte_at(wp[p], b) << sh;
+  }
+  uint32_t t[16];
+#pragma unroll
+  for (uint32_t b = 0; b < 16; ++b) t[b] = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 7; ++j) {
+    if (j >= dv.r) break;
+#pragma unroll
+    for (uint32_t b = 0; b < 16; ++b) t[b] |= ((h[j] >> b) & 1u) << j;
+  }
+#pragma unroll
+  for (uint32_t b = 0; b < 16; ++b) v[b] |= t[b];
 }
 
 // The first S sparse columns of a MajorView (the same layout as its prefix):
