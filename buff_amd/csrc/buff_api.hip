@@ -926,14 +926,14 @@ int max_op(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t* m
   uint64_t best = 0;
   const ChunkView& v = ch->view;
   if (ch->n > 0 && (v.P || v.r)) {
-    // [best u64 | pad to 256 | per-tile maxima | per-workgroup top bytes]
+    // [best u64 | 64 u32 slots | pad to 512 | per-tile maxima]
     const uint64_t ms = max_scratch_bytes(v.ntiles);
-    BUFF_TRY(ctx->scratch(ctx->s_best, 256 + ms));
+    BUFF_TRY(ctx->scratch(ctx->s_best, 512 + ms));
     uint8_t* sb = (uint8_t*)ctx->s_best.ptr;
-    HIP_TRY(hipMemsetAsync(sb, 0, 8, ctx->stream));
+    HIP_TRY(hipMemsetAsync(sb, 0, kMaxBestBytes, ctx->stream));
     {
       ProfScope sc(ctx, "k_max");
-      HIP_TRY(launch_max(v, (unsigned long long*)sb, sb + 256, ctx->stream, ctx->shape(Op::Max, v)));
+      HIP_TRY(launch_max(v, (unsigned long long*)sb, sb + 512, ctx->stream, ctx->shape(Op::Max, v)));
     }
     BUFF_TRY(d2h_small(ctx, &best, sb, 8));
   }
@@ -1243,7 +1243,8 @@ int tune_chunk(Ctx* ctx, const Chunk* ch, double pred, buff_tune_choice* out) {
   const LaunchShape probe{4, 0, 0};
   BUFF_TRY(ctx->scratch(ctx->s_bitmap, nwords * 8));
   BUFF_TRY(ctx->scratch(ctx->s_sums, 9 * 8 + sum_scratch_bytes(probe)));
-  BUFF_TRY(ctx->scratch(ctx->s_best, 256 + max_scratch_bytes(v.ntiles)));
+  BUFF_TRY(ctx->scratch(ctx->s_best, 512 + max_scratch_bytes(v.ntiles)));
+  HIP_TRY(hipMemsetAsync(ctx->s_best.ptr, 0, kMaxBestBytes, ctx->stream));   // the runs keep folding this chunk's max
   BUFF_TRY(ctx->scratch(ctx->s_values, ch->n * 8));
   BUFF_TRY(ctx->scratch(ctx->s_sink, 256));
   void* flush = nullptr;
@@ -1262,7 +1263,7 @@ int tune_chunk(Ctx* ctx, const Chunk* ch, double pred, buff_tune_choice* out) {
           return launch_sum(v, (unsigned long long*)ctx->s_sums.ptr, ctx->stream, ls);
         case Op::Max: {
           uint8_t* sb = (uint8_t*)ctx->s_best.ptr;
-          return launch_max(v, (unsigned long long*)sb, sb + 256, ctx->stream, ls);
+          return launch_max(v, (unsigned long long*)sb, sb + 512, ctx->stream, ls);
         }
         default:
           return launch_materialize(v, mp, (double*)ctx->s_values.ptr, ctx->stream, ls);

@@ -64,8 +64,10 @@ hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t
 // the sum of the n q-bit fields the reference reads from the packed tail,
 // out[1] += the sum of every row's top q tail bits (the caller zeroes out[0..1]).
 hipError_t launch_tail_fields(const ChunkView& cv, uint32_t q, unsigned long long* out, hipStream_t s);
-// Two passes using `scratch` (max_scratch_bytes(ntiles) bytes), or one pass
-// (ls.prefetch == 1); the caller zeroes `best`.
+// Two passes using `scratch` (max_scratch_bytes(ntiles) bytes) and 64 u32 slots
+// right after `best`, or one pass (ls.prefetch == 1); the caller zeroes the
+// kMaxBestBytes bytes at `best`.
+constexpr uint64_t kMaxBestBytes = 8 + 64 * 4;
 uint64_t max_scratch_bytes(uint64_t ntiles);
 hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* scratch, hipStream_t s,
                       const LaunchShape& ls);

@@ -521,13 +521,17 @@ hipError_t launch_tail_fields(const ChunkView& cv, uint32_t q, unsigned long lon
 // K6: max stored offset (split_double.rs:3919-4105). The reference narrows the
 // candidates byte by byte; its result is the largest stored offset, whose top
 // byte is the largest top-plane byte. Pass 1 streams the top plane once and
-// records each tile's largest byte plus one per workgroup; pass 2 rebuilds
-// whole offsets only in tiles whose largest byte equals the global one.
+// records each tile's largest byte and folds each workgroup's into one of 64
+// slots (atomicMax, workgroup w into slot w % 64: same-address device atomics
+// serialize); pass 2 takes the max of the slots and rebuilds whole offsets only
+// in tiles whose largest byte equals it.
 
+
+constexpr uint32_t kMaxSlots = 64;
 
 template <int U>
 __global__ __launch_bounds__(256) void k_max_top(ChunkView cv, uint8_t* __restrict__ tile_max,
-                                                 unsigned int* __restrict__ top) {
+                                                 unsigned int* __restrict__ slots) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = wave_global();
   const uint64_t nwaves = waves_in_grid();
@@ -554,29 +558,18 @@ __global__ __launch_bounds__(256) void k_max_top(ChunkView cv, uint8_t* __restri
   if (threadIdx.x == 0) {
     uint32_t b = 0;
     for (uint32_t k = 0; k < (blockDim.x >> 6); ++k) b = max(b, wbest[k]);
-    top[blockIdx.x] = b;
+    if (b) atomicMax(&slots[blockIdx.x % kMaxSlots], b);
   }
 }
 
-// Folds the per-workgroup top bytes of pass 1 into top[ntop] (one workgroup).
-__global__ __launch_bounds__(256) void k_max_top_final(unsigned int* __restrict__ top, uint32_t ntop) {
-  __shared__ uint32_t wtop[4];
-  uint32_t tl = 0;
-  for (uint32_t i = threadIdx.x; i < ntop; i += blockDim.x) tl = max(tl, top[i]);
-  tl = wave_max_u32(tl);
-  if (__lane_id() == 0) wtop[threadIdx.x >> 6] = tl;
-  __syncthreads();
-  if (threadIdx.x == 0) top[ntop] = max(max(wtop[0], wtop[1]), max(wtop[2], wtop[3]));
-}
-
 __global__ __launch_bounds__(256) void k_max_refine(ChunkView cv, const uint8_t* __restrict__ tile_max,
-                                                    const unsigned int* __restrict__ top, uint32_t ntop,
+                                                    const unsigned int* __restrict__ slots,
                                                     unsigned long long* __restrict__ best_out) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = wave_global();
   const uint64_t nwaves = waves_in_grid();
   const bool has_planes = cv.P > 0;
-  const uint32_t t = has_planes ? top[ntop] : 0u;
+  const uint32_t t = has_planes ? wave_max_u32_dpp(slots[lane]) : 0u;   // one slot per lane
   uint64_t best = 0;
   bool have = false;
   // 64 tiles per probe: one byte per lane
@@ -1114,18 +1107,17 @@ hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t
   }
 }
 
-uint64_t max_scratch_bytes(uint64_t ntiles) { return ((ntiles + 255) / 256) * 256 + 4 * 8192 + 256; }
+uint64_t max_scratch_bytes(uint64_t ntiles) { return ((ntiles + 255) / 256) * 256; }
 
-// scratch: per-tile maxima (ntiles bytes, padded), then one top byte per
-// pass-1 workgroup; the caller zeroes `best`.
+// scratch: per-tile maxima (ntiles bytes, padded); `best` is followed by the
+// 64 u32 slots of pass 1 (kMaxBestBytes in all, zeroed by the caller).
 template <int U>
-static uint32_t launch_max_top_u(const ChunkView& cv, uint8_t* scratch, unsigned int* top, hipStream_t s,
-                                 const LaunchShape& ls) {
+static void launch_max_top_u(const ChunkView& cv, uint8_t* scratch, unsigned int* slots, hipStream_t s,
+                             const LaunchShape& ls) {
   const uint32_t block = 64 * ls.waves_per_wg;
   const uint32_t wg = std::min<uint32_t>(
       grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, cap_of(ls, resident_wgs(k_max_top<U>, block))), 8192);
-  launch_k((k_max_top<U>), dim3(wg), dim3(block), 0, s, cv, scratch, top);
-  return wg;
+  launch_k((k_max_top<U>), dim3(wg), dim3(block), 0, s, cv, scratch, slots);
 }
 
 template <int U>
@@ -1147,19 +1139,16 @@ hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* sc
     return hipGetLastError();
   }
   // two passes: every tile's top byte, then the tiles holding the largest
-  unsigned int* top = reinterpret_cast<unsigned int*>(scratch + ((cv.ntiles + 255) / 256) * 256);
-  uint32_t wg1 = 0;
+  unsigned int* slots = reinterpret_cast<unsigned int*>(best + 1);
   if (cv.P > 0) {
     switch (ls.tiles_per_wave) {
-      case 4: wg1 = launch_max_top_u<4>(cv, scratch, top, s, ls); break;
-      case 16: wg1 = launch_max_top_u<16>(cv, scratch, top, s, ls); break;
-      default: wg1 = launch_max_top_u<8>(cv, scratch, top, s, ls); break;
+      case 4: launch_max_top_u<4>(cv, scratch, slots, s, ls); break;
+      case 16: launch_max_top_u<16>(cv, scratch, slots, s, ls); break;
+      default: launch_max_top_u<8>(cv, scratch, slots, s, ls); break;
     }
-    launch_k(k_max_top_final, dim3(1), dim3(256), 0, s, top, wg1);
   }
   const uint32_t wg2 = std::min(grid_for((cv.ntiles + 63) / 64, 4, 1024), resident_wgs(k_max_refine, 256));
-  launch_k(k_max_refine, dim3(wg2), dim3(256), 0, s, cv, (const uint8_t*)scratch, (const unsigned int*)top,
-                     wg1, best);
+  launch_k(k_max_refine, dim3(wg2), dim3(256), 0, s, cv, (const uint8_t*)scratch, (const unsigned int*)slots, best);
   return hipGetLastError();
 }
 
