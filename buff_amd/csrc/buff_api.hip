@@ -717,6 +717,10 @@ int materialize_cond_device(Ctx* ctx, const Chunk* ch, uint32_t flags, const uin
           HIP_TRY(launch_cond_values((const double*)ctx->s_condval.ptr, ch->n, cond, out, cap, scr, ctx->stream));
         }
         BUFF_TRY(d2h_small(ctx, &total, scr + cond_total_offset(ntiles), 8));
+        // the copy above waited for the gather: the decoded column goes back to
+        // the context's block cache (reclaimable under the HBM budget)
+        BUFF_TRY(ctx->release(ctx->s_condval.ptr));
+        ctx->s_condval = Scratch{};
       }
       if (rows) *rows = total;
       if (total > cap)
