@@ -1230,6 +1230,26 @@ def main():
                        "achieved_GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4),
                        "note": "10% filter" if k == "k_filter_gt" else ("full precision" if k == "k_materialize"
                                                                        else "")}
+    # the device's streaming read + write rate beside the kernels that read and
+    # write: a plain non-temporal copy of the walk column's float64 (n x 8
+    # bytes) into the materialize output, warm and after a cache eviction
+    stream_copy = {"bytes_per_launch": 2 * n * 8, "note": "k_copy_bytes: n x 8 bytes read + n x 8 bytes written"}
+    for label, evict in (("warm", False), ("cold", True)):
+        ctx.prof_reset()
+        for trial in range(trials + 1):   # the first launch is not timed
+            if evict:
+                flush.touch()
+            ctx.synchronize()
+            ctx.prof_enable(trial > 0)
+            out.copy_from(dx[0], n * 8)
+            ctx.synchronize()
+            ctx.prof_enable(False)
+        launches, total_ms = ctx.prof_query("k_copy_bytes")
+        if launches:
+            us = total_ms / launches * 1e3
+            gbps = 2.0 * n * 8 / (us * 1e-6) / 1e9
+            stream_copy[label] = {"avg_us": round(us, 2), "achieved_GBps": round(gbps, 1),
+                                  "frac": round(gbps / HBM_PEAK_GBPS, 4)}
     filter_project = bench_filter_project(ctx, cols[0], keep[0], infos[0], words, out, flush, buff_amd)
     window_max = bench_window_max(ctx, cols[0], keep[0], infos[0], words, out, flush, buff_amd)
     slice_sec = bench_slice(ctx, cols[0], dx[0], words, out, flush, buff_amd)
@@ -1264,6 +1284,7 @@ def main():
         "launch_shapes": tuned,
         "cold_cache": cold,
         "cold_filters": filters,
+        "stream_copy": stream_copy,
         "filter_project": filter_project,
         "window_max": window_max,
         "slice": slice_sec,

@@ -149,6 +149,7 @@ SIGNATURES = {
     "buff_memcpy_d2h": [vp, vp, vp, u64],
     "buff_dev_fill": [vp, vp, ctypes.c_uint8, u64],
     "buff_dev_touch": [vp, vp, u64],
+    "buff_dev_copy": [vp, vp, vp, u64],
     "buff_prof_enable": [vp, cint],
     "buff_prof_reset": [vp],
     "buff_prof_query": [vp, ctypes.c_char_p, u64p, f64p],
@@ -613,6 +614,13 @@ class DeviceBuffer:
     def touch(self):
         """Reads the buffer on the device (clean-line cache eviction)."""
         check(lib().buff_dev_touch(self.ctx.handle, self.ptr, self.nbytes))
+
+    def copy_from(self, src, nbytes=None):
+        """Device-to-device streaming copy of nbytes (default: all of src) into this buffer."""
+        nbytes = src.nbytes if nbytes is None else nbytes
+        if nbytes > self.nbytes or nbytes > src.nbytes:
+            raise ValueError("copy of %d bytes between buffers of %d and %d" % (nbytes, src.nbytes, self.nbytes))
+        check(lib().buff_dev_copy(self.ctx.handle, self.ptr, src.ptr, nbytes))
 
     def free(self):
         if self.ptr:

@@ -155,6 +155,17 @@ int Ctx::release(void* p) {
   return BUFF_OK;
 }
 
+int Ctx::max_slots(unsigned int** slots, uint32_t* tag) {
+  if (!s_maxslot.ptr || max_tag + 1 >= kMaxTagLimit) {
+    BUFF_TRY(scratch(s_maxslot, kMaxSlotBytes));
+    HIP_TRY(launch_fill_bytes(s_maxslot.ptr, kMaxSlotBytes, 0, stream));
+    max_tag = 0;
+  }
+  *tag = ++max_tag;
+  *slots = static_cast<unsigned int*>(s_maxslot.ptr);
+  return BUFF_OK;
+}
+
 int Ctx::scratch(Scratch& s, uint64_t bytes) {
   if (s.bytes >= bytes) return BUFF_OK;
   if (s.ptr) {
@@ -930,14 +941,16 @@ int max_op(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t* m
   uint64_t best = 0;
   const ChunkView& v = ch->view;
   if (ch->n > 0 && (v.P || v.r)) {
-    // [best u64 | 64 u32 slots | pad to 512 | per-tile maxima]
+    // [best u64 | pad to 512 | per-tile maxima]; the kernels clear best
     const uint64_t ms = max_scratch_bytes(v.ntiles);
     BUFF_TRY(ctx->scratch(ctx->s_best, 512 + ms));
     uint8_t* sb = (uint8_t*)ctx->s_best.ptr;
-    HIP_TRY(hipMemsetAsync(sb, 0, kMaxBestBytes, ctx->stream));
+    unsigned int* slots = nullptr;
+    uint32_t tag = 0;
+    BUFF_TRY(ctx->max_slots(&slots, &tag));
     {
       ProfScope sc(ctx, "k_max");
-      HIP_TRY(launch_max(v, (unsigned long long*)sb, sb + 512, ctx->stream, ctx->shape(Op::Max, v)));
+      HIP_TRY(launch_max(v, (unsigned long long*)sb, sb + 512, slots, tag, ctx->stream, ctx->shape(Op::Max, v)));
     }
     BUFF_TRY(d2h_small(ctx, &best, sb, 8));
   }
@@ -1248,7 +1261,6 @@ int tune_chunk(Ctx* ctx, const Chunk* ch, double pred, buff_tune_choice* out) {
   BUFF_TRY(ctx->scratch(ctx->s_bitmap, nwords * 8));
   BUFF_TRY(ctx->scratch(ctx->s_sums, 9 * 8 + sum_scratch_bytes(probe)));
   BUFF_TRY(ctx->scratch(ctx->s_best, 512 + max_scratch_bytes(v.ntiles)));
-  HIP_TRY(hipMemsetAsync(ctx->s_best.ptr, 0, kMaxBestBytes, ctx->stream));   // the runs keep folding this chunk's max
   BUFF_TRY(ctx->scratch(ctx->s_values, ch->n * 8));
   BUFF_TRY(ctx->scratch(ctx->s_sink, 256));
   void* flush = nullptr;
@@ -1267,7 +1279,10 @@ int tune_chunk(Ctx* ctx, const Chunk* ch, double pred, buff_tune_choice* out) {
           return launch_sum(v, (unsigned long long*)ctx->s_sums.ptr, ctx->stream, ls);
         case Op::Max: {
           uint8_t* sb = (uint8_t*)ctx->s_best.ptr;
-          return launch_max(v, (unsigned long long*)sb, sb + 512, ctx->stream, ls);
+          unsigned int* slots = nullptr;
+          uint32_t tag = 0;
+          if (ctx->max_slots(&slots, &tag) != BUFF_OK) return hipErrorUnknown;
+          return launch_max(v, (unsigned long long*)sb, sb + 512, slots, tag, ctx->stream, ls);
         }
         default:
           return launch_materialize(v, mp, (double*)ctx->s_values.ptr, ctx->stream, ls);
@@ -1410,6 +1425,19 @@ int buff_dev_fill(buff_ctx* ctx, void* dptr, uint8_t value, uint64_t bytes) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   ProfScope ps(c, "k_fill_bytes");
   HIP_TRY(launch_fill_bytes(dptr, bytes, value, c->stream));
+  return BUFF_OK;
+}
+
+int buff_dev_copy(buff_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(BUFF_E_INVALID, "null argument");
+  if (bytes % 16 || (uintptr_t)dst % 16 || (uintptr_t)src % 16)
+    return fail(BUFF_E_INVALID, "copy of %llu bytes: sizes and addresses must be multiples of 16",
+                (unsigned long long)bytes);
+  const uintptr_t d = (uintptr_t)dst, sp = (uintptr_t)src;
+  if (bytes && d < sp + bytes && sp < d + bytes) return fail(BUFF_E_INVALID, "overlapping copy");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  ProfScope ps(c, "k_copy_bytes");
+  HIP_TRY(launch_copy_bytes(dst, src, bytes, c->stream));
   return BUFF_OK;
 }
 

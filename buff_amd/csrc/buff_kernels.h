@@ -54,6 +54,8 @@ hipError_t launch_filter_cond(const ChunkView& cv, int op, const FilterTarget& t
 hipError_t launch_cond_fill(uint64_t* out, const uint64_t* cond, uint64_t nwords, uint64_t n, int copy,
                             hipStream_t s);
 hipError_t launch_fill_bytes(void* out, uint64_t bytes, uint8_t value, hipStream_t s);
+// bytes: a multiple of 16; in and out 16-byte aligned and not overlapping
+hipError_t launch_copy_bytes(void* out, const void* in, uint64_t bytes, hipStream_t s);
 hipError_t launch_touch_bytes(const void* in, uint64_t bytes, uint32_t* sink, hipStream_t s);
 // sums: 9 u64 results followed by sum_scratch_bytes(ls) of per-workgroup rows
 // (at most kMaxSumRows workgroups).
@@ -64,13 +66,16 @@ hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t
 // the sum of the n q-bit fields the reference reads from the packed tail,
 // out[1] += the sum of every row's top q tail bits (the caller zeroes out[0..1]).
 hipError_t launch_tail_fields(const ChunkView& cv, uint32_t q, unsigned long long* out, hipStream_t s);
-// Two passes using `scratch` (max_scratch_bytes(ntiles) bytes) and 64 u32 slots
-// right after `best`, or one pass (ls.prefetch == 1); the caller zeroes the
-// kMaxBestBytes bytes at `best`.
-constexpr uint64_t kMaxBestBytes = 8 + 64 * 4;
+// Two passes using `scratch` (max_scratch_bytes(ntiles) bytes) and `slots`
+// (kMaxSlotBytes), or one pass (ls.prefetch == 1). The max offset is left in
+// best[0] (16 bytes at `best` may be written). A slot holds (tag << 8) | byte:
+// `tag` must exceed every tag given with the same slots before, and be below
+// kMaxTagLimit; the slots are zeroed once before their first tag.
+constexpr uint64_t kMaxSlotBytes = 64 * 4;
+constexpr uint32_t kMaxTagLimit = 1u << 24;
 uint64_t max_scratch_bytes(uint64_t ntiles);
-hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* scratch, hipStream_t s,
-                      const LaunchShape& ls);
+hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* scratch, unsigned int* slots,
+                      uint32_t tag, hipStream_t s, const LaunchShape& ls);
 hipError_t launch_materialize(const ChunkView& cv, const MatParams& mp, double* out, hipStream_t s,
                               const LaunchShape& ls);
 hipError_t launch_serialize(const ChunkView& cv, uint8_t flipmask, uint8_t* body, uint64_t tail_bytes,

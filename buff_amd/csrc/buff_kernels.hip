@@ -338,6 +338,29 @@ __global__ void k_fill_bytes(uint4* __restrict__ out, uint64_t n16, uint32_t v) 
     out[i] = w;
 }
 
+// Copies n16 16-byte words with non-temporal loads and stores: the device's
+// streaming read + write rate, the ceiling of the kernels that read and write
+// (encode, materialize). A workgroup moves 16 KiB contiguous per step, four
+// loads per thread in flight (faster than a grid-strided copy: 0.74 of peak
+// against 0.63).
+__global__ __launch_bounds__(256) void k_copy_bytes(const u32x4_t* __restrict__ in, u32x4_t* __restrict__ out,
+                                                    uint64_t n16) {
+  constexpr uint64_t kPer = 4 * 256;
+  for (uint64_t b = (uint64_t)blockIdx.x * kPer; b < n16; b += (uint64_t)gridDim.x * kPer) {
+    u32x4_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint64_t i = b + k * 256 + threadIdx.x;
+      if (i < n16) v[k] = __builtin_nontemporal_load(gmem(in) + i);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint64_t i = b + k * 256 + threadIdx.x;
+      if (i < n16) __builtin_nontemporal_store(v[k], gmem_w(out) + i);
+    }
+  }
+}
+
 // Reads `bytes` (a multiple of 16) and keeps nothing: evicts the memory-side
 // cache with clean lines before a cold measurement.
 __global__ void k_touch_bytes(const uint4* __restrict__ in, uint64_t n16, uint32_t* __restrict__ sink) {
@@ -524,15 +547,20 @@ hipError_t launch_tail_fields(const ChunkView& cv, uint32_t q, unsigned long lon
 // records each tile's largest byte and folds each workgroup's into one of 64
 // slots (atomicMax, workgroup w into slot w % 64: same-address device atomics
 // serialize); pass 2 takes the max of the slots and rebuilds whole offsets only
-// in tiles whose largest byte equals it.
+// in tiles whose largest byte equals it. A slot holds (tag << 8) | byte, where
+// the tag grows by one per call: the values a call writes exceed every value
+// of earlier calls, so the slots are never cleared between calls. Pass 1 also
+// clears *best before pass 2 folds into it.
 
 
 constexpr uint32_t kMaxSlots = 64;
 
 template <int U>
 __global__ __launch_bounds__(256) void k_max_top(ChunkView cv, uint8_t* __restrict__ tile_max,
-                                                 unsigned int* __restrict__ slots) {
+                                                 unsigned int* __restrict__ slots, uint32_t tag,
+                                                 unsigned long long* __restrict__ best_out) {
   const uint32_t lane = __lane_id();
+  if (blockIdx.x == 0 && threadIdx.x == 0) *best_out = 0ull;
   const uint64_t wave = wave_global();
   const uint64_t nwaves = waves_in_grid();
   const uint64_t ngroups = (cv.ntiles + U - 1) / U;
@@ -558,18 +586,22 @@ __global__ __launch_bounds__(256) void k_max_top(ChunkView cv, uint8_t* __restri
   if (threadIdx.x == 0) {
     uint32_t b = 0;
     for (uint32_t k = 0; k < (blockDim.x >> 6); ++k) b = max(b, wbest[k]);
-    if (b) atomicMax(&slots[blockIdx.x % kMaxSlots], b);
+    atomicMax(&slots[blockIdx.x % kMaxSlots], (tag << 8) | b);
   }
 }
 
 __global__ __launch_bounds__(256) void k_max_refine(ChunkView cv, const uint8_t* __restrict__ tile_max,
-                                                    const unsigned int* __restrict__ slots,
+                                                    const unsigned int* __restrict__ slots, uint32_t tag,
                                                     unsigned long long* __restrict__ best_out) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = wave_global();
   const uint64_t nwaves = waves_in_grid();
   const bool has_planes = cv.P > 0;
-  const uint32_t t = has_planes ? wave_max_u32_dpp(slots[lane]) : 0u;   // one slot per lane
+  uint32_t t = 0;
+  if (has_planes) {
+    const uint32_t m = wave_max_u32_dpp(slots[lane]);   // one slot per lane
+    t = (m >> 8) == tag ? (m & 0xFFu) : 0u;             // every workgroup of pass 1 wrote this call's tag
+  }
   uint64_t best = 0;
   bool have = false;
   // 64 tiles per probe: one byte per lane
@@ -581,18 +613,29 @@ __global__ __launch_bounds__(256) void k_max_refine(ChunkView cv, const uint8_t*
       const uint32_t k = (uint32_t)__builtin_ctzll(hits);
       hits &= hits - 1;
       const uint64_t ct = base + k;
+      // every plane and tail bit-plane of the tile in flight together
+      uint4 wp[kMaxPlanes];
+      uint32_t th[7];
+#pragma unroll
+      for (uint32_t p = 0; p < kMaxPlanes; ++p) wp[p] = p < cv.P ? load_plane(cv, p, ct, lane) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+      for (uint32_t j = 0; j < 7; ++j) th[j] = j < cv.r ? load_tail(cv, j, ct, lane) : 0u;
       uint64_t off[16];
 #pragma unroll
       for (uint32_t b = 0; b < 16; ++b) off[b] = 0;
-      for (uint32_t p = 0; p < cv.P; ++p) {
-        const uint4 w = load_plane(cv, p, ct, lane);
 #pragma unroll
-        for (uint32_t b = 0; b < 16; ++b) off[b] = (off[b] << 8) | byte_at(w, b);
+      for (uint32_t p = 0; p < kMaxPlanes; ++p) {
+        if (p < cv.P) {
+#pragma unroll
+          for (uint32_t b = 0; b < 16; ++b) off[b] = (off[b] << 8) | byte_at(wp[p], b);
+        }
       }
-      for (uint32_t j0 = cv.r; j0 > 0; --j0) {
-        const uint32_t h = load_tail(cv, j0 - 1, ct, lane);
 #pragma unroll
-        for (uint32_t b = 0; b < 16; ++b) off[b] = (off[b] << 1) | ((h >> b) & 1u);
+      for (int j = 6; j >= 0; --j) {
+        if ((uint32_t)j < cv.r) {
+#pragma unroll
+          for (uint32_t b = 0; b < 16; ++b) off[b] = (off[b] << 1) | ((th[j] >> b) & 1u);
+        }
       }
       uint64_t m = 0;
 #pragma unroll
@@ -1045,6 +1088,14 @@ hipError_t launch_fill_bytes(void* out, uint64_t bytes, uint8_t value, hipStream
   return hipGetLastError();
 }
 
+hipError_t launch_copy_bytes(void* out, const void* in, uint64_t bytes, hipStream_t s) {
+  if (bytes % 16) return hipErrorInvalidValue;
+  const uint64_t n16 = bytes / 16;
+  const uint64_t wg = std::min<uint64_t>(std::max<uint64_t>((n16 + 1023) / 1024, 1), 16384);
+  if (n16) launch_k(k_copy_bytes, dim3((uint32_t)wg), dim3(256), 0, s, (const u32x4_t*)in, (u32x4_t*)out, n16);
+  return hipGetLastError();
+}
+
 hipError_t launch_touch_bytes(const void* in, uint64_t bytes, uint32_t* sink, hipStream_t s) {
   const uint64_t n16 = bytes / 16;
   if (n16) launch_k(k_touch_bytes, dim3(2048), dim3(256), 0, s, (const uint4*)in, n16, sink);
@@ -1112,12 +1163,12 @@ uint64_t max_scratch_bytes(uint64_t ntiles) { return ((ntiles + 255) / 256) * 25
 // scratch: per-tile maxima (ntiles bytes, padded); `best` is followed by the
 // 64 u32 slots of pass 1 (kMaxBestBytes in all, zeroed by the caller).
 template <int U>
-static void launch_max_top_u(const ChunkView& cv, uint8_t* scratch, unsigned int* slots, hipStream_t s,
-                             const LaunchShape& ls) {
+static void launch_max_top_u(const ChunkView& cv, uint8_t* scratch, unsigned int* slots, uint32_t tag,
+                             unsigned long long* best, hipStream_t s, const LaunchShape& ls) {
   const uint32_t block = 64 * ls.waves_per_wg;
   const uint32_t wg = std::min<uint32_t>(
       grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, cap_of(ls, resident_wgs(k_max_top<U>, block))), 8192);
-  launch_k((k_max_top<U>), dim3(wg), dim3(block), 0, s, cv, scratch, slots);
+  launch_k((k_max_top<U>), dim3(wg), dim3(block), 0, s, cv, scratch, slots, tag, best);
 }
 
 template <int U>
@@ -1128,8 +1179,12 @@ static void launch_max_fused_u(const ChunkView& cv, unsigned long long* best, hi
   launch_k((k_max_fused<U>), dim3(wg), dim3(block), 0, s, cv, best);
 }
 
-hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* scratch, hipStream_t s,
-                      const LaunchShape& ls) {
+hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* scratch, unsigned int* slots,
+                      uint32_t tag, hipStream_t s, const LaunchShape& ls) {
+  if (ls.prefetch == 1 || cv.P == 0) {   // no pass 1 to clear *best
+    const hipError_t e = launch_fill_bytes(best, 16, 0, s);
+    if (e != hipSuccess) return e;
+  }
   if (ls.prefetch == 1) {   // one pass
     switch (ls.tiles_per_wave) {
       case 4: launch_max_fused_u<4>(cv, best, s, ls); break;
@@ -1139,16 +1194,16 @@ hipError_t launch_max(const ChunkView& cv, unsigned long long* best, uint8_t* sc
     return hipGetLastError();
   }
   // two passes: every tile's top byte, then the tiles holding the largest
-  unsigned int* slots = reinterpret_cast<unsigned int*>(best + 1);
   if (cv.P > 0) {
     switch (ls.tiles_per_wave) {
-      case 4: launch_max_top_u<4>(cv, scratch, slots, s, ls); break;
-      case 16: launch_max_top_u<16>(cv, scratch, slots, s, ls); break;
-      default: launch_max_top_u<8>(cv, scratch, slots, s, ls); break;
+      case 4: launch_max_top_u<4>(cv, scratch, slots, tag, best, s, ls); break;
+      case 16: launch_max_top_u<16>(cv, scratch, slots, tag, best, s, ls); break;
+      default: launch_max_top_u<8>(cv, scratch, slots, tag, best, s, ls); break;
     }
   }
   const uint32_t wg2 = std::min(grid_for((cv.ntiles + 63) / 64, 4, 1024), resident_wgs(k_max_refine, 256));
-  launch_k(k_max_refine, dim3(wg2), dim3(256), 0, s, cv, (const uint8_t*)scratch, (const unsigned int*)slots, best);
+  launch_k(k_max_refine, dim3(wg2), dim3(256), 0, s, cv, (const uint8_t*)scratch, (const unsigned int*)slots, tag,
+           best);
   return hipGetLastError();
 }
 

@@ -70,7 +70,8 @@ struct Ctx {
   void* pinned = nullptr;
 
   Scratch s_partial, s_minmax, s_stream, s_bitmap, s_sums, s_best, s_values, s_sink, s_major, s_cond, s_condin, s_wmax, s_slice, s_csv, s_csvtab,
-      s_csvtmp, s_csvslow, s_condval;
+      s_csvtmp, s_csvslow, s_condval, s_maxslot;
+  uint32_t max_tag = 0;        // the last tag given with s_maxslot (launch_max)
   bool csv_tab_ready = false;  // s_csvtab holds the table of 5^q
 
   bool prof_on = false;
@@ -87,6 +88,8 @@ struct Ctx {
   int alloc(uint64_t bytes, void** out);
   int release(void* p);
   int scratch(Scratch& s, uint64_t bytes);
+  // max's slots and the next tag (slots zeroed on first use and when tags run out)
+  int max_slots(unsigned int** slots, uint32_t* tag);
   void prof_begin(const char* name, hipEvent_t* ev, bool kernels);
   void prof_end(const char* name, hipEvent_t* ev, bool kernels);
   int prof_collect();

@@ -130,6 +130,11 @@ int buff_dev_fill(buff_ctx* ctx, void* dptr, uint8_t value, uint64_t bytes);
 /* Reads device bytes and discards them (no host sync): refills the memory-side
  * cache with clean lines before a cold-cache measurement. */
 int buff_dev_touch(buff_ctx* ctx, const void* dptr, uint64_t bytes);
+/* Copies device bytes with non-temporal loads and stores on the context's
+ * stream (no host sync): the streaming read + write rate that encode and
+ * materialize are measured against. bytes, dst and src multiples of 16; the
+ * ranges must not overlap. No reference counterpart. */
+int buff_dev_copy(buff_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 
 /* Per-kernel timing with HIP events recorded on the context's stream.
  * buff_prof_query reports launches and summed durations for one kernel name. */
