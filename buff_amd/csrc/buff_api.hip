@@ -166,6 +166,18 @@ int Ctx::max_slots(unsigned int** slots, uint32_t* tag) {
   return BUFF_OK;
 }
 
+int Ctx::sum_slots(unsigned long long** set, unsigned long long** next) {
+  if (!s_sumslot.ptr) {
+    BUFF_TRY(scratch(s_sumslot, 2ull * kSumSetWords * 8));
+    HIP_TRY(launch_fill_bytes(s_sumslot.ptr, 2ull * kSumSetWords * 8, 0, stream));
+    sum_set = 0;
+  }
+  unsigned long long* base = static_cast<unsigned long long*>(s_sumslot.ptr);
+  *set = base + (uint64_t)sum_set * kSumSetWords;
+  *next = base + (uint64_t)(sum_set ^ 1u) * kSumSetWords;
+  return BUFF_OK;
+}
+
 int Ctx::scratch(Scratch& s, uint64_t bytes) {
   if (s.bytes >= bytes) return BUFF_OK;
   if (s.ptr) {
@@ -910,13 +922,19 @@ int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64
   uint64_t ps[kMaxPlanes + 1] = {0};
   const ChunkView& v = ch->view;
   if (ch->n > 0 && (v.P || v.r)) {
+    // one launch into a slot set ([k][slot] u64), folded here
     const LaunchShape ls = ctx->shape(Op::Sum, v);
-    BUFF_TRY(ctx->scratch(ctx->s_sums, sizeof(ps) + sum_scratch_bytes(ls)));
+    unsigned long long *set = nullptr, *next = nullptr;
+    BUFF_TRY(ctx->sum_slots(&set, &next));
     {
       ProfScope sc(ctx, "k_sum");
-      HIP_TRY(launch_sum(v, (unsigned long long*)ctx->s_sums.ptr, ctx->stream, ls));
+      HIP_TRY(launch_sum_slots(v, set, next, ctx->stream, ls));
+      ctx->sum_slots_used();
     }
-    BUFF_TRY(d2h_small(ctx, ps, ctx->s_sums.ptr, sizeof(ps)));
+    uint64_t slots[(kMaxPlanes + 1) * kSumSlots];
+    BUFF_TRY(d2h_small(ctx, slots, set, 8ull * (v.P + 1) * kSumSlots));
+    for (uint32_t k = 0; k <= v.P; ++k)
+      for (uint32_t i = 0; i < kSumSlots; ++i) ps[k] += slots[k * kSumSlots + i];
   }
   if (plane_sums) memcpy(plane_sums, ps, sizeof(ps));
   if (ref_sum) *ref_sum = reference_sum_fold(ch, ps);
@@ -1275,8 +1293,13 @@ int tune_chunk(Ctx* ctx, const Chunk* ch, double pred, buff_tune_choice* out) {
       switch (op) {
         case Op::FilterGt:
           return launch_filter(v, 0, tg, (uint64_t*)ctx->s_bitmap.ptr, nwords, nullptr, ctx->stream, ls);
-        case Op::Sum:
-          return launch_sum(v, (unsigned long long*)ctx->s_sums.ptr, ctx->stream, ls);
+        case Op::Sum: {
+          unsigned long long *set = nullptr, *next = nullptr;
+          if (ctx->sum_slots(&set, &next) != BUFF_OK) return hipErrorUnknown;
+          const hipError_t e = launch_sum_slots(v, set, next, ctx->stream, ls);
+          if (e == hipSuccess) ctx->sum_slots_used();
+          return e;
+        }
         case Op::Max: {
           uint8_t* sb = (uint8_t*)ctx->s_best.ptr;
           unsigned int* slots = nullptr;

@@ -62,6 +62,14 @@ hipError_t launch_touch_bytes(const void* in, uint64_t bytes, uint32_t* sink, hi
 constexpr uint32_t kMaxSumRows = 8192;
 uint64_t sum_scratch_bytes(const LaunchShape& ls);
 hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls);
+// One launch: per-plane sums (and the weighted tail sum at k = P) added into
+// the slot set `set` (u64 [kMaxPlanes + 1][kSumSlots], to be summed over the
+// slots); `next`, the set the following call adds into, is cleared. The first
+// set must be zero.
+constexpr uint32_t kSumSlots = 64;
+constexpr uint32_t kSumSetWords = (kMaxPlanes + 1) * kSumSlots;
+hipError_t launch_sum_slots(const ChunkView& cv, unsigned long long* set, unsigned long long* next, hipStream_t s,
+                            const LaunchShape& ls);
 // Tail part of the reduced-precision sum reading q < r bits per row: out[0] +=
 // the sum of the n q-bit fields the reference reads from the packed tail,
 // out[1] += the sum of every row's top q tail bits (the caller zeroes out[0..1]).

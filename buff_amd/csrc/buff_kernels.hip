@@ -378,10 +378,15 @@ __global__ void k_touch_bytes(const uint4* __restrict__ in, uint64_t n16, uint32
 // their bit position. P (planes) and R (tail bits) are template parameters and
 // every load of a step is unconditional (the last group clamps its tile index
 // and skips the duplicate), so all of them are in flight together. Each
-// workgroup writes one row of partial sums; k_sum_final folds the rows.
+// workgroup writes one row of partial sums, which k_sum_final folds; or, with
+// `next` given (one launch), adds its row into one of kSumSlots slot rows of
+// `rows` (u64 atomics, workgroup w into slot w % kSumSlots, laid out
+// [k][slot]) for the host to fold, and workgroup 0 clears `next`, the set the
+// following call adds into.
 
 template <int P, int R, int U>
-__global__ __launch_bounds__(256) void k_sum(ChunkView cv, unsigned long long* __restrict__ rows) {
+__global__ __launch_bounds__(256) void k_sum(ChunkView cv, unsigned long long* __restrict__ rows,
+                                             unsigned long long* __restrict__ next) {
   const uint32_t lane = __lane_id();
   const uint64_t wave = wave_global();
   const uint64_t nwaves = waves_in_grid();
@@ -448,8 +453,13 @@ __global__ __launch_bounds__(256) void k_sum(ChunkView cv, unsigned long long* _
   if (threadIdx.x <= (uint32_t)P) {
     uint64_t v = 0;
     for (uint32_t k = 0; k < (blockDim.x >> 6); ++k) v += wrow[k][threadIdx.x];
-    rows[(uint64_t)blockIdx.x * (kMaxPlanes + 1) + threadIdx.x] = v;
+    if (!next)
+      rows[(uint64_t)blockIdx.x * (kMaxPlanes + 1) + threadIdx.x] = v;
+    else if (v)
+      atomicAdd(&rows[threadIdx.x * kSumSlots + blockIdx.x % kSumSlots], (unsigned long long)v);
   }
+  if (next && blockIdx.x == 0)
+    for (uint32_t i = threadIdx.x; i < kSumSetWords; i += blockDim.x) next[i] = 0ull;
 }
 
 // Sums the per-workgroup rows of k_sum (one workgroup; each thread keeps its
@@ -1102,24 +1112,32 @@ hipError_t launch_touch_bytes(const void* in, uint64_t bytes, uint32_t* sink, hi
   return hipGetLastError();
 }
 
+// next == nullptr: k_sum's rows after sums[0 .. kMaxPlanes], folded by
+// k_sum_final into sums; else one launch adding into the slot set `sums`.
 template <int P, int R, int U>
-static void launch_sum_pru(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
+static void launch_sum_pru(const ChunkView& cv, unsigned long long* sums, unsigned long long* next, hipStream_t s,
+                           const LaunchShape& ls) {
   const uint32_t block = 64 * ls.waves_per_wg;
   const uint32_t wg = std::min<uint32_t>(
       grid_for((cv.ntiles + U - 1) / U, ls.waves_per_wg, cap_of(ls, resident_wgs(k_sum<P, R, U>, block))),
       kMaxSumRows);
+  if (next) {
+    launch_k((k_sum<P, R, U>), dim3(wg), dim3(block), 0, s, cv, sums, next);
+    return;
+  }
   unsigned long long* rows = sums + (kMaxPlanes + 1);
-  launch_k((k_sum<P, R, U>), dim3(wg), dim3(block), 0, s, cv, rows);
+  launch_k((k_sum<P, R, U>), dim3(wg), dim3(block), 0, s, cv, rows, (unsigned long long*)nullptr);
   launch_k((k_sum_final<P + 1>), dim3(1), dim3(256), 0, s, (const unsigned long long*)rows, wg, sums);
 }
 
 template <int P, int R>
-static void launch_sum_pr(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
+static void launch_sum_pr(const ChunkView& cv, unsigned long long* sums, unsigned long long* next, hipStream_t s,
+                          const LaunchShape& ls) {
   switch (ls.tiles_per_wave) {
-    case 1: launch_sum_pru<P, R, 1>(cv, sums, s, ls); break;
-    case 2: launch_sum_pru<P, R, 2>(cv, sums, s, ls); break;
-    case 4: launch_sum_pru<P, R, 4>(cv, sums, s, ls); break;
-    default: launch_sum_pru<P, R, (P + R) <= 3 ? 4 : 2>(cv, sums, s, ls); break;
+    case 1: launch_sum_pru<P, R, 1>(cv, sums, next, s, ls); break;
+    case 2: launch_sum_pru<P, R, 2>(cv, sums, next, s, ls); break;
+    case 4: launch_sum_pru<P, R, 4>(cv, sums, next, s, ls); break;
+    default: launch_sum_pru<P, R, (P + R) <= 3 ? 4 : 2>(cv, sums, next, s, ls); break;
   }
 }
 
@@ -1129,33 +1147,45 @@ uint64_t sum_scratch_bytes(const LaunchShape& ls) {
 }
 
 template <int P>
-static hipError_t launch_sum_p(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
+static hipError_t launch_sum_p(const ChunkView& cv, unsigned long long* sums, unsigned long long* next, hipStream_t s,
+                               const LaunchShape& ls) {
   switch (cv.r) {
-    case 0: launch_sum_pr<P, 0>(cv, sums, s, ls); break;
-    case 1: launch_sum_pr<P, 1>(cv, sums, s, ls); break;
-    case 2: launch_sum_pr<P, 2>(cv, sums, s, ls); break;
-    case 3: launch_sum_pr<P, 3>(cv, sums, s, ls); break;
-    case 4: launch_sum_pr<P, 4>(cv, sums, s, ls); break;
-    case 5: launch_sum_pr<P, 5>(cv, sums, s, ls); break;
-    case 6: launch_sum_pr<P, 6>(cv, sums, s, ls); break;
-    case 7: launch_sum_pr<P, 7>(cv, sums, s, ls); break;
+    case 0: launch_sum_pr<P, 0>(cv, sums, next, s, ls); break;
+    case 1: launch_sum_pr<P, 1>(cv, sums, next, s, ls); break;
+    case 2: launch_sum_pr<P, 2>(cv, sums, next, s, ls); break;
+    case 3: launch_sum_pr<P, 3>(cv, sums, next, s, ls); break;
+    case 4: launch_sum_pr<P, 4>(cv, sums, next, s, ls); break;
+    case 5: launch_sum_pr<P, 5>(cv, sums, next, s, ls); break;
+    case 6: launch_sum_pr<P, 6>(cv, sums, next, s, ls); break;
+    case 7: launch_sum_pr<P, 7>(cv, sums, next, s, ls); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
+static hipError_t launch_sum_any(const ChunkView& cv, unsigned long long* sums, unsigned long long* next,
+                                 hipStream_t s, const LaunchShape& ls) {
   switch (cv.P) {
-    case 0: return launch_sum_p<0>(cv, sums, s, ls);
-    case 1: return launch_sum_p<1>(cv, sums, s, ls);
-    case 2: return launch_sum_p<2>(cv, sums, s, ls);
-    case 3: return launch_sum_p<3>(cv, sums, s, ls);
-    case 4: return launch_sum_p<4>(cv, sums, s, ls);
-    case 5: return launch_sum_p<5>(cv, sums, s, ls);
-    case 6: return launch_sum_p<6>(cv, sums, s, ls);
-    case 7: return launch_sum_p<7>(cv, sums, s, ls);
+    case 0: return launch_sum_p<0>(cv, sums, next, s, ls);
+    case 1: return launch_sum_p<1>(cv, sums, next, s, ls);
+    case 2: return launch_sum_p<2>(cv, sums, next, s, ls);
+    case 3: return launch_sum_p<3>(cv, sums, next, s, ls);
+    case 4: return launch_sum_p<4>(cv, sums, next, s, ls);
+    case 5: return launch_sum_p<5>(cv, sums, next, s, ls);
+    case 6: return launch_sum_p<6>(cv, sums, next, s, ls);
+    case 7: return launch_sum_p<7>(cv, sums, next, s, ls);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_sum(const ChunkView& cv, unsigned long long* sums, hipStream_t s, const LaunchShape& ls) {
+  return launch_sum_any(cv, sums, nullptr, s, ls);
+}
+
+hipError_t launch_sum_slots(const ChunkView& cv, unsigned long long* set, unsigned long long* next, hipStream_t s,
+                            const LaunchShape& ls) {
+  if (!set || !next) return hipErrorInvalidValue;
+  return launch_sum_any(cv, set, next, s, ls);
 }
 
 uint64_t max_scratch_bytes(uint64_t ntiles) { return ((ntiles + 255) / 256) * 256; }

@@ -70,8 +70,9 @@ struct Ctx {
   void* pinned = nullptr;
 
   Scratch s_partial, s_minmax, s_stream, s_bitmap, s_sums, s_best, s_values, s_sink, s_major, s_cond, s_condin, s_wmax, s_slice, s_csv, s_csvtab,
-      s_csvtmp, s_csvslow, s_condval, s_maxslot;
+      s_csvtmp, s_csvslow, s_condval, s_maxslot, s_sumslot;
   uint32_t max_tag = 0;        // the last tag given with s_maxslot (launch_max)
+  uint32_t sum_set = 0;        // the s_sumslot set the next launch_sum_slots adds into
   bool csv_tab_ready = false;  // s_csvtab holds the table of 5^q
 
   bool prof_on = false;
@@ -90,6 +91,10 @@ struct Ctx {
   int scratch(Scratch& s, uint64_t bytes);
   // max's slots and the next tag (slots zeroed on first use and when tags run out)
   int max_slots(unsigned int** slots, uint32_t* tag);
+  // the sum's slot set for this call and the one it clears for the next
+  // (both zeroed on first use); sum_slots_used() after the launch went out
+  int sum_slots(unsigned long long** set, unsigned long long** next);
+  void sum_slots_used() { sum_set ^= 1u; }
   void prof_begin(const char* name, hipEvent_t* ev, bool kernels);
   void prof_end(const char* name, hipEvent_t* ev, bool kernels);
   int prof_collect();
