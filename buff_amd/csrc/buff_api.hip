@@ -178,6 +178,43 @@ int Ctx::sum_slots(unsigned long long** set, unsigned long long** next) {
   return BUFF_OK;
 }
 
+int Ctx::minmax_slots(int64_t** set, int64_t** next) {
+  if (!s_mmslot.ptr) {
+    BUFF_TRY(scratch(s_mmslot, 2ull * kMinMaxSetWords * 8));
+    HIP_TRY(launch_minmax_slots_init(static_cast<int64_t*>(s_mmslot.ptr), 2, stream));
+    mm_set = 0;
+  }
+  int64_t* base = static_cast<int64_t*>(s_mmslot.ptr);
+  *set = base + (uint64_t)mm_set * kMinMaxSetWords;
+  *next = base + (uint64_t)(mm_set ^ 1u) * kMinMaxSetWords;
+  return BUFF_OK;
+}
+
+int Ctx::minmax_pass(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, MinMaxOut* mm) {
+  const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
+  const uint32_t nwg = minmax_grid(ntiles);
+  BUFF_TRY(scratch(s_partial, 3ull * nwg * sizeof(int64_t)));
+  int64_t *set = nullptr, *next = nullptr;
+  BUFF_TRY(minmax_slots(&set, &next));
+  {
+    ProfScope ps(this, "k_minmax");
+    HIP_TRY(launch_minmax_slots(x, n, pexp, dlen, (int64_t*)s_partial.ptr, set, next, nwg, stream));
+    minmax_slots_used();
+  }
+  int64_t w[2 * kMinMaxSlots + 1];
+  BUFF_TRY(d2h_small(this, w, set, sizeof(w)));
+  int64_t a = INT64_MAX, b = INT64_MIN;
+  for (uint32_t i = 0; i < kMinMaxSlots; ++i) {
+    a = std::min(a, w[i]);
+    b = std::max(b, w[kMinMaxSlots + i]);
+  }
+  *mm = MinMaxOut{};
+  mm->min_all = a;
+  mm->max_rest = b;
+  mm->first = w[2 * kMinMaxSlots];
+  return BUFF_OK;
+}
+
 int Ctx::scratch(Scratch& s, uint64_t bytes) {
   if (s.bytes >= bytes) return BUFF_OK;
   if (s.ptr) {
@@ -310,18 +347,13 @@ int quirk_minmax(Ctx* ctx, const double* x, uint64_t n, int32_t pexp, uint32_t d
                  int64_t* mx_out) {
   int64_t mn = INT64_MAX, mx = INT64_MIN;
   if (n > 0) {
+    MinMaxOut mm;
+    BUFF_TRY(ctx->minmax_pass(x, n, pexp, dlen, &mm));
     const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
-    const uint32_t nwg = minmax_grid(ntiles);
-    BUFF_TRY(ctx->scratch(ctx->s_partial, 3ull * nwg * sizeof(int64_t)));
+    const uint32_t nwg = minmax_grid(ntiles);   // the grid the pass ran with: its partial rows
     BUFF_TRY(ctx->scratch(ctx->s_minmax, sizeof(MinMaxOut)));
     MinMaxOut* dmm = (MinMaxOut*)ctx->s_minmax.ptr;
     int64_t* part = (int64_t*)ctx->s_partial.ptr;
-    {
-      ProfScope ps(ctx, "k_minmax");
-      HIP_TRY(launch_minmax(x, n, pexp, dlen, part, part + 2 * nwg, dmm, nwg, false, ctx->stream));
-    }
-    MinMaxOut mm;
-    BUFF_TRY(d2h_small(ctx, &mm, dmm, sizeof(mm)));
     mn = mm.min_all;
     if (n == 1) {
       mx = mm.first == INT64_MAX ? INT64_MAX : INT64_MIN;

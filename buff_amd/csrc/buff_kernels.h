@@ -37,6 +37,16 @@ struct MatParams {
 uint32_t minmax_grid(uint64_t ntiles);
 hipError_t launch_minmax(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t* partial,
                          int64_t* partial2, MinMaxOut* out, uint32_t nwg, bool prefix, hipStream_t s);
+// One launch (n >= 1): the min over all rows and the max over rows >= 1 folded
+// into the slot set `set` (int64: kMinMaxSlots mins, kMinMaxSlots maxes, then
+// row 0's fixed value), for the host to fold; `next`, the set the following
+// call folds into, is reset; `partial` gets the per-workgroup rows the
+// prefix-min pass reads. Sets start reset (launch_minmax_slots_init).
+constexpr uint32_t kMinMaxSlots = 64;
+constexpr uint32_t kMinMaxSetWords = 2 * kMinMaxSlots + 8;
+hipError_t launch_minmax_slots(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t* partial,
+                               int64_t* set, int64_t* next, uint32_t nwg, hipStream_t s);
+hipError_t launch_minmax_slots_init(int64_t* sets, uint32_t nsets, hipStream_t s);
 hipError_t launch_encode(const double* x, const ChunkView& cv, int32_t pexp, hipStream_t s, const LaunchShape& ls);
 hipError_t launch_filter(const ChunkView& cv, int op, const FilterTarget& tg, uint64_t* out, uint64_t nwords,
                          unsigned long long* stats, hipStream_t s, const LaunchShape& ls);

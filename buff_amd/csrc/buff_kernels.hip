@@ -33,7 +33,8 @@ LaunchEvents& launch_events() {
 
 __global__ __launch_bounds__(256) void k_minmax(const double* __restrict__ x, uint64_t n, uint64_t ntiles,
                                                 uint64_t tpw, int32_t pexp, uint32_t dlen,
-                                                int64_t* __restrict__ partial) {
+                                                int64_t* __restrict__ partial, int64_t* __restrict__ set,
+                                                int64_t* __restrict__ next) {
   const uint32_t lane = __lane_id();
   const uint32_t wave = wave_in_wg();
   const uint64_t t0 = (uint64_t)blockIdx.x * tpw;
@@ -68,8 +69,27 @@ __global__ __launch_bounds__(256) void k_minmax(const double* __restrict__ x, ui
   if (lane == 0) { smn[wave] = mn; smx[wave] = mx; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    partial[blockIdx.x] = min(min(smn[0], smn[1]), min(smn[2], smn[3]));
-    partial[gridDim.x + blockIdx.x] = max(max(smx[0], smx[1]), max(smx[2], smx[3]));
+    const int64_t a = min(min(smn[0], smn[1]), min(smn[2], smn[3]));
+    const int64_t b = max(max(smx[0], smx[1]), max(smx[2], smx[3]));
+    partial[blockIdx.x] = a;   // kept for the prefix-min pass
+    partial[gridDim.x + blockIdx.x] = b;
+    if (set) {
+      const uint32_t slot = blockIdx.x % kMinMaxSlots;
+      if (a != INT64_MAX) atomicMin((long long*)&set[slot], (long long)a);
+      if (b != INT64_MIN) atomicMax((long long*)&set[kMinMaxSlots + slot], (long long)b);
+      if (blockIdx.x == 0) set[2 * kMinMaxSlots] = n ? fetch_fixed_aligned_bits(__double_as_longlong(x[0]), pexp, dlen) : 0;
+    }
+  }
+  if (set && blockIdx.x == 0)
+    for (uint32_t i = threadIdx.x; i < kMinMaxSetWords; i += blockDim.x)
+      next[i] = i < kMinMaxSlots ? INT64_MAX : (i < 2 * kMinMaxSlots ? INT64_MIN : 0);
+}
+
+// Sets of slots to their empty values (mins INT64_MAX, maxes INT64_MIN).
+__global__ __launch_bounds__(256) void k_minmax_slots_init(int64_t* __restrict__ sets, uint32_t nsets) {
+  for (uint32_t i = threadIdx.x; i < nsets * kMinMaxSetWords; i += blockDim.x) {
+    const uint32_t k = i % kMinMaxSetWords;
+    sets[i] = k < kMinMaxSlots ? INT64_MAX : (k < 2 * kMinMaxSlots ? INT64_MIN : 0);
   }
 }
 
@@ -991,7 +1011,8 @@ hipError_t launch_minmax(const double* x, uint64_t n, int32_t pexp, uint32_t dle
   const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
   const uint64_t tpw = (ntiles + nwg - 1) / nwg;
   if (!prefix) {
-    launch_k(k_minmax, dim3(nwg), dim3(256), 0, s, x, n, ntiles, tpw, pexp, dlen, partial);
+    launch_k(k_minmax, dim3(nwg), dim3(256), 0, s, x, n, ntiles, tpw, pexp, dlen, partial, (int64_t*)nullptr,
+             (int64_t*)nullptr);
     launch_k(k_minmax_final, dim3(1), dim3(256), 0, s, x, n, pexp, dlen, (const int64_t*)partial, nwg,
                        out);
   } else {
@@ -999,6 +1020,20 @@ hipError_t launch_minmax(const double* x, uint64_t n, int32_t pexp, uint32_t dle
                        (const int64_t*)partial, partial2);
     launch_k(k_minmax_prefix_final, dim3(1), dim3(64), 0, s, (const int64_t*)partial2, nwg, out);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_minmax_slots(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t* partial,
+                               int64_t* set, int64_t* next, uint32_t nwg, hipStream_t s) {
+  if (!set || !next || n == 0) return hipErrorInvalidValue;
+  const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
+  const uint64_t tpw = (ntiles + nwg - 1) / nwg;
+  launch_k(k_minmax, dim3(nwg), dim3(256), 0, s, x, n, ntiles, tpw, pexp, dlen, partial, set, next);
+  return hipGetLastError();
+}
+
+hipError_t launch_minmax_slots_init(int64_t* sets, uint32_t nsets, hipStream_t s) {
+  launch_k(k_minmax_slots_init, dim3(1), dim3(256), 0, s, sets, nsets);
   return hipGetLastError();
 }
 

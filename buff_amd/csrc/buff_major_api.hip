@@ -234,20 +234,10 @@ int major_encode_device(Ctx* ctx, const double* x, uint64_t n, uint64_t scale, u
     return fail(BUFF_E_REF_PANIC, "byte_residue_encode of an empty column samples fixed_vec[0..SAMPLE]");
   const uint32_t dlen = (uint32_t)dl;
   const int32_t pexp = precision_exp_of_prec(prec);
-  const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
 
   // plain min / max (buff_simd.rs:340-349: `if` / `if`)
-  const uint32_t nwg = minmax_grid(ntiles);
-  BUFF_TRY(ctx->scratch(ctx->s_partial, 3ull * nwg * sizeof(int64_t)));
-  BUFF_TRY(ctx->scratch(ctx->s_minmax, sizeof(MinMaxOut)));
-  MinMaxOut* dmm = (MinMaxOut*)ctx->s_minmax.ptr;
-  int64_t* part = (int64_t*)ctx->s_partial.ptr;
-  {
-    ProfScope ps(ctx, "k_minmax");
-    HIP_TRY(launch_minmax(x, n, pexp, dlen, part, part + 2 * nwg, dmm, nwg, false, ctx->stream));
-  }
   MinMaxOut mm;
-  BUFF_TRY(d2h_small(ctx, &mm, dmm, sizeof(mm)));
+  BUFF_TRY(ctx->minmax_pass(x, n, pexp, dlen, &mm));
   const int64_t mn = mm.min_all;
   const int64_t mx = std::max(mm.first, mm.max_rest);
   const uint32_t F = width_of_delta((int64_t)((uint64_t)mx - (uint64_t)mn));

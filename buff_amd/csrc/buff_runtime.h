@@ -70,9 +70,10 @@ struct Ctx {
   void* pinned = nullptr;
 
   Scratch s_partial, s_minmax, s_stream, s_bitmap, s_sums, s_best, s_values, s_sink, s_major, s_cond, s_condin, s_wmax, s_slice, s_csv, s_csvtab,
-      s_csvtmp, s_csvslow, s_condval, s_maxslot, s_sumslot;
+      s_csvtmp, s_csvslow, s_condval, s_maxslot, s_sumslot, s_mmslot;
   uint32_t max_tag = 0;        // the last tag given with s_maxslot (launch_max)
   uint32_t sum_set = 0;        // the s_sumslot set the next launch_sum_slots adds into
+  uint32_t mm_set = 0;         // the s_mmslot set the next launch_minmax_slots folds into
   bool csv_tab_ready = false;  // s_csvtab holds the table of 5^q
 
   bool prof_on = false;
@@ -95,6 +96,11 @@ struct Ctx {
   // (both zeroed on first use); sum_slots_used() after the launch went out
   int sum_slots(unsigned long long** set, unsigned long long** next);
   void sum_slots_used() { sum_set ^= 1u; }
+  // the min/max pass's slot sets, the same way
+  int minmax_slots(int64_t** set, int64_t** next);
+  void minmax_slots_used() { mm_set ^= 1u; }
+  // the min/max pass over n >= 1 rows of x in one launch, folded on the host
+  int minmax_pass(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, MinMaxOut* mm);
   void prof_begin(const char* name, hipEvent_t* ev, bool kernels);
   void prof_end(const char* name, hipEvent_t* ev, bool kernels);
   int prof_collect();
