@@ -583,6 +583,7 @@ int major_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_
   if (e == hipSuccess && rc == BUFF_OK) e = hipStreamSynchronize(ctx->stream);   // host buffers may go away
   if (e != hipSuccess) rc = fail(BUFF_E_DEVICE, "buff-major upload: %s", hipGetErrorString(e));
   if (rc != BUFF_OK) {
+    (void)hipStreamSynchronize(ctx->stream);   // copies queued before the failure still read the host vectors
     free_partial(ctx, ch);
     return rc;
   }
@@ -950,6 +951,7 @@ int residue_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint3
   if (e == hipSuccess && rc == BUFF_OK) e = hipStreamSynchronize(ctx->stream);   // host buffers may go away
   if (e != hipSuccess) rc = fail(BUFF_E_DEVICE, "residue upload: %s", hipGetErrorString(e));
   if (rc != BUFF_OK) {
+    (void)hipStreamSynchronize(ctx->stream);   // copies queued before the failure still read the host vectors
     free_partial(ctx, ch);
     return rc;
   }

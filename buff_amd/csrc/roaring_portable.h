@@ -122,7 +122,9 @@ inline void serialize(const uint32_t* words, uint64_t nbits, std::vector<uint8_t
 }
 
 // Parses a portable bitmap into BitVec words of nbits rows. Returns false
-// (with `why`) on a malformed bitmap or a row at or past nbits.
+// (with `why`) on a malformed bitmap (truncated, an unknown cookie, runs or
+// array values out of order, a run past its container) or a row at or past
+// nbits.
 inline bool deserialize(const uint8_t* p, uint64_t n, uint64_t nbits, std::vector<uint32_t>* words, uint64_t* card,
                         const char** why) {
   uint64_t at = 0;
@@ -183,10 +185,16 @@ inline bool deserialize(const uint8_t* p, uint64_t n, uint64_t nbits, std::vecto
     if (flags && ((flags[i / 8] >> (i % 8)) & 1)) {
       uint32_t nr;
       if (!u16(&nr)) return false;
+      uint32_t next = 0;   // runs are sorted, disjoint and inside the container
       for (uint32_t r = 0; r < nr && ok; ++r) {
         uint32_t s, l;
         if (!u16(&s) || !u16(&l)) return false;
+        if (s < next || s + l > 65535) {
+          *why = "runs out of order or past their container";
+          return false;
+        }
         for (uint32_t v = s; v <= s + l && ok; ++v) ok = set(hi | v);
+        next = s + l + 1;
       }
     } else if (cards[i] > kArrayMax) {
       if (!need(8192)) return false;
@@ -194,10 +202,16 @@ inline bool deserialize(const uint8_t* p, uint64_t n, uint64_t nbits, std::vecto
         if ((p[at + (b >> 3)] >> (b & 7)) & 1) ok = set(hi | b);
       at += 8192;
     } else {
+      uint32_t next = 0;   // array values strictly increase
       for (uint32_t k = 0; k < cards[i] && ok; ++k) {
         uint32_t v;
         if (!u16(&v)) return false;
+        if (v < next) {
+          *why = "array values out of order";
+          return false;
+        }
         ok = set(hi | v);
+        next = v + 1;
       }
     }
     if (!ok) {

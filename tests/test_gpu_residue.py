@@ -144,3 +144,24 @@ def test_encode_panics_and_limits(gpu_ctx):
         gpu_ctx.encode(x, 1000, flags=N.BUFF_RESIDUE)
     with pytest.raises(N.BuffError):
         gpu_ctx.encode(x, 1000, flags=N.BUFF_RESIDUE | N.BUFF_MAJORITY)
+
+
+def test_load_rejects_a_run_past_its_container(gpu_ctx, oracle):
+    """A CRoaring run container whose run leaves its 65536-row container (the
+    encoder never writes one) is a malformed stream on load, not rows of the
+    next container."""
+    x = np.full(70000, 100.0)
+    x[1000:1100] = 1e6 + np.arange(100)
+    s, cols = oracle.residue_encode(x, 1000)
+    h = cf.rapg_parse(s)
+    assert cols == 2 and h["S"] == 2 and list(h["cols"][0]["rows"]) == list(range(1000, 1100))
+    b = bytearray(s)
+    # column 0's bitmap at 25: cookie 12347 (one container), run flags, key 0 /
+    # card - 1, then the run container: count 1, start 1000, length - 1 = 99
+    assert bytes(b[25:40]).hex() == "3b30000001000063000100e8036300"
+    ch = gpu_ctx.load(np.frombuffer(bytes(b), dtype=np.uint8), 1000, flags=N.BUFF_RESIDUE)
+    ch.free()
+    b[38:40] = (0xFFFF).to_bytes(2, "little")   # the run now reaches row 1000 + 65535
+    with pytest.raises(N.BuffError) as e:
+        gpu_ctx.load(np.frombuffer(bytes(b), dtype=np.uint8), 1000, flags=N.BUFF_RESIDUE)
+    assert e.value.code == N.BUFF_E_INVALID, str(e.value)
