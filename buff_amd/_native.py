@@ -4,6 +4,7 @@ This is the Python side of the drop-in boundary: every call goes through the
 exported `extern "C"` entry points of the HIP library. There is no CPU
 fallback; loading fails loudly when the library is missing.
 """
+import collections.abc
 import ctypes
 import os
 
@@ -442,13 +443,58 @@ class Chunk:
             pass
 
 
+QUERY_DTYPE = np.dtype(Query)          # the same layout as buff_query
+RESULT_DTYPE = np.dtype(QueryResult)   # the same layout as buff_query_result
+
+
+class PreparedQueries:
+    """Query descriptors already in buff_query layout (what a native caller
+    holds): built once, passed to Batch.query / Server.run_trace /
+    Context.trace_per_call without per-call conversion."""
+
+    def __init__(self, qs):
+        if isinstance(qs, np.ndarray) and qs.dtype == QUERY_DTYPE:
+            a = np.ascontiguousarray(qs).copy()
+        else:
+            a = np.array([tuple(q) for q in qs], dtype=QUERY_DTYPE)
+        self.n = len(a)
+        self.buf = a if self.n else np.zeros(1, dtype=QUERY_DTYPE)
+        self.arr = (Query * len(self.buf)).from_buffer(self.buf)   # shares the numpy memory
+
+    def __len__(self):
+        return self.n
+
+
+def prepare_queries(qs):
+    return qs if isinstance(qs, PreparedQueries) else PreparedQueries(qs)
+
+
 def make_queries(qs):
-    """qs: iterable of (op, chunk, pred, precision, flags) -> ctypes array of Query."""
-    qs = list(qs)
-    arr = (Query * max(len(qs), 1))()
-    for i, (op, chunk, pred, precision, flags) in enumerate(qs):
-        arr[i] = Query(op, chunk, float(pred), int(precision), int(flags))
-    return arr, len(qs)
+    """qs: iterable of (op, chunk, pred, precision, flags), or PreparedQueries
+    -> (ctypes array of Query, count)."""
+    p = prepare_queries(qs)
+    return p.arr, p.n
+
+
+class QueryResults(collections.abc.Sequence):
+    """The buff_query_result array of a call, read element by element (no
+    per-call conversion of every result)."""
+
+    def __init__(self, res, n):
+        self._res = res
+        self._n = n
+
+    def __len__(self):
+        return self._n
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self._res[k] for k in range(*i.indices(self._n))]
+        if i < 0:
+            i += self._n
+        if not 0 <= i < self._n:
+            raise IndexError("query result %d of %d" % (i, self._n))
+        return self._res[i]
 
 
 class Batch:
@@ -470,7 +516,7 @@ class Batch:
         arr, nq = make_queries(qs)
         res = (QueryResult * max(nq, 1))()
         check(lib().buff_batch_query(self.ctx.handle, self.handle, arr, nq, res))
-        return [res[i] for i in range(nq)]
+        return QueryResults(res, nq)
 
     def tune(self, qs):
         arr, nq = make_queries(qs)
@@ -568,7 +614,7 @@ class Server:
             o = (ctypes.c_void_p * max(n, 1))(*[ctypes.c_void_p(v) if v else None for v in outs])
         check(lib().buff_server_run_trace(self.handle, arr, ptr(t, ctypes.c_double), o, n, res,
                                           ptr(lat, ctypes.c_double)))
-        return [res[i] for i in range(n)], lat[:n]
+        return QueryResults(res, n), lat[:n]
 
     def stats(self):
         b, q, m = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
@@ -700,7 +746,7 @@ class Context:
             o = (ctypes.c_void_p * max(n, 1))(*[ctypes.c_void_p(v) if v else None for v in outs])
         check(lib().buff_trace_per_call(self.handle, hs, len(chunks), arr, ptr(t, ctypes.c_double), o, n, res,
                                         ptr(lat, ctypes.c_double)))
-        return [res[i] for i in range(n)], lat[:n]
+        return QueryResults(res, n), lat[:n]
 
     def set_batch_tuning(self, grid, tiles_per_item, first_planes):
         """Pins the batch shape (grid, tiles per item, 1 or 8 first filter planes)
