@@ -654,7 +654,7 @@ int batch_tune(Ctx* ctx, Batch* b, const buff_query* qs, uint32_t nq, BatchTunin
   std::vector<buff_query_result> tmp(nq);
   BatchTuning best = ctx_tuning(ctx);
   float best_ms = 1e30f;
-  for (uint32_t grid : {1024u, 2048u, 4096u}) {
+  for (uint32_t grid : {1024u, 2048u, 4096u, 8192u}) {
     for (uint32_t tiles : {16u, 64u, 256u}) {
       for (uint32_t first : {1u, kMaxPlanes}) {
         BatchTuning t{grid, first, tiles, 0};

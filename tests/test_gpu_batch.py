@@ -182,7 +182,7 @@ def test_graph_replay_and_tuning(batch):
     assert [(r.status, r.count, r.value, r.exact, r.max_off) for r in r1] == \
            [(r.status, r.count, r.value, r.exact, r.max_off) for r in r1b]
     grid, tiles, first = b.tune(_mix(cols, 13, 200))
-    assert grid in (1024, 2048, 4096) and tiles in (16, 64, 256) and first in (1, 8)
+    assert grid in (1024, 2048, 4096, 8192) and tiles in (16, 64, 256) and first in (1, 8)
     _check(b, cols, qs1, b.query(qs1))
 
 
