@@ -1288,6 +1288,25 @@ def main():
             gbps = 2.0 * n * 8 / (us * 1e-6) / 1e9
             stream_copy[label] = {"avg_us": round(us, 2), "achieved_GBps": round(gbps, 1),
                                   "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+    # ... and the streaming read of one plane's worth of bytes (n: a top byte
+    # plane), the floor of the plane-reading kernels at this size
+    stream_read = {"bytes_per_launch": n, "note": "k_read_bytes: n bytes read (one byte plane), non-temporal"}
+    for label, evict in (("warm", False), ("cold", True)):
+        ctx.prof_reset()
+        for trial in range(trials + 1):   # the first launch is not timed
+            if evict:
+                flush.touch()
+            ctx.synchronize()
+            ctx.prof_enable(trial > 0)
+            dx[0].stream_read(n)
+            ctx.synchronize()
+            ctx.prof_enable(False)
+        launches, total_ms = ctx.prof_query("k_read_bytes")
+        if launches:
+            us = total_ms / launches * 1e3
+            gbps = float(n) / (us * 1e-6) / 1e9
+            stream_read[label] = {"avg_us": round(us, 2), "achieved_GBps": round(gbps, 1),
+                                  "frac": round(gbps / HBM_PEAK_GBPS, 4)}
     filter_project = bench_filter_project(ctx, cols[0], keep[0], infos[0], words, out, flush, buff_amd)
     window_max = bench_window_max(ctx, cols[0], keep[0], infos[0], words, out, flush, buff_amd)
     slice_sec = bench_slice(ctx, cols[0], dx[0], words, out, flush, buff_amd)
@@ -1323,6 +1342,7 @@ def main():
         "cold_cache": cold,
         "cold_filters": filters,
         "stream_copy": stream_copy,
+        "stream_read_plane": stream_read,
         "filter_project": filter_project,
         "window_max": window_max,
         "slice": slice_sec,

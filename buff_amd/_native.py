@@ -151,6 +151,7 @@ SIGNATURES = {
     "buff_dev_fill": [vp, vp, ctypes.c_uint8, u64],
     "buff_dev_touch": [vp, vp, u64],
     "buff_dev_copy": [vp, vp, vp, u64],
+    "buff_dev_read": [vp, vp, u64],
     "buff_prof_enable": [vp, cint],
     "buff_prof_reset": [vp],
     "buff_prof_query": [vp, ctypes.c_char_p, u64p, f64p],
@@ -667,6 +668,13 @@ class DeviceBuffer:
         if nbytes > self.nbytes or nbytes > src.nbytes:
             raise ValueError("copy of %d bytes between buffers of %d and %d" % (nbytes, src.nbytes, self.nbytes))
         check(lib().buff_dev_copy(self.ctx.handle, self.ptr, src.ptr, nbytes))
+
+    def stream_read(self, nbytes=None):
+        """Reads nbytes (default: all) with the non-temporal streaming-read kernel."""
+        nbytes = self.nbytes if nbytes is None else nbytes
+        if nbytes > self.nbytes:
+            raise ValueError("read of %d bytes from a buffer of %d" % (nbytes, self.nbytes))
+        check(lib().buff_dev_read(self.ctx.handle, self.ptr, nbytes))
 
     def free(self):
         if self.ptr:

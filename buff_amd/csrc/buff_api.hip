@@ -1496,6 +1496,18 @@ int buff_dev_copy(buff_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
   return BUFF_OK;
 }
 
+int buff_dev_read(buff_ctx* ctx, const void* src, uint64_t bytes) {
+  if (!ctx || (bytes && !src)) return fail(BUFF_E_INVALID, "null argument");
+  if (bytes % 16 || (uintptr_t)src % 16)
+    return fail(BUFF_E_INVALID, "read of %llu bytes: size and address must be multiples of 16",
+                (unsigned long long)bytes);
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  BUFF_TRY(c->scratch(c->s_sink, 256));
+  ProfScope ps(c, "k_read_bytes");
+  HIP_TRY(launch_read_bytes(src, bytes, (uint32_t*)c->s_sink.ptr, c->stream));
+  return BUFF_OK;
+}
+
 int buff_dev_touch(buff_ctx* ctx, const void* dptr, uint64_t bytes) {
   if (!ctx || (bytes && !dptr)) return fail(BUFF_E_INVALID, "null argument");
   Ctx* c = reinterpret_cast<Ctx*>(ctx);

@@ -66,6 +66,8 @@ hipError_t launch_cond_fill(uint64_t* out, const uint64_t* cond, uint64_t nwords
 hipError_t launch_fill_bytes(void* out, uint64_t bytes, uint8_t value, hipStream_t s);
 // bytes: a multiple of 16; in and out 16-byte aligned and not overlapping
 hipError_t launch_copy_bytes(void* out, const void* in, uint64_t bytes, hipStream_t s);
+// bytes: a multiple of 16, in 16-byte aligned
+hipError_t launch_read_bytes(const void* in, uint64_t bytes, uint32_t* sink, hipStream_t s);
 hipError_t launch_touch_bytes(const void* in, uint64_t bytes, uint32_t* sink, hipStream_t s);
 // sums: 9 u64 results followed by sum_scratch_bytes(ls) of per-workgroup rows
 // (at most kMaxSumRows workgroups).

@@ -157,6 +157,7 @@ extern "C" {
     pub fn buff_dev_fill(ctx: *mut BuffCtx, dptr: *mut c_void, value: u8, bytes: u64) -> c_int;
     pub fn buff_dev_touch(ctx: *mut BuffCtx, dptr: *const c_void, bytes: u64) -> c_int;
     pub fn buff_dev_copy(ctx: *mut BuffCtx, dst: *mut c_void, src: *const c_void, bytes: u64) -> c_int;
+    pub fn buff_dev_read(ctx: *mut BuffCtx, src: *const c_void, bytes: u64) -> c_int;
     pub fn buff_prof_enable(ctx: *mut BuffCtx, on: c_int) -> c_int;
     pub fn buff_prof_reset(ctx: *mut BuffCtx) -> c_int;
     pub fn buff_prof_query(ctx: *mut BuffCtx, kernel: *const c_char, launches: *mut u64, total_ms: *mut f64) -> c_int;

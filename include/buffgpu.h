@@ -135,6 +135,11 @@ int buff_dev_touch(buff_ctx* ctx, const void* dptr, uint64_t bytes);
  * materialize are measured against. bytes, dst and src multiples of 16; the
  * ranges must not overlap. No reference counterpart. */
 int buff_dev_copy(buff_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Reads device bytes with non-temporal loads in the plane kernels' order and
+ * discards them (no host sync): the streaming read rate that the plane-reading
+ * kernels (filters, max, sum) are measured against. bytes and src multiples of
+ * 16. No reference counterpart. */
+int buff_dev_read(buff_ctx* ctx, const void* src, uint64_t bytes);
 
 /* Per-kernel timing with HIP events recorded on the context's stream.
  * buff_prof_query reports launches and summed durations for one kernel name. */

@@ -2,7 +2,8 @@
 
 * buff_dev_copy (the streaming-copy ceiling kernel) copies exactly the bytes
   asked for, at several sizes, and rejects sizes or addresses that are not
-  multiples of 16 and overlapping ranges.
+  multiples of 16 and overlapping ranges; buff_dev_read (the streaming-read
+  ceiling) leaves memory unchanged and rejects the same misuses.
 * Sums, maxima and min/max passes fold into slot sets kept across calls (two
   alternating sets for sums and the min/max pass, per-call tags for the max):
   interleaved calls on different chunks, many times over, must each give the
@@ -36,6 +37,21 @@ def test_dev_copy_exact(gpu_ctx, nbytes):
     finally:
         src.free()
         dst.free()
+
+
+def test_dev_read_runs_and_rejects(gpu_ctx):
+    a = gpu_ctx.alloc(1 << 20)
+    try:
+        a.fill(7)
+        a.stream_read()            # reads and discards; the buffer is unchanged
+        a.stream_read(4096 + 16)
+        assert np.all(a.download(np.uint8, 1 << 20) == 7)
+        with pytest.raises(N.BuffError):
+            N.check(N.lib().buff_dev_read(gpu_ctx.handle, a.ptr, 100))       # not a multiple of 16
+        with pytest.raises(N.BuffError):
+            N.check(N.lib().buff_dev_read(gpu_ctx.handle, a.ptr + 8, 1024))  # unaligned address
+    finally:
+        a.free()
 
 
 def test_dev_copy_rejects(gpu_ctx):
