@@ -98,6 +98,8 @@ def parse():
     ap.add_argument("--no-config5", action="store_true", help="skip the served-form section")
     ap.add_argument("--config5-gb", type=float, default=48.0, help="resident compressed bytes (GB, 1e9)")
     ap.add_argument("--config5-requests", type=int, default=3000, help="requests per load point")
+    ap.add_argument("--config5-slots", type=int, default=2, help="batches in flight (continuous batching)")
+    ap.add_argument("--config5-max-batch", type=int, default=4096, help="requests per batch (continuous batching)")
     return ap.parse_args()
 
 
@@ -948,7 +950,8 @@ def bench_config5(ctx_unused, args, buff_amd):
     res = {"workload": "outlier-heavy chunks, lengths log-uniform in [2^20, 2^26], 3 decimals, buff-major layout",
            "chunks": len(chunks), "resident_bytes": int(resident), "rows": int(sum(lens)),
            "mix": "60% range filter, 25% sum, 15% materialize at 1-3 decimals (semantic)",
-           "hbm_budget_bytes": budget, "server_arena_bytes": arena_bytes}
+           "hbm_budget_bytes": budget, "server_arena_bytes": arena_bytes,
+           "server_slots": args.config5_slots, "server_max_batch": args.config5_max_batch}
     nreq = args.config5_requests
 
     # filter counts and exact sums of every pool column, for checking the served answers
@@ -1002,7 +1005,7 @@ def bench_config5(ctx_unused, args, buff_amd):
             wall = time.perf_counter() - t
             check(qs, out, mode, counts=False)
         else:
-            mb, sl = (4096, 2) if mode == "continuous_batching" else (1, 1)
+            mb, sl = (args.config5_max_batch, args.config5_slots) if mode == "continuous_batching" else (1, 1)
             srv = buff_amd.Server(ctx, max_batch=mb, slots=sl, arena_bytes=arena_bytes)
             try:
                 for ch in chunks:
