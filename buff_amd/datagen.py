@@ -6,6 +6,8 @@ call yields the same doubles on every machine with the same numpy.
     to `decimals` (the walk accumulates the unrounded values).
   * uniform: U[lo, hi] rounded to `decimals`.
   * outlier_heavy: 95% N(100, 0.5), 5% +-10^U[6, 9], rounded to `decimals`.
+  * positive_spikes / wide_spikes: N(100, 0.5) with a fraction of rows
+    100 + 10^U[3, 7] / 100 + 10^U[5, 9] (the residue format's outlier columns).
 Rounding is numpy.round (round half to even on x * 10^d).
 """
 import numpy as np
@@ -43,6 +45,16 @@ def positive_spikes(n, decimals, seed, frac=0.05):
     x = rng.normal(100.0, 0.5, n)
     out = rng.random(n) < frac
     x[out] = 100.0 + 10.0 ** rng.uniform(3.0, 7.0, int(out.sum()))
+    return np.round(x, decimals)
+
+
+def wide_spikes(n, decimals, seed, frac=0.03):
+    """95%+ N(100, 0.5) and `frac` of rows 100 + 10^U[5,9]: offsets up to 2^30
+    units, so three leading byte columns are mostly zero."""
+    rng = np.random.default_rng(seed)
+    x = rng.normal(100.0, 0.5, n)
+    out = rng.random(n) < frac
+    x[out] = 100.0 + 10.0 ** rng.uniform(5.0, 9.0, int(out.sum()))
     return np.round(x, decimals)
 
 

@@ -103,6 +103,8 @@ def _cases():
         "walk_p1": (dg.random_walk(9_000, 1, 8), 10),
         "constant": (np.full(3000, 2.5), 1000),                    # delta = 0: ilen wraps, 21 bytes
         "first_row_max": (np.concatenate([[2.0e7], dg.positive_spikes(20_000, 3, 10)]), 1000),
+        "spikes_ragged_2001": (dg.positive_spikes(2001, 3, 11), 1000),   # one row past the 2000 sampled
+        "wide_three_cols": (dg.wide_spikes(70_001, 3, 12), 1000),       # three outlier columns, a 1-bit tail
     }
 
 
