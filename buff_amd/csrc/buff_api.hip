@@ -103,6 +103,7 @@ Ctx::~Ctx() {
   free_blocks.clear();
   for (auto& p : prof_pool) (void)hipEventDestroy(p);
   if (pinned) (void)hipHostFree(pinned);
+  if (mbox) (void)hipHostFree(mbox);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -190,6 +191,33 @@ int Ctx::minmax_slots(int64_t** set, int64_t** next) {
   return BUFF_OK;
 }
 
+int Ctx::publish(const void* src, uint32_t mode, uint32_t nk, Publish* pub) {
+  pub->mbox = mbox_dev;
+  pub->seq = ++mbox_seq;
+  HIP_TRY(launch_publish(src, mode, nk, *pub, stream));
+  return BUFF_OK;
+}
+
+int Ctx::published(const Publish& pub, uint64_t* words, uint32_t nw) {
+  // The launch is the last work on the stream: spin on the sequence word, and
+  // now and then ask the stream, so a failed launch (or one that ended
+  // without publishing) ends the wait with its error.
+  for (uint32_t spin = 1;; ++spin) {
+    if (__atomic_load_n(&mbox[0], __ATOMIC_ACQUIRE) == pub.seq) break;
+    if (spin % 512 == 0) {
+      const hipError_t e = hipStreamQuery(stream);
+      if (e == hipSuccess) {
+        if (__atomic_load_n(&mbox[0], __ATOMIC_ACQUIRE) == pub.seq) break;
+        return fail(BUFF_E_DEVICE, "a reduction ended without publishing its result");
+      }
+      if (e != hipErrorNotReady) return fail(BUFF_E_DEVICE, "reduction: %s", hipGetErrorString(e));
+    }
+    __builtin_ia32_pause();
+  }
+  for (uint32_t i = 0; i < nw; ++i) words[i] = __atomic_load_n(&mbox[1 + i], __ATOMIC_RELAXED);
+  return BUFF_OK;
+}
+
 int Ctx::minmax_pass(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, MinMaxOut* mm) {
   const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
   const uint32_t nwg = minmax_grid(ntiles);
@@ -201,17 +229,14 @@ int Ctx::minmax_pass(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, M
     HIP_TRY(launch_minmax_slots(x, n, pexp, dlen, (int64_t*)s_partial.ptr, set, next, nwg, stream));
     minmax_slots_used();
   }
-  int64_t w[2 * kMinMaxSlots + 1];
-  BUFF_TRY(d2h_small(this, w, set, sizeof(w)));
-  int64_t a = INT64_MAX, b = INT64_MIN;
-  for (uint32_t i = 0; i < kMinMaxSlots; ++i) {
-    a = std::min(a, w[i]);
-    b = std::max(b, w[kMinMaxSlots + i]);
-  }
+  Publish pub;
+  BUFF_TRY(publish(set, kPublishMinMax, 0, &pub));
+  uint64_t w[3];
+  BUFF_TRY(published(pub, w, 3));
   *mm = MinMaxOut{};
-  mm->min_all = a;
-  mm->max_rest = b;
-  mm->first = w[2 * kMinMaxSlots];
+  mm->min_all = (int64_t)w[0];
+  mm->max_rest = (int64_t)w[1];
+  mm->first = (int64_t)w[2];
   return BUFF_OK;
 }
 
@@ -325,8 +350,17 @@ void chunk_set_width(Chunk* ch, uint32_t F) {
   ch->view.r = F >= 8 ? F % 8 : F;
 }
 
-// Two pinned u64 slots per context are plenty for the per-call results.
+// Small results come back through the mailbox (whole u64 words, up to
+// kPublishWords - 1), else with a copy and a stream wait.
 int d2h_small(Ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (bytes > 0 && bytes % 8 == 0 && bytes / 8 < kPublishWords && (uintptr_t)src % 8 == 0) {
+    uint64_t w[kPublishWords];
+    Publish pub;
+    BUFF_TRY(ctx->publish(src, kPublishCopy, (uint32_t)(bytes / 8), &pub));
+    BUFF_TRY(ctx->published(pub, w, (uint32_t)(bytes / 8)));
+    memcpy(dst, w, bytes);
+    return BUFF_OK;
+  }
   if (bytes > Ctx::kPinnedBytes) {
     HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -954,7 +988,7 @@ int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64
   uint64_t ps[kMaxPlanes + 1] = {0};
   const ChunkView& v = ch->view;
   if (ch->n > 0 && (v.P || v.r)) {
-    // one launch into a slot set ([k][slot] u64), folded here
+    // one launch into a slot set ([k][slot] u64), folded as it is handed over
     const LaunchShape ls = ctx->shape(Op::Sum, v);
     unsigned long long *set = nullptr, *next = nullptr;
     BUFF_TRY(ctx->sum_slots(&set, &next));
@@ -963,10 +997,9 @@ int sum_op(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, uint64
       HIP_TRY(launch_sum_slots(v, set, next, ctx->stream, ls));
       ctx->sum_slots_used();
     }
-    uint64_t slots[(kMaxPlanes + 1) * kSumSlots];
-    BUFF_TRY(d2h_small(ctx, slots, set, 8ull * (v.P + 1) * kSumSlots));
-    for (uint32_t k = 0; k <= v.P; ++k)
-      for (uint32_t i = 0; i < kSumSlots; ++i) ps[k] += slots[k * kSumSlots + i];
+    Publish pub;
+    BUFF_TRY(ctx->publish(set, kPublishSumSlots, v.P + 1, &pub));
+    BUFF_TRY(ctx->published(pub, ps, v.P + 1));
   }
   if (plane_sums) memcpy(plane_sums, ps, sizeof(ps));
   if (ref_sum) *ref_sum = reference_sum_fold(ch, ps);
@@ -1002,7 +1035,9 @@ int max_op(Ctx* ctx, const Chunk* ch, uint32_t flags, double* value, uint64_t* m
       ProfScope sc(ctx, "k_max");
       HIP_TRY(launch_max(v, (unsigned long long*)sb, sb + 512, slots, tag, ctx->stream, ctx->shape(Op::Max, v)));
     }
-    BUFF_TRY(d2h_small(ctx, &best, sb, 8));
+    Publish pub;
+    BUFF_TRY(ctx->publish(sb, kPublishCopy, 1, &pub));
+    BUFF_TRY(ctx->published(pub, &best, 1));
   }
   if (max_off) *max_off = best;
   // split_double.rs:4102: (max as i64 + base_int) as f64 / 2^dlen
@@ -1321,6 +1356,12 @@ int tune_chunk(Ctx* ctx, const Chunk* ch, double pred, buff_tune_choice* out) {
   const uint32_t tiles_opts[4][3] = {{2, 4, 8}, {1, 2, 4}, {4, 8, 16}, {0, 0, 0}};
   for (int oi = 0; oi < 4 && rc == BUFF_OK; ++oi) {
     const Op op = ops[oi];
+    // sums and maxima: every shape's result (handed over as the operations
+    // do) must equal the first (default shape) run's
+    const uint32_t nres = op == Op::Sum ? v.P + 1 : (op == Op::Max ? 1u : 0u);
+    const void* res = nullptr;   // where the last run left its result
+    uint64_t want[kMaxPlanes + 1], got[kMaxPlanes + 1];
+    bool have_want = false;
     auto run = [&](const LaunchShape& ls) -> hipError_t {
       switch (op) {
         case Op::FilterGt:
@@ -1330,6 +1371,7 @@ int tune_chunk(Ctx* ctx, const Chunk* ch, double pred, buff_tune_choice* out) {
           if (ctx->sum_slots(&set, &next) != BUFF_OK) return hipErrorUnknown;
           const hipError_t e = launch_sum_slots(v, set, next, ctx->stream, ls);
           if (e == hipSuccess) ctx->sum_slots_used();
+          res = set;
           return e;
         }
         case Op::Max: {
@@ -1337,6 +1379,7 @@ int tune_chunk(Ctx* ctx, const Chunk* ch, double pred, buff_tune_choice* out) {
           unsigned int* slots = nullptr;
           uint32_t tag = 0;
           if (ctx->max_slots(&slots, &tag) != BUFF_OK) return hipErrorUnknown;
+          res = sb;
           return launch_max(v, (unsigned long long*)sb, sb + 512, slots, tag, ctx->stream, ls);
         }
         default:
@@ -1350,6 +1393,16 @@ int tune_chunk(Ctx* ctx, const Chunk* ch, double pred, buff_tune_choice* out) {
         float us = 0;
         BUFF_TRY(tm.time(ctx, [&] { return run(ls); }, &us));
         if (rep > 0) *best_us = std::min(*best_us, us);
+        if (nres) {
+          Publish pub;
+          BUFF_TRY(ctx->publish(res, op == Op::Sum ? kPublishSumSlots : kPublishCopy, nres, &pub));
+          BUFF_TRY(ctx->published(pub, have_want ? got : want, nres));
+          if (have_want && memcmp(got, want, 8ull * nres) != 0)
+            return fail(BUFF_E_DEVICE, "tuning: launch shape (%u waves, %u workgroups, %u tiles, form %u) changed "
+                        "the %s", ls.waves_per_wg, ls.max_wg, ls.tiles_per_wave, ls.prefetch,
+                        op == Op::Sum ? "sum" : "max");
+          have_want = true;
+        }
       }
       return BUFF_OK;
     };
@@ -1413,6 +1466,16 @@ int buff_ctx_create(int device, uint64_t hbm_budget_bytes, buff_ctx** out) {
   c->budget = hbm_budget_bytes;
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc(&c->pinned, Ctx::kPinnedBytes, hipHostMallocDefault);
+  // the result mailbox, made here so that no operation allocates to read a result back
+  void* mb = nullptr;
+  if (e == hipSuccess) e = hipHostMalloc(&mb, kPublishWords * 8, hipHostMallocMapped | hipHostMallocCoherent);
+  if (e == hipSuccess) {
+    c->mbox = static_cast<uint64_t*>(mb);
+    memset(mb, 0, kPublishWords * 8);
+    void* d = nullptr;
+    e = hipHostGetDevicePointer(&d, mb, 0);
+    c->mbox_dev = static_cast<unsigned long long*>(d);
+  }
   if (e != hipSuccess) {
     delete c;
     return fail(BUFF_E_DEVICE, "context setup: %s", hipGetErrorString(e));

@@ -33,6 +33,23 @@ struct MatParams {
   double inv_scale;  // exactly 2^-dlen
 };
 
+// Hand-over of a reduction's result to the host without a stream wait: a
+// one-wave kernel, queued right after the reduction, folds its slots, writes
+// the result words to mbox[1 ..] (the device address of mapped, coherent host
+// memory) and then stores seq into mbox[0] with a system-scope release; a host
+// that sees seq there reads a complete result.
+struct Publish {
+  unsigned long long* mbox;   // kPublishWords u64
+  unsigned long long seq;     // new for every launch, nonzero
+};
+constexpr uint32_t kPublishWords = 64;
+enum : uint32_t {
+  kPublishSumSlots = 0,   // nk rows of kSumSlots u64 slots: the row sums
+  kPublishMinMax = 1,     // a min/max slot set: min, max over rows >= 1, row 0
+  kPublishCopy = 2,       // nk u64 words as they are (src 8-byte aligned)
+};
+hipError_t launch_publish(const void* src, uint32_t mode, uint32_t nk, const Publish& pub, hipStream_t s);
+
 // Workgroups of the min/max pass over ntiles tiles (all resident at once).
 uint32_t minmax_grid(uint64_t ntiles);
 hipError_t launch_minmax(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, int64_t* partial,

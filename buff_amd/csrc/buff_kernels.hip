@@ -1054,6 +1054,41 @@ hipError_t launch_minmax_slots(const double* x, uint64_t n, int32_t pexp, uint32
   return hipGetLastError();
 }
 
+// One wave: the folded result of the reduction before it on the stream to the
+// mailbox (lane k writes word k), then the sequence number (a system-scope
+// release after every lane's words).
+__global__ __launch_bounds__(64) void k_publish(const uint64_t* __restrict__ src, uint32_t mode, uint32_t nk,
+                                                Publish pub) {
+  const uint32_t lane = __lane_id();
+  uint64_t mine = 0;
+  uint32_t nw = nk;
+  if (mode == kPublishSumSlots) {
+    for (uint32_t k = 0; k < nk; ++k) {
+      const uint64_t t = wave_sum_u64(src[k * kSumSlots + lane]);
+      if (lane == k) mine = t;
+    }
+  } else if (mode == kPublishMinMax) {
+    const int64_t a = wave_min_i64((int64_t)src[lane]);
+    const int64_t b = wave_max_i64((int64_t)src[kMinMaxSlots + lane]);
+    mine = lane == 0 ? (uint64_t)a : (lane == 1 ? (uint64_t)b : src[2 * kMinMaxSlots]);
+    nw = 3;
+  } else if (lane < nk) {
+    mine = src[lane];
+  }
+  if (lane < nw) pub.mbox[1 + lane] = mine;
+  __threadfence_system();
+  __syncthreads();
+  if (lane == 0) __hip_atomic_store(pub.mbox, pub.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+hipError_t launch_publish(const void* src, uint32_t mode, uint32_t nk, const Publish& pub, hipStream_t s) {
+  if (!src || !pub.mbox || mode > kPublishCopy || nk >= kPublishWords ||
+      (mode == kPublishSumSlots && (nk == 0 || nk > kMaxPlanes + 1)))
+    return hipErrorInvalidValue;
+  launch_k(k_publish, dim3(1), dim3(64), 0, s, static_cast<const uint64_t*>(src), mode, nk, pub);
+  return hipGetLastError();
+}
+
 hipError_t launch_minmax_slots_init(int64_t* sets, uint32_t nsets, hipStream_t s) {
   launch_k(k_minmax_slots_init, dim3(1), dim3(256), 0, s, sets, nsets);
   return hipGetLastError();

@@ -70,7 +70,7 @@ int sum_noalloc(Ctx* ctx, const Chunk* ch, double* ref_sum, double* exact_sum, u
       return fail(BUFF_E_UNSUPPORTED, "the sum needs %llu bytes of scratch, above the %llu of buff_sum_noalloc",
                   (unsigned long long)need, (unsigned long long)kNoAllocScratchBytes);
     HIP_TRY(launch_sum(v, (unsigned long long*)dev, ctx->stream, ls));
-    BUFF_TRY(d2h_small(ctx, ps, dev, sizeof(ps)));   // the context's pinned staging, made at creation
+    BUFF_TRY(d2h_small(ctx, ps, dev, sizeof(ps)));   // through the context's mailbox, made at creation
   }
   if (plane_sums) memcpy(plane_sums, ps, sizeof(ps));
   if (ref_sum) *ref_sum = reference_sum_fold(ch, ps);

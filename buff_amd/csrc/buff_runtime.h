@@ -75,6 +75,11 @@ struct Ctx {
   uint32_t sum_set = 0;        // the s_sumslot set the next launch_sum_slots adds into
   uint32_t mm_set = 0;         // the s_mmslot set the next launch_minmax_slots folds into
   bool csv_tab_ready = false;  // s_csvtab holds the table of 5^q
+  // result mailbox of the reductions (launch_publish): kPublishWords u64 of
+  // mapped, coherent host memory
+  uint64_t* mbox = nullptr;
+  unsigned long long* mbox_dev = nullptr;
+  unsigned long long mbox_seq = 0;
 
   bool prof_on = false;
   std::vector<hipEvent_t> prof_pool;
@@ -101,6 +106,11 @@ struct Ctx {
   void minmax_slots_used() { mm_set ^= 1u; }
   // the min/max pass over n >= 1 rows of x in one launch, folded on the host
   int minmax_pass(const double* x, uint64_t n, int32_t pexp, uint32_t dlen, MinMaxOut* mm);
+  // queues the hand-over of a reduction's result (launch_publish into the
+  // mailbox made with the context); published() waits until it has arrived
+  // (or the stream failed or ended without it) and copies its nw result words
+  int publish(const void* src, uint32_t mode, uint32_t nk, Publish* pub);
+  int published(const Publish& pub, uint64_t* words, uint32_t nw);
   void prof_begin(const char* name, hipEvent_t* ev, bool kernels);
   void prof_end(const char* name, hipEvent_t* ev, bool kernels);
   int prof_collect();

@@ -16,6 +16,11 @@
  *    released with buff_chunk_free.
  *  - Functions taking a buff_ctx serialize on that context's HIP stream.
  *    Use one context per host thread for concurrency.
+ *  - A function that returns a result to the host (a sum, a max, a count,
+ *    encode's min/max) receives it in the context's mailbox, a few words of
+ *    mapped host memory that a one-wave kernel fills after the work, and
+ *    returns once it has arrived: all earlier work on the stream is complete,
+ *    only that kernel may still be retiring.
  *  - Bitmaps use the reference's BitVec<u32> layout (bit-vec/src/lib.rs:399-436,
  *    619-627): row i is bit i%32 of u32 word i/32, unused high bits are 0.
  *    On little-endian hosts a bitmap of ceil(n/64) u64 words has the same bytes.
