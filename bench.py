@@ -1274,42 +1274,37 @@ def main():
     # the device's streaming read + write rate beside the kernels that read and
     # write: a plain non-temporal copy of the walk column's float64 (n x 8
     # bytes) into the materialize output, warm and after a cache eviction
-    stream_copy = {"bytes_per_launch": 2 * n * 8, "note": "k_copy_bytes: n x 8 bytes read + n x 8 bytes written"}
-    for label, evict in (("warm", False), ("cold", True)):
-        ctx.prof_reset()
-        for trial in range(trials + 1):   # the first launch is not timed
-            if evict:
-                flush.touch()
-            ctx.synchronize()
-            ctx.prof_enable(trial > 0)
-            out.copy_from(dx[0], n * 8)
-            ctx.synchronize()
-            ctx.prof_enable(False)
-        launches, total_ms = ctx.prof_query("k_copy_bytes")
-        if launches:
-            us = total_ms / launches * 1e3
-            gbps = 2.0 * n * 8 / (us * 1e-6) / 1e9
-            stream_copy[label] = {"avg_us": round(us, 2), "achieved_GBps": round(gbps, 1),
-                                  "frac": round(gbps / HBM_PEAK_GBPS, 4)}
-    # ... and the streaming read of one plane's worth of bytes (n: a top byte
-    # plane), the floor of the plane-reading kernels at this size
-    stream_read = {"bytes_per_launch": n, "note": "k_read_bytes: n bytes read (one byte plane), non-temporal"}
-    for label, evict in (("warm", False), ("cold", True)):
-        ctx.prof_reset()
-        for trial in range(trials + 1):   # the first launch is not timed
-            if evict:
-                flush.touch()
-            ctx.synchronize()
-            ctx.prof_enable(trial > 0)
-            dx[0].stream_read(n)
-            ctx.synchronize()
-            ctx.prof_enable(False)
-        launches, total_ms = ctx.prof_query("k_read_bytes")
-        if launches:
-            us = total_ms / launches * 1e3
-            gbps = float(n) / (us * 1e-6) / 1e9
-            stream_read[label] = {"avg_us": round(us, 2), "achieved_GBps": round(gbps, 1),
-                                  "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+    def ceiling(kernel, nbytes_moved, launch, note):
+        sec = {"bytes_per_launch": nbytes_moved, "note": note}
+        for label, evict in (("warm", False), ("cold", True)):
+            ctx.prof_reset()
+            for trial in range(trials + 1):   # the first launch is not timed
+                if evict:
+                    flush.touch()
+                ctx.synchronize()
+                ctx.prof_enable(trial > 0)
+                launch()
+                ctx.synchronize()
+                ctx.prof_enable(False)
+            launches, total_ms = ctx.prof_query(kernel)
+            if launches:
+                us = total_ms / launches * 1e3
+                gbps = float(nbytes_moved) / (us * 1e-6) / 1e9
+                sec[label] = {"avg_us": round(us, 2), "achieved_GBps": round(gbps, 1),
+                              "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+        return sec
+
+    stream_copy = ceiling("k_copy_bytes", 2 * n * 8, lambda: out.copy_from(dx[0], n * 8),
+                          "k_copy_bytes: n x 8 bytes read + n x 8 bytes written")
+    # the streaming read of one plane's worth of bytes (n: a top byte plane),
+    # the floor of the plane-reading kernels at this size, and of the float64
+    # input (k_minmax's read)
+    stream_read = ceiling("k_read_bytes", n, lambda: dx[0].stream_read(n),
+                          "k_read_bytes: n bytes read (one byte plane), non-temporal")
+    stream_read_f64 = ceiling("k_read_bytes", n * 8, lambda: dx[0].stream_read(n * 8),
+                              "k_read_bytes: n x 8 bytes read (the float64 input), non-temporal")
+    # the streaming write of n float64 (materialize's output)
+    stream_write = ceiling("k_fill_bytes", n * 8, lambda: out.fill(0), "k_fill_bytes: n x 8 bytes written")
     filter_project = bench_filter_project(ctx, cols[0], keep[0], infos[0], words, out, flush, buff_amd)
     window_max = bench_window_max(ctx, cols[0], keep[0], infos[0], words, out, flush, buff_amd)
     slice_sec = bench_slice(ctx, cols[0], dx[0], words, out, flush, buff_amd)
@@ -1346,6 +1341,8 @@ def main():
         "cold_filters": filters,
         "stream_copy": stream_copy,
         "stream_read_plane": stream_read,
+        "stream_read_f64": stream_read_f64,
+        "stream_write": stream_write,
         "filter_project": filter_project,
         "window_max": window_max,
         "slice": slice_sec,

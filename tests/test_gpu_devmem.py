@@ -8,6 +8,9 @@
   alternating sets for sums and the min/max pass, per-call tags for the max):
   interleaved calls on different chunks, many times over, must each give the
   answer the chunk gives alone (checked against numpy on the stored offsets).
+* Encode takes float64 input at any 8-byte aligned device address: rows that
+  start 8 bytes into a buffer must give the same chunk, byte for byte, as the
+  same rows at a 256-byte aligned address.
 """
 from fractions import Fraction
 
@@ -100,3 +103,29 @@ def test_slot_reductions_interleaved(gpu_ctx):
     finally:
         for ch in chunks:
             ch.free()
+
+
+@pytest.mark.parametrize("n", [1024, 5 * 1024 + 3, 300_001])
+def test_encode_aligned_and_unaligned_input(gpu_ctx, n):
+    x = dg.random_walk(n, 4, 21)
+    a = gpu_ctx.alloc((n + 2) * 8)
+    try:
+        a.upload(np.concatenate([x, [0.0, 0.0]]))
+        ch0 = gpu_ctx.encode_device(a.ptr, n, 10 ** 4)   # 256-byte aligned
+        try:
+            want = ch0.serialize()
+            a.upload(np.concatenate([[0.0], x, [0.0]]))
+            ch1 = gpu_ctx.encode_device(a.ptr + 8, n, 10 ** 4)   # 8 bytes in: not 16-byte aligned
+            try:
+                assert np.array_equal(ch1.serialize(), want), n
+            finally:
+                ch1.free()
+            host = gpu_ctx.encode(x, 10 ** 4)
+            try:
+                assert np.array_equal(host.serialize(), want), n
+            finally:
+                host.free()
+        finally:
+            ch0.free()
+    finally:
+        a.free()
