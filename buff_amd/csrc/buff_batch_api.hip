@@ -89,6 +89,12 @@ struct QueryExec {
   std::vector<buff_query_result> res_host;
   BHeader h{};
   uint32_t nq = 0;
+  // a max on a buff-major chunk with sparse columns has no batched kernel: its
+  // queries (host_max) run the single-chunk stages on `aux`, a context of the
+  // executor's own, when the batch completes (nullptr: answered
+  // BUFF_E_UNSUPPORTED)
+  Ctx* aux = nullptr;
+  std::vector<uint32_t> host_max;
 };
 
 struct Batch {
@@ -380,6 +386,7 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
   uint64_t total_tiles = 0;
   ex.res_host.assign(nq, buff_query_result{});
   ex.nq = nq;
+  ex.host_max.clear();
   uint64_t arena_off = 0, top_off = 0;
   std::vector<uint64_t> arena_at(nq, ~0ull);
   for (uint32_t k = 0; k < nq; ++k) {
@@ -430,8 +437,11 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
         break;
       case BUFF_Q_MAX:
         if (sparse) {
-          r.status = fail(BUFF_E_UNSUPPORTED, "query %u: batched max of a buff-major chunk with sparse columns "
-                          "(the single-chunk buff_max answers it)", k);
+          if (ex.aux)
+            ex.host_max.push_back(k);   // no items: answered in exec_finish
+          else
+            r.status = fail(BUFF_E_UNSUPPORTED, "query %u: batched max of a buff-major chunk with sparse columns "
+                            "(the single-chunk buff_max answers it)", k);
           tiles = false;
           break;
         }
@@ -585,6 +595,13 @@ int exec_finish(QueryExec& ex, const std::vector<Chunk*>& chunks, const buff_que
         break;
     }
   }
+  for (uint32_t k : ex.host_max) {   // buff_max's stages, on the executor's own context
+    buff_query_result& r = ex.res_host[k];
+    if (r.status != BUFF_OK) continue;
+    r.max_off = 0;
+    r.value = 0;
+    r.status = major_max(ex.aux, chunks[qs[k].chunk], qs[k].flags, &r.value, &r.max_off, nullptr);
+  }
   return BUFF_OK;
 }
 
@@ -700,6 +717,7 @@ struct Req {
 
 struct Server {
   Ctx* ctx = nullptr;
+  Ctx* aux = nullptr;   // the worker's own context (maxima of buff-major chunks with sparse columns)
   buff_server_cfg cfg{};
   std::vector<Chunk*> chunks;
   DevBuf views, majors;   // per chunk: its (dense) view; the sparse columns of buff-major chunks
@@ -1014,13 +1032,18 @@ int buff_server_create(buff_ctx* ctx, const buff_server_cfg* cfg, buff_server** 
   sv->slot_busy.assign(sv->cfg.slots, false);
   int rc = reserve(c, sv->arena, sv->cfg.arena_bytes);
   sv->arena_free[0] = sv->arena.bytes;
+  buff_ctx* aux = nullptr;
+  if (rc == BUFF_OK) rc = buff_ctx_create(c->device, 0, &aux);
+  sv->aux = reinterpret_cast<Ctx*>(aux);
   for (auto& ex : sv->slots) {
     if (rc == BUFF_OK) rc = exec_init(ex, c, true);
     if (rc == BUFF_OK) rc = exec_reserve(ex, sv->cfg.max_batch, 0, ctx_tuning(c));
+    ex.aux = sv->aux;
   }
   if (rc != BUFF_OK) {
     for (auto& e2 : sv->slots) exec_free(e2);
     drop(c, sv->arena);
+    (void)buff_ctx_destroy(aux);
     delete sv;
     return rc;
   }
@@ -1229,6 +1252,7 @@ int buff_server_destroy(buff_server* server) {
   drop(sv->ctx, sv->views);
   drop(sv->ctx, sv->majors);
   drop(sv->ctx, sv->arena);
+  (void)buff_ctx_destroy(reinterpret_cast<buff_ctx*>(sv->aux));
   delete sv;
   return BUFF_OK;
 }

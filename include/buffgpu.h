@@ -418,7 +418,9 @@ int buff_server_create(buff_ctx* ctx, const buff_server_cfg* cfg, buff_server** 
  * (free it after the server). Calls from several client threads may overlap;
  * every call on the server is serialized on its lock. On a buff-major chunk
  * with sparse columns, filters, sums and materializations are batched like the
- * others; a max there is BUFF_E_UNSUPPORTED (buff_max answers it). */
+ * others; a max there (no batched kernel) runs buff_max's stages when its batch
+ * completes, on a context the server creates for itself, and answers what
+ * buff_max answers. */
 int buff_server_add_chunk(buff_server* server, buff_chunk* chunk, uint32_t* id);
 /* Queues one request (buff_query.chunk = server chunk id). out_dev: where a
  * filter writes its ceil(n/64) bitmap words or a materialization its n doubles

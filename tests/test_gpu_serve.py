@@ -332,12 +332,15 @@ def _major_want(oracle, x, s, q):
 def test_served_buff_major_matches_oracle(gpu_ctx, served_major, oracle):
     # buff-major chunks served with continuous batching: every answer equals the
     # single-chunk call's and the oracle's (numpy's on the true offsets for
-    # BUFF_SEMANTIC); a batched max on a chunk with sparse columns is refused
+    # BUFF_SEMANTIC); a max on a chunk with sparse columns runs the single-chunk
+    # stages when its batch completes and equals the oracle's too
     from fractions import Fraction
 
     chunks, lens, xs, streams = served_major
     reqs = _major_requests(lens, 21, 300)
-    reqs.append((N.BUFF_Q_MAX, 0, 0.0, -1, 0))
+    for k in range(len(chunks)):
+        reqs.insert(37 * k + 5, (N.BUFF_Q_MAX, k, 0.0, -1, 0))
+        reqs.append((N.BUFF_Q_MAX, k, 0.0, -1, N.BUFF_SEMANTIC))
     bufs = _outs(gpu_ctx, lens, [(N.BUFF_Q_GT if q[0] == N.BUFF_Q_EQ else q[0],) + q[1:] for q in reqs])
     srv = buff_amd.Server(gpu_ctx, max_batch=64, slots=2)
     try:
@@ -350,7 +353,16 @@ def test_served_buff_major_matches_oracle(gpu_ctx, served_major, oracle):
             op, k, pred, prec, flags = q
             n = lens[k]
             if op == N.BUFF_Q_MAX:
-                assert r.status == (N.BUFF_E_UNSUPPORTED if chunks[k].info().sparse_cols else 0), q
+                assert r.status == 0, (q, r.status)
+                alone = chunks[k].max_ex(flags=flags)
+                assert (r.value, r.max_off) == alone[:2], (q, r.value, r.max_off, alone)
+                if flags & N.BUFF_SEMANTIC:
+                    from tests import closed_form as cf
+                    f = cf.fixed(xs[k], int(chunks[k].info().dlen))
+                    assert r.max_off == int(f.max() - f.min()), q
+                else:
+                    v_ref, off_ref, _ = oracle.major_max(streams[k], 10 ** 3)
+                    assert r.value == v_ref and r.max_off == off_ref, (q, r.value, r.max_off, v_ref, off_ref)
                 continue
             assert r.status == 0, (q, r.status)
             want = _major_want(oracle, xs[k], streams[k], q)
