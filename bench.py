@@ -879,7 +879,8 @@ def bench_config4(ctx, args, buff_amd):
     res["one_call_per_query_python"] = {"seconds": round(single, 5), "queries_per_s": round(nq / single, 1),
                                         "note": "ctypes calls from a Python loop"}
     # the same calls from a native loop (buff_trace_per_call, every arrival at 0:
-    # each query is its single-chunk call followed by a stream synchronization)
+    # each query is its single-chunk call, a filter followed by a stream
+    # synchronization; a sum or max is done when its call returns)
     ctx.trace_per_call(batch.chunks, qs[:16], np.zeros(16))   # first launches of every kernel shape
     ctx.synchronize()
     t = time.perf_counter()
@@ -887,7 +888,7 @@ def bench_config4(ctx, args, buff_amd):
     native = time.perf_counter() - t
     res["one_call_per_query"] = {"seconds": round(native, 5), "queries_per_s": round(nq / native, 1),
                                  "last_completion_s": round(float(lat[-1]) * 1e-6, 5),
-                                 "note": "native loop: one single-chunk call per query, then a stream sync"}
+                                 "note": "native loop: one single-chunk call per query (a filter then a stream sync)"}
     res["batched_speedup"] = round(native / bt[len(bt) // 2], 2)
     res["batched_speedup_vs_python_loop"] = round(single / bt[len(bt) // 2], 2)
     words.free()

@@ -1219,7 +1219,10 @@ int buff_trace_per_call(buff_ctx* ctx, buff_chunk* const* chunks, uint32_t nchun
       default:
         r.status = fail(BUFF_E_INVALID, "request %u: unknown op %u", i, q.op);
     }
-    if (r.status == BUFF_OK) {
+    // a filter's bitmap or a materialization's values are on the device once
+    // the stream has drained; a sum or max is complete when the call returns
+    // (its result is on the host)
+    if (r.status == BUFF_OK && q.op != BUFF_Q_SUM && q.op != BUFF_Q_MAX) {
       const hipError_t e = hipStreamSynchronize(c->stream);
       if (e != hipSuccess) return fail(BUFF_E_DEVICE, "per-call replay: %s", hipGetErrorString(e));
     }

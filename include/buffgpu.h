@@ -446,8 +446,9 @@ int buff_server_run_trace(buff_server* server, const buff_query* qs, const doubl
                           uint32_t n, buff_query_result* results, double* latency_us);
 /* The same open-loop replay without a server: each request, in arrival order,
  * is the plain single-chunk call on the context's stream (buff_filter_device,
- * buff_sum, buff_max, buff_materialize_device; one query per launch) followed
- * by a stream synchronization; latency_us[i] = completion - arrival. Filters
+ * buff_sum, buff_max, buff_materialize_device; one query per launch), followed
+ * by a stream synchronization for a filter or materialization (a sum or max is
+ * complete when its call returns); latency_us[i] = completion - arrival. Filters
  * write their bitmap but no count (results[i].count = 0). Outputs of requests
  * without outs[i] go to one context buffer that each call overwrites. chunks:
  * the resident chunks requests name by index. The baseline that continuous
