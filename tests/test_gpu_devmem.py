@@ -8,6 +8,9 @@
   alternating sets for sums and the min/max pass, per-call tags for the max):
   interleaved calls on different chunks, many times over, must each give the
   answer the chunk gives alone (checked against numpy on the stored offsets).
+* Results reach the host through each context's mailbox: two contexts used
+  from two threads at once, each with its own chunks, must each get their own
+  sums, maxima and encodes, call after call.
 * Encode takes float64 input at any 8-byte aligned device address: rows that
   start 8 bytes into a buffer must give the same chunk, byte for byte, as the
   same rows at a 256-byte aligned address.
@@ -129,3 +132,44 @@ def test_encode_aligned_and_unaligned_input(gpu_ctx, n):
             ch0.free()
     finally:
         a.free()
+
+
+def test_mailboxes_of_two_contexts_in_two_threads(gpu_ctx):
+    import threading
+
+    import buff_amd
+
+    def work(ctx, seed, errors):
+        try:
+            x = dg.random_walk(500_000 + seed, 3, seed)
+            ch = ctx.encode(x, 1000)
+            try:
+                inf = ch.info()
+                off = (cf.fixed(x, inf.dlen) - inf.base).astype(np.uint64)
+                exact = float(Fraction(inf.n * inf.base + int(off.sum(dtype=np.uint64)), 1 << inf.dlen))
+                want = ch.serialize()
+                for i in range(40):
+                    assert ch.sum()[1] == exact, (seed, i)
+                    assert ch.max()[1] == int(off.max()), (seed, i)
+                    if i % 10 == 0:
+                        again = ctx.encode(x, 1000)
+                        try:
+                            assert np.array_equal(again.serialize(), want), (seed, i)
+                        finally:
+                            again.free()
+            finally:
+                ch.free()
+        except BaseException as e:   # reported by the main thread
+            errors.append(e)
+
+    other = buff_amd.Context(0)
+    try:
+        errors = []
+        ts = [threading.Thread(target=work, args=(c, seed, errors)) for c, seed in ((gpu_ctx, 11), (other, 12))]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        assert not errors, errors
+    finally:
+        other.destroy()
