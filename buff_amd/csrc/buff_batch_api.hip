@@ -89,12 +89,12 @@ struct QueryExec {
   std::vector<buff_query_result> res_host;
   BHeader h{};
   uint32_t nq = 0;
-  // a max on a buff-major chunk with sparse columns has no batched kernel: its
-  // queries (host_max) run the single-chunk stages on `aux`, a context of the
-  // executor's own, when the batch completes (nullptr: answered
-  // BUFF_E_UNSUPPORTED)
+  // queries without a batched kernel (a max on a buff-major chunk with sparse
+  // columns, every query on a residue chunk) run the single-chunk call on
+  // `aux`, a context of the executor's own, when the batch completes
+  // (host_ops; aux == nullptr: answered BUFF_E_UNSUPPORTED)
   Ctx* aux = nullptr;
-  std::vector<uint32_t> host_max;
+  std::vector<uint32_t> host_ops;
 };
 
 struct Batch {
@@ -386,7 +386,7 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
   uint64_t total_tiles = 0;
   ex.res_host.assign(nq, buff_query_result{});
   ex.nq = nq;
-  ex.host_max.clear();
+  ex.host_ops.clear();
   uint64_t arena_off = 0, top_off = 0;
   std::vector<uint64_t> arena_at(nq, ~0ull);
   for (uint32_t k = 0; k < nq; ++k) {
@@ -407,6 +407,13 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
     if (need) {
       arena_at[k] = arena_off;
       arena_off += need;
+    }
+    if (ch->flags & BUFF_RESIDUE) {   // no items: the single-chunk call in exec_finish
+      if (ex.aux)
+        ex.host_ops.push_back(k);
+      else
+        r.status = fail(BUFF_E_UNSUPPORTED, "query %u: residue chunks are answered by the single-chunk calls", k);
+      continue;
     }
     const bool sparse = ch->S > 0;   // buff-major with sparse columns: k_bq_major / k_bq_major_mat
     const bool sem = (in.flags & BUFF_SEMANTIC) != 0;
@@ -438,7 +445,7 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
       case BUFF_Q_MAX:
         if (sparse) {
           if (ex.aux)
-            ex.host_max.push_back(k);   // no items: answered in exec_finish
+            ex.host_ops.push_back(k);   // no items: answered in exec_finish
           else
             r.status = fail(BUFF_E_UNSUPPORTED, "query %u: batched max of a buff-major chunk with sparse columns "
                             "(the single-chunk buff_max answers it)", k);
@@ -595,13 +602,47 @@ int exec_finish(QueryExec& ex, const std::vector<Chunk*>& chunks, const buff_que
         break;
     }
   }
-  for (uint32_t k : ex.host_max) {   // buff_max's stages, on the executor's own context
+  // the single-chunk calls, on the executor's own context
+  for (uint32_t k : ex.host_ops) {
     buff_query_result& r = ex.res_host[k];
     if (r.status != BUFF_OK) continue;
-    r.max_off = 0;
-    r.value = 0;
-    r.status = major_max(ex.aux, chunks[qs[k].chunk], qs[k].flags, &r.value, &r.max_off, nullptr);
+    const buff_query& q = qs[k];
+    const Chunk* ch = chunks[q.chunk];
+    r = buff_query_result{};
+    r.out_offset = ex.res_host[k].out_offset;
+    switch (q.op) {
+      case BUFF_Q_GT:
+      case BUFF_Q_EQ: {
+        uint64_t* words = reinterpret_cast<uint64_t*>(ex.hq[k].out);
+        r.status = filter_device(ex.aux, ch, q.op == BUFF_Q_GT ? BUFF_OP_GT : BUFF_OP_EQ, q.pred, q.flags, words,
+                                 nullptr);
+        const uint64_t nw = (ch->n + 63) / 64;
+        if (r.status == BUFF_OK && nw) {   // the count, from the bitmap
+          std::vector<uint64_t> host(nw);
+          hipError_t e = hipMemcpyAsync(host.data(), words, nw * 8, hipMemcpyDeviceToHost, ex.aux->stream);
+          if (e == hipSuccess) e = hipStreamSynchronize(ex.aux->stream);
+          if (e != hipSuccess) {
+            r.status = fail(BUFF_E_DEVICE, "query %u: filter count: %s", k, hipGetErrorString(e));
+            break;
+          }
+          for (uint64_t w : host) r.count += (uint64_t)__builtin_popcountll(w);
+        }
+        break;
+      }
+      case BUFF_Q_SUM:   // buff_sum_ex: the flags do not change a sum of these layouts
+        r.status = sum_op(ex.aux, ch, &r.value, &r.exact, nullptr);
+        break;
+      case BUFF_Q_MAX:
+        r.status = max_op(ex.aux, ch, q.flags, &r.value, &r.max_off, nullptr);
+        break;
+      case BUFF_Q_MAT:
+        r.status = materialize_device(ex.aux, ch, q.precision, q.flags, reinterpret_cast<double*>(ex.hq[k].out));
+        break;
+      default:
+        r.status = fail(BUFF_E_INVALID, "query %u: unknown op %u", k, q.op);
+    }
   }
+  if (!ex.host_ops.empty()) HIP_TRY(hipStreamSynchronize(ex.aux->stream));   // outputs written
   return BUFF_OK;
 }
 
@@ -1057,8 +1098,7 @@ int buff_server_add_chunk(buff_server* server, buff_chunk* chunk, uint32_t* id) 
   if (!sv || !chunk) return fail(BUFF_E_INVALID, "null argument");
   Chunk* ch = reinterpret_cast<Chunk*>(chunk);
   if (ch->ctx != sv->ctx) return fail(BUFF_E_INVALID, "the chunk belongs to another context");
-  if (ch->flags & (BUFF_SLICE | BUFF_RESIDUE))
-    return fail(BUFF_E_UNSUPPORTED, "served chunks use the byte_fixed or buff-major layout");
+  if (ch->flags & BUFF_SLICE) return fail(BUFF_E_UNSUPPORTED, "served chunks use the byte_fixed, buff-major or residue layout");
   std::lock_guard<std::mutex> lk(sv->mu);
   if (id) *id = (uint32_t)sv->chunks.size();
   sv->chunks.push_back(ch);

@@ -413,14 +413,17 @@ typedef struct {
 } buff_server_cfg;
 
 int buff_server_create(buff_ctx* ctx, const buff_server_cfg* cfg, buff_server** out);
-/* Makes a resident byte_fixed or buff-major chunk of the same context
- * servable; *id is the chunk index requests name. The caller keeps ownership
- * (free it after the server). Calls from several client threads may overlap;
- * every call on the server is serialized on its lock. On a buff-major chunk
- * with sparse columns, filters, sums and materializations are batched like the
- * others; a max there (no batched kernel) runs buff_max's stages when its batch
- * completes, on a context the server creates for itself, and answers what
- * buff_max answers. */
+/* Makes a resident byte_fixed, buff-major or residue chunk of the same
+ * context servable; *id is the chunk index requests name. The caller keeps
+ * ownership (free it after the server). Calls from several client threads may
+ * overlap; every call on the server is serialized on its lock. On a buff-major
+ * chunk with sparse columns, filters, sums and materializations are batched
+ * like the others. Requests without a batched kernel (a max on a buff-major
+ * chunk with sparse columns, every request on a residue chunk) run the
+ * single-chunk call (buff_filter_device with the count of its bitmap,
+ * buff_sum_ex, buff_max_ex, buff_materialize_device) when their batch
+ * completes, on a context the server creates for itself, and answer what that
+ * call answers. */
 int buff_server_add_chunk(buff_server* server, buff_chunk* chunk, uint32_t* id);
 /* Queues one request (buff_query.chunk = server chunk id). out_dev: where a
  * filter writes its ceil(n/64) bitmap words or a materialization its n doubles

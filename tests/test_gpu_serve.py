@@ -399,3 +399,88 @@ def test_buff_major_semantic_reduced_precision(gpu_ctx, served_major, oracle):
             assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), (k, p)
         with pytest.raises(N.Unsupported):
             chunks[k].materialize(2)
+
+
+def test_served_residue_chunks_equal_single_chunk_calls(gpu_ctx, oracle):
+    # residue chunks have no batched kernels: the server answers their requests
+    # with the single-chunk calls when the batch completes (the reference's
+    # answers, or the intended ones with BUFF_SEMANTIC; a filter or max without
+    # BUFF_SEMANTIC is BUFF_E_UNSUPPORTED as alone), next to plain chunks
+    # that are batched
+    from tests.test_rapg_oracle import CASES
+
+    names = ["spikes_5pct", "walk", "outlier_heavy", "spikes_p5"]
+    xs = [CASES[m][0] for m in names] + [dg.random_walk(200_000, 3, 5)]
+    scales = [CASES[m][1] for m in names] + [1000]
+    flags = [N.BUFF_RESIDUE] * len(names) + [0]
+    chunks = [gpu_ctx.encode(x, s, flags=f) for x, s, f in zip(xs, scales, flags)]
+    lens = [len(x) for x in xs]
+    streams = [oracle.residue_encode(x, s)[0] for x, s in zip(xs[:-1], scales[:-1])]
+    sem = N.BUFF_SEMANTIC
+    reqs = []
+    for k, x in enumerate(xs):
+        hi = float(np.quantile(x, 0.9, method="lower"))
+        reqs += [(N.BUFF_Q_SUM, k, 0.0, -1, 0), (N.BUFF_Q_SUM, k, 0.0, -1, sem), (N.BUFF_Q_MAX, k, 0.0, -1, sem),
+                 (N.BUFF_Q_MAX, k, 0.0, -1, 0), (N.BUFF_Q_GT, k, hi, -1, sem), (N.BUFF_Q_GT, k, hi, -1, 0),
+                 (N.BUFF_Q_EQ, k, float(x[len(x) // 3]), -1, sem), (N.BUFF_Q_MAT, k, 0.0, -1, 0)]
+        reqs += [(N.BUFF_Q_MAT, k, 0.0, p, 0) for p in (0, 1, 2, 3)]
+    order = np.random.default_rng(4).permutation(len(reqs))
+    reqs = [reqs[i] for i in order]
+    bufs = []
+    for op, k, _, _, _ in reqs:
+        if op in (N.BUFF_Q_GT, N.BUFF_Q_EQ):
+            bufs.append(gpu_ctx.alloc(((lens[k] + 63) // 64) * 8))
+        elif op == N.BUFF_Q_MAT:
+            bufs.append(gpu_ctx.alloc(lens[k] * 8))
+        else:
+            bufs.append(None)
+
+    def alone(q):
+        """The single-chunk call's status and answer."""
+        op, k, pred, prec, fl = q
+        ch = chunks[k]
+        try:
+            if op == N.BUFF_Q_SUM:
+                return 0, ch.sum(fl)[:2]
+            if op == N.BUFF_Q_MAX:
+                return 0, ch.max_ex(flags=fl)[:2]
+            if op in (N.BUFF_Q_GT, N.BUFF_Q_EQ):
+                return 0, ch.filter("range" if op == N.BUFF_Q_GT else "equal", pred, flags=fl)
+            return 0, ch.materialize(prec, fl)
+        except N.BuffError as e:
+            return e.code, None
+
+    srv = buff_amd.Server(gpu_ctx, max_batch=16, slots=2)
+    try:
+        ids = [srv.add_chunk(c) for c in chunks]
+        assert ids == list(range(len(chunks)))
+        res, _ = srv.run_trace(reqs, np.zeros(len(reqs)), [b.ptr if b else 0 for b in bufs])
+        for q, r, b in zip(reqs, res, bufs):
+            op, k, pred, prec, fl = q
+            n = lens[k]
+            code, want = alone(q)
+            assert r.status == code, (q, r.status, code)
+            if code:
+                continue
+            if op == N.BUFF_Q_SUM:
+                assert (r.value, r.exact) == want, (q, r.value, r.exact, want)
+                if k < len(streams) and not fl:
+                    assert r.value == oracle.residue_sum(streams[k])[0], q
+            elif op == N.BUFF_Q_MAX:
+                assert (r.value, r.max_off) == want, (q, r.value, r.max_off, want)
+            elif op in (N.BUFF_Q_GT, N.BUFF_Q_EQ):
+                words = b.download(np.uint32, ((n + 63) // 64) * 2)[: (n + 31) // 32]
+                assert r.count == want[1] and np.array_equal(words, want[0]), q
+            else:
+                got = b.download(np.float64, n)[: len(want)]
+                assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), q
+                if k < len(streams) and prec < 0:
+                    ref = oracle.residue_decode(streams[k], -1)
+                    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), q
+    finally:
+        srv.destroy()
+        for c in chunks:
+            c.free()
+        for b in bufs:
+            if b:
+                b.free()
