@@ -608,8 +608,9 @@ int exec_finish(QueryExec& ex, const std::vector<Chunk*>& chunks, const buff_que
     if (r.status != BUFF_OK) continue;
     const buff_query& q = qs[k];
     const Chunk* ch = chunks[q.chunk];
+    const uint64_t out_offset = r.out_offset;   // the output address planned for it
     r = buff_query_result{};
-    r.out_offset = ex.res_host[k].out_offset;
+    r.out_offset = out_offset;
     switch (q.op) {
       case BUFF_Q_GT:
       case BUFF_Q_EQ: {

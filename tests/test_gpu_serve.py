@@ -484,3 +484,36 @@ def test_served_residue_chunks_equal_single_chunk_calls(gpu_ctx, oracle):
         for b in bufs:
             if b:
                 b.free()
+
+
+def test_served_residue_outputs_in_the_arena(gpu_ctx, oracle):
+    # a residue request without an output address writes into the server's
+    # arena like any other: its result names the range, which reads back as the
+    # single-chunk answer and returns to the arena on release
+    from tests.test_rapg_oracle import CASES
+
+    x, scale = CASES["spikes_5pct"]
+    n = len(x)
+    ch = gpu_ctx.encode(x, scale, flags=N.BUFF_RESIDUE)
+    srv = buff_amd.Server(gpu_ctx, max_batch=8, slots=2, arena_bytes=64 << 20)
+    try:
+        srv.add_chunk(ch)
+        pred = float(np.quantile(x, 0.9, method="lower"))
+        t1 = srv.submit(N.BUFF_Q_GT, 0, pred, flags=N.BUFF_SEMANTIC)
+        t2 = srv.submit(N.BUFF_Q_MAT, 0, precision=-1)
+        r1, _ = srv.wait(t1)
+        r2, _ = srv.wait(t2)
+        assert r1.status == 0 and r1.out_offset and r2.status == 0 and r2.out_offset
+        words, count = ch.filter("range", pred, flags=N.BUFF_SEMANTIC)
+        assert r1.count == count
+        assert np.array_equal(_d2h(gpu_ctx, r1.out_offset, np.uint32, (n + 31) // 32), words)
+        want = oracle.residue_decode(oracle.residue_encode(x, scale)[0], -1)
+        got = _d2h(gpu_ctx, r2.out_offset, np.float64, n)
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+        assert srv.arena_stats()[0] > 0
+        srv.release(r1.out_offset)
+        srv.release(r2.out_offset)
+        assert srv.arena_stats()[0] == 0
+    finally:
+        srv.destroy()
+        ch.free()
