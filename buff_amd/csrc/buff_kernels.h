@@ -56,7 +56,7 @@ hipError_t launch_minmax(const double* x, uint64_t n, int32_t pexp, uint32_t dle
                          int64_t* partial2, MinMaxOut* out, uint32_t nwg, bool prefix, hipStream_t s);
 // One launch (n >= 1): the min over all rows and the max over rows >= 1 folded
 // into the slot set `set` (int64: kMinMaxSlots mins, kMinMaxSlots maxes, then
-// row 0's fixed value), for the host to fold; `next`, the set the following
+// row 0's fixed value), for launch_publish to fold; `next`, the set the following
 // call folds into, is reset; `partial` gets the per-workgroup rows the
 // prefix-min pass reads. Sets start reset (launch_minmax_slots_init).
 constexpr uint32_t kMinMaxSlots = 64;

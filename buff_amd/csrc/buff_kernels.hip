@@ -432,7 +432,7 @@ __global__ void k_touch_bytes(const uint4* __restrict__ in, uint64_t n16, uint32
 // workgroup writes one row of partial sums, which k_sum_final folds; or, with
 // `next` given (one launch), adds its row into one of kSumSlots slot rows of
 // `rows` (u64 atomics, workgroup w into slot w % kSumSlots, laid out
-// [k][slot]) for the host to fold, and workgroup 0 clears `next`, the set the
+// [k][slot]) for launch_publish to fold, and workgroup 0 clears `next`, the set the
 // following call adds into.
 
 template <int P, int R, int U>
