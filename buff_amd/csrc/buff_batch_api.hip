@@ -408,11 +408,15 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
       arena_at[k] = arena_off;
       arena_off += need;
     }
-    if (ch->flags & BUFF_RESIDUE) {   // no items: the single-chunk call in exec_finish
+    const bool residue = (ch->flags & BUFF_RESIDUE) != 0;
+    // a residue chunk's sums and full-precision materializations run batched (its
+    // resident form is buff-major's with majority byte 0, or byte_fixed's); its
+    // other queries have no items and run the single-chunk call in exec_finish
+    if (residue && in.op != BUFF_Q_SUM && !(in.op == BUFF_Q_MAT && in.precision < 0)) {
       if (ex.aux)
         ex.host_ops.push_back(k);
       else
-        r.status = fail(BUFF_E_UNSUPPORTED, "query %u: residue chunks are answered by the single-chunk calls", k);
+        r.status = fail(BUFF_E_UNSUPPORTED, "query %u: this query of a residue chunk runs as its single-chunk call", k);
       continue;
     }
     const bool sparse = ch->S > 0;   // buff-major with sparse columns: k_bq_major / k_bq_major_mat
@@ -466,12 +470,14 @@ int exec_launch(QueryExec& ex, const std::vector<Chunk*>& chunks, const void* vi
           if (!sem)
             for (uint32_t j = 0; j < ch->S && rc == BUFF_OK; ++j)
               if (ch->sp[j].n_out == 0)
-                rc = fail(BUFF_E_REF_PANIC, "query %u: sparse column %u has no outliers (buff_simd.rs:683)", k, j);
+                rc = fail(BUFF_E_REF_PANIC, "query %u: sparse column %u has no outliers (%s)", k, j,
+                          residue ? "split_double.rs:1405" : "buff_simd.rs:683");
           memset(&q.mp, 0, sizeof(q.mp));
-          q.mp.mode = sem ? kMatSemantic : kMatFull;
+          // a residue decode (byte_residue_decode) is the true offsets
+          q.mp.mode = (sem || residue) ? kMatSemantic : kMatFull;
           q.mp.inv_scale = std::ldexp(1.0, -(int)ch->dlen);
           if (rc == BUFF_OK) rc = major_precision_drop(ch, in.precision, &q.mp.drop);
-          tiles = tiles && major_decoded_rows(ch, in.flags) > 0;
+          tiles = tiles && (residue && !sem ? residue_decoded_rows(ch, -1) : major_decoded_rows(ch, in.flags)) > 0;
           q.sparse = sparse_word;
         } else if (rc == BUFF_OK) {
           rc = mat_plan(ch, in.precision, in.flags, &q.mp);
@@ -581,9 +587,13 @@ int exec_finish(QueryExec& ex, const std::vector<Chunk*>& chunks, const buff_que
         uint64_t ps[kMaxPlanes + 1] = {0};
         for (uint32_t p = 0; p < ch->view.P; ++p) ps[p] = br.v[p];
         ps[ch->view.P] = br.v[kMaxPlanes];
-        if (ch->S > 0) {   // dense planes, then the outlier sums of the sparse columns (k_bq_major)
-          uint64_t outsum[kMaxSparse] = {0};
-          for (uint32_t j = 0; j < ch->S; ++j) outsum[j] = br.v[ch->view.P + j];
+        uint64_t outsum[kMaxSparse] = {0};   // dense planes, then the sparse columns' outlier sums (k_bq_major)
+        for (uint32_t j = 0; j < ch->S; ++j) outsum[j] = br.v[ch->view.P + j];
+        if (ch->flags & BUFF_RESIDUE) {
+          r.status = residue_full_sum_fold(ch, outsum, ps, &r.value, &r.exact);
+          break;
+        }
+        if (ch->S > 0) {
           major_sum_fold(ch, outsum, ps, &r.value, &r.exact, nullptr);
           break;
         }

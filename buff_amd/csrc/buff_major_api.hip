@@ -995,6 +995,9 @@ int residue_read(const Chunk* ch, int32_t precision, ResidueRead* rr) {
 
 }  // namespace
 
+int residue_sum_fold(const Chunk* ch, const ResidueRead& rr, bool semantic, const uint64_t* osum, const uint64_t* ps,
+                     const uint64_t* fields, double* ref_sum, double* exact_sum, uint64_t* sums_out);
+
 uint64_t residue_decoded_rows(const Chunk* ch, int32_t precision) {
   if (ch->S == 0) return ch->n;
   ResidueRead rr;
@@ -1075,6 +1078,15 @@ int residue_sum_op(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags,
     if (pv.P || pv.r) BUFF_TRY(d2h_small(ctx, ps, sb + dense_off, sizeof(ps)));
     if (rr.fields) BUFF_TRY(d2h_small(ctx, fields, ctx->s_best.ptr, sizeof(fields)));
   }
+  return residue_sum_fold(ch, rr, semantic, osum, ps, fields, ref_sum, exact_sum, sums_out);
+}
+
+// The fold of residue_sum_op from its u64 sums: osum, the columns' outlier byte
+// sums; ps, the sums of the rr.k planes read and, at ps[rr.k], the tail's;
+// fields, the q-bit fields' sums when rr.fields.
+int residue_sum_fold(const Chunk* ch, const ResidueRead& rr, bool semantic, const uint64_t* osum, const uint64_t* ps,
+                     const uint64_t* fields, double* ref_sum, double* exact_sum, uint64_t* sums_out) {
+  const ChunkView& v = ch->view;
   // sums in read order: columns, whole planes, the q-bit part
   uint64_t sums[2 * kMaxPlanes + 1] = {0};
   uint32_t m = 0;
@@ -1125,6 +1137,15 @@ int residue_sum_op(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags,
   }
   *ref_sum = sum + sum_base;
   return BUFF_OK;
+}
+
+// byte_residue_sum (the full read) from the columns' outlier byte sums and the
+// planes' and tail's sums (the batched sums of a residue chunk).
+int residue_full_sum_fold(const Chunk* ch, const uint64_t* osum, const uint64_t* ps, double* ref_sum,
+                          double* exact_sum) {
+  ResidueRead rr;
+  BUFF_TRY(residue_read(ch, -1, &rr));
+  return residue_sum_fold(ch, rr, false, osum, ps, nullptr, ref_sum, exact_sum, nullptr);
 }
 
 }  // namespace buffgpu

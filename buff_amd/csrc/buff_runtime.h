@@ -215,6 +215,10 @@ int residue_serialize(Ctx* ctx, const Chunk* ch, uint8_t* dst, uint64_t* len);
 int residue_load(Ctx* ctx, const uint8_t* s, uint64_t len, uint64_t scale, uint32_t flags, Chunk** out);
 uint64_t residue_decoded_rows(const Chunk* ch, int32_t precision);
 int residue_materialize_device(Ctx* ctx, const Chunk* ch, int32_t precision, double* out, uint64_t* rows);
+// byte_residue_sum from a residue chunk's outlier byte sums (one per column) and
+// the sums of its planes and tail (plane_sums' layout)
+int residue_full_sum_fold(const Chunk* ch, const uint64_t* osum, const uint64_t* ps, double* ref_sum,
+                          double* exact_sum);
 int residue_sum_op(Ctx* ctx, const Chunk* ch, int32_t precision, uint32_t flags, double* ref_sum, double* exact_sum,
                    uint64_t* sums_out);
 int major_precision_drop(const Chunk* ch, int32_t precision, uint32_t* drop);

@@ -418,12 +418,13 @@ int buff_server_create(buff_ctx* ctx, const buff_server_cfg* cfg, buff_server** 
  * ownership (free it after the server). Calls from several client threads may
  * overlap; every call on the server is serialized on its lock. On a buff-major
  * chunk with sparse columns, filters, sums and materializations are batched
- * like the others. Requests without a batched kernel (a max on a buff-major
- * chunk with sparse columns, every request on a residue chunk) run the
- * single-chunk call (buff_filter_device with the count of its bitmap,
- * buff_sum_ex, buff_max_ex, buff_materialize_device) when their batch
- * completes, on a context the server creates for itself, and answer what that
- * call answers. */
+ * like the others, and so are a residue chunk's sums and full-precision
+ * materializations. Requests without a batched kernel (a max on a buff-major
+ * chunk with sparse columns; a residue chunk's filters, maxima and
+ * reduced-precision materializations) run the single-chunk call
+ * (buff_filter_device with the count of its bitmap, buff_max_ex,
+ * buff_materialize_device) when their batch completes, on a context the server
+ * creates for itself, and answer what that call answers. */
 int buff_server_add_chunk(buff_server* server, buff_chunk* chunk, uint32_t* id);
 /* Queues one request (buff_query.chunk = server chunk id). out_dev: where a
  * filter writes its ceil(n/64) bitmap words or a materialization its n doubles
