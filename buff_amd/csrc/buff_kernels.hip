@@ -658,6 +658,10 @@ __global__ __launch_bounds__(256) void k_max_refine(ChunkView cv, const uint8_t*
   const uint64_t wave = wave_global();
   const uint64_t nwaves = waves_in_grid();
   const bool has_planes = cv.P > 0;
+  // 64 tiles per probe, one byte per lane; the first probe's bytes are loaded
+  // together with the slots
+  uint64_t base = wave * 64;
+  uint32_t tm = (has_planes && base + lane < cv.ntiles) ? tile_max[base + lane] : 0u;
   uint32_t t = 0;
   if (has_planes) {
     const uint32_t m = wave_max_u32_dpp(slots[lane]);   // one slot per lane
@@ -665,10 +669,11 @@ __global__ __launch_bounds__(256) void k_max_refine(ChunkView cv, const uint8_t*
   }
   uint64_t best = 0;
   bool have = false;
-  // 64 tiles per probe: one byte per lane
-  for (uint64_t base = wave * 64; base < cv.ntiles; base += nwaves * 64) {
+  for (; base < cv.ntiles; base += nwaves * 64) {
     const uint64_t c = base + lane;
-    const bool hit = c < cv.ntiles && (!has_planes || tile_max[c] == t);
+    const bool hit = c < cv.ntiles && (!has_planes || tm == t);
+    const uint64_t nb = base + nwaves * 64;
+    tm = (has_planes && nb + lane < cv.ntiles) ? tile_max[nb + lane] : 0u;   // the next probe's bytes
     uint64_t hits = __ballot(hit);
     while (hits) {
       const uint32_t k = (uint32_t)__builtin_ctzll(hits);
