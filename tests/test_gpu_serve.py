@@ -454,29 +454,36 @@ def test_served_residue_chunks_equal_single_chunk_calls(gpu_ctx, oracle):
     try:
         ids = [srv.add_chunk(c) for c in chunks]
         assert ids == list(range(len(chunks)))
-        res, _ = srv.run_trace(reqs, np.zeros(len(reqs)), [b.ptr if b else 0 for b in bufs])
-        for q, r, b in zip(reqs, res, bufs):
-            op, k, pred, prec, fl = q
-            n = lens[k]
-            code, want = alone(q)
-            assert r.status == code, (q, r.status, code)
-            if code:
-                continue
-            if op == N.BUFF_Q_SUM:
-                assert (r.value, r.exact) == want, (q, r.value, r.exact, want)
-                if k < len(streams) and not fl:
-                    assert r.value == oracle.residue_sum(streams[k])[0], q
-            elif op == N.BUFF_Q_MAX:
-                assert (r.value, r.max_off) == want, (q, r.value, r.max_off, want)
-            elif op in (N.BUFF_Q_GT, N.BUFF_Q_EQ):
-                words = b.download(np.uint32, ((n + 63) // 64) * 2)[: (n + 31) // 32]
-                assert r.count == want[1] and np.array_equal(words, want[0]), q
-            else:
-                got = b.download(np.float64, n)[: len(want)]
-                assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), q
-                if k < len(streams) and prec < 0:
-                    ref = oracle.residue_decode(streams[k], -1)
-                    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), q
+        # every request at t = 0, then arrivals spread over 20 ms (single-chunk
+        # calls of one batch run while the other slot's batch is in flight)
+        spread = np.sort(np.random.default_rng(6).uniform(0, 0.02, len(reqs)))
+        for arrivals in (np.zeros(len(reqs)), spread):
+            res, _ = srv.run_trace(reqs, arrivals, [b.ptr if b else 0 for b in bufs])
+            for q, r, b in zip(reqs, res, bufs):
+                op, k, pred, prec, fl = q
+                n = lens[k]
+                code, want = alone(q)
+                assert r.status == code, (q, r.status, code)
+                if code:
+                    continue
+                if op == N.BUFF_Q_SUM:
+                    assert (r.value, r.exact) == want, (q, r.value, r.exact, want)
+                    if k < len(streams) and not fl:
+                        assert r.value == oracle.residue_sum(streams[k])[0], q
+                elif op == N.BUFF_Q_MAX:
+                    assert (r.value, r.max_off) == want, (q, r.value, r.max_off, want)
+                elif op in (N.BUFF_Q_GT, N.BUFF_Q_EQ):
+                    words = b.download(np.uint32, ((n + 63) // 64) * 2)[: (n + 31) // 32]
+                    assert r.count == want[1] and np.array_equal(words, want[0]), q
+                else:
+                    got = b.download(np.float64, n)[: len(want)]
+                    assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), q
+                    if k < len(streams) and prec < 0:
+                        ref = oracle.residue_decode(streams[k], -1)
+                        assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), q
+            for b in bufs:   # a stale output from the first pass cannot pass for the second's
+                if b:
+                    b.fill(0xFF)
     finally:
         srv.destroy()
         for c in chunks:
